@@ -1,0 +1,94 @@
+"""ctypes binding of libccamd.so (the gfx950 solver-step library, C ABI in include/ccamd.h).
+
+The library is built in-tree by __graft_entry__.build() (make -C core_collapse_amd/csrc).  There is
+no CPU fallback: if the shared library is missing or no HIP device is visible, every entry point
+raises, so a test or benchmark can never pass on a silent substitute.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libccamd.so")
+
+DP = C.POINTER(C.c_double)
+
+CC_OK = 0
+CC_FP64 = 0
+CC_MIXED = 1
+STATUS = {0: "CC_OK", 1: "CC_ERR_INVALID_ARG", 2: "CC_ERR_NON_POW2", 3: "CC_ERR_BAD_DT", 4: "CC_ERR_HIP",
+          5: "CC_ERR_OOM", 6: "CC_ERR_NO_DEVICE"}
+
+
+class CCError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s: %s" % (STATUS.get(status, status), msg))
+        self.status = status
+
+
+class Mhd2dParams(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("cfl", C.c_double), ("x0", C.c_double), ("x1", C.c_double),
+                ("y0", C.c_double), ("y1", C.c_double), ("Nx", C.c_int), ("Ny", C.c_int),
+                ("floor_rho", C.c_double), ("floor_P", C.c_double), ("periodic_x", C.c_int),
+                ("periodic_y", C.c_int), ("eta", C.c_double), ("gravity_on", C.c_int), ("G", C.c_double),
+                ("sor_iters", C.c_int), ("sor_omega", C.c_double), ("precision", C.c_int)]
+
+
+# name -> (restype, argtypes); every function returns cc_status unless stated
+_PROTOS = {
+    "cc_last_error": (C.c_char_p, []),
+    "cc_version": (C.c_int, []),
+    "cc_device_info": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int)]),
+    "cc_mhd2d_create": (C.c_int, [C.POINTER(Mhd2dParams), C.POINTER(C.c_void_p)]),
+    "cc_mhd2d_destroy": (C.c_int, [C.c_void_p]),
+    "cc_mhd2d_upload": (C.c_int, [C.c_void_p] + [DP] * 8),
+    "cc_mhd2d_download": (C.c_int, [C.c_void_p] + [DP] * 8),
+    "cc_mhd2d_gravity": (C.c_int, [C.c_void_p]),
+    "cc_mhd2d_get_gravity": (C.c_int, [C.c_void_p, DP, DP]),
+    "cc_mhd2d_set_gravity": (C.c_int, [C.c_void_p, DP, DP]),
+    "cc_mhd2d_compute_dt": (C.c_int, [C.c_void_p, DP]),
+    "cc_mhd2d_stage": (C.c_int, [C.c_void_p, C.c_double]),
+    "cc_mhd2d_rk2_step": (C.c_int, [C.c_void_p, C.c_double]),
+    "cc_mhd2d_run": (C.c_int, [C.c_void_p, C.c_int, DP, C.c_double, DP]),
+    "cc_mhd2d_divb_max": (C.c_int, [C.c_void_p, DP]),
+    "cc_mhd2d_sync": (C.c_int, [C.c_void_p]),
+    "cc_mhd2d_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "cc_mhd2d_kernel_time": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int), DP]),
+    "cc_mhd2d_reset_profiling": (C.c_int, [C.c_void_p]),
+    "cc_hlld2d_batch": (C.c_int, [C.c_int, C.c_int, DP, DP, DP, C.c_double, C.c_double, C.c_int, DP]),
+    "cc_poisson2d_sor": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                   C.c_int, C.c_double, C.c_int, DP, DP, DP, DP]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libccamd.so once; raise if it is absent (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libccamd.so not built at %s: run __graft_entry__.build() "
+                               "(make -C core_collapse_amd/csrc)" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _PROTOS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_PROTOS)
+
+
+def check(status):
+    if status != CC_OK:
+        raise CCError(status, lib().cc_last_error().decode(errors="replace"))
+
+
+def device_info():
+    name = C.create_string_buffer(256)
+    cu = C.c_int(0)
+    check(lib().cc_device_info(name, 256, C.byref(cu)))
+    return name.value.decode(), cu.value

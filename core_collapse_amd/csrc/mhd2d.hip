@@ -1,0 +1,801 @@
+// mhd2d.hip -- gfx950 solver step for 2D resistive MHD with HLLD + constrained transport + SOR
+// self-gravity (restating mhd2d_resistive_hlld_ct_gravity.cpp, cited as M2:<line>).
+//
+// One forward-Euler stage (advance_stage, M2:690-834) is three kernels:
+//   k_flux2d      PLM-MC reconstruction + HLLD on x- and y-faces of a 32x8 tile from primitives
+//                 staged once per cell in LDS, flux difference + floors (old faces B0)  -> U*,
+//                 face EMFs Ez_x, Ez_y
+//   k_corner2d    corner Ez = 1/4 sum of face EMFs + eta Jz(B0)                          -> Ez
+//   k_ct_src2d    CT face update, gravity source, floors (B1), Joule heating, Bz diffusion,
+//                 floors (B1); the stage-2 instance also forms the Heun average, its floors
+//                 and the next step's CFL signal-speed maximum (wave64 shuffles + one atomic).
+// Heun (rk2_step, M2:838-869) = stage(U^n, g) -> U1; g1 = Poisson(U1.rho); stage(U1, g1) + average.
+// In the mixed-precision form, reconstruction and the Riemann solver run in FP32 on primitives
+// recovered in FP64; flux differences, conserved updates, EMF averages and CT stay FP64.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "poisson2d.hpp"
+#include "riemann.hpp"
+
+namespace cc {
+
+struct Geom2D {
+    int Nx, Ny, px, py;
+    double dx, dy, gamma, gm1, floor_rho, floor_P, eta, cfl;
+    int gravity_on;
+};
+
+struct State2D {
+    double *rho, *mx, *my, *mz, *E, *Bz;  // Nx*Ny
+    double *Bx, *By;                      // (Nx+1)*Ny, Nx*(Ny+1)
+};
+
+struct Scalars {
+    unsigned long long maxs;  // ordered bits of the running signal-speed maximum (compute_dt, M2:581)
+    int step;
+    int pad;
+    double dt, t, t_end;
+    double divb;
+};
+
+// ------------------------------------------------------------------ cell helpers (FP64)
+__device__ __forceinline__ void enforce_floors2d(const Geom2D& g, double& rho, double& mx, double& my, double& mz,
+                                                 double& E, double Bx, double By, double Bz) {  // M2:114-124
+    rho = smax(rho, g.floor_rho);
+    const double u = mx / rho, v = my / rho, w = mz / rho;
+    const double kin = 0.5 * rho * (u * u + v * v + w * w);
+    const double mag = 0.5 * (Bx * Bx + By * By + Bz * Bz);
+    const double P = g.gm1 * (E - kin - mag);
+    if (P < g.floor_P) {
+        const double eint = g.floor_P / g.gm1;
+        E = kin + mag + eint;
+    }
+}
+
+__device__ __forceinline__ Prim<double> cons_to_prim2d(const Geom2D& g, double rho, double mx, double my, double mz,
+                                                       double E, double Bx, double By, double Bz) {  // M2:126-135
+    rho = smax(rho, g.floor_rho);
+    const double u = mx / rho, v = my / rho, w = mz / rho;
+    const double kin = 0.5 * rho * (u * u + v * v + w * w);
+    const double mag = 0.5 * (Bx * Bx + By * By + Bz * Bz);
+    double P = g.gm1 * (E - kin - mag);
+    if (P < g.floor_P) P = g.floor_P;
+    return Prim<double>{rho, u, v, w, P, Bx, By, Bz};
+}
+
+__device__ __forceinline__ double signal_speed2d(const Geom2D& g, const Prim<double>& W) {  // M2:585-587
+    IdealEOS2D<double> eos{g.gm1, g.gamma, g.floor_rho};
+    const double sx = fabs(W.u) + fast_speed<double>(eos, W, W.Bx);
+    const double sy = fabs(W.v) + fast_speed<double>(eos, W, W.By);
+    return smax(sx, sy);
+}
+
+// ------------------------------------------------------------------ flux + FV kernel
+template <typename R>
+struct FluxTile {
+    static constexpr int TX = 32, TY = 8, RX = TX + 4, RY = TY + 4;
+};
+
+template <typename R>
+__device__ __forceinline__ Prim<R> lds_prim(const R (*sW)[FluxTile<R>::RY][FluxTile<R>::RX], int ly, int lx) {
+    return Prim<R>{sW[0][ly][lx], sW[1][ly][lx], sW[2][ly][lx], sW[3][ly][lx],
+                   sW[4][ly][lx], sW[5][ly][lx], sW[6][ly][lx], sW[7][ly][lx]};
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_flux2d(Geom2D g, State2D U0, const Scalars* __restrict__ sc, State2D Us,
+                                                double* __restrict__ ezx, double* __restrict__ ezy) {
+    using T = FluxTile<R>;
+    constexpr int TX = T::TX, TY = T::TY, RX = T::RX, RY = T::RY;
+    __shared__ R sW[8][RY][RX];
+    __shared__ R sFx[6][TY][TX + 1];
+    __shared__ R sFy[6][TY + 1][TX];
+    const int tid = threadIdx.x;
+    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+    const int Nx = g.Nx, Ny = g.Ny;
+
+    // 1. primitives of the tile + 2-cell halo, boundary-mapped as get_prim_cc does (M2:278-286)
+    for (int c = tid; c < RX * RY; c += 256) {
+        const int lx = c % RX, ly = c / RX;
+        const int i = bmap(i0 - 2 + lx, Nx, g.px), j = bmap(j0 - 2 + ly, Ny, g.py);
+        const size_t q = (size_t)j * Nx + i;
+        const double bx = 0.5 * (U0.Bx[(size_t)j * (Nx + 1) + i] + U0.Bx[(size_t)j * (Nx + 1) + i + 1]);
+        const double by = 0.5 * (U0.By[(size_t)j * Nx + i] + U0.By[(size_t)(j + 1) * Nx + i]);
+        const Prim<double> W = cons_to_prim2d(g, U0.rho[q], U0.mx[q], U0.my[q], U0.mz[q], U0.E[q], bx, by, U0.Bz[q]);
+        sW[0][ly][lx] = (R)W.rho; sW[1][ly][lx] = (R)W.u; sW[2][ly][lx] = (R)W.v; sW[3][ly][lx] = (R)W.w;
+        sW[4][ly][lx] = (R)W.P; sW[5][ly][lx] = (R)W.Bx; sW[6][ly][lx] = (R)W.By; sW[7][ly][lx] = (R)W.Bz;
+    }
+    __syncthreads();
+    const IdealEOS2D<R> eos{(R)g.gm1, (R)g.gamma, (R)g.floor_rho};
+    const R frho = (R)g.floor_rho, fP = (R)g.floor_P;
+
+    // 2. x-faces i0..i0+TX of rows j0..j0+TY-1 (M2:706-725)
+    for (int f = tid; f < (TX + 1) * TY; f += 256) {
+        const int fx = f % (TX + 1), fy = f / (TX + 1);
+        const int i = i0 + fx, j = j0 + fy;
+        if (i > Nx || j >= Ny) continue;
+        const int ly = fy + 2;
+        Prim<R> a, b, WL, WR;
+        plm_mc<R>(lds_prim<R>(sW, ly, fx), lds_prim<R>(sW, ly, fx + 1), lds_prim<R>(sW, ly, fx + 2), 0, frho, fP, a,
+                  WL);  // right state of cell i-1
+        plm_mc<R>(lds_prim<R>(sW, ly, fx + 1), lds_prim<R>(sW, ly, fx + 2), lds_prim<R>(sW, ly, fx + 3), 0, frho, fP,
+                  WR, b);  // left state of cell i
+        const double Bn = U0.Bx[(size_t)j * (Nx + 1) + i];
+        R F[7];
+        hlld_x<R>(eos, WL, WR, (R)Bn, fabs(Bn) < 1e-12, F);
+        sFx[0][fy][fx] = F[0]; sFx[1][fy][fx] = F[1]; sFx[2][fy][fx] = F[2];
+        sFx[3][fy][fx] = F[3]; sFx[4][fy][fx] = F[4]; sFx[5][fy][fx] = F[6];
+        if (fx < TX || i == Nx) ezx[(size_t)j * (Nx + 1) + i] = -(double)F[5];
+    }
+    // 3. y-faces j0..j0+TY of columns i0..i0+TX-1 (M2:728-747), by the (u,v)/(Bx,By) swap of M2:540-561
+    for (int f = tid; f < TX * (TY + 1); f += 256) {
+        const int fx = f % TX, fy = f / TX;
+        const int i = i0 + fx, j = j0 + fy;
+        if (i >= Nx || j > Ny) continue;
+        const int lx = fx + 2;
+        Prim<R> a, b, WL, WR;
+        plm_mc<R>(lds_prim<R>(sW, fy, lx), lds_prim<R>(sW, fy + 1, lx), lds_prim<R>(sW, fy + 2, lx), 1, frho, fP, a, WL);
+        plm_mc<R>(lds_prim<R>(sW, fy + 1, lx), lds_prim<R>(sW, fy + 2, lx), lds_prim<R>(sW, fy + 3, lx), 1, frho, fP,
+                  WR, b);
+        const double Bn = U0.By[(size_t)j * Nx + i];
+        const R bn = (R)Bn;
+        const Prim<R> aL{WL.rho, WL.v, WL.u, WL.w, WL.P, bn, WL.Bx, WL.Bz};
+        const Prim<R> aR{WR.rho, WR.v, WR.u, WR.w, WR.P, bn, WR.Bx, WR.Bz};
+        R F[7];
+        hlld_x<R>(eos, aL, aR, bn, fabs(Bn) < 1e-12, F);
+        sFy[0][fy][fx] = F[0]; sFy[1][fy][fx] = F[2]; sFy[2][fy][fx] = F[1];
+        sFy[3][fy][fx] = F[3]; sFy[4][fy][fx] = F[4]; sFy[5][fy][fx] = F[6];
+        if (fy < TY || j == Ny) ezy[(size_t)j * Nx + i] = (double)F[5];
+    }
+    __syncthreads();
+
+    // 4. flux difference and floors with the old faces (M2:751-779)
+    const int tx = tid % TX, ty = tid / TX;
+    const int i = i0 + tx, j = j0 + ty;
+    if (i >= Nx || j >= Ny) return;
+    const size_t q = (size_t)j * Nx + i;
+    const double dt = sc->dt, invdx = 1.0 / g.dx, invdy = 1.0 / g.dy;
+    double d[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++)
+        d[k] = -((double)sFx[k][ty][tx + 1] - (double)sFx[k][ty][tx]) * invdx -
+               ((double)sFy[k][ty + 1][tx] - (double)sFy[k][ty][tx]) * invdy;
+    double rho = U0.rho[q] + dt * d[0];
+    double mx = U0.mx[q] + dt * d[1];
+    double my = U0.my[q] + dt * d[2];
+    double mz = U0.mz[q] + dt * d[3];
+    double E = U0.E[q] + dt * d[4];
+    const double Bz = U0.Bz[q] + dt * d[5];
+    const double bx = 0.5 * (U0.Bx[(size_t)j * (Nx + 1) + i] + U0.Bx[(size_t)j * (Nx + 1) + i + 1]);
+    const double by = 0.5 * (U0.By[(size_t)j * Nx + i] + U0.By[(size_t)(j + 1) * Nx + i]);
+    enforce_floors2d(g, rho, mx, my, mz, E, bx, by, Bz);
+    Us.rho[q] = rho; Us.mx[q] = mx; Us.my[q] = my; Us.mz[q] = mz; Us.E[q] = E; Us.Bz[q] = Bz;
+}
+
+// ------------------------------------------------------------------ corner EMF (M2:781-804)
+__global__ void k_corner2d(Geom2D g, const double* __restrict__ ezx, const double* __restrict__ ezy,
+                           const double* __restrict__ Bx, const double* __restrict__ By, double* __restrict__ ez) {
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x, je = blockIdx.y;
+    const int Nx = g.Nx, Ny = g.Ny;
+    if (ie > Nx) return;
+    const int jm = bmap(je - 1, Ny, g.py), jc = bmap(je, Ny, g.py);
+    const int im = bmap(ie - 1, Nx, g.px), ic = bmap(ie, Nx, g.px);
+    const double ez_ideal = 0.25 * (ezx[(size_t)jm * (Nx + 1) + ie] + ezx[(size_t)jc * (Nx + 1) + ie] +
+                                    ezy[(size_t)je * Nx + im] + ezy[(size_t)je * Nx + ic]);
+    double ez_res = 0.0;
+    if (g.eta > 0.0) {  // Jz_edge, M2:565-575
+        const double dBydx = (By[(size_t)je * Nx + ic] - By[(size_t)je * Nx + im]) / g.dx;
+        const double dBxdy = (Bx[(size_t)jc * (Nx + 1) + ie] - Bx[(size_t)jm * (Nx + 1) + ie]) / g.dy;
+        ez_res = g.eta * (dBydx - dBxdy);
+    }
+    ez[(size_t)je * (Nx + 1) + ie] = ez_ideal + ez_res;
+}
+
+// ------------------------------------------------------------------ CT + sources (+ Heun) kernel
+// B1 of x-face (fi, fj) and y-face (fi, fj) from the corner EMFs (M2:807-830).
+__device__ __forceinline__ double ct_bx(const Geom2D& g, const double* Bx0, const double* ez, int fi, int fj, double dt) {
+    return Bx0[(size_t)fj * (g.Nx + 1) + fi] +
+           (-(dt / g.dy)) * (ez[(size_t)(fj + 1) * (g.Nx + 1) + fi] - ez[(size_t)fj * (g.Nx + 1) + fi]);
+}
+__device__ __forceinline__ double ct_by(const Geom2D& g, const double* By0, const double* ez, int fi, int fj, double dt) {
+    return By0[(size_t)fj * g.Nx + fi] +
+           (dt / g.dx) * (ez[(size_t)fj * (g.Nx + 1) + fi + 1] - ez[(size_t)fj * (g.Nx + 1) + fi]);
+}
+
+template <bool FINAL>
+__global__ __launch_bounds__(256) void k_ct_src2d(Geom2D g, State2D Us, const double* __restrict__ Bx0,
+                                                  const double* __restrict__ By0, const double* __restrict__ ez,
+                                                  const double* __restrict__ gx, const double* __restrict__ gy,
+                                                  Scalars* __restrict__ sc, State2D Un, State2D Out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    const int Nx = g.Nx, Ny = g.Ny;
+    const bool active = i < Nx;
+    double speed = 0.0;
+    if (active) {
+        const size_t q = (size_t)j * Nx + i;
+        const double dt = sc->dt;
+        const double bxL = ct_bx(g, Bx0, ez, i, j, dt), bxR = ct_bx(g, Bx0, ez, i + 1, j, dt);
+        const double byD = ct_by(g, By0, ez, i, j, dt), byU = ct_by(g, By0, ez, i, j + 1, dt);
+        const double bx1 = 0.5 * (bxL + bxR), by1 = 0.5 * (byD + byU);
+        double rho = Us.rho[q], mx = Us.mx[q], my = Us.my[q], mz = Us.mz[q], E = Us.E[q], Bz = Us.Bz[q];
+        if (g.gravity_on) {  // add_gravity_sources, M2:602-629
+            const double gxq = gx[q], gyq = gy[q];
+            const double u = mx / smax(rho, 1e-300), v = my / smax(rho, 1e-300);
+            mx += dt * rho * gxq;
+            my += dt * rho * gyq;
+            E += dt * rho * (u * gxq + v * gyq);
+            enforce_floors2d(g, rho, mx, my, mz, E, bx1, by1, Bz);
+        }
+        if (g.eta > 0.0) {  // add_joule_heating_and_Bz_resistive, M2:631-686
+            const int iE = bmap(i + 1, Nx, g.px), iW = bmap(i - 1, Nx, g.px);
+            const int jN = bmap(j + 1, Ny, g.py), jS = bmap(j - 1, Ny, g.py);
+            const double BzE = Us.Bz[(size_t)j * Nx + iE], BzW = Us.Bz[(size_t)j * Nx + iW];
+            const double BzN = Us.Bz[(size_t)jN * Nx + i], BzS = Us.Bz[(size_t)jS * Nx + i];
+            const double dBzdy = (BzN - BzS) / (2 * g.dy);
+            const double dBzdx = (BzE - BzW) / (2 * g.dx);
+            const double dBydx = (ct_by(g, By0, ez, iE, j, dt) - byD) / g.dx;
+            const double dBxdy = (ct_bx(g, Bx0, ez, i, jN, dt) - bxL) / g.dy;
+            const double Jx = dBzdy, Jy = -dBzdx, Jz = dBydx - dBxdy;
+            const double J2 = Jx * Jx + Jy * Jy + Jz * Jz;
+            E += dt * g.eta * J2;
+            const double idx2 = 1.0 / (g.dx * g.dx), idy2 = 1.0 / (g.dy * g.dy);
+            const double lap = (BzE - 2 * Bz + BzW) * idx2 + (BzN - 2 * Bz + BzS) * idy2;
+            Bz = Bz + dt * g.eta * lap;
+            enforce_floors2d(g, rho, mx, my, mz, E, bx1, by1, Bz);
+        }
+        if (!FINAL) {
+            Out.rho[q] = rho; Out.mx[q] = mx; Out.my[q] = my; Out.mz[q] = mz; Out.E[q] = E; Out.Bz[q] = Bz;
+            Out.Bx[(size_t)j * (Nx + 1) + i] = bxL;
+            if (i == Nx - 1) Out.Bx[(size_t)j * (Nx + 1) + Nx] = bxR;
+            Out.By[(size_t)j * Nx + i] = byD;
+            if (j == Ny - 1) Out.By[(size_t)Ny * Nx + i] = byU;
+        } else {  // Heun average and floors, M2:848-868
+            rho = 0.5 * (Un.rho[q] + rho);
+            mx = 0.5 * (Un.mx[q] + mx);
+            my = 0.5 * (Un.my[q] + my);
+            mz = 0.5 * (Un.mz[q] + mz);
+            E = 0.5 * (Un.E[q] + E);
+            Bz = 0.5 * (Un.Bz[q] + Bz);
+            const double nbxL = 0.5 * (Un.Bx[(size_t)j * (Nx + 1) + i] + bxL);
+            const double nbxR = 0.5 * (Un.Bx[(size_t)j * (Nx + 1) + i + 1] + bxR);
+            const double nbyD = 0.5 * (Un.By[(size_t)j * Nx + i] + byD);
+            const double nbyU = 0.5 * (Un.By[(size_t)(j + 1) * Nx + i] + byU);
+            const double nbx = 0.5 * (nbxL + nbxR), nby = 0.5 * (nbyD + nbyU);
+            enforce_floors2d(g, rho, mx, my, mz, E, nbx, nby, Bz);
+            Out.rho[q] = rho; Out.mx[q] = mx; Out.my[q] = my; Out.mz[q] = mz; Out.E[q] = E; Out.Bz[q] = Bz;
+            Out.Bx[(size_t)j * (Nx + 1) + i] = nbxL;
+            if (i == Nx - 1) Out.Bx[(size_t)j * (Nx + 1) + Nx] = nbxR;
+            Out.By[(size_t)j * Nx + i] = nbyD;
+            if (j == Ny - 1) Out.By[(size_t)Ny * Nx + i] = nbyU;
+            // next step's compute_dt (M2:579-598) on the final state
+            speed = signal_speed2d(g, cons_to_prim2d(g, rho, mx, my, mz, E, nbx, nby, Bz));
+        }
+    }
+    if (FINAL) {
+        speed = wave_max(speed);
+        if ((threadIdx.x & 63) == 0 && speed > 0.0) atomic_max_pos(&sc->maxs, speed);
+    }
+}
+
+// ------------------------------------------------------------------ dt, diagnostics, small kernels
+__global__ __launch_bounds__(256) void k_speed_max2d(Geom2D g, State2D U, Scalars* __restrict__ sc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    double speed = 0.0;
+    if (i < g.Nx) {
+        const size_t q = (size_t)j * g.Nx + i;
+        const double bx = 0.5 * (U.Bx[(size_t)j * (g.Nx + 1) + i] + U.Bx[(size_t)j * (g.Nx + 1) + i + 1]);
+        const double by = 0.5 * (U.By[(size_t)j * g.Nx + i] + U.By[(size_t)(j + 1) * g.Nx + i]);
+        speed = signal_speed2d(g, cons_to_prim2d(g, U.rho[q], U.mx[q], U.my[q], U.mz[q], U.E[q], bx, by, U.Bz[q]));
+    }
+    speed = wave_max(speed);
+    if ((threadIdx.x & 63) == 0 && speed > 0.0) atomic_max_pos(&sc->maxs, speed);
+}
+
+__global__ void k_reset_maxs(Scalars* sc) { sc->maxs = (unsigned long long)__double_as_longlong(1e-14); }
+
+// dt from the signal-speed maximum (M2:590-597), clamped to t_end (M2:1114); resets the maximum.
+__global__ void k_dt_finalize(Geom2D g, Scalars* sc, double* dts) {
+    const double maxs = __longlong_as_double((long long)sc->maxs);
+    const double h = smin(g.dx, g.dy);
+    const double dt_h = g.cfl * h / maxs;
+    double dt_r = 1e100;
+    if (g.eta > 0.0) dt_r = 0.25 * h * h / g.eta;
+    double dt = smin(dt_h, dt_r);
+    if (sc->t + dt > sc->t_end) dt = sc->t_end - sc->t;
+    sc->dt = dt;
+    sc->t = sc->t + dt;
+    if (dts) dts[sc->step] = dt;
+    sc->step += 1;
+    sc->maxs = (unsigned long long)__double_as_longlong(1e-14);
+}
+
+__global__ void k_set_scalars(Scalars* sc, double dt, double t, double t_end, int reset_step) {
+    sc->dt = dt;
+    sc->t = t;
+    sc->t_end = t_end;
+    if (reset_step) sc->step = 0;
+    sc->maxs = (unsigned long long)__double_as_longlong(1e-14);
+    sc->divb = 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_divb_max2d(Geom2D g, State2D U, Scalars* __restrict__ sc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    double v = 0.0;
+    if (i < g.Nx) {  // divB_cc, M2:873-879 (faces i+1 <= Nx and j+1 <= Ny need no wrapping)
+        const double BxR = U.Bx[(size_t)j * (g.Nx + 1) + i + 1], BxL = U.Bx[(size_t)j * (g.Nx + 1) + i];
+        const double ByU = U.By[(size_t)(j + 1) * g.Nx + i], ByD = U.By[(size_t)j * g.Nx + i];
+        v = fabs((BxR - BxL) / g.dx + (ByU - ByD) / g.dy);
+    }
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0 && v > 0.0) atomic_max_pos((unsigned long long*)&sc->divb, v);
+}
+
+// Batched HLLD for tests (axis 0: hlld_x M2:372; axis 1: hlld_y M2:540).
+template <typename R>
+__global__ void k_hlld_batch(int axis, int n, const double* __restrict__ WLa, const double* __restrict__ WRa,
+                             const double* __restrict__ Bna, double gamma, double floor_rho, double* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double* a = WLa + 8 * (size_t)k;
+    const double* b = WRa + 8 * (size_t)k;
+    const double Bn = Bna[k];
+    const IdealEOS2D<R> eos{(R)(gamma - 1.0), (R)gamma, (R)floor_rho};
+    Prim<R> WL{(R)a[0], (R)a[1], (R)a[2], (R)a[3], (R)a[4], (R)a[5], (R)a[6], (R)a[7]};
+    Prim<R> WR{(R)b[0], (R)b[1], (R)b[2], (R)b[3], (R)b[4], (R)b[5], (R)b[6], (R)b[7]};
+    R F[7];
+    double* o = out + 8 * (size_t)k;
+    if (axis == 0) {
+        hlld_x<R>(eos, WL, WR, (R)Bn, fabs(Bn) < 1e-12, F);
+        for (int c = 0; c < 7; c++) o[c] = (double)F[c];
+        o[7] = -(double)F[5];
+    } else {
+        const Prim<R> aL{WL.rho, WL.v, WL.u, WL.w, WL.P, (R)Bn, WL.Bx, WL.Bz};
+        const Prim<R> aR{WR.rho, WR.v, WR.u, WR.w, WR.P, (R)Bn, WR.Bx, WR.Bz};
+        hlld_x<R>(eos, aL, aR, (R)Bn, fabs(Bn) < 1e-12, F);
+        o[0] = F[0]; o[1] = F[2]; o[2] = F[1]; o[3] = F[3]; o[4] = F[4]; o[5] = F[5]; o[6] = F[6]; o[7] = F[5];
+    }
+}
+
+}  // namespace cc
+
+// ==================================================================== host handle
+using namespace cc;
+
+struct cc_mhd2d {
+    cc_mhd2d_params p{};
+    Geom2D g{};
+    hipStream_t stream = nullptr;
+    State2D S[3]{};  // state sets; cur/aux roles rotate
+    int cur = 0;
+    State2D Us{};    // post-flux cells (only the 6 cell arrays are used)
+    double *ezx = nullptr, *ezy = nullptr, *ez = nullptr;
+    double *gx = nullptr, *gy = nullptr, *g1x = nullptr, *g1y = nullptr;
+    Scalars* sc = nullptr;
+    double* dts_dev = nullptr;
+    int dts_cap = 0;
+    Poisson2D pois;
+    Profiler prof;
+    std::vector<void*> allocs;
+    hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};  // step graph starting from state set k
+    bool graph_ok[3] = {false, false, false};
+
+    size_t N() const { return (size_t)p.Nx * p.Ny; }
+    size_t NFx() const { return (size_t)(p.Nx + 1) * p.Ny; }
+    size_t NFy() const { return (size_t)p.Nx * (p.Ny + 1); }
+    Profiler* pr() { return prof.on ? &prof : nullptr; }
+
+    cc_status alloc(double** ptr, size_t n) {
+        if (hipMalloc((void**)ptr, n * sizeof(double)) != hipSuccess) {
+            set_error(CC_ERR_OOM, "cc_mhd2d: device allocation failed");
+            return CC_ERR_OOM;
+        }
+        allocs.push_back(*ptr);
+        CC_HIP(hipMemsetAsync(*ptr, 0, n * sizeof(double), stream));
+        return CC_OK;
+    }
+    cc_status alloc_state(State2D& s) {
+        CC_TRY(alloc(&s.rho, N())); CC_TRY(alloc(&s.mx, N())); CC_TRY(alloc(&s.my, N()));
+        CC_TRY(alloc(&s.mz, N())); CC_TRY(alloc(&s.E, N())); CC_TRY(alloc(&s.Bz, N()));
+        CC_TRY(alloc(&s.Bx, NFx())); CC_TRY(alloc(&s.By, NFy()));
+        return CC_OK;
+    }
+
+    // One forward-Euler stage: in -> out with gravity (gxp, gyp); FINAL adds the Heun average with `hn`.
+    template <bool FINAL>
+    cc_status stage(const State2D& in, const State2D& out, const double* gxp, const double* gyp, const State2D& hn) {
+        const int Nx = p.Nx, Ny = p.Ny;
+        {
+            ProfScope ps(pr(), stream, "flux");
+            dim3 grid((Nx + 31) / 32, (Ny + 7) / 8);
+            if (p.precision == CC_MIXED)
+                hipLaunchKernelGGL(k_flux2d<float>, grid, dim3(256), 0, stream, g, in, sc, Us, ezx, ezy);
+            else
+                hipLaunchKernelGGL(k_flux2d<double>, grid, dim3(256), 0, stream, g, in, sc, Us, ezx, ezy);
+            CC_CHECK_LAUNCH();
+        }
+        {
+            ProfScope ps(pr(), stream, "corner");
+            hipLaunchKernelGGL(k_corner2d, dim3((Nx + 1 + 127) / 128, Ny + 1), dim3(128), 0, stream, g, ezx, ezy, in.Bx,
+                               in.By, ez);
+            CC_CHECK_LAUNCH();
+        }
+        {
+            ProfScope ps(pr(), stream, "ct");
+            hipLaunchKernelGGL(k_ct_src2d<FINAL>, dim3((Nx + 255) / 256, Ny), dim3(256), 0, stream, g, Us, in.Bx, in.By,
+                               ez, gxp, gyp, sc, hn, out);
+            CC_CHECK_LAUNCH();
+        }
+        return CC_OK;
+    }
+
+    cc_status gravity_of(const double* rho, double* gxp, double* gyp) {
+        if (!p.gravity_on) return CC_OK;  // g stays zero (M2:185)
+        return pois.solve(rho, gxp, gyp, stream, pr());
+    }
+
+    // The Heun step from state set a into set c using set b for U1 (rk2_step, M2:838-869).
+    // With `driver`, gravity and dt come first as in the loop of M2:1112-1114.
+    cc_status heun(int a, int b, int c, bool driver, double* dts) {
+        if (driver) {
+            CC_TRY(gravity_of(S[a].rho, gx, gy));
+            ProfScope ps(pr(), stream, "dt");
+            hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1), 0, stream, g, sc, dts);
+            CC_CHECK_LAUNCH();
+        }
+        CC_TRY(stage<false>(S[a], S[b], gx, gy, S[a]));
+        CC_TRY(gravity_of(S[b].rho, g1x, g1y));
+        CC_TRY(stage<true>(S[b], S[c], g1x, g1y, S[a]));
+        return CC_OK;
+    }
+
+    cc_status speed_max(const State2D& s) {
+        ProfScope ps(pr(), stream, "dt");
+        hipLaunchKernelGGL(k_speed_max2d, dim3((p.Nx + 255) / 256, p.Ny), dim3(256), 0, stream, g, s, sc);
+        CC_CHECK_LAUNCH();
+        return CC_OK;
+    }
+
+    ~cc_mhd2d() {
+        for (auto& ge : graph)
+            if (ge) hipGraphExecDestroy(ge);
+        for (void* a : allocs) hipFree(a);
+        if (sc) hipFree(sc);
+        if (dts_dev) hipFree(dts_dev);
+        pois.destroy();
+        if (stream) hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+cc_status check_params2d(const cc_mhd2d_params* p) {
+    if (!p) { set_error(CC_ERR_INVALID_ARG, "null params"); return CC_ERR_INVALID_ARG; }
+    if (p->Nx < 2 || p->Ny < 2 || p->Nx > 65536 || p->Ny > 65536) {
+        set_error(CC_ERR_INVALID_ARG, "Nx, Ny must be in [2, 65536]");
+        return CC_ERR_INVALID_ARG;
+    }
+    if (!(p->x1 > p->x0) || !(p->y1 > p->y0) || !(p->gamma > 1.0) || p->sor_iters < 0) {
+        set_error(CC_ERR_INVALID_ARG, "invalid domain, gamma or sor_iters");
+        return CC_ERR_INVALID_ARG;
+    }
+    if (p->precision != CC_FP64 && p->precision != CC_MIXED) {
+        set_error(CC_ERR_INVALID_ARG, "precision must be CC_FP64 or CC_MIXED");
+        return CC_ERR_INVALID_ARG;
+    }
+    return CC_OK;
+}
+}  // namespace
+
+extern "C" {
+
+cc_status cc_mhd2d_create(const cc_mhd2d_params* p, cc_mhd2d** out) {
+    if (!out) { set_error(CC_ERR_INVALID_ARG, "null output handle"); return CC_ERR_INVALID_ARG; }
+    *out = nullptr;
+    CC_TRY(check_params2d(p));
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        set_error(CC_ERR_NO_DEVICE, "no HIP device visible");
+        return CC_ERR_NO_DEVICE;
+    }
+    cc_mhd2d* h = new cc_mhd2d();
+    h->p = *p;
+    Geom2D& g = h->g;
+    g.Nx = p->Nx; g.Ny = p->Ny; g.px = p->periodic_x != 0; g.py = p->periodic_y != 0;
+    g.dx = (p->x1 - p->x0) / p->Nx; g.dy = (p->y1 - p->y0) / p->Ny;
+    g.gamma = p->gamma; g.gm1 = p->gamma - 1.0; g.floor_rho = p->floor_rho; g.floor_P = p->floor_P;
+    g.eta = p->eta; g.cfl = p->cfl; g.gravity_on = p->gravity_on != 0;
+    cc_status st = CC_OK;
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_error(CC_ERR_HIP, "hipStreamCreate failed");
+        delete h;
+        return CC_ERR_HIP;
+    }
+    for (int k = 0; k < 3 && st == CC_OK; k++) st = h->alloc_state(h->S[k]);
+    if (st == CC_OK) st = h->alloc_state(h->Us);
+    if (st == CC_OK) st = h->alloc(&h->ezx, h->NFx());
+    if (st == CC_OK) st = h->alloc(&h->ezy, h->NFy());
+    if (st == CC_OK) st = h->alloc(&h->ez, (size_t)(p->Nx + 1) * (p->Ny + 1));
+    if (st == CC_OK) st = h->alloc(&h->gx, h->N());
+    if (st == CC_OK) st = h->alloc(&h->gy, h->N());
+    if (st == CC_OK) st = h->alloc(&h->g1x, h->N());
+    if (st == CC_OK) st = h->alloc(&h->g1y, h->N());
+    if (st == CC_OK && hipMalloc((void**)&h->sc, sizeof(Scalars)) != hipSuccess) {
+        set_error(CC_ERR_OOM, "scalar allocation failed");
+        st = CC_ERR_OOM;
+    }
+    if (st == CC_OK && p->gravity_on)
+        st = h->pois.init(p->Nx, p->Ny, g.px, g.py, g.dx, g.dy, p->G, p->sor_iters, p->sor_omega, p->precision);
+    if (st == CC_OK) {
+        hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, h->stream, h->sc, 0.0, 0.0, 1e300, 1);
+        if (hipStreamSynchronize(h->stream) != hipSuccess) {
+            set_error(CC_ERR_HIP, "stream sync failed in create");
+            st = CC_ERR_HIP;
+        }
+    }
+    if (st != CC_OK) {
+        delete h;
+        return st;
+    }
+    *out = h;
+    return CC_OK;
+}
+
+cc_status cc_mhd2d_destroy(cc_mhd2d* h) {
+    if (!h) return CC_OK;
+    hipStreamSynchronize(h->stream);
+    delete h;
+    return CC_OK;
+}
+
+#define CHECK_H()                                                   \
+    do {                                                            \
+        if (!h) {                                                   \
+            set_error(CC_ERR_INVALID_ARG, "null handle");           \
+            return CC_ERR_INVALID_ARG;                              \
+        }                                                           \
+    } while (0)
+
+cc_status cc_mhd2d_upload(cc_mhd2d* h, const double* rho, const double* mx, const double* my, const double* mz,
+                          const double* E, const double* Bz, const double* Bx_f, const double* By_f) {
+    CHECK_H();
+    if (!rho || !mx || !my || !mz || !E || !Bz || !Bx_f || !By_f) {
+        set_error(CC_ERR_INVALID_ARG, "null state pointer");
+        return CC_ERR_INVALID_ARG;
+    }
+    State2D& s = h->S[h->cur];
+    const size_t n = h->N() * sizeof(double);
+    CC_HIP(hipMemcpyAsync(s.rho, rho, n, hipMemcpyHostToDevice, h->stream));
+    CC_HIP(hipMemcpyAsync(s.mx, mx, n, hipMemcpyHostToDevice, h->stream));
+    CC_HIP(hipMemcpyAsync(s.my, my, n, hipMemcpyHostToDevice, h->stream));
+    CC_HIP(hipMemcpyAsync(s.mz, mz, n, hipMemcpyHostToDevice, h->stream));
+    CC_HIP(hipMemcpyAsync(s.E, E, n, hipMemcpyHostToDevice, h->stream));
+    CC_HIP(hipMemcpyAsync(s.Bz, Bz, n, hipMemcpyHostToDevice, h->stream));
+    CC_HIP(hipMemcpyAsync(s.Bx, Bx_f, h->NFx() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    CC_HIP(hipMemcpyAsync(s.By, By_f, h->NFy() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    CC_HIP(hipStreamSynchronize(h->stream));
+    return CC_OK;
+}
+
+cc_status cc_mhd2d_download(cc_mhd2d* h, double* rho, double* mx, double* my, double* mz, double* E, double* Bz,
+                            double* Bx_f, double* By_f) {
+    CHECK_H();
+    State2D& s = h->S[h->cur];
+    const size_t n = h->N() * sizeof(double);
+    struct { double* dst; const double* src; size_t bytes; } cp[8] = {
+        {rho, s.rho, n}, {mx, s.mx, n}, {my, s.my, n}, {mz, s.mz, n}, {E, s.E, n}, {Bz, s.Bz, n},
+        {Bx_f, s.Bx, h->NFx() * sizeof(double)}, {By_f, s.By, h->NFy() * sizeof(double)}};
+    for (auto& c : cp)
+        if (c.dst) CC_HIP(hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToHost, h->stream));
+    CC_HIP(hipStreamSynchronize(h->stream));
+    return CC_OK;
+}
+
+cc_status cc_mhd2d_gravity(cc_mhd2d* h) {
+    CHECK_H();
+    CC_TRY(h->gravity_of(h->S[h->cur].rho, h->gx, h->gy));
+    return CC_OK;
+}
+
+cc_status cc_mhd2d_get_gravity(cc_mhd2d* h, double* gx, double* gy) {
+    CHECK_H();
+    if (gx) CC_HIP(hipMemcpyAsync(gx, h->gx, h->N() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (gy) CC_HIP(hipMemcpyAsync(gy, h->gy, h->N() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    CC_HIP(hipStreamSynchronize(h->stream));
+    return CC_OK;
+}
+
+cc_status cc_mhd2d_set_gravity(cc_mhd2d* h, const double* gx, const double* gy) {
+    CHECK_H();
+    if (!gx || !gy) { set_error(CC_ERR_INVALID_ARG, "null gravity pointer"); return CC_ERR_INVALID_ARG; }
+    CC_HIP(hipMemcpyAsync(h->gx, gx, h->N() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    CC_HIP(hipMemcpyAsync(h->gy, gy, h->N() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    CC_HIP(hipStreamSynchronize(h->stream));
+    return CC_OK;
+}
+
+cc_status cc_mhd2d_compute_dt(cc_mhd2d* h, double* dt) {
+    CHECK_H();
+    if (!dt) { set_error(CC_ERR_INVALID_ARG, "null dt"); return CC_ERR_INVALID_ARG; }
+    hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, h->stream, h->sc, 0.0, 0.0, 1e300, 1);
+    CC_TRY(h->speed_max(h->S[h->cur]));
+    hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1), 0, h->stream, h->g, h->sc, (double*)nullptr);
+    CC_CHECK_LAUNCH();
+    Scalars s;
+    CC_HIP(hipMemcpyAsync(&s, h->sc, sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));
+    CC_HIP(hipStreamSynchronize(h->stream));
+    *dt = s.dt;
+    return CC_OK;
+}
+
+cc_status cc_mhd2d_stage(cc_mhd2d* h, double dt) {
+    CHECK_H();
+    hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, h->stream, h->sc, dt, 0.0, 1e300, 0);
+    const int a = h->cur, b = (h->cur + 1) % 3;
+    CC_TRY(h->stage<false>(h->S[a], h->S[b], h->gx, h->gy, h->S[a]));
+    h->cur = b;
+    return CC_OK;
+}
+
+cc_status cc_mhd2d_rk2_step(cc_mhd2d* h, double dt) {
+    CHECK_H();
+    if (!std::isfinite(dt) || dt < 0.0) { set_error(CC_ERR_BAD_DT, "dt must be finite and >= 0"); return CC_ERR_BAD_DT; }
+    hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, h->stream, h->sc, dt, 0.0, 1e300, 0);
+    const int a = h->cur, b = (a + 1) % 3, c = (a + 2) % 3;
+    CC_TRY(h->heun(a, b, c, false, nullptr));
+    h->cur = c;
+    return CC_OK;
+}
+
+cc_status cc_mhd2d_run(cc_mhd2d* h, int nsteps, double* t_inout, double t_end, double* dts) {
+    CHECK_H();
+    if (nsteps < 0 || !t_inout) { set_error(CC_ERR_INVALID_ARG, "nsteps < 0 or null t"); return CC_ERR_INVALID_ARG; }
+    if (nsteps == 0) return CC_OK;
+    if (nsteps > h->dts_cap) {
+        if (h->dts_dev) hipFree(h->dts_dev);
+        h->dts_dev = nullptr;
+        if (hipMalloc((void**)&h->dts_dev, (size_t)nsteps * sizeof(double)) != hipSuccess) {
+            set_error(CC_ERR_OOM, "dt log allocation failed");
+            return CC_ERR_OOM;
+        }
+        h->dts_cap = nsteps;
+        for (auto& ge : h->graph)
+            if (ge) { hipGraphExecDestroy(ge); ge = nullptr; }
+        for (auto& ok : h->graph_ok) ok = false;
+    }
+    hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, h->stream, h->sc, 0.0, *t_inout, t_end, 1);
+    CC_TRY(h->speed_max(h->S[h->cur]));
+    const bool use_graph = !h->prof.on;
+    for (int s = 0; s < nsteps; ++s) {
+        const int a = h->cur, b = (a + 1) % 3, c = (a + 2) % 3;
+        if (use_graph) {
+            if (!h->graph_ok[a]) {
+                hipGraph_t gr;
+                CC_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+                cc_status st = h->heun(a, b, c, true, h->dts_dev);
+                hipError_t e = hipStreamEndCapture(h->stream, &gr);
+                if (st != CC_OK) return st;
+                if (e != hipSuccess) { set_error(CC_ERR_HIP, "graph capture failed"); return CC_ERR_HIP; }
+                CC_HIP(hipGraphInstantiate(&h->graph[a], gr, nullptr, nullptr, 0));
+                hipGraphDestroy(gr);
+                h->graph_ok[a] = true;
+            }
+            CC_HIP(hipGraphLaunch(h->graph[a], h->stream));
+        } else {
+            CC_TRY(h->heun(a, b, c, true, h->dts_dev));
+        }
+        h->cur = c;
+    }
+    Scalars s;
+    CC_HIP(hipMemcpyAsync(&s, h->sc, sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));
+    if (dts) CC_HIP(hipMemcpyAsync(dts, h->dts_dev, (size_t)nsteps * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    CC_HIP(hipStreamSynchronize(h->stream));
+    *t_inout = s.t;
+    if (!std::isfinite(s.dt) || s.dt < 0.0) {
+        set_error(CC_ERR_BAD_DT, "non-finite or negative dt on device");
+        return CC_ERR_BAD_DT;
+    }
+    return CC_OK;
+}
+
+cc_status cc_mhd2d_divb_max(cc_mhd2d* h, double* out) {
+    CHECK_H();
+    if (!out) { set_error(CC_ERR_INVALID_ARG, "null output"); return CC_ERR_INVALID_ARG; }
+    hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, h->stream, h->sc, 0.0, 0.0, 1e300, 0);
+    hipLaunchKernelGGL(k_divb_max2d, dim3((h->p.Nx + 255) / 256, h->p.Ny), dim3(256), 0, h->stream, h->g,
+                       h->S[h->cur], h->sc);
+    CC_CHECK_LAUNCH();
+    Scalars s;
+    CC_HIP(hipMemcpyAsync(&s, h->sc, sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));
+    CC_HIP(hipStreamSynchronize(h->stream));
+    *out = s.divb;
+    return CC_OK;
+}
+
+cc_status cc_mhd2d_sync(cc_mhd2d* h) {
+    CHECK_H();
+    CC_HIP(hipStreamSynchronize(h->stream));
+    return CC_OK;
+}
+
+cc_status cc_mhd2d_set_profiling(cc_mhd2d* h, int on) {
+    CHECK_H();
+    h->prof.on = on != 0;
+    return CC_OK;
+}
+
+cc_status cc_mhd2d_kernel_time(cc_mhd2d* h, const char* kernel, int* count, double* total_ms) {
+    CHECK_H();
+    if (!kernel || !count || !total_ms) { set_error(CC_ERR_INVALID_ARG, "null argument"); return CC_ERR_INVALID_ARG; }
+    CC_HIP(hipStreamSynchronize(h->stream));
+    h->prof.get_total(kernel, count, total_ms);
+    return CC_OK;
+}
+
+cc_status cc_mhd2d_reset_profiling(cc_mhd2d* h) {
+    CHECK_H();
+    CC_HIP(hipStreamSynchronize(h->stream));
+    h->prof.reset();
+    return CC_OK;
+}
+
+cc_status cc_hlld2d_batch(int axis, int n, const double* WL, const double* WR, const double* Bn, double gamma,
+                          double floor_rho, int precision, double* out) {
+    if (n < 0 || (axis != 0 && axis != 1) || (n > 0 && (!WL || !WR || !Bn || !out))) {
+        set_error(CC_ERR_INVALID_ARG, "invalid batch arguments");
+        return CC_ERR_INVALID_ARG;
+    }
+    if (n == 0) return CC_OK;
+    double *dL, *dR, *dB, *dO;
+    const size_t b8 = (size_t)n * 8 * sizeof(double);
+    if (hipMalloc(&dL, b8) != hipSuccess) { set_error(CC_ERR_OOM, "batch alloc"); return CC_ERR_OOM; }
+    hipMalloc(&dR, b8);
+    hipMalloc(&dB, (size_t)n * sizeof(double));
+    hipMalloc(&dO, b8);
+    hipMemcpy(dL, WL, b8, hipMemcpyHostToDevice);
+    hipMemcpy(dR, WR, b8, hipMemcpyHostToDevice);
+    hipMemcpy(dB, Bn, (size_t)n * sizeof(double), hipMemcpyHostToDevice);
+    if (precision == CC_MIXED)
+        hipLaunchKernelGGL(k_hlld_batch<float>, dim3((n + 255) / 256), dim3(256), 0, 0, axis, n, dL, dR, dB, gamma,
+                           floor_rho, dO);
+    else
+        hipLaunchKernelGGL(k_hlld_batch<double>, dim3((n + 255) / 256), dim3(256), 0, 0, axis, n, dL, dR, dB, gamma,
+                           floor_rho, dO);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(out, dO, b8, hipMemcpyDeviceToHost);
+    hipFree(dL); hipFree(dR); hipFree(dB); hipFree(dO);
+    if (e != hipSuccess) { set_error(CC_ERR_HIP, hipGetErrorString(e)); return CC_ERR_HIP; }
+    return CC_OK;
+}
+
+cc_status cc_poisson2d_sor(int Nx, int Ny, int periodic_x, int periodic_y, double dx, double dy, double G,
+                           int sor_iters, double omega, int precision, const double* rho, double* gx, double* gy,
+                           double* phi) {
+    if (Nx < 2 || Ny < 2 || !rho || sor_iters < 0 || !(dx > 0) || !(dy > 0)) {
+        set_error(CC_ERR_INVALID_ARG, "invalid Poisson arguments");
+        return CC_ERR_INVALID_ARG;
+    }
+    Poisson2D P;
+    CC_TRY(P.init(Nx, Ny, !!periodic_x, !!periodic_y, dx, dy, G, sor_iters, omega, precision));
+    const size_t N = (size_t)Nx * Ny;
+    double *drho, *dgx, *dgy;
+    if (hipMalloc(&drho, N * 8) != hipSuccess) { set_error(CC_ERR_OOM, "alloc"); return CC_ERR_OOM; }
+    hipMalloc(&dgx, N * 8);
+    hipMalloc(&dgy, N * 8);
+    hipMemcpy(drho, rho, N * 8, hipMemcpyHostToDevice);
+    cc_status st = P.solve(drho, dgx, dgy, 0, nullptr);
+    if (st == CC_OK && phi) st = P.phi_mean_removed(P.phi, 0);
+    hipError_t e = hipDeviceSynchronize();
+    if (st == CC_OK && e == hipSuccess) {
+        if (gx) hipMemcpy(gx, dgx, N * 8, hipMemcpyDeviceToHost);
+        if (gy) hipMemcpy(gy, dgy, N * 8, hipMemcpyDeviceToHost);
+        if (phi) hipMemcpy(phi, P.phi, N * 8, hipMemcpyDeviceToHost);
+    }
+    hipFree(drho); hipFree(dgx); hipFree(dgy);
+    if (st != CC_OK) return st;
+    if (e != hipSuccess) { set_error(CC_ERR_HIP, hipGetErrorString(e)); return CC_ERR_HIP; }
+    return CC_OK;
+}
+
+}  // extern "C"

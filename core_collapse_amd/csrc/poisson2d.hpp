@@ -1,0 +1,63 @@
+// poisson2d.hpp -- 2D Poisson gravity by SOR on gfx950.
+//
+// Restates compute_poisson_gravity_periodic (mhd2d_resistive_hlld_ct_gravity.cpp:178-237):
+// rhs = 4 pi G (rho - mean rho); phi0 = 0; `iters` lexicographic Gauss-Seidel SOR sweeps;
+// g = -central grad(phi).  The per-sweep mean removal of the reference (M2:219-222) commutes
+// with the sweep (the stencil weights sum to the diagonal), so removing it once at the end
+// gives the same iterate and g does not depend on it at all.
+//
+// Two schedules reproduce the lexicographic iterate exactly (up to rounding):
+//  * clamped ("outflow") boundaries on both axes: a time-skewed, LDS-tiled wavefront.  Cell
+//    (i,j) of sweep s runs at wavefront step i+j+2s, i.e. every step updates one colour of a
+//    red-black checkerboard inside a moving band; tiles of 64 x WJ cells x S sweeps are
+//    parallelograms in (i+s, j+s, s) and run level by level (one launch per tile level).
+//  * any periodic axis: the wrap-around neighbours forbid skewing, so sweeps run in order on
+//    one workgroup, anti-diagonal by anti-diagonal.
+// Mixed precision (tiled schedule): each tile evaluates the FP64 residual of its region once,
+// then runs its S sweeps on an FP32 correction in LDS and adds it back in FP64 (tile-local
+// iterative refinement).
+#pragma once
+#include <vector>
+
+#include "common.hpp"
+#include "profiler.hpp"
+
+namespace cc {
+
+struct SorArgs {
+    int Nx, Ny, nsw;
+    int px, py;
+    double idx2, idy2, inv_denom, omega, c4piG;
+    float kxf, kyf, omegaf;  // FP32 correction-sweep coefficients: idx2/denom, idy2/denom, omega
+};
+
+class Poisson2D {
+   public:
+    cc_status init(int Nx, int Ny, bool px, bool py, double dx, double dy, double G, int iters, double omega,
+                   int precision);
+    void destroy();
+    // g = -grad phi(rho); rho, gx, gy are device arrays of Nx*Ny.
+    cc_status solve(const double* rho, double* gx, double* gy, hipStream_t s, Profiler* prof);
+    // phi with its mean removed (after solve), device array.
+    cc_status phi_mean_removed(double* out_dev, hipStream_t s);
+    bool tiled() const { return tiled_; }
+    int levels() const { return (int)level_off_.size() - 1; }
+    int tiles_total() const { return level_off_.empty() ? 0 : level_off_.back(); }
+    ~Poisson2D() { destroy(); }
+
+    int Nx = 0, Ny = 0;
+    bool px = true, py = true, tiled_ = false;
+    double dx = 1, dy = 1, G = 1, omega = 1.8;
+    int iters = 0, precision = 0;
+    double* phi = nullptr;      // Nx*Ny
+    double* partial = nullptr;  // reduction partials
+    double* rhobar = nullptr;   // device scalar(s): [0] mean rho, [1] mean phi
+    int* sched = nullptr;       // (TS,TJ,TI) per tile, grouped by level
+    std::vector<int> level_off_;
+    SorArgs args{};
+};
+
+// Deterministic device mean of n doubles into out (two-pass fixed-order reduction).
+cc_status device_mean(const double* x, size_t n, double* partial, double* out, hipStream_t s);
+
+}  // namespace cc

@@ -1,0 +1,215 @@
+// riemann.hpp -- device-side equations of state, PLM-MC limiter and the HLLD / HLLC Riemann solvers.
+//
+// The solvers are templated on the arithmetic type R (double for the FP64 step, float for the
+// mixed-precision step) and on an EOS policy that supplies the total energy and sound speed, so
+// the 2D ideal-gas code (mhd2d_resistive_hlld_ct_gravity.cpp), the 3D hybrid-EOS code
+// (mhd3d_resistive_hlld_ct.cpp) and the 3D ideal-gas variant share one HLLD.
+#pragma once
+#include "common.hpp"
+
+namespace cc {
+
+template <typename R>
+struct Prim {
+    R rho, u, v, w, P, Bx, By, Bz;
+};
+
+template <typename R>
+struct Flux7 {  // x-normal layout: rho, mx, my, mz, E, By, Bz
+    R f[7];
+};
+
+// MC limiter, M2:106-112 (same in M3:106-112, EH:85-92).
+template <typename R>
+__device__ __forceinline__ R minmod(R a, R b) {
+    if (a * b <= R(0)) return R(0);
+    return (fabs(a) < fabs(b)) ? a : b;
+}
+template <typename R>
+__device__ __forceinline__ R mc_limiter(R dm, R dp) {
+    return minmod<R>(R(0.5) * (dm + dp), minmod<R>(R(2) * dm, R(2) * dp));
+}
+
+// ---------------------------------------------------------------- EOS policies
+// Ideal gas with the expression order of M2 (E = P/(g-1) + 0.5 rho v^2 + 0.5 B^2, M2:357 / M2:397 / M2:482).
+template <typename R>
+struct IdealEOS2D {
+    R gm1;        // gamma - 1
+    R gamma;
+    R floor_rho;
+    __device__ __forceinline__ R total_energy(const Prim<R>& W) const {
+        R B2 = W.Bx * W.Bx + W.By * W.By + W.Bz * W.Bz;
+        return W.P / gm1 + R(0.5) * W.rho * (W.u * W.u + W.v * W.v + W.w * W.w) + R(0.5) * B2;
+    }
+    __device__ __forceinline__ R sound_speed2(const Prim<R>& W) const {  // M2:148-150
+        return gamma * W.P / smax(W.rho, floor_rho);
+    }
+};
+
+// Fast magnetosonic speed along the axis whose field component is Bn (M2:152-174, M3:214-248).
+template <typename R, typename EOS>
+__device__ __forceinline__ R fast_speed(const EOS& eos, const Prim<R>& W, R Bn) {
+    R a2 = eos.sound_speed2(W);
+    R b2 = W.Bx * W.Bx + W.By * W.By + W.Bz * W.Bz;
+    R rho = smax(W.rho, Tiny<R>::v());
+    R vA2 = b2 / rho;
+    R vAn2 = (Bn * Bn) / rho;
+    R disc = (a2 + vA2) * (a2 + vA2) - R(4) * a2 * vAn2;
+    disc = smax(R(0), disc);
+    R cf2 = R(0.5) * ((a2 + vA2) + sqrt(disc));
+    return sqrt(smax(R(0), cf2));
+}
+
+template <typename R, typename EOS>
+__device__ __forceinline__ void flux_ideal_x(const EOS& eos, const Prim<R>& W, R F[7]) {  // M2:351-370
+    R B2 = W.Bx * W.Bx + W.By * W.By + W.Bz * W.Bz;
+    R pt = W.P + R(0.5) * B2;
+    R E = eos.total_energy(W);
+    R vdotB = W.u * W.Bx + W.v * W.By + W.w * W.Bz;
+    F[0] = W.rho * W.u;
+    F[1] = W.rho * W.u * W.u + pt - W.Bx * W.Bx;
+    F[2] = W.rho * W.u * W.v - W.Bx * W.By;
+    F[3] = W.rho * W.u * W.w - W.Bx * W.Bz;
+    F[4] = (E + pt) * W.u - W.Bx * vdotB;
+    F[5] = W.u * W.By - W.v * W.Bx;
+    F[6] = W.u * W.Bz - W.w * W.Bx;
+}
+
+// HLLD, x-normal, with the reference's formulas (M2:372-530 / M3:415-568), including its
+// non-textbook v*, w* denominators, the single-star energy formula reused for the double-star
+// states, the double-star flux through S_L / S_R, and the HLL fallback when |Bn| < 1e-12.
+// bn_small carries that fallback test evaluated on the FP64 face field.
+template <typename R, typename EOS>
+__device__ __forceinline__ void hlld_x(const EOS& eos, Prim<R> WL, Prim<R> WR, R Bn, bool bn_small, R out[7]) {
+    WL.Bx = Bn;
+    WR.Bx = Bn;
+    const R rhoL = WL.rho, rhoR = WR.rho, uL = WL.u, uR = WR.u;
+    R FL[7], FR[7];
+    flux_ideal_x<R>(eos, WL, FL);
+    flux_ideal_x<R>(eos, WR, FR);
+    R cfL = fast_speed<R>(eos, WL, WL.Bx);
+    R cfR = fast_speed<R>(eos, WR, WR.Bx);
+    R SL = smin(uL - cfL, uR - cfR);
+    R SR = smax(uL + cfL, uR + cfR);
+    if (R(0) <= SL) {
+#pragma unroll
+        for (int k = 0; k < 7; k++) out[k] = FL[k];
+        return;
+    }
+    if (R(0) >= SR) {
+#pragma unroll
+        for (int k = 0; k < 7; k++) out[k] = FR[k];
+        return;
+    }
+    const R EL = eos.total_energy(WL), ER = eos.total_energy(WR);
+    if (bn_small) {  // M2:449-467 (HLL)
+        R UL[7] = {WL.rho, WL.rho * WL.u, WL.rho * WL.v, WL.rho * WL.w, EL, WL.By, WL.Bz};
+        R UR[7] = {WR.rho, WR.rho * WR.u, WR.rho * WR.v, WR.rho * WR.w, ER, WR.By, WR.Bz};
+        R inv = R(1) / (SR - SL);
+#pragma unroll
+        for (int k = 0; k < 7; k++) out[k] = (SR * FL[k] - SL * FR[k] + SL * SR * (UR[k] - UL[k])) * inv;
+        return;
+    }
+    const R BL2 = WL.Bx * WL.Bx + WL.By * WL.By + WL.Bz * WL.Bz;
+    const R BR2 = WR.Bx * WR.Bx + WR.By * WR.By + WR.Bz * WR.Bz;
+    const R ptL = WL.P + R(0.5) * BL2, ptR = WR.P + R(0.5) * BR2;
+
+    R denom = rhoL * (SL - uL) - rhoR * (SR - uR);
+    R SM = R(0);
+    if (fabs(denom) > R(1e-14)) SM = (ptR - ptL + rhoL * uL * (SL - uL) - rhoR * uR * (SR - uR)) / denom;
+
+    const R rhoL_s = rhoL * (SL - uL) / (SL - SM);
+    const R rhoR_s = rhoR * (SR - uR) / (SR - SM);
+    const R pt_s = ptL + rhoL * (SL - uL) * (SM - uL);
+    const R sL = SL - uL, sR = SR - uR;
+    const R Bn2 = Bn * Bn;
+    const R dL = rhoL * sL * (SL - SM) - Bn2;
+    const R dR = rhoR * sR * (SR - SM) - Bn2;
+
+    R ByL_s = WL.By, BzL_s = WL.Bz, ByR_s = WR.By, BzR_s = WR.Bz;
+    R vL_s = WL.v, wL_s = WL.w, vR_s = WR.v, wR_s = WR.w;
+    if (fabs(dL) > R(1e-14)) {
+        R num = rhoL * sL * (SL - uL) - Bn2;
+        ByL_s = WL.By * num / dL;
+        BzL_s = WL.Bz * num / dL;
+        R den = rhoL * (SL - uL) * (SL - SM);
+        vL_s = WL.v - Bn * (ByL_s - WL.By) / den;
+        wL_s = WL.w - Bn * (BzL_s - WL.Bz) / den;
+    }
+    if (fabs(dR) > R(1e-14)) {
+        R num = rhoR * sR * (SR - uR) - Bn2;
+        ByR_s = WR.By * num / dR;
+        BzR_s = WR.Bz * num / dR;
+        R den = rhoR * (SR - uR) * (SR - SM);
+        vR_s = WR.v - Bn * (ByR_s - WR.By) / den;
+        wR_s = WR.w - Bn * (BzR_s - WR.Bz) / den;
+    }
+    const R sqrt_rhoL = sqrt(smax(rhoL_s, Tiny<R>::v()));
+    const R sqrt_rhoR = sqrt(smax(rhoR_s, Tiny<R>::v()));
+    const R SLL = SM - fabs(Bn) / sqrt_rhoL;
+    const R SRR = SM + fabs(Bn) / sqrt_rhoR;
+
+    // Select the branch first (M2:526-529) and evaluate only its star state; the others are unused.
+    const bool left = (R(0) < SM);
+    const bool outer = left ? (R(0) < SLL) : !(R(0) < SRR);
+    // component-wise selection keeps everything in registers
+    const R Wr_rho = left ? WL.rho : WR.rho, Wr_u = left ? WL.u : WR.u, Wr_v = left ? WL.v : WR.v;
+    const R Wr_w = left ? WL.w : WR.w, Wr_P = left ? WL.P : WR.P;
+    const R Wr_By = left ? WL.By : WR.By, Wr_Bz = left ? WL.Bz : WR.Bz;
+    const R S = left ? SL : SR;
+    const R Er = left ? EL : ER;
+    const R rho_s = left ? rhoL_s : rhoR_s;
+    R v_s, w_s, By_s, Bz_s;
+    if (outer) {
+        v_s = left ? vL_s : vR_s;
+        w_s = left ? wL_s : wR_s;
+        By_s = left ? ByL_s : ByR_s;
+        Bz_s = left ? BzL_s : BzR_s;
+    } else {  // M2:469-478
+        const R signBn = (Bn >= R(0)) ? R(1) : R(-1);
+        const R denom2 = sqrt_rhoL + sqrt_rhoR;
+        v_s = (sqrt_rhoL * vL_s + sqrt_rhoR * vR_s + (ByR_s - ByL_s) * signBn) / denom2;
+        w_s = (sqrt_rhoL * wL_s + sqrt_rhoR * wR_s + (BzR_s - BzL_s) * signBn) / denom2;
+        By_s = (sqrt_rhoL * ByR_s + sqrt_rhoR * ByL_s + sqrt_rhoL * sqrt_rhoR * (vR_s - vL_s) * signBn) / denom2;
+        Bz_s = (sqrt_rhoL * BzR_s + sqrt_rhoR * BzL_s + sqrt_rhoL * sqrt_rhoR * (wR_s - wL_s) * signBn) / denom2;
+    }
+    // U_star (M2:486-505) and flux_from_state (M2:507-519); Wr.Bx = Bn on both sides
+    const R Br2 = Bn * Bn + Wr_By * Wr_By + Wr_Bz * Wr_Bz;
+    const R ptr = Wr_P + R(0.5) * Br2;
+    const R vdotB_s = SM * Bn + v_s * By_s + w_s * Bz_s;
+    const R vdotB_w = Wr_u * Bn + Wr_v * Wr_By + Wr_w * Wr_Bz;
+    const R Es = ((S - Wr_u) * Er - (ptr * Wr_u - Bn * vdotB_w) + (pt_s * SM - Bn * vdotB_s)) / (S - SM);
+    const R Ust[7] = {rho_s, rho_s * SM, rho_s * v_s, rho_s * w_s, Es, By_s, Bz_s};
+    const R U0[7] = {Wr_rho, Wr_rho * Wr_u, Wr_rho * Wr_v, Wr_rho * Wr_w, Er, Wr_By, Wr_Bz};
+#pragma unroll
+    for (int k = 0; k < 7; k++) out[k] = (left ? FL[k] : FR[k]) + S * (Ust[k] - U0[k]);
+}
+
+// PLM-MC reconstruction of the centre cell c from its neighbours m, p along the sweep axis
+// (M2:290-336).  'ax' selects the normal field component (0: Bx, 1: By, 2: Bz), which is not
+// reconstructed.  Returns the left (minus) and right (plus) face states of the centre cell.
+template <typename R>
+__device__ __forceinline__ void plm_mc(const Prim<R>& Wm, const Prim<R>& Wc, const Prim<R>& Wp, int ax, R floor_rho,
+                                       R floor_P, Prim<R>& WL, Prim<R>& WR) {
+    R sr = mc_limiter<R>(Wc.rho - Wm.rho, Wp.rho - Wc.rho);
+    R su = mc_limiter<R>(Wc.u - Wm.u, Wp.u - Wc.u);
+    R sv = mc_limiter<R>(Wc.v - Wm.v, Wp.v - Wc.v);
+    R sw = mc_limiter<R>(Wc.w - Wm.w, Wp.w - Wc.w);
+    R sP = mc_limiter<R>(Wc.P - Wm.P, Wp.P - Wc.P);
+    R sBx = (ax == 0) ? R(0) : mc_limiter<R>(Wc.Bx - Wm.Bx, Wp.Bx - Wc.Bx);
+    R sBy = (ax == 1) ? R(0) : mc_limiter<R>(Wc.By - Wm.By, Wp.By - Wc.By);
+    R sBz = (ax == 2) ? R(0) : mc_limiter<R>(Wc.Bz - Wm.Bz, Wp.Bz - Wc.Bz);
+    WL.rho = Wc.rho - R(0.5) * sr; WR.rho = Wc.rho + R(0.5) * sr;
+    WL.u = Wc.u - R(0.5) * su;     WR.u = Wc.u + R(0.5) * su;
+    WL.v = Wc.v - R(0.5) * sv;     WR.v = Wc.v + R(0.5) * sv;
+    WL.w = Wc.w - R(0.5) * sw;     WR.w = Wc.w + R(0.5) * sw;
+    WL.P = Wc.P - R(0.5) * sP;     WR.P = Wc.P + R(0.5) * sP;
+    // the normal component keeps the centre value (its slope is zero above; M2:306, 330)
+    WL.Bx = Wc.Bx - R(0.5) * sBx;  WR.Bx = Wc.Bx + R(0.5) * sBx;
+    WL.By = Wc.By - R(0.5) * sBy;  WR.By = Wc.By + R(0.5) * sBy;
+    WL.Bz = Wc.Bz - R(0.5) * sBz;  WR.Bz = Wc.Bz + R(0.5) * sBz;
+    WL.rho = smax(WL.rho, floor_rho); WR.rho = smax(WR.rho, floor_rho);
+    WL.P = smax(WL.P, floor_P);       WR.P = smax(WR.P, floor_P);
+}
+
+}  // namespace cc

@@ -1,0 +1,132 @@
+/*
+ * ccamd.h -- C ABI of the MI355X (gfx950) solver-step library libccamd.so.
+ *
+ * The reference (MohhamedRafi/core_collapse) has no plugin or FFI API: each program is a
+ * zero-argument executable whose step is a set of internal-linkage C++ functions.  Every
+ * entry point below replaces one of those functions (cited as file:line) with plain
+ * pointers and sizes; state crosses the boundary in the reference's own SoA layouts
+ * (cells i-fastest, N+1 faces per face axis, FP64).  The library owns all device buffers;
+ * host buffers stay caller-owned and are copied in and out.
+ *
+ * Threading: one handle = one HIP stream; a handle is not thread-safe, independent handles
+ * are.  Step calls are asynchronous on the handle's stream; download / dt / diagnostics
+ * calls synchronise it.  No call exits the process; errors are returned as cc_status and
+ * described by cc_last_error() (thread-local).
+ *
+ * Files cited: M2 = mhd2d_resistive_hlld_ct_gravity.cpp, M3 = mhd3d_resistive_hlld_ct.cpp,
+ *              EH = euler2d_gravity_hllc.cpp, NS = navier_stokes_2d_hllc_poisson.cpp.
+ */
+#ifndef CCAMD_H
+#define CCAMD_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    CC_OK = 0,
+    CC_ERR_INVALID_ARG = 1, /* bad size / null pointer / unsupported parameter combination */
+    CC_ERR_NON_POW2 = 2,    /* FFT Poisson needs power-of-two grids (NS:614-617 returns 1) */
+    CC_ERR_BAD_DT = 3,      /* non-finite or non-positive dt (EH:699-702 returns 1)         */
+    CC_ERR_HIP = 4,         /* HIP runtime error                                            */
+    CC_ERR_OOM = 5,         /* device allocation failed                                     */
+    CC_ERR_NO_DEVICE = 6    /* no gfx950 device visible                                     */
+} cc_status;
+
+typedef enum {
+    CC_FP64 = 0, /* every step in FP64: the reference arithmetic                                   */
+    CC_MIXED = 1 /* FP32 reconstruction, Riemann fluxes and Poisson sweeps; FP64 conserved update,
+                    pressure recovery, face B / CT and the Poisson residual                      */
+} cc_precision;
+
+/* Last error message of the calling thread ("" if none). */
+const char* cc_last_error(void);
+/* Library version (major*10000 + minor*100 + patch). */
+int cc_version(void);
+/* Name and compute-unit count of the current device (for logs). */
+cc_status cc_device_info(char* name, int name_len, int* cu_count);
+
+/* ============================================================================================
+ * 2D resistive MHD + SOR self-gravity (M2).
+ * ========================================================================================== */
+
+/* The hot-path subset of struct Params (M2:44-72); the IC knobs (M2:74-85) live with the ICs. */
+typedef struct cc_mhd2d_params {
+    double gamma, cfl, x0, x1, y0, y1;
+    int Nx, Ny;
+    double floor_rho, floor_P;
+    int periodic_x, periodic_y;
+    double eta;
+    int gravity_on;
+    double G;
+    int sor_iters;    /* M2:71 */
+    double sor_omega; /* M2:72 */
+    int precision;    /* cc_precision */
+} cc_mhd2d_params;
+
+typedef struct cc_mhd2d cc_mhd2d;
+
+/* Allocates device state, temporaries and the SOR schedule for one grid. */
+cc_status cc_mhd2d_create(const cc_mhd2d_params* p, cc_mhd2d** out);
+cc_status cc_mhd2d_destroy(cc_mhd2d* h);
+
+/* State in the reference layout (M2:88-95): cells q = j*Nx+i; Bx_f[(Nx+1)*Ny] at j*(Nx+1)+i_f;
+ * By_f[Nx*(Ny+1)] at j_f*Nx+i.  upload copies host -> device, download device -> host (syncs). */
+cc_status cc_mhd2d_upload(cc_mhd2d* h, const double* rho, const double* mx, const double* my, const double* mz,
+                          const double* E, const double* Bz, const double* Bx_f, const double* By_f);
+cc_status cc_mhd2d_download(cc_mhd2d* h, double* rho, double* mx, double* my, double* mz, double* E, double* Bz,
+                            double* Bx_f, double* By_f);
+
+/* g = -grad(phi), phi from 4*pi*G*(rho - mean rho) by lexicographic SOR on the current state's rho
+ * (compute_poisson_gravity_periodic, M2:178-237; called at M2:1112).  The result is kept on the
+ * device for the next cc_mhd2d_rk2_step / cc_mhd2d_stage. */
+cc_status cc_mhd2d_gravity(cc_mhd2d* h);
+cc_status cc_mhd2d_get_gravity(cc_mhd2d* h, double* gx, double* gy);
+cc_status cc_mhd2d_set_gravity(cc_mhd2d* h, const double* gx, const double* gy);
+
+/* CFL / resistive time step of the current state (compute_dt, M2:579-598).  Syncs. */
+cc_status cc_mhd2d_compute_dt(cc_mhd2d* h, double* dt);
+
+/* One forward-Euler stage in place, with the stored g (advance_stage, M2:690-834). */
+cc_status cc_mhd2d_stage(cc_mhd2d* h, double dt);
+
+/* One Heun step in place with the stored g, including the inner Poisson solve on U1
+ * (rk2_step, M2:838-869). */
+cc_status cc_mhd2d_rk2_step(cc_mhd2d* h, double dt);
+
+/* nsteps iterations of the driver loop M2:1111-1117 (gravity, dt, clamp to t_end, rk2_step),
+ * with dt kept on the device and the loop captured in HIP graphs.  *t_inout holds t on entry
+ * and the final t on return; dts (nullable, nsteps doubles) receives each step's dt.  Syncs. */
+cc_status cc_mhd2d_run(cc_mhd2d* h, int nsteps, double* t_inout, double t_end, double* dts);
+
+/* max |div B| over cells of the face field (divB_cc, M2:873-879, printed at M2:1119-1126).  Syncs. */
+cc_status cc_mhd2d_divb_max(cc_mhd2d* h, double* out);
+
+/* Wait for all queued work of the handle. */
+cc_status cc_mhd2d_sync(cc_mhd2d* h);
+
+/* Per-kernel timing with HIP events recorded on the handle's stream around each launch
+ * (kernel names: "sor", "flux", "corner", "ct", "grad", "mean", "dt").  Enabling adds an
+ * event pair per launch; totals accumulate until reset. */
+cc_status cc_mhd2d_set_profiling(cc_mhd2d* h, int on);
+cc_status cc_mhd2d_kernel_time(cc_mhd2d* h, const char* kernel, int* count, double* total_ms);
+cc_status cc_mhd2d_reset_profiling(cc_mhd2d* h);
+
+/* Batched HLLD fluxes on independent face states (hlld_x M2:372-530 for axis 0, hlld_y
+ * M2:540-561 for axis 1), ideal EOS.  W* are n x 8 {rho,u,v,w,P,Bx,By,Bz}; out is n x 8:
+ * axis 0 {frho,fmx,fmy,fmz,fE,fBy,fBz,Ez}, axis 1 {frho,fmx,fmy,fmz,fE,fBx,fBz,Ez}. */
+cc_status cc_hlld2d_batch(int axis, int n, const double* WL, const double* WR, const double* Bn, double gamma,
+                          double floor_rho, int precision, double* out);
+
+/* Stand-alone Poisson gravity on a host density field (compute_poisson_gravity_periodic,
+ * M2:178-237) with the given boundary flags, sweep count and omega.  phi (nullable) receives
+ * the final iterate with its mean removed, as the reference holds it after its last sweep. */
+cc_status cc_poisson2d_sor(int Nx, int Ny, int periodic_x, int periodic_y, double dx, double dy, double G,
+                           int sor_iters, double omega, int precision, const double* rho, double* gx, double* gy,
+                           double* phi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CCAMD_H */

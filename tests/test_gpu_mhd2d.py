@@ -1,0 +1,206 @@
+"""GPU parity of the 2D MHD step (libccamd.so through its C ABI) against the CPU oracle.
+
+The oracle restates mhd2d_resistive_hlld_ct_gravity.cpp bit for bit (pinned in
+test_oracle_mhd2d.py).  GPU FP64 differs from it only by FMA contraction and the order of a few
+reductions, so FP64 comparisons use relative tolerances of 1e-9..1e-12 (stated per test); the
+mixed-precision path is held to the 1e-4 relative L1 density bar of the benchmark.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from core_collapse_amd import CC_FP64, CC_MIXED
+from core_collapse_amd import ic
+from core_collapse_amd import mhd2d as M
+from oracle import bind as OB
+from tests.helpers import m2_arrays, m2_dict, m2_oparams, oracle, random_mhd2d_state, rel_err, rel_l1
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_prims(rng, n):
+    W = np.empty((n, 8))
+    W[:, 0] = rng.uniform(0.1, 5.0, n)
+    W[:, 1:4] = rng.normal(0, 1.0, (n, 3))
+    W[:, 4] = rng.uniform(0.01, 3.0, n)
+    W[:, 5:8] = rng.normal(0, 1.0, (n, 3))
+    return W
+
+
+def _oracle_hlld(axis, WL, WR, Bn, gamma=5.0 / 3.0):
+    o = oracle()
+    p = OB.m2_params(gamma=gamma)
+    out = np.zeros((len(Bn), 8))
+    fn = o.or_m2_hlld_x if axis == 0 else o.or_m2_hlld_y
+    for k in range(len(Bn)):
+        r = np.zeros(8)
+        fn(C.byref(p), OB.ptr(np.ascontiguousarray(WL[k])), OB.ptr(np.ascontiguousarray(WR[k])),
+           C.c_double(Bn[k]), OB.ptr(r))
+        out[k] = r
+    return out
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_hlld_batch_fp64_matches_oracle(axis):
+    rng = np.random.default_rng(11 + axis)
+    n = 4000
+    WL, WR = _random_prims(rng, n), _random_prims(rng, n)
+    Bn = rng.normal(0, 1.0, n)
+    Bn[::7] = 0.0          # HLL fallback branch (|Bn| < 1e-12)
+    WR[::11] = WL[::11]    # uniform states: exact physical flux in every branch
+    got = M.hlld_batch(axis, WL, WR, Bn)
+    want = _oracle_hlld(axis, WL, WR, Bn)
+    scale = np.abs(want).max(axis=1, keepdims=True) + 1e-12
+    err = np.abs(got - want) / scale
+    # FP64 with FMA contraction vs the unfused oracle: allow 1e-10 relative per face
+    assert np.quantile(err, 0.999) < 1e-10, np.quantile(err, 0.999)
+    assert err.max() < 1e-6, err.max()
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_hlld_batch_mixed_close_to_oracle(axis):
+    rng = np.random.default_rng(21 + axis)
+    n = 4000
+    WL, WR = _random_prims(rng, n), _random_prims(rng, n)
+    Bn = rng.normal(0, 1.0, n)
+    got = M.hlld_batch(axis, WL, WR, Bn, precision=CC_MIXED)
+    want = _oracle_hlld(axis, WL, WR, Bn)
+    scale = np.abs(want).max(axis=1, keepdims=True) + 1e-12
+    err = np.abs(got - want) / scale
+    assert np.median(err) < 1e-5, np.median(err)
+
+
+def _oracle_poisson(p, rho):
+    o = oracle()
+    op = m2_oparams(p)
+    gx, gy = np.zeros(rho.size), np.zeros(rho.size)
+    o.or_m2_poisson(C.byref(op), OB.ptr(np.ascontiguousarray(rho.reshape(-1))), C.c_double(p.dx), C.c_double(p.dy),
+                    OB.ptr(gx), OB.ptr(gy))
+    return gx.reshape(rho.shape), gy.reshape(rho.shape)
+
+
+@pytest.mark.parametrize("Nx,Ny,periodic,iters", [(32, 32, False, 600), (72, 40, False, 600), (130, 67, False, 37),
+                                                   (24, 24, True, 200), (20, 28, True, 600)])
+def test_poisson_sor_fp64_matches_oracle(Nx, Ny, periodic, iters):
+    rng = np.random.default_rng(Nx * 1000 + Ny)
+    p = M.Params(Nx=Nx, Ny=Ny, periodic_x=periodic, periodic_y=periodic, sor_iters=iters, G=1.0)
+    rho = 1.0 + rng.uniform(0, 1, (Ny, Nx)) + 5.0 * np.exp(-((np.arange(Nx)[None, :] - Nx / 2) ** 2 +
+                                                           (np.arange(Ny)[:, None] - Ny / 3) ** 2) / 20.0)
+    gx, gy = M.poisson_sor(rho, p.dx, p.dy, p.G, iters, p.sor_omega, periodic, periodic)
+    ox, oy = _oracle_poisson(p, rho)
+    # same lexicographic iterate; differences are rounding only (FMA, deferred mean removal)
+    assert rel_err(gx, ox) < 1e-9 and rel_err(gy, oy) < 1e-9, (rel_err(gx, ox), rel_err(gy, oy))
+
+
+def test_poisson_sor_mixed_close_to_oracle():
+    rng = np.random.default_rng(5)
+    Nx, Ny = 96, 80
+    p = M.Params(Nx=Nx, Ny=Ny, periodic_x=False, periodic_y=False, sor_iters=600, G=1.0)
+    rho = 1.0 + rng.uniform(0, 1, (Ny, Nx))
+    gx, gy = M.poisson_sor(rho, p.dx, p.dy, p.G, 600, 1.8, False, False, precision=CC_MIXED)
+    ox, oy = _oracle_poisson(p, rho)
+    assert rel_err(gx, ox) < 1e-5 and rel_err(gy, oy) < 1e-5, (rel_err(gx, ox), rel_err(gy, oy))
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_advance_stage_matches_oracle(periodic):
+    p = M.Params(Nx=40, Ny=24, periodic_x=periodic, periodic_y=periodic, eta=2e-3, G=1.0, sor_iters=50)
+    st = random_mhd2d_state(p, 3)
+    rng = np.random.default_rng(9)
+    gx, gy = rng.normal(0, 0.5, (p.Ny, p.Nx)), rng.normal(0, 0.5, (p.Ny, p.Nx))
+    dt = 2e-3
+    with M.Mhd2D(p) as h:
+        h.upload(st)
+        h.set_gravity(gx, gy)
+        h.advance_stage(dt)
+        got = h.download()
+    o = oracle()
+    S0 = m2_arrays(p, st)
+    S1 = S0.copy()
+    o.or_m2_advance_stage(C.byref(m2_oparams(p)), C.byref(S0.cstate()), C.byref(S1.cstate()),
+                          OB.ptr(np.ascontiguousarray(gx.reshape(-1))), OB.ptr(np.ascontiguousarray(gy.reshape(-1))),
+                          C.c_double(p.dx), C.c_double(p.dy), C.c_double(dt))
+    want = m2_dict(p, S1)
+    for f in M.FIELDS:
+        assert rel_err(got[f], want[f]) < 1e-11, (f, rel_err(got[f], want[f]))
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_rk2_step_matches_oracle(periodic):
+    p = M.Params(Nx=32, Ny=36, periodic_x=periodic, periodic_y=periodic, eta=1e-3, G=2.0, sor_iters=120)
+    st = random_mhd2d_state(p, 4)
+    dt = 1.5e-3
+    with M.Mhd2D(p) as h:
+        h.upload(st)
+        h.compute_poisson_gravity_periodic()
+        h.rk2_step(dt)
+        got = h.download()
+    o = oracle()
+    op = m2_oparams(p)
+    S = m2_arrays(p, st)
+    gx, gy = np.zeros(p.Nx * p.Ny), np.zeros(p.Nx * p.Ny)
+    o.or_m2_poisson(C.byref(op), OB.ptr(S.a["rho"]), C.c_double(p.dx), C.c_double(p.dy), OB.ptr(gx), OB.ptr(gy))
+    o.or_m2_rk2_step(C.byref(op), C.byref(S.cstate()), OB.ptr(gx), OB.ptr(gy), C.c_double(p.dx), C.c_double(p.dy),
+                     C.c_double(dt))
+    want = m2_dict(p, S)
+    for f in M.FIELDS:
+        assert rel_err(got[f], want[f]) < 1e-10, (f, rel_err(got[f], want[f]))
+
+
+def test_compute_dt_matches_oracle():
+    p = M.Params(Nx=48, Ny=40)
+    st = random_mhd2d_state(p, 6)
+    with M.Mhd2D(p) as h:
+        h.upload(st)
+        dt = h.compute_dt()
+    o = oracle()
+    S = m2_arrays(p, st)
+    want = o.or_m2_compute_dt(C.byref(m2_oparams(p)), C.byref(S.cstate()), C.c_double(p.dx), C.c_double(p.dy))
+    assert abs(dt - want) <= 1e-14 * want, (dt, want)
+
+
+@pytest.mark.parametrize("N,nsteps", [(48, 8)])
+def test_collapse_run_matches_oracle(N, nsteps):
+    """The benchmark's Gaussian-collapse problem (clamped boundaries, gravity, resistivity),
+    nsteps of the driver loop with device-side dt, against the oracle's loop."""
+    p = ic.collapse2d_params(N)
+    st = ic.collapse2d_ic(p)
+    with M.Mhd2D(p) as h:
+        h.upload(st)
+        t, dts = h.run(nsteps, 0.0, 1e300, return_dts=True)
+        got = h.download()
+    o = oracle()
+    S = m2_arrays(p, st)
+    odts = np.zeros(nsteps)
+    ot = o.or_m2_run(C.byref(m2_oparams(p)), C.byref(S.cstate()), C.c_double(p.dx), C.c_double(p.dy), nsteps,
+                     C.c_double(0.0), C.c_double(1e300), OB.ptr(odts))
+    want = m2_dict(p, S)
+    assert rel_err(dts, odts) < 1e-11, (dts, odts)
+    assert abs(t - ot) < 1e-11 * ot
+    for f in M.FIELDS:
+        assert rel_err(got[f], want[f]) < 1e-9, (f, rel_err(got[f], want[f]))
+
+
+def test_divb_stays_at_roundoff():
+    p = M.Params(Nx=64, Ny=64, eta=1e-3, G=4.0, sor_iters=100)
+    st = ic.mhd2d_reference_ic(p)
+    with M.Mhd2D(p) as h:
+        h.upload(st)
+        d0 = h.divB_max()
+        h.run(10)
+        d1 = h.divB_max()
+    assert d0 < 1e-11 and d1 < 1e-11, (d0, d1)
+
+
+def test_mixed_vs_fp64_l1_density():
+    N, nsteps = 96, 20
+    out = {}
+    for prec in (CC_FP64, CC_MIXED):
+        p = ic.collapse2d_params(N, precision=prec)
+        with M.Mhd2D(p) as h:
+            h.upload(ic.collapse2d_ic(p))
+            h.run(nsteps)
+            out[prec] = h.download()
+    e = rel_l1(out[CC_MIXED]["rho"], out[CC_FP64]["rho"])
+    assert e <= 1e-4, e

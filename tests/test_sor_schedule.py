@@ -1,0 +1,139 @@
+"""CPU proof of the GPU SOR schedule (core_collapse_amd/csrc/poisson2d.hip).
+
+The tiled kernel claims to reproduce lexicographic Gauss-Seidel SOR (M2:207-218) with clamped
+boundaries.  Here the kernel's schedule is emulated cell by cell in Python (same tile geometry,
+same level order, same per-cell formula) and compared bit for bit with the plain lexicographic
+loop; a second check shows that no two tiles of one level touch a cell that the other writes,
+so the order in which the GPU runs the tiles of a level cannot change the result.
+"""
+import numpy as np
+import pytest
+
+
+def cell_update(phi, rhs, i, j, Nx, Ny, idx2, idy2, inv_denom, omega):
+    pE = phi[j, min(i + 1, Nx - 1)]
+    pW = phi[j, max(i - 1, 0)]
+    pN = phi[min(j + 1, Ny - 1), i]
+    pS = phi[max(j - 1, 0), i]
+    pc = phi[j, i]
+    pnew = ((pE + pW) * idx2 + (pN + pS) * idy2 - rhs[j, i]) * inv_denom
+    phi[j, i] = pc + omega * (pnew - pc)
+
+
+def lexicographic(rhs, nsw, idx2, idy2, omega):
+    Ny, Nx = rhs.shape
+    phi = np.zeros_like(rhs)
+    inv = 1.0 / (2.0 * (idx2 + idy2))
+    for _ in range(nsw):
+        for j in range(Ny):
+            for i in range(Nx):
+                cell_update(phi, rhs, i, j, Nx, Ny, idx2, idy2, inv, omega)
+    return phi
+
+
+def tile_levels(Nx, Ny, nsw, W, WJ, S):
+    """The host-side schedule of Poisson2D::init: non-empty tiles grouped by TS+TJ+TI."""
+    nTS = (nsw + S - 1) // S
+    nTI = (Nx + nsw - 1 + W - 1) // W
+    nTJ = (Ny + nsw - 1 + WJ - 1) // WJ
+    levels = {}
+    for TS in range(nTS):
+        s0, s1 = S * TS, min(S * TS + S, nsw)
+        for TJ in range(nTJ):
+            for TI in range(nTI):
+                ok = any(W * TI - s + W - 1 >= 0 and W * TI - s <= Nx - 1 and WJ * TJ - s + WJ - 1 >= 0 and
+                         WJ * TJ - s <= Ny - 1 for s in range(s0, s1))
+                if ok:
+                    levels.setdefault(TS + TJ + TI, []).append((TS, TJ, TI))
+    return [levels[k] for k in sorted(levels)]
+
+
+def tile_cells(tile, Nx, Ny, nsw, W, WJ, S):
+    """(step, sweep, i, j) of every update of a tile, in the kernel's step order."""
+    TS, TJ, TI = tile
+    s0, s1 = S * TS, min(S * TS + S, nsw)
+    out = []
+    for d in range(W + WJ - 1):
+        for s in range(s0, s1):
+            for lane in range(W):
+                lj = d - lane
+                if not (0 <= lj < WJ):
+                    continue
+                i, j = W * TI - s + lane, WJ * TJ - s + lj
+                if 0 <= i < Nx and 0 <= j < Ny:
+                    out.append((d, s, i, j))
+    return out
+
+
+def tile_updates(i, j, s0, s1, TI, TJ, W, WJ):
+    """The kernel's write-back predicate (tile_updates in poisson2d.hip)."""
+    lo = max(s0, W * TI - i, WJ * TJ - j)
+    hi = min(s1 - 1, W * TI - i + W - 1, WJ * TJ - j + WJ - 1)
+    return lo <= hi
+
+
+def tiled(rhs, nsw, idx2, idy2, omega, W, WJ, S, reverse_levels=False):
+    """The kernel: copy the tile's box, update the copy in step order, write back the cells the
+    predicate selects."""
+    Ny, Nx = rhs.shape
+    phi = np.zeros_like(rhs)
+    inv = 1.0 / (2.0 * (idx2 + idy2))
+    for lev in tile_levels(Nx, Ny, nsw, W, WJ, S):
+        for tile in (reversed(lev) if reverse_levels else lev):
+            TS, TJ, TI = tile
+            s0, s1 = S * TS, min(S * TS + S, nsw)
+            box = phi.copy()
+            for (_, _, i, j) in tile_cells(tile, Nx, Ny, nsw, W, WJ, S):
+                cell_update(box, rhs, i, j, Nx, Ny, idx2, idy2, inv, omega)
+            ib0, ib1 = max(0, W * TI - (s1 - 1) - 1), min(Nx - 1, W * TI - s0 + W)
+            jb0, jb1 = max(0, WJ * TJ - (s1 - 1) - 1), min(Ny - 1, WJ * TJ - s0 + WJ)
+            changed = box != phi
+            for j in range(jb0, jb1 + 1):
+                for i in range(ib0, ib1 + 1):
+                    if tile_updates(i, j, s0, s1, TI, TJ, W, WJ):
+                        phi[j, i] = box[j, i]
+            # every cell the tile changed lies in its box and passes the predicate
+            cj, ci = np.nonzero(changed)
+            assert all(ib0 <= i <= ib1 and jb0 <= j <= jb1 and tile_updates(i, j, s0, s1, TI, TJ, W, WJ)
+                       for i, j in zip(ci, cj))
+    return phi
+
+
+@pytest.mark.parametrize("Nx,Ny,nsw,W,WJ,S", [(9, 7, 5, 4, 3, 2), (11, 13, 9, 4, 4, 3), (6, 10, 7, 3, 5, 4),
+                                               (70, 66, 18, 64, 64, 16)])
+def test_tiled_schedule_equals_lexicographic_bitwise(Nx, Ny, nsw, W, WJ, S):
+    rng = np.random.default_rng(Nx * 100 + Ny)
+    rhs = rng.standard_normal((Ny, Nx))
+    idx2, idy2 = float(Nx * Nx), float(Ny * Ny)
+    ref = lexicographic(rhs, nsw, idx2, idy2, 1.8)
+    for rev in (False, True):
+        got = tiled(rhs, nsw, idx2, idy2, 1.8, W, WJ, S, reverse_levels=rev)
+        assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("Nx,Ny,nsw,W,WJ,S", [(9, 7, 9, 4, 3, 2), (13, 11, 12, 4, 4, 3), (150, 140, 40, 64, 64, 16)])
+def test_no_conflicts_within_a_level(Nx, Ny, nsw, W, WJ, S):
+    for lev in tile_levels(Nx, Ny, nsw, W, WJ, S):
+        writes, reads = [], []
+        for tile in lev:
+            cells = tile_cells(tile, Nx, Ny, nsw, W, WJ, S)
+            w = {(i, j) for (_, _, i, j) in cells}
+            r = set(w)
+            for (_, _, i, j) in cells:
+                r |= {(min(i + 1, Nx - 1), j), (max(i - 1, 0), j), (i, min(j + 1, Ny - 1)), (i, max(j - 1, 0))}
+            writes.append(w)
+            reads.append(r)
+        for a in range(len(lev)):
+            for b in range(len(lev)):
+                if a != b:
+                    assert not (writes[a] & reads[b]), (lev[a], lev[b])
+
+
+def test_every_update_happens_once():
+    Nx, Ny, nsw, W, WJ, S = 37, 29, 23, 8, 6, 4
+    count = {}
+    for lev in tile_levels(Nx, Ny, nsw, W, WJ, S):
+        for tile in lev:
+            for (_, s, i, j) in tile_cells(tile, Nx, Ny, nsw, W, WJ, S):
+                count[(s, i, j)] = count.get((s, i, j), 0) + 1
+    assert len(count) == nsw * Nx * Ny and set(count.values()) == {1}
