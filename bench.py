@@ -1,0 +1,215 @@
+#!/usr/bin/env python3
+"""Benchmark of the solver step on one MI355X.
+
+Workload: 2D resistive MHD, HLLD + constrained transport + SOR self-gravity (two 600-sweep Poisson
+solves per Heun step), Gaussian collapse on a 2048 x 2048 grid over [-0.5,0.5]^2 with clamped
+boundaries, gamma 5/3, G 1, eta 1e-4, CFL 0.4 (the 2D MHD program's step with the collapse IC).
+The same K steps run once in plain FP64 and once in the mixed form (FP32 reconstruction, Riemann
+solver and SOR sweeps; FP64 update, B/CT and Poisson residual), from the same seeded IC.
+
+A zone-update is one cell advanced one full Heun step including its two Poisson solves.
+`value` is the mixed-precision zone-updates/s with all state resident in HBM; the FP64 figure,
+the ratio, the mixed-vs-FP64 relative L1 density error and max|div B| ride along.
+
+roofline: dominant kernel = the tiled SOR kernel (k_sor_tile_*), timed with HIP events on the
+handle's stream around every launch inside the timed region.  Algorithmic bytes per launch =
+(cell-sweeps in the launch) x 24 B (FP64: read phi, read rhs, write phi) or x 12 B (mixed: FP32
+phi and rhs) -- the per-sweep figure of the survey's B_alg.  The kernel keeps its tile in LDS for
+16 sweeps, so its algorithmic rate can exceed the HBM peak; `traffic` (measured HBM bytes per
+launch, from rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/) shows the real HBM load.
+
+cpu_baseline: the reference program's own step functions (compiled from its sources in
+oracle/_ref, single thread) on a bounded sample of the same problem (see "sample").
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "zone-updates/s mixed vs FP64 on 1 MI355X at ≤1e-4 L1 err; % HBM roofline"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--N", type=int, default=2048)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound on the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (for counter calibration) and exit")
+    return ap.parse_args()
+
+
+def run_gpu(N, precision, warmup, steps):
+    from core_collapse_amd import ic
+    from core_collapse_amd import mhd2d as M
+    p = ic.collapse2d_params(N, precision=precision)
+    st = ic.collapse2d_ic(p)
+    h = M.Mhd2D(p)
+    h.upload(st)
+    t = h.run(warmup) if warmup > 0 else 0.0
+    h.set_profiling(True)
+    h.reset_profiling()
+    h.sync()
+    t0 = time.perf_counter()
+    t = h.run(steps, t)          # synchronises at the end
+    elapsed = time.perf_counter() - t0
+    prof = {k: h.kernel_time(k) for k in ("sor", "flux", "corner", "ct", "grad", "mean", "dt")}
+    h.set_profiling(False)
+    out = h.download()
+    divb = h.divB_max()
+    sor_tiles = None
+    h.close()
+    return {"elapsed": elapsed, "t": t, "rho": out["rho"], "divb": divb, "prof": prof, "params": p}
+
+
+def sor_cell_sweeps_per_solve(N, iters=600):
+    return float(N) * N * iters
+
+
+def cpu_baseline(N, budget_s):
+    """Time the reference's own compiled step (driver loop M2:1111-1117) on a sample grid."""
+    from core_collapse_amd import ic
+    from oracle import bind as OB
+    kind = "reference"
+    try:
+        lib = OB.Lib("ref_m2")
+    except FileNotFoundError:
+        lib = OB.Lib("oracle")
+        kind = "port"
+    n = 512 if N >= 512 else N
+    p = ic.collapse2d_params(n)
+    st = ic.collapse2d_ic(p)
+    op = OB.m2_params(gamma=p.gamma, cfl=p.cfl, x0=p.x0, x1=p.x1, y0=p.y0, y1=p.y1, Nx=n, Ny=n,
+                      floor_rho=p.floor_rho, floor_P=p.floor_P, periodic_x=0, periodic_y=0, eta=p.eta,
+                      gravity_on=1, G=p.G, sor_iters=p.sor_iters, sor_omega=p.sor_omega)
+    S = OB.M2Arrays(n, n, {k: v.reshape(-1) for k, v in st.items()})
+    steps, t, tsum = 0, 0.0, 0.0
+    while steps < 3 and (steps == 0 or tsum < budget_s):
+        t0 = time.perf_counter()
+        t = lib.or_m2_run(C.byref(op), C.byref(S.cstate()), C.c_double(p.dx), C.c_double(p.dy), 1, C.c_double(t),
+                          C.c_double(1e300), None)
+        tsum += time.perf_counter() - t0
+        steps += 1
+    zups = n * n * steps / tsum
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": zups, "unit": "zone-updates/s", "cores": 1, "kind": kind,
+            "sample": "%d Heun step(s) of the same collapse problem at %dx%d (FP64, 600-sweep SOR x2 per step, "
+                      "%.1f s), single thread, %s, nproc=%d" % (steps, n, n, tsum, cpu, os.cpu_count() or 0)}
+
+
+def load_traffic():
+    path = os.path.join(ROOT, "profiles", "sor_tile_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the solver step is single-GPU; see DESIGN.md "
+              "for how it would shard); got --gpus %d" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    from core_collapse_amd import CC_FP64, CC_MIXED
+    if a.copy_calibration:
+        from core_collapse_amd import _lib
+        ms = C.c_double()
+        nbytes = a.copy_calibration << 20
+        _lib.check(_lib.lib().cc_copy_calibration(nbytes, 5, C.byref(ms)))
+        print(json.dumps({"copy_bytes": nbytes, "read_bytes": nbytes, "write_bytes": nbytes, "avg_ms": ms.value,
+                          "GBps": 2 * nbytes / ms.value / 1e6}))
+        return
+    N, K, W = a.N, a.steps, a.warmup
+    print("bench: %dx%d collapse, warmup %d, timed %d steps" % (N, N, W, K), file=sys.stderr, flush=True)
+    res = {}
+    for prec in (CC_FP64, CC_MIXED):
+        res[prec] = run_gpu(N, prec, W, K)
+        r = res[prec]
+        print("bench: %s %.3f ms/step" % ("fp64" if prec == CC_FP64 else "mixed", 1e3 * r["elapsed"] / K),
+              file=sys.stderr, flush=True)
+    zone_updates = float(N) * N * K
+    v64 = zone_updates / res[CC_FP64]["elapsed"]
+    vmx = zone_updates / res[CC_MIXED]["elapsed"]
+    rho64, rhomx = res[CC_FP64]["rho"], res[CC_MIXED]["rho"]
+    l1 = float(np.abs(rhomx - rho64).sum() / np.abs(rho64).sum())
+    linf = float(np.abs(rhomx - rho64).max() / np.abs(rho64).max())
+
+    def roof(r, bytes_per_sweep):
+        n, ms = r["prof"]["sor"]
+        if n == 0 or ms <= 0:
+            return None
+        sweeps = sor_cell_sweeps_per_solve(N) * 2 * K  # two solves per step
+        bytes_per_launch = sweeps * bytes_per_sweep / n
+        avg_s = ms / n * 1e-3
+        ach = bytes_per_launch / avg_s / 1e9
+        return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_sor_tile", "launches": n, "avg_launch_us": round(avg_s * 1e6, 2),
+                "alg_bytes_per_launch": bytes_per_launch}
+
+    roof_mx = roof(res[CC_MIXED], 12.0)
+    roof_64 = roof(res[CC_FP64], 24.0)
+    tr = load_traffic()
+    if roof_mx and tr and "mixed_bytes_per_launch" in tr:
+        roof_mx["traffic"] = tr["mixed_bytes_per_launch"]
+    if roof_64 and tr and "fp64_bytes_per_launch" in tr:
+        roof_64["traffic"] = tr["fp64_bytes_per_launch"]
+    kernels = {}
+    for prec, tag in ((CC_FP64, "fp64"), (CC_MIXED, "mixed")):
+        kernels[tag] = {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in res[prec]["prof"].items()
+                        if v[0] > 0}
+    cpu = None if a.no_cpu else cpu_baseline(N, a.cpu_seconds)
+    line = {
+        "metric": METRIC,
+        "value": vmx,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": 1e3 * res[CC_MIXED]["elapsed"] / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 fluxes/SOR sweeps + f64 update/CT/residual",
+        "data": "synthetic",
+        "config": {"workload": "2D resistive MHD HLLD+CT+SOR self-gravity, Gaussian collapse %dx%d, clamped "
+                               "boundaries, 600 SOR sweeps x 2 solves per Heun step" % (N, N),
+                   "grid": [N, N], "gamma": 5.0 / 3.0, "G": 1.0, "eta": 1e-4, "cfl": 0.4, "sor_iters": 600,
+                   "sor_omega": 1.8, "seed": 42},
+        "fp64": {"value": v64, "ms_per_step": 1e3 * res[CC_FP64]["elapsed"] / K, "roofline": roof_64,
+                 "max_divB": res[CC_FP64]["divb"]},
+        "mixed_over_fp64": vmx / v64,
+        "l1_rho_rel_err": l1,
+        "linf_rho_rel_err": linf,
+        "max_divB": res[CC_MIXED]["divb"],
+        "roofline": roof_mx,
+        "cpu_baseline": cpu,
+        "kernel_times": kernels,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

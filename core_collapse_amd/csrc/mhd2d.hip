@@ -41,6 +41,15 @@ struct Scalars {
     double divb;
 };
 
+// Block (256 threads) maximum of a non-negative value, written to part[block] by thread 0.
+__device__ __forceinline__ void block_max_to(double v, double* part) {
+    __shared__ double red[4];
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = smax(smax(red[0], red[1]), smax(red[2], red[3]));
+}
+
 // ------------------------------------------------------------------ cell helpers (FP64)
 __device__ __forceinline__ void enforce_floors2d(const Geom2D& g, double& rho, double& mx, double& my, double& mz,
                                                  double& E, double Bx, double By, double Bz) {  // M2:114-124
@@ -209,7 +218,8 @@ template <bool FINAL>
 __global__ __launch_bounds__(256) void k_ct_src2d(Geom2D g, State2D Us, const double* __restrict__ Bx0,
                                                   const double* __restrict__ By0, const double* __restrict__ ez,
                                                   const double* __restrict__ gx, const double* __restrict__ gy,
-                                                  Scalars* __restrict__ sc, State2D Un, State2D Out) {
+                                                  Scalars* __restrict__ sc, State2D Un, State2D Out,
+                                                  double* __restrict__ spd) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     const int Nx = g.Nx, Ny = g.Ny;
     const bool active = i < Nx;
@@ -274,14 +284,11 @@ __global__ __launch_bounds__(256) void k_ct_src2d(Geom2D g, State2D Us, const do
             speed = signal_speed2d(g, cons_to_prim2d(g, rho, mx, my, mz, E, nbx, nby, Bz));
         }
     }
-    if (FINAL) {
-        speed = wave_max(speed);
-        if ((threadIdx.x & 63) == 0 && speed > 0.0) atomic_max_pos(&sc->maxs, speed);
-    }
+    if (FINAL) block_max_to(speed, spd);
 }
 
 // ------------------------------------------------------------------ dt, diagnostics, small kernels
-__global__ __launch_bounds__(256) void k_speed_max2d(Geom2D g, State2D U, Scalars* __restrict__ sc) {
+__global__ __launch_bounds__(256) void k_speed_max2d(Geom2D g, State2D U, double* __restrict__ spd) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     double speed = 0.0;
     if (i < g.Nx) {
@@ -290,15 +297,23 @@ __global__ __launch_bounds__(256) void k_speed_max2d(Geom2D g, State2D U, Scalar
         const double by = 0.5 * (U.By[(size_t)j * g.Nx + i] + U.By[(size_t)(j + 1) * g.Nx + i]);
         speed = signal_speed2d(g, cons_to_prim2d(g, U.rho[q], U.mx[q], U.my[q], U.mz[q], U.E[q], bx, by, U.Bz[q]));
     }
-    speed = wave_max(speed);
-    if ((threadIdx.x & 63) == 0 && speed > 0.0) atomic_max_pos(&sc->maxs, speed);
+    block_max_to(speed, spd);
 }
 
 __global__ void k_reset_maxs(Scalars* sc) { sc->maxs = (unsigned long long)__double_as_longlong(1e-14); }
 
-// dt from the signal-speed maximum (M2:590-597), clamped to t_end (M2:1114); resets the maximum.
-__global__ void k_dt_finalize(Geom2D g, Scalars* sc, double* dts) {
-    const double maxs = __longlong_as_double((long long)sc->maxs);
+// dt from the signal-speed maximum (M2:590-597; maxs seeded with 1e-14 as at M2:581) over the
+// per-block partial maxima of the last speed pass, clamped to t_end (M2:1114).
+__global__ __launch_bounds__(256) void k_dt_finalize(Geom2D g, Scalars* sc, double* dts,
+                                                     const double* __restrict__ spd, int nspd) {
+    __shared__ double red[4];
+    double m = 1e-14;
+    for (int k = threadIdx.x; k < nspd; k += 256) m = smax(m, spd[k]);
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double maxs = smax(smax(red[0], red[1]), smax(red[2], red[3]));
     const double h = smin(g.dx, g.dy);
     const double dt_h = g.cfl * h / maxs;
     double dt_r = 1e100;
@@ -309,7 +324,6 @@ __global__ void k_dt_finalize(Geom2D g, Scalars* sc, double* dts) {
     sc->t = sc->t + dt;
     if (dts) dts[sc->step] = dt;
     sc->step += 1;
-    sc->maxs = (unsigned long long)__double_as_longlong(1e-14);
 }
 
 __global__ void k_set_scalars(Scalars* sc, double dt, double t, double t_end, int reset_step) {
@@ -364,6 +378,8 @@ __global__ void k_hlld_batch(int axis, int n, const double* __restrict__ WLa, co
 // ==================================================================== host handle
 using namespace cc;
 
+static constexpr int kDtLog = 1 << 16;
+
 struct cc_mhd2d {
     cc_mhd2d_params p{};
     Geom2D g{};
@@ -374,8 +390,9 @@ struct cc_mhd2d {
     double *ezx = nullptr, *ezy = nullptr, *ez = nullptr;
     double *gx = nullptr, *gy = nullptr, *g1x = nullptr, *g1y = nullptr;
     Scalars* sc = nullptr;
-    double* dts_dev = nullptr;
-    int dts_cap = 0;
+    double* dts_dev = nullptr;  // device log of per-step dt (kDtLog entries)
+    double* spd = nullptr;      // per-block signal-speed maxima of the last speed pass
+    int nspd = 0;
     Poisson2D pois;
     Profiler prof;
     std::vector<void*> allocs;
@@ -425,7 +442,7 @@ struct cc_mhd2d {
         {
             ProfScope ps(pr(), stream, "ct");
             hipLaunchKernelGGL(k_ct_src2d<FINAL>, dim3((Nx + 255) / 256, Ny), dim3(256), 0, stream, g, Us, in.Bx, in.By,
-                               ez, gxp, gyp, sc, hn, out);
+                               ez, gxp, gyp, sc, hn, out, spd);
             CC_CHECK_LAUNCH();
         }
         return CC_OK;
@@ -442,7 +459,7 @@ struct cc_mhd2d {
         if (driver) {
             CC_TRY(gravity_of(S[a].rho, gx, gy));
             ProfScope ps(pr(), stream, "dt");
-            hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1), 0, stream, g, sc, dts);
+            hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(256), 0, stream, g, sc, dts, spd, nspd);
             CC_CHECK_LAUNCH();
         }
         CC_TRY(stage<false>(S[a], S[b], gx, gy, S[a]));
@@ -453,7 +470,7 @@ struct cc_mhd2d {
 
     cc_status speed_max(const State2D& s) {
         ProfScope ps(pr(), stream, "dt");
-        hipLaunchKernelGGL(k_speed_max2d, dim3((p.Nx + 255) / 256, p.Ny), dim3(256), 0, stream, g, s, sc);
+        hipLaunchKernelGGL(k_speed_max2d, dim3((p.Nx + 255) / 256, p.Ny), dim3(256), 0, stream, g, s, spd);
         CC_CHECK_LAUNCH();
         return CC_OK;
     }
@@ -463,7 +480,6 @@ struct cc_mhd2d {
             if (ge) hipGraphExecDestroy(ge);
         for (void* a : allocs) hipFree(a);
         if (sc) hipFree(sc);
-        if (dts_dev) hipFree(dts_dev);
         pois.destroy();
         if (stream) hipStreamDestroy(stream);
     }
@@ -521,6 +537,9 @@ cc_status cc_mhd2d_create(const cc_mhd2d_params* p, cc_mhd2d** out) {
     if (st == CC_OK) st = h->alloc(&h->gy, h->N());
     if (st == CC_OK) st = h->alloc(&h->g1x, h->N());
     if (st == CC_OK) st = h->alloc(&h->g1y, h->N());
+    if (st == CC_OK) st = h->alloc(&h->dts_dev, kDtLog);
+    h->nspd = ((p->Nx + 255) / 256) * p->Ny;
+    if (st == CC_OK) st = h->alloc(&h->spd, (size_t)h->nspd);
     if (st == CC_OK && hipMalloc((void**)&h->sc, sizeof(Scalars)) != hipSuccess) {
         set_error(CC_ERR_OOM, "scalar allocation failed");
         st = CC_ERR_OOM;
@@ -620,7 +639,7 @@ cc_status cc_mhd2d_compute_dt(cc_mhd2d* h, double* dt) {
     if (!dt) { set_error(CC_ERR_INVALID_ARG, "null dt"); return CC_ERR_INVALID_ARG; }
     hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, h->stream, h->sc, 0.0, 0.0, 1e300, 1);
     CC_TRY(h->speed_max(h->S[h->cur]));
-    hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1), 0, h->stream, h->g, h->sc, (double*)nullptr);
+    hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(256), 0, h->stream, h->g, h->sc, (double*)nullptr, h->spd, h->nspd);
     CC_CHECK_LAUNCH();
     Scalars s;
     CC_HIP(hipMemcpyAsync(&s, h->sc, sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));
@@ -652,17 +671,15 @@ cc_status cc_mhd2d_run(cc_mhd2d* h, int nsteps, double* t_inout, double t_end, d
     CHECK_H();
     if (nsteps < 0 || !t_inout) { set_error(CC_ERR_INVALID_ARG, "nsteps < 0 or null t"); return CC_ERR_INVALID_ARG; }
     if (nsteps == 0) return CC_OK;
-    if (nsteps > h->dts_cap) {
-        if (h->dts_dev) hipFree(h->dts_dev);
-        h->dts_dev = nullptr;
-        if (hipMalloc((void**)&h->dts_dev, (size_t)nsteps * sizeof(double)) != hipSuccess) {
-            set_error(CC_ERR_OOM, "dt log allocation failed");
-            return CC_ERR_OOM;
+    if (nsteps > kDtLog) {  // the device dt log holds kDtLog steps: run in chunks
+        double t = *t_inout;
+        for (int done = 0; done < nsteps;) {
+            const int n = std::min(kDtLog, nsteps - done);
+            CC_TRY(cc_mhd2d_run(h, n, &t, t_end, dts ? dts + done : nullptr));
+            done += n;
         }
-        h->dts_cap = nsteps;
-        for (auto& ge : h->graph)
-            if (ge) { hipGraphExecDestroy(ge); ge = nullptr; }
-        for (auto& ok : h->graph_ok) ok = false;
+        *t_inout = t;
+        return CC_OK;
     }
     hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, h->stream, h->sc, 0.0, *t_inout, t_end, 1);
     CC_TRY(h->speed_max(h->S[h->cur]));

@@ -126,6 +126,13 @@ cc_status cc_poisson2d_sor(int Nx, int Ny, int periodic_x, int periodic_y, doubl
                            int sor_iters, double omega, int precision, const double* rho, double* gx, double* gy,
                            double* phi);
 
+/* ============================================================================================
+ * Measurement helper (no reference counterpart): a plain device-to-device copy of nbytes,
+ * launched reps times; avg_ms receives the mean kernel time.  Its rocprofv3 FETCH_SIZE /
+ * WRITE_SIZE readings per dispatch calibrate those counters against a known byte count.
+ * ========================================================================================== */
+cc_status cc_copy_calibration(size_t nbytes, int reps, double* avg_ms);
+
 #ifdef __cplusplus
 }
 #endif
