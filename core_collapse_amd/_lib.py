@@ -33,6 +33,16 @@ class Mhd2dParams(C.Structure):
                 ("sor_iters", C.c_int), ("sor_omega", C.c_double), ("precision", C.c_int)]
 
 
+class Mhd3dParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("gamma1", "gamma2", "K1", "rho_nuc", "gamma_th", "rho_bounce",
+                                           "gamma_bounce", "Kbounce", "cfl", "x0", "x1", "y0", "y1", "z0",
+                                           "z1")] + \
+               [("Nx", C.c_int), ("Ny", C.c_int), ("Nz", C.c_int), ("floor_rho", C.c_double),
+                ("floor_P", C.c_double), ("periodic_x", C.c_int), ("periodic_y", C.c_int),
+                ("periodic_z", C.c_int), ("eta", C.c_double), ("eos", C.c_int), ("gamma", C.c_double),
+                ("precision", C.c_int), ("degen_tau", C.c_double)]
+
+
 # name -> (restype, argtypes); every function returns cc_status unless stated
 _PROTOS = {
     "cc_last_error": (C.c_char_p, []),
@@ -55,6 +65,20 @@ _PROTOS = {
     "cc_mhd2d_kernel_time": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int), DP]),
     "cc_mhd2d_reset_profiling": (C.c_int, [C.c_void_p]),
     "cc_hlld2d_batch": (C.c_int, [C.c_int, C.c_int, DP, DP, DP, C.c_double, C.c_double, C.c_int, DP]),
+    "cc_mhd3d_create": (C.c_int, [C.POINTER(Mhd3dParams), C.POINTER(C.c_void_p)]),
+    "cc_mhd3d_destroy": (C.c_int, [C.c_void_p]),
+    "cc_mhd3d_upload": (C.c_int, [C.c_void_p] + [DP] * 8),
+    "cc_mhd3d_download": (C.c_int, [C.c_void_p] + [DP] * 8),
+    "cc_mhd3d_compute_dt": (C.c_int, [C.c_void_p, DP]),
+    "cc_mhd3d_stage": (C.c_int, [C.c_void_p, C.c_double]),
+    "cc_mhd3d_rk2_step": (C.c_int, [C.c_void_p, C.c_double]),
+    "cc_mhd3d_run": (C.c_int, [C.c_void_p, C.c_int, DP, C.c_double, DP]),
+    "cc_mhd3d_divb_max": (C.c_int, [C.c_void_p, DP]),
+    "cc_mhd3d_sync": (C.c_int, [C.c_void_p]),
+    "cc_mhd3d_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "cc_mhd3d_kernel_time": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int), DP]),
+    "cc_mhd3d_reset_profiling": (C.c_int, [C.c_void_p]),
+    "cc_hlld3d_batch": (C.c_int, [C.POINTER(Mhd3dParams), C.c_int, DP, DP, DP, DP]),
     "cc_copy_calibration": (C.c_int, [C.c_size_t, C.c_int, DP]),
     "cc_poisson2d_sor": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                    C.c_int, C.c_double, C.c_int, DP, DP, DP, DP]),

@@ -56,8 +56,8 @@ __host__ __device__ __forceinline__ int bmap(int i, int n, bool periodic) {
 
 // Smallest positive value used where the reference writes 1e-300 (M2:155, M2:444); it underflows in FP32.
 template <typename R> struct Tiny;
-template <> struct Tiny<double> { static __device__ __forceinline__ double v() { return 1e-300; } };
-template <> struct Tiny<float> { static __device__ __forceinline__ float v() { return 1e-30f; } };
+template <> struct Tiny<double> { static __host__ __device__ __forceinline__ double v() { return 1e-300; } };
+template <> struct Tiny<float> { static __host__ __device__ __forceinline__ float v() { return 1e-30f; } };
 
 // Ordered-bits atomic max for non-negative doubles (positive IEEE doubles order like their bit patterns).
 __device__ __forceinline__ void atomic_max_pos(unsigned long long* addr, double v) {

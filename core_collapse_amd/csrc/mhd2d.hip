@@ -10,8 +10,9 @@
 //                 floors (B1); the stage-2 instance also forms the Heun average, its floors
 //                 and the next step's CFL signal-speed maximum (wave64 shuffles + one atomic).
 // Heun (rk2_step, M2:838-869) = stage(U^n, g) -> U1; g1 = Poisson(U1.rho); stage(U1, g1) + average.
-// In the mixed-precision form, reconstruction and the Riemann solver run in FP32 on primitives
-// recovered in FP64; flux differences, conserved updates, EMF averages and CT stay FP64.
+// In the mixed-precision form the HLLD runs in FP32 (recomputed in FP64 for faces near the Alfven
+// degeneracy) on FP64 primitives and PLM states; flux differences, conserved updates, EMF averages
+// and CT stay FP64.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -19,6 +20,7 @@
 
 #include "poisson2d.hpp"
 #include "riemann.hpp"
+#include "stepcommon.hpp"
 
 namespace cc {
 
@@ -32,23 +34,6 @@ struct State2D {
     double *rho, *mx, *my, *mz, *E, *Bz;  // Nx*Ny
     double *Bx, *By;                      // (Nx+1)*Ny, Nx*(Ny+1)
 };
-
-struct Scalars {
-    unsigned long long maxs;  // ordered bits of the running signal-speed maximum (compute_dt, M2:581)
-    int step;
-    int pad;
-    double dt, t, t_end;
-    double divb;
-};
-
-// Block (256 threads) maximum of a non-negative value, written to part[block] by thread 0.
-__device__ __forceinline__ void block_max_to(double v, double* part) {
-    __shared__ double red[4];
-    v = wave_max(v);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = smax(smax(red[0], red[1]), smax(red[2], red[3]));
-}
 
 // ------------------------------------------------------------------ cell helpers (FP64)
 __device__ __forceinline__ void enforce_floors2d(const Geom2D& g, double& rho, double& mx, double& my, double& mz,
@@ -77,31 +62,41 @@ __device__ __forceinline__ Prim<double> cons_to_prim2d(const Geom2D& g, double r
 
 __device__ __forceinline__ double signal_speed2d(const Geom2D& g, const Prim<double>& W) {  // M2:585-587
     IdealEOS2D<double> eos{g.gm1, g.gamma, g.floor_rho};
-    const double sx = fabs(W.u) + fast_speed<double>(eos, W, W.Bx);
-    const double sy = fabs(W.v) + fast_speed<double>(eos, W, W.By);
+    const NoAux ax{};
+    const double sx = fabs(W.u) + fast_speed<double>(eos, W, W.Bx, ax);
+    const double sy = fabs(W.v) + fast_speed<double>(eos, W, W.By, ax);
     return smax(sx, sy);
 }
 
 // ------------------------------------------------------------------ flux + FV kernel
-template <typename R>
-struct FluxTile {
-    static constexpr int TX = 32, TY = 8, RX = TX + 4, RY = TY + 4;
-};
+constexpr int TX2 = 32, TY2 = 8, RX2 = TX2 + 4, RY2 = TY2 + 4;
 
-template <typename R>
-__device__ __forceinline__ Prim<R> lds_prim(const R (*sW)[FluxTile<R>::RY][FluxTile<R>::RX], int ly, int lx) {
-    return Prim<R>{sW[0][ly][lx], sW[1][ly][lx], sW[2][ly][lx], sW[3][ly][lx],
-                   sW[4][ly][lx], sW[5][ly][lx], sW[6][ly][lx], sW[7][ly][lx]};
+// The HLLD of one face from FP64 face states: FP64, or (MIXED) FP32 with FP64 fallback near the
+// Alfven degeneracy.
+template <bool MIXED>
+__device__ __forceinline__ void face_hlld2d(const Geom2D& g, const Prim<double>& WL, const Prim<double>& WR, double Bn,
+                                            double F[7]) {
+    const IdealEOS2D<double> e64{g.gm1, g.gamma, g.floor_rho};
+    if (MIXED) {
+        const IdealEOS2D<float> e32{(float)g.gm1, (float)g.gamma, (float)g.floor_rho};
+        hlld_x_mixed(e32, e64, WL, WR, Bn, F);
+    } else {
+        hlld_x<double>(e64, WL, WR, Bn, fabs(Bn) < 1e-12, F);
+    }
 }
 
-template <typename R>
+__device__ __forceinline__ Prim<double> lds_prim2(const double (*sW)[RY2][RX2], int ly, int lx) {
+    return Prim<double>{sW[0][ly][lx], sW[1][ly][lx], sW[2][ly][lx], sW[3][ly][lx],
+                        sW[4][ly][lx], sW[5][ly][lx], sW[6][ly][lx], sW[7][ly][lx]};
+}
+
+template <bool MIXED>
 __global__ __launch_bounds__(256) void k_flux2d(Geom2D g, State2D U0, const Scalars* __restrict__ sc, State2D Us,
                                                 double* __restrict__ ezx, double* __restrict__ ezy) {
-    using T = FluxTile<R>;
-    constexpr int TX = T::TX, TY = T::TY, RX = T::RX, RY = T::RY;
-    __shared__ R sW[8][RY][RX];
-    __shared__ R sFx[6][TY][TX + 1];
-    __shared__ R sFy[6][TY + 1][TX];
+    constexpr int TX = TX2, TY = TY2, RX = RX2, RY = RY2;
+    __shared__ double sW[8][RY][RX];
+    __shared__ double sFx[6][TY][TX + 1];
+    __shared__ double sFy[6][TY + 1][TX];
     const int tid = threadIdx.x;
     const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
     const int Nx = g.Nx, Ny = g.Ny;
@@ -114,12 +109,11 @@ __global__ __launch_bounds__(256) void k_flux2d(Geom2D g, State2D U0, const Scal
         const double bx = 0.5 * (U0.Bx[(size_t)j * (Nx + 1) + i] + U0.Bx[(size_t)j * (Nx + 1) + i + 1]);
         const double by = 0.5 * (U0.By[(size_t)j * Nx + i] + U0.By[(size_t)(j + 1) * Nx + i]);
         const Prim<double> W = cons_to_prim2d(g, U0.rho[q], U0.mx[q], U0.my[q], U0.mz[q], U0.E[q], bx, by, U0.Bz[q]);
-        sW[0][ly][lx] = (R)W.rho; sW[1][ly][lx] = (R)W.u; sW[2][ly][lx] = (R)W.v; sW[3][ly][lx] = (R)W.w;
-        sW[4][ly][lx] = (R)W.P; sW[5][ly][lx] = (R)W.Bx; sW[6][ly][lx] = (R)W.By; sW[7][ly][lx] = (R)W.Bz;
+        sW[0][ly][lx] = W.rho; sW[1][ly][lx] = W.u; sW[2][ly][lx] = W.v; sW[3][ly][lx] = W.w;
+        sW[4][ly][lx] = W.P; sW[5][ly][lx] = W.Bx; sW[6][ly][lx] = W.By; sW[7][ly][lx] = W.Bz;
     }
     __syncthreads();
-    const IdealEOS2D<R> eos{(R)g.gm1, (R)g.gamma, (R)g.floor_rho};
-    const R frho = (R)g.floor_rho, fP = (R)g.floor_P;
+    const double frho = g.floor_rho, fP = g.floor_P;
 
     // 2. x-faces i0..i0+TX of rows j0..j0+TY-1 (M2:706-725)
     for (int f = tid; f < (TX + 1) * TY; f += 256) {
@@ -127,17 +121,16 @@ __global__ __launch_bounds__(256) void k_flux2d(Geom2D g, State2D U0, const Scal
         const int i = i0 + fx, j = j0 + fy;
         if (i > Nx || j >= Ny) continue;
         const int ly = fy + 2;
-        Prim<R> a, b, WL, WR;
-        plm_mc<R>(lds_prim<R>(sW, ly, fx), lds_prim<R>(sW, ly, fx + 1), lds_prim<R>(sW, ly, fx + 2), 0, frho, fP, a,
-                  WL);  // right state of cell i-1
-        plm_mc<R>(lds_prim<R>(sW, ly, fx + 1), lds_prim<R>(sW, ly, fx + 2), lds_prim<R>(sW, ly, fx + 3), 0, frho, fP,
-                  WR, b);  // left state of cell i
-        const double Bn = U0.Bx[(size_t)j * (Nx + 1) + i];
-        R F[7];
-        hlld_x<R>(eos, WL, WR, (R)Bn, fabs(Bn) < 1e-12, F);
+        Prim<double> a, b, WL, WR;
+        plm_mc<double>(lds_prim2(sW, ly, fx), lds_prim2(sW, ly, fx + 1), lds_prim2(sW, ly, fx + 2), 0, frho, fP, a,
+                       WL);  // right state of cell i-1
+        plm_mc<double>(lds_prim2(sW, ly, fx + 1), lds_prim2(sW, ly, fx + 2), lds_prim2(sW, ly, fx + 3), 0, frho, fP,
+                       WR, b);  // left state of cell i
+        double F[7];
+        face_hlld2d<MIXED>(g, WL, WR, U0.Bx[(size_t)j * (Nx + 1) + i], F);
         sFx[0][fy][fx] = F[0]; sFx[1][fy][fx] = F[1]; sFx[2][fy][fx] = F[2];
         sFx[3][fy][fx] = F[3]; sFx[4][fy][fx] = F[4]; sFx[5][fy][fx] = F[6];
-        if (fx < TX || i == Nx) ezx[(size_t)j * (Nx + 1) + i] = -(double)F[5];
+        if (fx < TX || i == Nx) ezx[(size_t)j * (Nx + 1) + i] = -F[5];
     }
     // 3. y-faces j0..j0+TY of columns i0..i0+TX-1 (M2:728-747), by the (u,v)/(Bx,By) swap of M2:540-561
     for (int f = tid; f < TX * (TY + 1); f += 256) {
@@ -145,19 +138,18 @@ __global__ __launch_bounds__(256) void k_flux2d(Geom2D g, State2D U0, const Scal
         const int i = i0 + fx, j = j0 + fy;
         if (i >= Nx || j > Ny) continue;
         const int lx = fx + 2;
-        Prim<R> a, b, WL, WR;
-        plm_mc<R>(lds_prim<R>(sW, fy, lx), lds_prim<R>(sW, fy + 1, lx), lds_prim<R>(sW, fy + 2, lx), 1, frho, fP, a, WL);
-        plm_mc<R>(lds_prim<R>(sW, fy + 1, lx), lds_prim<R>(sW, fy + 2, lx), lds_prim<R>(sW, fy + 3, lx), 1, frho, fP,
-                  WR, b);
+        Prim<double> a, b, WL, WR;
+        plm_mc<double>(lds_prim2(sW, fy, lx), lds_prim2(sW, fy + 1, lx), lds_prim2(sW, fy + 2, lx), 1, frho, fP, a, WL);
+        plm_mc<double>(lds_prim2(sW, fy + 1, lx), lds_prim2(sW, fy + 2, lx), lds_prim2(sW, fy + 3, lx), 1, frho, fP,
+                       WR, b);
         const double Bn = U0.By[(size_t)j * Nx + i];
-        const R bn = (R)Bn;
-        const Prim<R> aL{WL.rho, WL.v, WL.u, WL.w, WL.P, bn, WL.Bx, WL.Bz};
-        const Prim<R> aR{WR.rho, WR.v, WR.u, WR.w, WR.P, bn, WR.Bx, WR.Bz};
-        R F[7];
-        hlld_x<R>(eos, aL, aR, bn, fabs(Bn) < 1e-12, F);
+        const Prim<double> aL{WL.rho, WL.v, WL.u, WL.w, WL.P, Bn, WL.Bx, WL.Bz};
+        const Prim<double> aR{WR.rho, WR.v, WR.u, WR.w, WR.P, Bn, WR.Bx, WR.Bz};
+        double F[7];
+        face_hlld2d<MIXED>(g, aL, aR, Bn, F);
         sFy[0][fy][fx] = F[0]; sFy[1][fy][fx] = F[2]; sFy[2][fy][fx] = F[1];
         sFy[3][fy][fx] = F[3]; sFy[4][fy][fx] = F[4]; sFy[5][fy][fx] = F[6];
-        if (fy < TY || j == Ny) ezy[(size_t)j * Nx + i] = (double)F[5];
+        if (fy < TY || j == Ny) ezy[(size_t)j * Nx + i] = F[5];
     }
     __syncthreads();
 
@@ -170,8 +162,7 @@ __global__ __launch_bounds__(256) void k_flux2d(Geom2D g, State2D U0, const Scal
     double d[6];
 #pragma unroll
     for (int k = 0; k < 6; k++)
-        d[k] = -((double)sFx[k][ty][tx + 1] - (double)sFx[k][ty][tx]) * invdx -
-               ((double)sFy[k][ty + 1][tx] - (double)sFy[k][ty][tx]) * invdy;
+        d[k] = -(sFx[k][ty][tx + 1] - sFx[k][ty][tx]) * invdx - (sFy[k][ty + 1][tx] - sFy[k][ty][tx]) * invdy;
     double rho = U0.rho[q] + dt * d[0];
     double mx = U0.mx[q] + dt * d[1];
     double my = U0.my[q] + dt * d[2];
@@ -300,30 +291,13 @@ __global__ __launch_bounds__(256) void k_speed_max2d(Geom2D g, State2D U, double
     block_max_to(speed, spd);
 }
 
-__global__ void k_reset_maxs(Scalars* sc) { sc->maxs = (unsigned long long)__double_as_longlong(1e-14); }
 
-// dt from the signal-speed maximum (M2:590-597; maxs seeded with 1e-14 as at M2:581) over the
-// per-block partial maxima of the last speed pass, clamped to t_end (M2:1114).
+// dt from the signal-speed maximum over the per-block partial maxima of the last speed pass,
+// seeded with 1e-14 as at M2:581.
 __global__ __launch_bounds__(256) void k_dt_finalize(Geom2D g, Scalars* sc, double* dts,
                                                      const double* __restrict__ spd, int nspd) {
-    __shared__ double red[4];
-    double m = 1e-14;
-    for (int k = threadIdx.x; k < nspd; k += 256) m = smax(m, spd[k]);
-    m = wave_max(m);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    const double maxs = smax(smax(red[0], red[1]), smax(red[2], red[3]));
-    const double h = smin(g.dx, g.dy);
-    const double dt_h = g.cfl * h / maxs;
-    double dt_r = 1e100;
-    if (g.eta > 0.0) dt_r = 0.25 * h * h / g.eta;
-    double dt = smin(dt_h, dt_r);
-    if (sc->t + dt > sc->t_end) dt = sc->t_end - sc->t;
-    sc->dt = dt;
-    sc->t = sc->t + dt;
-    if (dts) dts[sc->step] = dt;
-    sc->step += 1;
+    const double maxs = block_reduce_max(spd, nspd, 1e-14);
+    if (threadIdx.x == 0) finalize_dt(sc, maxs, smin(g.dx, g.dy), g.cfl, g.eta, dts);
 }
 
 __global__ void k_set_scalars(Scalars* sc, double dt, double t, double t_end, int reset_step) {
@@ -331,7 +305,6 @@ __global__ void k_set_scalars(Scalars* sc, double dt, double t, double t_end, in
     sc->t = t;
     sc->t_end = t_end;
     if (reset_step) sc->step = 0;
-    sc->maxs = (unsigned long long)__double_as_longlong(1e-14);
     sc->divb = 0.0;
 }
 
@@ -347,8 +320,9 @@ __global__ __launch_bounds__(256) void k_divb_max2d(Geom2D g, State2D U, Scalars
     if ((threadIdx.x & 63) == 0 && v > 0.0) atomic_max_pos((unsigned long long*)&sc->divb, v);
 }
 
-// Batched HLLD for tests (axis 0: hlld_x M2:372; axis 1: hlld_y M2:540).
-template <typename R>
+// Batched HLLD for tests (axis 0: hlld_x M2:372; axis 1: hlld_y M2:540), through the same face
+// routine as the flux kernel.
+template <bool MIXED>
 __global__ void k_hlld_batch(int axis, int n, const double* __restrict__ WLa, const double* __restrict__ WRa,
                              const double* __restrict__ Bna, double gamma, double floor_rho, double* __restrict__ out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -356,19 +330,20 @@ __global__ void k_hlld_batch(int axis, int n, const double* __restrict__ WLa, co
     const double* a = WLa + 8 * (size_t)k;
     const double* b = WRa + 8 * (size_t)k;
     const double Bn = Bna[k];
-    const IdealEOS2D<R> eos{(R)(gamma - 1.0), (R)gamma, (R)floor_rho};
-    Prim<R> WL{(R)a[0], (R)a[1], (R)a[2], (R)a[3], (R)a[4], (R)a[5], (R)a[6], (R)a[7]};
-    Prim<R> WR{(R)b[0], (R)b[1], (R)b[2], (R)b[3], (R)b[4], (R)b[5], (R)b[6], (R)b[7]};
-    R F[7];
+    Geom2D g{};
+    g.gamma = gamma; g.gm1 = gamma - 1.0; g.floor_rho = floor_rho;
+    Prim<double> WL{a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7]};
+    Prim<double> WR{b[0], b[1], b[2], b[3], b[4], b[5], b[6], b[7]};
+    double F[7];
     double* o = out + 8 * (size_t)k;
     if (axis == 0) {
-        hlld_x<R>(eos, WL, WR, (R)Bn, fabs(Bn) < 1e-12, F);
-        for (int c = 0; c < 7; c++) o[c] = (double)F[c];
-        o[7] = -(double)F[5];
+        face_hlld2d<MIXED>(g, WL, WR, Bn, F);
+        for (int c = 0; c < 7; c++) o[c] = F[c];
+        o[7] = -F[5];
     } else {
-        const Prim<R> aL{WL.rho, WL.v, WL.u, WL.w, WL.P, (R)Bn, WL.Bx, WL.Bz};
-        const Prim<R> aR{WR.rho, WR.v, WR.u, WR.w, WR.P, (R)Bn, WR.Bx, WR.Bz};
-        hlld_x<R>(eos, aL, aR, (R)Bn, fabs(Bn) < 1e-12, F);
+        const Prim<double> aL{WL.rho, WL.v, WL.u, WL.w, WL.P, Bn, WL.Bx, WL.Bz};
+        const Prim<double> aR{WR.rho, WR.v, WR.u, WR.w, WR.P, Bn, WR.Bx, WR.Bz};
+        face_hlld2d<MIXED>(g, aL, aR, Bn, F);
         o[0] = F[0]; o[1] = F[2]; o[2] = F[1]; o[3] = F[3]; o[4] = F[4]; o[5] = F[5]; o[6] = F[6]; o[7] = F[5];
     }
 }
@@ -428,9 +403,9 @@ struct cc_mhd2d {
             ProfScope ps(pr(), stream, "flux");
             dim3 grid((Nx + 31) / 32, (Ny + 7) / 8);
             if (p.precision == CC_MIXED)
-                hipLaunchKernelGGL(k_flux2d<float>, grid, dim3(256), 0, stream, g, in, sc, Us, ezx, ezy);
+                hipLaunchKernelGGL(k_flux2d<true>, grid, dim3(256), 0, stream, g, in, sc, Us, ezx, ezy);
             else
-                hipLaunchKernelGGL(k_flux2d<double>, grid, dim3(256), 0, stream, g, in, sc, Us, ezx, ezy);
+                hipLaunchKernelGGL(k_flux2d<false>, grid, dim3(256), 0, stream, g, in, sc, Us, ezx, ezy);
             CC_CHECK_LAUNCH();
         }
         {
@@ -774,10 +749,10 @@ cc_status cc_hlld2d_batch(int axis, int n, const double* WL, const double* WR, c
     hipMemcpy(dR, WR, b8, hipMemcpyHostToDevice);
     hipMemcpy(dB, Bn, (size_t)n * sizeof(double), hipMemcpyHostToDevice);
     if (precision == CC_MIXED)
-        hipLaunchKernelGGL(k_hlld_batch<float>, dim3((n + 255) / 256), dim3(256), 0, 0, axis, n, dL, dR, dB, gamma,
+        hipLaunchKernelGGL(k_hlld_batch<true>, dim3((n + 255) / 256), dim3(256), 0, 0, axis, n, dL, dR, dB, gamma,
                            floor_rho, dO);
     else
-        hipLaunchKernelGGL(k_hlld_batch<double>, dim3((n + 255) / 256), dim3(256), 0, 0, axis, n, dL, dR, dB, gamma,
+        hipLaunchKernelGGL(k_hlld_batch<false>, dim3((n + 255) / 256), dim3(256), 0, 0, axis, n, dL, dR, dB, gamma,
                            floor_rho, dO);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpy(out, dO, b8, hipMemcpyDeviceToHost);

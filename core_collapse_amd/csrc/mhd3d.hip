@@ -1,0 +1,875 @@
+// mhd3d.hip -- gfx950 solver step for 3D resistive MHD with HLLD + constrained transport
+// (restating mhd3d_resistive_hlld_ct.cpp, cited as M3:<line>), with the reference's hybrid
+// cold + thermal + bounce EOS or an ideal gas.
+//
+// One forward-Euler stage (advance_stage, M3:694-1030) is four kernels:
+//   k_flux3d    2.5D sweep: a 32x8 column of cells walks up z; each z-plane's primitives are
+//               recovered once (FP64) into LDS with a 2-cell halo, the x- and y-face fluxes of the
+//               plane come from LDS, the z-face fluxes from a 4-deep register queue of the
+//               column's primitives; flux difference + floors (old faces)      -> U*, 6 face EMFs
+//   k_edge3d    Ex, Ey, Ez edges = 1/4 sum of face EMFs + eta * curl of cell-centred B0
+//   k_ct3d      face-field induction on the three face arrays                  -> B1
+//   k_joule3d   Joule heating from cell-centred B1 differences + floors; the stage-2 instance also
+//               forms the Heun average, its floors and the next step's signal-speed maximum.
+// Mixed precision: primitives and PLM in FP64, the HLLD in FP32, recomputed in FP64 for faces near
+// the Alfven degeneracy (where the reference's star states are ill-conditioned); flux differences,
+// the conserved update, EMF averages and CT in FP64.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "profiler.hpp"
+#include "riemann.hpp"
+#include "stepcommon.hpp"
+
+namespace cc {
+
+struct Geom3D {
+    int Nx, Ny, Nz, px, py, pz;
+    double dx, dy, dz, floor_rho, floor_P, eta, cfl;
+    float tau;  // mixed precision: relative Alfven-degeneracy margin for the FP64 fallback
+};
+
+// FP64 EOS for pressure recovery and floors: hybrid (M3:114-212) or ideal.
+struct Eos3D {
+    int ideal;
+    double gamma;  // ideal
+    HybridEOS<double> hy;
+};
+
+struct State3D {
+    double *rho, *mx, *my, *mz, *E;  // Nx*Ny*Nz, (k*Ny+j)*Nx+i
+    double *Bx, *By, *Bz;            // (Nx+1)Ny Nz, Nx(Ny+1)Nz, Nx Ny(Nz+1)
+};
+
+struct EmfFaces {
+    double *Ey_x, *Ez_x, *Ez_y, *Ex_y, *Ex_z, *Ey_z;
+};
+
+struct Edges {
+    double *Ex, *Ey, *Ez;  // Nx(Ny+1)(Nz+1), (Nx+1)Ny(Nz+1), (Nx+1)(Ny+1)Nz
+};
+
+// ------------------------------------------------------------------ indexing (M3:250-308)
+__device__ __forceinline__ size_t fx_(const Geom3D& g, int i, int j, int k) {
+    return ((size_t)k * g.Ny + j) * (size_t)(g.Nx + 1) + i;
+}
+__device__ __forceinline__ size_t fy_(const Geom3D& g, int i, int j, int k) {
+    return ((size_t)k * (g.Ny + 1) + j) * (size_t)g.Nx + i;
+}
+__device__ __forceinline__ size_t fz_(const Geom3D& g, int i, int j, int k) {
+    return ((size_t)k * g.Ny + j) * (size_t)g.Nx + i;
+}
+// face-index maps: wrap modulo n+1 (periodic) or clamp to [0, n]
+__device__ __forceinline__ int fmap(int i, int n, bool per) { return per ? imod(i, n + 1) : iclamp(i, 0, n); }
+
+__device__ __forceinline__ double get_bx(const Geom3D& g, const double* B, int i, int j, int k) {
+    return B[fx_(g, fmap(i, g.Nx, g.px), bmap(j, g.Ny, g.py), bmap(k, g.Nz, g.pz))];
+}
+__device__ __forceinline__ double get_by(const Geom3D& g, const double* B, int i, int j, int k) {
+    return B[fy_(g, bmap(i, g.Nx, g.px), fmap(j, g.Ny, g.py), bmap(k, g.Nz, g.pz))];
+}
+__device__ __forceinline__ double get_bz(const Geom3D& g, const double* B, int i, int j, int k) {
+    return B[fz_(g, bmap(i, g.Nx, g.px), bmap(j, g.Ny, g.py), fmap(k, g.Nz, g.pz))];
+}
+// cell-centred field from faces with the reference's index handling (M3:280-296)
+__device__ __forceinline__ double bxc(const Geom3D& g, const double* B, int i, int j, int k) {
+    return 0.5 * (get_bx(g, B, i, j, k) + get_bx(g, B, i + 1, j, k));
+}
+__device__ __forceinline__ double byc(const Geom3D& g, const double* B, int i, int j, int k) {
+    return 0.5 * (get_by(g, B, i, j, k) + get_by(g, B, i, j + 1, k));
+}
+__device__ __forceinline__ double bzc(const Geom3D& g, const double* B, int i, int j, int k) {
+    return 0.5 * (get_bz(g, B, i, j, k) + get_bz(g, B, i, j, k + 1));
+}
+
+// ------------------------------------------------------------------ FP64 EOS: floors, recovery
+__device__ __forceinline__ double hy_pressure(const HybridEOS<double>& h, double rho, double e_spec) {  // M3:146-152
+    rho = smax(rho, h.floor_rho);
+    const auto c = h.cold(rho);
+    const double eth = smax(0.0, e_spec - c.ec);
+    const double Pth = (h.gamma_th - 1.0) * rho * eth;
+    return smax(c.Pc + Pth, h.floor_P);
+}
+
+__device__ __forceinline__ void floors3d(const Eos3D& e, double& rho, double& mx, double& my, double& mz, double& E,
+                                         double Bx, double By, double Bz) {
+    const HybridEOS<double>& h = e.hy;
+    rho = smax(rho, h.floor_rho);
+    const double u = mx / rho, v = my / rho, w = mz / rho;
+    const double kin = 0.5 * rho * (u * u + v * v + w * w);
+    const double mag = 0.5 * (Bx * Bx + By * By + Bz * Bz);
+    if (e.ideal) {  // M2:114-124
+        const double P = (e.gamma - 1.0) * (E - kin - mag);
+        if (P < h.floor_P) E = kin + mag + h.floor_P / (e.gamma - 1.0);
+        return;
+    }
+    double eint_tot = E - kin - mag;  // M3:165-183
+    if (eint_tot < 0.0) eint_tot = 0.0;
+    const double e_spec = eint_tot / rho;
+    double P = hy_pressure(h, rho, e_spec);
+    if (P < h.floor_P) P = h.floor_P;
+    const auto c = h.cold(rho);
+    const double Pth = smax(0.0, P - c.Pc);
+    const double eth = (h.gamma_th > 1.0) ? (Pth / ((h.gamma_th - 1.0) * rho)) : 0.0;
+    E = kin + mag + rho * (c.ec + eth);
+}
+
+__device__ __forceinline__ Prim<double> cons_to_prim3d(const Eos3D& e, double rho, double mx, double my, double mz,
+                                                       double E, double Bx, double By, double Bz) {
+    const HybridEOS<double>& h = e.hy;
+    rho = smax(rho, h.floor_rho);
+    const double u = mx / rho, v = my / rho, w = mz / rho;
+    const double kin = 0.5 * rho * (u * u + v * v + w * w);
+    const double mag = 0.5 * (Bx * Bx + By * By + Bz * Bz);
+    double P;
+    if (e.ideal) {  // M2:126-135
+        P = (e.gamma - 1.0) * (E - kin - mag);
+        if (P < h.floor_P) P = h.floor_P;
+    } else {  // M3:185-197
+        double eint_tot = E - kin - mag;
+        if (eint_tot < 0.0) eint_tot = 0.0;
+        const double e_spec = eint_tot / rho;
+        P = hy_pressure(h, rho, e_spec);
+        if (P < h.floor_P) P = h.floor_P;
+    }
+    return Prim<double>{rho, u, v, w, P, Bx, By, Bz};
+}
+
+__device__ __forceinline__ Prim<double> prim_at(const Geom3D& g, const Eos3D& e, const State3D& U, int i, int j,
+                                                int k) {  // get_prim_cc, M3:298-308
+    i = bmap(i, g.Nx, g.px);
+    j = bmap(j, g.Ny, g.py);
+    k = bmap(k, g.Nz, g.pz);
+    const size_t q = ((size_t)k * g.Ny + j) * g.Nx + i;
+    return cons_to_prim3d(e, U.rho[q], U.mx[q], U.my[q], U.mz[q], U.E[q], bxc(g, U.Bx, i, j, k),
+                          byc(g, U.By, i, j, k), bzc(g, U.Bz, i, j, k));
+}
+
+__device__ __forceinline__ double signal_speed3d(const Eos3D& e, const Prim<double>& W) {  // M3:633-636
+    double a2;
+    if (e.ideal) {
+        a2 = e.gamma * W.P / smax(W.rho, e.hy.floor_rho);
+    } else {
+        a2 = e.hy.sound_speed2(W, e.hy.aux(W));
+    }
+    auto fs = [&](double Bn) {
+        const double b2 = W.Bx * W.Bx + W.By * W.By + W.Bz * W.Bz;
+        const double rho = smax(W.rho, 1e-300);
+        const double vA2 = b2 / rho, vAn2 = (Bn * Bn) / rho;
+        double disc = (a2 + vA2) * (a2 + vA2) - 4.0 * a2 * vAn2;
+        disc = smax(0.0, disc);
+        return sqrt(smax(0.0, 0.5 * ((a2 + vA2) + sqrt(disc))));
+    };
+    return smax(smax(fabs(W.u) + fs(W.Bx), fabs(W.v) + fs(W.By)), fabs(W.w) + fs(W.Bz));
+}
+
+// ------------------------------------------------------------------ flux kernel (2.5D)
+constexpr int TX3 = 32, TY3 = 8, KZ3 = 16, RX3 = TX3 + 4, RY3 = TY3 + 4;
+
+template <bool MIXED, typename EOS64, typename EOS32>
+__device__ __forceinline__ void face_hlld3d(const EOS64& e64, const EOS32& e32, const Prim<double>& WL,
+                                            const Prim<double>& WR, double Bn, double F[7], float tau) {
+    if (MIXED)
+        hlld_x_mixed(e32, e64, WL, WR, Bn, F, tau);
+    else
+        hlld_x<double>(e64, WL, WR, Bn, fabs(Bn) < 1e-12, F);
+}
+
+template <bool MIXED, typename EOS64, typename EOS32>
+__global__ __launch_bounds__(256) void k_flux3d(Geom3D g, Eos3D e, EOS64 eos64, EOS32 eos32, State3D U0,
+                                                const Scalars* __restrict__ sc, State3D Us, EmfFaces ef) {
+    __shared__ double sW[2][8][RY3][RX3];
+    __shared__ double sFx[5][TY3][TX3 + 1];
+    __shared__ double sFy[5][TY3 + 1][TX3];
+    const int tid = threadIdx.x;
+    const int tx = tid % TX3, ty = tid / TX3;
+    const int i0 = blockIdx.x * TX3, j0 = blockIdx.y * TY3, k0 = blockIdx.z * KZ3;
+    const int k1 = min(k0 + KZ3, g.Nz);
+    const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool col = i < Nx && j < Ny;
+    const double frho = g.floor_rho, fP = g.floor_P;
+    const double dt = sc->dt, invdx = 1.0 / g.dx, invdy = 1.0 / g.dy, invdz = 1.0 / g.dz;
+
+    auto fill_plane = [&](int buf, int k) {  // primitives of logical plane k with a 2-cell x/y halo
+        for (int c = tid; c < RX3 * RY3; c += 256) {
+            const int lx = c % RX3, ly = c / RX3;
+            const Prim<double> W = prim_at(g, e, U0, i0 - 2 + lx, j0 - 2 + ly, k);
+            sW[buf][0][ly][lx] = W.rho; sW[buf][1][ly][lx] = W.u; sW[buf][2][ly][lx] = W.v;
+            sW[buf][3][ly][lx] = W.w; sW[buf][4][ly][lx] = W.P; sW[buf][5][ly][lx] = W.Bx;
+            sW[buf][6][ly][lx] = W.By; sW[buf][7][ly][lx] = W.Bz;
+        }
+    };
+    auto lds = [&](int buf, int ly, int lx) {
+        return Prim<double>{sW[buf][0][ly][lx], sW[buf][1][ly][lx], sW[buf][2][ly][lx], sW[buf][3][ly][lx],
+                            sW[buf][4][ly][lx], sW[buf][5][ly][lx], sW[buf][6][ly][lx], sW[buf][7][ly][lx]};
+    };
+
+    // z register queue of this column: planes k-2, k-1, k, k+1
+    Prim<double> q0 = prim_at(g, e, U0, i, j, k0 - 2);
+    Prim<double> q1 = prim_at(g, e, U0, i, j, k0 - 1);
+    Prim<double> q2, q3;
+    fill_plane(0, k0);
+    __syncthreads();
+    q2 = lds(0, ty + 2, tx + 2);
+    double Hprev[5] = {0, 0, 0, 0, 0};
+    double dpl[5] = {0, 0, 0, 0, 0};  // in-plane flux difference of the previous plane's cell
+
+    for (int k = k0; k <= k1; ++k) {
+        const int cur = (k - k0) & 1;
+        if (k + 1 < k1) fill_plane(cur ^ 1, k + 1);
+        __syncthreads();
+        q3 = (k + 1 < k1) ? lds(cur ^ 1, ty + 2, tx + 2) : prim_at(g, e, U0, i, j, k + 1);
+
+        // z-face k between cells k-1 and k (M3:777-800), by the (u,v,w)->(w,u,v) map of M3:604-622
+        if (col) {
+            Prim<double> a, b, WL, WR;
+            plm_mc<double>(q0, q1, q2, 2, frho, fP, a, WL);
+            plm_mc<double>(q1, q2, q3, 2, frho, fP, WR, b);
+            const double Bn = U0.Bz[fz_(g, i, j, k)];
+            const Prim<double> aL{WL.rho, WL.w, WL.u, WL.v, WL.P, Bn, WL.Bx, WL.By};
+            const Prim<double> aR{WR.rho, WR.w, WR.u, WR.v, WR.P, Bn, WR.Bx, WR.By};
+            double F[7];
+            face_hlld3d<MIXED>(eos64, eos32, aL, aR, Bn, F, g.tau);
+            const double H[5] = {F[0], F[2], F[3], F[1], F[4]};
+            if (k < k1 || k == Nz) {
+                ef.Ex_z[fz_(g, i, j, k)] = F[6];
+                ef.Ey_z[fz_(g, i, j, k)] = -F[5];
+            }
+            if (k > k0) {  // complete cell k-1 (M3:802-834)
+                const int kc = k - 1;
+                const size_t q = ((size_t)kc * Ny + j) * Nx + i;
+                double d[5];
+#pragma unroll
+                for (int c = 0; c < 5; c++) d[c] = dpl[c] - (H[c] - Hprev[c]) * invdz;
+                double rho = U0.rho[q] + dt * d[0], mx = U0.mx[q] + dt * d[1], my = U0.my[q] + dt * d[2];
+                double mz = U0.mz[q] + dt * d[3], E = U0.E[q] + dt * d[4];
+                floors3d(e, rho, mx, my, mz, E, bxc(g, U0.Bx, i, j, kc), byc(g, U0.By, i, j, kc),
+                         bzc(g, U0.Bz, i, j, kc));
+                Us.rho[q] = rho; Us.mx[q] = mx; Us.my[q] = my; Us.mz[q] = mz; Us.E[q] = E;
+            }
+#pragma unroll
+            for (int c = 0; c < 5; c++) Hprev[c] = H[c];
+        }
+
+        if (k < k1) {
+            // x-faces of plane k (M3:727-750)
+            for (int f = tid; f < (TX3 + 1) * TY3; f += 256) {
+                const int fx = f % (TX3 + 1), fy = f / (TX3 + 1);
+                const int fi = i0 + fx, fj = j0 + fy;
+                if (fi > Nx || fj >= Ny) continue;
+                const int ly = fy + 2;
+                Prim<double> a, b, WL, WR;
+                plm_mc<double>(lds(cur, ly, fx), lds(cur, ly, fx + 1), lds(cur, ly, fx + 2), 0, frho, fP, a, WL);
+                plm_mc<double>(lds(cur, ly, fx + 1), lds(cur, ly, fx + 2), lds(cur, ly, fx + 3), 0, frho, fP, WR, b);
+                double F[7];
+                face_hlld3d<MIXED>(eos64, eos32, WL, WR, U0.Bx[fx_(g, fi, fj, k)], F, g.tau);
+#pragma unroll
+                for (int c = 0; c < 5; c++) sFx[c][fy][fx] = F[c];
+                if (fx < TX3 || fi == Nx) {
+                    ef.Ey_x[fx_(g, fi, fj, k)] = F[6];
+                    ef.Ez_x[fx_(g, fi, fj, k)] = -F[5];
+                }
+            }
+            // y-faces of plane k (M3:752-775), by the (u,v)/(Bx,By) swap of M3:577-595
+            for (int f = tid; f < TX3 * (TY3 + 1); f += 256) {
+                const int fx = f % TX3, fy = f / TX3;
+                const int fi = i0 + fx, fj = j0 + fy;
+                if (fi >= Nx || fj > Ny) continue;
+                const int lx = fx + 2;
+                Prim<double> a, b, WL, WR;
+                plm_mc<double>(lds(cur, fy, lx), lds(cur, fy + 1, lx), lds(cur, fy + 2, lx), 1, frho, fP, a, WL);
+                plm_mc<double>(lds(cur, fy + 1, lx), lds(cur, fy + 2, lx), lds(cur, fy + 3, lx), 1, frho, fP, WR, b);
+                const double Bn = U0.By[fy_(g, fi, fj, k)];
+                const Prim<double> aL{WL.rho, WL.v, WL.u, WL.w, WL.P, Bn, WL.Bx, WL.Bz};
+                const Prim<double> aR{WR.rho, WR.v, WR.u, WR.w, WR.P, Bn, WR.Bx, WR.Bz};
+                double F[7];
+                face_hlld3d<MIXED>(eos64, eos32, aL, aR, Bn, F, g.tau);
+                sFy[0][fy][fx] = F[0]; sFy[1][fy][fx] = F[2]; sFy[2][fy][fx] = F[1];
+                sFy[3][fy][fx] = F[3]; sFy[4][fy][fx] = F[4];
+                if (fy < TY3 || fj == Ny) {
+                    ef.Ez_y[fy_(g, fi, fj, k)] = F[5];
+                    ef.Ex_y[fy_(g, fi, fj, k)] = -F[6];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < 5; c++)
+                dpl[c] = -(sFx[c][ty][tx + 1] - sFx[c][ty][tx]) * invdx - (sFy[c][ty + 1][tx] - sFy[c][ty][tx]) * invdy;
+        }
+        __syncthreads();
+        q0 = q1;
+        q1 = q2;
+        q2 = q3;
+    }
+}
+
+// ------------------------------------------------------------------ edge EMFs (M3:836-939)
+__global__ void k_edge3d(Geom3D g, EmfFaces ef, State3D B0, Edges ed) {
+    const int ia = blockIdx.x * blockDim.x + threadIdx.x, ja = blockIdx.y, ka = blockIdx.z;
+    const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+    if (ia > Nx) return;
+    const bool px = g.px, py = g.py, pz = g.pz;
+    auto exy = [&](int i, int j, int k) { return ef.Ex_y[fy_(g, bmap(i, Nx, px), fmap(j, Ny, py), bmap(k, Nz, pz))]; };
+    auto ezx = [&](int i, int j, int k) { return ef.Ez_x[fx_(g, fmap(i, Nx, px), bmap(j, Ny, py), bmap(k, Nz, pz))]; };
+    auto eyx = [&](int i, int j, int k) { return ef.Ey_x[fx_(g, fmap(i, Nx, px), bmap(j, Ny, py), bmap(k, Nz, pz))]; };
+    auto exz = [&](int i, int j, int k) { return ef.Ex_z[fz_(g, bmap(i, Nx, px), bmap(j, Ny, py), fmap(k, Nz, pz))]; };
+    auto eyz = [&](int i, int j, int k) { return ef.Ey_z[fz_(g, bmap(i, Nx, px), bmap(j, Ny, py), fmap(k, Nz, pz))]; };
+    auto ezy = [&](int i, int j, int k) { return ef.Ez_y[fy_(g, bmap(i, Nx, px), fmap(j, Ny, py), bmap(k, Nz, pz))]; };
+    const double eta = g.eta;
+    if (ia < Nx) {  // Ex(i, jf, kf)
+        const int i = ia, jf = ja, kf = ka;
+        const double ex_ideal = 0.25 * (exy(i, jf, kf - 1) + exy(i, jf, kf) + exz(i, jf - 1, kf) + exz(i, jf, kf));
+        double ex_res = 0.0;
+        if (eta > 0.0) {
+            const double dBzdy = (bzc(g, B0.Bz, i, jf, kf) - bzc(g, B0.Bz, i, jf - 1, kf)) / g.dy;
+            const double dBydz = (byc(g, B0.By, i, jf, kf) - byc(g, B0.By, i, jf, kf - 1)) / g.dz;
+            ex_res = eta * (dBzdy - dBydz);
+        }
+        ed.Ex[((size_t)kf * (Ny + 1) + jf) * Nx + i] = ex_ideal + ex_res;
+    }
+    if (ja < Ny) {  // Ey(if, j, kf)
+        const int i = ia, j = ja, kf = ka;
+        const double ey_ideal = 0.25 * (eyx(i, j, kf - 1) + eyx(i, j, kf) + eyz(i - 1, j, kf) + eyz(i, j, kf));
+        double ey_res = 0.0;
+        if (eta > 0.0) {
+            const double dBxdz = (bxc(g, B0.Bx, i, j, kf) - bxc(g, B0.Bx, i, j, kf - 1)) / g.dz;
+            const double dBzdx = (bzc(g, B0.Bz, i, j, kf) - bzc(g, B0.Bz, i - 1, j, kf)) / g.dx;
+            ey_res = eta * (dBxdz - dBzdx);
+        }
+        ed.Ey[((size_t)kf * Ny + j) * (Nx + 1) + i] = ey_ideal + ey_res;
+    }
+    if (ka < Nz) {  // Ez(if, jf, k)
+        const int i = ia, jf = ja, k = ka;
+        const double ez_ideal = 0.25 * (ezx(i, jf - 1, k) + ezx(i, jf, k) + ezy(i - 1, jf, k) + ezy(i, jf, k));
+        double ez_res = 0.0;
+        if (eta > 0.0) {
+            const double dBydx = (byc(g, B0.By, i, jf, k) - byc(g, B0.By, i - 1, jf, k)) / g.dx;
+            const double dBxdy = (bxc(g, B0.Bx, i, jf, k) - bxc(g, B0.Bx, i, jf - 1, k)) / g.dy;
+            ez_res = eta * (dBydx - dBxdy);
+        }
+        ed.Ez[((size_t)k * (Ny + 1) + jf) * (Nx + 1) + i] = ez_ideal + ez_res;
+    }
+}
+
+// ------------------------------------------------------------------ CT (M3:941-1015)
+__global__ void k_ct3d(Geom3D g, Edges ed, State3D B0, State3D B1, const Scalars* __restrict__ sc) {
+    const int ia = blockIdx.x * blockDim.x + threadIdx.x, ja = blockIdx.y, ka = blockIdx.z;
+    const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+    if (ia > Nx) return;
+    const double dt = sc->dt;
+    auto EX = [&](int i, int jf, int kf) { return ed.Ex[((size_t)kf * (Ny + 1) + jf) * Nx + i]; };
+    auto EY = [&](int i, int j, int kf) { return ed.Ey[((size_t)kf * Ny + j) * (Nx + 1) + i]; };
+    auto EZ = [&](int i, int jf, int k) { return ed.Ez[((size_t)k * (Ny + 1) + jf) * (Nx + 1) + i]; };
+    // face indices j+1 <= Ny and k+1 <= Nz need no wrapping (imod(n, n+1) = n, clamp(n, 0, n) = n)
+    if (ja < Ny && ka < Nz) {
+        const int i = ia, j = ja, k = ka;
+        const size_t f = fx_(g, i, j, k);
+        B1.Bx[f] = B0.Bx[f] + (-(dt / g.dy) * (EZ(i, j + 1, k) - EZ(i, j, k)) + (dt / g.dz) * (EY(i, j, k + 1) - EY(i, j, k)));
+    }
+    if (ia < Nx && ka < Nz) {
+        const int i = ia, j = ja, k = ka;
+        const size_t f = fy_(g, i, j, k);
+        B1.By[f] = B0.By[f] + (+(dt / g.dx) * (EZ(i + 1, j, k) - EZ(i, j, k)) - (dt / g.dz) * (EX(i, j, k + 1) - EX(i, j, k)));
+    }
+    if (ia < Nx && ja < Ny) {
+        const int i = ia, j = ja, k = ka;
+        const size_t f = fz_(g, i, j, k);
+        B1.Bz[f] = B0.Bz[f] + (-(dt / g.dx) * (EY(i + 1, j, k) - EY(i, j, k)) + (dt / g.dy) * (EX(i, j + 1, k) - EX(i, j, k)));
+    }
+}
+
+// ------------------------------------------------------------------ Joule + floors (+ Heun)
+template <bool FINAL>
+__global__ __launch_bounds__(256) void k_joule3d(Geom3D g, Eos3D e, State3D Us, State3D B1,
+                                                 const Scalars* __restrict__ sc, State3D Un, State3D Out,
+                                                 double* __restrict__ spd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+    double speed = 0.0;
+    if (i < Nx) {
+        const size_t q = ((size_t)k * Ny + j) * Nx + i;
+        double rho = Us.rho[q], mx = Us.mx[q], my = Us.my[q], mz = Us.mz[q], E = Us.E[q];
+        const double dt = sc->dt;
+        if (g.eta > 0.0) {  // add_joule_heating, M3:652-692
+            const double dBzdy = (bzc(g, B1.Bz, i, j + 1, k) - bzc(g, B1.Bz, i, j - 1, k)) / (2 * g.dy);
+            const double dBydz = (byc(g, B1.By, i, j, k + 1) - byc(g, B1.By, i, j, k - 1)) / (2 * g.dz);
+            const double dBxdz = (bxc(g, B1.Bx, i, j, k + 1) - bxc(g, B1.Bx, i, j, k - 1)) / (2 * g.dz);
+            const double dBzdx = (bzc(g, B1.Bz, i + 1, j, k) - bzc(g, B1.Bz, i - 1, j, k)) / (2 * g.dx);
+            const double dBydx = (byc(g, B1.By, i + 1, j, k) - byc(g, B1.By, i - 1, j, k)) / (2 * g.dx);
+            const double dBxdy = (bxc(g, B1.Bx, i, j + 1, k) - bxc(g, B1.Bx, i, j - 1, k)) / (2 * g.dy);
+            const double Jx = dBzdy - dBydz, Jy = dBxdz - dBzdx, Jz = dBydx - dBxdy;
+            const double J2 = Jx * Jx + Jy * Jy + Jz * Jz;
+            E += dt * g.eta * J2;
+        }
+        const double bxL = B1.Bx[fx_(g, i, j, k)], bxR = B1.Bx[fx_(g, i + 1, j, k)];
+        const double byD = B1.By[fy_(g, i, j, k)], byU = B1.By[fy_(g, i, j + 1, k)];
+        const double bzB = B1.Bz[fz_(g, i, j, k)], bzF = B1.Bz[fz_(g, i, j, k + 1)];
+        floors3d(e, rho, mx, my, mz, E, 0.5 * (bxL + bxR), 0.5 * (byD + byU), 0.5 * (bzB + bzF));  // M3:1019-1029
+        if (!FINAL) {
+            Out.rho[q] = rho; Out.mx[q] = mx; Out.my[q] = my; Out.mz[q] = mz; Out.E[q] = E;
+        } else {  // Heun average and floors, M3:1042-1065
+            rho = 0.5 * (Un.rho[q] + rho);
+            mx = 0.5 * (Un.mx[q] + mx);
+            my = 0.5 * (Un.my[q] + my);
+            mz = 0.5 * (Un.mz[q] + mz);
+            E = 0.5 * (Un.E[q] + E);
+            const double nxL = 0.5 * (Un.Bx[fx_(g, i, j, k)] + bxL), nxR = 0.5 * (Un.Bx[fx_(g, i + 1, j, k)] + bxR);
+            const double nyD = 0.5 * (Un.By[fy_(g, i, j, k)] + byD), nyU = 0.5 * (Un.By[fy_(g, i, j + 1, k)] + byU);
+            const double nzB = 0.5 * (Un.Bz[fz_(g, i, j, k)] + bzB), nzF = 0.5 * (Un.Bz[fz_(g, i, j, k + 1)] + bzF);
+            const double bx = 0.5 * (nxL + nxR), by = 0.5 * (nyD + nyU), bz = 0.5 * (nzB + nzF);
+            floors3d(e, rho, mx, my, mz, E, bx, by, bz);
+            Out.rho[q] = rho; Out.mx[q] = mx; Out.my[q] = my; Out.mz[q] = mz; Out.E[q] = E;
+            Out.Bx[fx_(g, i, j, k)] = nxL;
+            if (i == Nx - 1) Out.Bx[fx_(g, Nx, j, k)] = nxR;
+            Out.By[fy_(g, i, j, k)] = nyD;
+            if (j == Ny - 1) Out.By[fy_(g, i, Ny, k)] = nyU;
+            Out.Bz[fz_(g, i, j, k)] = nzB;
+            if (k == Nz - 1) Out.Bz[fz_(g, i, j, Nz)] = nzF;
+            speed = signal_speed3d(e, cons_to_prim3d(e, rho, mx, my, mz, E, bx, by, bz));
+        }
+    }
+    if (FINAL) block_max_to(speed, spd);
+}
+
+__global__ __launch_bounds__(256) void k_speed_max3d(Geom3D g, Eos3D e, State3D U, double* __restrict__ spd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    double speed = 0.0;
+    if (i < g.Nx) speed = signal_speed3d(e, prim_at(g, e, U, i, j, k));
+    block_max_to(speed, spd);
+}
+
+__global__ __launch_bounds__(256) void k_dt_finalize3d(Geom3D g, Scalars* sc, double* dts, const double* spd, int nspd) {
+    const double maxs = block_reduce_max(spd, nspd, 1e-14);
+    if (threadIdx.x == 0) finalize_dt(sc, maxs, smin(smin(g.dx, g.dy), g.dz), g.cfl, g.eta, dts);
+}
+
+__global__ __launch_bounds__(256) void k_divb_max3d(Geom3D g, State3D U, double* __restrict__ spd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    double v = 0.0;
+    if (i < g.Nx)  // divB_cc, M3:1070-1078 (face i+1 <= Nx etc. need no wrapping)
+        v = fabs((U.Bx[fx_(g, i + 1, j, k)] - U.Bx[fx_(g, i, j, k)]) / g.dx +
+                 (U.By[fy_(g, i, j + 1, k)] - U.By[fy_(g, i, j, k)]) / g.dy +
+                 (U.Bz[fz_(g, i, j, k + 1)] - U.Bz[fz_(g, i, j, k)]) / g.dz);
+    block_max_to(v, spd);
+}
+
+__global__ __launch_bounds__(256) void k_reduce_max_to(const double* spd, int nspd, double* out) {
+    const double m = block_reduce_max(spd, nspd, 0.0);
+    if (threadIdx.x == 0) out[0] = m;
+}
+
+template <bool MIXED, typename EOS64, typename EOS32>
+__global__ void k_hlld3d_batch(EOS64 e64, EOS32 e32, float tau, int n, const double* __restrict__ WLa,
+                               const double* __restrict__ WRa, const double* __restrict__ Bna, double* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const double* a = WLa + 8 * (size_t)t;
+    const double* b = WRa + 8 * (size_t)t;
+    const Prim<double> WL{a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7]};
+    const Prim<double> WR{b[0], b[1], b[2], b[3], b[4], b[5], b[6], b[7]};
+    double F[7];
+    face_hlld3d<MIXED>(e64, e32, WL, WR, Bna[t], F, tau);
+    for (int c = 0; c < 7; c++) out[7 * (size_t)t + c] = F[c];
+}
+
+template <typename R>
+HybridEOS<R> make_hybrid(const cc_mhd3d_params* p) {
+    HybridEOS<R> h;
+    const double K2 = p->K1 * std::pow(p->rho_nuc, p->gamma1 - p->gamma2);
+    h.gamma1 = (R)p->gamma1; h.gamma2 = (R)p->gamma2; h.K1 = (R)p->K1; h.K2 = (R)K2; h.rho_nuc = (R)p->rho_nuc;
+    h.gamma_th = (R)p->gamma_th; h.rho_bounce = (R)p->rho_bounce; h.gamma_bounce = (R)p->gamma_bounce;
+    h.Kbounce = (R)p->Kbounce;
+    h.ec1_n = (R)(p->K1 * std::pow(p->rho_nuc, p->gamma1 - 1.0) / (p->gamma1 - 1.0));
+    h.ec2_n = (R)(K2 * std::pow(p->rho_nuc, p->gamma2 - 1.0) / (p->gamma2 - 1.0));
+    h.rb_g = (R)std::pow(p->rho_bounce, p->gamma_bounce);
+    h.rb_g1 = (R)std::pow(p->rho_bounce, p->gamma_bounce - 1.0);
+    h.floor_rho = (R)p->floor_rho;
+    h.floor_P = (R)p->floor_P;
+    return h;
+}
+
+}  // namespace cc
+
+// ==================================================================== host handle
+using namespace cc;
+
+static constexpr int kDtLog3 = 1 << 16;
+
+struct cc_mhd3d {
+    cc_mhd3d_params p{};
+    Geom3D g{};
+    Eos3D e{};
+    hipStream_t stream = nullptr;
+    State3D S[3]{};
+    int cur = 0;
+    State3D Us{};   // cells after the flux update
+    State3D Bt{};   // stage-2 face field before averaging
+    EmfFaces ef{};
+    Edges ed{};
+    Scalars* sc = nullptr;
+    double* dts_dev = nullptr;
+    double* spd = nullptr;
+    double* red = nullptr;
+    int nspd = 0;
+    Profiler prof;
+    std::vector<void*> allocs;
+    hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
+
+    size_t N() const { return (size_t)p.Nx * p.Ny * p.Nz; }
+    size_t NFx() const { return (size_t)(p.Nx + 1) * p.Ny * p.Nz; }
+    size_t NFy() const { return (size_t)p.Nx * (p.Ny + 1) * p.Nz; }
+    size_t NFz() const { return (size_t)p.Nx * p.Ny * (p.Nz + 1); }
+    Profiler* pr() { return prof.on ? &prof : nullptr; }
+
+    cc_status alloc(double** ptr, size_t n) {
+        if (hipMalloc((void**)ptr, n * sizeof(double)) != hipSuccess) {
+            set_error(CC_ERR_OOM, "cc_mhd3d: device allocation failed");
+            return CC_ERR_OOM;
+        }
+        allocs.push_back(*ptr);
+        CC_HIP(hipMemsetAsync(*ptr, 0, n * sizeof(double), stream));
+        return CC_OK;
+    }
+    cc_status alloc_cells(State3D& s) {
+        CC_TRY(alloc(&s.rho, N())); CC_TRY(alloc(&s.mx, N())); CC_TRY(alloc(&s.my, N()));
+        CC_TRY(alloc(&s.mz, N())); CC_TRY(alloc(&s.E, N()));
+        return CC_OK;
+    }
+    cc_status alloc_faces(State3D& s) {
+        CC_TRY(alloc(&s.Bx, NFx())); CC_TRY(alloc(&s.By, NFy())); CC_TRY(alloc(&s.Bz, NFz()));
+        return CC_OK;
+    }
+
+    template <bool MIXED, typename EOS64, typename EOS32>
+    cc_status launch_flux(const EOS64& e64, const EOS32& e32, const State3D& in) {
+        dim3 grid((p.Nx + TX3 - 1) / TX3, (p.Ny + TY3 - 1) / TY3, (p.Nz + KZ3 - 1) / KZ3);
+        hipLaunchKernelGGL((k_flux3d<MIXED, EOS64, EOS32>), grid, dim3(256), 0, stream, g, e, e64, e32, in, sc, Us, ef);
+        CC_CHECK_LAUNCH();
+        return CC_OK;
+    }
+
+    cc_status flux(const State3D& in) {
+        ProfScope ps(pr(), stream, "flux");
+        const bool mixed = p.precision == CC_MIXED;
+        if (e.ideal) {
+            const IdealEOS3D<double> e64{p.gamma - 1.0, p.gamma, p.floor_rho};
+            const IdealEOS3D<float> e32{(float)(p.gamma - 1.0), (float)p.gamma, (float)p.floor_rho};
+            return mixed ? launch_flux<true>(e64, e32, in) : launch_flux<false>(e64, e32, in);
+        }
+        const HybridEOS<float> h32 = make_hybrid<float>(&p);
+        return mixed ? launch_flux<true>(e.hy, h32, in) : launch_flux<false>(e.hy, h32, in);
+    }
+
+    // One stage in -> out; FINAL averages with hn (Heun) and writes the averaged faces to out.
+    template <bool FINAL>
+    cc_status stage(const State3D& in, const State3D& out, const State3D& hn) {
+        const int Nx = p.Nx, Ny = p.Ny, Nz = p.Nz;
+        CC_TRY(flux(in));
+        {
+            ProfScope ps(pr(), stream, "edge");
+            hipLaunchKernelGGL(k_edge3d, dim3((Nx + 1 + 127) / 128, Ny + 1, Nz + 1), dim3(128), 0, stream, g, ef, in, ed);
+            CC_CHECK_LAUNCH();
+        }
+        const State3D& bdst = FINAL ? Bt : out;
+        {
+            ProfScope ps(pr(), stream, "ct");
+            hipLaunchKernelGGL(k_ct3d, dim3((Nx + 1 + 127) / 128, Ny + 1, Nz + 1), dim3(128), 0, stream, g, ed, in, bdst, sc);
+            CC_CHECK_LAUNCH();
+        }
+        {
+            ProfScope ps(pr(), stream, "joule");
+            hipLaunchKernelGGL(k_joule3d<FINAL>, dim3((Nx + 255) / 256, Ny, Nz), dim3(256), 0, stream, g, e, Us, bdst, sc, hn,
+                               out, spd);
+            CC_CHECK_LAUNCH();
+        }
+        return CC_OK;
+    }
+
+    cc_status heun(int a, int b, int c, bool driver, double* dts) {
+        if (driver) {
+            ProfScope ps(pr(), stream, "dt");
+            hipLaunchKernelGGL(k_dt_finalize3d, dim3(1), dim3(256), 0, stream, g, sc, dts, spd, nspd);
+            CC_CHECK_LAUNCH();
+        }
+        CC_TRY(stage<false>(S[a], S[b], S[a]));
+        CC_TRY(stage<true>(S[b], S[c], S[a]));
+        return CC_OK;
+    }
+
+    cc_status speed_max(const State3D& s) {
+        hipLaunchKernelGGL(k_speed_max3d, dim3((p.Nx + 255) / 256, p.Ny, p.Nz), dim3(256), 0, stream, g, e, s, spd);
+        CC_CHECK_LAUNCH();
+        return CC_OK;
+    }
+
+    ~cc_mhd3d() {
+        for (auto& ge : graph)
+            if (ge) hipGraphExecDestroy(ge);
+        for (void* a : allocs) hipFree(a);
+        if (sc) hipFree(sc);
+        if (stream) hipStreamDestroy(stream);
+    }
+};
+
+extern "C" {
+
+cc_status cc_mhd3d_create(const cc_mhd3d_params* p, cc_mhd3d** out) {
+    if (!out || !p) { set_error(CC_ERR_INVALID_ARG, "null argument"); return CC_ERR_INVALID_ARG; }
+    *out = nullptr;
+    if (p->Nx < 2 || p->Ny < 2 || p->Nz < 2 || p->Nx > 4096 || p->Ny > 4096 || p->Nz > 4096 ||
+        (size_t)p->Nx * p->Ny * p->Nz > (size_t)1 << 31) {
+        set_error(CC_ERR_INVALID_ARG, "grid must be within [2, 4096]^3 and at most 2^31 cells");
+        return CC_ERR_INVALID_ARG;
+    }
+    if (p->precision != CC_FP64 && p->precision != CC_MIXED) { set_error(CC_ERR_INVALID_ARG, "bad precision"); return CC_ERR_INVALID_ARG; }
+    if (p->eos != 0 && p->eos != 1) { set_error(CC_ERR_INVALID_ARG, "eos must be 0 (hybrid) or 1 (ideal)"); return CC_ERR_INVALID_ARG; }
+    if (p->eos == 1 && !(p->gamma > 1.0)) { set_error(CC_ERR_INVALID_ARG, "ideal EOS needs gamma > 1"); return CC_ERR_INVALID_ARG; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { set_error(CC_ERR_NO_DEVICE, "no HIP device"); return CC_ERR_NO_DEVICE; }
+    cc_mhd3d* h = new cc_mhd3d();
+    h->p = *p;
+    Geom3D& g = h->g;
+    g.Nx = p->Nx; g.Ny = p->Ny; g.Nz = p->Nz; g.px = p->periodic_x != 0; g.py = p->periodic_y != 0; g.pz = p->periodic_z != 0;
+    g.dx = (p->x1 - p->x0) / p->Nx; g.dy = (p->y1 - p->y0) / p->Ny; g.dz = (p->z1 - p->z0) / p->Nz;
+    g.floor_rho = p->floor_rho; g.floor_P = p->floor_P; g.eta = p->eta; g.cfl = p->cfl;
+    g.tau = p->degen_tau > 0.0 ? (float)p->degen_tau : kDegenerateTau;
+    h->e.ideal = p->eos == 1;
+    h->e.gamma = p->gamma;
+    h->e.hy = make_hybrid<double>(p);
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_error(CC_ERR_HIP, "hipStreamCreate failed");
+        delete h;
+        return CC_ERR_HIP;
+    }
+    cc_status st = CC_OK;
+    for (int k = 0; k < 3 && st == CC_OK; k++) {
+        st = h->alloc_cells(h->S[k]);
+        if (st == CC_OK) st = h->alloc_faces(h->S[k]);
+    }
+    if (st == CC_OK) st = h->alloc_cells(h->Us);
+    if (st == CC_OK) st = h->alloc_faces(h->Bt);
+    double** ep[6] = {&h->ef.Ey_x, &h->ef.Ez_x, &h->ef.Ez_y, &h->ef.Ex_y, &h->ef.Ex_z, &h->ef.Ey_z};
+    size_t en[6] = {h->NFx(), h->NFx(), h->NFy(), h->NFy(), h->NFz(), h->NFz()};
+    for (int k = 0; k < 6 && st == CC_OK; k++) st = h->alloc(ep[k], en[k]);
+    if (st == CC_OK) st = h->alloc(&h->ed.Ex, (size_t)p->Nx * (p->Ny + 1) * (p->Nz + 1));
+    if (st == CC_OK) st = h->alloc(&h->ed.Ey, (size_t)(p->Nx + 1) * p->Ny * (p->Nz + 1));
+    if (st == CC_OK) st = h->alloc(&h->ed.Ez, (size_t)(p->Nx + 1) * (p->Ny + 1) * p->Nz);
+    if (st == CC_OK) st = h->alloc(&h->dts_dev, kDtLog3);
+    h->nspd = ((p->Nx + 255) / 256) * p->Ny * p->Nz;
+    if (st == CC_OK) st = h->alloc(&h->spd, (size_t)h->nspd);
+    if (st == CC_OK) st = h->alloc(&h->red, 4);
+    if (st == CC_OK && hipMalloc((void**)&h->sc, sizeof(Scalars)) != hipSuccess) {
+        set_error(CC_ERR_OOM, "scalar allocation failed");
+        st = CC_ERR_OOM;
+    }
+    if (st == CC_OK) {
+        hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, h->stream, h->sc, 0.0, 0.0, 1e300, 1);
+        if (hipStreamSynchronize(h->stream) != hipSuccess) { set_error(CC_ERR_HIP, "sync failed"); st = CC_ERR_HIP; }
+    }
+    if (st != CC_OK) { delete h; return st; }
+    *out = h;
+    return CC_OK;
+}
+
+cc_status cc_mhd3d_destroy(cc_mhd3d* h) {
+    if (!h) return CC_OK;
+    hipStreamSynchronize(h->stream);
+    delete h;
+    return CC_OK;
+}
+
+#define CHECK_H3()                                                  \
+    do {                                                            \
+        if (!h) { set_error(CC_ERR_INVALID_ARG, "null handle"); return CC_ERR_INVALID_ARG; } \
+    } while (0)
+
+cc_status cc_mhd3d_upload(cc_mhd3d* h, const double* rho, const double* mx, const double* my, const double* mz,
+                          const double* E, const double* Bx_f, const double* By_f, const double* Bz_f) {
+    CHECK_H3();
+    if (!rho || !mx || !my || !mz || !E || !Bx_f || !By_f || !Bz_f) { set_error(CC_ERR_INVALID_ARG, "null state pointer"); return CC_ERR_INVALID_ARG; }
+    State3D& s = h->S[h->cur];
+    const size_t n = h->N() * 8;
+    CC_HIP(hipMemcpyAsync(s.rho, rho, n, hipMemcpyHostToDevice, h->stream));
+    CC_HIP(hipMemcpyAsync(s.mx, mx, n, hipMemcpyHostToDevice, h->stream));
+    CC_HIP(hipMemcpyAsync(s.my, my, n, hipMemcpyHostToDevice, h->stream));
+    CC_HIP(hipMemcpyAsync(s.mz, mz, n, hipMemcpyHostToDevice, h->stream));
+    CC_HIP(hipMemcpyAsync(s.E, E, n, hipMemcpyHostToDevice, h->stream));
+    CC_HIP(hipMemcpyAsync(s.Bx, Bx_f, h->NFx() * 8, hipMemcpyHostToDevice, h->stream));
+    CC_HIP(hipMemcpyAsync(s.By, By_f, h->NFy() * 8, hipMemcpyHostToDevice, h->stream));
+    CC_HIP(hipMemcpyAsync(s.Bz, Bz_f, h->NFz() * 8, hipMemcpyHostToDevice, h->stream));
+    CC_HIP(hipStreamSynchronize(h->stream));
+    return CC_OK;
+}
+
+cc_status cc_mhd3d_download(cc_mhd3d* h, double* rho, double* mx, double* my, double* mz, double* E, double* Bx_f,
+                            double* By_f, double* Bz_f) {
+    CHECK_H3();
+    State3D& s = h->S[h->cur];
+    const size_t n = h->N() * 8;
+    struct { double* d; const double* s; size_t b; } cp[8] = {
+        {rho, s.rho, n}, {mx, s.mx, n}, {my, s.my, n}, {mz, s.mz, n}, {E, s.E, n},
+        {Bx_f, s.Bx, h->NFx() * 8}, {By_f, s.By, h->NFy() * 8}, {Bz_f, s.Bz, h->NFz() * 8}};
+    for (auto& c : cp)
+        if (c.d) CC_HIP(hipMemcpyAsync(c.d, c.s, c.b, hipMemcpyDeviceToHost, h->stream));
+    CC_HIP(hipStreamSynchronize(h->stream));
+    return CC_OK;
+}
+
+cc_status cc_mhd3d_compute_dt(cc_mhd3d* h, double* dt) {
+    CHECK_H3();
+    if (!dt) { set_error(CC_ERR_INVALID_ARG, "null dt"); return CC_ERR_INVALID_ARG; }
+    hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, h->stream, h->sc, 0.0, 0.0, 1e300, 1);
+    CC_TRY(h->speed_max(h->S[h->cur]));
+    hipLaunchKernelGGL(k_dt_finalize3d, dim3(1), dim3(256), 0, h->stream, h->g, h->sc, (double*)nullptr, h->spd, h->nspd);
+    CC_CHECK_LAUNCH();
+    Scalars s;
+    CC_HIP(hipMemcpyAsync(&s, h->sc, sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));
+    CC_HIP(hipStreamSynchronize(h->stream));
+    *dt = s.dt;
+    return CC_OK;
+}
+
+cc_status cc_mhd3d_stage(cc_mhd3d* h, double dt) {
+    CHECK_H3();
+    hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, h->stream, h->sc, dt, 0.0, 1e300, 0);
+    const int a = h->cur, b = (a + 1) % 3;
+    CC_TRY(h->stage<false>(h->S[a], h->S[b], h->S[a]));
+    h->cur = b;
+    return CC_OK;
+}
+
+cc_status cc_mhd3d_rk2_step(cc_mhd3d* h, double dt) {
+    CHECK_H3();
+    if (!std::isfinite(dt) || dt < 0.0) { set_error(CC_ERR_BAD_DT, "dt must be finite and >= 0"); return CC_ERR_BAD_DT; }
+    hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, h->stream, h->sc, dt, 0.0, 1e300, 0);
+    const int a = h->cur, b = (a + 1) % 3, c = (a + 2) % 3;
+    CC_TRY(h->heun(a, b, c, false, nullptr));
+    h->cur = c;
+    return CC_OK;
+}
+
+cc_status cc_mhd3d_run(cc_mhd3d* h, int nsteps, double* t_inout, double t_end, double* dts) {
+    CHECK_H3();
+    if (nsteps < 0 || !t_inout) { set_error(CC_ERR_INVALID_ARG, "nsteps < 0 or null t"); return CC_ERR_INVALID_ARG; }
+    if (nsteps == 0) return CC_OK;
+    if (nsteps > kDtLog3) {
+        double t = *t_inout;
+        for (int done = 0; done < nsteps;) {
+            const int n = std::min(kDtLog3, nsteps - done);
+            CC_TRY(cc_mhd3d_run(h, n, &t, t_end, dts ? dts + done : nullptr));
+            done += n;
+        }
+        *t_inout = t;
+        return CC_OK;
+    }
+    hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, h->stream, h->sc, 0.0, *t_inout, t_end, 1);
+    CC_TRY(h->speed_max(h->S[h->cur]));
+    const bool use_graph = !h->prof.on;
+    for (int s = 0; s < nsteps; ++s) {
+        const int a = h->cur, b = (a + 1) % 3, c = (a + 2) % 3;
+        if (use_graph) {
+            if (!h->graph[a]) {
+                hipGraph_t gr;
+                CC_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+                cc_status st = h->heun(a, b, c, true, h->dts_dev);
+                hipError_t e = hipStreamEndCapture(h->stream, &gr);
+                if (st != CC_OK) return st;
+                if (e != hipSuccess) { set_error(CC_ERR_HIP, "graph capture failed"); return CC_ERR_HIP; }
+                CC_HIP(hipGraphInstantiate(&h->graph[a], gr, nullptr, nullptr, 0));
+                hipGraphDestroy(gr);
+            }
+            CC_HIP(hipGraphLaunch(h->graph[a], h->stream));
+        } else {
+            CC_TRY(h->heun(a, b, c, true, h->dts_dev));
+        }
+        h->cur = c;
+    }
+    Scalars s;
+    CC_HIP(hipMemcpyAsync(&s, h->sc, sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));
+    if (dts) CC_HIP(hipMemcpyAsync(dts, h->dts_dev, (size_t)nsteps * 8, hipMemcpyDeviceToHost, h->stream));
+    CC_HIP(hipStreamSynchronize(h->stream));
+    *t_inout = s.t;
+    if (!std::isfinite(s.dt) || s.dt < 0.0) { set_error(CC_ERR_BAD_DT, "non-finite or negative dt on device"); return CC_ERR_BAD_DT; }
+    return CC_OK;
+}
+
+cc_status cc_mhd3d_divb_max(cc_mhd3d* h, double* out) {
+    CHECK_H3();
+    if (!out) { set_error(CC_ERR_INVALID_ARG, "null output"); return CC_ERR_INVALID_ARG; }
+    hipLaunchKernelGGL(k_divb_max3d, dim3((h->p.Nx + 255) / 256, h->p.Ny, h->p.Nz), dim3(256), 0, h->stream, h->g,
+                       h->S[h->cur], h->spd);
+    hipLaunchKernelGGL(k_reduce_max_to, dim3(1), dim3(256), 0, h->stream, h->spd, h->nspd, h->red);
+    CC_CHECK_LAUNCH();
+    CC_HIP(hipMemcpyAsync(out, h->red, 8, hipMemcpyDeviceToHost, h->stream));
+    CC_HIP(hipStreamSynchronize(h->stream));
+    return CC_OK;
+}
+
+cc_status cc_mhd3d_sync(cc_mhd3d* h) {
+    CHECK_H3();
+    CC_HIP(hipStreamSynchronize(h->stream));
+    return CC_OK;
+}
+
+cc_status cc_mhd3d_set_profiling(cc_mhd3d* h, int on) {
+    CHECK_H3();
+    h->prof.on = on != 0;
+    return CC_OK;
+}
+
+cc_status cc_mhd3d_kernel_time(cc_mhd3d* h, const char* kernel, int* count, double* total_ms) {
+    CHECK_H3();
+    if (!kernel || !count || !total_ms) { set_error(CC_ERR_INVALID_ARG, "null argument"); return CC_ERR_INVALID_ARG; }
+    CC_HIP(hipStreamSynchronize(h->stream));
+    h->prof.get_total(kernel, count, total_ms);
+    return CC_OK;
+}
+
+cc_status cc_mhd3d_reset_profiling(cc_mhd3d* h) {
+    CHECK_H3();
+    CC_HIP(hipStreamSynchronize(h->stream));
+    h->prof.reset();
+    return CC_OK;
+}
+
+cc_status cc_hlld3d_batch(const cc_mhd3d_params* p, int n, const double* WL, const double* WR, const double* Bn,
+                          double* out) {
+    if (!p || n < 0 || (n > 0 && (!WL || !WR || !Bn || !out))) { set_error(CC_ERR_INVALID_ARG, "invalid batch arguments"); return CC_ERR_INVALID_ARG; }
+    if (n == 0) return CC_OK;
+    double *dL, *dR, *dB, *dO;
+    const size_t b8 = (size_t)n * 8 * 8;
+    if (hipMalloc(&dL, b8) != hipSuccess) { set_error(CC_ERR_OOM, "batch alloc"); return CC_ERR_OOM; }
+    hipMalloc(&dR, b8);
+    hipMalloc(&dB, (size_t)n * 8);
+    hipMalloc(&dO, (size_t)n * 7 * 8);
+    hipMemcpy(dL, WL, b8, hipMemcpyHostToDevice);
+    hipMemcpy(dR, WR, b8, hipMemcpyHostToDevice);
+    hipMemcpy(dB, Bn, (size_t)n * 8, hipMemcpyHostToDevice);
+    const dim3 grid((n + 255) / 256);
+    const bool mixed = p->precision == CC_MIXED;
+    const float tau = p->degen_tau > 0.0 ? (float)p->degen_tau : kDegenerateTau;
+    if (p->eos == 1) {
+        const IdealEOS3D<double> e64{p->gamma - 1.0, p->gamma, p->floor_rho};
+        const IdealEOS3D<float> e32{(float)(p->gamma - 1.0), (float)p->gamma, (float)p->floor_rho};
+        if (mixed)
+            hipLaunchKernelGGL((k_hlld3d_batch<true, IdealEOS3D<double>, IdealEOS3D<float>>), grid, dim3(256), 0, 0, e64, e32, tau, n, dL, dR, dB, dO);
+        else
+            hipLaunchKernelGGL((k_hlld3d_batch<false, IdealEOS3D<double>, IdealEOS3D<float>>), grid, dim3(256), 0, 0, e64, e32, tau, n, dL, dR, dB, dO);
+    } else {
+        const HybridEOS<double> e64 = make_hybrid<double>(p);
+        const HybridEOS<float> e32 = make_hybrid<float>(p);
+        if (mixed)
+            hipLaunchKernelGGL((k_hlld3d_batch<true, HybridEOS<double>, HybridEOS<float>>), grid, dim3(256), 0, 0, e64, e32, tau, n, dL, dR, dB, dO);
+        else
+            hipLaunchKernelGGL((k_hlld3d_batch<false, HybridEOS<double>, HybridEOS<float>>), grid, dim3(256), 0, 0, e64, e32, tau, n, dL, dR, dB, dO);
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(out, dO, (size_t)n * 7 * 8, hipMemcpyDeviceToHost);
+    hipFree(dL); hipFree(dR); hipFree(dB); hipFree(dO);
+    if (e != hipSuccess) { set_error(CC_ERR_HIP, hipGetErrorString(e)); return CC_ERR_HIP; }
+    return CC_OK;
+}
+
+}  // extern "C"

