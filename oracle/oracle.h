@@ -52,6 +52,7 @@ void or_m2_rk2_step(const or_m2_params* p, or_m2_state* U, const double* gx, con
 void or_m2_divb(const or_m2_params* p, const double* Bx_f, const double* By_f, double dx, double dy, double* out);
 double or_m2_run(const or_m2_params* p, or_m2_state* U, double dx, double dy, int nsteps, double t0, double t_end,
                  double* dts_out);
+int or_m2_last_branch(void); /* restatement only: branch of the last hlld call (see oracle_mhd2d.c) */
 
 /* ---- 3D resistive MHD (mhd3d_resistive_hlld_ct.cpp, Params M3:43-89) ---- */
 typedef struct {

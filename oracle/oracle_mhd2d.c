@@ -298,6 +298,11 @@ static void state7(const or_m2_params* p, const Prim* W, double out[7]) {
     out[4] = E; out[5] = W->By; out[6] = W->Bz;
 }
 
+/* Branch taken by the last hlld_x call (test bookkeeping for stratified golden vectors):
+ * 0 F_L (S_L >= 0), 1 F_R (S_R <= 0), 2 HLL (|Bn| < 1e-12), 3 U*_L, 4 U**_L, 5 U**_R, 6 U*_R. */
+static int g_last_branch = -1;
+int or_m2_last_branch(void) { return g_last_branch; }
+
 static FluxX hlld_x(const or_m2_params* p, Prim WL, Prim WR, double Bn) {
     WL.Bx = Bn; WR.Bx = Bn;
     double rhoL = WL.rho, rhoR = WR.rho, uL = WL.u, uR = WR.u, vL = WL.v, vR = WR.v, wL = WL.w, wR = WR.w;
@@ -309,8 +314,8 @@ static FluxX hlld_x(const or_m2_params* p, Prim WL, Prim WR, double Bn) {
     double cfR = fast_speed_n(p, &WR, WR.Bx);
     double SL = dmin(uL - cfL, uR - cfR);
     double SR = dmax(uL + cfL, uR + cfR);
-    if (0.0 <= SL) return FL;
-    if (0.0 >= SR) return FR;
+    if (0.0 <= SL) { g_last_branch = 0; return FL; }
+    if (0.0 >= SR) { g_last_branch = 1; return FR; }
 
     /* M2:393-401: cons7 uses rho*W.u etc. identical to state7 */
     double UL[7], UR[7];
@@ -364,6 +369,7 @@ static FluxX hlld_x(const or_m2_params* p, Prim WL, Prim WR, double Bn) {
     double SRR = SM + fabs(Bn) / sqrt_rhoR;
 
     if (fabs(Bn) < 1e-12) {
+        g_last_branch = 2;
         double inv = 1.0 / (SR - SL);
         double FUL[7] = {FL.frho, FL.fmx, FL.fmy, FL.fmz, FL.fE, FL.fBy, FL.fBz};
         double FUR[7] = {FR.frho, FR.fmx, FR.fmy, FR.fmz, FR.fE, FR.fBy, FR.fBz};
@@ -385,10 +391,10 @@ static FluxX hlld_x(const or_m2_params* p, Prim WL, Prim WR, double Bn) {
     const Prim* Wref;
     double S, rho_s, v_s, w_s, By_s, Bz_s;
     const FluxX* Fref;
-    if (0.0 < SLL)      { Wref = &WL; S = SL; rho_s = rhoL_s; v_s = vL_s; w_s = wL_s; By_s = ByL_s; Bz_s = BzL_s; Fref = &FL; }
-    else if (0.0 < SM)  { Wref = &WL; S = SL; rho_s = rhoL_s; v_s = v_ss; w_s = w_ss; By_s = By_ss; Bz_s = Bz_ss; Fref = &FL; }
-    else if (0.0 < SRR) { Wref = &WR; S = SR; rho_s = rhoR_s; v_s = v_ss; w_s = w_ss; By_s = By_ss; Bz_s = Bz_ss; Fref = &FR; }
-    else                { Wref = &WR; S = SR; rho_s = rhoR_s; v_s = vR_s; w_s = wR_s; By_s = ByR_s; Bz_s = BzR_s; Fref = &FR; }
+    if (0.0 < SLL)      { g_last_branch = 3; Wref = &WL; S = SL; rho_s = rhoL_s; v_s = vL_s; w_s = wL_s; By_s = ByL_s; Bz_s = BzL_s; Fref = &FL; }
+    else if (0.0 < SM)  { g_last_branch = 4; Wref = &WL; S = SL; rho_s = rhoL_s; v_s = v_ss; w_s = w_ss; By_s = By_ss; Bz_s = Bz_ss; Fref = &FL; }
+    else if (0.0 < SRR) { g_last_branch = 5; Wref = &WR; S = SR; rho_s = rhoR_s; v_s = v_ss; w_s = w_ss; By_s = By_ss; Bz_s = Bz_ss; Fref = &FR; }
+    else                { g_last_branch = 6; Wref = &WR; S = SR; rho_s = rhoR_s; v_s = vR_s; w_s = wR_s; By_s = ByR_s; Bz_s = BzR_s; Fref = &FR; }
 
     double Ust[7];
     {
