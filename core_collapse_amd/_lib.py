@@ -43,6 +43,22 @@ class Mhd3dParams(C.Structure):
                 ("precision", C.c_int), ("degen_tau", C.c_double)]
 
 
+class Euler2dParams(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("cfl", C.c_double), ("x0", C.c_double), ("x1", C.c_double),
+                ("y0", C.c_double), ("y1", C.c_double), ("Nx", C.c_int), ("Ny", C.c_int),
+                ("floor_rho", C.c_double), ("floor_P", C.c_double), ("periodic_x", C.c_int),
+                ("periodic_y", C.c_int), ("gravity_on", C.c_int), ("G", C.c_double), ("max_iter", C.c_int),
+                ("tol", C.c_double), ("omega", C.c_double), ("precision", C.c_int)]
+
+
+class Ns2dParams(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("cfl", C.c_double), ("x0", C.c_double), ("x1", C.c_double),
+                ("y0", C.c_double), ("y1", C.c_double), ("Nx", C.c_int), ("Ny", C.c_int),
+                ("floor_rho", C.c_double), ("floor_P", C.c_double), ("viscosity_on", C.c_int),
+                ("mu", C.c_double), ("cfl_visc", C.c_double), ("gravity_on", C.c_int), ("G", C.c_double),
+                ("precision", C.c_int)]
+
+
 # name -> (restype, argtypes); every function returns cc_status unless stated
 _PROTOS = {
     "cc_last_error": (C.c_char_p, []),
@@ -79,6 +95,36 @@ _PROTOS = {
     "cc_mhd3d_kernel_time": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int), DP]),
     "cc_mhd3d_reset_profiling": (C.c_int, [C.c_void_p]),
     "cc_hlld3d_batch": (C.c_int, [C.POINTER(Mhd3dParams), C.c_int, DP, DP, DP, DP]),
+    "cc_euler2d_create": (C.c_int, [C.POINTER(Euler2dParams), C.POINTER(C.c_void_p)]),
+    "cc_euler2d_destroy": (C.c_int, [C.c_void_p]),
+    "cc_euler2d_upload": (C.c_int, [C.c_void_p] + [DP] * 4),
+    "cc_euler2d_download": (C.c_int, [C.c_void_p] + [DP] * 4),
+    "cc_euler2d_gravity": (C.c_int, [C.c_void_p, DP, DP, DP, C.POINTER(C.c_int)]),
+    "cc_euler2d_compute_dt": (C.c_int, [C.c_void_p, DP]),
+    "cc_euler2d_rhs": (C.c_int, [C.c_void_p] + [DP] * 4),
+    "cc_euler2d_rk2_step": (C.c_int, [C.c_void_p, C.c_double]),
+    "cc_euler2d_run": (C.c_int, [C.c_void_p, C.c_int, DP, C.c_double, DP]),
+    "cc_euler2d_sweep_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_longlong),
+                                         C.POINTER(C.c_int)]),
+    "cc_euler2d_sync": (C.c_int, [C.c_void_p]),
+    "cc_euler2d_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "cc_euler2d_kernel_time": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int), DP]),
+    "cc_euler2d_reset_profiling": (C.c_int, [C.c_void_p]),
+    "cc_hllc2d_batch": (C.c_int, [C.c_int, C.c_int, C.c_int, DP, DP, C.c_double, C.c_double, C.c_int, DP]),
+    "cc_ns2d_create": (C.c_int, [C.POINTER(Ns2dParams), C.POINTER(C.c_void_p)]),
+    "cc_ns2d_destroy": (C.c_int, [C.c_void_p]),
+    "cc_ns2d_upload": (C.c_int, [C.c_void_p, DP]),
+    "cc_ns2d_download": (C.c_int, [C.c_void_p, DP]),
+    "cc_ns2d_poisson": (C.c_int, [C.c_void_p, DP, DP]),
+    "cc_ns2d_rhs": (C.c_int, [C.c_void_p, DP]),
+    "cc_ns2d_compute_dt": (C.c_int, [C.c_void_p, DP]),
+    "cc_ns2d_heun_step": (C.c_int, [C.c_void_p, C.c_double]),
+    "cc_ns2d_run": (C.c_int, [C.c_void_p, C.c_int, DP, C.c_double, DP]),
+    "cc_ns2d_sync": (C.c_int, [C.c_void_p]),
+    "cc_ns2d_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "cc_ns2d_kernel_time": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int), DP]),
+    "cc_ns2d_reset_profiling": (C.c_int, [C.c_void_p]),
+    "cc_fft2d": (C.c_int, [C.c_int, C.c_int, C.c_int, DP]),
     "cc_copy_calibration": (C.c_int, [C.c_size_t, C.c_int, DP]),
     "cc_poisson2d_sor": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                    C.c_int, C.c_double, C.c_int, DP, DP, DP, DP]),

@@ -256,3 +256,125 @@ def mhd3d_reference_state(p=None, **kw):
     rho, mx, my, mz, E = mhd3d_prim_to_cons(p, *d["prim"])
     return {"rho": rho, "mx": mx, "my": my, "mz": mz, "E": E, "Bx_f": d["Bx_f"], "By_f": d["By_f"],
             "Bz_f": d["Bz_f"]}
+
+
+# ------------------------------------------------------------------ libstdc++ distributions
+class StdRandom:
+    """std::mt19937(seed) with libstdc++'s (GCC 11) distributions, as the EH and NS ICs draw them.
+
+    canonical(): generate_canonical<double, 53> -- two 32-bit draws, (r1 + r2 * 2^32) / 2^64 in
+    double arithmetic, clamped below 1.  uniform(a, b): canonical() * (b - a) + a.  normal():
+    Marsaglia's polar method, x, y = 2 canonical() - 1 until 0 < x^2 + y^2 <= 1, mult =
+    sqrt(-2 log(r2) / r2), returns y * mult and keeps x * mult for the next call.
+    """
+
+    def __init__(self, seed, chunk=1 << 16):
+        self._bg = np.random.MT19937()
+        self._bg._legacy_seeding(int(seed))
+        self._buf = []
+        self._chunk = chunk
+        self._saved = None
+
+    def _raw(self):
+        if not self._buf:
+            self._buf = [float(v) for v in self._bg.random_raw(self._chunk)][::-1]
+        return self._buf.pop()
+
+    def canonical(self):
+        s = self._raw() + self._raw() * 4294967296.0
+        r = s / 18446744073709551616.0
+        return r if r < 1.0 else np.nextafter(1.0, 0.0)
+
+    def uniform(self, a, b):
+        return self.canonical() * (b - a) + a
+
+    def normal(self):
+        import math
+        if self._saved is not None:
+            v, self._saved = self._saved, None
+            return v
+        while True:
+            x = 2.0 * self.canonical() - 1.0
+            y = 2.0 * self.canonical() - 1.0
+            r2 = x * x + y * y
+            if not (r2 > 1.0 or r2 == 0.0):
+                break
+        mult = math.sqrt(-2 * math.log(r2) / r2)
+        self._saved = x * mult
+        return y * mult
+
+
+def euler2d_reference_ic(p=None, rho0=1.0, P0=1.0, amp_rho=0.10, amp_P=0.10, vmax=0.05, sigma=0.08, bump_amp=0.5,
+                         seed=42):
+    """init_random_blob of euler2d_gravity_hllc.cpp (EH:574-642): uniform background with 10 %
+    random density and pressure fluctuations, a central Gaussian bump, random velocities in
+    [-vmax, vmax], floors.  Returns SoA float64 arrays of shape (Ny, Nx)."""
+    import math
+    from .euler2d import Params as EHParams
+    p = p or EHParams()
+    Nx, Ny, dx, dy = p.Nx, p.Ny, p.dx, p.dy
+    rng = StdRandom(seed)
+    xc, yc = 0.5 * (p.x0 + p.x1), 0.5 * (p.y0 + p.y1)
+    out = {f: np.empty((Ny, Nx)) for f in ("rho", "mx", "my", "E")}
+    g1 = p.gamma - 1.0
+    for j in range(Ny):
+        y = p.y0 + (j + 0.5) * dy
+        for i in range(Nx):
+            x = p.x0 + (i + 0.5) * dx
+            fr = 2.0 * rng.uniform(0.0, 1.0) - 1.0
+            fp = 2.0 * rng.uniform(0.0, 1.0) - 1.0
+            rho = rho0 * (1.0 + amp_rho * fr)
+            P = P0 * (1.0 + amp_P * fp)
+            rx, ry = x - xc, y - yc
+            r2 = rx * rx + ry * ry
+            rho += bump_amp * rho0 * math.exp(-r2 / (2.0 * sigma * sigma))
+            u = vmax * rng.uniform(-1.0, 1.0)
+            v = vmax * rng.uniform(-1.0, 1.0)
+            rho = p.floor_rho if rho < p.floor_rho else rho  # std::max (EH:628-629)
+            P = p.floor_P if P < p.floor_P else P
+            mx, my = rho * u, rho * v
+            E = P / g1 + 0.5 * rho * (u * u + v * v)
+            # enforce_floors (EH:62-72)
+            rho = rho if not (rho < p.floor_rho) else p.floor_rho
+            uu, vv = mx / rho, my / rho
+            kin = 0.5 * rho * (uu * uu + vv * vv)
+            if g1 * (E - kin) < p.floor_P:
+                E = kin + p.floor_P / g1
+            out["rho"][j, i], out["mx"][j, i], out["my"][j, i], out["E"][j, i] = rho, mx, my, E
+    return out
+
+
+def ns2d_reference_ic(p=None, seed=42, rho0=1.0, P0=0.05, A=1.0, sigma_frac=0.10, Arho=0.01, Mach0=0.05):
+    """The collapse IC in main() of navier_stokes_2d_hllc_poisson.cpp (NS:638-688): Gaussian
+    overdensity rho0 (1 + A exp(-r^2 / 2 sigma^2) + Arho N(0,1)), uniform P0, velocities
+    Mach0 a0 N(0,1), drawn per cell in the order noise, u, v.  Returns the AoS state (Ny, Nx, 4)."""
+    import math
+    from .ns2d import Params as NSParams
+    p = p or NSParams()
+    Nx, Ny = p.Nx, p.Ny
+    Lx, Ly = p.x1 - p.x0, p.y1 - p.y0
+    dx, dy = Lx / Nx, Ly / Ny
+    rng = StdRandom(seed)
+    sigma = sigma_frac * min(Lx, Ly)
+    a0 = math.sqrt(p.gamma * P0 / rho0)
+    vamp = Mach0 * a0
+    xc, yc = 0.5 * (p.x0 + p.x1), 0.5 * (p.y0 + p.y1)
+    U = np.empty((Ny, Nx, 4))
+    clamp = lambda v, lo: lo if v < lo else v  # noqa: E731  clamp_min, NS:51
+    for j in range(Ny):
+        y = p.y0 + (j + 0.5) * dy
+        for i in range(Nx):
+            x = p.x0 + (i + 0.5) * dx
+            rx, ry = x - xc, y - yc
+            r2 = rx * rx + ry * ry
+            bump = A * math.exp(-0.5 * r2 / (sigma * sigma))
+            noise = Arho * rng.normal()
+            d = clamp(rho0 * (1.0 + bump + noise), p.floor_rho)
+            P = clamp(P0, p.floor_P)
+            u = vamp * rng.normal()
+            v = vamp * rng.normal()
+            # prim_to_cons NS:134-145
+            d = clamp(d, p.floor_rho)
+            P = clamp(P, p.floor_P)
+            U[j, i] = (d, d * u, d * v, P / (p.gamma - 1.0) + 0.5 * d * (u * u + v * v))
+    return U

@@ -179,6 +179,100 @@ cc_status cc_hlld3d_batch(const cc_mhd3d_params* p, int n, const double* WL, con
                           double* out);
 
 /* ============================================================================================
+ * 2D Euler HLLC + SOR self-gravity (EH).
+ * ========================================================================================== */
+
+/* The hot-path subset of struct Params (EH:29-56) plus the solver constants of EH:329-331
+ * (max_iter 50000, tol 1e-10, omega 1.7 in the reference). */
+typedef struct cc_euler2d_params {
+    double gamma, cfl, x0, x1, y0, y1;
+    int Nx, Ny;
+    double floor_rho, floor_P;
+    int periodic_x, periodic_y; /* hydro boundaries; the Poisson solve is always periodic (EH:300-304) */
+    int gravity_on;
+    double G;
+    int max_iter;
+    double tol, omega;
+    int precision; /* cc_precision: CC_MIXED runs the HLLC in FP32 */
+} cc_euler2d_params;
+
+typedef struct cc_euler2d cc_euler2d;
+
+cc_status cc_euler2d_create(const cc_euler2d_params* p, cc_euler2d** out);
+cc_status cc_euler2d_destroy(cc_euler2d* h);
+/* State in the reference layout (struct State, EH:58-60): SoA, q = j*Nx+i. */
+cc_status cc_euler2d_upload(cc_euler2d* h, const double* rho, const double* mx, const double* my, const double* E);
+cc_status cc_euler2d_download(cc_euler2d* h, double* rho, double* mx, double* my, double* E);
+/* g (and phi, nullable) of the current state (compute_poisson_gravity_periodic, EH:288-386), and the
+ * number of SOR sweeps it took.  Syncs. */
+cc_status cc_euler2d_gravity(cc_euler2d* h, double* gx, double* gy, double* phi, int* sweeps);
+/* compute_dt, EH:390-404.  Syncs. */
+cc_status cc_euler2d_compute_dt(cc_euler2d* h, double* dt);
+/* rhs(U) including its Poisson solve (EH:406-527) into caller buffers.  Syncs. */
+cc_status cc_euler2d_rhs(cc_euler2d* h, double* drho, double* dmx, double* dmy, double* dE);
+/* rk2_step, EH:529-550, in place. */
+cc_status cc_euler2d_rk2_step(cc_euler2d* h, double dt);
+/* nsteps of the loop EH:697-708 (compute_dt, "Bad dt" check -> CC_ERR_BAD_DT, clamp to t_end,
+ * rk2_step).  *t_inout holds t on entry and the final t on return; dts (nullable) gets each dt. */
+cc_status cc_euler2d_run(cc_euler2d* h, int nsteps, double* t_inout, double t_end, double* dts);
+/* SOR sweep counts: of the last solve, summed over solves since the last reset, number of solves. */
+cc_status cc_euler2d_sweep_stats(cc_euler2d* h, int* last, long long* total, int* solves);
+cc_status cc_euler2d_sync(cc_euler2d* h);
+/* Per-kernel HIP-event timing ("sor", "flux", "dt"). */
+cc_status cc_euler2d_set_profiling(cc_euler2d* h, int on);
+cc_status cc_euler2d_kernel_time(cc_euler2d* h, const char* kernel, int* count, double* total_ms);
+cc_status cc_euler2d_reset_profiling(cc_euler2d* h);
+
+/* Batched HLLC on independent face states.  variant 0: EH hllc_x / hllc_y (EH:125-219);
+ * variant 1: NS hllc_x / hllc_y (NS:212-282).  W* are n x 4 {rho,u,v,P}; out n x 4
+ * {f_rho, f_mx, f_my, f_E}; axis 0 = x faces, 1 = y faces. */
+cc_status cc_hllc2d_batch(int variant, int axis, int n, const double* WL, const double* WR, double gamma,
+                          double floor_rho, int precision, double* out);
+
+/* ============================================================================================
+ * 2D Navier-Stokes HLLC + FFT self-gravity (NS), periodic.
+ * ========================================================================================== */
+
+/* The hot-path subset of struct Params (NS:54-86). */
+typedef struct cc_ns2d_params {
+    double gamma, cfl, x0, x1, y0, y1;
+    int Nx, Ny; /* powers of two, <= 4096 (NS:614-617 -> CC_ERR_NON_POW2) */
+    double floor_rho, floor_P;
+    int viscosity_on;
+    double mu, cfl_visc;
+    int gravity_on;
+    double G;
+    int precision; /* cc_precision: CC_MIXED runs the HLLC in FP32 */
+} cc_ns2d_params;
+
+typedef struct cc_ns2d cc_ns2d;
+
+cc_status cc_ns2d_create(const cc_ns2d_params* p, cc_ns2d** out);
+cc_status cc_ns2d_destroy(cc_ns2d* h);
+/* AoS state as std::vector<U4> (NS:88-93): U[4*q + {0: rho, 1: mx, 2: my, 3: E}], q = i + Nx*j. */
+cc_status cc_ns2d_upload(cc_ns2d* h, const double* U);
+cc_status cc_ns2d_download(cc_ns2d* h, double* U);
+/* phi of a host density field (solve_poisson_periodic_fft, NS:285-335).  Syncs. */
+cc_status cc_ns2d_poisson(cc_ns2d* h, const double* rho, double* phi);
+/* compute_rhs (NS:363-518) of the current state into dU (AoS).  Syncs. */
+cc_status cc_ns2d_rhs(cc_ns2d* h, double* dU);
+/* compute_dt (NS:521-546).  Syncs. */
+cc_status cc_ns2d_compute_dt(cc_ns2d* h, double* dt);
+/* The Heun update of the driver loop (NS:761-781) in place. */
+cc_status cc_ns2d_heun_step(cc_ns2d* h, double dt);
+/* nsteps of the loop NS:758-785 (compute_dt, clamp to t_end, Heun).  Syncs. */
+cc_status cc_ns2d_run(cc_ns2d* h, int nsteps, double* t_inout, double t_end, double* dts);
+cc_status cc_ns2d_sync(cc_ns2d* h);
+/* Per-kernel HIP-event timing ("fft", "flux", "dt"). */
+cc_status cc_ns2d_set_profiling(cc_ns2d* h, int on);
+cc_status cc_ns2d_kernel_time(cc_ns2d* h, const char* kernel, int* count, double* total_ms);
+cc_status cc_ns2d_reset_profiling(cc_ns2d* h);
+
+/* fft2d (NS:182-195) on the device: interleaved complex data[2*(i + Nx*j) + {0: re, 1: im}], rows
+ * then columns, inverse scaled by 1/n per axis.  Powers of two <= 4096. */
+cc_status cc_fft2d(int Nx, int Ny, int inverse, double* data);
+
+/* ============================================================================================
  * Measurement helper (no reference counterpart): a plain device-to-device copy of nbytes,
  * launched reps times; avg_ms receives the mean kernel time.  Its rocprofv3 FETCH_SIZE /
  * WRITE_SIZE readings per dispatch calibrate those counters against a known byte count.

@@ -108,6 +108,8 @@ double or_eh_compute_dt(const or_eh_params* p, const or_eh_state* U);
 void or_eh_rhs(const or_eh_params* p, const or_eh_state* U, or_eh_state* dU);
 void or_eh_rk2_step(const or_eh_params* p, or_eh_state* U, double dt);
 double or_eh_run(const or_eh_params* p, or_eh_state* U, int nsteps, double t0, double t_end, double* dts_out);
+/* reference harness only: the program's default IC (init_random_blob, EH:574-642) */
+void or_eh_init_random_blob(const or_eh_params* p, or_eh_state* U);
 
 /* ---- 2D Navier-Stokes HLLC + FFT gravity (navier_stokes_2d_hllc_poisson.cpp, Params NS:54-86) ---- */
 typedef struct {
@@ -123,12 +125,17 @@ typedef struct {
 /* AoS state as in the reference (NS:88-93): U[q*4 + {0:d,1:mx,2:my,3:E}], q = i + Nx*j */
 void or_ns_hllc_x(const or_ns_params* p, const double* WL4, const double* WR4, double* out4);
 void or_ns_hllc_y(const or_ns_params* p, const double* WL4, const double* WR4, double* out4);
+void or_ns_fft2d(int Nx, int Ny, int inverse, double* data); /* interleaved complex, q = i + Nx*j */
 int or_ns_poisson_fft(const or_ns_params* p, const double* rho, double* phi);
 void or_ns_gravity_accel(const or_ns_params* p, const double* phi, double* gx, double* gy);
 void or_ns_compute_rhs(const or_ns_params* p, const double* U, double* dU);
 double or_ns_compute_dt(const or_ns_params* p, const double* U);
 void or_ns_heun_step(const or_ns_params* p, double* U, double dt);
 double or_ns_run(const or_ns_params* p, double* U, int nsteps, double t0, double t_end, double* dts_out);
+/* reference harness only: libstdc++'s std::normal_distribution / uniform_real_distribution over
+ * std::mt19937(seed), n draws (the generators of the ICs at NS:638-639 and EH:585-587) */
+void or_std_normal(unsigned seed, int n, double* out);
+void or_std_uniform(unsigned seed, int n, double a, double b, double* out);
 
 #ifdef __cplusplus
 }

@@ -163,10 +163,126 @@ def m3_vectors(ref):
     np.savez_compressed(os.path.join(HERE, "m3_step.npz"), **res)
 
 
+def _hllc_branch(W, V, gamma, floor_rho, axis, guard):
+    """0: F_L, 1: F_R, 2: star L, 3: star R (classification only; the vectors hold the reference output)."""
+    un, vn = (W[1], V[1]) if axis == 0 else (W[2], V[2])
+    aL = np.sqrt(gamma * W[3] / max(W[0], floor_rho))
+    aR = np.sqrt(gamma * V[3] / max(V[0], floor_rho))
+    SL, SR = min(un - aL, vn - aR), max(un + aL, vn + aR)
+    rL, rR = max(W[0], floor_rho), max(V[0], floor_rho)
+    num = (V[3] - W[3]) + rL * un * (SL - un) - rR * vn * (SR - vn)
+    den = rL * (SL - un) - rR * (SR - vn)
+    SM = num / den if abs(den) > guard else 0.0
+    return 0 if SL >= 0 else 1 if SR <= 0 else 2 if SM > 0 else 3
+
+
+def _hllc_vectors(ref, prefix, params, guard, fx, fy, rng, want=250):
+    bins = {b: [] for b in range(4)}
+    tries = 0
+    while min(len(v) for v in bins.values()) < want and tries < 200000:
+        tries += 1
+        W = np.array([rng.uniform(0.1, 4), *rng.normal(0, 1, 2), rng.uniform(0.01, 3)])
+        V = np.array([rng.uniform(0.1, 4), *rng.normal(0, 1, 2), rng.uniform(0.01, 3)])
+        mode = tries % 5
+        if mode == 1:
+            W[1:3] += 5.0 * np.sign(rng.normal())
+            V[1:3] = W[1:3] + rng.normal(0, 0.1, 2)
+        if mode == 2:
+            V = W.copy()
+        b = _hllc_branch(W, V, params.gamma, params.floor_rho, tries % 2, guard)
+        if len(bins[b]) < want:
+            bins[b].append((W, V))
+    pairs = [x for b in range(4) for x in bins[b]]
+    WL = np.array([a for a, _ in pairs])
+    WR = np.array([b for _, b in pairs])
+    Fx, Fy = np.zeros_like(WL), np.zeros_like(WL)
+    for k in range(len(WL)):
+        getattr(ref, fx)(C.byref(params), OB.ptr(WL[k]), OB.ptr(WR[k]), OB.ptr(Fx[k]))
+        getattr(ref, fy)(C.byref(params), OB.ptr(WL[k]), OB.ptr(WR[k]), OB.ptr(Fy[k]))
+    return dict(WL=WL, WR=WR, Fx=Fx, Fy=Fy)
+
+
+def eh_vectors(ref):
+    rng = np.random.default_rng(77)
+    res = {}
+    for k, v in _hllc_vectors(ref, "eh", OB.eh_params(), 1e-14, "or_eh_hllc_x", "or_eh_hllc_y", rng).items():
+        res["hllc_" + k] = v
+    # Poisson, rhs, rk2 and short runs on the program's own IC (init_random_blob, EH:574-642)
+    for tag, (Nx, Ny, per) in {"a": (16, 16, 1), "b": (24, 20, 0)}.items():
+        p = OB.eh_params(Nx=Nx, Ny=Ny, periodic_x=per, periodic_y=per)
+        S = OB.EHArrays(Nx, Ny)
+        ref.or_eh_init_random_blob(C.byref(p), C.byref(S.cstate()))
+        for f in S.FIELDS:
+            res[tag + "_ic_" + f] = S.a[f].copy()
+        gx, gy = np.zeros(Nx * Ny), np.zeros(Nx * Ny)
+        ref.or_eh_poisson(C.byref(p), C.byref(S.cstate()), OB.ptr(gx), OB.ptr(gy), None)
+        res[tag + "_gx"], res[tag + "_gy"] = gx, gy
+        res[tag + "_dt"] = np.array([ref.or_eh_compute_dt(C.byref(p), C.byref(S.cstate()))])
+        dU = OB.EHArrays(Nx, Ny)
+        ref.or_eh_rhs(C.byref(p), C.byref(S.cstate()), C.byref(dU.cstate()))
+        for f in S.FIELDS:
+            res[tag + "_rhs_" + f] = dU.a[f]
+        T = S.copy()
+        ref.or_eh_rk2_step(C.byref(p), C.byref(T.cstate()), C.c_double(2e-3))
+        for f in S.FIELDS:
+            res[tag + "_rk2_" + f] = T.a[f]
+        R = S.copy()
+        dts = np.zeros(10)
+        t = ref.or_eh_run(C.byref(p), C.byref(R.cstate()), 10, C.c_double(0.0), C.c_double(5.0), OB.ptr(dts))
+        res[tag + "_run_dts"], res[tag + "_run_t"] = dts, np.array([t])
+        for f in S.FIELDS:
+            res[tag + "_run_" + f] = R.a[f]
+    np.savez_compressed(os.path.join(HERE, "eh_step.npz"), **res)
+
+
+def ns_vectors(ref):
+    rng = np.random.default_rng(78)
+    res = {}
+    for k, v in _hllc_vectors(ref, "ns", OB.ns_params(), 0.0, "or_ns_hllc_x", "or_ns_hllc_y", rng).items():
+        res["hllc_" + k] = v
+    n = 4000
+    for name, fn, args in (("normal", "or_std_normal", ()), ("uniform", "or_std_uniform", (C.c_double(-1.0),
+                                                                                           C.c_double(1.0)))):
+        a = np.zeros(n)
+        getattr(ref, fn)(42, n, *args, OB.ptr(a))
+        res["std_" + name] = a
+    for (Nx, Ny) in ((16, 16), (32, 8)):
+        d = rng.normal(0, 1, (Ny, Nx, 2))
+        res["fft_in_%dx%d" % (Nx, Ny)] = d.copy()
+        for inv in (0, 1):
+            e = d.reshape(-1).copy()
+            ref.or_ns_fft2d(Nx, Ny, inv, OB.ptr(e))
+            res["fft_out%d_%dx%d" % (inv, Nx, Ny)] = e.reshape(Ny, Nx, 2)
+    from core_collapse_amd import ic
+    from core_collapse_amd.ns2d import Params as NSP
+    for tag, (Nx, Ny, vis) in {"a": (32, 32, 1), "b": (16, 32, 0)}.items():
+        p = OB.ns_params(Nx=Nx, Ny=Ny, viscosity_on=vis, mu=1e-3)
+        U = ic.ns2d_reference_ic(NSP(Nx=Nx, Ny=Ny)).reshape(-1).copy()
+        res[tag + "_ic"] = U.copy()
+        rho = np.maximum(U[0::4], p.floor_rho).copy()
+        phi = np.zeros(Nx * Ny)
+        ref.or_ns_poisson_fft(C.byref(p), OB.ptr(rho), OB.ptr(phi))
+        res[tag + "_phi"] = phi
+        res[tag + "_dt"] = np.array([ref.or_ns_compute_dt(C.byref(p), OB.ptr(U))])
+        dU = np.zeros(4 * Nx * Ny)
+        ref.or_ns_compute_rhs(C.byref(p), OB.ptr(U), OB.ptr(dU))
+        res[tag + "_rhs"] = dU
+        T = U.copy()
+        ref.or_ns_heun_step(C.byref(p), OB.ptr(T), C.c_double(2e-3))
+        res[tag + "_heun"] = T
+        R = U.copy()
+        dts = np.zeros(20)
+        t = ref.or_ns_run(C.byref(p), OB.ptr(R), 20, C.c_double(0.0), C.c_double(10.0), OB.ptr(dts))
+        res[tag + "_run"], res[tag + "_run_dts"], res[tag + "_run_t"] = R, dts, np.array([t])
+    np.savez_compressed(os.path.join(HERE, "ns_step.npz"), **res)
+
+
 if __name__ == "__main__":
     orc = OB.Lib("oracle")
     m2_vectors(OB.Lib("ref_m2"), orc)
     m3_vectors(OB.Lib("ref_m3"))
+    eh_vectors(OB.Lib("ref_eh"))
+    ns_vectors(OB.Lib("ref_ns"))
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -1,0 +1,155 @@
+"""Pinning the 2D Navier-Stokes oracle (oracle/oracle_ns.c), CPU only: golden vectors from the
+reference's own functions (tests/golden/ns_step.npz, bit for bit), libstdc++'s normal / uniform
+distributions over std::mt19937 as restated in core_collapse_amd/ic.py, and closed forms -- the FFT
+against numpy.fft, the Poisson solve against the exact spectral solution with continuous k^2, HLLC
+uniform / supersonic / stationary-contact fluxes, and conservation of a periodic run.
+
+The default IC of the program (NS:638-688) lives inside its main(), so its formula is not
+callable by the reference build: only its random draws are pinned (parity unpinned for the formula)."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from oracle import bind as OB
+from tests.helpers import oracle, ref
+
+G = os.path.join(os.path.dirname(__file__), "golden")
+SIZES = {"a": (32, 32, 1), "b": (16, 32, 0)}
+
+
+def _flux(lib, fn, p, WL, WR):
+    out = np.zeros((len(WL), 4))
+    for k in range(len(WL)):
+        getattr(lib, fn)(C.byref(p), OB.ptr(np.ascontiguousarray(WL[k])), OB.ptr(np.ascontiguousarray(WR[k])),
+                         OB.ptr(out[k]))
+    return out
+
+
+def test_hllc_golden_bitwise():
+    d = np.load(os.path.join(G, "ns_step.npz"))
+    p = OB.ns_params()
+    assert np.array_equal(_flux(oracle(), "or_ns_hllc_x", p, d["hllc_WL"], d["hllc_WR"]), d["hllc_Fx"])
+    assert np.array_equal(_flux(oracle(), "or_ns_hllc_y", p, d["hllc_WL"], d["hllc_WR"]), d["hllc_Fy"])
+
+
+@pytest.mark.parametrize("shape", [(16, 16), (32, 8)])
+def test_fft_golden_bitwise_and_numpy(shape):
+    Nx, Ny = shape
+    d = np.load(os.path.join(G, "ns_step.npz"))
+    x = d["fft_in_%dx%d" % (Nx, Ny)]
+    for inv in (0, 1):
+        e = x.reshape(-1).copy()
+        oracle().or_ns_fft2d(Nx, Ny, inv, OB.ptr(e))
+        assert np.array_equal(e.reshape(Ny, Nx, 2), d["fft_out%d_%dx%d" % (inv, Nx, Ny)])
+        z = x[..., 0] + 1j * x[..., 1]
+        want = np.fft.ifft2(z) if inv else np.fft.fft2(z)
+        got = e.reshape(Ny, Nx, 2)
+        assert np.abs(got[..., 0] + 1j * got[..., 1] - want).max() < 1e-12 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("tag", ["a", "b"])
+def test_poisson_dt_rhs_heun_run_golden_bitwise(tag):
+    d = np.load(os.path.join(G, "ns_step.npz"))
+    Nx, Ny, vis = SIZES[tag]
+    o = oracle()
+    p = OB.ns_params(Nx=Nx, Ny=Ny, viscosity_on=vis, mu=1e-3)
+    U = d[tag + "_ic"].copy()
+    rho = np.maximum(U[0::4], p.floor_rho).copy()
+    phi = np.zeros(Nx * Ny)
+    o.or_ns_poisson_fft(C.byref(p), OB.ptr(rho), OB.ptr(phi))
+    assert np.array_equal(phi, d[tag + "_phi"])
+    assert o.or_ns_compute_dt(C.byref(p), OB.ptr(U)) == d[tag + "_dt"][0]
+    dU = np.zeros(4 * Nx * Ny)
+    o.or_ns_compute_rhs(C.byref(p), OB.ptr(U), OB.ptr(dU))
+    assert np.array_equal(dU, d[tag + "_rhs"])
+    T = U.copy()
+    o.or_ns_heun_step(C.byref(p), OB.ptr(T), C.c_double(2e-3))
+    assert np.array_equal(T, d[tag + "_heun"])
+    R = U.copy()
+    dts = np.zeros(20)
+    t = o.or_ns_run(C.byref(p), OB.ptr(R), 20, C.c_double(0.0), C.c_double(10.0), OB.ptr(dts))
+    assert np.array_equal(dts, d[tag + "_run_dts"]) and t == d[tag + "_run_t"][0]
+    assert np.array_equal(R, d[tag + "_run"])
+
+
+def test_libstdcxx_distributions_restated_bitwise():
+    from core_collapse_amd import ic
+    d = np.load(os.path.join(G, "ns_step.npz"))
+    g = ic.StdRandom(42)
+    assert np.array_equal(np.array([g.normal() for _ in range(len(d["std_normal"]))]), d["std_normal"])
+    u = ic.StdRandom(42)
+    assert np.array_equal(np.array([u.uniform(-1.0, 1.0) for _ in range(len(d["std_uniform"]))]), d["std_uniform"])
+
+
+def test_poisson_equals_continuous_spectral_solution():
+    Nx, Ny = 32, 16
+    p = OB.ns_params(Nx=Nx, Ny=Ny, G=3.0, x1=2.0)
+    rng = np.random.default_rng(4)
+    rho = 1 + rng.random((Ny, Nx))
+    phi = np.zeros(Nx * Ny)
+    assert oracle().or_ns_poisson_fft(C.byref(p), OB.ptr(rho.reshape(-1).copy()), OB.ptr(phi)) == 0
+    kx = 2 * np.pi * np.fft.fftfreq(Nx, d=2.0 / Nx)
+    ky = 2 * np.pi * np.fft.fftfreq(Ny, d=1.0 / Ny)
+    k2 = ky[:, None] ** 2 + kx[None, :] ** 2
+    k2[0, 0] = 1.0
+    ph = -4 * np.pi * 3.0 * np.fft.fft2(rho - rho.mean()) / k2
+    ph[0, 0] = 0.0
+    exact = np.real(np.fft.ifft2(ph))
+    assert np.abs(phi.reshape(Ny, Nx) - exact).max() < 1e-12 * np.abs(exact).max()
+    assert oracle().or_ns_poisson_fft(C.byref(OB.ns_params(Nx=24, Ny=16)), OB.ptr(rho.reshape(-1).copy()),
+                                      OB.ptr(phi)) == 1  # non-power-of-two (NS:614-617)
+
+
+def test_hllc_closed_forms():
+    o = oracle()
+    p = OB.ns_params()
+    rng = np.random.default_rng(5)
+    for k in range(300):
+        rho, u, v, P = rng.uniform(0.2, 3), *rng.normal(0, 1.5, 2), rng.uniform(0.05, 2)
+        W = np.array([rho, u, v, P])
+        E = P / 0.4 + 0.5 * rho * (u * u + v * v)
+        fx = [rho * u, rho * u * u + P, rho * u * v, (E + P) * u]
+        fy = [rho * v, rho * u * v, rho * v * v + P, (E + P) * v]
+        assert np.allclose(_flux(o, "or_ns_hllc_x", p, W[None], W[None])[0], fx, rtol=1e-12, atol=1e-12)
+        assert np.allclose(_flux(o, "or_ns_hllc_y", p, W[None], W[None])[0], fy, rtol=1e-12, atol=1e-12)
+    for k in range(50):
+        P = rng.uniform(0.1, 2)
+        L = np.array([rng.uniform(0.1, 1), 0.0, 0.0, P]); R = np.array([rng.uniform(1, 5), 0.0, 0.0, P])
+        assert np.allclose(_flux(o, "or_ns_hllc_x", p, L[None], R[None])[0], [0, P, 0, 0], atol=1e-13)
+
+
+def test_periodic_run_conserves():
+    from core_collapse_amd import ic
+    from core_collapse_amd.ns2d import Params
+    U = ic.ns2d_reference_ic(Params(Nx=16, Ny=16)).reshape(-1).copy()
+    for grav in (0, 1):
+        p = OB.ns_params(Nx=16, Ny=16, gravity_on=grav, mu=1e-3)
+        R = U.copy()
+        oracle().or_ns_run(C.byref(p), OB.ptr(R), 5, C.c_double(0.0), C.c_double(10.0), None)
+        s0, s1 = U.reshape(-1, 4).sum(0), R.reshape(-1, 4).sum(0)
+        assert abs(s1[0] - s0[0]) < 1e-12 * s0[0]
+        if not grav:
+            assert np.allclose(s1, s0, rtol=0, atol=1e-11 * np.abs(s0).max())
+
+
+def test_oracle_equals_reference_randomized():
+    r = ref("ref_ns")
+    if r is None:
+        pytest.skip("oracle/_ref/libref_ns.so is not built here (reference source absent)")
+    o = oracle()
+    rng = np.random.default_rng(12)
+    WL = np.array([[rng.uniform(0.05, 4), *rng.normal(0, 2, 2), rng.uniform(0.001, 2)] for _ in range(3000)])
+    WR = np.array([[rng.uniform(0.05, 4), *rng.normal(0, 2, 2), rng.uniform(0.001, 2)] for _ in range(3000)])
+    p = OB.ns_params()
+    for fn in ("or_ns_hllc_x", "or_ns_hllc_y"):
+        assert np.array_equal(_flux(o, fn, p, WL, WR), _flux(r, fn, p, WL, WR))
+    from core_collapse_amd import ic
+    from core_collapse_amd.ns2d import Params
+    p = OB.ns_params(Nx=16, Ny=8, mu=2e-3)
+    A = ic.ns2d_reference_ic(Params(Nx=16, Ny=8)).reshape(-1).copy()
+    B = A.copy()
+    assert o.or_ns_run(C.byref(p), OB.ptr(A), 4, C.c_double(0.0), C.c_double(10.0), None) == \
+        r.or_ns_run(C.byref(p), OB.ptr(B), 4, C.c_double(0.0), C.c_double(10.0), None)
+    assert np.array_equal(A, B)
