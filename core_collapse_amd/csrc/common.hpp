@@ -47,7 +47,18 @@ template <typename T>
 __host__ __device__ __forceinline__ T smax(T a, T b) { return (a < b) ? b : a; }
 template <typename T>
 __host__ __device__ __forceinline__ T smin(T a, T b) { return (b < a) ? b : a; }
-__host__ __device__ __forceinline__ int imod(int i, int n) { int r = i % n; return (r < 0) ? r + n : r; }
+// Periodic wrap.  Callers pass in-range indices shifted by a few cells, so the common cases are
+// resolved without an integer division (which expands to a long float-reciprocal sequence on the GPU).
+__host__ __device__ __forceinline__ int imod(int i, int n) {
+    if (i >= 0) {
+        if (i < n) return i;
+        if (i < 2 * n) return i - n;
+    } else if (i >= -n) {
+        return i + n;
+    }
+    const int r = i % n;
+    return (r < 0) ? r + n : r;
+}
 __host__ __device__ __forceinline__ int iclamp(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 // Boundary map of a cell (or face) index: periodic wrap modulo n, or clamp to [0, n-1].
 __host__ __device__ __forceinline__ int bmap(int i, int n, bool periodic) {

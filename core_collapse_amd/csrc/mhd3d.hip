@@ -188,7 +188,8 @@ __global__ __launch_bounds__(256) void k_flux3d(Geom3D g, Eos3D e, EOS64 eos64, 
     const int k1 = min(k0 + KZ3, g.Nz);
     const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
     const int i = i0 + tx, j = j0 + ty;
-    const bool col = i < Nx && j < Ny;
+    const bool in_tile = tid < TX3 * TY3;
+    const bool col = in_tile && i < Nx && j < Ny;
     const double frho = g.floor_rho, fP = g.floor_P;
     const double dt = sc->dt, invdx = 1.0 / g.dx, invdy = 1.0 / g.dy, invdz = 1.0 / g.dz;
 
@@ -207,12 +208,14 @@ __global__ __launch_bounds__(256) void k_flux3d(Geom3D g, Eos3D e, EOS64 eos64, 
     };
 
     // z register queue of this column: planes k-2, k-1, k, k+1
-    Prim<double> q0 = prim_at(g, e, U0, i, j, k0 - 2);
-    Prim<double> q1 = prim_at(g, e, U0, i, j, k0 - 1);
-    Prim<double> q2, q3;
+    Prim<double> q0{}, q1{}, q2{}, q3{};
+    if (in_tile) {
+        q0 = prim_at(g, e, U0, i, j, k0 - 2);
+        q1 = prim_at(g, e, U0, i, j, k0 - 1);
+    }
     fill_plane(0, k0);
     __syncthreads();
-    q2 = lds(0, ty + 2, tx + 2);
+    if (in_tile) q2 = lds(0, ty + 2, tx + 2);
     double Hprev[5] = {0, 0, 0, 0, 0};
     double dpl[5] = {0, 0, 0, 0, 0};  // in-plane flux difference of the previous plane's cell
 
@@ -220,7 +223,7 @@ __global__ __launch_bounds__(256) void k_flux3d(Geom3D g, Eos3D e, EOS64 eos64, 
         const int cur = (k - k0) & 1;
         if (k + 1 < k1) fill_plane(cur ^ 1, k + 1);
         __syncthreads();
-        q3 = (k + 1 < k1) ? lds(cur ^ 1, ty + 2, tx + 2) : prim_at(g, e, U0, i, j, k + 1);
+        if (in_tile) q3 = (k + 1 < k1) ? lds(cur ^ 1, ty + 2, tx + 2) : prim_at(g, e, U0, i, j, k + 1);
 
         // z-face k between cells k-1 and k (M3:777-800), by the (u,v,w)->(w,u,v) map of M3:604-622
         if (col) {
@@ -294,9 +297,12 @@ __global__ __launch_bounds__(256) void k_flux3d(Geom3D g, Eos3D e, EOS64 eos64, 
                 }
             }
             __syncthreads();
+            if (in_tile) {
 #pragma unroll
-            for (int c = 0; c < 5; c++)
-                dpl[c] = -(sFx[c][ty][tx + 1] - sFx[c][ty][tx]) * invdx - (sFy[c][ty + 1][tx] - sFy[c][ty][tx]) * invdy;
+                for (int c = 0; c < 5; c++)
+                    dpl[c] = -(sFx[c][ty][tx + 1] - sFx[c][ty][tx]) * invdx -
+                             (sFy[c][ty + 1][tx] - sFy[c][ty][tx]) * invdy;
+            }
         }
         __syncthreads();
         q0 = q1;
@@ -569,13 +575,13 @@ struct cc_mhd3d {
         CC_TRY(flux(in));
         {
             ProfScope ps(pr(), stream, "edge");
-            hipLaunchKernelGGL(k_edge3d, dim3((Nx + 1 + 127) / 128, Ny + 1, Nz + 1), dim3(128), 0, stream, g, ef, in, ed);
+            hipLaunchKernelGGL(k_edge3d, dim3((Nx + 1 + 63) / 64, Ny + 1, Nz + 1), dim3(64), 0, stream, g, ef, in, ed);
             CC_CHECK_LAUNCH();
         }
         const State3D& bdst = FINAL ? Bt : out;
         {
             ProfScope ps(pr(), stream, "ct");
-            hipLaunchKernelGGL(k_ct3d, dim3((Nx + 1 + 127) / 128, Ny + 1, Nz + 1), dim3(128), 0, stream, g, ed, in, bdst, sc);
+            hipLaunchKernelGGL(k_ct3d, dim3((Nx + 1 + 63) / 64, Ny + 1, Nz + 1), dim3(64), 0, stream, g, ed, in, bdst, sc);
             CC_CHECK_LAUNCH();
         }
         {
