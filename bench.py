@@ -20,6 +20,13 @@ launch, from rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/) shows the real HBM load
 
 cpu_baseline: the reference program's own step functions (compiled from its sources in
 oracle/_ref, single thread) on a bounded sample of the same problem (see "sample").
+
+The line also carries "turb3d": the 3D resistive MHD step (HLLD + CT, no gravity) on the 256^3
+periodic turbulence problem (ideal gas gamma 5/3, beta 1 field along z, k^-2 solenoidal velocity at
+rms Mach 1, eta 5e-5, CFL 0.4), FP64 and mixed, with the roofline of its dominant kernel (the fused
+flux kernel: per cell it reads 5 cell + 3 face words and writes 5 cell + 6 face-EMF words, 152 B)
+and the CPU restatement's rate on a 64^3 sample (the reference has no ideal-gas EOS, so the CPU
+leg is the oracle's port of its step with that EOS).  --workload selects one of the two problems.
 """
 import argparse
 import ctypes as C
@@ -46,6 +53,9 @@ def parse():
     ap.add_argument("--N", type=int, default=2048)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound on the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=("all", "collapse2d", "turb3d"), default="all")
+    ap.add_argument("--N3", type=int, default=256, help="grid of the 3D turbulence problem")
+    ap.add_argument("--steps3", type=int, default=10)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (for counter calibration) and exit")
     return ap.parse_args()
@@ -117,6 +127,87 @@ def cpu_baseline(N, budget_s):
                       "%.1f s), single thread, %s, nproc=%d" % (steps, n, n, tsum, cpu, os.cpu_count() or 0)}
 
 
+def run_gpu3d(N, precision, warmup, steps):
+    from core_collapse_amd import ic
+    from core_collapse_amd import mhd3d as M3
+    p = ic.turbulence3d_params(N, precision=precision)
+    h = M3.Mhd3D(p)
+    h.upload(ic.turbulence3d_ic(p))
+    t = h.run(warmup) if warmup > 0 else 0.0
+    h.set_profiling(True)
+    h.reset_profiling()
+    h.sync()
+    t0 = time.perf_counter()
+    t = h.run(steps, t)
+    elapsed = time.perf_counter() - t0
+    prof = {k: h.kernel_time(k) for k in ("flux", "edge", "ct", "joule", "dt")}
+    h.set_profiling(False)
+    out = h.download()
+    divb = h.divB_max()
+    h.close()
+    return {"elapsed": elapsed, "rho": out["rho"], "divb": divb, "prof": prof}
+
+
+def cpu_baseline3d(n=64, budget_s=12.0):
+    """The oracle's restatement of the 3D step (ideal-gas EOS mode) on a 64^3 sample."""
+    from core_collapse_amd import ic
+    from oracle import bind as OB
+    lib = OB.Lib("oracle")
+    p = ic.turbulence3d_params(n)
+    st = ic.turbulence3d_ic(p)
+    op = OB.m3_params(Nx=n, Ny=n, Nz=n, cfl=p.cfl, eta=p.eta, ideal_eos=1, gamma_ideal=p.gamma)
+    S = OB.M3Arrays(n, n, n, {k: v.reshape(-1) for k, v in st.items()})
+    steps, tsum = 0, 0.0
+    while steps < 3 and (steps == 0 or tsum < budget_s):
+        t0 = time.perf_counter()
+        lib.or_m3_run(C.byref(op), C.byref(S.cstate()), C.c_double(p.dx), C.c_double(p.dy), C.c_double(p.dz), 1,
+                      C.c_double(0.0), C.c_double(1e300), None)
+        tsum += time.perf_counter() - t0
+        steps += 1
+    return {"value": n ** 3 * steps / tsum, "unit": "zone-updates/s", "cores": 1, "kind": "port",
+            "sample": "%d Heun step(s) of the same turbulence problem at %d^3 (FP64, %.1f s), single thread, "
+                      "oracle restatement (the reference program has no ideal-gas EOS)" % (steps, n, tsum)}
+
+
+def bench3d(N, warmup, steps, with_cpu):
+    from core_collapse_amd import CC_FP64, CC_MIXED
+    res = {prec: run_gpu3d(N, prec, warmup, steps) for prec in (CC_FP64, CC_MIXED)}
+    zu = float(N) ** 3 * steps
+
+    def roof(r):
+        n, ms = r["prof"]["flux"]
+        if n == 0 or ms <= 0:
+            return None
+        bpl = float(N) ** 3 * 152.0
+        avg = ms / n * 1e-3
+        return {"bound": "hbm", "achieved": round(bpl / avg / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(bpl / avg / 1e9 / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_flux3d",
+                "launches": n, "avg_launch_us": round(avg * 1e6, 2), "alg_bytes_per_launch": bpl}
+
+    a, b = res[CC_MIXED]["rho"], res[CC_FP64]["rho"]
+    out = {
+        "workload": "3D resistive MHD HLLD+CT, periodic turbulence %d^3, ideal gas gamma 5/3, beta 1 along z, "
+                    "rms Mach 1, eta 5e-5, CFL 0.4" % N,
+        "steps": steps, "warmup": warmup,
+        "value": zu / res[CC_MIXED]["elapsed"], "unit": "zone-updates/s",
+        "ms_per_step": 1e3 * res[CC_MIXED]["elapsed"] / steps,
+        "fp64": {"value": zu / res[CC_FP64]["elapsed"], "ms_per_step": 1e3 * res[CC_FP64]["elapsed"] / steps,
+                 "roofline": roof(res[CC_FP64]), "max_divB": res[CC_FP64]["divb"]},
+        "mixed_over_fp64": res[CC_FP64]["elapsed"] / res[CC_MIXED]["elapsed"],
+        "l1_rho_rel_err": float(np.abs(a - b).sum() / np.abs(b).sum()),
+        "linf_rho_rel_err": float(np.abs(a - b).max() / np.abs(b).max()),
+        "max_divB": res[CC_MIXED]["divb"],
+        "roofline": roof(res[CC_MIXED]),
+        "zone_update_alg_bytes": 320,
+        "kernel_times": {tag: {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in res[prec]["prof"].items()
+                               if v[0] > 0} for prec, tag in ((CC_FP64, "fp64"), (CC_MIXED, "mixed"))},
+    }
+    out["hbm_frac_alg"] = round(out["value"] * 320 / 1e9 / HBM_PEAK_GBS, 4)
+    if with_cpu:
+        out["cpu_baseline"] = cpu_baseline3d()
+    return out
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "sor_tile_traffic.json")
     try:
@@ -142,6 +233,15 @@ def main():
                           "GBps": 2 * nbytes / ms.value / 1e6}))
         return
     N, K, W = a.N, a.steps, a.warmup
+    t3d = None
+    if a.workload in ("all", "turb3d"):
+        print("bench: %d^3 turbulence, warmup %d, timed %d steps" % (a.N3, 2, a.steps3), file=sys.stderr, flush=True)
+        t3d = bench3d(a.N3, 2, a.steps3, not a.no_cpu)
+        print("bench: turb3d mixed %.3f ms/step, fp64 %.3f ms/step" % (t3d["ms_per_step"], t3d["fp64"]["ms_per_step"]),
+              file=sys.stderr, flush=True)
+        if a.workload == "turb3d":
+            print(json.dumps({"turb3d": t3d}))
+            return
     print("bench: %dx%d collapse, warmup %d, timed %d steps" % (N, N, W, K), file=sys.stderr, flush=True)
     res = {}
     for prec in (CC_FP64, CC_MIXED):
@@ -207,6 +307,7 @@ def main():
         "roofline": roof_mx,
         "cpu_baseline": cpu,
         "kernel_times": kernels,
+        "turb3d": t3d,
     }
     print(json.dumps(line))
 

@@ -1,15 +1,19 @@
 // poisson2d.hip -- SOR Poisson gravity kernels (see poisson2d.hpp for the schedule).
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 
 #include "poisson2d.hpp"
 
 namespace cc {
 
 static constexpr int kReduceBlocks = 512;
-static constexpr int kTileW = 64;  // tile width along i = one wave64 (lane = column)
-static constexpr int kTileWJ = 64; // tile height along j
-static constexpr int kTileS = 16;  // sweeps per tile = waves per workgroup
+static constexpr int kTileW = 64;      // tile width along i = one wave64 (lane = column)
+// Tile shapes (sweeps S per tile = 16 waves x S/16, height WJ) compiled in; Poisson2D::init picks
+// one per precision (default below, or CC_SOR_TILE="S,WJ[,g]" -- g: FP64 source term from global).
+static constexpr SorTile kTileFp64Default{16, 64, true};
+static constexpr SorTile kTileMixedDefault{16, 64, true};
 
 // ------------------------------------------------------------------ reductions
 __global__ __launch_bounds__(256) void k_partial_sum(const double* __restrict__ x, size_t n,
@@ -44,94 +48,199 @@ cc_status device_mean(const double* x, size_t n, double* partial, double* out, h
 
 // ------------------------------------------------------------------ tiled lexicographic SOR
 // Cell (i,j) belongs to tile (TS,TJ,TI) at sweep s when s in [S*TS, S*TS+S), i+s in
-// [W*TI, W*TI+W), j+s in [WJ*TJ, WJ*TJ+WJ).  Within a tile, wave w runs sweep S*TS+w and lane l
-// walks column i = W*TI - s + l upward: at local step d it updates row j = WJ*TJ - s + (d - l).
-// All updates of one step are on distinct anti-diagonals (one colour of the checkerboard) and
-// every value they read was produced at step d-1 or earlier, or by an earlier tile level.
-template <int S>
+// [W*TI, W*TI+W), j+s in [WJ*TJ, WJ*TJ+WJ).  Within a tile, lane l of the sweep's wave walks
+// column i = W*TI - s + l upward: at local step d it updates row j = WJ*TJ - s + (d - l).  Every
+// update of one step has the same i + j + 2s (one colour of the red-black checkerboard of that
+// time level), so the updates of a step are independent and every value they read was produced
+// at an earlier step or by an earlier tile level.  Each of the 16 waves carries S/16 consecutive
+// sweeps; its cells of one step are diagonal neighbours and do not read each other.
+//
+// The tile's box is staged in LDS with a one-cell margin so that the staging and index arithmetic
+// are the same for every tile.
+template <int S, int WJ>
 struct TileGeom {
-    static constexpr int BI = kTileW + S + 1;           // box width (union of the S shifted squares + halo)
-    static constexpr int BJ = kTileWJ + S + 1;
-    static constexpr int P = (BI % 2 == 0) ? BI : BI + 1;  // even pitch: lanes of a diagonal hit distinct banks
+    static constexpr int BI = kTileW + S + 1;                       // widest box (union of shifted squares + halo)
+    static constexpr int BJ = WJ + S + 1;
+    static constexpr int P = ((BI + 2) % 2 == 0) ? BI + 2 : BI + 3;  // pitch with margin, even
+    static constexpr int R = BJ + 2;                                 // rows with margin
+    static constexpr int STEPS = kTileW + WJ - 1;
 };
 
 // Does the tile update cell (i,j) at some sweep s in [s0,s1)?  (i in [W*TI-s, W*TI-s+W) <=> s in
 // [W*TI-i, W*TI-i+W-1], likewise for j.)
+template <int WJ>
 __device__ __forceinline__ bool tile_updates(int i, int j, int s0, int s1, int TI, int TJ) {
-    const int lo = max(s0, max(kTileW * TI - i, kTileWJ * TJ - j));
-    const int hi = min(s1 - 1, min(kTileW * TI - i + kTileW - 1, kTileWJ * TJ - j + kTileWJ - 1));
+    const int lo = max(s0, max(kTileW * TI - i, WJ * TJ - j));
+    const int hi = min(s1 - 1, min(kTileW * TI - i + kTileW - 1, WJ * TJ - j + WJ - 1));
     return lo <= hi;
 }
 
-template <int S>
-__global__ __launch_bounds__(64 * S) void k_sor_tile_fp64(SorArgs a, const int* __restrict__ tiles,
-                                                           double* __restrict__ phi, const double* __restrict__ rho,
-                                                           const double* __restrict__ rhobar) {
-    using G = TileGeom<S>;
-    __shared__ double sphi[G::BJ * G::P];
-    __shared__ double srhs[G::BJ * G::P];
-    const int TS = tiles[3 * blockIdx.x], TJ = tiles[3 * blockIdx.x + 1], TI = tiles[3 * blockIdx.x + 2];
-    const int s0 = S * TS, s1 = min(s0 + S, a.nsw);
-    const int ib0 = max(0, kTileW * TI - (s1 - 1) - 1), ib1 = min(a.Nx - 1, kTileW * TI - s0 + kTileW);
-    const int jb0 = max(0, kTileWJ * TJ - (s1 - 1) - 1), jb1 = min(a.Ny - 1, kTileWJ * TJ - s0 + kTileWJ);
-    const int ni = ib1 - ib0 + 1, nj = jb1 - jb0 + 1;
-    const double rb = rhobar[0];
-    for (int c = threadIdx.x; c < ni * nj; c += blockDim.x) {
-        const int li = c % ni, lj = c / ni;
-        const size_t g = (size_t)(jb0 + lj) * a.Nx + (ib0 + li);
-        sphi[lj * G::P + li] = phi[g];
-        srhs[lj * G::P + li] = a.c4piG * (rho[g] - rb);
-    }
-    __syncthreads();
-    const int lane = threadIdx.x & 63;
-    const int s = s0 + (threadIdx.x >> 6);
-    const int ci = kTileW * TI - s + lane;
-    const bool col_ok = (s < s1) && ci >= 0 && ci < a.Nx;
-    const int li = ci - ib0;
-    const int liE = min(ci + 1, a.Nx - 1) - ib0, liW = max(ci - 1, 0) - ib0;
-    for (int d = 0; d < kTileW + kTileWJ - 1; ++d) {
-        const int lj = d - lane;
-        const int cj = kTileWJ * TJ - s + lj;
-        if (col_ok && lj >= 0 && lj < kTileWJ && cj >= 0 && cj < a.Ny) {
-            const int r = cj - jb0;
-            const int rN = min(cj + 1, a.Ny - 1) - jb0, rS = max(cj - 1, 0) - jb0;
-            const double pc = sphi[r * G::P + li];
-            const double pnew = ((sphi[r * G::P + liE] + sphi[r * G::P + liW]) * a.idx2 +
-                                 (sphi[rN * G::P + li] + sphi[rS * G::P + li]) * a.idy2 - srhs[r * G::P + li]) *
-                                a.inv_denom;
-            sphi[r * G::P + li] = pc + a.omega * (pnew - pc);
+struct TileBox {
+    int TS, TJ, TI, s0, s1, ib0, ib1, jb0, jb1, ni, nj;
+};
+
+template <int S, int WJ>
+__device__ __forceinline__ TileBox tile_box(const SorArgs& a, const int* __restrict__ tiles) {
+    TileBox b;
+    b.TS = tiles[3 * blockIdx.x]; b.TJ = tiles[3 * blockIdx.x + 1]; b.TI = tiles[3 * blockIdx.x + 2];
+    b.s0 = S * b.TS; b.s1 = min(b.s0 + S, a.nsw);
+    b.ib0 = max(0, kTileW * b.TI - (b.s1 - 1) - 1); b.ib1 = min(a.Nx - 1, kTileW * b.TI - b.s0 + kTileW);
+    b.jb0 = max(0, WJ * b.TJ - (b.s1 - 1) - 1); b.jb1 = min(a.Ny - 1, WJ * b.TJ - b.s0 + WJ);
+    b.ni = b.ib1 - b.ib0 + 1; b.nj = b.jb1 - b.jb0 + 1;
+    return b;
+}
+
+// Per-lane walk of one sweep: active steps [lo, hi), the LDS index of its step-0 row, its column.
+struct LaneWalk {
+    int lo, hi, base, ci, cj0;
+};
+
+template <int S, int WJ>
+__device__ __forceinline__ LaneWalk lane_walk(const SorArgs& a, const TileBox& b, int s, int lane) {
+    using G = TileGeom<S, WJ>;
+    LaneWalk w;
+    w.ci = kTileW * b.TI - s + lane;
+    const bool ok = s < b.s1 && w.ci >= 0 && w.ci < a.Nx;
+    const int ljlo = max(0, s - WJ * b.TJ), ljhi = min(WJ, a.Ny - WJ * b.TJ + s);  // 0 <= cj < Ny
+    w.lo = ok ? lane + ljlo : 1 << 30;
+    w.hi = ok ? lane + ljhi : 0;
+    w.cj0 = WJ * b.TJ - s - lane;  // cj at step d is cj0 + d
+    w.base = (w.cj0 - b.jb0 + 1) * G::P + (w.ci - b.ib0 + 1);
+    return w;
+}
+
+// The steps of one tile on the LDS box `x` (with one-cell margin).  `update(m, idx, pc, pE, pW, pN, pS)`
+// returns the new value of sweep m's cell at LDS index idx; `next(d)` runs at the start of step d
+// (before its updates) for prefetching.  A tile whose box does not reach a domain edge never
+// clamps: its steps are straight-line code in one masked region per sweep.  Tiles on a domain edge
+// clamp the neighbour indices (M2:197-198).
+template <int S, int WJ, typename T, typename F, typename N>
+__device__ __forceinline__ void tile_steps(T* __restrict__ x, const SorArgs& a, const TileBox& b,
+                                           const LaneWalk (&w)[S / 16], F&& update, N&& next) {
+    using G = TileGeom<S, WJ>;
+    constexpr int SPW = S / 16, P = G::P;
+    const bool interior = b.ib0 > 0 && b.ib1 < a.Nx - 1 && b.jb0 > 0 && b.jb1 < a.Ny - 1;  // block-uniform
+    if (interior) {
+        for (int d = 0; d < G::STEPS; ++d) {
+            next(d);
+#pragma unroll
+            for (int m = 0; m < SPW; ++m) {
+                if (d >= w[m].lo && d < w[m].hi) {
+                    const int idx = w[m].base + d * P;
+                    x[idx] = update(m, idx, x[idx], x[idx + 1], x[idx - 1], x[idx + P], x[idx - P]);
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
+    } else {
+        int oE[SPW], oW[SPW];
+#pragma unroll
+        for (int m = 0; m < SPW; ++m) {
+            oE[m] = w[m].ci < a.Nx - 1 ? 1 : 0;
+            oW[m] = w[m].ci > 0 ? -1 : 0;
+        }
+        for (int d = 0; d < G::STEPS; ++d) {
+            next(d);
+#pragma unroll
+            for (int m = 0; m < SPW; ++m) {
+                if (d >= w[m].lo && d < w[m].hi) {
+                    const int idx = w[m].base + d * P, cj = w[m].cj0 + d;
+                    const int oN = cj < a.Ny - 1 ? P : 0, oS = cj > 0 ? -P : 0;
+                    x[idx] = update(m, idx, x[idx], x[idx + oE[m]], x[idx + oW[m]], x[idx + oN], x[idx + oS]);
+                }
+            }
+            __syncthreads();
+        }
     }
-    for (int c = threadIdx.x; c < ni * nj; c += blockDim.x) {
-        const int li2 = c % ni, lj2 = c / ni;
-        const int i = ib0 + li2, j = jb0 + lj2;
-        if (tile_updates(i, j, s0, s1, TI, TJ)) phi[(size_t)j * a.Nx + i] = sphi[lj2 * G::P + li2];
+}
+
+// FP64: phi is staged in LDS; the source term 4 pi G (rho - mean) either too (RHS_LDS: twice the
+// LDS, one tile per CU) or it is loaded from global memory one step ahead (half the LDS, two tiles
+// per CU).
+template <int S, int WJ, bool RHS_LDS>
+__global__ __launch_bounds__(1024) void k_sor_tile_fp64(SorArgs a, const int* __restrict__ tiles,
+                                                         double* __restrict__ phi, const double* __restrict__ rho,
+                                                         const double* __restrict__ rhobar) {
+    using G = TileGeom<S, WJ>;
+    constexpr int SPW = S / 16, P = G::P;
+    __shared__ double sphi[G::R * P];
+    __shared__ double srhs[RHS_LDS ? G::R * P : 1];
+    const TileBox b = tile_box<S, WJ>(a, tiles);
+    const double rb = rhobar[0];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int c = tid; c < (b.ni + 2) * (b.nj + 2); c += 1024) {
+        const int x = c % (b.ni + 2), y = c / (b.ni + 2);
+        double v = 0.0, r = 0.0;  // margin cells are never read
+        if (x > 0 && x <= b.ni && y > 0 && y <= b.nj) {
+            const size_t g = (size_t)(b.jb0 - 1 + y) * a.Nx + (b.ib0 - 1 + x);
+            v = phi[g];
+            if (RHS_LDS) r = a.c4piG * (rho[g] - rb);
+        }
+        sphi[y * P + x] = v;
+        if (RHS_LDS) srhs[y * P + x] = r;
+    }
+    LaneWalk w[SPW];
+    long long g0[SPW];
+    double rc[SPW], rn[SPW];
+#pragma unroll
+    for (int m = 0; m < SPW; ++m) {
+        w[m] = lane_walk<S, WJ>(a, b, b.s0 + wv * SPW + m, lane);
+        g0[m] = (long long)w[m].cj0 * a.Nx + w[m].ci;  // global index of the step-0 row
+        rc[m] = rn[m] = 0.0;
+    }
+    auto src = [&](int m, int d) {  // rho of sweep m's cell at step d (index 0 when inactive)
+        const bool act = d >= w[m].lo && d < w[m].hi;
+        const long long g = act ? g0[m] + (long long)d * a.Nx : 0;
+        return rho[g];
+    };
+    if (!RHS_LDS) {
+#pragma unroll
+        for (int m = 0; m < SPW; ++m) rn[m] = src(m, 0);
+    }
+    const double idx2 = a.idx2, idy2 = a.idy2, inv_denom = a.inv_denom, omega = a.omega, c4 = a.c4piG;
+    __syncthreads();
+    tile_steps<S, WJ>(
+        sphi, a, b, w,
+        [&](int m, int idx, double pc, double pE, double pW, double pN, double pS) {
+            const double r = RHS_LDS ? srhs[idx] : c4 * (rc[m] - rb);
+            const double pnew = ((pE + pW) * idx2 + (pN + pS) * idy2 - r) * inv_denom;
+            return pc + omega * (pnew - pc);
+        },
+        [&](int d) {
+            if (!RHS_LDS) {
+#pragma unroll
+                for (int m = 0; m < SPW; ++m) {
+                    rc[m] = rn[m];
+                    rn[m] = src(m, d + 1);
+                }
+            }
+        });
+    for (int c = tid; c < b.ni * b.nj; c += 1024) {
+        const int x = c % b.ni, y = c / b.ni;
+        const int i = b.ib0 + x, j = b.jb0 + y;
+        if (tile_updates<WJ>(i, j, b.s0, b.s1, b.TI, b.TJ)) phi[(size_t)j * a.Nx + i] = sphi[(y + 1) * P + x + 1];
     }
 }
 
 // Mixed precision: phi = phi_hi (FP64, fixed for the tile) + delta (FP32 in LDS).  With
 // c = (stencil(phi_hi) - rhs)/denom - phi_hi evaluated once in FP64, one SOR update is
 // delta += omega * (c + kx (dE + dW) + ky (dN + dS) - delta), exactly the FP64 update rearranged.
-template <int S>
-__global__ __launch_bounds__(64 * S) void k_sor_tile_mixed(SorArgs a, const int* __restrict__ tiles,
-                                                            double* __restrict__ phi, const double* __restrict__ rho,
-                                                            const double* __restrict__ rhobar) {
-    using G = TileGeom<S>;
-    __shared__ float sd[G::BJ * G::P];
-    __shared__ float sc[G::BJ * G::P];
-    const int TS = tiles[3 * blockIdx.x], TJ = tiles[3 * blockIdx.x + 1], TI = tiles[3 * blockIdx.x + 2];
-    const int s0 = S * TS, s1 = min(s0 + S, a.nsw);
-    const int ib0 = max(0, kTileW * TI - (s1 - 1) - 1), ib1 = min(a.Nx - 1, kTileW * TI - s0 + kTileW);
-    const int jb0 = max(0, kTileWJ * TJ - (s1 - 1) - 1), jb1 = min(a.Ny - 1, kTileWJ * TJ - s0 + kTileWJ);
-    const int ni = ib1 - ib0 + 1, nj = jb1 - jb0 + 1;
+template <int S, int WJ>
+__global__ __launch_bounds__(1024) void k_sor_tile_mixed(SorArgs a, const int* __restrict__ tiles,
+                                                          double* __restrict__ phi, const double* __restrict__ rho,
+                                                          const double* __restrict__ rhobar) {
+    using G = TileGeom<S, WJ>;
+    constexpr int SPW = S / 16, P = G::P;
+    __shared__ float sd[G::R * P];
+    __shared__ float sc[G::R * P];
+    const TileBox b = tile_box<S, WJ>(a, tiles);
     const double rb = rhobar[0];
-    for (int c = threadIdx.x; c < ni * nj; c += blockDim.x) {
-        const int li = c % ni, lj = c / ni;
-        const int i = ib0 + li, j = jb0 + lj;
-        sd[lj * G::P + li] = 0.f;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int c = tid; c < (b.ni + 2) * (b.nj + 2); c += 1024) {
+        const int x = c % (b.ni + 2), y = c / (b.ni + 2);
+        const int i = b.ib0 - 1 + x, j = b.jb0 - 1 + y;
         float cv = 0.f;
-        if (tile_updates(i, j, s0, s1, TI, TJ)) {
+        if (x > 0 && x <= b.ni && y > 0 && y <= b.nj && tile_updates<WJ>(i, j, b.s0, b.s1, b.TI, b.TJ)) {
             const size_t g = (size_t)j * a.Nx + i;
             const double pc = phi[g];
             const double pE = phi[(size_t)j * a.Nx + min(i + 1, a.Nx - 1)];
@@ -141,34 +250,27 @@ __global__ __launch_bounds__(64 * S) void k_sor_tile_mixed(SorArgs a, const int*
             const double rhs = a.c4piG * (rho[g] - rb);
             cv = (float)(((pE + pW) * a.idx2 + (pN + pS) * a.idy2 - rhs) * a.inv_denom - pc);
         }
-        sc[lj * G::P + li] = cv;
+        sd[y * P + x] = 0.f;
+        sc[y * P + x] = cv;
     }
+    LaneWalk w[SPW];
+#pragma unroll
+    for (int m = 0; m < SPW; ++m) w[m] = lane_walk<S, WJ>(a, b, b.s0 + wv * SPW + m, lane);
+    const float kx = a.kxf, ky = a.kyf, om = a.omegaf;
     __syncthreads();
-    const int lane = threadIdx.x & 63;
-    const int s = s0 + (threadIdx.x >> 6);
-    const int ci = kTileW * TI - s + lane;
-    const bool col_ok = (s < s1) && ci >= 0 && ci < a.Nx;
-    const int li = ci - ib0;
-    const int liE = min(ci + 1, a.Nx - 1) - ib0, liW = max(ci - 1, 0) - ib0;
-    for (int d = 0; d < kTileW + kTileWJ - 1; ++d) {
-        const int lj = d - lane;
-        const int cj = kTileWJ * TJ - s + lj;
-        if (col_ok && lj >= 0 && lj < kTileWJ && cj >= 0 && cj < a.Ny) {
-            const int r = cj - jb0;
-            const int rN = min(cj + 1, a.Ny - 1) - jb0, rS = max(cj - 1, 0) - jb0;
-            const float dc = sd[r * G::P + li];
-            const float gs = sc[r * G::P + li] + (sd[r * G::P + liE] + sd[r * G::P + liW]) * a.kxf +
-                             (sd[rN * G::P + li] + sd[rS * G::P + li]) * a.kyf;
-            sd[r * G::P + li] = dc + a.omegaf * (gs - dc);
-        }
-        __syncthreads();
-    }
-    for (int c = threadIdx.x; c < ni * nj; c += blockDim.x) {
-        const int li2 = c % ni, lj2 = c / ni;
-        const int i = ib0 + li2, j = jb0 + lj2;
-        if (tile_updates(i, j, s0, s1, TI, TJ)) {
+    tile_steps<S, WJ>(
+        sd, a, b, w,
+        [&](int, int idx, float dc, float dE, float dW, float dN, float dS) {
+            const float gs = sc[idx] + (dE + dW) * kx + (dN + dS) * ky;
+            return dc + om * (gs - dc);
+        },
+        [](int) {});
+    for (int c = tid; c < b.ni * b.nj; c += 1024) {
+        const int x = c % b.ni, y = c / b.ni;
+        const int i = b.ib0 + x, j = b.jb0 + y;
+        if (tile_updates<WJ>(i, j, b.s0, b.s1, b.TI, b.TJ)) {
             const size_t g = (size_t)j * a.Nx + i;
-            phi[g] = phi[g] + (double)sd[lj2 * G::P + li2];
+            phi[g] = phi[g] + (double)sd[(y + 1) * P + x + 1];
         }
     }
 }
@@ -225,6 +327,46 @@ __global__ void k_sub_mean(const double* __restrict__ x, size_t n, const double*
         out[q] = x[q] - m;
 }
 
+// ------------------------------------------------------------------ tile dispatch
+// Compiled shapes (S, WJ[, source term in LDS]); FP64 with the source term in LDS needs 16 B per
+// box cell, which fits 160 KB of LDS only at WJ = 64.
+#define CC_SOR_TILES_FP64(X) X(16, 64, true) X(32, 64, true) X(32, 64, false) X(16, 128, false) X(32, 128, false)
+#define CC_SOR_TILES_MIXED(X) X(16, 64) X(32, 64) X(16, 128) X(32, 128)
+
+static bool sor_tile_supported(const SorTile& t, bool mixed) {
+#define CC_SOR_MATCH64(S_, WJ_, R_) if (!mixed && t.S == S_ && t.WJ == WJ_ && t.rhs_lds == R_) return true;
+#define CC_SOR_MATCH32(S_, WJ_) if (mixed && t.S == S_ && t.WJ == WJ_) return true;
+    CC_SOR_TILES_FP64(CC_SOR_MATCH64)
+    CC_SOR_TILES_MIXED(CC_SOR_MATCH32)
+#undef CC_SOR_MATCH64
+#undef CC_SOR_MATCH32
+    return false;
+}
+
+static cc_status launch_tile(const SorTile& t, bool mixed, int nt, hipStream_t s, const SorArgs& args, const int* sched,
+                             double* phi, const double* rho, const double* rhobar) {
+#define CC_SOR_LAUNCH64(S_, WJ_, R_)                                                                          \
+    if (!mixed && t.S == S_ && t.WJ == WJ_ && t.rhs_lds == R_) {                                              \
+        hipLaunchKernelGGL((k_sor_tile_fp64<S_, WJ_, R_>), dim3(nt), dim3(1024), 0, s, args, sched, phi, rho, \
+                           rhobar);                                                                           \
+        CC_CHECK_LAUNCH();                                                                                    \
+        return CC_OK;                                                                                         \
+    }
+#define CC_SOR_LAUNCH32(S_, WJ_)                                                                           \
+    if (mixed && t.S == S_ && t.WJ == WJ_) {                                                               \
+        hipLaunchKernelGGL((k_sor_tile_mixed<S_, WJ_>), dim3(nt), dim3(1024), 0, s, args, sched, phi, rho, \
+                           rhobar);                                                                        \
+        CC_CHECK_LAUNCH();                                                                                 \
+        return CC_OK;                                                                                      \
+    }
+    CC_SOR_TILES_FP64(CC_SOR_LAUNCH64)
+    CC_SOR_TILES_MIXED(CC_SOR_LAUNCH32)
+#undef CC_SOR_LAUNCH64
+#undef CC_SOR_LAUNCH32
+    set_error(CC_ERR_INVALID_ARG, "SOR tile shape not compiled in");
+    return CC_ERR_INVALID_ARG;
+}
+
 // ------------------------------------------------------------------ host side
 cc_status Poisson2D::init(int Nx_, int Ny_, bool px_, bool py_, double dx_, double dy_, double G_, int iters_,
                           double omega_, int precision_) {
@@ -247,7 +389,19 @@ cc_status Poisson2D::init(int Nx_, int Ny_, bool px_, bool py_, double dx_, doub
 
     level_off_.assign(1, 0);
     if (tiled_ && iters > 0) {
-        const int W = kTileW, WJ = kTileWJ, S = kTileS;
+        SorTile t = precision == CC_MIXED ? kTileMixedDefault : kTileFp64Default;
+        if (const char* e = std::getenv("CC_SOR_TILE")) {
+            int S_ = 0, WJ_ = 0;
+            char g = 0;
+            const int n = std::sscanf(e, "%d,%d,%c", &S_, &WJ_, &g);
+            if (n >= 2) t = SorTile{S_, WJ_, !(n == 3 && g == 'g')};
+        }
+        if (!sor_tile_supported(t, precision == CC_MIXED)) {
+            set_error(CC_ERR_INVALID_ARG, "Poisson2D: unsupported SOR tile shape");
+            return CC_ERR_INVALID_ARG;
+        }
+        tile_ = t;
+        const int W = kTileW, WJ = t.WJ, S = t.S;
         const int nTS = (iters + S - 1) / S;
         const int nTI = (Nx + iters - 1 + W - 1) / W;
         const int nTJ = (Ny + iters - 1 + WJ - 1) / WJ;
@@ -305,12 +459,8 @@ cc_status Poisson2D::solve(const double* rho, double* gx, double* gy, hipStream_
         for (int l = 0; l < nl; ++l) {
             const int nt = level_off_[l + 1] - level_off_[l];
             ProfScope ps(prof, s, "sor");
-            if (precision == CC_MIXED)
-                hipLaunchKernelGGL(k_sor_tile_mixed<kTileS>, dim3(nt), dim3(64 * kTileS), 0, s, args,
-                                   sched + 3 * level_off_[l], phi, rho, rhobar);
-            else
-                hipLaunchKernelGGL(k_sor_tile_fp64<kTileS>, dim3(nt), dim3(64 * kTileS), 0, s, args,
-                                   sched + 3 * level_off_[l], phi, rho, rhobar);
+            CC_TRY(launch_tile(tile_, precision == CC_MIXED, nt, s, args, sched + 3 * level_off_[l], phi, rho,
+                               rhobar));
             CC_CHECK_LAUNCH();
         }
     } else if (iters > 0) {

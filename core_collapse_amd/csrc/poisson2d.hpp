@@ -24,6 +24,13 @@
 
 namespace cc {
 
+// Tile shape of the tiled schedule: S sweeps per tile (16 waves x S/16), WJ rows; FP64 keeps the
+// source term in LDS (rhs_lds) or loads it from global memory one step ahead.
+struct SorTile {
+    int S = 32, WJ = 64;
+    bool rhs_lds = true;
+};
+
 struct SorArgs {
     int Nx, Ny, nsw;
     int px, py;
@@ -53,6 +60,7 @@ class Poisson2D {
     double* partial = nullptr;  // reduction partials
     double* rhobar = nullptr;   // device scalar(s): [0] mean rho, [1] mean phi
     int* sched = nullptr;       // (TS,TJ,TI) per tile, grouped by level
+    SorTile tile_{};            // tile shape of the tiled schedule
     std::vector<int> level_off_;
     SorArgs args{};
 };

@@ -111,7 +111,8 @@ def test_tiled_schedule_equals_lexicographic_bitwise(Nx, Ny, nsw, W, WJ, S):
         assert np.array_equal(got, ref)
 
 
-@pytest.mark.parametrize("Nx,Ny,nsw,W,WJ,S", [(9, 7, 9, 4, 3, 2), (13, 11, 12, 4, 4, 3), (150, 140, 40, 64, 64, 16)])
+@pytest.mark.parametrize("Nx,Ny,nsw,W,WJ,S", [(9, 7, 9, 4, 3, 2), (13, 11, 12, 4, 4, 3), (150, 140, 40, 64, 64, 16),
+                                               (150, 300, 70, 64, 64, 32), (150, 300, 70, 64, 128, 32)])
 def test_no_conflicts_within_a_level(Nx, Ny, nsw, W, WJ, S):
     for lev in tile_levels(Nx, Ny, nsw, W, WJ, S):
         writes, reads = [], []
@@ -137,3 +138,75 @@ def test_every_update_happens_once():
             for (_, s, i, j) in tile_cells(tile, Nx, Ny, nsw, W, WJ, S):
                 count[(s, i, j)] = count.get((s, i, j), 0) + 1
     assert len(count) == nsw * Nx * Ny and set(count.values()) == {1}
+
+
+def tiled_ghost(rhs, nsw, idx2, idy2, omega, W, WJ, S, reverse_levels=False):
+    """The kernel as written since the ghost-margin form: the tile's box is copied with a one-cell
+    margin; margin cells on a domain edge mirror the boundary cell (clamped neighbour) and are
+    rewritten together with it, margin cells inside the domain are never read (NaN here, so a read
+    would show); every stencil read is an unclamped box offset.  Each lane walks its column over the
+    steps [lane + ljlo, lane + ljhi) that the kernel computes once per lane."""
+    Ny, Nx = rhs.shape
+    phi = np.zeros_like(rhs)
+    inv = 1.0 / (2.0 * (idx2 + idy2))
+    for lev in tile_levels(Nx, Ny, nsw, W, WJ, S):
+        for tile in (reversed(lev) if reverse_levels else lev):
+            TS, TJ, TI = tile
+            s0, s1 = S * TS, min(S * TS + S, nsw)
+            ib0, ib1 = max(0, W * TI - (s1 - 1) - 1), min(Nx - 1, W * TI - s0 + W)
+            jb0, jb1 = max(0, WJ * TJ - (s1 - 1) - 1), min(Ny - 1, WJ * TJ - s0 + WJ)
+            ni, nj = ib1 - ib0 + 1, jb1 - jb0 + 1
+            assert ni + 2 <= W + S + 3 and nj + 2 <= WJ + S + 3
+            box = np.full((nj + 2, ni + 2), np.nan)
+            box[1:-1, 1:-1] = phi[jb0:jb1 + 1, ib0:ib1 + 1]
+            if ib0 == 0:
+                box[1:-1, 0] = box[1:-1, 1]
+            if ib1 == Nx - 1:
+                box[1:-1, -1] = box[1:-1, -2]
+            if jb0 == 0:
+                box[0, 1:-1] = box[1, 1:-1]
+            if jb1 == Ny - 1:
+                box[-1, 1:-1] = box[-2, 1:-1]
+            for d in range(W + WJ - 1):
+                for s in range(s0, s1):
+                    for lane in range(W):
+                        ci = W * TI - s + lane
+                        if not (0 <= ci < Nx):
+                            continue
+                        ljlo, ljhi = max(0, s - WJ * TJ), min(WJ, Ny - WJ * TJ + s)
+                        if not (lane + ljlo <= d < lane + ljhi):
+                            continue
+                        cj = WJ * TJ - s + d - lane
+                        assert 0 <= cj < Ny
+                        x, y = ci - ib0 + 1, cj - jb0 + 1
+                        assert 1 <= x <= ni and 1 <= y <= nj  # LDS reads stay inside the box + margin
+                        pc = box[y, x]
+                        pnew = ((box[y, x + 1] + box[y, x - 1]) * idx2 + (box[y + 1, x] + box[y - 1, x]) * idy2
+                                - rhs[cj, ci]) * inv
+                        v = pc + omega * (pnew - pc)
+                        assert np.isfinite(v)
+                        box[y, x] = v
+                        if ci == 0:
+                            box[y, x - 1] = v
+                        if ci == Nx - 1:
+                            box[y, x + 1] = v
+                        if cj == 0:
+                            box[y - 1, x] = v
+                        if cj == Ny - 1:
+                            box[y + 1, x] = v
+            for j in range(jb0, jb1 + 1):
+                for i in range(ib0, ib1 + 1):
+                    if tile_updates(i, j, s0, s1, TI, TJ, W, WJ):
+                        phi[j, i] = box[j - jb0 + 1, i - ib0 + 1]
+    return phi
+
+
+@pytest.mark.parametrize("Nx,Ny,nsw,W,WJ,S", [(9, 7, 5, 4, 3, 2), (11, 13, 9, 4, 4, 3), (10, 9, 12, 4, 8, 4),
+                                               (21, 19, 13, 8, 16, 8), (70, 40, 9, 64, 64, 4)])
+def test_ghost_margin_kernel_equals_lexicographic_bitwise(Nx, Ny, nsw, W, WJ, S):
+    rng = np.random.default_rng(Nx * 100 + Ny + 7)
+    rhs = rng.standard_normal((Ny, Nx))
+    idx2, idy2 = float(Nx * Nx), float(Ny * Ny)
+    ref = lexicographic(rhs, nsw, idx2, idy2, 1.8)
+    for rev in (False, True):
+        assert np.array_equal(tiled_ghost(rhs, nsw, idx2, idy2, 1.8, W, WJ, S, reverse_levels=rev), ref)
