@@ -16,7 +16,7 @@ handle's stream around every launch inside the timed region.  Algorithmic bytes 
 (cell-sweeps in the launch) x 24 B (FP64: read phi, read rhs, write phi) or x 12 B (mixed: FP32
 phi and rhs) -- the per-sweep figure of the survey's B_alg.  The kernel keeps its tile in LDS for
 16 sweeps, so its algorithmic rate can exceed the HBM peak; `traffic` (measured HBM bytes per
-launch, from rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/) shows the real HBM load.
+launch, from rocprofv3 FETCH_SIZE/WRITE_SIZE in profiles/hbm_traffic.json) shows the real HBM load.
 
 cpu_baseline: the reference program's own step functions (compiled from its sources in
 oracle/_ref, single thread) on a bounded sample of the same problem (see "sample").
@@ -185,6 +185,11 @@ def bench3d(N, warmup, steps, with_cpu):
                 "launches": n, "avg_launch_us": round(avg * 1e6, 2), "alg_bytes_per_launch": bpl}
 
     a, b = res[CC_MIXED]["rho"], res[CC_FP64]["rho"]
+    r64, rmx = roof(res[CC_FP64]), roof(res[CC_MIXED])
+    if r64:
+        r64["traffic"] = load_traffic("cc::k_flux3d<false")
+    if rmx:
+        rmx["traffic"] = load_traffic("cc::k_flux3d<true")
     out = {
         "workload": "3D resistive MHD HLLD+CT, periodic turbulence %d^3, ideal gas gamma 5/3, beta 1 along z, "
                     "rms Mach 1, eta 5e-5, CFL 0.4" % N,
@@ -192,12 +197,12 @@ def bench3d(N, warmup, steps, with_cpu):
         "value": zu / res[CC_MIXED]["elapsed"], "unit": "zone-updates/s",
         "ms_per_step": 1e3 * res[CC_MIXED]["elapsed"] / steps,
         "fp64": {"value": zu / res[CC_FP64]["elapsed"], "ms_per_step": 1e3 * res[CC_FP64]["elapsed"] / steps,
-                 "roofline": roof(res[CC_FP64]), "max_divB": res[CC_FP64]["divb"]},
+                 "roofline": r64, "max_divB": res[CC_FP64]["divb"]},
         "mixed_over_fp64": res[CC_FP64]["elapsed"] / res[CC_MIXED]["elapsed"],
         "l1_rho_rel_err": float(np.abs(a - b).sum() / np.abs(b).sum()),
         "linf_rho_rel_err": float(np.abs(a - b).max() / np.abs(b).max()),
         "max_divB": res[CC_MIXED]["divb"],
-        "roofline": roof(res[CC_MIXED]),
+        "roofline": rmx,
         "zone_update_alg_bytes": 320,
         "kernel_times": {tag: {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in res[prec]["prof"].items()
                                if v[0] > 0} for prec, tag in ((CC_FP64, "fp64"), (CC_MIXED, "mixed"))},
@@ -208,13 +213,19 @@ def bench3d(N, warmup, steps, with_cpu):
     return out
 
 
-def load_traffic():
-    path = os.path.join(ROOT, "profiles", "sor_tile_traffic.json")
+def load_traffic(prefix):
+    """Measured HBM bytes per launch of the first kernel whose name starts with `prefix`, from the
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary in profiles/hbm_traffic.json (None if absent)."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     try:
         with open(path) as f:
-            return json.load(f)
-    except (OSError, ValueError):
+            k = json.load(f)["kernels"]
+    except (OSError, ValueError, KeyError):
         return None
+    for name, b in k.items():
+        if name.startswith(prefix):
+            return b
+    return None
 
 
 def main():
@@ -271,11 +282,10 @@ def main():
 
     roof_mx = roof(res[CC_MIXED], 12.0)
     roof_64 = roof(res[CC_FP64], 24.0)
-    tr = load_traffic()
-    if roof_mx and tr and "mixed_bytes_per_launch" in tr:
-        roof_mx["traffic"] = tr["mixed_bytes_per_launch"]
-    if roof_64 and tr and "fp64_bytes_per_launch" in tr:
-        roof_64["traffic"] = tr["fp64_bytes_per_launch"]
+    if roof_mx:
+        roof_mx["traffic"] = load_traffic("cc::k_sor_tile_mixed")
+    if roof_64:
+        roof_64["traffic"] = load_traffic("cc::k_sor_tile_fp64")
     kernels = {}
     for prec, tag in ((CC_FP64, "fp64"), (CC_MIXED, "mixed")):
         kernels[tag] = {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in res[prec]["prof"].items()
