@@ -23,8 +23,9 @@ oracle/_ref, single thread) on a bounded sample of the same problem (see "sample
 
 The line also carries "turb3d": the 3D resistive MHD step (HLLD + CT, no gravity) on the 256^3
 periodic turbulence problem (ideal gas gamma 5/3, beta 1 field along z, k^-2 solenoidal velocity at
-rms Mach 1, eta 5e-5, CFL 0.4), FP64 and mixed, with the roofline of its dominant kernel (the fused
-flux kernel: per cell it reads 5 cell + 3 face words and writes 5 cell + 6 face-EMF words, 152 B)
+rms Mach 1, eta 5e-5, CFL 0.4), FP64 and mixed, with the roofline of its dominant kernel (the face
+kernel, one launch per axis: per face it reads one cell's 8 primitives and the face B and writes 5
+fluxes + 2 EMFs, 128 B in FP64 and 68 B with FP32 primitives and fluxes)
 and the CPU restatement's rate on a 64^3 sample (the reference has no ideal-gas EOS, so the CPU
 leg is the oracle's port of its step with that EOS).  --workload selects one of the two problems.
 """
@@ -140,7 +141,7 @@ def run_gpu3d(N, precision, warmup, steps):
     t0 = time.perf_counter()
     t = h.run(steps, t)
     elapsed = time.perf_counter() - t0
-    prof = {k: h.kernel_time(k) for k in ("flux", "edge", "ct", "joule", "dt")}
+    prof = {k: h.kernel_time(k) for k in ("prim", "face", "update", "edge", "ct", "joule", "dt")}
     h.set_profiling(False)
     out = h.download()
     divb = h.divB_max()
@@ -174,22 +175,23 @@ def bench3d(N, warmup, steps, with_cpu):
     res = {prec: run_gpu3d(N, prec, warmup, steps) for prec in (CC_FP64, CC_MIXED)}
     zu = float(N) ** 3 * steps
 
-    def roof(r):
-        n, ms = r["prof"]["flux"]
+    def roof(r, word):
+        n, ms = r["prof"]["face"]
         if n == 0 or ms <= 0:
             return None
-        bpl = float(N) ** 3 * 152.0
+        # per face: read the 8 primitives of one cell and the face B (FP64), write 5 fluxes + 2 EMFs
+        bpl = float(N) ** 3 * (8 * word + 8 + 7 * word)
         avg = ms / n * 1e-3
         return {"bound": "hbm", "achieved": round(bpl / avg / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(bpl / avg / 1e9 / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_flux3d",
+                "frac": round(bpl / avg / 1e9 / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_face3d",
                 "launches": n, "avg_launch_us": round(avg * 1e6, 2), "alg_bytes_per_launch": bpl}
 
     a, b = res[CC_MIXED]["rho"], res[CC_FP64]["rho"]
-    r64, rmx = roof(res[CC_FP64]), roof(res[CC_MIXED])
+    r64, rmx = roof(res[CC_FP64], 8), roof(res[CC_MIXED], 4)
     if r64:
-        r64["traffic"] = load_traffic("cc::k_flux3d<false")
+        r64["traffic"] = load_traffic("cc::k_face3d<0, false")
     if rmx:
-        rmx["traffic"] = load_traffic("cc::k_flux3d<true")
+        rmx["traffic"] = load_traffic("cc::k_face3d<0, true")
     out = {
         "workload": "3D resistive MHD HLLD+CT, periodic turbulence %d^3, ideal gas gamma 5/3, beta 1 along z, "
                     "rms Mach 1, eta 5e-5, CFL 0.4" % N,

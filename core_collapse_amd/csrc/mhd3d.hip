@@ -2,20 +2,21 @@
 // (restating mhd3d_resistive_hlld_ct.cpp, cited as M3:<line>), with the reference's hybrid
 // cold + thermal + bounce EOS or an ideal gas.
 //
-// One forward-Euler stage (advance_stage, M3:694-1030) is four kernels:
-//   k_flux3d    2.5D sweep: a 32x8 column of cells walks up z; each z-plane's primitives are
-//               recovered once (FP64) into LDS with a 2-cell halo, the x- and y-face fluxes of the
-//               plane come from LDS, the z-face fluxes from a 4-deep register queue of the
-//               column's primitives; flux difference + floors (old faces)      -> U*, 6 face EMFs
+// One forward-Euler stage (advance_stage, M3:694-1030) is:
+//   k_prim3d    primitives of every cell (FP64 recovery), stored in the flux precision
+//   k_face3d    x, y and z faces, one face per thread: PLM-MC + HLLD -> 5 face fluxes, 2 face EMFs
+//   k_update3d  flux difference + floors (old faces), FP64                      -> U*
 //   k_edge3d    Ex, Ey, Ez edges = 1/4 sum of face EMFs + eta * curl of cell-centred B0
 //   k_ct3d      face-field induction on the three face arrays                  -> B1
 //   k_joule3d   Joule heating from cell-centred B1 differences + floors; the stage-2 instance also
 //               forms the Heun average, its floors and the next step's signal-speed maximum.
-// Mixed precision: primitives and PLM in FP64, the HLLD in FP32, recomputed in FP64 for faces near
-// the Alfven degeneracy (where the reference's star states are ill-conditioned); flux differences,
-// the conserved update, EMF averages and CT in FP64.
+// Mixed precision: primitives recovered in FP64 and stored in FP32; PLM and HLLD in FP32 (HLLD
+// recomputed in FP64 for faces near the Alfven degeneracy, where the reference's star states are
+// ill-conditioned); face fluxes and EMFs stored in FP32; flux differences, the conserved update,
+// EMF averages and CT in FP64.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "profiler.hpp"
@@ -40,10 +41,6 @@ struct Eos3D {
 struct State3D {
     double *rho, *mx, *my, *mz, *E;  // Nx*Ny*Nz, (k*Ny+j)*Nx+i
     double *Bx, *By, *Bz;            // (Nx+1)Ny Nz, Nx(Ny+1)Nz, Nx Ny(Nz+1)
-};
-
-struct EmfFaces {
-    double *Ey_x, *Ez_x, *Ez_y, *Ex_y, *Ex_z, *Ey_z;
 };
 
 struct Edges {
@@ -164,165 +161,173 @@ __device__ __forceinline__ double signal_speed3d(const Eos3D& e, const Prim<doub
     return smax(smax(fabs(W.u) + fs(W.Bx), fabs(W.v) + fs(W.By)), fabs(W.w) + fs(W.Bz));
 }
 
-// ------------------------------------------------------------------ flux kernel (2.5D)
-constexpr int TX3 = 32, TY3 = 8, KZ3 = 16, RX3 = TX3 + 4, RY3 = TY3 + 4;
+// ------------------------------------------------------------------ flux path: primitives, faces, update
+// Primitives of every cell (FP64 recovery, M3:298-308), stored in the flux precision R as 8 SoA arrays.
+template <typename R>
+struct PrimArrays {
+    R* c[8];  // rho, u, v, w, P, Bx, By, Bz
+};
+// Face fluxes of one axis in the reference's component order (rho, mx, my, mz, E).
+template <typename R>
+struct FaceFlux {
+    R* f[5];
+};
+template <typename R>
+struct EmfArr {
+    R *Ey_x, *Ez_x, *Ez_y, *Ex_y, *Ex_z, *Ey_z;
+};
 
+template <typename R>
+__global__ __launch_bounds__(256) void k_prim3d(Geom3D g, Eos3D e, State3D U0, PrimArrays<R> W) {
+    const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= (size_t)g.Nx * g.Ny * g.Nz) return;
+    const int i = (int)(q % g.Nx);
+    const size_t r = q / g.Nx;
+    const int j = (int)(r % g.Ny), k = (int)(r / g.Ny);
+    const Prim<double> P = cons_to_prim3d(e, U0.rho[q], U0.mx[q], U0.my[q], U0.mz[q], U0.E[q], bxc(g, U0.Bx, i, j, k),
+                                          byc(g, U0.By, i, j, k), bzc(g, U0.Bz, i, j, k));
+    W.c[0][q] = (R)P.rho; W.c[1][q] = (R)P.u; W.c[2][q] = (R)P.v; W.c[3][q] = (R)P.w;
+    W.c[4][q] = (R)P.P; W.c[5][q] = (R)P.Bx; W.c[6][q] = (R)P.By; W.c[7][q] = (R)P.Bz;
+}
+
+template <typename R>
+__device__ __forceinline__ Prim<R> load_prim(const PrimArrays<R>& W, size_t q) {
+    return Prim<R>{W.c[0][q], W.c[1][q], W.c[2][q], W.c[3][q], W.c[4][q], W.c[5][q], W.c[6][q], W.c[7][q]};
+}
+
+// Mixed precision on FP32 face states: FP32 HLLD, recomputed in FP64 on the same states when the face
+// is within tau of the Alfven degeneracy.
+template <typename EOS32, typename EOS64>
+__device__ __forceinline__ void hlld_x_mixed32(const EOS32& e32, const EOS64& e64, const Prim<float>& L,
+                                               const Prim<float>& R, double Bn, float out[7], float tau) {
+    const bool bn_small = fabs(Bn) < 1e-12;
+    bool deg = false;
+    hlld_x<float>(e32, L, R, (float)Bn, bn_small, out, &deg, tau);
+    if (deg) {
+        const Prim<double> l{L.rho, L.u, L.v, L.w, L.P, L.Bx, L.By, L.Bz};
+        const Prim<double> r{R.rho, R.u, R.v, R.w, R.P, R.Bx, R.By, R.Bz};
+        double o[7];
+        hlld_x<double>(e64, l, r, Bn, bn_small, o);
+#pragma unroll
+        for (int c = 0; c < 7; ++c) out[c] = (float)o[c];
+    }
+}
+
+// The face solver of the flux path on FP64 face states (for the batched entry point): FP64 HLLD, or
+// the mixed path's FP32 states and FP32 HLLD with its FP64 fallback.
 template <bool MIXED, typename EOS64, typename EOS32>
 __device__ __forceinline__ void face_hlld3d(const EOS64& e64, const EOS32& e32, const Prim<double>& WL,
                                             const Prim<double>& WR, double Bn, double F[7], float tau) {
-    if (MIXED)
-        hlld_x_mixed(e32, e64, WL, WR, Bn, F, tau);
-    else
+    if constexpr (MIXED) {
+        const Prim<float> l{(float)WL.rho, (float)WL.u, (float)WL.v, (float)WL.w, (float)WL.P, (float)WL.Bx, (float)WL.By, (float)WL.Bz};
+        const Prim<float> r{(float)WR.rho, (float)WR.u, (float)WR.v, (float)WR.w, (float)WR.P, (float)WR.Bx, (float)WR.By, (float)WR.Bz};
+        float o[7];
+        hlld_x_mixed32(e32, e64, l, r, Bn, o, tau);
+#pragma unroll
+        for (int c = 0; c < 7; ++c) F[c] = o[c];
+    } else {
         hlld_x<double>(e64, WL, WR, Bn, fabs(Bn) < 1e-12, F);
+    }
 }
 
-template <bool MIXED, typename EOS64, typename EOS32>
-__global__ __launch_bounds__(256) void k_flux3d(Geom3D g, Eos3D e, EOS64 eos64, EOS32 eos32, State3D U0,
-                                                const Scalars* __restrict__ sc, State3D Us, EmfFaces ef) {
-    __shared__ double sW[2][8][RY3][RX3];
-    __shared__ double sFx[5][TY3][TX3 + 1];
-    __shared__ double sFy[5][TY3 + 1][TX3];
-    const int tid = threadIdx.x;
-    const int tx = tid % TX3, ty = tid / TX3;
-    const int i0 = blockIdx.x * TX3, j0 = blockIdx.y * TY3, k0 = blockIdx.z * KZ3;
-    const int k1 = min(k0 + KZ3, g.Nz);
+// One face of axis AX per thread (x: M3:727-750, y: M3:752-775 via the (u,v)/(Bx,By) swap of
+// M3:577-595, z: M3:777-800 via the (u,v,w)->(w,u,v) map of M3:604-622).  PLM-MC on the four cells
+// f-2 .. f+1 along the axis (boundary-mapped), HLLD, then the five fluxes in reference order and the
+// two face EMFs (Ey_x = F(Bz), Ez_x = -F(By); Ez_y = G(Bx), Ex_y = -G(Bz); Ex_z = H(By), Ey_z = -H(Bx)).
+template <int AX, bool MIXED, typename EOS64, typename EOS32>
+__global__ __launch_bounds__(256) void k_face3d(Geom3D g, EOS64 e64, EOS32 e32,
+                                                PrimArrays<typename std::conditional<MIXED, float, double>::type> W,
+                                                const double* __restrict__ Bface,
+                                                FaceFlux<typename std::conditional<MIXED, float, double>::type> F,
+                                                EmfArr<typename std::conditional<MIXED, float, double>::type> ef) {
+    using R = typename std::conditional<MIXED, float, double>::type;
     const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
-    const int i = i0 + tx, j = j0 + ty;
-    const bool in_tile = tid < TX3 * TY3;
-    const bool col = in_tile && i < Nx && j < Ny;
-    const double frho = g.floor_rho, fP = g.floor_P;
-    const double dt = sc->dt, invdx = 1.0 / g.dx, invdy = 1.0 / g.dy, invdz = 1.0 / g.dz;
-
-    auto fill_plane = [&](int buf, int k) {  // primitives of logical plane k with a 2-cell x/y halo
-        for (int c = tid; c < RX3 * RY3; c += 256) {
-            const int lx = c % RX3, ly = c / RX3;
-            const Prim<double> W = prim_at(g, e, U0, i0 - 2 + lx, j0 - 2 + ly, k);
-            sW[buf][0][ly][lx] = W.rho; sW[buf][1][ly][lx] = W.u; sW[buf][2][ly][lx] = W.v;
-            sW[buf][3][ly][lx] = W.w; sW[buf][4][ly][lx] = W.P; sW[buf][5][ly][lx] = W.Bx;
-            sW[buf][6][ly][lx] = W.By; sW[buf][7][ly][lx] = W.Bz;
-        }
+    const int nx = Nx + (AX == 0), ny = Ny + (AX == 1);
+    const size_t nf = (size_t)nx * ny * (Nz + (AX == 2));
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= nf) return;
+    const int i = (int)(t % nx);
+    const size_t r = t / nx;
+    const int j = (int)(r % ny), k = (int)(r / ny);
+    const int f = AX == 0 ? i : (AX == 1 ? j : k);
+    auto cell = [&](int o) {  // index of the cell o steps from the face along the axis
+        int ii = i, jj = j, kk = k;
+        if (AX == 0) ii = bmap(f + o, Nx, g.px);
+        if (AX == 1) jj = bmap(f + o, Ny, g.py);
+        if (AX == 2) kk = bmap(f + o, Nz, g.pz);
+        return ((size_t)kk * Ny + jj) * Nx + ii;
     };
-    auto lds = [&](int buf, int ly, int lx) {
-        return Prim<double>{sW[buf][0][ly][lx], sW[buf][1][ly][lx], sW[buf][2][ly][lx], sW[buf][3][ly][lx],
-                            sW[buf][4][ly][lx], sW[buf][5][ly][lx], sW[buf][6][ly][lx], sW[buf][7][ly][lx]};
-    };
-
-    // z register queue of this column: planes k-2, k-1, k, k+1
-    Prim<double> q0{}, q1{}, q2{}, q3{};
-    if (in_tile) {
-        q0 = prim_at(g, e, U0, i, j, k0 - 2);
-        q1 = prim_at(g, e, U0, i, j, k0 - 1);
+    const Prim<R> w0 = load_prim(W, cell(-2)), w1 = load_prim(W, cell(-1)), w2 = load_prim(W, cell(0)),
+                  w3 = load_prim(W, cell(1));
+    const R frho = (R)g.floor_rho, fP = (R)g.floor_P;
+    Prim<R> a, b, WL, WR;
+    plm_mc<R>(w0, w1, w2, AX, frho, fP, a, WL);  // right state of cell f-1
+    plm_mc<R>(w1, w2, w3, AX, frho, fP, WR, b);  // left state of cell f
+    const double Bn = Bface[t];
+    Prim<R> L = WL, Rr = WR;
+    if (AX == 1) {
+        L = Prim<R>{WL.rho, WL.v, WL.u, WL.w, WL.P, (R)Bn, WL.Bx, WL.Bz};
+        Rr = Prim<R>{WR.rho, WR.v, WR.u, WR.w, WR.P, (R)Bn, WR.Bx, WR.Bz};
+    } else if (AX == 2) {
+        L = Prim<R>{WL.rho, WL.w, WL.u, WL.v, WL.P, (R)Bn, WL.Bx, WL.By};
+        Rr = Prim<R>{WR.rho, WR.w, WR.u, WR.v, WR.P, (R)Bn, WR.Bx, WR.By};
     }
-    fill_plane(0, k0);
-    __syncthreads();
-    if (in_tile) q2 = lds(0, ty + 2, tx + 2);
-    double Hprev[5] = {0, 0, 0, 0, 0};
-    double dpl[5] = {0, 0, 0, 0, 0};  // in-plane flux difference of the previous plane's cell
-
-    for (int k = k0; k <= k1; ++k) {
-        const int cur = (k - k0) & 1;
-        if (k + 1 < k1) fill_plane(cur ^ 1, k + 1);
-        __syncthreads();
-        if (in_tile) q3 = (k + 1 < k1) ? lds(cur ^ 1, ty + 2, tx + 2) : prim_at(g, e, U0, i, j, k + 1);
-
-        // z-face k between cells k-1 and k (M3:777-800), by the (u,v,w)->(w,u,v) map of M3:604-622
-        if (col) {
-            Prim<double> a, b, WL, WR;
-            plm_mc<double>(q0, q1, q2, 2, frho, fP, a, WL);
-            plm_mc<double>(q1, q2, q3, 2, frho, fP, WR, b);
-            const double Bn = U0.Bz[fz_(g, i, j, k)];
-            const Prim<double> aL{WL.rho, WL.w, WL.u, WL.v, WL.P, Bn, WL.Bx, WL.By};
-            const Prim<double> aR{WR.rho, WR.w, WR.u, WR.v, WR.P, Bn, WR.Bx, WR.By};
-            double F[7];
-            face_hlld3d<MIXED>(eos64, eos32, aL, aR, Bn, F, g.tau);
-            const double H[5] = {F[0], F[2], F[3], F[1], F[4]};
-            if (k < k1 || k == Nz) {
-                ef.Ex_z[fz_(g, i, j, k)] = F[6];
-                ef.Ey_z[fz_(g, i, j, k)] = -F[5];
-            }
-            if (k > k0) {  // complete cell k-1 (M3:802-834)
-                const int kc = k - 1;
-                const size_t q = ((size_t)kc * Ny + j) * Nx + i;
-                double d[5];
-#pragma unroll
-                for (int c = 0; c < 5; c++) d[c] = dpl[c] - (H[c] - Hprev[c]) * invdz;
-                double rho = U0.rho[q] + dt * d[0], mx = U0.mx[q] + dt * d[1], my = U0.my[q] + dt * d[2];
-                double mz = U0.mz[q] + dt * d[3], E = U0.E[q] + dt * d[4];
-                floors3d(e, rho, mx, my, mz, E, bxc(g, U0.Bx, i, j, kc), byc(g, U0.By, i, j, kc),
-                         bzc(g, U0.Bz, i, j, kc));
-                Us.rho[q] = rho; Us.mx[q] = mx; Us.my[q] = my; Us.mz[q] = mz; Us.E[q] = E;
-            }
-#pragma unroll
-            for (int c = 0; c < 5; c++) Hprev[c] = H[c];
-        }
-
-        if (k < k1) {
-            // x-faces of plane k (M3:727-750)
-            for (int f = tid; f < (TX3 + 1) * TY3; f += 256) {
-                const int fx = f % (TX3 + 1), fy = f / (TX3 + 1);
-                const int fi = i0 + fx, fj = j0 + fy;
-                if (fi > Nx || fj >= Ny) continue;
-                const int ly = fy + 2;
-                Prim<double> a, b, WL, WR;
-                plm_mc<double>(lds(cur, ly, fx), lds(cur, ly, fx + 1), lds(cur, ly, fx + 2), 0, frho, fP, a, WL);
-                plm_mc<double>(lds(cur, ly, fx + 1), lds(cur, ly, fx + 2), lds(cur, ly, fx + 3), 0, frho, fP, WR, b);
-                double F[7];
-                face_hlld3d<MIXED>(eos64, eos32, WL, WR, U0.Bx[fx_(g, fi, fj, k)], F, g.tau);
-#pragma unroll
-                for (int c = 0; c < 5; c++) sFx[c][fy][fx] = F[c];
-                if (fx < TX3 || fi == Nx) {
-                    ef.Ey_x[fx_(g, fi, fj, k)] = F[6];
-                    ef.Ez_x[fx_(g, fi, fj, k)] = -F[5];
-                }
-            }
-            // y-faces of plane k (M3:752-775), by the (u,v)/(Bx,By) swap of M3:577-595
-            for (int f = tid; f < TX3 * (TY3 + 1); f += 256) {
-                const int fx = f % TX3, fy = f / TX3;
-                const int fi = i0 + fx, fj = j0 + fy;
-                if (fi >= Nx || fj > Ny) continue;
-                const int lx = fx + 2;
-                Prim<double> a, b, WL, WR;
-                plm_mc<double>(lds(cur, fy, lx), lds(cur, fy + 1, lx), lds(cur, fy + 2, lx), 1, frho, fP, a, WL);
-                plm_mc<double>(lds(cur, fy + 1, lx), lds(cur, fy + 2, lx), lds(cur, fy + 3, lx), 1, frho, fP, WR, b);
-                const double Bn = U0.By[fy_(g, fi, fj, k)];
-                const Prim<double> aL{WL.rho, WL.v, WL.u, WL.w, WL.P, Bn, WL.Bx, WL.Bz};
-                const Prim<double> aR{WR.rho, WR.v, WR.u, WR.w, WR.P, Bn, WR.Bx, WR.Bz};
-                double F[7];
-                face_hlld3d<MIXED>(eos64, eos32, aL, aR, Bn, F, g.tau);
-                sFy[0][fy][fx] = F[0]; sFy[1][fy][fx] = F[2]; sFy[2][fy][fx] = F[1];
-                sFy[3][fy][fx] = F[3]; sFy[4][fy][fx] = F[4];
-                if (fy < TY3 || fj == Ny) {
-                    ef.Ez_y[fy_(g, fi, fj, k)] = F[5];
-                    ef.Ex_y[fy_(g, fi, fj, k)] = -F[6];
-                }
-            }
-            __syncthreads();
-            if (in_tile) {
-#pragma unroll
-                for (int c = 0; c < 5; c++)
-                    dpl[c] = -(sFx[c][ty][tx + 1] - sFx[c][ty][tx]) * invdx -
-                             (sFy[c][ty + 1][tx] - sFy[c][ty][tx]) * invdy;
-            }
-        }
-        __syncthreads();
-        q0 = q1;
-        q1 = q2;
-        q2 = q3;
+    R o[7];
+    if constexpr (MIXED)
+        hlld_x_mixed32(e32, e64, L, Rr, Bn, o, g.tau);
+    else
+        hlld_x<double>(e64, L, Rr, Bn, fabs(Bn) < 1e-12, o);
+    if (AX == 0) {
+        F.f[0][t] = o[0]; F.f[1][t] = o[1]; F.f[2][t] = o[2]; F.f[3][t] = o[3]; F.f[4][t] = o[4];
+        ef.Ey_x[t] = o[6];
+        ef.Ez_x[t] = -o[5];
+    } else if (AX == 1) {
+        F.f[0][t] = o[0]; F.f[1][t] = o[2]; F.f[2][t] = o[1]; F.f[3][t] = o[3]; F.f[4][t] = o[4];
+        ef.Ez_y[t] = o[5];
+        ef.Ex_y[t] = -o[6];
+    } else {
+        F.f[0][t] = o[0]; F.f[1][t] = o[2]; F.f[2][t] = o[3]; F.f[3][t] = o[1]; F.f[4][t] = o[4];
+        ef.Ex_z[t] = o[6];
+        ef.Ey_z[t] = -o[5];
     }
+}
+
+// Flux difference, conserved update and floors with the old faces (M3:802-834), in FP64.
+template <typename R>
+__global__ __launch_bounds__(256) void k_update3d(Geom3D g, Eos3D e, State3D U0, FaceFlux<R> FX, FaceFlux<R> FY,
+                                                  FaceFlux<R> FZ, const Scalars* __restrict__ sc, State3D Us) {
+    const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= (size_t)g.Nx * g.Ny * g.Nz) return;
+    const int i = (int)(q % g.Nx);
+    const size_t r = q / g.Nx;
+    const int j = (int)(r % g.Ny), k = (int)(r / g.Ny);
+    const size_t xL = fx_(g, i, j, k), yD = fy_(g, i, j, k), zB = fz_(g, i, j, k);
+    const size_t xR = xL + 1, yU = yD + g.Nx, zF = zB + (size_t)g.Nx * g.Ny;
+    const double invdx = 1.0 / g.dx, invdy = 1.0 / g.dy, invdz = 1.0 / g.dz, dt = sc->dt;
+    double d[5];
+#pragma unroll
+    for (int c = 0; c < 5; c++)
+        d[c] = -((double)FX.f[c][xR] - (double)FX.f[c][xL]) * invdx - ((double)FY.f[c][yU] - (double)FY.f[c][yD]) * invdy -
+               ((double)FZ.f[c][zF] - (double)FZ.f[c][zB]) * invdz;
+    double rho = U0.rho[q] + dt * d[0], mx = U0.mx[q] + dt * d[1], my = U0.my[q] + dt * d[2];
+    double mz = U0.mz[q] + dt * d[3], E = U0.E[q] + dt * d[4];
+    floors3d(e, rho, mx, my, mz, E, bxc(g, U0.Bx, i, j, k), byc(g, U0.By, i, j, k), bzc(g, U0.Bz, i, j, k));
+    Us.rho[q] = rho; Us.mx[q] = mx; Us.my[q] = my; Us.mz[q] = mz; Us.E[q] = E;
 }
 
 // ------------------------------------------------------------------ edge EMFs (M3:836-939)
-__global__ void k_edge3d(Geom3D g, EmfFaces ef, State3D B0, Edges ed) {
+template <typename R>
+__global__ void k_edge3d(Geom3D g, EmfArr<R> ef, State3D B0, Edges ed) {
     const int ia = blockIdx.x * blockDim.x + threadIdx.x, ja = blockIdx.y, ka = blockIdx.z;
     const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
     if (ia > Nx) return;
     const bool px = g.px, py = g.py, pz = g.pz;
-    auto exy = [&](int i, int j, int k) { return ef.Ex_y[fy_(g, bmap(i, Nx, px), fmap(j, Ny, py), bmap(k, Nz, pz))]; };
-    auto ezx = [&](int i, int j, int k) { return ef.Ez_x[fx_(g, fmap(i, Nx, px), bmap(j, Ny, py), bmap(k, Nz, pz))]; };
-    auto eyx = [&](int i, int j, int k) { return ef.Ey_x[fx_(g, fmap(i, Nx, px), bmap(j, Ny, py), bmap(k, Nz, pz))]; };
-    auto exz = [&](int i, int j, int k) { return ef.Ex_z[fz_(g, bmap(i, Nx, px), bmap(j, Ny, py), fmap(k, Nz, pz))]; };
-    auto eyz = [&](int i, int j, int k) { return ef.Ey_z[fz_(g, bmap(i, Nx, px), bmap(j, Ny, py), fmap(k, Nz, pz))]; };
-    auto ezy = [&](int i, int j, int k) { return ef.Ez_y[fy_(g, bmap(i, Nx, px), fmap(j, Ny, py), bmap(k, Nz, pz))]; };
+    auto exy = [&](int i, int j, int k) { return (double)ef.Ex_y[fy_(g, bmap(i, Nx, px), fmap(j, Ny, py), bmap(k, Nz, pz))]; };
+    auto ezx = [&](int i, int j, int k) { return (double)ef.Ez_x[fx_(g, fmap(i, Nx, px), bmap(j, Ny, py), bmap(k, Nz, pz))]; };
+    auto eyx = [&](int i, int j, int k) { return (double)ef.Ey_x[fx_(g, fmap(i, Nx, px), bmap(j, Ny, py), bmap(k, Nz, pz))]; };
+    auto exz = [&](int i, int j, int k) { return (double)ef.Ex_z[fz_(g, bmap(i, Nx, px), bmap(j, Ny, py), fmap(k, Nz, pz))]; };
+    auto eyz = [&](int i, int j, int k) { return (double)ef.Ey_z[fz_(g, bmap(i, Nx, px), bmap(j, Ny, py), fmap(k, Nz, pz))]; };
+    auto ezy = [&](int i, int j, int k) { return (double)ef.Ez_y[fy_(g, bmap(i, Nx, px), fmap(j, Ny, py), bmap(k, Nz, pz))]; };
     const double eta = g.eta;
     if (ia < Nx) {  // Ex(i, jf, kf)
         const int i = ia, jf = ja, kf = ka;
@@ -512,7 +517,9 @@ struct cc_mhd3d {
     int cur = 0;
     State3D Us{};   // cells after the flux update
     State3D Bt{};   // stage-2 face field before averaging
-    EmfFaces ef{};
+    void* prim[8] = {};     // primitives, flux precision
+    void* flx[3][5] = {};   // face fluxes per axis, flux precision
+    void* emf[6] = {};      // face EMFs Ey_x, Ez_x, Ez_y, Ex_y, Ex_z, Ey_z, flux precision
     Edges ed{};
     Scalars* sc = nullptr;
     double* dts_dev = nullptr;
@@ -548,16 +555,80 @@ struct cc_mhd3d {
         return CC_OK;
     }
 
+    cc_status alloc_bytes(void** ptr, size_t bytes) {
+        if (hipMalloc(ptr, bytes) != hipSuccess) {
+            set_error(CC_ERR_OOM, "cc_mhd3d: device allocation failed");
+            return CC_ERR_OOM;
+        }
+        allocs.push_back(*ptr);
+        CC_HIP(hipMemsetAsync(*ptr, 0, bytes, stream));
+        return CC_OK;
+    }
+    cc_status alloc_flux_path() {
+        const size_t s = p.precision == CC_MIXED ? sizeof(float) : sizeof(double);
+        const size_t nf[3] = {NFx(), NFy(), NFz()};
+        for (auto& a : prim) CC_TRY(alloc_bytes(&a, N() * s));
+        for (int ax = 0; ax < 3; ++ax)
+            for (auto& a : flx[ax]) CC_TRY(alloc_bytes(&a, nf[ax] * s));
+        const size_t ne[6] = {NFx(), NFx(), NFy(), NFy(), NFz(), NFz()};
+        for (int c = 0; c < 6; ++c) CC_TRY(alloc_bytes(&emf[c], ne[c] * s));
+        return CC_OK;
+    }
+    template <typename R>
+    PrimArrays<R> W() const {
+        PrimArrays<R> w;
+        for (int c = 0; c < 8; ++c) w.c[c] = static_cast<R*>(prim[c]);
+        return w;
+    }
+    template <typename R>
+    FaceFlux<R> FF(int ax) const {
+        FaceFlux<R> f;
+        for (int c = 0; c < 5; ++c) f.f[c] = static_cast<R*>(flx[ax][c]);
+        return f;
+    }
+    template <typename R>
+    EmfArr<R> EF() const {
+        return EmfArr<R>{static_cast<R*>(emf[0]), static_cast<R*>(emf[1]), static_cast<R*>(emf[2]),
+                         static_cast<R*>(emf[3]), static_cast<R*>(emf[4]), static_cast<R*>(emf[5])};
+    }
+    static unsigned blocks(size_t n) { return (unsigned)((n + 255) / 256); }
+
     template <bool MIXED, typename EOS64, typename EOS32>
     cc_status launch_flux(const EOS64& e64, const EOS32& e32, const State3D& in) {
-        dim3 grid((p.Nx + TX3 - 1) / TX3, (p.Ny + TY3 - 1) / TY3, (p.Nz + KZ3 - 1) / KZ3);
-        hipLaunchKernelGGL((k_flux3d<MIXED, EOS64, EOS32>), grid, dim3(256), 0, stream, g, e, e64, e32, in, sc, Us, ef);
-        CC_CHECK_LAUNCH();
+        using R = typename std::conditional<MIXED, float, double>::type;
+        {
+            ProfScope ps(pr(), stream, "prim");
+            hipLaunchKernelGGL(k_prim3d<R>, dim3(blocks(N())), dim3(256), 0, stream, g, e, in, W<R>());
+            CC_CHECK_LAUNCH();
+        }
+        {
+            ProfScope ps(pr(), stream, "face");
+            hipLaunchKernelGGL((k_face3d<0, MIXED, EOS64, EOS32>), dim3(blocks(NFx())), dim3(256), 0, stream, g, e64, e32,
+                               W<R>(), in.Bx, FF<R>(0), EF<R>());
+            CC_CHECK_LAUNCH();
+        }
+        {
+            ProfScope ps(pr(), stream, "face");
+            hipLaunchKernelGGL((k_face3d<1, MIXED, EOS64, EOS32>), dim3(blocks(NFy())), dim3(256), 0, stream, g, e64, e32,
+                               W<R>(), in.By, FF<R>(1), EF<R>());
+            CC_CHECK_LAUNCH();
+        }
+        {
+            ProfScope ps(pr(), stream, "face");
+            hipLaunchKernelGGL((k_face3d<2, MIXED, EOS64, EOS32>), dim3(blocks(NFz())), dim3(256), 0, stream, g, e64, e32,
+                               W<R>(), in.Bz, FF<R>(2), EF<R>());
+            CC_CHECK_LAUNCH();
+        }
+        {
+            ProfScope ps(pr(), stream, "update");
+            hipLaunchKernelGGL(k_update3d<R>, dim3(blocks(N())), dim3(256), 0, stream, g, e, in, FF<R>(0), FF<R>(1),
+                               FF<R>(2), sc, Us);
+            CC_CHECK_LAUNCH();
+        }
         return CC_OK;
     }
 
     cc_status flux(const State3D& in) {
-        ProfScope ps(pr(), stream, "flux");
         const bool mixed = p.precision == CC_MIXED;
         if (e.ideal) {
             const IdealEOS3D<double> e64{p.gamma - 1.0, p.gamma, p.floor_rho};
@@ -575,7 +646,12 @@ struct cc_mhd3d {
         CC_TRY(flux(in));
         {
             ProfScope ps(pr(), stream, "edge");
-            hipLaunchKernelGGL(k_edge3d, dim3((Nx + 1 + 63) / 64, Ny + 1, Nz + 1), dim3(64), 0, stream, g, ef, in, ed);
+            if (p.precision == CC_MIXED)
+                hipLaunchKernelGGL(k_edge3d<float>, dim3((Nx + 1 + 63) / 64, Ny + 1, Nz + 1), dim3(64), 0, stream, g,
+                                   EF<float>(), in, ed);
+            else
+                hipLaunchKernelGGL(k_edge3d<double>, dim3((Nx + 1 + 63) / 64, Ny + 1, Nz + 1), dim3(64), 0, stream, g,
+                                   EF<double>(), in, ed);
             CC_CHECK_LAUNCH();
         }
         const State3D& bdst = FINAL ? Bt : out;
@@ -656,9 +732,7 @@ cc_status cc_mhd3d_create(const cc_mhd3d_params* p, cc_mhd3d** out) {
     }
     if (st == CC_OK) st = h->alloc_cells(h->Us);
     if (st == CC_OK) st = h->alloc_faces(h->Bt);
-    double** ep[6] = {&h->ef.Ey_x, &h->ef.Ez_x, &h->ef.Ez_y, &h->ef.Ex_y, &h->ef.Ex_z, &h->ef.Ey_z};
-    size_t en[6] = {h->NFx(), h->NFx(), h->NFy(), h->NFy(), h->NFz(), h->NFz()};
-    for (int k = 0; k < 6 && st == CC_OK; k++) st = h->alloc(ep[k], en[k]);
+    if (st == CC_OK) st = h->alloc_flux_path();
     if (st == CC_OK) st = h->alloc(&h->ed.Ex, (size_t)p->Nx * (p->Ny + 1) * (p->Nz + 1));
     if (st == CC_OK) st = h->alloc(&h->ed.Ey, (size_t)(p->Nx + 1) * p->Ny * (p->Nz + 1));
     if (st == CC_OK) st = h->alloc(&h->ed.Ez, (size_t)(p->Nx + 1) * (p->Ny + 1) * p->Nz);

@@ -169,7 +169,7 @@ cc_status cc_mhd3d_run(cc_mhd3d* h, int nsteps, double* t_inout, double t_end, d
 /* max |divB_cc| (M3:1070-1078).  Syncs. */
 cc_status cc_mhd3d_divb_max(cc_mhd3d* h, double* out);
 cc_status cc_mhd3d_sync(cc_mhd3d* h);
-/* Per-kernel HIP-event timing ("flux", "edge", "ct", "joule", "dt"). */
+/* Per-kernel HIP-event timing ("prim", "face", "update", "edge", "ct", "joule", "dt"). */
 cc_status cc_mhd3d_set_profiling(cc_mhd3d* h, int on);
 cc_status cc_mhd3d_kernel_time(cc_mhd3d* h, const char* kernel, int* count, double* total_ms);
 cc_status cc_mhd3d_reset_profiling(cc_mhd3d* h);

@@ -146,7 +146,7 @@ def test_turbulence_run_matches_oracle_and_divb(tilted):
 
 def test_mixed_vs_fp64_turbulence_tilted():
     """Mixed vs FP64 at 32^3 over 10 steps with a tilted field (no face near |Bn| = 1e-12 or the Alfven
-    degeneracy): relative L1 density error <= 1e-4 (measured ~2e-8).  With the field along z only,
+    degeneracy): relative L1 density error <= 1e-4.  With the field along z only,
     the reference scheme amplifies any 1e-7 perturbation (FP32 fluxes) to ~1e-3 within 10 steps --
     the FP64 GPU path against the FP64 oracle (FMA rounding only) already differs by ~1e-7 pointwise
     there -- so that IC is reported by the benchmark, not held to 1e-4 here."""
@@ -162,15 +162,28 @@ def test_mixed_vs_fp64_turbulence_tilted():
     assert out[CC_MIXED][1] < 1e-12 and out[CC_FP64][1] < 1e-12
 
 
-def test_mixed_fallback_tau_two_equals_fp64_riemann():
-    """degen_tau >= 1 sends every HLLD face through the FP64 fallback, so a single stage matches the
-    FP64 stage up to the FP32 supersonic/HLL faces; on the tilted state those are rare and the stage
-    agrees to 1e-12."""
-    res = {}
-    for prec, tau in ((CC_FP64, 0.0), (CC_MIXED, 2.0)):
-        p = ic.turbulence3d_params(24, precision=prec, degen_tau=tau)
-        with M.Mhd3D(p) as h:
-            h.upload(_turb_state(p, True))
-            h.advance_stage(1e-3)
-            res[prec] = h.download()
-    assert rel_l1(res[CC_MIXED]["rho"], res[CC_FP64]["rho"]) < 1e-9
+def test_mixed_fallback_tau_two_is_fp64_hlld_on_fp32_states():
+    """The mixed path carries FP32 face states and stores FP32 fluxes; with degen_tau >= 1 every
+    subsonic face takes the FP64 fallback, whose stored result must then be the FP64 HLLD of the
+    FP32-rounded states, rounded to FP32.  Supersonic and |Bn| < 1e-12 faces stay on the FP32 solver
+    (a third of these random faces), so the exact bar is on the median face and the rest are held to
+    FP32 accuracy."""
+    rng = np.random.default_rng(77)
+    n = 2000
+    WL, WR = _prims(rng, n, False), _prims(rng, n, False)
+    Bn = rng.normal(0, 1.0, n)
+    p = M.Params(eos=1, precision=CC_MIXED, degen_tau=2.0)
+    got = M.hlld_batch(p, WL, WR, Bn)
+    WLr, WRr = WL.astype(np.float32).astype(np.float64), WR.astype(np.float32).astype(np.float64)
+    o = oracle()
+    op = m3_oparams(M.Params(eos=1))
+    want = np.zeros((n, 7))
+    for k in range(n):
+        r = np.zeros(7)
+        o.or_m3_hlld_x(C.byref(op), OB.ptr(np.ascontiguousarray(WLr[k])), OB.ptr(np.ascontiguousarray(WRr[k])),
+                       C.c_double(Bn[k]), OB.ptr(r))
+        want[k] = r
+    want = want.astype(np.float32).astype(np.float64)
+    err = np.abs(got - want) / (np.abs(want).max(axis=1, keepdims=True) + 1e-12)
+    e = err.max(axis=1)
+    assert np.quantile(e, 0.5) < 1e-12 and e.max() < 1e-6, np.quantile(e, [0.5, 0.9, 1.0])
