@@ -59,6 +59,42 @@ class Ns2dParams(C.Structure):
                 ("precision", C.c_int)]
 
 
+class Mhd2dIc(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("rho_bg", "rho_core", "r_core", "r_edge", "P0", "vin", "rot_omega",
+                                           "B0x", "Bshear", "gamma_eff")]
+
+
+class Collapse2dIc(C.Structure):
+    _fields_ = [("seed", C.c_uint)] + [(n, C.c_double) for n in ("contrast", "width", "P0", "vamp", "B0")]
+
+
+class Mhd3dIc(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("rho_bg", "rho_core", "r_core", "r_edge", "P0", "vin", "rot_omega",
+                                           "B0x", "B0z", "Bshear", "gamma_init")]
+
+
+class Turbulence3dIc(C.Structure):
+    _fields_ = [("seed", C.c_uint), ("kmin", C.c_int), ("kmax", C.c_int), ("mach", C.c_double),
+                ("beta", C.c_double)]
+
+
+class Euler2dIc(C.Structure):
+    _fields_ = [("seed", C.c_uint)] + [(n, C.c_double) for n in ("rho0", "P0", "amp_rho", "amp_P", "vmax",
+                                                                  "sigma", "bump_amp")]
+
+
+class Ns2dIc(C.Structure):
+    _fields_ = [("seed", C.c_uint)] + [(n, C.c_double) for n in ("rho0", "P0", "A", "sigma_frac", "Arho",
+                                                                  "Mach0")]
+
+
+FRAME_FIELDS = {
+    "mhd2d": ("rho", "P", "Mach", "u", "v", "vabs", "Bx", "By", "Bz", "divB"),
+    "mhd3d": ("rho", "P", "Mach", "u", "v", "w", "vabs", "Bx", "By", "Bz", "divB"),
+    "euler2d": ("rho", "P", "Mach"),
+    "ns2d": ("rho", "P", "Mach", "u", "v", "momx", "momy", "Phi"),
+}
+
 # name -> (restype, argtypes); every function returns cc_status unless stated
 _PROTOS = {
     "cc_last_error": (C.c_char_p, []),
@@ -128,6 +164,30 @@ _PROTOS = {
     "cc_copy_calibration": (C.c_int, [C.c_size_t, C.c_int, DP]),
     "cc_poisson2d_sor": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                    C.c_int, C.c_double, C.c_int, DP, DP, DP, DP]),
+    "cc_mhd2d_frame": (C.c_int, [C.c_void_p, DP]),
+    "cc_mhd2d_totals": (C.c_int, [C.c_void_p, DP]),
+    "cc_mhd2d_compare": (C.c_int, [C.c_void_p, C.c_void_p, DP]),
+    "cc_mhd3d_frame": (C.c_int, [C.c_void_p, DP]),
+    "cc_mhd3d_totals": (C.c_int, [C.c_void_p, DP]),
+    "cc_mhd3d_compare": (C.c_int, [C.c_void_p, C.c_void_p, DP]),
+    "cc_euler2d_frame": (C.c_int, [C.c_void_p, DP]),
+    "cc_euler2d_totals": (C.c_int, [C.c_void_p, DP]),
+    "cc_euler2d_compare": (C.c_int, [C.c_void_p, C.c_void_p, DP]),
+    "cc_ns2d_frame": (C.c_int, [C.c_void_p, DP]),
+    "cc_ns2d_totals": (C.c_int, [C.c_void_p, DP]),
+    "cc_ns2d_compare": (C.c_int, [C.c_void_p, C.c_void_p, DP]),
+    "cc_mhd2d_ic_defaults": (None, [C.POINTER(Mhd2dIc)]),
+    "cc_mhd2d_ic_reference": (C.c_int, [C.POINTER(Mhd2dParams), C.POINTER(Mhd2dIc)] + [DP] * 8),
+    "cc_collapse2d_ic_defaults": (None, [C.POINTER(Collapse2dIc)]),
+    "cc_mhd2d_ic_collapse": (C.c_int, [C.POINTER(Mhd2dParams), C.POINTER(Collapse2dIc)] + [DP] * 8),
+    "cc_mhd3d_ic_defaults": (None, [C.POINTER(Mhd3dIc)]),
+    "cc_mhd3d_ic_reference": (C.c_int, [C.POINTER(Mhd3dParams), C.POINTER(Mhd3dIc)] + [DP] * 8),
+    "cc_turbulence3d_ic_defaults": (None, [C.POINTER(Turbulence3dIc)]),
+    "cc_mhd3d_ic_turbulence": (C.c_int, [C.POINTER(Mhd3dParams), C.POINTER(Turbulence3dIc)] + [DP] * 8),
+    "cc_euler2d_ic_defaults": (None, [C.POINTER(Euler2dIc)]),
+    "cc_euler2d_ic_reference": (C.c_int, [C.POINTER(Euler2dParams), C.POINTER(Euler2dIc)] + [DP] * 4),
+    "cc_ns2d_ic_defaults": (None, [C.POINTER(Ns2dIc)]),
+    "cc_ns2d_ic_reference": (C.c_int, [C.POINTER(Ns2dParams), C.POINTER(Ns2dIc), DP]),
 }
 
 _lib = None
@@ -163,3 +223,39 @@ def device_info():
     cu = C.c_int(0)
     check(lib().cc_device_info(name, 256, C.byref(cu)))
     return name.value.decode(), cu.value
+
+
+# ---------------------------------------------------------------------------------------- diagnostics
+TOTALS = {
+    "mhd2d": ("mass", "momx", "momy", "momz", "energy", "divB_max"),
+    "mhd3d": ("mass", "momx", "momy", "momz", "energy", "divB_max"),
+    "euler2d": ("mass", "momx", "momy", "energy"),
+    "ns2d": ("mass", "momx", "momy", "energy"),
+}
+
+
+def frame_of(kind, h, shape):
+    """The snapshot frame of handle h (cc_<kind>_frame) as {field: array of `shape`}."""
+    import numpy as np
+    names = FRAME_FIELDS[kind]
+    buf = np.empty((len(names),) + tuple(shape))
+    check(getattr(lib(), "cc_%s_frame" % kind)(h, buf.ctypes.data_as(DP)))
+    return {n: buf[i] for i, n in enumerate(names)}
+
+
+def totals_of(kind, h):
+    """In-situ totals of handle h (cc_<kind>_totals) as {name: value}."""
+    import numpy as np
+    names = TOTALS[kind]
+    out = np.empty(len(names))
+    check(getattr(lib(), "cc_%s_totals" % kind)(h, out.ctypes.data_as(DP)))
+    return dict(zip(names, out.tolist()))
+
+
+def compare_of(kind, ha, hb):
+    """Frame comparison of two handles (cc_<kind>_compare): {field: (mean|a-b|, sum|a-b|/sum|b|, max|a-b|)}."""
+    import numpy as np
+    names = FRAME_FIELDS[kind]
+    out = np.empty(3 * len(names))
+    check(getattr(lib(), "cc_%s_compare" % kind)(ha, hb, out.ctypes.data_as(DP)))
+    return {n: tuple(out[3 * i:3 * i + 3].tolist()) for i, n in enumerate(names)}

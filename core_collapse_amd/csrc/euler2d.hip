@@ -20,6 +20,7 @@
 #include <cstring>
 #include <vector>
 
+#include "diag.hpp"
 #include "hllc.hpp"
 #include "profiler.hpp"
 #include "riemann.hpp"
@@ -91,7 +92,9 @@ __device__ __forceinline__ void block_sum_max(double& s, double& m, double* red_
 }
 
 // ------------------------------------------------------------------ SOR to tolerance (EH:288-386)
-template <bool LDS>
+// LDS: phi lives in LDS (N doubles of dynamic LDS); RLDS: the source term 4 pi G (rho - rhobar) too
+// (2N doubles), so a diagonal step waits on LDS only, never on a global load.
+template <bool LDS, bool RLDS>
 __global__ __launch_bounds__(1024) void k_eh_sor(SorEH a, const double* __restrict__ rho, double* __restrict__ gphi,
                                                  double* __restrict__ gx, double* __restrict__ gy,
                                                  int* __restrict__ sweeps) {
@@ -100,6 +103,7 @@ __global__ __launch_bounds__(1024) void k_eh_sor(SorEH a, const double* __restri
     double* phi = LDS ? sphi : gphi;
     const int Nx = a.Nx, Ny = a.Ny, B = blockDim.x, tid = threadIdx.x;
     const int N = Nx * Ny;
+    double* rsrc = sphi + N;  // RLDS only
     // rhobar of the floored density (EH:306-315) and phi0 = 0 (EH:325)
     double s = 0.0, m = 0.0;
     for (int k = tid; k < N; k += B) {
@@ -108,6 +112,10 @@ __global__ __launch_bounds__(1024) void k_eh_sor(SorEH a, const double* __restri
     }
     block_sum_max(s, m, red_s, red_m);
     const double rhobar = s / (double)N;
+    if (RLDS) {
+        for (int k = tid; k < N; k += B) rsrc[k] = a.c4piG * (smax(rho[k], a.floor_rho) - rhobar);
+        __syncthreads();
+    }
     int it = 0;
     while (it < a.max_iter) {
         double mres = 0.0;
@@ -121,7 +129,7 @@ __global__ __launch_bounds__(1024) void k_eh_sor(SorEH a, const double* __restri
                 const double phiW = phi[row + (i == 0 ? Nx - 1 : i - 1)];
                 const double phiN = phi[(j + 1 == Ny ? 0 : j + 1) * Nx + i];
                 const double phiS = phi[(j == 0 ? Ny - 1 : j - 1) * Nx + i];
-                const double r = a.c4piG * (smax(rho[k], a.floor_rho) - rhobar);
+                const double r = RLDS ? rsrc[k] : a.c4piG * (smax(rho[k], a.floor_rho) - rhobar);
                 const double phi_new = ((phiE + phiW) * a.idx2 + (phiN + phiS) * a.idy2 - r) / a.denom;
                 double pk = phi[k];
                 const double delta = phi_new - pk;
@@ -327,6 +335,33 @@ __global__ void k_hllc_batch(int variant, int axis, int n, const double* __restr
     o[3] = F[3];
 }
 
+// Snapshot frame (the store lambda of EH:661-691): enforce_floors on a copy (EH:62-72), then
+// cons_to_prim_full (EH:74-83); Mach = |v| / a for a > 0, else 0.
+__global__ __launch_bounds__(256) void k_eh_frame(GeomEH g, StateEH U, double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= g.Nx) return;
+    const size_t N = (size_t)g.Nx * g.Ny, q = (size_t)j * g.Nx + i;
+    double rho = smax(U.rho[q], g.floor_rho);
+    const double mx = U.mx[q], my = U.my[q];
+    double E = U.E[q];
+    {
+        const double u = mx / rho, v = my / rho;
+        const double kin = 0.5 * rho * (u * u + v * v);
+        const double P = g.gm1 * (E - kin);
+        if (P < g.floor_P) E = kin + g.floor_P / g.gm1;
+    }
+    const double r = smax(rho, g.floor_rho);
+    const double u = mx / r, v = my / r;
+    const double kin = 0.5 * r * (u * u + v * v);
+    double P = g.gm1 * (E - kin);
+    if (P < g.floor_P) P = g.floor_P;
+    const double a = sqrt(g.gamma * P / r);
+    const double vmag = sqrt(u * u + v * v);
+    out[q] = rho;
+    out[N + q] = P;
+    out[2 * N + q] = (a > 0.0) ? (vmag / a) : 0.0;
+}
+
 }  // namespace cc
 
 using namespace cc;
@@ -345,10 +380,18 @@ struct cc_euler2d {
     long long total_sweeps = 0;
     int solves = 0;
     Profiler prof;
+    FrameBuffer frame;
+    FieldStats stats;
     std::vector<void*> allocs;
 
     size_t N() const { return (size_t)p.Nx * p.Ny; }
     Profiler* pr() { return prof.on ? &prof : nullptr; }
+    cc_status make_frame() {
+        CC_TRY(frame.ensure((size_t)CC_EULER2D_FRAME_FIELDS * N()));
+        hipLaunchKernelGGL(k_eh_frame, dim3((p.Nx + 255) / 256, p.Ny), dim3(256), 0, stream, g, U, frame.d);
+        CC_CHECK_LAUNCH();
+        return CC_OK;
+    }
 
     cc_status alloc(void** ptr, size_t bytes) {
         if (hipMalloc(ptr, bytes) != hipSuccess) {
@@ -369,11 +412,17 @@ struct cc_euler2d {
     cc_status gravity(const StateEH& s) {
         if (!p.gravity_on) return CC_OK;  // g stays zero (EH:293)
         ProfScope ps(pr(), stream, "sor");
-        const size_t lds = sor_lds ? N() * 8 : 0;
-        if (sor_lds)
-            hipLaunchKernelGGL(k_eh_sor<true>, dim3(1), dim3(sor_threads), lds, stream, sa, s.rho, phi, gx, gy, sweeps);
+        const bool rlds = N() <= 4096;  // phi and the source term in 64 KB of LDS
+        const size_t lds = sor_lds ? N() * 8 * (rlds ? 2 : 1) : 0;
+        if (sor_lds && rlds)
+            hipLaunchKernelGGL((k_eh_sor<true, true>), dim3(1), dim3(sor_threads), lds, stream, sa, s.rho, phi, gx, gy,
+                               sweeps);
+        else if (sor_lds)
+            hipLaunchKernelGGL((k_eh_sor<true, false>), dim3(1), dim3(sor_threads), lds, stream, sa, s.rho, phi, gx,
+                               gy, sweeps);
         else
-            hipLaunchKernelGGL(k_eh_sor<false>, dim3(1), dim3(sor_threads), 0, stream, sa, s.rho, phi, gx, gy, sweeps);
+            hipLaunchKernelGGL((k_eh_sor<false, false>), dim3(1), dim3(sor_threads), 0, stream, sa, s.rho, phi, gx, gy,
+                               sweeps);
         CC_CHECK_LAUNCH();
         return CC_OK;
     }
@@ -601,6 +650,38 @@ cc_status cc_euler2d_sweep_stats(cc_euler2d* h, int* last, long long* total, int
     if (total) *total = h->total_sweeps;
     if (solves) *solves = h->solves;
     return CC_OK;
+}
+
+cc_status cc_euler2d_frame(cc_euler2d* h, double* out) {
+    CHECK_H();
+    if (!out) { set_error(CC_ERR_INVALID_ARG, "null output"); return CC_ERR_INVALID_ARG; }
+    CC_TRY(h->make_frame());
+    CC_HIP(hipMemcpyAsync(out, h->frame.d, sizeof(double) * CC_EULER2D_FRAME_FIELDS * h->N(), hipMemcpyDeviceToHost,
+                          h->stream));
+    CC_HIP(hipStreamSynchronize(h->stream));
+    return CC_OK;
+}
+
+cc_status cc_euler2d_totals(cc_euler2d* h, double* out4) {
+    CHECK_H();
+    if (!out4) { set_error(CC_ERR_INVALID_ARG, "null output"); return CC_ERR_INVALID_ARG; }
+    FieldPtrs a{{h->U.rho, h->U.mx, h->U.my, h->U.E}};
+    double r[12];
+    CC_TRY(h->stats.run(a, nullptr, 4, h->N(), h->stream, r));
+    for (int f = 0; f < 4; f++) out4[f] = r[3 * f] * (h->g.dx * h->g.dy);
+    return CC_OK;
+}
+
+cc_status cc_euler2d_compare(cc_euler2d* a, cc_euler2d* b, double* out) {
+    if (!a || !b || !out) { set_error(CC_ERR_INVALID_ARG, "null handle or output"); return CC_ERR_INVALID_ARG; }
+    if (a->p.Nx != b->p.Nx || a->p.Ny != b->p.Ny) {
+        set_error(CC_ERR_INVALID_ARG, "cc_euler2d_compare: grids differ");
+        return CC_ERR_INVALID_ARG;
+    }
+    CC_TRY(a->make_frame());
+    CC_TRY(b->make_frame());
+    CC_HIP(hipStreamSynchronize(b->stream));
+    return compare_frames(a->stats, a->frame.d, b->frame.d, CC_EULER2D_FRAME_FIELDS, a->N(), a->stream, out);
 }
 
 cc_status cc_euler2d_sync(cc_euler2d* h) {

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <vector>
 
+#include "diag.hpp"
 #include "poisson2d.hpp"
 #include "riemann.hpp"
 #include "stepcommon.hpp"
@@ -320,6 +321,30 @@ __global__ __launch_bounds__(256) void k_divb_max2d(Geom2D g, State2D U, Scalars
     if ((threadIdx.x & 63) == 0 && v > 0.0) atomic_max_pos((unsigned long long*)&sc->divb, v);
 }
 
+// Snapshot frame (the store lambda of M2:1065-1106): cons_to_prim on face-averaged B, sound speed,
+// Mach, cell divB.  No FP contraction, so the frame rounds as the reference's does.
+__global__ __launch_bounds__(256) void k_frame2d(Geom2D g, State2D U, double* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= g.Nx) return;
+    const size_t N = (size_t)g.Nx * g.Ny, q = (size_t)j * g.Nx + i, fl = (size_t)j * (g.Nx + 1) + i;
+    const double bxL = U.Bx[fl], bxR = U.Bx[fl + 1], byD = U.By[q], byU = U.By[q + g.Nx];
+    const double bx = 0.5 * (bxL + bxR), by = 0.5 * (byD + byU), bz = U.Bz[q];
+    const double rho = smax(U.rho[q], g.floor_rho);  // cons_to_prim, M2:126-135
+    const double u = U.mx[q] / rho, v = U.my[q] / rho, w = U.mz[q] / rho;
+    const double kin = 0.5 * rho * (u * u + v * v + w * w);
+    const double mag = 0.5 * (bx * bx + by * by + bz * bz);
+    double P = g.gm1 * (U.E[q] - kin - mag);
+    if (P < g.floor_P) P = g.floor_P;
+    const double cs = sqrt(smax(0.0, g.gamma * P / smax(rho, g.floor_rho)));  // M2:148-150, M2:1092
+    const double vabs = sqrt(u * u + v * v + w * w);
+    const double mach = vabs / smax(cs, 1e-14);
+    const double divb = (bxR - bxL) / g.dx + (byU - byD) / g.dy;  // divB_cc, M2:873-879
+    const double f[10] = {rho, P, mach, u, v, vabs, bx, by, bz, divb};
+#pragma unroll
+    for (int k = 0; k < 10; k++) out[(size_t)k * N + q] = f[k];
+}
+
 // Batched HLLD for tests (axis 0: hlld_x M2:372; axis 1: hlld_y M2:540), through the same face
 // routine as the flux kernel.
 template <bool MIXED>
@@ -370,6 +395,8 @@ struct cc_mhd2d {
     int nspd = 0;
     Poisson2D pois;
     Profiler prof;
+    FrameBuffer frame;
+    FieldStats stats;
     std::vector<void*> allocs;
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};  // step graph starting from state set k
     bool graph_ok[3] = {false, false, false};
@@ -440,6 +467,14 @@ struct cc_mhd2d {
         CC_TRY(stage<false>(S[a], S[b], gx, gy, S[a]));
         CC_TRY(gravity_of(S[b].rho, g1x, g1y));
         CC_TRY(stage<true>(S[b], S[c], g1x, g1y, S[a]));
+        return CC_OK;
+    }
+
+    // The snapshot frame of the current state into the frame buffer.
+    cc_status make_frame() {
+        CC_TRY(frame.ensure((size_t)CC_MHD2D_FRAME_FIELDS * N()));
+        hipLaunchKernelGGL(k_frame2d, dim3((p.Nx + 255) / 256, p.Ny), dim3(256), 0, stream, g, S[cur], frame.d);
+        CC_CHECK_LAUNCH();
         return CC_OK;
     }
 
@@ -703,6 +738,40 @@ cc_status cc_mhd2d_divb_max(cc_mhd2d* h, double* out) {
     CC_HIP(hipStreamSynchronize(h->stream));
     *out = s.divb;
     return CC_OK;
+}
+
+cc_status cc_mhd2d_frame(cc_mhd2d* h, double* out) {
+    CHECK_H();
+    if (!out) { set_error(CC_ERR_INVALID_ARG, "null output"); return CC_ERR_INVALID_ARG; }
+    CC_TRY(h->make_frame());
+    CC_HIP(hipMemcpyAsync(out, h->frame.d, sizeof(double) * CC_MHD2D_FRAME_FIELDS * h->N(), hipMemcpyDeviceToHost,
+                          h->stream));
+    CC_HIP(hipStreamSynchronize(h->stream));
+    return CC_OK;
+}
+
+cc_status cc_mhd2d_totals(cc_mhd2d* h, double* out6) {
+    CHECK_H();
+    if (!out6) { set_error(CC_ERR_INVALID_ARG, "null output"); return CC_ERR_INVALID_ARG; }
+    const State2D& s = h->S[h->cur];
+    FieldPtrs a{{s.rho, s.mx, s.my, s.mz, s.E}};
+    double r[15];
+    CC_TRY(h->stats.run(a, nullptr, 5, h->N(), h->stream, r));
+    const double dA = h->g.dx * h->g.dy;
+    for (int f = 0; f < 5; f++) out6[f] = r[3 * f] * dA;
+    return cc_mhd2d_divb_max(h, out6 + 5);
+}
+
+cc_status cc_mhd2d_compare(cc_mhd2d* a, cc_mhd2d* b, double* out) {
+    if (!a || !b || !out) { set_error(CC_ERR_INVALID_ARG, "null handle or output"); return CC_ERR_INVALID_ARG; }
+    if (a->p.Nx != b->p.Nx || a->p.Ny != b->p.Ny) {
+        set_error(CC_ERR_INVALID_ARG, "cc_mhd2d_compare: grids differ");
+        return CC_ERR_INVALID_ARG;
+    }
+    CC_TRY(a->make_frame());
+    CC_TRY(b->make_frame());
+    CC_HIP(hipStreamSynchronize(b->stream));
+    return compare_frames(a->stats, a->frame.d, b->frame.d, CC_MHD2D_FRAME_FIELDS, a->N(), a->stream, out);
 }
 
 cc_status cc_mhd2d_sync(cc_mhd2d* h) {

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "profiler.hpp"
+#include "diag.hpp"
 #include "riemann.hpp"
 #include "stepcommon.hpp"
 
@@ -466,6 +467,33 @@ __global__ __launch_bounds__(256) void k_divb_max3d(Geom3D g, State3D U, double*
     block_max_to(v, spd);
 }
 
+// Snapshot frame (the store block of M3:1293-1325): cons_to_prim on face-averaged B, sound speed of
+// the EOS, Mach (floor 1e-12), cell divB.  No FP contraction.
+__global__ __launch_bounds__(256) void k_frame3d(Geom3D g, Eos3D e, State3D U, double* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= g.Nx) return;
+    const size_t N = (size_t)g.Nx * g.Ny * g.Nz, q = ((size_t)k * g.Ny + j) * g.Nx + i;
+    const double bxL = U.Bx[fx_(g, i, j, k)], bxR = U.Bx[fx_(g, i + 1, j, k)];
+    const double byD = U.By[fy_(g, i, j, k)], byU = U.By[fy_(g, i, j + 1, k)];
+    const double bzB = U.Bz[fz_(g, i, j, k)], bzF = U.Bz[fz_(g, i, j, k + 1)];
+    const double bx = 0.5 * (bxL + bxR), by = 0.5 * (byD + byU), bz = 0.5 * (bzB + bzF);
+    const Prim<double> W = cons_to_prim3d(e, U.rho[q], U.mx[q], U.my[q], U.mz[q], U.E[q], bx, by, bz);
+    double cs2;
+    if (e.ideal) {
+        cs2 = e.gamma * W.P / smax(W.rho, e.hy.floor_rho);
+    } else {
+        cs2 = e.hy.sound_speed2(W, e.hy.aux(W));  // M3:206-212
+    }
+    const double cs = sqrt(cs2);
+    const double vabs = sqrt(W.u * W.u + W.v * W.v + W.w * W.w);
+    const double mach = vabs / smax(cs, 1e-12);
+    const double divb = (bxR - bxL) / g.dx + (byU - byD) / g.dy + (bzF - bzB) / g.dz;  // M3:1070-1078
+    const double f[11] = {W.rho, W.P, mach, W.u, W.v, W.w, vabs, W.Bx, W.By, W.Bz, divb};
+#pragma unroll
+    for (int c = 0; c < 11; c++) out[(size_t)c * N + q] = f[c];
+}
+
 __global__ __launch_bounds__(256) void k_reduce_max_to(const double* spd, int nspd, double* out) {
     const double m = block_reduce_max(spd, nspd, 0.0);
     if (threadIdx.x == 0) out[0] = m;
@@ -527,10 +555,20 @@ struct cc_mhd3d {
     double* red = nullptr;
     int nspd = 0;
     Profiler prof;
+    FrameBuffer frame;
+    FieldStats stats;
     std::vector<void*> allocs;
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
 
     size_t N() const { return (size_t)p.Nx * p.Ny * p.Nz; }
+    // The snapshot frame of the current state into the frame buffer.
+    cc_status make_frame() {
+        CC_TRY(frame.ensure((size_t)CC_MHD3D_FRAME_FIELDS * N()));
+        hipLaunchKernelGGL(k_frame3d, dim3((p.Nx + 255) / 256, p.Ny, p.Nz), dim3(256), 0, stream, g, e, S[cur],
+                           frame.d);
+        CC_CHECK_LAUNCH();
+        return CC_OK;
+    }
     size_t NFx() const { return (size_t)(p.Nx + 1) * p.Ny * p.Nz; }
     size_t NFy() const { return (size_t)p.Nx * (p.Ny + 1) * p.Nz; }
     size_t NFz() const { return (size_t)p.Nx * p.Ny * (p.Nz + 1); }
@@ -885,6 +923,40 @@ cc_status cc_mhd3d_divb_max(cc_mhd3d* h, double* out) {
     CC_HIP(hipMemcpyAsync(out, h->red, 8, hipMemcpyDeviceToHost, h->stream));
     CC_HIP(hipStreamSynchronize(h->stream));
     return CC_OK;
+}
+
+cc_status cc_mhd3d_frame(cc_mhd3d* h, double* out) {
+    CHECK_H3();
+    if (!out) { set_error(CC_ERR_INVALID_ARG, "null output"); return CC_ERR_INVALID_ARG; }
+    CC_TRY(h->make_frame());
+    CC_HIP(hipMemcpyAsync(out, h->frame.d, sizeof(double) * CC_MHD3D_FRAME_FIELDS * h->N(), hipMemcpyDeviceToHost,
+                          h->stream));
+    CC_HIP(hipStreamSynchronize(h->stream));
+    return CC_OK;
+}
+
+cc_status cc_mhd3d_totals(cc_mhd3d* h, double* out6) {
+    CHECK_H3();
+    if (!out6) { set_error(CC_ERR_INVALID_ARG, "null output"); return CC_ERR_INVALID_ARG; }
+    const State3D& s = h->S[h->cur];
+    FieldPtrs a{{s.rho, s.mx, s.my, s.mz, s.E}};
+    double r[15];
+    CC_TRY(h->stats.run(a, nullptr, 5, h->N(), h->stream, r));
+    const double dV = h->g.dx * h->g.dy * h->g.dz;
+    for (int f = 0; f < 5; f++) out6[f] = r[3 * f] * dV;
+    return cc_mhd3d_divb_max(h, out6 + 5);
+}
+
+cc_status cc_mhd3d_compare(cc_mhd3d* a, cc_mhd3d* b, double* out) {
+    if (!a || !b || !out) { set_error(CC_ERR_INVALID_ARG, "null handle or output"); return CC_ERR_INVALID_ARG; }
+    if (a->p.Nx != b->p.Nx || a->p.Ny != b->p.Ny || a->p.Nz != b->p.Nz) {
+        set_error(CC_ERR_INVALID_ARG, "cc_mhd3d_compare: grids differ");
+        return CC_ERR_INVALID_ARG;
+    }
+    CC_TRY(a->make_frame());
+    CC_TRY(b->make_frame());
+    CC_HIP(hipStreamSynchronize(b->stream));
+    return compare_frames(a->stats, a->frame.d, b->frame.d, CC_MHD3D_FRAME_FIELDS, a->N(), a->stream, out);
 }
 
 cc_status cc_mhd3d_sync(cc_mhd3d* h) {

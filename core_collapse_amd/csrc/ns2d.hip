@@ -19,6 +19,7 @@
 #include <cstring>
 #include <vector>
 
+#include "diag.hpp"
 #include "fft.hpp"
 #include "hllc.hpp"
 #include "profiler.hpp"
@@ -297,6 +298,34 @@ __global__ __launch_bounds__(256) void k_ns_dt(GeomNS g, const double* __restric
     }
 }
 
+// The AoS state as four planes {rho, mx, my, E} (for the totals).
+__global__ __launch_bounds__(256) void k_ns_unpack(int N, const double4* __restrict__ U, double* __restrict__ out) {
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= N) return;
+    const double4 u = U[q];
+    out[q] = u.x;
+    out[(size_t)N + q] = u.y;
+    out[2 * (size_t)N + q] = u.z;
+    out[3 * (size_t)N + q] = u.w;
+}
+
+// Snapshot frame fields except Phi (the store lambda of NS:698-749): {rho, P, Mach, u, v, momx, momy}.
+__global__ __launch_bounds__(256) void k_ns_frame(GeomNS g, const double4* __restrict__ U, double* __restrict__ out) {
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    const int N = g.Nx * g.Ny;
+    if (q >= N) return;
+    const double4 W = prim_ns(g, U[q]);
+    const double a = sound_ns(g, W);
+    const double vmag = sqrt(W.y * W.y + W.z * W.z);
+    out[q] = W.x;
+    out[(size_t)N + q] = W.w;
+    out[2 * (size_t)N + q] = (a > 0.0) ? (vmag / a) : 0.0;
+    out[3 * (size_t)N + q] = W.y;
+    out[4 * (size_t)N + q] = W.z;
+    out[5 * (size_t)N + q] = W.x * W.y;
+    out[6 * (size_t)N + q] = W.x * W.z;
+}
+
 }  // namespace cc
 
 using namespace cc;
@@ -312,6 +341,8 @@ struct cc_ns2d {
     Fft2D fft;
     FftGreen green{};
     Profiler prof;
+    FrameBuffer frame;
+    FieldStats stats;
     std::vector<void*> allocs;
 
     size_t N() const { return (size_t)p.Nx * p.Ny; }
@@ -349,6 +380,22 @@ struct cc_ns2d {
             CC_CHECK_LAUNCH();
         }
         return poisson_from_plane();
+    }
+    // The snapshot frame of the current state, Phi from the FFT solve of its density (zero without
+    // gravity, NS:741-745).  The solve leaves phi as the next step's first solve would.
+    cc_status make_frame() {
+        const size_t n = N();
+        CC_TRY(frame.ensure((size_t)CC_NS2D_FRAME_FIELDS * n));
+        hipLaunchKernelGGL(k_ns_frame, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, g, U, frame.d);
+        CC_CHECK_LAUNCH();
+        double* phi_out = frame.d + 7 * n;
+        if (p.gravity_on) {
+            CC_TRY(poisson_of(U));
+            CC_HIP(hipMemcpyAsync(phi_out, phi, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+        } else {
+            CC_HIP(hipMemsetAsync(phi_out, 0, n * sizeof(double), stream));
+        }
+        return CC_OK;
     }
     template <int MODE>
     cc_status stage(const double4* in, const double4* old, double4* out, double dt) {
@@ -544,6 +591,43 @@ cc_status cc_ns2d_run(cc_ns2d* h, int nsteps, double* t_inout, double t_end, dou
     CC_HIP(hipStreamSynchronize(h->stream));
     *t_inout = t;
     return CC_OK;
+}
+
+cc_status cc_ns2d_frame(cc_ns2d* h, double* out) {
+    CHECK_H();
+    if (!out) { set_error(CC_ERR_INVALID_ARG, "null output"); return CC_ERR_INVALID_ARG; }
+    CC_TRY(h->make_frame());
+    CC_HIP(hipMemcpyAsync(out, h->frame.d, sizeof(double) * CC_NS2D_FRAME_FIELDS * h->N(), hipMemcpyDeviceToHost,
+                          h->stream));
+    CC_HIP(hipStreamSynchronize(h->stream));
+    return CC_OK;
+}
+
+cc_status cc_ns2d_totals(cc_ns2d* h, double* out4) {
+    CHECK_H();
+    if (!out4) { set_error(CC_ERR_INVALID_ARG, "null output"); return CC_ERR_INVALID_ARG; }
+    // the AoS state is unpacked into the first four planes of the frame buffer and summed there
+    const size_t n = h->N();
+    CC_TRY(h->frame.ensure((size_t)CC_NS2D_FRAME_FIELDS * n));
+    hipLaunchKernelGGL(k_ns_unpack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, (int)n, h->U, h->frame.d);
+    CC_CHECK_LAUNCH();
+    FieldPtrs a{{h->frame.d, h->frame.d + n, h->frame.d + 2 * n, h->frame.d + 3 * n}};
+    double r[12];
+    CC_TRY(h->stats.run(a, nullptr, 4, n, h->stream, r));
+    for (int f = 0; f < 4; f++) out4[f] = r[3 * f] * (h->g.dx * h->g.dy);
+    return CC_OK;
+}
+
+cc_status cc_ns2d_compare(cc_ns2d* a, cc_ns2d* b, double* out) {
+    if (!a || !b || !out) { set_error(CC_ERR_INVALID_ARG, "null handle or output"); return CC_ERR_INVALID_ARG; }
+    if (a->p.Nx != b->p.Nx || a->p.Ny != b->p.Ny) {
+        set_error(CC_ERR_INVALID_ARG, "cc_ns2d_compare: grids differ");
+        return CC_ERR_INVALID_ARG;
+    }
+    CC_TRY(a->make_frame());
+    CC_TRY(b->make_frame());
+    CC_HIP(hipStreamSynchronize(b->stream));
+    return compare_frames(a->stats, a->frame.d, b->frame.d, CC_NS2D_FRAME_FIELDS, a->N(), a->stream, out);
 }
 
 cc_status cc_ns2d_sync(cc_ns2d* h) {

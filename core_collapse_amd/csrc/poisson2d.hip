@@ -276,32 +276,120 @@ __global__ __launch_bounds__(1024) void k_sor_tile_mixed(SorArgs a, const int* _
 }
 
 // ------------------------------------------------------------------ periodic / mixed-BC SOR
-// One workgroup; sweeps in order; within a sweep the anti-diagonals i+j = d in increasing d,
-// which is the lexicographic order's dependency order including the wrap-around reads
-// ((0,j) reads the old (N-1,j); (N-1,j) reads the new (0,j)).
+// One workgroup, time-skewed: cell (i, j) of sweep s is updated at step T = i + j + c s with
+// c = max(Nx, Ny).  Within a sweep this is the anti-diagonal order, the lexicographic order's
+// dependency order including the wrap-around reads ((0,j) reads the old (Nx-1,j); (Nx-1,j) reads the
+// new (0,j); likewise in y).  Sweep s+1 may start on a row while sweep s is still on later rows:
+// every value a cell reads was written at an earlier step and the cell it overwrites is read by no
+// later-step cell of the previous sweep, as long as c >= Nx and c >= Ny (the wrap-around reads of
+// the previous sweep's last column / row, written c - Nx + 1 and c - Ny + 1 steps earlier).  With
+// c >= Nx a row has at most one cell per step, so one thread per row does all its cells.
 __global__ __launch_bounds__(1024) void k_sor_wavefront_1wg(SorArgs a, double* __restrict__ phi,
                                                             const double* __restrict__ rho,
                                                             const double* __restrict__ rhobar) {
     const double rb = rhobar[0];
     const int Nx = a.Nx, Ny = a.Ny;
-    for (int it = 0; it < a.nsw; ++it) {
-        for (int d = 0; d < Nx + Ny - 1; ++d) {
-            const int ilo = max(0, d - (Ny - 1)), ihi = min(d, Nx - 1);
-            for (int i = ilo + (int)threadIdx.x; i <= ihi; i += (int)blockDim.x) {
-                const int j = d - i;
-                const int iE = bmap(i + 1, Nx, a.px), iW = bmap(i - 1, Nx, a.px);
-                const int jN = bmap(j + 1, Ny, a.py), jS = bmap(j - 1, Ny, a.py);
-                const size_t k = (size_t)j * Nx + i;
-                const double pc = phi[k];
-                const double rhs = a.c4piG * (rho[k] - rb);
-                const double pnew = ((phi[(size_t)j * Nx + iE] + phi[(size_t)j * Nx + iW]) * a.idx2 +
-                                     (phi[(size_t)jN * Nx + i] + phi[(size_t)jS * Nx + i]) * a.idy2 - rhs) *
-                                    a.inv_denom;
-                phi[k] = pc + a.omega * (pnew - pc);
-            }
-            __syncthreads();
+    const int c = Nx > Ny ? Nx : Ny;
+    const long long steps = (long long)c * (a.nsw - 1) + (Nx - 1) + (Ny - 1) + 1;
+    for (long long T = 0; T < steps; ++T) {
+        for (int j = (int)threadIdx.x; j < Ny; j += (int)blockDim.x) {
+            const long long t = T - j;
+            if (t < 0) continue;
+            const long long s = t / c;
+            const int i = (int)(t - s * c);
+            if (s >= a.nsw || i >= Nx) continue;
+            const int iE = bmap(i + 1, Nx, a.px), iW = bmap(i - 1, Nx, a.px);
+            const int jN = bmap(j + 1, Ny, a.py), jS = bmap(j - 1, Ny, a.py);
+            const size_t k = (size_t)j * Nx + i;
+            const double pc = phi[k];
+            const double rhs = a.c4piG * (rho[k] - rb);
+            const double pnew = ((phi[(size_t)j * Nx + iE] + phi[(size_t)j * Nx + iW]) * a.idx2 +
+                                 (phi[(size_t)jN * Nx + i] + phi[(size_t)jS * Nx + i]) * a.idy2 - rhs) *
+                                a.inv_denom;
+            phi[k] = pc + a.omega * (pnew - pc);
         }
+        __syncthreads();
     }
+}
+
+// The same time-skewed schedule with one thread per row (Ny <= 1024) and no memory on a step's
+// critical path.  A row's cells run at consecutive steps, so its west neighbour is the thread's
+// previous result (a register); the south neighbour is row j-1's result of the previous step
+// (exchanged through LDS, double-buffered by step parity); the periodic seams are registers (a
+// row's first cell of this sweep, its last cell of the previous sweep) or LDS rings of the last
+// kSorRing results of rows 0 and Ny-1.  Everything else -- east and north neighbours and the cell
+// itself from the previous sweep, and rho -- is loaded two steps ahead.  Those locations are
+// written by the previous sweep, >= c - 2 steps before the load, and are next written after it.
+constexpr int kSorRing = 2048;
+
+struct SorOps {
+    double E, N, pc, r;
+};
+
+__global__ __launch_bounds__(1024) void k_sor_skew_rows(SorArgs a, double* __restrict__ phi,
+                                                        const double* __restrict__ rho,
+                                                        const double* __restrict__ rhobar) {
+    __shared__ double xs[2][1024];
+    __shared__ double ring0[kSorRing], ringN[kSorRing];
+    const double rb = rhobar[0];
+    const int Nx = a.Nx, Ny = a.Ny, j = (int)threadIdx.x;
+    const bool px = a.px != 0, py = a.py != 0;
+    const int c = Nx > Ny ? Nx : Ny;
+    const long long steps = (long long)c * (a.nsw - 1) + (Nx - 1) + (Ny - 1) + 1;
+    const size_t N = (size_t)Nx * Ny, dump = N + (size_t)j;  // phi has blockDim dump slots after the cells
+    const bool row = j < Ny;
+    // position of this row at the step being computed (i, s) and at the step being prefetched
+    int i = -j, s = 0, ip = -j + 2, sp = 0;
+    if (ip >= c) { ip -= c; ++sp; }
+    double out = 0.0, first = 0.0, wlast = 0.0;
+    auto issue = [&](int ii, int ss, SorOps& o) {
+        const bool act = row && ss < a.nsw && ii >= 0 && ii < Nx;
+        const size_t k = (size_t)j * Nx + (size_t)(act ? ii : 0);
+        o.E = phi[(act && ii + 1 < Nx) ? k + 1 : dump];
+        o.N = phi[(act && j + 1 < Ny) ? k + (size_t)Nx : dump];
+        o.pc = phi[act ? k : dump];
+        o.r = rho[act ? k : 0];
+    };
+    SorOps q[2];
+    issue(i, s, q[0]);
+    {
+        int i1 = i + 1, s1 = s;
+        if (i1 == c) { i1 = 0; ++s1; }
+        issue(i1, s1, q[1]);
+    }
+    auto step = [&](long long T, SorOps& o) {
+        const int par = (int)(T & 1);
+        const bool act = row && s < a.nsw && i >= 0 && i < Nx;
+        double S;
+        if (j > 0) S = xs[par ^ 1][j - 1];
+        else if (py) S = (s == 0) ? 0.0 : ringN[(int)((T - (c - Ny + 1)) & (kSorRing - 1))];
+        else S = o.pc;
+        const double Nb = (j < Ny - 1) ? o.N : (py ? ring0[(int)((T - (Ny - 1)) & (kSorRing - 1))] : o.pc);
+        const double E = (i < Nx - 1) ? o.E : (px ? first : o.pc);
+        const double W = (i > 0) ? out : (px ? (s == 0 ? 0.0 : wlast) : o.pc);
+        const double rhs = a.c4piG * (o.r - rb);
+        const double pnew = ((E + W) * a.idx2 + (Nb + S) * a.idy2 - rhs) * a.inv_denom;
+        const double v = o.pc + a.omega * (pnew - o.pc);
+        phi[act ? (size_t)j * Nx + (size_t)(i < 0 ? 0 : i) : dump] = v;
+        if (act) {
+            xs[par][j] = v;
+            if (j == 0) ring0[(int)(T & (kSorRing - 1))] = v;
+            if (j == Ny - 1) ringN[(int)(T & (kSorRing - 1))] = v;
+            out = v;
+            if (i == 0) first = v;
+            if (i == Nx - 1) wlast = v;
+        }
+        issue(ip, sp, o);  // operands of step T + 2
+        if (++i == c) { i = 0; ++s; }
+        if (++ip == c) { ip = 0; ++sp; }
+        __syncthreads();
+    };
+    long long T = 0;
+    for (; T + 1 < steps; T += 2) {
+        step(T, q[0]);
+        step(T + 1, q[1]);
+    }
+    if (T < steps) step(T, q[0]);
 }
 
 // ------------------------------------------------------------------ gradient, misc
@@ -375,7 +463,9 @@ cc_status Poisson2D::init(int Nx_, int Ny_, bool px_, bool py_, double dx_, doub
     precision = precision_;
     tiled_ = !px && !py;
     const size_t N = (size_t)Nx * Ny;
-    if (hipMalloc(&phi, N * sizeof(double)) != hipSuccess || hipMalloc(&partial, kReduceBlocks * sizeof(double)) != hipSuccess ||
+    // phi carries 1024 per-thread dump slots after the cells (used by k_sor_skew_rows)
+    if (hipMalloc(&phi, (N + 1024) * sizeof(double)) != hipSuccess ||
+        hipMalloc(&partial, kReduceBlocks * sizeof(double)) != hipSuccess ||
         hipMalloc(&rhobar, 4 * sizeof(double)) != hipSuccess) {
         set_error(CC_ERR_OOM, "Poisson2D: device allocation failed");
         return CC_ERR_OOM;
@@ -465,7 +555,10 @@ cc_status Poisson2D::solve(const double* rho, double* gx, double* gy, hipStream_
         }
     } else if (iters > 0) {
         ProfScope ps(prof, s, "sor");
-        hipLaunchKernelGGL(k_sor_wavefront_1wg, dim3(1), dim3(1024), 0, s, args, phi, rho, rhobar);
+        if (Ny <= 1024 && std::max(Nx, Ny) < kSorRing && std::getenv("CC_SOR_WAVEFRONT") == nullptr)
+            hipLaunchKernelGGL(k_sor_skew_rows, dim3(1), dim3((Ny + 63) / 64 * 64), 0, s, args, phi, rho, rhobar);
+        else
+            hipLaunchKernelGGL(k_sor_wavefront_1wg, dim3(1), dim3(1024), 0, s, args, phi, rho, rhobar);
         CC_CHECK_LAUNCH();
     }
     {

@@ -11,8 +11,9 @@
 //    (i,j) of sweep s runs at wavefront step i+j+2s, i.e. every step updates one colour of a
 //    red-black checkerboard inside a moving band; tiles of 64 x WJ cells x S sweeps are
 //    parallelograms in (i+s, j+s, s) and run level by level (one launch per tile level).
-//  * any periodic axis: the wrap-around neighbours forbid skewing, so sweeps run in order on
-//    one workgroup, anti-diagonal by anti-diagonal.
+//  * any periodic axis: one workgroup, anti-diagonal by anti-diagonal; the wrap-around
+//    neighbours allow sweeps to overlap only at a skew of max(Nx, Ny) steps per sweep (against
+//    Nx + Ny - 1 without overlap).
 // Mixed precision (tiled schedule): each tile evaluates the FP64 residual of its region once,
 // then runs its S sweeps on an FP32 correction in LDS and adds it back in FP64 (tile-local
 // iterative refinement).

@@ -139,6 +139,19 @@ class Euler2D:
     def sync(self):
         check(_lib.lib().cc_euler2d_sync(self._h))
 
+    # -- snapshot frame and in-situ diagnostics ------------------------------------------------
+    def frame(self):
+        """The fields one snapshot stores, computed on the device: {name: array}."""
+        return _lib.frame_of("euler2d", self._h, (self.params.Ny, self.params.Nx))
+
+    def totals(self):
+        """Cell-volume-weighted totals of the conserved variables (and max|divB| for MHD)."""
+        return _lib.totals_of("euler2d", self._h)
+
+    def compare(self, other):
+        """Per frame field (mean|a-b|, sum|a-b|/sum|b|, max|a-b|) of this handle against `other`."""
+        return _lib.compare_of("euler2d", self._h, other._h)
+
     def set_profiling(self, on=True):
         check(_lib.lib().cc_euler2d_set_profiling(self._h, int(bool(on))))
 

@@ -113,6 +113,20 @@ cc_status cc_mhd2d_set_profiling(cc_mhd2d* h, int on);
 cc_status cc_mhd2d_kernel_time(cc_mhd2d* h, const char* kernel, int* count, double* total_ms);
 cc_status cc_mhd2d_reset_profiling(cc_mhd2d* h);
 
+/* Snapshot frame of the current state, computed on the device as the store lambda of M2:1065-1106
+ * does: CC_MHD2D_FRAME_FIELDS planes of Ny*Nx cells, in the order of the output datasets
+ * {rho, P, Mach, u, v, vabs, Bx, By, Bz, divB} (M2:908-922), written to the host buffer `out`.  Syncs. */
+#define CC_MHD2D_FRAME_FIELDS 10
+cc_status cc_mhd2d_frame(cc_mhd2d* h, double* out);
+/* In-situ totals of the current state, reduced on the device: out[0..5] = {mass, x-, y-, z-momentum,
+ * total energy} as cell sums times dx*dy (the integrals of viz_h5.py:640-657 taken on the conserved
+ * variables), out[5] = max|divB| (M2:1119-1126).  Syncs. */
+cc_status cc_mhd2d_totals(cc_mhd2d* h, double* out6);
+/* L1/Linf comparison of the frames of two handles on one grid (e.g. CC_MIXED against CC_FP64), on the
+ * device.  For each frame field f: out[3f] = mean |a-b|, out[3f+1] = sum |a-b| / sum |b| (relative L1;
+ * 0 when sum |b| is 0), out[3f+2] = max |a-b|.  Syncs both handles. */
+cc_status cc_mhd2d_compare(cc_mhd2d* a, cc_mhd2d* b, double* out);
+
 /* Batched HLLD fluxes on independent face states (hlld_x M2:372-530 for axis 0, hlld_y
  * M2:540-561 for axis 1), ideal EOS.  W* are n x 8 {rho,u,v,w,P,Bx,By,Bz}; out is n x 8:
  * axis 0 {frho,fmx,fmy,fmz,fE,fBy,fBz,Ez}, axis 1 {frho,fmx,fmy,fmz,fE,fBx,fBz,Ez}. */
@@ -173,6 +187,14 @@ cc_status cc_mhd3d_sync(cc_mhd3d* h);
 cc_status cc_mhd3d_set_profiling(cc_mhd3d* h, int on);
 cc_status cc_mhd3d_kernel_time(cc_mhd3d* h, const char* kernel, int* count, double* total_ms);
 cc_status cc_mhd3d_reset_profiling(cc_mhd3d* h);
+/* Snapshot frame (the store block of M3:1293-1325): CC_MHD3D_FRAME_FIELDS volumes of Nz*Ny*Nx cells in
+ * dataset order {rho, P, Mach, u, v, w, vabs, Bx, By, Bz, divB} (M3:1112-1125).  Syncs. */
+#define CC_MHD3D_FRAME_FIELDS 11
+cc_status cc_mhd3d_frame(cc_mhd3d* h, double* out);
+/* {mass, x-, y-, z-momentum, total energy} * dx*dy*dz and max|divB|, reduced on the device.  Syncs. */
+cc_status cc_mhd3d_totals(cc_mhd3d* h, double* out6);
+/* Per-field {mean|a-b|, sum|a-b|/sum|b|, max|a-b|} of two handles' frames (as cc_mhd2d_compare). */
+cc_status cc_mhd3d_compare(cc_mhd3d* a, cc_mhd3d* b, double* out);
 /* Batched hlld_x (M3:415-568) with the EOS and precision of p; W* n x 8, out n x 7
  * {frho,fmx,fmy,fmz,fE,fBy,fBz}. */
 cc_status cc_hlld3d_batch(const cc_mhd3d_params* p, int n, const double* WL, const double* WR, const double* Bn,
@@ -222,6 +244,14 @@ cc_status cc_euler2d_sync(cc_euler2d* h);
 cc_status cc_euler2d_set_profiling(cc_euler2d* h, int on);
 cc_status cc_euler2d_kernel_time(cc_euler2d* h, const char* kernel, int* count, double* total_ms);
 cc_status cc_euler2d_reset_profiling(cc_euler2d* h);
+/* Snapshot frame (the store lambda of EH:661-691: floors on a copy, then cons_to_prim_full):
+ * CC_EULER2D_FRAME_FIELDS planes {rho, P, Mach} of Ny*Nx cells (EH:718-731).  Syncs. */
+#define CC_EULER2D_FRAME_FIELDS 3
+cc_status cc_euler2d_frame(cc_euler2d* h, double* out);
+/* {mass, x-, y-momentum, total energy} * dx*dy, reduced on the device.  Syncs. */
+cc_status cc_euler2d_totals(cc_euler2d* h, double* out4);
+/* Per-field {mean|a-b|, sum|a-b|/sum|b|, max|a-b|} of two handles' frames (as cc_mhd2d_compare). */
+cc_status cc_euler2d_compare(cc_euler2d* a, cc_euler2d* b, double* out);
 
 /* Batched HLLC on independent face states.  variant 0: EH hllc_x / hllc_y (EH:125-219);
  * variant 1: NS hllc_x / hllc_y (NS:212-282).  W* are n x 4 {rho,u,v,P}; out n x 4
@@ -267,10 +297,88 @@ cc_status cc_ns2d_sync(cc_ns2d* h);
 cc_status cc_ns2d_set_profiling(cc_ns2d* h, int on);
 cc_status cc_ns2d_kernel_time(cc_ns2d* h, const char* kernel, int* count, double* total_ms);
 cc_status cc_ns2d_reset_profiling(cc_ns2d* h);
+/* Snapshot frame (the store lambda of NS:698-749, including the FFT solve for Phi of the frame's
+ * density): CC_NS2D_FRAME_FIELDS planes {rho, P, Mach, u, v, momx, momy, Phi} of Ny*Nx cells
+ * (NS:593-605).  Syncs. */
+#define CC_NS2D_FRAME_FIELDS 8
+cc_status cc_ns2d_frame(cc_ns2d* h, double* out);
+/* {mass, x-, y-momentum, total energy} * dx*dy, reduced on the device.  Syncs. */
+cc_status cc_ns2d_totals(cc_ns2d* h, double* out4);
+/* Per-field {mean|a-b|, sum|a-b|/sum|b|, max|a-b|} of two handles' frames (as cc_mhd2d_compare). */
+cc_status cc_ns2d_compare(cc_ns2d* a, cc_ns2d* b, double* out);
 
 /* fft2d (NS:182-195) on the device: interleaved complex data[2*(i + Nx*j) + {0: re, 1: im}], rows
  * then columns, inverse scaled by 1/n per axis.  Powers of two <= 4096. */
 cc_status cc_fft2d(int Nx, int Ny, int inverse, double* data);
+
+/* ============================================================================================
+ * Initial conditions (host only: no device is needed).  The reference builds its ICs inline in
+ * main(); these functions restate them, with their knobs gathered in structs whose defaults
+ * (cc_*_ic_defaults) are the reference values, and add the benchmark-problem ICs.  Output arrays
+ * are caller-owned, in the layouts of the matching upload calls.
+ * ========================================================================================== */
+
+/* IC knobs of M2 (Params M2:74-85, gamma_eff of M2:1011). */
+typedef struct cc_mhd2d_ic {
+    double rho_bg, rho_core, r_core, r_edge, P0, vin, rot_omega, B0x, Bshear, gamma_eff;
+} cc_mhd2d_ic;
+void cc_mhd2d_ic_defaults(cc_mhd2d_ic* k);
+/* The default IC of M2 (M2:954-1048): face B from a corner vector potential Az, tanh core with
+ * infall and rotation, prim_to_cons + floors. */
+cc_status cc_mhd2d_ic_reference(const cc_mhd2d_params* p, const cc_mhd2d_ic* k, double* rho, double* mx, double* my,
+                                double* mz, double* E, double* Bz, double* Bx_f, double* By_f);
+
+/* Gaussian collapse (no reference counterpart; the 2D benchmark problem): rho = 1 + (contrast-1)
+ * exp(-r^2/width^2) about the box centre, p = P0, (u, v) = vamp * (z0, z1) with one Box-Muller pair
+ * per cell in row-major order from std::mt19937(seed) (u1 = (r1+0.5)/2^32, u2 = (r2+0.5)/2^32,
+ * z0 = sqrt(-2 ln u1) cos(2 pi u2), z1 = sqrt(-2 ln u1) sin(2 pi u2)), uniform Bx = B0. */
+typedef struct cc_collapse2d_ic {
+    unsigned seed;
+    double contrast, width, P0, vamp, B0;
+} cc_collapse2d_ic;
+void cc_collapse2d_ic_defaults(cc_collapse2d_ic* k); /* seed 42, contrast 10, width 0.1, P0 0.1, vamp 1e-3, B0 0.05 */
+cc_status cc_mhd2d_ic_collapse(const cc_mhd2d_params* p, const cc_collapse2d_ic* k, double* rho, double* mx, double* my,
+                               double* mz, double* E, double* Bz, double* Bx_f, double* By_f);
+
+/* IC knobs of M3 (Params M3:53, 76-88). */
+typedef struct cc_mhd3d_ic {
+    double rho_bg, rho_core, r_core, r_edge, P0, vin, rot_omega, B0x, B0z, Bshear, gamma_init;
+} cc_mhd3d_ic;
+void cc_mhd3d_ic_defaults(cc_mhd3d_ic* k);
+/* The default IC of M3 (M3:1159-1272) through the EOS of p (hybrid for eos 0). */
+cc_status cc_mhd3d_ic_reference(const cc_mhd3d_params* p, const cc_mhd3d_ic* k, double* rho, double* mx, double* my,
+                                double* mz, double* E, double* Bx_f, double* By_f, double* Bz_f);
+
+/* Solenoidal turbulence (no reference counterpart; the 3D benchmark problem, ideal EOS): rho = 1,
+ * p = 1/gamma, uniform Bz = sqrt(2 p / beta); velocity from random-phase modes on integer wavevectors
+ * kmin <= |k| <= kmax with |v(k)| = (0.5 + a) |k|^-2 and phase 2 pi b (a, b = mt19937 draws / 2^32,
+ * component by component, modes in FFT order), projected solenoidal, inverse FFT, scaled to rms
+ * Mach `mach`.  Nx, Ny, Nz powers of two. */
+typedef struct cc_turbulence3d_ic {
+    unsigned seed;
+    int kmin, kmax;
+    double mach, beta;
+} cc_turbulence3d_ic;
+void cc_turbulence3d_ic_defaults(cc_turbulence3d_ic* k); /* seed 42, k 1..8, Mach 1, beta 1 */
+cc_status cc_mhd3d_ic_turbulence(const cc_mhd3d_params* p, const cc_turbulence3d_ic* k, double* rho, double* mx,
+                                 double* my, double* mz, double* E, double* Bx_f, double* By_f, double* Bz_f);
+
+/* init_random_blob of EH (EH:574-642) and its constants (EH:585-603). */
+typedef struct cc_euler2d_ic {
+    unsigned seed;
+    double rho0, P0, amp_rho, amp_P, vmax, sigma, bump_amp;
+} cc_euler2d_ic;
+void cc_euler2d_ic_defaults(cc_euler2d_ic* k);
+cc_status cc_euler2d_ic_reference(const cc_euler2d_params* p, const cc_euler2d_ic* k, double* rho, double* mx,
+                                  double* my, double* E);
+
+/* The collapse IC of NS (NS:638-688) and its constants (Params::seed NS:69, NS:642-660); AoS output. */
+typedef struct cc_ns2d_ic {
+    unsigned seed;
+    double rho0, P0, A, sigma_frac, Arho, Mach0;
+} cc_ns2d_ic;
+void cc_ns2d_ic_defaults(cc_ns2d_ic* k);
+cc_status cc_ns2d_ic_reference(const cc_ns2d_params* p, const cc_ns2d_ic* k, double* U);
 
 /* ============================================================================================
  * Measurement helper (no reference counterpart): a plain device-to-device copy of nbytes,

@@ -53,6 +53,8 @@ void or_m2_divb(const or_m2_params* p, const double* Bx_f, const double* By_f, d
 double or_m2_run(const or_m2_params* p, or_m2_state* U, double dx, double dy, int nsteps, double t0, double t_end,
                  double* dts_out);
 int or_m2_last_branch(void); /* restatement only: branch of the last hlld call (see oracle_mhd2d.c) */
+/* the snapshot frame of the store lambda (M2:1065-1106): 10 planes {rho,P,Mach,u,v,vabs,Bx,By,Bz,divB} */
+void or_m2_frame(const or_m2_params* p, const or_m2_state* U, double dx, double dy, double* out);
 
 /* ---- 3D resistive MHD (mhd3d_resistive_hlld_ct.cpp, Params M3:43-89) ---- */
 typedef struct {
@@ -85,6 +87,8 @@ void or_m3_rk2_step(const or_m3_params* p, or_m3_state* U, double dx, double dy,
 void or_m3_divb(const or_m3_params* p, const or_m3_state* U, double dx, double dy, double dz, double* out);
 double or_m3_run(const or_m3_params* p, or_m3_state* U, double dx, double dy, double dz, int nsteps, double t0,
                  double t_end, double* dts_out);
+/* the snapshot frame of the store block (M3:1293-1325): 11 volumes {rho,P,Mach,u,v,w,vabs,Bx,By,Bz,divB} */
+void or_m3_frame(const or_m3_params* p, const or_m3_state* U, double dx, double dy, double dz, double* out);
 
 /* ---- 2D Euler HLLC + SOR gravity (euler2d_gravity_hllc.cpp, Params EH:29-56) ---- */
 typedef struct {
@@ -108,6 +112,8 @@ double or_eh_compute_dt(const or_eh_params* p, const or_eh_state* U);
 void or_eh_rhs(const or_eh_params* p, const or_eh_state* U, or_eh_state* dU);
 void or_eh_rk2_step(const or_eh_params* p, or_eh_state* U, double dt);
 double or_eh_run(const or_eh_params* p, or_eh_state* U, int nsteps, double t0, double t_end, double* dts_out);
+/* the snapshot frame of the store lambda (EH:661-691): planes {rho, P, Mach} */
+void or_eh_frame(const or_eh_params* p, const or_eh_state* U, double* out);
 /* reference harness only: the program's default IC (init_random_blob, EH:574-642) */
 void or_eh_init_random_blob(const or_eh_params* p, or_eh_state* U);
 
@@ -132,6 +138,8 @@ void or_ns_compute_rhs(const or_ns_params* p, const double* U, double* dU);
 double or_ns_compute_dt(const or_ns_params* p, const double* U);
 void or_ns_heun_step(const or_ns_params* p, double* U, double dt);
 double or_ns_run(const or_ns_params* p, double* U, int nsteps, double t0, double t_end, double* dts_out);
+/* the snapshot frame of the store lambda (NS:698-749): planes {rho,P,Mach,u,v,momx,momy,Phi} */
+int or_ns_frame(const or_ns_params* p, const double* U, double* out);
 /* reference harness only: libstdc++'s std::normal_distribution / uniform_real_distribution over
  * std::mt19937(seed), n draws (the generators of the ICs at NS:638-639 and EH:585-587) */
 void or_std_normal(unsigned seed, int n, double* out);

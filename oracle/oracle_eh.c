@@ -333,3 +333,21 @@ double or_eh_run(const or_eh_params* p, or_eh_state* U, int nsteps, double t, do
     }
     return t;
 }
+
+/* The snapshot frame (the store lambda of EH:661-691): floors on a copy, cons_to_prim_full, planes
+ * {rho, P, Mach}. */
+void or_eh_frame(const or_eh_params* p, const or_eh_state* U, double* out) {
+    const size_t N = (size_t)p->Nx * (size_t)p->Ny;
+    for (int j = 0; j < p->Ny; j++)
+        for (int i = 0; i < p->Nx; i++) {
+            const size_t q = idx(i, j, p->Nx);
+            double rho = U->rho[q], mx = U->mx[q], my = U->my[q], E = U->E[q];
+            enforce_floors(p, &rho, &mx, &my, &E);
+            double u, v, P, a;
+            cons_to_prim_full(p, rho, mx, my, E, &u, &v, &P, &a);
+            const double vmag = sqrt(u * u + v * v);
+            out[q] = rho;
+            out[N + q] = P;
+            out[2 * N + q] = (a > 0.0) ? (vmag / a) : 0.0;
+        }
+}

@@ -763,3 +763,23 @@ double or_m2_run(const or_m2_params* p, or_m2_state* U, double dx, double dy, in
     free(gx); free(gy);
     return t;
 }
+
+/* The snapshot frame (the store lambda of M2:1065-1106): 10 planes {rho, P, Mach, u, v, vabs, Bx, By,
+ * Bz, divB} of Nx*Ny cells. */
+void or_m2_frame(const or_m2_params* p, const or_m2_state* U, double dx, double dy, double* out) {
+    const size_t N = (size_t)p->Nx * (size_t)p->Ny;
+    for (int j = 0; j < p->Ny; j++)
+        for (int i = 0; i < p->Nx; i++) {
+            const size_t q = idx2(i, j, p->Nx);
+            const double Bx_cc = bx_cc(p, U->Bx_f, i, j), By_cc = by_cc(p, U->By_f, i, j);
+            const Prim W = cons_to_prim(p, U->rho[q], U->mx[q], U->my[q], U->mz[q], U->E[q], Bx_cc, By_cc, U->Bz[q]);
+            const double cs = sqrt(dmax(0.0, sound_speed2(p, &W)));
+            const double vabs = sqrt(W.u * W.u + W.v * W.v + W.w * W.w);
+            const double Mach = vabs / dmax(cs, 1e-14);
+            const double BxR = get_Bx_face(p, U->Bx_f, i + 1, j), BxL = get_Bx_face(p, U->Bx_f, i, j);
+            const double ByU = get_By_face(p, U->By_f, i, j + 1), ByD = get_By_face(p, U->By_f, i, j);
+            const double f[10] = {W.rho, W.P, Mach, W.u, W.v, vabs, Bx_cc, By_cc, W.Bz,
+                                  (BxR - BxL) / dx + (ByU - ByD) / dy};
+            for (int k = 0; k < 10; k++) out[(size_t)k * N + q] = f[k];
+        }
+}

@@ -636,3 +636,25 @@ double or_m3_run(const or_m3_params* p, or_m3_state* U, double dx, double dy, do
     }
     return t;
 }
+
+/* The snapshot frame (the store block of M3:1293-1325): 11 volumes {rho, P, Mach, u, v, w, vabs, Bx,
+ * By, Bz, divB} of Nx*Ny*Nz cells. */
+void or_m3_frame(const or_m3_params* p, const or_m3_state* U, double dx, double dy, double dz, double* out) {
+    const size_t N = (size_t)p->Nx * (size_t)p->Ny * (size_t)p->Nz;
+    for (int k = 0; k < p->Nz; k++)
+        for (int j = 0; j < p->Ny; j++)
+            for (int i = 0; i < p->Nx; i++) {
+                const size_t q = idx3(i, j, k, p->Nx, p->Ny);
+                const double Bx = bx_cc(p, U, i, j, k), By = by_cc(p, U, i, j, k), Bz = bz_cc(p, U, i, j, k);
+                const Prim W = cons_to_prim(p, U->rho[q], U->mx[q], U->my[q], U->mz[q], U->E[q], Bx, By, Bz);
+                const double cs = sqrt(sound_speed2(p, &W));
+                const double vabs = sqrt(W.u * W.u + W.v * W.v + W.w * W.w);
+                const double Mach = vabs / dmax(cs, 1e-12);
+                const double BxR = get_Bx_face(p, U->Bx_f, i + 1, j, k), BxL = get_Bx_face(p, U->Bx_f, i, j, k);
+                const double ByU = get_By_face(p, U->By_f, i, j + 1, k), ByD = get_By_face(p, U->By_f, i, j, k);
+                const double BzF = get_Bz_face(p, U->Bz_f, i, j, k + 1), BzB = get_Bz_face(p, U->Bz_f, i, j, k);
+                const double f[11] = {W.rho, W.P, Mach, W.u, W.v, W.w, vabs, W.Bx, W.By, W.Bz,
+                                      (BxR - BxL) / dx + (ByU - ByD) / dy + (BzF - BzB) / dz};
+                for (int c = 0; c < 11; c++) out[(size_t)c * N + q] = f[c];
+            }
+}

@@ -334,3 +334,26 @@ double or_ns_run(const or_ns_params* p, double* U, int nsteps, double t, double 
     }
     return t;
 }
+
+/* The snapshot frame (the store lambda of NS:698-749): planes {rho, P, Mach, u, v, momx, momy, Phi},
+ * Phi by the FFT solve of the frame's density (zero without gravity).  Returns the solver's status. */
+int or_ns_frame(const or_ns_params* p, const double* U, double* out) {
+    const size_t N = (size_t)p->Nx * (size_t)p->Ny;
+    for (size_t q = 0; q < N; q++) {
+        const W4 W = cons_to_prim(p, U + 4 * q);
+        const double a = sound_speed(p, W);
+        const double vmag = sqrt(W.u * W.u + W.v * W.v);
+        out[q] = W.d;
+        out[N + q] = W.P;
+        out[2 * N + q] = (a > 0.0) ? (vmag / a) : 0.0;
+        out[3 * N + q] = W.u;
+        out[4 * N + q] = W.v;
+        out[5 * N + q] = W.d * W.u;
+        out[6 * N + q] = W.d * W.v;
+    }
+    if (!p->gravity_on) {
+        for (size_t q = 0; q < N; q++) out[7 * N + q] = 0.0;
+        return 0;
+    }
+    return or_ns_poisson_fft(p, out, out + 7 * N);
+}

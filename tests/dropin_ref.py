@@ -1,0 +1,161 @@
+"""Oracle-side models of the drop-in programs (test infrastructure): the reference main() loops
+restated over the oracle's step and frame functions, and readers of the program fixtures.
+
+Each run_* function follows one reference main loop -- cadence, clamping, log lines -- and returns
+(times, frames, log lines), frames as {dataset name: array} per stored snapshot.
+"""
+import ctypes as C
+import hashlib
+import json
+import os
+
+import numpy as np
+
+from oracle import bind as OB
+from tests.helpers import oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+ROOT = os.path.dirname(HERE)
+BIN = os.path.join(ROOT, "core_collapse_amd", "bin")
+
+FIELDS = {
+    "m2": ["/rho", "/P", "/Mach", "/u", "/v", "/vabs", "/Bx", "/By", "/Bz", "/divB"],
+    "m3": ["/rho", "/P", "/Mach", "/u", "/v", "/w", "/vabs", "/Bx", "/By", "/Bz", "/divB"],
+    "eh": ["/rho", "/P", "/Mach"],
+    "ns": ["/rho", "/P", "/Mach", "/u", "/v", "/momx", "/momy", "/Phi"],
+}
+PROGRAM = {"m2": "mhd2d_resistive_hlld_ct_gravity", "m3": "mhd3d_resistive_hlld_ct", "eh": "euler2d_gravity_hllc",
+           "ns": "navier_stokes_2d_hllc_poisson"}
+OUTFILE = {"m2": "out2d_mhd.h5", "m3": "out3d_mhd.h5", "eh": "out2d_hllc.h5", "ns": "out_ns2d_poisson.h5"}
+
+
+def fixture(tag):
+    """(metadata, arrays) of the reference program's default run; skips the calling test when the
+    fixture has not been generated (tests/golden/gen_dropin_golden.py)."""
+    import pytest
+    allmeta = json.load(open(os.path.join(GOLDEN, "dropin.json")))
+    if tag not in allmeta:
+        pytest.skip("no fixture for %s (run tests/golden/gen_dropin_golden.py)" % tag)
+    meta = allmeta[tag]
+    arrays = dict(np.load(os.path.join(GOLDEN, "dropin_%s.npz" % tag)))
+    return meta, arrays
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+def split_frame(tag, buf, shape):
+    return {name: buf[k].reshape(shape).copy() for k, name in enumerate(FIELDS[tag])}
+
+
+def frame_m2(op, S, dx, dy):
+    N = op.Nx * op.Ny
+    buf = np.empty(10 * N)
+    oracle().or_m2_frame(C.byref(op), C.byref(S.cstate()), C.c_double(dx), C.c_double(dy), OB.ptr(buf))
+    return split_frame("m2", buf.reshape(10, N), (op.Ny, op.Nx))
+
+
+def frame_m3(op, S, dx, dy, dz):
+    N = op.Nx * op.Ny * op.Nz
+    buf = np.empty(11 * N)
+    oracle().or_m3_frame(C.byref(op), C.byref(S.cstate()), C.c_double(dx), C.c_double(dy), C.c_double(dz),
+                         OB.ptr(buf))
+    return split_frame("m3", buf.reshape(11, N), (op.Nz, op.Ny, op.Nx))
+
+
+def frame_eh(op, S):
+    N = op.Nx * op.Ny
+    buf = np.empty(3 * N)
+    oracle().or_eh_frame(C.byref(op), C.byref(S.cstate()), OB.ptr(buf))
+    return split_frame("eh", buf.reshape(3, N), (op.Ny, op.Nx))
+
+
+def frame_ns(op, U):
+    N = op.Nx * op.Ny
+    buf = np.empty(8 * N)
+    assert oracle().or_ns_frame(C.byref(op), OB.ptr(U), OB.ptr(buf)) == 0
+    return split_frame("ns", buf.reshape(8, N), (op.Ny, op.Nx))
+
+
+def _fmt(x, prec):
+    """std::ostream << double at setprecision(prec) (default floatfield: %g)."""
+    return "%.*g" % (prec, x)
+
+
+def run_m2(op, S, t_end, dt_store):
+    """main() of M2 (M2:1108-1135) over the oracle: gravity, dt, clamp, rk2 per step."""
+    o = oracle()
+    dx, dy = (op.x1 - op.x0) / op.Nx, (op.y1 - op.y0) / op.Ny
+    frames, times, log = [frame_m2(op, S, dx, dy)], [0.0], []
+    t, next_store = 0.0, 0.0
+    dts = np.zeros(1)
+    while t < t_end:
+        t = o.or_m2_run(C.byref(op), C.byref(S.cstate()), C.c_double(dx), C.c_double(dy), 1, C.c_double(t),
+                        C.c_double(t_end), OB.ptr(dts))
+        if t >= next_store - 1e-15:
+            fr = frame_m2(op, S, dx, dy)
+            log.append("t=%s dt=%s max|divB|=%s" % (_fmt(t, 7), _fmt(dts[0], 7),
+                                                   _fmt(np.abs(fr["/divB"]).max(), 12)))
+            frames.append(fr)
+            times.append(t)
+            next_store += dt_store
+    return np.array(times), frames, log
+
+
+def run_m3(op, S, t_end, dt_store):
+    """main() of M3 (M3:1280-1327) over the oracle: no t = 0 frame."""
+    o = oracle()
+    dx, dy, dz = (op.x1 - op.x0) / op.Nx, (op.y1 - op.y0) / op.Ny, (op.z1 - op.z0) / op.Nz
+    frames, times, log = [], [], []
+    t, tnext = 0.0, 0.0
+    dts = np.zeros(1)
+    while t < t_end:
+        t = o.or_m3_run(C.byref(op), C.byref(S.cstate()), C.c_double(dx), C.c_double(dy), C.c_double(dz), 1,
+                        C.c_double(t), C.c_double(t_end), OB.ptr(dts))
+        if t >= tnext - 1e-12:
+            frames.append(frame_m3(op, S, dx, dy, dz))
+            times.append(t)
+            log.append("Stored frame %d at t=%s" % (len(frames) - 1, _fmt(t, 6)))
+            tnext += dt_store
+    return np.array(times), frames, log
+
+
+def run_eh(op, S, t_end, dt_store):
+    """main() of EH (EH:692-712) over the oracle."""
+    o = oracle()
+    frames, times, log = [frame_eh(op, S)], [0.0], []
+    t, next_store = 0.0, dt_store
+    while t < t_end - 1e-14:
+        dt = o.or_eh_compute_dt(C.byref(op), C.byref(S.cstate()))
+        assert np.isfinite(dt) and dt > 0.0
+        if t + dt > t_end:
+            dt = t_end - t
+        o.or_eh_rk2_step(C.byref(op), C.byref(S.cstate()), C.c_double(dt))
+        t += dt
+        if t + 1e-14 >= next_store or t >= t_end - 1e-14:
+            frames.append(frame_eh(op, S))
+            times.append(t)
+            next_store += dt_store
+            log.append("t=%s dt=%s snapshots=%d" % (_fmt(t, 10), _fmt(dt, 10), len(times)))
+    return np.array(times), frames, log
+
+
+def run_ns(op, U, t_end, dt_store):
+    """main() of NS (NS:748-789) over the oracle."""
+    o = oracle()
+    frames, times, log = [frame_ns(op, U)], [0.0], []
+    t, next_store = 0.0, dt_store
+    while t < t_end - 1e-14:
+        dt = o.or_ns_compute_dt(C.byref(op), OB.ptr(U))
+        if t + dt > t_end:
+            dt = t_end - t
+        o.or_ns_heun_step(C.byref(op), OB.ptr(U), C.c_double(dt))
+        t += dt
+        if t + 1e-14 >= next_store or t >= t_end - 1e-14:
+            frames.append(frame_ns(op, U))
+            times.append(t)
+            next_store += dt_store
+            log.append("t=%s dt=%s snapshots=%d" % (_fmt(t, 10), _fmt(dt, 10), len(times)))
+    return np.array(times), frames, log

@@ -115,7 +115,7 @@ def m3_dict(p, arr):
 
 def random_mhd3d_state(p, seed):
     """Smooth random 3D state with a divergence-free face field (from a corner vector potential)."""
-    from core_collapse_amd import ic
+    from tests import ic_restated as ic
     rng = np.random.default_rng(seed)
     Nx, Ny, Nz = p.Nx, p.Ny, p.Nz
     x = (np.arange(Nx) + 0.5) / Nx

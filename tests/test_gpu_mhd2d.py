@@ -93,6 +93,23 @@ def test_poisson_sor_fp64_matches_oracle(Nx, Ny, periodic, iters):
     assert rel_err(gx, ox) < 1e-9 and rel_err(gy, oy) < 1e-9, (rel_err(gx, ox), rel_err(gy, oy))
 
 
+@pytest.mark.parametrize("Nx,Ny,px,py,iters", [(18, 18, True, True, 150), (64, 64, True, True, 100),
+                                                (65, 65, True, False, 80), (100, 128, False, True, 60),
+                                                (129, 130, True, True, 40), (192, 192, True, True, 30),
+                                                (40, 256, True, True, 20), (300, 250, True, True, 10),
+                                                (16, 16, True, True, 100), (200, 300, True, True, 5)])
+def test_poisson_sor_periodic_schedules_match_oracle(Nx, Ny, px, py, iters):
+    """Periodic and mixed boundaries: the one-wave schedule (Ny <= 256 rows over 1-4 row slots per lane,
+    Nx, Ny >= 18) and the one-workgroup schedule (other sizes) give the lexicographic iterate."""
+    rng = np.random.default_rng(Nx * 7 + Ny)
+    p = M.Params(Nx=Nx, Ny=Ny, periodic_x=px, periodic_y=py, sor_iters=iters, G=1.0)
+    rho = 1.0 + rng.uniform(0, 1, (Ny, Nx)) + 5.0 * np.exp(-((np.arange(Nx)[None, :] - Nx / 3) ** 2 +
+                                                           (np.arange(Ny)[:, None] - Ny / 2) ** 2) / 30.0)
+    gx, gy = M.poisson_sor(rho, p.dx, p.dy, p.G, iters, p.sor_omega, px, py)
+    ox, oy = _oracle_poisson(p, rho)
+    assert rel_err(gx, ox) < 1e-9 and rel_err(gy, oy) < 1e-9, (rel_err(gx, ox), rel_err(gy, oy))
+
+
 def test_poisson_sor_mixed_close_to_oracle():
     rng = np.random.default_rng(5)
     Nx, Ny = 96, 80

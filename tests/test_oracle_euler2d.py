@@ -1,6 +1,6 @@
 """Pinning the 2D Euler oracle (oracle/oracle_eh.c), CPU only: golden vectors from the reference's own
-functions (tests/golden/eh_step.npz, bit for bit), the program's default IC restated in
-core_collapse_amd/ic.py, and closed forms -- uniform, supersonic and stationary-contact HLLC fluxes,
+functions (tests/golden/eh_step.npz, bit for bit), the program's default IC as libccamd builds it
+(core_collapse_amd/ic.py), and closed forms -- uniform, supersonic and stationary-contact HLLC fluxes,
 the converged SOR against a direct spectral solve of the same 5-point periodic Poisson problem, and
 conservation of a periodic run."""
 import ctypes as C

@@ -1,6 +1,6 @@
 """Pinning the 2D Navier-Stokes oracle (oracle/oracle_ns.c), CPU only: golden vectors from the
 reference's own functions (tests/golden/ns_step.npz, bit for bit), libstdc++'s normal / uniform
-distributions over std::mt19937 as restated in core_collapse_amd/ic.py, and closed forms -- the FFT
+distributions over std::mt19937 as restated in tests/ic_restated.py, and closed forms -- the FFT
 against numpy.fft, the Poisson solve against the exact spectral solution with continuous k^2, HLLC
 uniform / supersonic / stationary-contact fluxes, and conservation of a periodic run.
 
@@ -75,7 +75,7 @@ def test_poisson_dt_rhs_heun_run_golden_bitwise(tag):
 
 
 def test_libstdcxx_distributions_restated_bitwise():
-    from core_collapse_amd import ic
+    from tests import ic_restated as ic
     d = np.load(os.path.join(G, "ns_step.npz"))
     g = ic.StdRandom(42)
     assert np.array_equal(np.array([g.normal() for _ in range(len(d["std_normal"]))]), d["std_normal"])
