@@ -318,15 +318,19 @@ __global__ __launch_bounds__(1024) void k_sor_wavefront_1wg(SorArgs a, double* _
 // (exchanged through LDS, double-buffered by step parity); the periodic seams are registers (a
 // row's first cell of this sweep, its last cell of the previous sweep) or LDS rings of the last
 // kSorRing results of rows 0 and Ny-1.  Everything else -- east and north neighbours and the cell
-// itself from the previous sweep, and rho -- is loaded two steps ahead.  Those locations are
-// written by the previous sweep, >= c - 2 steps before the load, and are next written after it.
+// itself from the previous sweep, and rho -- is loaded PD steps ahead into a register ring.
+// Memory order: a thread's store at step t has completed by step t + PD (it then waits for loads
+// issued after the store, and vector memory operations complete in issue order); the loads issued
+// at step T read previous-sweep values stored at steps <= T + PD - c + 1 <= T - PD - 1 (c >= 2 PD + 2),
+// after the barrier that ends step T - 1, and those locations are next written after step T.
 constexpr int kSorRing = 2048;
 
 struct SorOps {
     double E, N, pc, r;
 };
 
-__global__ __launch_bounds__(1024) void k_sor_skew_rows(SorArgs a, double* __restrict__ phi,
+template <int PD, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_sor_skew_rows(SorArgs a, double* __restrict__ phi,
                                                         const double* __restrict__ rho,
                                                         const double* __restrict__ rhobar) {
     __shared__ double xs[2][1024];
@@ -335,61 +339,61 @@ __global__ __launch_bounds__(1024) void k_sor_skew_rows(SorArgs a, double* __res
     const int Nx = a.Nx, Ny = a.Ny, j = (int)threadIdx.x;
     const bool px = a.px != 0, py = a.py != 0;
     const int c = Nx > Ny ? Nx : Ny;
-    const long long steps = (long long)c * (a.nsw - 1) + (Nx - 1) + (Ny - 1) + 1;
-    const size_t N = (size_t)Nx * Ny, dump = N + (size_t)j;  // phi has blockDim dump slots after the cells
+    const int steps = c * (a.nsw - 1) + (Nx - 1) + (Ny - 1) + 1;  // < 2^31 (checked by the host)
+    const int dump = Nx * Ny + j;  // phi has blockDim dump slots after the cells
     const bool row = j < Ny;
-    // position of this row at the step being computed (i, s) and at the step being prefetched
-    int i = -j, s = 0, ip = -j + 2, sp = 0;
-    if (ip >= c) { ip -= c; ++sp; }
+    const int base = j * Nx;
+    const int offS = c - Ny + 1, offN = Ny - 1;  // ring distances of the periodic y seam
+    // position (i, s) of this row at the step being computed and (ip, sp) at the step being loaded;
+    // writes of a step without a cell go to dump slots (phi) or are never read (LDS), so every
+    // step runs the same instructions
+    int i = -j, s = 0;
     double out = 0.0, first = 0.0, wlast = 0.0;
     auto issue = [&](int ii, int ss, SorOps& o) {
         const bool act = row && ss < a.nsw && ii >= 0 && ii < Nx;
-        const size_t k = (size_t)j * Nx + (size_t)(act ? ii : 0);
+        const int k = base + ii;
         o.E = phi[(act && ii + 1 < Nx) ? k + 1 : dump];
-        o.N = phi[(act && j + 1 < Ny) ? k + (size_t)Nx : dump];
+        o.N = phi[(act && j + 1 < Ny) ? k + Nx : dump];
         o.pc = phi[act ? k : dump];
         o.r = rho[act ? k : 0];
     };
-    SorOps q[2];
-    issue(i, s, q[0]);
-    {
-        int i1 = i + 1, s1 = s;
-        if (i1 == c) { i1 = 0; ++s1; }
-        issue(i1, s1, q[1]);
+    SorOps q[PD];
+    int ip = i, sp = s;
+#pragma unroll
+    for (int u = 0; u < PD; ++u) {
+        issue(ip, sp, q[u]);
+        if (++ip == c) { ip = 0; ++sp; }
     }
-    auto step = [&](long long T, SorOps& o) {
-        const int par = (int)(T & 1);
+    auto step = [&](int T, int par, SorOps& o) {
         const bool act = row && s < a.nsw && i >= 0 && i < Nx;
-        double S;
-        if (j > 0) S = xs[par ^ 1][j - 1];
-        else if (py) S = (s == 0) ? 0.0 : ringN[(int)((T - (c - Ny + 1)) & (kSorRing - 1))];
-        else S = o.pc;
-        const double Nb = (j < Ny - 1) ? o.N : (py ? ring0[(int)((T - (Ny - 1)) & (kSorRing - 1))] : o.pc);
+        const double Sr = py ? ((s == 0) ? 0.0 : ringN[(T - offS) & (kSorRing - 1)]) : o.pc;
+        const double S = (j > 0) ? xs[par ^ 1][j - 1] : Sr;
+        const double Nb = (j < Ny - 1) ? o.N : (py ? ring0[(T - offN) & (kSorRing - 1)] : o.pc);
         const double E = (i < Nx - 1) ? o.E : (px ? first : o.pc);
         const double W = (i > 0) ? out : (px ? (s == 0 ? 0.0 : wlast) : o.pc);
         const double rhs = a.c4piG * (o.r - rb);
         const double pnew = ((E + W) * a.idx2 + (Nb + S) * a.idy2 - rhs) * a.inv_denom;
         const double v = o.pc + a.omega * (pnew - o.pc);
-        phi[act ? (size_t)j * Nx + (size_t)(i < 0 ? 0 : i) : dump] = v;
-        if (act) {
-            xs[par][j] = v;
-            if (j == 0) ring0[(int)(T & (kSorRing - 1))] = v;
-            if (j == Ny - 1) ringN[(int)(T & (kSorRing - 1))] = v;
-            out = v;
-            if (i == 0) first = v;
-            if (i == Nx - 1) wlast = v;
-        }
-        issue(ip, sp, o);  // operands of step T + 2
+        phi[act ? base + i : dump] = v;
+        xs[par][j] = v;
+        if (j == 0) ring0[T & (kSorRing - 1)] = v;
+        if (j == Ny - 1) ringN[T & (kSorRing - 1)] = v;
+        first = (i == 0) ? v : first;
+        wlast = (i == Nx - 1) ? v : wlast;
+        out = v;
+        issue(ip, sp, o);  // operands of step T + PD
         if (++i == c) { i = 0; ++s; }
         if (++ip == c) { ip = 0; ++sp; }
         __syncthreads();
     };
-    long long T = 0;
-    for (; T + 1 < steps; T += 2) {
-        step(T, q[0]);
-        step(T + 1, q[1]);
+    int T = 0;
+    for (; T + PD <= steps; T += PD) {
+#pragma unroll
+        for (int u = 0; u < PD; ++u) step(T + u, u & 1, q[u]);  // T is a multiple of PD (even)
     }
-    if (T < steps) step(T, q[0]);
+#pragma unroll
+    for (int u = 0; u < PD; ++u)
+        if (T + u < steps) step(T + u, u & 1, q[u]);
 }
 
 // ------------------------------------------------------------------ gradient, misc
@@ -555,8 +559,18 @@ cc_status Poisson2D::solve(const double* rho, double* gx, double* gy, hipStream_
         }
     } else if (iters > 0) {
         ProfScope ps(prof, s, "sor");
-        if (Ny <= 1024 && std::max(Nx, Ny) < kSorRing && std::getenv("CC_SOR_WAVEFRONT") == nullptr)
-            hipLaunchKernelGGL(k_sor_skew_rows, dim3(1), dim3((Ny + 63) / 64 * 64), 0, s, args, phi, rho, rhobar);
+        const int c = std::max(Nx, Ny);
+        const dim3 rows((Ny + 63) / 64 * 64);
+        const bool skew = Ny <= 1024 && c < kSorRing && (double)c * iters + Nx + Ny < 2.0e9 &&
+                          (double)Nx * Ny + 1024 < 2.0e9 && std::getenv("CC_SOR_WAVEFRONT") == nullptr;
+        // prefetch distance PD needs c >= 2 PD + 2; its ring fits the registers of <= 256 threads at
+        // PD = 8, of <= 1024 threads at PD = 4
+        if (skew && c >= 2 * 8 + 2 && Ny <= 256)
+            hipLaunchKernelGGL((k_sor_skew_rows<8, 256>), dim3(1), rows, 0, s, args, phi, rho, rhobar);
+        else if (skew && c >= 2 * 4 + 2)
+            hipLaunchKernelGGL((k_sor_skew_rows<4, 1024>), dim3(1), rows, 0, s, args, phi, rho, rhobar);
+        else if (skew && c >= 2 * 2 + 2)
+            hipLaunchKernelGGL((k_sor_skew_rows<2, 1024>), dim3(1), rows, 0, s, args, phi, rho, rhobar);
         else
             hipLaunchKernelGGL(k_sor_wavefront_1wg, dim3(1), dim3(1024), 0, s, args, phi, rho, rhobar);
         CC_CHECK_LAUNCH();

@@ -120,7 +120,7 @@ int main(int argc, char** argv) {
             double maxdiv = 0.0;
             check(cc_mhd2d_divb_max(h, &maxdiv), "cc_mhd2d_divb_max");
             std::cout << "t=" << std::setprecision(7) << t << " dt=" << dt << " max|divB|=" << std::setprecision(12)
-                      << maxdiv << "\n";
+                      << maxdiv << "\n" << std::flush;
             store(t);
             next_store += dt_store;
         }

@@ -121,7 +121,7 @@ int main(int argc, char** argv) {
                 std::fprintf(stderr, "%s\n", w.error().c_str());
                 return 1;
             }
-            std::cout << "Stored frame " << w.frames() - 1 << " at t=" << t << "\n";
+            std::cout << "Stored frame " << w.frames() - 1 << " at t=" << t << "\n" << std::flush;
             tnext += dt_store;
         }
     }

@@ -98,7 +98,7 @@ int main(int argc, char** argv) {
         if (t + 1e-14 >= next_store || t >= t_end - 1e-14) {
             store(t);
             next_store += dt_store;
-            std::cerr << "t=" << std::setprecision(10) << t << " dt=" << dt << " snapshots=" << w.frames() << "\n";
+            std::cerr << "t=" << std::setprecision(10) << t << " dt=" << dt << " snapshots=" << w.frames() << "\n" << std::flush;
         }
     }
     if (!w.finish({{"/x", centres(p.x0, p.x1, p.Nx)}, {"/y", centres(p.y0, p.y1, p.Ny)}})) {

@@ -1,7 +1,8 @@
-"""CPU proof of the GPU SOR schedule (core_collapse_amd/csrc/poisson2d.hip).
+"""CPU proof of the GPU SOR schedules (core_collapse_amd/csrc/poisson2d.hip).
 
 The tiled kernel claims to reproduce lexicographic Gauss-Seidel SOR (M2:207-218) with clamped
-boundaries.  Here the kernel's schedule is emulated cell by cell in Python (same tile geometry,
+boundaries; the periodic / mixed-boundary kernels (time skew c = max(Nx, Ny), one thread per row)
+claim the same with wrap-around neighbours (the last two tests).  Here the kernel's schedule is emulated cell by cell in Python (same tile geometry,
 same level order, same per-cell formula) and compared bit for bit with the plain lexicographic
 loop; a second check shows that no two tiles of one level touch a cell that the other writes,
 so the order in which the GPU runs the tiles of a level cannot change the result.
@@ -210,3 +211,144 @@ def test_ghost_margin_kernel_equals_lexicographic_bitwise(Nx, Ny, nsw, W, WJ, S)
     ref = lexicographic(rhs, nsw, idx2, idy2, 1.8)
     for rev in (False, True):
         assert np.array_equal(tiled_ghost(rhs, nsw, idx2, idy2, 1.8, W, WJ, S, reverse_levels=rev), ref)
+
+
+# ------------------------------------------------------------------ periodic / mixed boundaries
+# k_sor_wavefront_1wg and k_sor_skew_rows: cell (i, j) of sweep s at step i + j + c s, c = max(Nx, Ny).
+
+def lexicographic_bc(rhs, nsw, idx2, idy2, omega, px, py):
+    Ny, Nx = rhs.shape
+    phi = np.zeros_like(rhs)
+    inv = 1.0 / (2.0 * (idx2 + idy2))
+    bm = lambda i, n, p: (i % n) if p else min(max(i, 0), n - 1)  # noqa: E731
+    for _ in range(nsw):
+        for j in range(Ny):
+            for i in range(Nx):
+                pE, pW = phi[j, bm(i + 1, Nx, px)], phi[j, bm(i - 1, Nx, px)]
+                pN, pS = phi[bm(j + 1, Ny, py), i], phi[bm(j - 1, Ny, py), i]
+                pc = phi[j, i]
+                pnew = ((pE + pW) * idx2 + (pN + pS) * idy2 - rhs[j, i]) * inv
+                phi[j, i] = pc + omega * (pnew - pc)
+    return phi
+
+
+def skewed_bc(rhs, nsw, idx2, idy2, omega, px, py, order):
+    """The time-skewed schedule; within a step the cells are updated in `order` (0: rows up,
+    1: rows down, 2: all reads before all writes) -- equal results show they are independent."""
+    Ny, Nx = rhs.shape
+    phi = np.zeros_like(rhs)
+    inv = 1.0 / (2.0 * (idx2 + idy2))
+    bm = lambda i, n, p: (i % n) if p else min(max(i, 0), n - 1)  # noqa: E731
+    c = max(Nx, Ny)
+    for T in range(c * (nsw - 1) + Nx + Ny - 1):
+        pending = []
+        for j in (range(Ny) if order != 1 else range(Ny - 1, -1, -1)):
+            s, i = divmod(T - j, c)
+            if T - j < 0 or s >= nsw or i >= Nx:
+                continue
+            pE, pW = phi[j, bm(i + 1, Nx, px)], phi[j, bm(i - 1, Nx, px)]
+            pN, pS = phi[bm(j + 1, Ny, py), i], phi[bm(j - 1, Ny, py), i]
+            pc = phi[j, i]
+            v = pc + omega * (((pE + pW) * idx2 + (pN + pS) * idy2 - rhs[j, i]) * inv - pc)
+            if order == 2:
+                pending.append((j, i, v))
+            else:
+                phi[j, i] = v
+        for j, i, v in pending:
+            phi[j, i] = v
+    return phi
+
+
+def skew_rows(rhs, nsw, idx2, idy2, omega, px, py, PD, loads_first=False):
+    """k_sor_skew_rows step by step: one thread per row, west from the thread's previous result,
+    south through the double-buffered LDS exchange, periodic seams from registers and the rings of
+    rows 0 and Ny-1, the rest loaded PD steps ahead (a load sees memory as it is when issued).
+    loads_first issues a step's loads before the step's stores: the kernel orders them only within
+    a thread, so equal results show no load reads a cell stored in the same step."""
+    Ny, Nx = rhs.shape
+    N, RING = Nx * Ny, 2048
+    phi = np.zeros(N + 64)
+    r = rhs.reshape(-1)
+    inv = 1.0 / (2.0 * (idx2 + idy2))
+    c = max(Nx, Ny)
+    steps = c * (nsw - 1) + Nx + Ny - 1
+    xs = np.zeros((2, 64)); ring0 = np.zeros(RING); ringN = np.zeros(RING)
+    out = np.zeros(64); first = np.zeros(64); wlast = np.zeros(64)
+    pos = [[-j, 0] for j in range(64)]
+
+    def issue(j, ii, ss):
+        act = j < Ny and ss < nsw and 0 <= ii < Nx
+        k = j * Nx + (ii if act else 0)
+        dump = N + j
+        return (phi[k + 1 if act and ii + 1 < Nx else dump], phi[k + Nx if act and j + 1 < Ny else dump],
+                phi[k if act else dump], r[k if act else 0])
+
+    q = [[None] * 64 for _ in range(PD)]
+    pre = [[-j, 0] for j in range(64)]
+    for u in range(PD):
+        for j in range(64):
+            q[u][j] = issue(j, *pre[j])
+            pre[j][0] += 1
+            if pre[j][0] == c:
+                pre[j] = [0, pre[j][1] + 1]
+    for T in range(steps):
+        u, par = T % PD, T & 1
+        stores = []
+        for j in range(64):
+            i, s = pos[j]
+            E0, N0, pc, rr = q[u][j]
+            act = j < Ny and s < nsw and 0 <= i < Nx
+            Sr = ((0.0 if s == 0 else ringN[(T - (c - Ny + 1)) % RING]) if py else pc)
+            S = xs[par ^ 1][j - 1] if j > 0 else Sr
+            Nb = N0 if j < Ny - 1 else (ring0[(T - (Ny - 1)) % RING] if py else pc)
+            E = E0 if i < Nx - 1 else (first[j] if px else pc)
+            W = out[j] if i > 0 else ((0.0 if s == 0 else wlast[j]) if px else pc)
+            with np.errstate(all="ignore"):
+                v = pc + omega * (((E + W) * idx2 + (Nb + S) * idy2 - rr) * inv - pc)
+            stores.append((j * Nx + i if act else N + j, v))
+            xs[par][j] = v
+            if j == 0:
+                ring0[T % RING] = v
+            if j == Ny - 1:
+                ringN[T % RING] = v
+            first[j] = v if i == 0 else first[j]
+            wlast[j] = v if i == Nx - 1 else wlast[j]
+            out[j] = v
+        if not loads_first:
+            for k, v in stores:
+                phi[k] = v
+        for j in range(64):
+            q[u][j] = issue(j, *pre[j])
+            pre[j][0] += 1
+            if pre[j][0] == c:
+                pre[j] = [0, pre[j][1] + 1]
+            pos[j][0] += 1
+            if pos[j][0] == c:
+                pos[j] = [0, pos[j][1] + 1]
+        if loads_first:
+            for k, v in stores:
+                phi[k] = v
+    return phi[:N].reshape(Ny, Nx)
+
+
+@pytest.mark.parametrize("Nx,Ny,px,py,nsw", [(12, 12, 1, 1, 5), (10, 17, 1, 1, 4), (19, 8, 1, 1, 4), (15, 9, 1, 0, 4),
+                                             (9, 14, 0, 1, 4), (13, 13, 0, 0, 3), (2, 5, 1, 1, 3)])
+def test_time_skewed_periodic_schedule_equals_lexicographic_bitwise(Nx, Ny, px, py, nsw):
+    rng = np.random.default_rng(Nx * 31 + Ny)
+    rhs = 4.0 * np.pi * (rng.uniform(0, 1, (Ny, Nx)) - 0.5)
+    want = lexicographic_bc(rhs, nsw, float(Nx * Nx), float(Ny * Ny), 1.8, px, py)
+    for order in (0, 1, 2):
+        assert np.array_equal(skewed_bc(rhs, nsw, float(Nx * Nx), float(Ny * Ny), 1.8, px, py, order), want)
+
+
+@pytest.mark.parametrize("Nx,Ny,px,py,nsw,PD", [(18, 18, 1, 1, 4, 8), (20, 30, 1, 0, 3, 8), (40, 7, 1, 1, 3, 8),
+                                                (12, 12, 1, 1, 5, 2), (9, 14, 0, 1, 4, 4), (25, 19, 0, 0, 3, 8)])
+def test_row_thread_kernel_equals_lexicographic_bitwise(Nx, Ny, px, py, nsw, PD):
+    if max(Nx, Ny) < 2 * PD + 2:
+        pytest.skip("the kernel needs c >= 2 PD + 2")
+    rng = np.random.default_rng(Nx * 7 + Ny)
+    rhs = 4.0 * np.pi * (rng.uniform(0, 1, (Ny, Nx)) - 0.5)
+    want = lexicographic_bc(rhs, nsw, float(Nx * Nx), float(Ny * Ny), 1.8, px, py)
+    for loads_first in (False, True):
+        got = skew_rows(rhs, nsw, float(Nx * Nx), float(Ny * Ny), 1.8, px, py, PD, loads_first)
+        assert np.array_equal(got, want), loads_first
