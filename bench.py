@@ -54,7 +54,11 @@ def parse():
     ap.add_argument("--N", type=int, default=2048)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound on the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("all", "collapse2d", "turb3d"), default="all")
+    ap.add_argument("--workload", choices=("all", "collapse2d", "turb3d", "batch2d"), default="all")
+    ap.add_argument("--batch", type=int, default=64, help="instances of the batched collapse")
+    ap.add_argument("--steps-batch", type=int, default=200)
+    ap.add_argument("--seq-steps", type=int, default=10,
+                    help="steps per instance of the one-after-another comparison (0: skip)")
     ap.add_argument("--N3", type=int, default=256, help="grid of the 3D turbulence problem")
     ap.add_argument("--steps3", type=int, default=10)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -215,6 +219,55 @@ def bench3d(N, warmup, steps, with_cpu):
     return out
 
 
+def bench_batch2d(n_inst, steps, warmup, nmin, nmax, seq_steps):
+    """The batched collapse: n_inst independent instances from the batch sampler (grids log-uniform in
+    [nmin, nmax]^2, contrast, eta, B0 and IC seed per instance), mixed precision, all advanced together
+    by one graph launch per step (parallel branches, per-instance dt).  Also times each instance run
+    alone for seq_steps steps, to set the batch against running the instances one after another."""
+    from core_collapse_amd import CC_MIXED
+    from core_collapse_amd import ic
+    from core_collapse_amd import mhd2d as M
+    sample = ic.collapse2d_batch_sample(n_inst, 42, nmin, nmax)
+    sims = []
+    for e in sample:
+        p = ic.collapse2d_params(e["N"], eta=e["eta"], precision=CC_MIXED)
+        h = M.Mhd2D(p)
+        h.upload(ic.collapse2d_ic(p, **e["ic"]))
+        sims.append(h)
+    cells = float(sum(e["N"] ** 2 for e in sample))
+    b = M.Mhd2DBatch(sims)
+    t = b.run(warmup) if warmup > 0 else None
+    t0 = time.perf_counter()
+    t = b.run(steps, t)
+    elapsed = time.perf_counter() - t0
+    b.close()
+    seq = 0.0
+    if seq_steps > 0:
+        for h, ti in zip(sims, t):
+            h.run(1, ti)
+            h.sync()
+            t1 = time.perf_counter()
+            h.run(seq_steps, ti)
+            seq += (time.perf_counter() - t1) * steps / seq_steps
+    divb = max(h.divB_max() for h in sims)
+    for h in sims:
+        h.close()
+    zu = cells * steps
+    return {
+        "workload": "batch of %d 2D collapses (HLLD+CT+SOR gravity, clamped boundaries), grids log-uniform in "
+                    "[%d, %d]^2, contrast U[5,50], eta U[1e-5,1e-3], B0 U[0.01,0.1] along x, IC seed 42+i, "
+                    "mixed precision, per-instance dt" % (n_inst, nmin, nmax),
+        "instances": n_inst, "cells": cells, "grids": [e["N"] for e in sample],
+        "steps": steps, "warmup": warmup,
+        "value": zu / elapsed, "unit": "zone-updates/s", "ms_per_step": 1e3 * elapsed / steps,
+        "sequential_estimate": {"value": zu / seq if seq > 0 else None, "steps_timed": seq_steps,
+                                "ms_per_step": 1e3 * seq / steps if seq > 0 else None},
+        "batch_over_sequential": seq / elapsed if seq > 0 else None,
+        "sor_alg_GBps": cells * steps * 2 * 600 * 12 / elapsed / 1e9,
+        "max_divB": divb, "t_min": float(np.min(t)), "t_max": float(np.max(t)),
+    }
+
+
 def load_traffic(prefix):
     """Measured HBM bytes per launch of the first kernel whose name starts with `prefix`, from the
     rocprofv3 FETCH_SIZE/WRITE_SIZE summary in profiles/hbm_traffic.json (None if absent)."""
@@ -254,6 +307,15 @@ def main():
               file=sys.stderr, flush=True)
         if a.workload == "turb3d":
             print(json.dumps({"turb3d": t3d}))
+            return
+    bt = None
+    if a.workload in ("all", "batch2d"):
+        print("bench: batch of %d collapses, timed %d steps" % (a.batch, a.steps_batch), file=sys.stderr, flush=True)
+        bt = bench_batch2d(a.batch, a.steps_batch, 3, 128, 1024, a.seq_steps)
+        print("bench: batch2d %.3f ms/step (sequential %s ms/step)" % (bt["ms_per_step"],
+              bt["sequential_estimate"]["ms_per_step"]), file=sys.stderr, flush=True)
+        if a.workload == "batch2d":
+            print(json.dumps({"batch2d": bt}))
             return
     print("bench: %dx%d collapse, warmup %d, timed %d steps" % (N, N, W, K), file=sys.stderr, flush=True)
     res = {}
@@ -320,6 +382,7 @@ def main():
         "cpu_baseline": cpu,
         "kernel_times": kernels,
         "turb3d": t3d,
+        "batch2d": bt,
     }
     print(json.dumps(line))
 

@@ -186,6 +186,11 @@ _PROTOS = {
     "cc_mhd3d_ic_turbulence": (C.c_int, [C.POINTER(Mhd3dParams), C.POINTER(Turbulence3dIc)] + [DP] * 8),
     "cc_euler2d_ic_defaults": (None, [C.POINTER(Euler2dIc)]),
     "cc_euler2d_ic_reference": (C.c_int, [C.POINTER(Euler2dParams), C.POINTER(Euler2dIc)] + [DP] * 4),
+    "cc_mhd2d_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
+    "cc_mhd2d_batch_destroy": (C.c_int, [C.c_void_p]),
+    "cc_mhd2d_batch_run": (C.c_int, [C.c_void_p, C.c_int, DP, DP]),
+    "cc_collapse2d_batch_sample": (C.c_int, [C.c_uint, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                             C.POINTER(Collapse2dIc), DP]),
     "cc_ns2d_ic_defaults": (None, [C.POINTER(Ns2dIc)]),
     "cc_ns2d_ic_reference": (C.c_int, [C.POINTER(Ns2dParams), C.POINTER(Ns2dIc), DP]),
 }

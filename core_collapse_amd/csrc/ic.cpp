@@ -249,6 +249,23 @@ cc_status cc_mhd2d_ic_collapse(const cc_mhd2d_params* p, const cc_collapse2d_ic*
     return CC_OK;
 }
 
+cc_status cc_collapse2d_batch_sample(unsigned seed, int n, int nmin, int nmax, int* N, cc_collapse2d_ic* ic,
+                                     double* eta) {
+    if (n < 0 || nmin < 1 || nmax < nmin || !N || !ic || !eta) return bad("cc_collapse2d_batch_sample: bad arguments");
+    std::mt19937 rng(seed);
+    auto U = [&]() { return ((double)rng() + 0.5) / 4294967296.0; };
+    for (int i = 0; i < n; i++) {
+        const double uN = U(), uc = U(), ue = U(), uB = U();
+        N[i] = (int)std::lround(nmin * std::pow((double)nmax / nmin, uN));
+        cc_collapse2d_ic_defaults(&ic[i]);
+        ic[i].seed = 42u + (unsigned)i;
+        ic[i].contrast = 5.0 + 45.0 * uc;
+        ic[i].B0 = 0.01 + 0.09 * uB;
+        eta[i] = 1e-5 + (1e-3 - 1e-5) * ue;
+    }
+    return CC_OK;
+}
+
 // ================================================================== M3
 void cc_mhd3d_ic_defaults(cc_mhd3d_ic* k) {
     if (!k) return;

@@ -801,6 +801,122 @@ cc_status cc_mhd2d_reset_profiling(cc_mhd2d* h) {
     return CC_OK;
 }
 
+}  // extern "C"
+
+// ==================================================================== batches of instances
+struct cc_mhd2d_batch {
+    std::vector<cc_mhd2d*> h;
+    hipStream_t root = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};  // one step of every instance, by rotation phase
+
+    // One step of every instance as parallel branches of one graph: the root stream forks to the
+    // instances' streams and joins them back.
+    cc_status capture(int a) {
+        const int b = (a + 1) % 3, c = (a + 2) % 3;
+        CC_HIP(hipStreamBeginCapture(root, hipStreamCaptureModeGlobal));
+        cc_status st = CC_OK;
+        if (hipEventRecord(fork, root) != hipSuccess) st = CC_ERR_HIP;
+        for (size_t i = 0; i < h.size() && st == CC_OK; i++) {
+            if (hipStreamWaitEvent(h[i]->stream, fork, 0) != hipSuccess) { st = CC_ERR_HIP; break; }
+            st = h[i]->heun(a, b, c, true, h[i]->dts_dev);
+            if (st == CC_OK && (hipEventRecord(join[i], h[i]->stream) != hipSuccess ||
+                                hipStreamWaitEvent(root, join[i], 0) != hipSuccess))
+                st = CC_ERR_HIP;
+        }
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(root, &gr);
+        if (st != CC_OK || e != hipSuccess) {
+            if (gr) hipGraphDestroy(gr);
+            if (st == CC_OK) set_error(CC_ERR_HIP, "batch graph capture failed");
+            return st == CC_OK ? CC_ERR_HIP : st;
+        }
+        const hipError_t ei = hipGraphInstantiate(&graph[a], gr, nullptr, nullptr, 0);
+        hipGraphDestroy(gr);
+        if (ei != hipSuccess) {
+            set_error(CC_ERR_HIP, "batch graph instantiation failed");
+            return CC_ERR_HIP;
+        }
+        return CC_OK;
+    }
+    ~cc_mhd2d_batch() {
+        for (auto& g : graph)
+            if (g) hipGraphExecDestroy(g);
+        for (auto e : join)
+            if (e) hipEventDestroy(e);
+        if (fork) hipEventDestroy(fork);
+        if (root) hipStreamDestroy(root);
+    }
+};
+
+extern "C" {
+
+cc_status cc_mhd2d_batch_create(cc_mhd2d* const* handles, int n, cc_mhd2d_batch** out) {
+    if (!handles || n < 1 || !out) { set_error(CC_ERR_INVALID_ARG, "cc_mhd2d_batch_create: bad arguments"); return CC_ERR_INVALID_ARG; }
+    *out = nullptr;
+    for (int i = 0; i < n; i++)
+        if (!handles[i]) { set_error(CC_ERR_INVALID_ARG, "cc_mhd2d_batch_create: null handle"); return CC_ERR_INVALID_ARG; }
+    cc_mhd2d_batch* b = new cc_mhd2d_batch();
+    b->h.assign(handles, handles + n);
+    b->join.assign((size_t)n, nullptr);
+    bool ok = hipStreamCreateWithFlags(&b->root, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; i < n && ok; i++) ok = hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        delete b;
+        set_error(CC_ERR_HIP, "cc_mhd2d_batch_create: stream / event creation failed");
+        return CC_ERR_HIP;
+    }
+    *out = b;
+    return CC_OK;
+}
+
+cc_status cc_mhd2d_batch_destroy(cc_mhd2d_batch* b) {
+    if (!b) return CC_OK;
+    hipStreamSynchronize(b->root);
+    delete b;
+    return CC_OK;
+}
+
+cc_status cc_mhd2d_batch_run(cc_mhd2d_batch* b, int nsteps, double* t_inout, const double* t_end) {
+    if (!b || nsteps < 0 || !t_inout || !t_end) { set_error(CC_ERR_INVALID_ARG, "cc_mhd2d_batch_run: bad arguments"); return CC_ERR_INVALID_ARG; }
+    if (nsteps > kDtLog) { set_error(CC_ERR_INVALID_ARG, "cc_mhd2d_batch_run: at most 65536 steps per call"); return CC_ERR_INVALID_ARG; }
+    const int a0 = b->h[0]->cur;
+    for (auto* h : b->h)
+        if (h->cur != a0) { set_error(CC_ERR_INVALID_ARG, "cc_mhd2d_batch_run: instances at different rotation phases"); return CC_ERR_INVALID_ARG; }
+    if (nsteps == 0) return CC_OK;
+    // per-instance start: t, t_end, the signal speeds of the current state (as cc_mhd2d_run)
+    for (size_t i = 0; i < b->h.size(); i++) {
+        cc_mhd2d* h = b->h[i];
+        CC_HIP(hipStreamSynchronize(h->stream));
+        hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, h->stream, h->sc, 0.0, t_inout[i], t_end[i], 1);
+        CC_TRY(h->speed_max(h->S[h->cur]));
+        CC_HIP(hipEventRecord(b->join[i], h->stream));
+        CC_HIP(hipStreamWaitEvent(b->root, b->join[i], 0));
+    }
+    int cur = a0;
+    for (int s = 0; s < nsteps; ++s) {
+        if (!b->graph[cur]) CC_TRY(b->capture(cur));
+        CC_HIP(hipGraphLaunch(b->graph[cur], b->root));
+        cur = (cur + 2) % 3;
+    }
+    CC_HIP(hipStreamSynchronize(b->root));
+    cc_status st = CC_OK;
+    for (size_t i = 0; i < b->h.size(); i++) {
+        cc_mhd2d* h = b->h[i];
+        h->cur = cur;
+        Scalars sc;
+        CC_HIP(hipMemcpy(&sc, h->sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+        t_inout[i] = sc.t;
+        if (!std::isfinite(sc.dt) || sc.dt < 0.0) {
+            set_error(CC_ERR_BAD_DT, "non-finite or negative dt on device in batch instance " + std::to_string(i));
+            st = CC_ERR_BAD_DT;
+        }
+    }
+    return st;
+}
+
 cc_status cc_hlld2d_batch(int axis, int n, const double* WL, const double* WR, const double* Bn, double gamma,
                           double floor_rho, int precision, double* out) {
     if (n < 0 || (axis != 0 && axis != 1) || (n > 0 && (!WL || !WR || !Bn || !out))) {

@@ -71,6 +71,19 @@ def collapse2d_ic(p: Mhd2dParams, **knobs):
     return st
 
 
+def collapse2d_batch_sample(n=64, seed=42, nmin=128, nmax=1024):
+    """The batch of collapse instances (cc_collapse2d_batch_sample): n dicts {"N", "eta", "ic"} --
+    grid N log-uniform in [nmin, nmax], eta U[1e-5, 1e-3], IC knobs with contrast U[5, 50],
+    B0 U[0.01, 0.1] and seed 42 + i; the instance's parameters are collapse2d_params(N, eta=eta)."""
+    N = (C.c_int * n)()
+    knobs = (_lib.Collapse2dIc * n)()
+    eta = np.empty(n)
+    check(_lib.lib().cc_collapse2d_batch_sample(C.c_uint(seed), n, nmin, nmax, N, knobs, _p(eta)))
+    return [dict(N=int(N[i]), eta=float(eta[i]),
+                 ic=dict(seed=knobs[i].seed, contrast=knobs[i].contrast, width=knobs[i].width, P0=knobs[i].P0,
+                         vamp=knobs[i].vamp, B0=knobs[i].B0)) for i in range(n)]
+
+
 def _m3_state(p):
     Nx, Ny, Nz = p.Nx, p.Ny, p.Nz
     c = (Nz, Ny, Nx)

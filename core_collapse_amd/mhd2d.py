@@ -178,6 +178,35 @@ class Mhd2D:
         return n.value, ms.value
 
 
+class Mhd2DBatch:
+    """Independent Mhd2D instances stepped together: one HIP graph launch per step runs every
+    instance's Heun step as a parallel branch, each with its own dt (cc_mhd2d_batch_*)."""
+
+    def __init__(self, sims):
+        self.sims = list(sims)
+        self._b = C.c_void_p()
+        arr = (C.c_void_p * len(self.sims))(*[s._h for s in self.sims])
+        check(_lib.lib().cc_mhd2d_batch_create(arr, len(self.sims), C.byref(self._b)))
+
+    def run(self, nsteps, t=None, t_end=None):
+        n = len(self.sims)
+        tt = np.zeros(n) if t is None else np.array(t, dtype=np.float64)
+        te = np.full(n, 1e300) if t_end is None else np.broadcast_to(np.asarray(t_end, dtype=np.float64), (n,)).copy()
+        check(_lib.lib().cc_mhd2d_batch_run(self._b, int(nsteps), _p(tt), _p(te)))
+        return tt
+
+    def close(self):
+        if self._b:
+            _lib.lib().cc_mhd2d_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def hlld_batch(axis, WL, WR, Bn, gamma=5.0 / 3.0, floor_rho=1e-10, precision=CC_FP64):
     """hlld_x (axis 0, M2:372) / hlld_y (axis 1, M2:540) on n independent states (n x 8 arrays)."""
     WL = np.ascontiguousarray(WL, dtype=np.float64)

@@ -127,6 +127,19 @@ cc_status cc_mhd2d_totals(cc_mhd2d* h, double* out6);
  * 0 when sum |b| is 0), out[3f+2] = max |a-b|.  Syncs both handles. */
 cc_status cc_mhd2d_compare(cc_mhd2d* a, cc_mhd2d* b, double* out);
 
+/* Batches of independent 2D MHD runs (no reference counterpart: the benchmark's batch of collapse
+ * instances of uneven size).  The instances' Heun steps run as the parallel branches of one HIP
+ * graph -- one graph launch per step for the whole batch -- each with its own dt on the device.
+ * The handles stay owned by the caller; all must be at the same point of their state rotation
+ * (e.g. freshly uploaded, or stepped only through the batch) and must not be used by other calls
+ * while the batch runs. */
+typedef struct cc_mhd2d_batch cc_mhd2d_batch;
+cc_status cc_mhd2d_batch_create(cc_mhd2d* const* handles, int n, cc_mhd2d_batch** out);
+cc_status cc_mhd2d_batch_destroy(cc_mhd2d_batch* b);
+/* nsteps steps of every instance (the loop of cc_mhd2d_run: gravity, dt, clamp to t_end[i], Heun).
+ * t_inout[i] holds instance i's t on entry and return.  Syncs. */
+cc_status cc_mhd2d_batch_run(cc_mhd2d_batch* b, int nsteps, double* t_inout, const double* t_end);
+
 /* Batched HLLD fluxes on independent face states (hlld_x M2:372-530 for axis 0, hlld_y
  * M2:540-561 for axis 1), ideal EOS.  W* are n x 8 {rho,u,v,w,P,Bx,By,Bz}; out is n x 8:
  * axis 0 {frho,fmx,fmy,fmz,fE,fBy,fBz,Ez}, axis 1 {frho,fmx,fmy,fmz,fE,fBx,fBz,Ez}. */
@@ -339,6 +352,13 @@ typedef struct cc_collapse2d_ic {
 void cc_collapse2d_ic_defaults(cc_collapse2d_ic* k); /* seed 42, contrast 10, width 0.1, P0 0.1, vamp 1e-3, B0 0.05 */
 cc_status cc_mhd2d_ic_collapse(const cc_mhd2d_params* p, const cc_collapse2d_ic* k, double* rho, double* mx, double* my,
                                double* mz, double* E, double* Bz, double* Bx_f, double* By_f);
+
+/* The batch of collapse instances (no reference counterpart): for instance i, from std::mt19937(seed)
+ * uniform draws u = (r + 0.5) / 2^32 in the order u_N, u_c, u_eta, u_B: grid N[i] = round(nmin *
+ * (nmax / nmin)^u_N) (log-uniform), contrast 5 + 45 u_c, eta[i] = 1e-5 + (1e-3 - 1e-5) u_eta,
+ * B0 = 0.01 + 0.09 u_B, IC seed 42 + i, width 0.1, P0 0.1, vamp 1e-3. */
+cc_status cc_collapse2d_batch_sample(unsigned seed, int n, int nmin, int nmax, int* N, cc_collapse2d_ic* ic,
+                                     double* eta);
 
 /* IC knobs of M3 (Params M3:53, 76-88). */
 typedef struct cc_mhd3d_ic {

@@ -221,3 +221,45 @@ def test_mixed_vs_fp64_l1_density():
             out[prec] = h.download()
     e = rel_l1(out[CC_MIXED]["rho"], out[CC_FP64]["rho"])
     assert e <= 1e-4, e
+
+
+@pytest.mark.parametrize("prec", [CC_FP64, CC_MIXED])
+def test_batch_equals_instances_run_alone(prec):
+    """cc_mhd2d_batch_run: independent collapse instances of uneven sizes, stepped as parallel
+    branches of one graph with per-instance dt, give bit for bit the times, dts and states of the
+    same instances run one by one -- over two calls (the second continuing the first), with one
+    instance stopping at its own t_end inside the run."""
+    from core_collapse_amd.mhd2d import Mhd2D, Mhd2DBatch
+    sample = ic.collapse2d_batch_sample(5, seed=9, nmin=24, nmax=80)
+
+    def make():
+        sims = []
+        for e in sample:
+            p = ic.collapse2d_params(e["N"], eta=e["eta"], precision=prec, sor_iters=80)
+            h = Mhd2D(p)
+            h.upload(ic.collapse2d_ic(p, **e["ic"]))
+            sims.append(h)
+        return sims
+
+    alone = make()
+    p2 = alone[2].params
+    dt0 = oracle().or_m2_compute_dt(C.byref(m2_oparams(p2)), C.byref(m2_arrays(p2, alone[2].download()).cstate()),
+                                    C.c_double(p2.dx), C.c_double(p2.dy))
+    t_end = np.full(len(sample), 1e300)
+    t_end[2] = 8.5 * dt0   # reached within the 19 steps
+    t_alone = np.array([h.run(12, 0.0, t_end[i]) for i, h in enumerate(alone)])
+    t_alone = np.array([h.run(7, t_alone[i], t_end[i]) for i, h in enumerate(alone)])
+    batch_sims = make()
+    b = Mhd2DBatch(batch_sims)
+    print("batch sizes", [e["N"] for e in sample], flush=True)
+    t = b.run(12, None, t_end)
+    t = b.run(7, t, t_end)
+    b.close()
+    assert np.array_equal(t, t_alone), (t, t_alone)
+    assert abs(t[2] - t_end[2]) <= 1e-15 * t_end[2]
+    for h1, h2 in zip(alone, batch_sims):
+        s1, s2 = h1.download(), h2.download()
+        for f in M.FIELDS:
+            assert np.array_equal(s1[f], s2[f]), f
+    for h in alone + batch_sims:
+        h.close()
