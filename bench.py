@@ -54,7 +54,9 @@ def parse():
     ap.add_argument("--N", type=int, default=2048)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound on the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("all", "collapse2d", "turb3d", "batch2d"), default="all")
+    ap.add_argument("--workload", choices=("all", "collapse2d", "turb3d", "batch2d", "sphere3d"), default="all")
+    ap.add_argument("--N5", type=int, default=512, help="grid of the 3D Euler sphere problem")
+    ap.add_argument("--steps5", type=int, default=10)
     ap.add_argument("--batch", type=int, default=64, help="instances of the batched collapse")
     ap.add_argument("--steps-batch", type=int, default=200)
     ap.add_argument("--seq-steps", type=int, default=10,
@@ -268,6 +270,96 @@ def bench_batch2d(n_inst, steps, warmup, nmin, nmax, seq_steps):
     }
 
 
+def run_gpu_sphere(N, precision, warmup, steps):
+    from core_collapse_amd import euler3d as E3
+    from core_collapse_amd import ic
+    p = ic.sphere3d_params(N, precision=precision)
+    h = E3.Euler3D(p)
+    h.upload(ic.sphere3d_ic(p))
+    t = h.run(warmup) if warmup > 0 else 0.0
+    h.set_profiling(True)
+    h.reset_profiling()
+    h.sync()
+    t0 = time.perf_counter()
+    t = h.run(steps, t)
+    elapsed = time.perf_counter() - t0
+    prof = {k: h.kernel_time(k) for k in ("prim", "face", "update", "mean", "fft_x", "fft_y", "fft_z", "check",
+                                          "grad", "dt")}
+    h.set_profiling(False)
+    stats = h.poisson_stats()
+    out = h.download()
+    tot = h.totals()
+    h.close()
+    return {"elapsed": elapsed, "rho": out["rho"], "prof": prof, "poisson": stats, "mass": tot["mass"]}
+
+
+def cpu_baseline_sphere(n=32, budget_s=12.0):
+    """The oracle's restatement of the 3D Euler + spectral gravity step on a 32^3 sample."""
+    from core_collapse_amd import ic
+    from oracle import bind as OB
+    lib = OB.Lib("oracle")
+    p = ic.sphere3d_params(n)
+    st = ic.sphere3d_ic(p)
+    op = OB.e3_params(Nx=n, Ny=n, Nz=n, cfl=p.cfl, gamma=p.gamma, G=p.G, gravity_on=1)
+    S = OB.E3Arrays(n, n, n, st)
+    steps, t, tsum = 0, 0.0, 0.0
+    while steps < 3 and (steps == 0 or tsum < budget_s):
+        t0 = time.perf_counter()
+        t = lib.or_e3_run(C.byref(op), C.byref(S.cstate()), 1, C.c_double(t), C.c_double(1e300), None)
+        tsum += time.perf_counter() - t0
+        steps += 1
+    return {"value": n ** 3 * steps / tsum, "unit": "zone-updates/s", "cores": 1, "kind": "port",
+            "sample": "%d Heun step(s) of the same sphere problem at %d^3 (FP64, exact spectral gravity, %.1f s), "
+                      "single thread, oracle restatement (no reference program has this step)" % (steps, n, tsum)}
+
+
+def bench_sphere3d(N, warmup, steps, with_cpu):
+    """3D Euler HLLC + FFT self-gravity, uniform sphere, FP64 and mixed (FP32 primitives, fluxes and FFT;
+    FP64 state, update and Poisson residual)."""
+    from core_collapse_amd import CC_FP64, CC_MIXED
+    res = {prec: run_gpu_sphere(N, prec, warmup, steps) for prec in (CC_FP64, CC_MIXED)}
+    zu = float(N) ** 3 * steps
+    cells = float(N) ** 3
+
+    def roof(r, word):
+        n, ms = r["prof"]["face"]
+        if n == 0 or ms <= 0:
+            return None
+        # one "face" record = the three axis launches; per face: read 5 primitives, write 5 fluxes
+        bpl = cells * 3 * 10 * word
+        avg = ms / n * 1e-3
+        return {"bound": "hbm", "achieved": round(bpl / avg / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(bpl / avg / 1e9 / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_e3_face (x, y, z)",
+                "launches": 3 * n, "avg_launch_us": round(avg * 1e6 / 3, 2), "alg_bytes_per_launch": bpl / 3}
+
+    def balg(r):
+        k = max(r["poisson"]["iters_last"], 1)
+        return 392 + 192 * k
+
+    a, b = res[CC_MIXED]["rho"], res[CC_FP64]["rho"]
+    out = {
+        "workload": "3D Euler HLLC + FFT Poisson self-gravity, periodic %d^3, gamma 5/3, G 1, uniform sphere rho 10 "
+                    "(r < 0.2) in 0.1, 1%% seeded noise, p 0.01, at rest; FFT refined in FP64 to 1e-10" % N,
+        "steps": steps, "warmup": warmup,
+        "value": zu / res[CC_MIXED]["elapsed"], "unit": "zone-updates/s",
+        "ms_per_step": 1e3 * res[CC_MIXED]["elapsed"] / steps,
+        "fp64": {"value": zu / res[CC_FP64]["elapsed"], "ms_per_step": 1e3 * res[CC_FP64]["elapsed"] / steps,
+                 "roofline": roof(res[CC_FP64], 8), "poisson": res[CC_FP64]["poisson"]},
+        "mixed_over_fp64": res[CC_FP64]["elapsed"] / res[CC_MIXED]["elapsed"],
+        "l1_rho_rel_err": float(np.abs(a - b).sum() / np.abs(b).sum()),
+        "linf_rho_rel_err": float(np.abs(a - b).max() / np.abs(b).max()),
+        "roofline": roof(res[CC_MIXED], 4),
+        "poisson": res[CC_MIXED]["poisson"],
+        "zone_update_alg_bytes": balg(res[CC_MIXED]),
+        "kernel_times": {tag: {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in res[prec]["prof"].items()
+                               if v[0] > 0} for prec, tag in ((CC_FP64, "fp64"), (CC_MIXED, "mixed"))},
+    }
+    out["hbm_frac_alg"] = round(out["value"] * balg(res[CC_MIXED]) / 1e9 / HBM_PEAK_GBS, 4)
+    if with_cpu:
+        out["cpu_baseline"] = cpu_baseline_sphere()
+    return out
+
+
 def load_traffic(prefix):
     """Measured HBM bytes per launch of the first kernel whose name starts with `prefix`, from the
     rocprofv3 FETCH_SIZE/WRITE_SIZE summary in profiles/hbm_traffic.json (None if absent)."""
@@ -307,6 +399,15 @@ def main():
               file=sys.stderr, flush=True)
         if a.workload == "turb3d":
             print(json.dumps({"turb3d": t3d}))
+            return
+    sp = None
+    if a.workload in ("all", "sphere3d"):
+        print("bench: %d^3 sphere, warmup %d, timed %d steps" % (a.N5, 2, a.steps5), file=sys.stderr, flush=True)
+        sp = bench_sphere3d(a.N5, 2, a.steps5, not a.no_cpu)
+        print("bench: sphere3d mixed %.3f ms/step, fp64 %.3f ms/step" % (sp["ms_per_step"], sp["fp64"]["ms_per_step"]),
+              file=sys.stderr, flush=True)
+        if a.workload == "sphere3d":
+            print(json.dumps({"sphere3d": sp}))
             return
     bt = None
     if a.workload in ("all", "batch2d"):
@@ -383,6 +484,7 @@ def main():
         "kernel_times": kernels,
         "turb3d": t3d,
         "batch2d": bt,
+        "sphere3d": sp,
     }
     print(json.dumps(line))
 

@@ -59,6 +59,17 @@ class Ns2dParams(C.Structure):
                 ("precision", C.c_int)]
 
 
+class Euler3dParams(C.Structure):
+    _fields_ = [("Nx", C.c_int), ("Ny", C.c_int), ("Nz", C.c_int)] + \
+               [(n, C.c_double) for n in ("x0", "x1", "y0", "y1", "z0", "z1", "gamma", "cfl", "floor_rho", "floor_P")] + \
+               [("gravity_on", C.c_int), ("G", C.c_double), ("poisson_tol", C.c_double),
+                ("poisson_max_iter", C.c_int), ("precision", C.c_int)]
+
+
+class Sphere3dIc(C.Structure):
+    _fields_ = [("seed", C.c_uint)] + [(n, C.c_double) for n in ("rho_in", "rho_out", "radius", "noise", "P0")]
+
+
 class Mhd2dIc(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("rho_bg", "rho_core", "r_core", "r_edge", "P0", "vin", "rot_omega",
                                            "B0x", "Bshear", "gamma_eff")]
@@ -184,6 +195,24 @@ _PROTOS = {
     "cc_mhd3d_ic_reference": (C.c_int, [C.POINTER(Mhd3dParams), C.POINTER(Mhd3dIc)] + [DP] * 8),
     "cc_turbulence3d_ic_defaults": (None, [C.POINTER(Turbulence3dIc)]),
     "cc_mhd3d_ic_turbulence": (C.c_int, [C.POINTER(Mhd3dParams), C.POINTER(Turbulence3dIc)] + [DP] * 8),
+    "cc_euler3d_create": (C.c_int, [C.POINTER(Euler3dParams), C.POINTER(C.c_void_p)]),
+    "cc_euler3d_destroy": (C.c_int, [C.c_void_p]),
+    "cc_euler3d_upload": (C.c_int, [C.c_void_p] + [DP] * 5),
+    "cc_euler3d_download": (C.c_int, [C.c_void_p] + [DP] * 5),
+    "cc_euler3d_gravity": (C.c_int, [C.c_void_p] + [DP] * 4),
+    "cc_euler3d_poisson_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                           DP, DP, C.POINTER(C.c_int)]),
+    "cc_euler3d_rhs": (C.c_int, [C.c_void_p] + [DP] * 5),
+    "cc_euler3d_compute_dt": (C.c_int, [C.c_void_p, DP]),
+    "cc_euler3d_rk2_step": (C.c_int, [C.c_void_p, C.c_double]),
+    "cc_euler3d_run": (C.c_int, [C.c_void_p, C.c_int, DP, C.c_double, DP]),
+    "cc_euler3d_totals": (C.c_int, [C.c_void_p, DP]),
+    "cc_euler3d_sync": (C.c_int, [C.c_void_p]),
+    "cc_euler3d_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "cc_euler3d_kernel_time": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int), DP]),
+    "cc_euler3d_reset_profiling": (C.c_int, [C.c_void_p]),
+    "cc_sphere3d_ic_defaults": (None, [C.POINTER(Sphere3dIc)]),
+    "cc_euler3d_ic_sphere": (C.c_int, [C.POINTER(Euler3dParams), C.POINTER(Sphere3dIc)] + [DP] * 5),
     "cc_euler2d_ic_defaults": (None, [C.POINTER(Euler2dIc)]),
     "cc_euler2d_ic_reference": (C.c_int, [C.POINTER(Euler2dParams), C.POINTER(Euler2dIc)] + [DP] * 4),
     "cc_mhd2d_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
@@ -236,6 +265,7 @@ TOTALS = {
     "mhd3d": ("mass", "momx", "momy", "momz", "energy", "divB_max"),
     "euler2d": ("mass", "momx", "momy", "energy"),
     "ns2d": ("mass", "momx", "momy", "energy"),
+    "euler3d": ("mass", "momx", "momy", "momz", "energy"),
 }
 
 

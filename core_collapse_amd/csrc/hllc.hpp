@@ -90,4 +90,54 @@ __host__ __device__ __forceinline__ void hllc_ns(R gamma, R floor_rho, const Hyd
     out[3] = F3 + S * (Es - E);
 }
 
+// The EH HLLC with two transverse velocities, for the 3D Euler step along axis AX (0, 1, 2) on
+// primitives (rho, u, v, w, P); out = {f_rho, f_mx, f_my, f_mz, f_E}.  Kinetic energy
+// 0.5 rho ((u^2 + v^2) + w^2) and the cross momentum flux rho*a*b (a < b in the order u, v, w), so
+// that with w = 0 on x / y faces every value equals hllc_eh's bit for bit.
+template <typename R, int AX>
+__host__ __device__ __forceinline__ void hllc_e3(R gamma, R floor_rho, const R L[5], const R Rt[5], R out[5]) {
+    const R gm1 = gamma - R(1);
+    const R unL = L[1 + AX], unR = Rt[1 + AX];
+    const R aL = sqrt(gamma * L[4] / smax(L[0], floor_rho));
+    const R aR = sqrt(gamma * Rt[4] / smax(Rt[0], floor_rho));
+    const R SL = smin(unL - aL, unR - aR);
+    const R SR = smax(unL + aL, unR + aR);
+    const R rL = smax(L[0], floor_rho), rR = smax(Rt[0], floor_rho);
+    const R EL = R(0.5) * rL * ((L[1] * L[1] + L[2] * L[2]) + L[3] * L[3]) + L[4] / gm1;
+    const R ER = R(0.5) * rR * ((Rt[1] * Rt[1] + Rt[2] * Rt[2]) + Rt[3] * Rt[3]) + Rt[4] / gm1;
+    const R num = (Rt[4] - L[4]) + rL * unL * (SL - unL) - rR * unR * (SR - unR);
+    const R den = rL * (SL - unL) - rR * (SR - unR);
+    const R SM = (fabs(den) > R(1e-14)) ? (num / den) : R(0);
+    const bool up_left = R(0) <= SL, up_right = R(0) >= SR, star_left = R(0) < SM;
+    const bool use_left = up_left || (!up_right && star_left);
+    const R* W = use_left ? L : Rt;
+    const R rho = W[0], un = W[1 + AX], P = W[4];
+    const R Ef = P / gm1 + R(0.5) * rho * ((W[1] * W[1] + W[2] * W[2]) + W[3] * W[3]);
+    R F[5];
+    F[0] = rho * un;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (c == AX) F[1 + c] = rho * un * un + P;
+        else if (c < AX) F[1 + c] = rho * W[1 + c] * un;
+        else F[1 + c] = rho * un * W[1 + c];
+    }
+    F[4] = (Ef + P) * un;
+    if (up_left || up_right) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) out[c] = F[c];
+        return;
+    }
+    const R S = use_left ? SL : SR;
+    const R c0 = use_left ? rL : rR, cE = use_left ? EL : ER;
+    const R rS = c0 * (S - un) / (S - SM);
+    const R ES = ((S - un) * cE - P * un + P * SM) / (S - SM);
+    out[0] = F[0] + S * (rS - c0);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        const R mS = (m == AX) ? rS * SM : rS * W[1 + m];
+        out[1 + m] = F[1 + m] + S * (mS - c0 * W[1 + m]);
+    }
+    out[4] = F[4] + S * (ES - cE);
+}
+
 }  // namespace cc

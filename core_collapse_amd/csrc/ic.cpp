@@ -266,6 +266,43 @@ cc_status cc_collapse2d_batch_sample(unsigned seed, int n, int nmin, int nmax, i
     return CC_OK;
 }
 
+// ================================================================== 3D Euler uniform sphere
+void cc_sphere3d_ic_defaults(cc_sphere3d_ic* k) {
+    if (!k) return;
+    *k = cc_sphere3d_ic{42u, 10.0, 0.1, 0.2, 0.01, 0.01};
+}
+
+cc_status cc_euler3d_ic_sphere(const cc_euler3d_params* p, const cc_sphere3d_ic* kn, double* rho, double* mx,
+                               double* my, double* mz, double* E) {
+    if (!p || !kn || !rho || !mx || !my || !mz || !E || p->Nx < 1 || p->Ny < 1 || p->Nz < 1 || !(p->gamma > 1.0))
+        return bad("cc_euler3d_ic_sphere: bad parameters or null output");
+    const int Nx = p->Nx, Ny = p->Ny, Nz = p->Nz;
+    const double dx = (p->x1 - p->x0) / Nx, dy = (p->y1 - p->y0) / Ny, dz = (p->z1 - p->z0) / Nz;
+    const double cx = 0.5 * (p->x0 + p->x1), cy = 0.5 * (p->y0 + p->y1), cz = 0.5 * (p->z0 + p->z1);
+    std::mt19937 rng(kn->seed);
+    double z1 = 0.0;
+    const size_t n = (size_t)Nx * Ny * Nz;
+    for (size_t q = 0; q < n; q++) {
+        const int i = (int)(q % Nx), j = (int)((q / Nx) % Ny), k = (int)(q / ((size_t)Nx * Ny));
+        double z;
+        if ((q & 1) == 0) {  // a new Box-Muller pair: z0 now, z1 for the next cell
+            const double u1 = ((double)rng() + 0.5) / 4294967296.0;
+            const double u2 = ((double)rng() + 0.5) / 4294967296.0;
+            const double rad = std::sqrt(-2.0 * std::log(u1));
+            z = rad * std::cos(2.0 * kPi * u2);
+            z1 = rad * std::sin(2.0 * kPi * u2);
+        } else {
+            z = z1;
+        }
+        const double X = p->x0 + (i + 0.5) * dx - cx, Y = p->y0 + (j + 0.5) * dy - cy, Z = p->z0 + (k + 0.5) * dz - cz;
+        const double r = std::sqrt(X * X + Y * Y + Z * Z);
+        rho[q] = (r < kn->radius ? kn->rho_in : kn->rho_out) * (1.0 + kn->noise * z);
+        mx[q] = 0.0; my[q] = 0.0; mz[q] = 0.0;
+        E[q] = kn->P0 / (p->gamma - 1.0);
+    }
+    return CC_OK;
+}
+
 // ================================================================== M3
 void cc_mhd3d_ic_defaults(cc_mhd3d_ic* k) {
     if (!k) return;

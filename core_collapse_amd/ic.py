@@ -144,3 +144,25 @@ def ns2d_reference_ic(p=None, **knobs):
     ps = p.c_struct()
     check(_lib.lib().cc_ns2d_ic_reference(C.byref(ps), C.byref(k), _p(U)))
     return U
+
+
+def sphere3d_params(N=512, precision=0, **kw):
+    """The 3D Euler self-gravity benchmark: periodic [0,1)^3, gamma 5/3, G 1, CFL 0.4, FFT gravity
+    refined in FP64 to a relative residual of 1e-10."""
+    from .euler3d import Params as E3
+    d = dict(Nx=N, Ny=N, Nz=N, gamma=5.0 / 3.0, G=1.0, cfl=0.4, poisson_tol=1e-10, poisson_max_iter=8,
+             precision=precision)
+    d.update(kw)
+    return E3(**d)
+
+
+def sphere3d_ic(p, **knobs):
+    """Uniform sphere (cc_euler3d_ic_sphere): rho = rho_in (10) inside radius (0.2) of the box centre,
+    rho_out (0.1) outside, times (1 + noise N(0,1)) with noise 0.01 from std::mt19937(seed = 42), p = P0
+    (0.01), at rest.  Returns SoA arrays of shape (Nz, Ny, Nx)."""
+    k = _knobs(_lib.Sphere3dIc, "cc_sphere3d_ic_defaults", knobs)
+    sh = (p.Nz, p.Ny, p.Nx)
+    st = {f: np.empty(sh) for f in ("rho", "mx", "my", "mz", "E")}
+    ps = p.c_struct()
+    check(_lib.lib().cc_euler3d_ic_sphere(C.byref(ps), C.byref(k), *[_p(st[f]) for f in ("rho", "mx", "my", "mz", "E")]))
+    return st

@@ -325,6 +325,52 @@ cc_status cc_ns2d_compare(cc_ns2d* a, cc_ns2d* b, double* out);
 cc_status cc_fft2d(int Nx, int Ny, int inverse, double* data);
 
 /* ============================================================================================
+ * 3D Euler HLLC + FFT self-gravity, periodic (the uniform-sphere collapse benchmark).  No reference
+ * program has this step: it is the EH step (EH:406-550: floored primitives, 4-point MUSCL-MC, the
+ * EH HLLC, flux divergence + rho g source, Heun with floors) extended to three axes, with gravity
+ * from the 7-point discrete Poisson equation solved by FFT and FP64 iterative refinement.  The
+ * state layout is EH's SoA extended by mz: cell q = (k*Ny + j)*Nx + i.
+ * ========================================================================================== */
+typedef struct cc_euler3d_params {
+    int Nx, Ny, Nz; /* powers of two in [8, 1024] (-> CC_ERR_NON_POW2) */
+    double x0, x1, y0, y1, z0, z1;
+    double gamma, cfl, floor_rho, floor_P;
+    int gravity_on;
+    double G;
+    double poisson_tol;   /* stop refining when max|rhs - L_h phi| <= tol * max|rhs| */
+    int poisson_max_iter; /* at most this many corrections per solve, in [0, 64] */
+    int precision;        /* CC_FP64: everything FP64; CC_MIXED: FP32 primitives / fluxes / FFT */
+} cc_euler3d_params;
+
+typedef struct cc_euler3d cc_euler3d;
+
+cc_status cc_euler3d_create(const cc_euler3d_params* p, cc_euler3d** out);
+cc_status cc_euler3d_destroy(cc_euler3d* h);
+cc_status cc_euler3d_upload(cc_euler3d* h, const double* rho, const double* mx, const double* my, const double* mz,
+                            const double* E);
+cc_status cc_euler3d_download(cc_euler3d* h, double* rho, double* mx, double* my, double* mz, double* E);
+/* Gravity of the current state (any output may be null; phi has zero mean).  Syncs. */
+cc_status cc_euler3d_gravity(cc_euler3d* h, double* gx, double* gy, double* gz, double* phi);
+/* Refinement record over all solves so far: solves run, corrections of the last solve and the most
+ * any needed, the last and worst final relative residual, solves that stopped at max_iter. */
+cc_status cc_euler3d_poisson_stats(cc_euler3d* h, int* solves, int* iters_last, int* iters_max, double* resid_last,
+                                   double* resid_worst, int* unconverged);
+/* rhs (gravity included) of the current state.  Syncs. */
+cc_status cc_euler3d_rhs(cc_euler3d* h, double* drho, double* dmx, double* dmy, double* dmz, double* dE);
+cc_status cc_euler3d_compute_dt(cc_euler3d* h, double* dt);
+cc_status cc_euler3d_rk2_step(cc_euler3d* h, double dt);
+/* nsteps of the driver loop (dt on the device, clamp to t_end, Heun), graph-captured.  Syncs. */
+cc_status cc_euler3d_run(cc_euler3d* h, int nsteps, double* t_inout, double t_end, double* dts);
+/* {mass, x-, y-, z-momentum, total energy} * dx*dy*dz, reduced on the device.  Syncs. */
+cc_status cc_euler3d_totals(cc_euler3d* h, double* out5);
+cc_status cc_euler3d_sync(cc_euler3d* h);
+/* Per-kernel HIP-event timing ("prim", "face", "update", "mean", "fft_x", "fft_y", "fft_z",
+ * "check", "grad", "dt"). */
+cc_status cc_euler3d_set_profiling(cc_euler3d* h, int on);
+cc_status cc_euler3d_kernel_time(cc_euler3d* h, const char* kernel, int* count, double* total_ms);
+cc_status cc_euler3d_reset_profiling(cc_euler3d* h);
+
+/* ============================================================================================
  * Initial conditions (host only: no device is needed).  The reference builds its ICs inline in
  * main(); these functions restate them, with their knobs gathered in structs whose defaults
  * (cc_*_ic_defaults) are the reference values, and add the benchmark-problem ICs.  Output arrays
@@ -399,6 +445,18 @@ typedef struct cc_ns2d_ic {
 } cc_ns2d_ic;
 void cc_ns2d_ic_defaults(cc_ns2d_ic* k);
 cc_status cc_ns2d_ic_reference(const cc_ns2d_params* p, const cc_ns2d_ic* k, double* U);
+
+/* The uniform sphere of the 3D Euler benchmark: rho = rho_in inside |x - centre| < radius, rho_out
+ * outside, times (1 + noise z) with one standard normal z per cell (Box-Muller pairs on
+ * std::mt19937(seed), u = (r + 0.5)/2^32, cells in storage order take z0, z1 of successive pairs);
+ * p = P0, at rest. */
+typedef struct cc_sphere3d_ic {
+    unsigned seed;
+    double rho_in, rho_out, radius, noise, P0;
+} cc_sphere3d_ic;
+void cc_sphere3d_ic_defaults(cc_sphere3d_ic* k);
+cc_status cc_euler3d_ic_sphere(const cc_euler3d_params* p, const cc_sphere3d_ic* k, double* rho, double* mx,
+                               double* my, double* mz, double* E);
 
 /* ============================================================================================
  * Measurement helper (no reference counterpart): a plain device-to-device copy of nbytes,

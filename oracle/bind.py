@@ -58,6 +58,16 @@ class NSParams(C.Structure):
                 ("mu", C.c_double), ("cfl_visc", C.c_double), ("gravity_on", C.c_int), ("G", C.c_double)]
 
 
+class E3Params(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("gamma", "cfl", "x0", "x1", "y0", "y1", "z0", "z1")] + \
+               [("Nx", C.c_int), ("Ny", C.c_int), ("Nz", C.c_int), ("floor_rho", C.c_double),
+                ("floor_P", C.c_double), ("gravity_on", C.c_int), ("G", C.c_double)]
+
+
+class E3State(C.Structure):
+    _fields_ = [(n, DP) for n in ("rho", "mx", "my", "mz", "E")]
+
+
 def m2_params(**kw):
     """Reference defaults of M2:44-86 (mhd2d_resistive_hlld_ct_gravity.cpp), overridable."""
     d = dict(gamma=5.0 / 3.0, cfl=0.45, x0=0.0, x1=1.0, y0=0.0, y1=1.0, Nx=192, Ny=192, floor_rho=1e-10,
@@ -93,6 +103,14 @@ def ns_params(**kw):
     return NSParams(**d)
 
 
+def e3_params(**kw):
+    """The uniform-sphere benchmark's parameters (periodic [0,1)^3, gamma 5/3, G 1, CFL 0.4)."""
+    d = dict(gamma=5.0 / 3.0, cfl=0.4, x0=0.0, x1=1.0, y0=0.0, y1=1.0, z0=0.0, z1=1.0, Nx=32, Ny=32, Nz=32,
+             floor_rho=1e-10, floor_P=1e-10, gravity_on=1, G=1.0)
+    d.update(kw)
+    return E3Params(**d)
+
+
 def ptr(a):
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(DP)
@@ -116,7 +134,8 @@ class Lib:
                 getattr(self.lib, name).restype = C.c_double
         for name in ("or_m2_fast_speed_x", "or_m2_fast_speed_y", "or_m2_compute_dt", "or_m2_run",
                      "or_m3_sound_speed2", "or_m3_total_energy", "or_m3_compute_dt", "or_m3_run",
-                     "or_eh_compute_dt", "or_eh_run", "or_ns_compute_dt", "or_ns_run"):
+                     "or_eh_compute_dt", "or_eh_run", "or_ns_compute_dt", "or_ns_run", "or_e3_compute_dt",
+                     "or_e3_run"):
             try:
                 getattr(self.lib, name).restype = C.c_double
             except AttributeError:
@@ -195,6 +214,29 @@ class EHArrays:
 
     def cstate(self):
         return EHState(*[ptr(self.a[f]) for f in self.FIELDS])
+
+    def __getitem__(self, k):
+        return self.a[k]
+
+
+class E3Arrays:
+    FIELDS = ("rho", "mx", "my", "mz", "E")
+
+    def __init__(self, Nx, Ny, Nz, arrays=None):
+        self.Nx, self.Ny, self.Nz = Nx, Ny, Nz
+        self.a = {}
+        for f in self.FIELDS:
+            if arrays is not None:
+                self.a[f] = np.ascontiguousarray(arrays[f], dtype=np.float64).reshape(-1).copy()
+                assert self.a[f].size == Nx * Ny * Nz, f
+            else:
+                self.a[f] = np.zeros(Nx * Ny * Nz)
+
+    def copy(self):
+        return E3Arrays(self.Nx, self.Ny, self.Nz, self.a)
+
+    def cstate(self):
+        return E3State(*[ptr(self.a[f]) for f in self.FIELDS])
 
     def __getitem__(self, k):
         return self.a[k]

@@ -117,6 +117,30 @@ void or_eh_frame(const or_eh_params* p, const or_eh_state* U, double* out);
 /* reference harness only: the program's default IC (init_random_blob, EH:574-642) */
 void or_eh_init_random_blob(const or_eh_params* p, or_eh_state* U);
 
+/* EH rhs (EH:406-527) with the gravity field given instead of solved (gx, gy of N cells) */
+void or_eh_rhs_g(const or_eh_params* p, const or_eh_state* U, or_eh_state* dU, const double* gx, const double* gy);
+
+/* ---- 3D Euler HLLC + FFT gravity (the periodic uniform-sphere benchmark; no reference program:
+ *      the EH step extended to three axes, oracle_e3.c) ---- */
+typedef struct {
+    double gamma, cfl, x0, x1, y0, y1, z0, z1;
+    int Nx, Ny, Nz;
+    double floor_rho, floor_P;
+    int gravity_on;
+    double G;
+} or_e3_params;
+
+typedef struct { double *rho, *mx, *my, *mz, *E; } or_e3_state;
+
+void or_e3_hllc(const or_e3_params* p, int axis, const double* WL5, const double* WR5, double* out5);
+void or_e3_rhs_g(const or_e3_params* p, const or_e3_state* U, or_e3_state* dU, const double* gx, const double* gy,
+                 const double* gz);
+void or_e3_poisson(const or_e3_params* p, const double* rho, double* gx, double* gy, double* gz, double* phi_out);
+void or_e3_rhs(const or_e3_params* p, const or_e3_state* U, or_e3_state* dU);
+double or_e3_compute_dt(const or_e3_params* p, const or_e3_state* U);
+void or_e3_rk2_step(const or_e3_params* p, or_e3_state* U, double dt);
+double or_e3_run(const or_e3_params* p, or_e3_state* U, int nsteps, double t0, double t_end, double* dts_out);
+
 /* ---- 2D Navier-Stokes HLLC + FFT gravity (navier_stokes_2d_hllc_poisson.cpp, Params NS:54-86) ---- */
 typedef struct {
     double gamma, cfl, x0, x1, y0, y1;

@@ -278,6 +278,10 @@ static void rhs_with(const or_eh_params* p, const or_eh_state* U, or_eh_state* d
     free(W); free(Fx); free(Fy);
 }
 
+void or_eh_rhs_g(const or_eh_params* p, const or_eh_state* U, or_eh_state* dU, const double* gx, const double* gy) {
+    rhs_with(p, U, dU, gx, gy);
+}
+
 void or_eh_rhs(const or_eh_params* p, const or_eh_state* U, or_eh_state* dU) {
     const size_t N = (size_t)p->Nx * p->Ny;
     double* gx = (double*)malloc(N * 8);
