@@ -15,12 +15,11 @@ template <typename R> struct Cx;
 template <> struct Cx<float> { using T = float2; };
 template <> struct Cx<double> { using T = double2; };
 
-template <typename T>
-__device__ __forceinline__ T cmul(T a, T b) {
-    return T{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
-}
-
-__device__ __forceinline__ int brev(int v, int logn) { return (int)(__builtin_bitreverse32((unsigned)v) >> (32 - logn)); }
+template <typename T> __device__ __forceinline__ T cadd(T a, T b) { return T{a.x + b.x, a.y + b.y}; }
+template <typename T> __device__ __forceinline__ T csub(T a, T b) { return T{a.x - b.x, a.y - b.y}; }
+template <typename T> __device__ __forceinline__ T cmul(T a, T b) { return T{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+template <typename T> __device__ __forceinline__ T cmi(T a) { return T{a.y, -a.x}; }  // a * (-i)
+template <typename T> __device__ __forceinline__ T cconj(T a) { return T{a.x, -a.y}; }
 
 struct P3Args {
     int Nx, Ny, Nz, lx, ly, lz;  // sizes and their log2
@@ -31,53 +30,100 @@ struct P3Args {
 constexpr int kThreads = 256;
 constexpr int kStatBlocks = 512;
 
-// In-place radix-2 transforms of 2^lognl lines of 2^logn points in LDS; point p of line l is at
-// a[p * ps + l * ls].  LF: consecutive threads take consecutive lines (the ps > 1 layouts), else
-// consecutive butterflies of one line.  Twiddle of span 2h, index j: tw[j * n / (2h)] (conjugated
-// for the inverse).  DIT expects bit-reversed input and leaves natural order; DIF the reverse.
-template <typename T, bool INV, bool LF>
-__device__ __forceinline__ void lds_dit(T* a, int logn, int lognl, int ps, int ls, const T* __restrict__ tw) {
-    const int nb = 1 << (logn - 1 + lognl);
-    const int hmask = (1 << (logn - 1)) - 1;
-    for (int s = 0; s < logn; ++s) {
-        const int h = 1 << s, tstride = 1 << (logn - 1 - s);
-        for (int b = threadIdx.x; b < nb; b += kThreads) {
-            const int l = LF ? (b & ((1 << lognl) - 1)) : (b >> (logn - 1));
-            const int k = LF ? (b >> lognl) : (b & hmask);
-            const int j = k & (h - 1);
-            const int i0 = ((k >> s) << (s + 1)) + j, i1 = i0 + h;
-            T w = tw[j * tstride];
-            if (INV) w.y = -w.y;
-            T* p0 = a + i0 * ps + l * ls;
-            T* p1 = a + i1 * ps + l * ls;
-            const T x0 = *p0, x1 = cmul(*p1, w);
-            *p0 = T{x0.x + x1.x, x0.y + x1.y};
-            *p1 = T{x0.x - x1.x, x0.y - x1.y};
+// Forward DFTs of 2, 4 and 8 points in registers (natural order in and out).
+template <typename T> __device__ __forceinline__ void dft2(T* y) {
+    const T a = y[0];
+    y[0] = cadd(a, y[1]);
+    y[1] = csub(a, y[1]);
+}
+template <typename T> __device__ __forceinline__ void dft4(T* y) {
+    const T a0 = cadd(y[0], y[2]), a1 = csub(y[0], y[2]), a2 = cadd(y[1], y[3]), a3 = cmi(csub(y[1], y[3]));
+    y[0] = cadd(a0, a2); y[2] = csub(a0, a2);
+    y[1] = cadd(a1, a3); y[3] = csub(a1, a3);
+}
+template <typename T> __device__ __forceinline__ void dft8(T* y) {
+    using R = decltype(y[0].x);
+    T e[4] = {y[0], y[2], y[4], y[6]}, o[4] = {y[1], y[3], y[5], y[7]};
+    dft4(e);
+    dft4(o);
+    const R h = (R)0.70710678118654752440;  // X_k = E_k + w^k O_k, X_{k+4} = E_k - w^k O_k, w = exp(-i pi/4)
+    const T o1 = T{(o[1].x + o[1].y) * h, (o[1].y - o[1].x) * h};
+    const T o2 = cmi(o[2]);
+    const T o3 = T{(o[3].y - o[3].x) * h, (-o[3].x - o[3].y) * h};
+    y[0] = cadd(e[0], o[0]); y[4] = csub(e[0], o[0]);
+    y[1] = cadd(e[1], o1); y[5] = csub(e[1], o1);
+    y[2] = cadd(e[2], o2); y[6] = csub(e[2], o2);
+    y[3] = cadd(e[3], o3); y[7] = csub(e[3], o3);
+}
+
+// One Stockham stage of radix 2^LR on a line of n = 2^logn points held 8 per thread: thread j owns
+// elements j + m*T (T = n/8 = 2^lt threads per line, m = 0..7) and runs the butterflies
+// b = j + u*T (u < 8/R) on inputs b + m*n/R (= its registers u + m*8/R) with twiddles
+// exp(-2 pi i m k / (Ns R)), k = b mod Ns, from the n-entry table tw.  Outputs go to
+// (b / Ns) Ns R + k + m Ns: through the LDS line sl (element p at sl[p * ps]), or -- in the last
+// stage, where that is the thread's own register u + m*8/R -- straight back into x.
+template <typename T, int LR>
+__device__ __forceinline__ void stock_stage(T (&x)[8], int j, int lt, int lns, int logn, const T* __restrict__ tw,
+                                            T* sl, int ps, bool last) {
+    constexpr int R = 1 << LR, U = 8 / R;
+    const int Ns = 1 << lns;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int b = j + (u << lt);
+        const int k = b & (Ns - 1);
+        T y[R];
+#pragma unroll
+        for (int m = 0; m < R; ++m) y[m] = x[u + m * U];
+        if (lns > 0) {
+#pragma unroll
+            for (int m = 1; m < R; ++m) y[m] = cmul(y[m], tw[(m * k) << (logn - lns - LR)]);
         }
-        __syncthreads();
+        if (R == 8) dft8(y);
+        else if (R == 4) dft4(y);
+        else dft2(y);
+        if (last) {
+#pragma unroll
+            for (int m = 0; m < R; ++m) x[u + m * U] = y[m];
+        } else {
+            const int base = ((b >> lns) << (lns + LR)) + k;
+#pragma unroll
+            for (int m = 0; m < R; ++m) sl[(base + (m << lns)) * ps] = y[m];
+        }
     }
 }
 
-template <typename T, bool INV, bool LF>
-__device__ __forceinline__ void lds_dif(T* a, int logn, int lognl, int ps, int ls, const T* __restrict__ tw) {
-    const int nb = 1 << (logn - 1 + lognl);
-    const int hmask = (1 << (logn - 1)) - 1;
-    for (int s = logn - 1; s >= 0; --s) {
-        const int h = 1 << s, tstride = 1 << (logn - 1 - s);
-        for (int b = threadIdx.x; b < nb; b += kThreads) {
-            const int l = LF ? (b & ((1 << lognl) - 1)) : (b >> (logn - 1));
-            const int k = LF ? (b >> lognl) : (b & hmask);
-            const int j = k & (h - 1);
-            const int i0 = ((k >> s) << (s + 1)) + j, i1 = i0 + h;
-            T w = tw[j * tstride];
-            if (INV) w.y = -w.y;
-            T* p0 = a + i0 * ps + l * ls;
-            T* p1 = a + i1 * ps + l * ls;
-            const T x0 = *p0, x1 = *p1;
-            *p0 = T{x0.x + x1.x, x0.y + x1.y};
-            *p1 = cmul(T{x0.x - x1.x, x0.y - x1.y}, w);
+// Forward FFT of the line whose elements j + m*T the thread holds in x (natural order in and out):
+// radix-8 stages, then one radix-2 or radix-4 stage for log2 n mod 3.  Every thread of the block
+// must call it (it synchronises the block between stages).
+template <typename T>
+__device__ __forceinline__ void fft_line(T (&x)[8], int j, int lt, int logn, const T* __restrict__ tw, T* sl, int ps) {
+    const int n8 = logn / 3, rem = logn % 3;
+    int lns = 0;
+    for (int st = 0; st < n8; ++st) {
+        const bool last = st == n8 - 1 && rem == 0;
+        stock_stage<T, 3>(x, j, lt, lns, logn, tw, sl, ps, last);
+        lns += 3;
+        if (!last) {
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < 8; ++m) x[m] = sl[(j + (m << lt)) * ps];
+            __syncthreads();
         }
-        __syncthreads();
+    }
+    if (rem == 1) stock_stage<T, 1>(x, j, lt, lns, logn, tw, sl, ps, true);
+    else if (rem == 2) stock_stage<T, 2>(x, j, lt, lns, logn, tw, sl, ps, true);
+}
+
+// Maximum over the block (blockDim a multiple of 64, at most 1024) into part[blockIdx.x].
+__device__ __forceinline__ void block_max_any(double v, double* part) {
+    __shared__ double red[16];
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = smax(m, red[w]);
+        part[blockIdx.x] = m;
     }
 }
 
@@ -140,45 +186,44 @@ __global__ __launch_bounds__(kThreads) void k_p3_flat(double* __restrict__ phi, 
     for (size_t q = (size_t)blockIdx.x * kThreads + threadIdx.x; q < n; q += (size_t)gridDim.x * kThreads) phi[q] = 0.0;
 }
 
-// ---- forward x pass with the residual: r = rhs - L_h phi (FP64), max|r| per block, x-FFT of r
+// ---- forward x pass with the residual: r = rhs - L_h phi (FP64), max|r| per block, x-FFT of r.
+// Block: 2^lnl lines of Nx points, Nx/8 threads per line.
 template <typename R>
-__global__ __launch_bounds__(kThreads) void k_p3_xfwd(P3Args a, int lognl, const double* __restrict__ rho,
-                                                      const double* __restrict__ phi, typename Cx<R>::T* __restrict__ buf,
-                                                      const typename Cx<R>::T* __restrict__ tw,
-                                                      double* __restrict__ part, const Poisson3DStatus* __restrict__ st) {
+__global__ __launch_bounds__(1024) void k_p3_xfwd(P3Args a, int lnl, const double* __restrict__ rho,
+                                                  const double* __restrict__ phi, typename Cx<R>::T* __restrict__ buf,
+                                                  const typename Cx<R>::T* __restrict__ tw, double* __restrict__ part,
+                                                  const Poisson3DStatus* __restrict__ st) {
     using T = typename Cx<R>::T;
     if (st->done) return;
     extern __shared__ unsigned char smem_raw[];
     T* s = reinterpret_cast<T*>(smem_raw);
-    const int Nx = a.Nx, Ny = a.Ny, Nz = a.Nz;
-    const size_t line0 = (size_t)blockIdx.x << lognl;
-    const size_t sxy = (size_t)Nx * Ny;
+    const int Nx = a.Nx, Ny = a.Ny, Nz = a.Nz, lt = a.lx - 3;
+    const int j = threadIdx.x & ((1 << lt) - 1), l = threadIdx.x >> lt;
+    const size_t line = ((size_t)blockIdx.x << lnl) + l;
+    const int jy = (int)(line & (size_t)(Ny - 1)), kz = (int)(line >> a.ly);
+    const size_t sxy = (size_t)Nx * Ny, row = line * Nx;
+    const size_t rn = (size_t)kz * sxy + (size_t)((jy + 1) & (Ny - 1)) * Nx;
+    const size_t rs = (size_t)kz * sxy + (size_t)((jy - 1) & (Ny - 1)) * Nx;
+    const size_t ru = (size_t)((kz + 1) & (Nz - 1)) * sxy + (size_t)jy * Nx;
+    const size_t rd = (size_t)((kz - 1) & (Nz - 1)) * sxy + (size_t)jy * Nx;
     const double rhobar = st->rhobar;
     double rmax = 0.0;
-    const int ne = Nx << lognl;
-    for (int e = threadIdx.x; e < ne; e += kThreads) {
-        const int l = e >> a.lx, i = e & (Nx - 1);
-        const size_t line = line0 + l;
-        const int j = (int)(line & (size_t)(Ny - 1)), k = (int)(line >> a.ly);
-        const size_t q = line * Nx + i;
+    T x[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const int i = j + (m << lt);
+        const size_t q = row + i;
         const double p0 = phi[q];
-        const double pe = phi[q - i + ((i + 1) & (Nx - 1))], pw = phi[q - i + ((i - 1) & (Nx - 1))];
-        const size_t row = (size_t)k * sxy;
-        const double pn = phi[row + (size_t)((j + 1) & (Ny - 1)) * Nx + i];
-        const double ps = phi[row + (size_t)((j - 1) & (Ny - 1)) * Nx + i];
-        const size_t col = (size_t)j * Nx + i;
-        const double pu = phi[(size_t)((k + 1) & (Nz - 1)) * sxy + col];
-        const double pd = phi[(size_t)((k - 1) & (Nz - 1)) * sxy + col];
-        const double lap = (pe + pw - 2.0 * p0) * a.idx2 + (pn + ps - 2.0 * p0) * a.idy2 + (pu + pd - 2.0 * p0) * a.idz2;
+        const double lap = (phi[row + ((i + 1) & (Nx - 1))] + phi[row + ((i - 1) & (Nx - 1))] - 2.0 * p0) * a.idx2 +
+                           (phi[rn + i] + phi[rs + i] - 2.0 * p0) * a.idy2 + (phi[ru + i] + phi[rd + i] - 2.0 * p0) * a.idz2;
         const double r = a.c4piG * (smax(rho[q], a.floor_rho) - rhobar) - lap;
         rmax = smax(rmax, fabs(r));
-        s[(l << a.lx) + brev(i, a.lx)] = T{(R)r, (R)0};
+        x[m] = T{(R)r, (R)0};
     }
-    __syncthreads();
-    lds_dit<T, false, false>(s, a.lx, lognl, 1, Nx, tw);
-    T* out = buf + line0 * Nx;
-    for (int e = threadIdx.x; e < ne; e += kThreads) out[e] = s[e];
-    block_max_to(rmax, part);
+    fft_line(x, j, lt, a.lx, tw, s + ((size_t)l << a.lx), 1);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) buf[row + j + (m << lt)] = x[m];
+    block_max_any(rmax, part);
 }
 
 // The refinement check (one block): converged, or out of iterations -> done.
@@ -203,93 +248,81 @@ __global__ __launch_bounds__(kThreads) void k_p3_check(P3Args a, const double* _
     }
 }
 
-// y pass: 2^logc consecutive x columns of one z plane per block
+// y pass: 2^lc consecutive x columns of one z plane per block, Ny/8 threads per column.
 template <typename R, bool INV>
-__global__ __launch_bounds__(kThreads) void k_p3_y(P3Args a, int logc, typename Cx<R>::T* __restrict__ buf,
-                                                   const typename Cx<R>::T* __restrict__ tw,
-                                                   const Poisson3DStatus* __restrict__ st) {
+__global__ __launch_bounds__(1024) void k_p3_y(P3Args a, int lc, typename Cx<R>::T* __restrict__ buf,
+                                               const typename Cx<R>::T* __restrict__ tw,
+                                               const Poisson3DStatus* __restrict__ st) {
     using T = typename Cx<R>::T;
     if (st->done) return;
     extern __shared__ unsigned char smem_raw[];
     T* s = reinterpret_cast<T*>(smem_raw);
-    const int C = 1 << logc, Nx = a.Nx;
-    const int xb = blockIdx.x & ((Nx >> logc) - 1), k = blockIdx.x >> (a.lx - logc);
-    T* base = buf + (size_t)k * Nx * a.Ny + (xb << logc);
-    const int ne = a.Ny << logc;
-    for (int e = threadIdx.x; e < ne; e += kThreads) {
-        const int c = e & (C - 1), j = e >> logc;
-        s[((INV ? j : brev(j, a.ly)) << logc) + c] = base[(size_t)j * Nx + c];
+    const int Nx = a.Nx, lt = a.ly - 3, C = 1 << lc;
+    const int c = threadIdx.x & (C - 1), j = threadIdx.x >> lc;
+    const int xb = blockIdx.x & ((Nx >> lc) - 1), k = blockIdx.x >> (a.lx - lc);
+    T* base = buf + (size_t)k * Nx * a.Ny + (xb << lc) + c;
+    T x[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const T v = base[(size_t)(j + (m << lt)) * Nx];
+        x[m] = INV ? cconj(v) : v;
     }
-    __syncthreads();
-    if (INV)
-        lds_dif<T, true, true>(s, a.ly, logc, C, 1, tw);
-    else
-        lds_dit<T, false, true>(s, a.ly, logc, C, 1, tw);
-    for (int e = threadIdx.x; e < ne; e += kThreads) {
-        const int c = e & (C - 1), j = e >> logc;
-        base[(size_t)j * Nx + c] = s[((INV ? brev(j, a.ly) : j) << logc) + c];
-    }
+    fft_line(x, j, lt, a.ly, tw, s + c, C);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) base[(size_t)(j + (m << lt)) * Nx] = INV ? cconj(x[m]) : x[m];
 }
 
 // z pass: forward transform, spectral division by the discrete Laplacian (and 1/N), inverse
-// transform, for 2^logc consecutive x columns of one y row per block
+// transform (conj . forward . conj), for 2^lc consecutive x columns of one y row per block.
 template <typename R>
-__global__ __launch_bounds__(kThreads) void k_p3_z(P3Args a, int logc, typename Cx<R>::T* __restrict__ buf,
-                                                   const typename Cx<R>::T* __restrict__ tw, const double* __restrict__ lam,
-                                                   const Poisson3DStatus* __restrict__ st) {
+__global__ __launch_bounds__(1024) void k_p3_z(P3Args a, int lc, typename Cx<R>::T* __restrict__ buf,
+                                               const typename Cx<R>::T* __restrict__ tw, const double* __restrict__ lam,
+                                               const Poisson3DStatus* __restrict__ st) {
     using T = typename Cx<R>::T;
     if (st->done) return;
     extern __shared__ unsigned char smem_raw[];
     T* s = reinterpret_cast<T*>(smem_raw);
-    const int C = 1 << logc, Nx = a.Nx;
-    const int xb = blockIdx.x & ((Nx >> logc) - 1), j = blockIdx.x >> (a.lx - logc);
+    const int Nx = a.Nx, lt = a.lz - 3, C = 1 << lc;
+    const int c = threadIdx.x & (C - 1), j = threadIdx.x >> lc;
+    const int xb = blockIdx.x & ((Nx >> lc) - 1), jy = blockIdx.x >> (a.lx - lc);
     const size_t zs = (size_t)Nx * a.Ny;
-    T* base = buf + (size_t)j * Nx + (xb << logc);
-    const int ne = a.Nz << logc;
-    for (int e = threadIdx.x; e < ne; e += kThreads) {
-        const int c = e & (C - 1), k = e >> logc;
-        s[(brev(k, a.lz) << logc) + c] = base[(size_t)k * zs + c];
+    T* base = buf + (size_t)jy * Nx + (xb << lc) + c;
+    T x[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) x[m] = base[(size_t)(j + (m << lt)) * zs];
+    fft_line(x, j, lt, a.lz, tw, s + c, C);
+    const int kx = (xb << lc) + c;
+    const double lxy = lam[kx] + lam[Nx + jy];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const int kz = j + (m << lt);
+        const double l = lxy + lam[Nx + a.Ny + kz];
+        const R f = (kx == 0 && jy == 0 && kz == 0) ? (R)0 : (R)(a.inv_n / l);
+        x[m] = T{x[m].x * f, -(x[m].y * f)};  // scaled and conjugated for the inverse
     }
-    __syncthreads();
-    lds_dit<T, false, true>(s, a.lz, logc, C, 1, tw);
-    const double* lx = lam;
-    const double* ly = lam + Nx;
-    const double* lz = lam + Nx + a.Ny;
-    for (int e = threadIdx.x; e < ne; e += kThreads) {
-        const int c = e & (C - 1), k = e >> logc;
-        const int kx = (xb << logc) + c;
-        const double l = lx[kx] + ly[j] + lz[k];
-        const double m = (kx == 0 && j == 0 && k == 0) ? 0.0 : a.inv_n / l;
-        const T v = s[e];
-        s[e] = T{(R)(v.x * (R)m), (R)(v.y * (R)m)};
-    }
-    __syncthreads();
-    lds_dif<T, true, true>(s, a.lz, logc, C, 1, tw + (size_t)0);
-    for (int e = threadIdx.x; e < ne; e += kThreads) {
-        const int c = e & (C - 1), k = e >> logc;
-        base[(size_t)k * zs + c] = s[(brev(k, a.lz) << logc) + c];
-    }
+    fft_line(x, j, lt, a.lz, tw, s + c, C);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) base[(size_t)(j + (m << lt)) * zs] = cconj(x[m]);
 }
 
 // inverse x pass and the FP64 correction phi += Re
 template <typename R>
-__global__ __launch_bounds__(kThreads) void k_p3_xinv(P3Args a, int lognl, const typename Cx<R>::T* __restrict__ buf,
-                                                      const typename Cx<R>::T* __restrict__ tw, double* __restrict__ phi,
-                                                      const Poisson3DStatus* __restrict__ st) {
+__global__ __launch_bounds__(1024) void k_p3_xinv(P3Args a, int lnl, const typename Cx<R>::T* __restrict__ buf,
+                                                  const typename Cx<R>::T* __restrict__ tw, double* __restrict__ phi,
+                                                  const Poisson3DStatus* __restrict__ st) {
     using T = typename Cx<R>::T;
     if (st->done) return;
     extern __shared__ unsigned char smem_raw[];
     T* s = reinterpret_cast<T*>(smem_raw);
-    const int Nx = a.Nx;
-    const size_t off = ((size_t)blockIdx.x << lognl) * Nx;
-    const int ne = Nx << lognl;
-    for (int e = threadIdx.x; e < ne; e += kThreads) s[e] = buf[off + e];
-    __syncthreads();
-    lds_dif<T, true, false>(s, a.lx, lognl, 1, Nx, tw);
-    for (int e = threadIdx.x; e < ne; e += kThreads) {
-        const int l = e >> a.lx, i = e & (Nx - 1);
-        phi[off + e] += (double)s[(l << a.lx) + brev(i, a.lx)].x;
-    }
+    const int lt = a.lx - 3;
+    const int j = threadIdx.x & ((1 << lt) - 1), l = threadIdx.x >> lt;
+    const size_t row = (((size_t)blockIdx.x << lnl) + l) * a.Nx;
+    T x[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) x[m] = cconj(buf[row + j + (m << lt)]);
+    fft_line(x, j, lt, a.lx, tw, s + ((size_t)l << a.lx), 1);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) phi[row + j + (m << lt)] += (double)x[m].x;  // Re(conj(.)) = Re(.)
 }
 
 // g = -central grad phi
@@ -317,25 +350,33 @@ int ilog2(int n) {
 }
 
 struct Geo {
-    int lognl_x, logc_y, logc_z, nblk_x, nblk_y, nblk_z;
+    int lnl_x, lc_y, lc_z, nblk_x, nblk_y, nblk_z, thr_x, thr_y, thr_z;
     size_t lds_x, lds_y, lds_z;
 };
 
+// x pass: 2^lnl lines of Nx/8 threads (<= 256 threads); y / z passes: 2^lc columns of N/8 threads,
+// as many columns as keep the block <= 1024 threads, its LDS <= 64 KB and the columns <= 16.
 Geo geometry(int Nx, int Ny, int Nz, bool mixed) {
-    const size_t tsz = mixed ? 8 : 16;
-    const int ex = mixed ? 4096 : 2048;   // elements per x-pass block (32 KB)
-    const int eyz = mixed ? 8192 : 4096;  // elements per y / z-pass block (64 KB)
+    const int tsz = mixed ? 8 : 16;
     Geo g;
     const int lx = ilog2(Nx), ly = ilog2(Ny), lz = ilog2(Nz);
-    g.lognl_x = std::max(0, std::min(ilog2(ex) - lx, ly + lz));
-    g.logc_y = std::max(0, std::min(ilog2(eyz) - ly, lx));
-    g.logc_z = std::max(0, std::min(ilog2(eyz) - lz, lx));
-    g.nblk_x = (Ny * Nz) >> g.lognl_x;
-    g.nblk_y = (Nx >> g.logc_y) * Nz;
-    g.nblk_z = (Nx >> g.logc_z) * Ny;
-    g.lds_x = ((size_t)Nx << g.lognl_x) * tsz;
-    g.lds_y = ((size_t)Ny << g.logc_y) * tsz;
-    g.lds_z = ((size_t)Nz << g.logc_z) * tsz;
+    g.lnl_x = std::min(8 - (lx - 3), ly + lz);
+    auto cols = [&](int ln) {
+        int lc = std::min(lx, 4);
+        while (lc > 0 && ((1 << lc) * (1 << (ln - 3)) > 1024 || ((size_t)1 << lc) * ((size_t)1 << ln) * tsz > 65536)) --lc;
+        return lc;
+    };
+    g.lc_y = cols(ly);
+    g.lc_z = cols(lz);
+    g.thr_x = (1 << g.lnl_x) << (lx - 3);
+    g.thr_y = (1 << g.lc_y) << (ly - 3);
+    g.thr_z = (1 << g.lc_z) << (lz - 3);
+    g.nblk_x = (Ny * Nz) >> g.lnl_x;
+    g.nblk_y = (Nx >> g.lc_y) * Nz;
+    g.nblk_z = (Nx >> g.lc_z) * Ny;
+    g.lds_x = ((size_t)Nx << g.lnl_x) * tsz;
+    g.lds_y = ((size_t)Ny << g.lc_y) * tsz;
+    g.lds_z = ((size_t)Nz << g.lc_z) * tsz;
     return g;
 }
 
@@ -375,7 +416,7 @@ cc_status solve_t(Poisson3D& P, const double* rho, double* gx, double* gy, doubl
     for (int it = 0; it <= P.max_iter; ++it) {
         {
             ProfScope ps(prof, s, "fft_x");
-            hipLaunchKernelGGL((k_p3_xfwd<R>), dim3(g.nblk_x), dim3(kThreads), g.lds_x, s, a, g.lognl_x, rho, P.phi, buf,
+            hipLaunchKernelGGL((k_p3_xfwd<R>), dim3(g.nblk_x), dim3(g.thr_x), g.lds_x, s, a, g.lnl_x, rho, P.phi, buf,
                                twx, P.part, P.st);
             CC_CHECK_LAUNCH();
         }
@@ -387,25 +428,25 @@ cc_status solve_t(Poisson3D& P, const double* rho, double* gx, double* gy, doubl
         if (it == P.max_iter) break;
         {
             ProfScope ps(prof, s, "fft_y");
-            hipLaunchKernelGGL((k_p3_y<R, false>), dim3(g.nblk_y), dim3(kThreads), g.lds_y, s, a, g.logc_y, buf, twy,
+            hipLaunchKernelGGL((k_p3_y<R, false>), dim3(g.nblk_y), dim3(g.thr_y), g.lds_y, s, a, g.lc_y, buf, twy,
                                P.st);
             CC_CHECK_LAUNCH();
         }
         {
             ProfScope ps(prof, s, "fft_z");
-            hipLaunchKernelGGL((k_p3_z<R>), dim3(g.nblk_z), dim3(kThreads), g.lds_z, s, a, g.logc_z, buf, twz, P.lam,
+            hipLaunchKernelGGL((k_p3_z<R>), dim3(g.nblk_z), dim3(g.thr_z), g.lds_z, s, a, g.lc_z, buf, twz, P.lam,
                                P.st);
             CC_CHECK_LAUNCH();
         }
         {
             ProfScope ps(prof, s, "fft_y");
-            hipLaunchKernelGGL((k_p3_y<R, true>), dim3(g.nblk_y), dim3(kThreads), g.lds_y, s, a, g.logc_y, buf, twy,
+            hipLaunchKernelGGL((k_p3_y<R, true>), dim3(g.nblk_y), dim3(g.thr_y), g.lds_y, s, a, g.lc_y, buf, twy,
                                P.st);
             CC_CHECK_LAUNCH();
         }
         {
             ProfScope ps(prof, s, "fft_x");
-            hipLaunchKernelGGL((k_p3_xinv<R>), dim3(g.nblk_x), dim3(kThreads), g.lds_x, s, a, g.lognl_x, buf, twx,
+            hipLaunchKernelGGL((k_p3_xinv<R>), dim3(g.nblk_x), dim3(g.thr_x), g.lds_x, s, a, g.lnl_x, buf, twx,
                                P.phi, P.st);
             CC_CHECK_LAUNCH();
         }
@@ -445,7 +486,7 @@ cc_status Poisson3D::init(int Nx_, int Ny_, int Nz_, double dx_, double dy_, dou
               hipMalloc(&st, sizeof(Poisson3DStatus)) == hipSuccess &&
               hipMalloc(&lam, (size_t)(Nx + Ny + Nz) * sizeof(double)) == hipSuccess;
     const int dims[3] = {Nx, Ny, Nz};
-    for (int ax = 0; ax < 3 && ok; ++ax) ok = hipMalloc(&tw[ax], (size_t)(dims[ax] / 2) * tsz) == hipSuccess;
+    for (int ax = 0; ax < 3 && ok; ++ax) ok = hipMalloc(&tw[ax], (size_t)dims[ax] * tsz) == hipSuccess;
     if (!ok) {
         destroy();
         set_error(CC_ERR_OOM, "Poisson3D: device allocation failed");
@@ -463,8 +504,8 @@ cc_status Poisson3D::init(int Nx_, int Ny_, int Nz_, double dx_, double dy_, dou
     CC_HIP(hipMemcpy(lam, lh.data(), lh.size() * sizeof(double), hipMemcpyHostToDevice));
     for (int ax = 0; ax < 3; ++ax) {
         const int na = dims[ax];
-        std::vector<double> t(na);  // (re, im) pairs of exp(-2 pi i m / N)
-        for (int m = 0; m < na / 2; ++m) {
+        std::vector<double> t(2 * na);  // (re, im) pairs of exp(-2 pi i m / N), m < N
+        for (int m = 0; m < na; ++m) {
             const double ang = -2.0 * M_PI * (double)m / (double)na;
             t[2 * m] = std::cos(ang);
             t[2 * m + 1] = std::sin(ang);

@@ -14,10 +14,11 @@
 // remaining kernels of the solve return at once, so a solve never waits for the host and can be
 // captured in a graph.  phi is kept between solves (warm start).
 //
-// Each FFT pass is one kernel that stages a batch of whole lines in LDS: radix-2, decimation in
-// time (bit-reversed load, natural-order result) for forward transforms and decimation in
-// frequency (natural-order input, bit-reversed result, read back through the bit reversal) for
-// inverse ones, with the twiddles exp(-2 pi i j / N) from a per-axis table computed in FP64.
+// Each FFT pass is one kernel over a batch of whole lines: every thread holds 8 points of a line
+// in registers and runs Stockham radix-8 stages (one radix-2 or radix-4 stage for log2 N mod 3),
+// exchanging points through LDS between stages; input and output are in natural order, inverse
+// transforms are conj . forward . conj, and the twiddles exp(-2 pi i m / N) come from a per-axis
+// table computed in FP64.
 #pragma once
 #include "common.hpp"
 #include "profiler.hpp"
@@ -50,7 +51,7 @@ class Poisson3D {
     double* part = nullptr;       // block partials
     int npart = 0;
     Poisson3DStatus* st = nullptr;
-    void* tw[3] = {nullptr, nullptr, nullptr};   // per axis N_a/2 complex<R> twiddles
+    void* tw[3] = {nullptr, nullptr, nullptr};   // per axis N_a complex<R> twiddles
     double* lam = nullptr;                       // eigenvalues, Nx + Ny + Nz
 };
 
