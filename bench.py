@@ -216,6 +216,12 @@ def bench3d(N, warmup, steps, with_cpu):
                                if v[0] > 0} for prec, tag in ((CC_FP64, "fp64"), (CC_MIXED, "mixed"))},
     }
     out["hbm_frac_alg"] = round(out["value"] * 320 / 1e9 / HBM_PEAK_GBS, 4)
+    for prec, tag, R, mx in ((CC_FP64, "fp64", "double", "false"), (CC_MIXED, "mixed", "float", "true")):
+        groups = {"prim": ["cc::k_prim3d<%s>" % R],
+                  "face": ["cc::k_face3d<%d, %s," % (a, mx) for a in range(3)],
+                  "update": ["cc::k_update3d<%s>" % R], "edge": ["cc::k_edge3d<%s>" % R], "ct": ["cc::k_ct3d"],
+                  "joule": ["cc::k_joule3d<true>", "cc::k_joule3d<false>"]}
+        out.setdefault("hbm_measured", {})[tag] = measured_hbm(res[prec]["prof"], groups)
     if with_cpu:
         out["cpu_baseline"] = cpu_baseline3d()
     return out
@@ -293,8 +299,8 @@ def run_gpu_sphere(N, precision, warmup, steps):
     return {"elapsed": elapsed, "rho": out["rho"], "prof": prof, "poisson": stats, "mass": tot["mass"]}
 
 
-def cpu_baseline_sphere(n=32, budget_s=12.0):
-    """The oracle's restatement of the 3D Euler + spectral gravity step on a 32^3 sample."""
+def cpu_baseline_sphere(n=128, budget_s=12.0):
+    """The oracle's restatement of the 3D Euler + spectral gravity step on a 128^3 sample."""
     from core_collapse_amd import ic
     from oracle import bind as OB
     lib = OB.Lib("oracle")
@@ -321,16 +327,20 @@ def bench_sphere3d(N, warmup, steps, with_cpu):
     zu = float(N) ** 3 * steps
     cells = float(N) ** 3
 
-    def roof(r, word):
-        n, ms = r["prof"]["face"]
+    def roof(r, word, tname):
+        n, ms = r["prof"]["update"]
         if n == 0 or ms <= 0:
             return None
-        # one "face" record = the three axis launches; per face: read 5 primitives, write 5 fluxes
-        bpl = cells * 3 * 10 * word
+        # the stage update, one launch per stage (half of them the final stage): per cell it reads 15 face
+        # fluxes (3 axes x 5, in the flux precision), the state (5 FP64), gravity (3 FP64), on the final
+        # stage also U^n (5 FP64), and writes the state: 15 w + 104 B or 15 w + 144 B, 15 w + 124 B on average
+        bpl = cells * (15 * word + 124)
         avg = ms / n * 1e-3
+        t1, t2 = load_traffic("cc::k_e3_update<1, %s>" % tname), load_traffic("cc::k_e3_update<2, %s>" % tname)
         return {"bound": "hbm", "achieved": round(bpl / avg / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(bpl / avg / 1e9 / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_e3_face (x, y, z)",
-                "launches": 3 * n, "avg_launch_us": round(avg * 1e6 / 3, 2), "alg_bytes_per_launch": bpl / 3}
+                "frac": round(bpl / avg / 1e9 / HBM_PEAK_GBS, 4),
+                "traffic": 0.5 * (t1 + t2) if t1 and t2 else None, "kernel": "k_e3_update",
+                "launches": n, "avg_launch_us": round(avg * 1e6, 2), "alg_bytes_per_launch": bpl}
 
     def balg(r):
         k = max(r["poisson"]["iters_last"], 1)
@@ -344,19 +354,46 @@ def bench_sphere3d(N, warmup, steps, with_cpu):
         "value": zu / res[CC_MIXED]["elapsed"], "unit": "zone-updates/s",
         "ms_per_step": 1e3 * res[CC_MIXED]["elapsed"] / steps,
         "fp64": {"value": zu / res[CC_FP64]["elapsed"], "ms_per_step": 1e3 * res[CC_FP64]["elapsed"] / steps,
-                 "roofline": roof(res[CC_FP64], 8), "poisson": res[CC_FP64]["poisson"]},
+                 "roofline": roof(res[CC_FP64], 8, "double"), "poisson": res[CC_FP64]["poisson"]},
         "mixed_over_fp64": res[CC_FP64]["elapsed"] / res[CC_MIXED]["elapsed"],
         "l1_rho_rel_err": float(np.abs(a - b).sum() / np.abs(b).sum()),
         "linf_rho_rel_err": float(np.abs(a - b).max() / np.abs(b).max()),
-        "roofline": roof(res[CC_MIXED], 4),
+        "roofline": roof(res[CC_MIXED], 4, "float"),
         "poisson": res[CC_MIXED]["poisson"],
         "zone_update_alg_bytes": balg(res[CC_MIXED]),
         "kernel_times": {tag: {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in res[prec]["prof"].items()
                                if v[0] > 0} for prec, tag in ((CC_FP64, "fp64"), (CC_MIXED, "mixed"))},
     }
     out["hbm_frac_alg"] = round(out["value"] * balg(res[CC_MIXED]) / 1e9 / HBM_PEAK_GBS, 4)
+    for prec, tag, R in ((CC_FP64, "fp64", "double"), (CC_MIXED, "mixed", "float")):
+        groups = {"prim": ["cc::k_e3_prim<%s>" % R], "face": tuple("cc::k_e3_face<%s, %d>" % (R, a) for a in range(3)),
+                  "update": ["cc::k_e3_update<1, %s>" % R, "cc::k_e3_update<2, %s>" % R], "grad": ["cc::k_p3_grad"]}
+        out.setdefault("hbm_measured", {})[tag] = measured_hbm(res[prec]["prof"], groups)
     if with_cpu:
         out["cpu_baseline"] = cpu_baseline_sphere()
+    return out
+
+
+def measured_hbm(prof, groups):
+    """Per profiler scope: rocprof-measured HBM bytes per launch (profiles/hbm_traffic.json, averaged over
+    the kernels the scope launches) divided by the scope's average launch time from the HIP events of
+    this run (a tuple of kernels: one scope that launches each of them once); plus the same ratio over all
+    listed scopes together (bytes and time summed)."""
+    out, tb, tt = {}, 0.0, 0.0
+    for scope, names in groups.items():
+        n, ms = prof.get(scope, (0, 0.0))
+        together = isinstance(names, tuple)  # a tuple: one timed scope launches all of them
+        tr = [load_traffic(k) for k in names]
+        if n == 0 or ms <= 0 or any(t is None for t in tr):
+            continue
+        b = sum(tr) if together else sum(tr) / len(tr)
+        avg = ms / n * 1e-3
+        out[scope] = {"bytes_per_launch": b, "avg_launch_us": round(avg * 1e6, 2),
+                      "GBps": round(b / avg / 1e9, 1), "frac": round(b / avg / 1e9 / HBM_PEAK_GBS, 4)}
+        tb += b * n
+        tt += ms * 1e-3
+    if tt > 0:
+        out["all"] = {"GBps": round(tb / tt / 1e9, 1), "frac": round(tb / tt / 1e9 / HBM_PEAK_GBS, 4)}
     return out
 
 

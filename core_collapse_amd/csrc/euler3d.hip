@@ -194,6 +194,14 @@ __global__ __launch_bounds__(256) void k_e3_speed(GeomE3 g, StateE3 U, double* _
     block_max_to(s, spd);
 }
 
+// First level of the speed reduction: the per-block maxima of a launch over all cells -> kDtPart values.
+constexpr int kDtPart = 256;
+__global__ __launch_bounds__(256) void k_e3_partmax(const double* __restrict__ part, int n, double* __restrict__ out) {
+    double m = 0.0;
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) m = smax(m, part[k]);
+    block_max_to(m, out);
+}
+
 // dt = cfl min(dx, dy, dz) / max speed, clamped to t_end, t advanced, dt logged
 __global__ __launch_bounds__(256) void k_e3_dt(GeomE3 g, const double* __restrict__ spd, int nspd, Scalars* sc,
                                                double* __restrict__ dts) {
@@ -227,6 +235,7 @@ struct cc_euler3d {
     double* dts_dev = nullptr;
     double* spd = nullptr;
     double* dtv = nullptr;
+    double* spd2 = nullptr;  // kDtPart second-level maxima
     int nspd = 0;
     Poisson3D pois;
     Profiler prof;
@@ -302,7 +311,8 @@ struct cc_euler3d {
     cc_status heun(bool driver) {
         if (driver) {
             ProfScope ps(pr(), stream, "dt");
-            hipLaunchKernelGGL(k_e3_dt, dim3(1), dim3(256), 0, stream, g, spd, nspd, sc, dts_dev);
+            hipLaunchKernelGGL(k_e3_partmax, dim3(kDtPart), dim3(256), 0, stream, spd, nspd, spd2);
+            hipLaunchKernelGGL(k_e3_dt, dim3(1), dim3(256), 0, stream, g, spd2, kDtPart, sc, dts_dev);
             CC_CHECK_LAUNCH();
         }
         CC_TRY(gravity_of(S.rho));
@@ -392,6 +402,7 @@ cc_status cc_euler3d_create(const cc_euler3d_params* p, cc_euler3d** out) {
     h->nspd = (int)h->blocks();
     if (st == CC_OK) st = h->alloc(&h->spd, (size_t)h->nspd);
     if (st == CC_OK) st = h->alloc(&h->dtv, 1);
+    if (st == CC_OK) st = h->alloc(&h->spd2, kDtPart);
     if (st == CC_OK) st = h->alloc(&h->sc, 1);
     if (st == CC_OK && p->gravity_on)
         st = h->pois.init(p->Nx, p->Ny, p->Nz, g.dx, g.dy, g.dz, p->G, p->floor_rho, p->poisson_tol, p->poisson_max_iter,
@@ -489,7 +500,8 @@ cc_status cc_euler3d_compute_dt(cc_euler3d* h, double* dt) {
     CHECK_E3();
     if (!dt) { set_error(CC_ERR_INVALID_ARG, "null dt"); return CC_ERR_INVALID_ARG; }
     CC_TRY(h->speed_max());
-    hipLaunchKernelGGL(k_e3_dt_value, dim3(1), dim3(256), 0, h->stream, h->g, h->spd, h->nspd, h->dtv);
+    hipLaunchKernelGGL(k_e3_partmax, dim3(kDtPart), dim3(256), 0, h->stream, h->spd, h->nspd, h->spd2);
+    hipLaunchKernelGGL(k_e3_dt_value, dim3(1), dim3(256), 0, h->stream, h->g, h->spd2, kDtPart, h->dtv);
     CC_CHECK_LAUNCH();
     CC_HIP(hipMemcpyAsync(dt, h->dtv, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     CC_HIP(hipStreamSynchronize(h->stream));

@@ -28,7 +28,7 @@ struct P3Args {
 };
 
 constexpr int kThreads = 256;
-constexpr int kStatBlocks = 512;
+constexpr int kStatBlocks = 2048;
 
 // Forward DFTs of 2, 4 and 8 points in registers (natural order in and out).
 template <typename T> __device__ __forceinline__ void dft2(T* y) {
