@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--steps5", type=int, default=10)
     ap.add_argument("--batch", type=int, default=64, help="instances of the batched collapse")
     ap.add_argument("--steps-batch", type=int, default=200)
+    ap.add_argument("--sor-autotune", type=int, default=1,
+                    help="1: time the compiled SOR tile shapes once per 2D grid size and use the fastest")
     ap.add_argument("--seq-steps", type=int, default=10,
                     help="steps per instance of the one-after-another comparison (0: skip)")
     ap.add_argument("--N3", type=int, default=256, help="grid of the 3D turbulence problem")
@@ -86,9 +88,9 @@ def run_gpu(N, precision, warmup, steps):
     h.set_profiling(False)
     out = h.download()
     divb = h.divB_max()
-    sor_tiles = None
     h.close()
-    return {"elapsed": elapsed, "t": t, "rho": out["rho"], "divb": divb, "prof": prof, "params": p}
+    tile = M.sor_tile_choice(N, N, p.sor_iters, precision)
+    return {"elapsed": elapsed, "t": t, "rho": out["rho"], "divb": divb, "prof": prof, "params": p, "tile": tile}
 
 
 def sor_cell_sweeps_per_solve(N, iters=600):
@@ -236,6 +238,7 @@ def bench_batch2d(n_inst, steps, warmup, nmin, nmax, seq_steps):
     from core_collapse_amd import ic
     from core_collapse_amd import mhd2d as M
     sample = ic.collapse2d_batch_sample(n_inst, 42, nmin, nmax)
+    t_setup = time.perf_counter()
     sims = []
     for e in sample:
         p = ic.collapse2d_params(e["N"], eta=e["eta"], precision=CC_MIXED)
@@ -243,6 +246,8 @@ def bench_batch2d(n_inst, steps, warmup, nmin, nmax, seq_steps):
         h.upload(ic.collapse2d_ic(p, **e["ic"]))
         sims.append(h)
     cells = float(sum(e["N"] ** 2 for e in sample))
+    t_setup = time.perf_counter() - t_setup
+    tiles = {str(e["N"]): M.sor_tile_choice(e["N"], e["N"], 600, CC_MIXED) for e in sample}
     b = M.Mhd2DBatch(sims)
     t = b.run(warmup) if warmup > 0 else None
     t0 = time.perf_counter()
@@ -272,6 +277,7 @@ def bench_batch2d(n_inst, steps, warmup, nmin, nmax, seq_steps):
                                 "ms_per_step": 1e3 * seq / steps if seq > 0 else None},
         "batch_over_sequential": seq / elapsed if seq > 0 else None,
         "sor_alg_GBps": cells * steps * 2 * 600 * 12 / elapsed / 1e9,
+        "setup_s": round(t_setup, 2), "sor_tiles": tiles,
         "max_divB": divb, "t_min": float(np.min(t)), "t_max": float(np.max(t)),
     }
 
@@ -428,6 +434,9 @@ def main():
                           "GBps": 2 * nbytes / ms.value / 1e6}))
         return
     N, K, W = a.N, a.steps, a.warmup
+    if a.sor_autotune:
+        from core_collapse_amd import mhd2d as M2
+        M2.sor_autotune(True)
     t3d = None
     if a.workload in ("all", "turb3d"):
         print("bench: %d^3 turbulence, warmup %d, timed %d steps" % (a.N3, 2, a.steps3), file=sys.stderr, flush=True)
@@ -485,9 +494,9 @@ def main():
     roof_mx = roof(res[CC_MIXED], 12.0)
     roof_64 = roof(res[CC_FP64], 24.0)
     if roof_mx:
-        roof_mx["traffic"] = load_traffic("cc::k_sor_tile_mixed")
+        roof_mx["traffic"] = load_traffic("cc::k_sor_tile_mixed<%d, %d>" % tuple(res[CC_MIXED]["tile"]))
     if roof_64:
-        roof_64["traffic"] = load_traffic("cc::k_sor_tile_fp64")
+        roof_64["traffic"] = load_traffic("cc::k_sor_tile_fp64<%d, %d," % tuple(res[CC_FP64]["tile"]))
     kernels = {}
     for prec, tag in ((CC_FP64, "fp64"), (CC_MIXED, "mixed")):
         kernels[tag] = {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in res[prec]["prof"].items()
@@ -518,6 +527,7 @@ def main():
         "max_divB": res[CC_MIXED]["divb"],
         "roofline": roof_mx,
         "cpu_baseline": cpu,
+        "sor_tile": {"fp64": res[CC_FP64]["tile"], "mixed": res[CC_MIXED]["tile"]},
         "kernel_times": kernels,
         "turb3d": t3d,
         "batch2d": bt,

@@ -215,6 +215,8 @@ _PROTOS = {
     "cc_euler3d_ic_sphere": (C.c_int, [C.POINTER(Euler3dParams), C.POINTER(Sphere3dIc)] + [DP] * 5),
     "cc_euler2d_ic_defaults": (None, [C.POINTER(Euler2dIc)]),
     "cc_euler2d_ic_reference": (C.c_int, [C.POINTER(Euler2dParams), C.POINTER(Euler2dIc)] + [DP] * 4),
+    "cc_sor_autotune": (C.c_int, [C.c_int]),
+    "cc_sor_tile_choice": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "cc_mhd2d_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
     "cc_mhd2d_batch_destroy": (C.c_int, [C.c_void_p]),
     "cc_mhd2d_batch_run": (C.c_int, [C.c_void_p, C.c_int, DP, DP]),

@@ -3,6 +3,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <limits>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
 
 #include "poisson2d.hpp"
 
@@ -483,50 +488,138 @@ cc_status Poisson2D::init(int Nx_, int Ny_, bool px_, bool py_, double dx_, doub
 
     level_off_.assign(1, 0);
     if (tiled_ && iters > 0) {
-        SorTile t = precision == CC_MIXED ? kTileMixedDefault : kTileFp64Default;
+        const bool mixed = precision == CC_MIXED;
+        SorTile t = mixed ? kTileMixedDefault : kTileFp64Default;
+        bool from_env = false;
         if (const char* e = std::getenv("CC_SOR_TILE")) {
             int S_ = 0, WJ_ = 0;
             char g = 0;
             const int n = std::sscanf(e, "%d,%d,%c", &S_, &WJ_, &g);
-            if (n >= 2) t = SorTile{S_, WJ_, !(n == 3 && g == 'g')};
+            if (n >= 2) { t = SorTile{S_, WJ_, !(n == 3 && g == 'g')}; from_env = true; }
         }
-        if (!sor_tile_supported(t, precision == CC_MIXED)) {
+        if (!from_env && sor_autotune_enabled()) CC_TRY(autotuned_tile(t));
+        if (!sor_tile_supported(t, mixed)) {
             set_error(CC_ERR_INVALID_ARG, "Poisson2D: unsupported SOR tile shape");
             return CC_ERR_INVALID_ARG;
         }
         tile_ = t;
-        const int W = kTileW, WJ = t.WJ, S = t.S;
-        const int nTS = (iters + S - 1) / S;
-        const int nTI = (Nx + iters - 1 + W - 1) / W;
-        const int nTJ = (Ny + iters - 1 + WJ - 1) / WJ;
-        std::vector<std::vector<int>> lv(nTS + nTI + nTJ);
-        for (int TS = 0; TS < nTS; ++TS) {
-            const int s0 = S * TS, s1 = std::min(s0 + S, iters);
-            for (int TJ = 0; TJ < nTJ; ++TJ)
-                for (int TI = 0; TI < nTI; ++TI) {
-                    bool nonempty = false;
-                    for (int s = s0; s < s1 && !nonempty; ++s) {
-                        const int ia = W * TI - s, ib = ia + W - 1, ja = WJ * TJ - s, jb = ja + WJ - 1;
-                        nonempty = ib >= 0 && ia <= Nx - 1 && jb >= 0 && ja <= Ny - 1;
-                    }
-                    if (nonempty) {
-                        auto& v = lv[TS + TJ + TI];
-                        v.push_back(TS); v.push_back(TJ); v.push_back(TI);
-                    }
-                }
-        }
         std::vector<int> flat;
-        for (auto& v : lv) {
-            if (v.empty()) continue;
-            flat.insert(flat.end(), v.begin(), v.end());
-            level_off_.push_back((int)flat.size() / 3);
-        }
+        sor_levels(t, Nx, Ny, iters, flat, level_off_);
         if (hipMalloc(&sched, flat.size() * sizeof(int)) != hipSuccess) {
             set_error(CC_ERR_OOM, "Poisson2D: schedule allocation failed");
             return CC_ERR_OOM;
         }
         CC_HIP(hipMemcpy(sched, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice));
     }
+    return CC_OK;
+}
+
+// The tile levels of the tiled schedule: (TS, TJ, TI) of every tile that updates at least one cell,
+// grouped by level TS + TJ + TI (flat, 3 ints per tile), and the level offsets (in tiles).
+void sor_levels(const SorTile& t, int Nx, int Ny, int iters, std::vector<int>& flat, std::vector<int>& level_off) {
+    const int W = kTileW, WJ = t.WJ, S = t.S;
+    const int nTS = (iters + S - 1) / S;
+    const int nTI = (Nx + iters - 1 + W - 1) / W;
+    const int nTJ = (Ny + iters - 1 + WJ - 1) / WJ;
+    std::vector<std::vector<int>> lv(nTS + nTI + nTJ);
+    for (int TS = 0; TS < nTS; ++TS) {
+        const int s0 = S * TS, s1 = std::min(s0 + S, iters);
+        for (int TJ = 0; TJ < nTJ; ++TJ)
+            for (int TI = 0; TI < nTI; ++TI) {
+                bool nonempty = false;
+                for (int s = s0; s < s1 && !nonempty; ++s) {
+                    const int ia = W * TI - s, ib = ia + W - 1, ja = WJ * TJ - s, jb = ja + WJ - 1;
+                    nonempty = ib >= 0 && ia <= Nx - 1 && jb >= 0 && ja <= Ny - 1;
+                }
+                if (nonempty) {
+                    auto& v = lv[TS + TJ + TI];
+                    v.push_back(TS); v.push_back(TJ); v.push_back(TI);
+                }
+            }
+    }
+    flat.clear();
+    level_off.assign(1, 0);
+    for (auto& v : lv) {
+        if (v.empty()) continue;
+        flat.insert(flat.end(), v.begin(), v.end());
+        level_off.push_back((int)flat.size() / 3);
+    }
+}
+
+// ------------------------------------------------------------------ tile autotuner
+// With autotuning on, the tile shape of a grid is the compiled shape whose full solve (iters sweeps)
+// runs fastest on this device, measured once per (Nx, Ny, iters, precision) in the process with HIP
+// events on a zero density and kept in a process-wide table.
+namespace {
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int>, SorTile> g_tuned;
+bool g_autotune = false;
+}  // namespace
+
+bool sor_autotune_enabled() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    return g_autotune;
+}
+
+cc_status Poisson2D::autotuned_tile(SorTile& out) {
+    const bool mixed = precision == CC_MIXED;
+    const auto key = std::make_tuple(Nx, Ny, iters, precision);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) { out = it->second; return CC_OK; }
+    }
+    std::vector<SorTile> cands;
+#define CC_SOR_CAND64(S_, WJ_, R_) if (!mixed) cands.push_back(SorTile{S_, WJ_, R_});
+#define CC_SOR_CAND32(S_, WJ_) if (mixed) cands.push_back(SorTile{S_, WJ_, true});
+    CC_SOR_TILES_FP64(CC_SOR_CAND64)
+    CC_SOR_TILES_MIXED(CC_SOR_CAND32)
+#undef CC_SOR_CAND64
+#undef CC_SOR_CAND32
+    const size_t N = (size_t)Nx * Ny;
+    double* rz = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipMalloc(&rz, N * sizeof(double)) != hipSuccess) {
+        set_error(CC_ERR_OOM, "SOR autotune: allocation failed");
+        return CC_ERR_OOM;
+    }
+    cc_status st = CC_OK;
+    if (hipMemset(rz, 0, N * sizeof(double)) != hipSuccess || hipMemset(phi, 0, N * sizeof(double)) != hipSuccess ||
+        hipMemset(rhobar, 0, 4 * sizeof(double)) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
+        set_error(CC_ERR_HIP, "SOR autotune: setup failed");
+        st = CC_ERR_HIP;
+    }
+    float best = std::numeric_limits<float>::max();
+    SorTile bt = out;
+    for (const SorTile& c : cands) {
+        if (st != CC_OK) break;
+        std::vector<int> flat, off;
+        sor_levels(c, Nx, Ny, iters, flat, off);
+        int* sc = nullptr;
+        if (hipMalloc(&sc, flat.size() * sizeof(int)) != hipSuccess) { st = CC_ERR_OOM; set_error(st, "SOR autotune: allocation failed"); break; }
+        float ms = std::numeric_limits<float>::max();
+        if (hipMemcpy(sc, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) st = CC_ERR_HIP;
+        for (int rep = 0; rep < 2 && st == CC_OK; ++rep) {  // the first run pays the first-launch costs
+            hipEventRecord(e0, 0);
+            for (size_t l = 0; l + 1 < off.size() && st == CC_OK; ++l)
+                st = launch_tile(c, mixed, off[l + 1] - off[l], 0, args, sc + 3 * off[l], phi, rz, rhobar);
+            hipEventRecord(e1, 0);
+            float t = 0.f;
+            if (st == CC_OK && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&t, e0, e1) == hipSuccess)
+                ms = rep ? std::min(ms, t) : ms;
+            else if (st == CC_OK) { st = CC_ERR_HIP; set_error(st, "SOR autotune: timing failed"); }
+        }
+        hipFree(sc);
+        if (st == CC_OK && ms < best) { best = ms; bt = c; }
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipFree(rz);
+    if (st != CC_OK) return st;
+    out = bt;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[key] = bt;
     return CC_OK;
 }
 
@@ -594,3 +687,26 @@ cc_status Poisson2D::phi_mean_removed(double* out_dev, hipStream_t s) {
 }
 
 }  // namespace cc
+
+extern "C" {
+
+cc_status cc_sor_autotune(int enable) {
+    std::lock_guard<std::mutex> lk(cc::g_tune_mu);
+    cc::g_autotune = enable != 0;
+    return CC_OK;
+}
+
+cc_status cc_sor_tile_choice(int Nx, int Ny, int sor_iters, int precision, int* S, int* WJ) {
+    if (!S || !WJ) { cc::set_error(CC_ERR_INVALID_ARG, "null output"); return CC_ERR_INVALID_ARG; }
+    cc::SorTile t = precision == CC_MIXED ? cc::kTileMixedDefault : cc::kTileFp64Default;
+    {
+        std::lock_guard<std::mutex> lk(cc::g_tune_mu);
+        auto it = cc::g_tuned.find(std::make_tuple(Nx, Ny, sor_iters, precision));
+        if (it != cc::g_tuned.end()) t = it->second;
+    }
+    *S = t.S;
+    *WJ = t.WJ;
+    return CC_OK;
+}
+
+}  // extern "C"

@@ -43,6 +43,8 @@ class Poisson2D {
    public:
     cc_status init(int Nx, int Ny, bool px, bool py, double dx, double dy, double G, int iters, double omega,
                    int precision);
+    // The fastest compiled tile shape for this grid (timed once per grid in the process, see cc_sor_autotune).
+    cc_status autotuned_tile(SorTile& out);
     void destroy();
     // g = -grad phi(rho); rho, gx, gy are device arrays of Nx*Ny.
     cc_status solve(const double* rho, double* gx, double* gy, hipStream_t s, Profiler* prof);
@@ -65,6 +67,10 @@ class Poisson2D {
     std::vector<int> level_off_;
     SorArgs args{};
 };
+
+// The tile levels of the tiled schedule for a tile shape (flat (TS, TJ, TI) triples, level offsets).
+void sor_levels(const SorTile& t, int Nx, int Ny, int iters, std::vector<int>& flat, std::vector<int>& level_off);
+bool sor_autotune_enabled();
 
 // Deterministic device mean of n doubles into out (two-pass fixed-order reduction).
 cc_status device_mean(const double* x, size_t n, double* partial, double* out, hipStream_t s);

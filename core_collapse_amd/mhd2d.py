@@ -178,6 +178,18 @@ class Mhd2D:
         return n.value, ms.value
 
 
+def sor_autotune(enable=True):
+    """Time the compiled SOR tile shapes once per grid size for handles created afterwards (cc_sor_autotune)."""
+    check(_lib.lib().cc_sor_autotune(int(bool(enable))))
+
+
+def sor_tile_choice(Nx, Ny, sor_iters=600, precision=CC_FP64):
+    """(sweeps per tile, rows per tile) of the SOR schedule a grid gets (cc_sor_tile_choice)."""
+    S, WJ = C.c_int(), C.c_int()
+    check(_lib.lib().cc_sor_tile_choice(int(Nx), int(Ny), int(sor_iters), int(precision), C.byref(S), C.byref(WJ)))
+    return S.value, WJ.value
+
+
 class Mhd2DBatch:
     """Independent Mhd2D instances stepped together: one HIP graph launch per step runs every
     instance's Heun step as a parallel branch, each with its own dt (cc_mhd2d_batch_*)."""

@@ -213,6 +213,14 @@ cc_status cc_mhd3d_compare(cc_mhd3d* a, cc_mhd3d* b, double* out);
 cc_status cc_hlld3d_batch(const cc_mhd3d_params* p, int n, const double* WL, const double* WR, const double* Bn,
                           double* out);
 
+/* SOR tile autotuning (no reference counterpart).  With enable != 0, every 2D MHD handle created
+ * afterwards with clamped boundaries times the compiled SOR tile shapes (sweeps per tile x rows) on
+ * its grid once -- one full solve each, HIP events -- and keeps the fastest; the choice is cached per
+ * (Nx, Ny, sor_iters, precision) for the process.  The environment variable CC_SOR_TILE overrides it.
+ * cc_sor_tile_choice reports the shape a grid gets (the default shape if it was never tuned). */
+cc_status cc_sor_autotune(int enable);
+cc_status cc_sor_tile_choice(int Nx, int Ny, int sor_iters, int precision, int* S, int* WJ);
+
 /* ============================================================================================
  * 2D Euler HLLC + SOR self-gravity (EH).
  * ========================================================================================== */

@@ -263,3 +263,35 @@ def test_batch_equals_instances_run_alone(prec):
             assert np.array_equal(s1[f], s2[f]), f
     for h in alone + batch_sims:
         h.close()
+
+
+@pytest.mark.parametrize("prec", [CC_FP64, CC_MIXED])
+def test_sor_autotune_picks_a_compiled_shape_and_keeps_parity(prec):
+    """cc_sor_autotune: the handle times every compiled tile shape once for its grid and runs with the
+    fastest; the solve is the same lexicographic iterate whatever the shape (FP64 against the oracle at
+    the tolerance of test_collapse_run_matches_oracle, mixed at the 1e-4 L1 bar)."""
+    N, nsteps = 80, 4
+    p = ic.collapse2d_params(N, precision=prec)
+    st = ic.collapse2d_ic(p)
+    M.sor_autotune(True)
+    try:
+        with M.Mhd2D(p) as h:
+            h.upload(st)
+            t = h.run(nsteps)
+            got = h.download()
+        S, WJ = M.sor_tile_choice(N, N, p.sor_iters, prec)
+        print("autotuned SOR tile", S, WJ, flush=True)
+        assert (S, WJ) in {(16, 64), (32, 64), (16, 128), (32, 128)}
+    finally:
+        M.sor_autotune(False)
+    o = oracle()
+    S_ = m2_arrays(p, st)
+    ot = o.or_m2_run(C.byref(m2_oparams(p)), C.byref(S_.cstate()), C.c_double(p.dx), C.c_double(p.dy), nsteps,
+                     C.c_double(0.0), C.c_double(1e300), None)
+    want = m2_dict(p, S_)
+    if prec == CC_FP64:
+        assert abs(t - ot) < 1e-11 * ot
+        for f in M.FIELDS:
+            assert rel_err(got[f], want[f]) < 1e-9, f
+    else:
+        assert rel_l1(got["rho"], want["rho"]) <= 1e-4
