@@ -180,11 +180,12 @@ struct EmfArr {
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_prim3d(Geom3D g, Eos3D e, State3D U0, PrimArrays<R> W) {
-    const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (q >= (size_t)g.Nx * g.Ny * g.Nz) return;
-    const int i = (int)(q % g.Nx);
-    const size_t r = q / g.Nx;
-    const int j = (int)(r % g.Ny), k = (int)(r / g.Ny);
+    const unsigned qq = blockIdx.x * 256u + threadIdx.x;  // cells < 2^31 (checked at create)
+    if (qq >= (unsigned)g.Nx * (unsigned)g.Ny * (unsigned)g.Nz) return;
+    const size_t q = qq;
+    const int i = (int)(qq % (unsigned)g.Nx);
+    const unsigned r = qq / (unsigned)g.Nx;
+    const int j = (int)(r % (unsigned)g.Ny), k = (int)(r / (unsigned)g.Ny);
     const Prim<double> P = cons_to_prim3d(e, U0.rho[q], U0.mx[q], U0.my[q], U0.mz[q], U0.E[q], bxc(g, U0.Bx, i, j, k),
                                           byc(g, U0.By, i, j, k), bzc(g, U0.Bz, i, j, k));
     W.c[0][q] = (R)P.rho; W.c[1][q] = (R)P.u; W.c[2][q] = (R)P.v; W.c[3][q] = (R)P.w;
@@ -244,12 +245,21 @@ __global__ __launch_bounds__(256) void k_face3d(Geom3D g, EOS64 e64, EOS32 e32,
     using R = typename std::conditional<MIXED, float, double>::type;
     const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
     const int nx = Nx + (AX == 0), ny = Ny + (AX == 1);
-    const size_t nf = (size_t)nx * ny * (Nz + (AX == 2));
-    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (t >= nf) return;
-    const int i = (int)(t % nx);
-    const size_t r = t / nx;
-    const int j = (int)(r % ny), k = (int)(r / ny);
+    int i, j, k;
+    if (AX == 0) {  // consecutive threads take consecutive faces of the (Nx+1) x Ny x Nz array
+        const unsigned tt = blockIdx.x * 256u + threadIdx.x;
+        if (tt >= (unsigned)nx * (unsigned)ny * (unsigned)Nz) return;
+        i = (int)(tt % (unsigned)nx);
+        const unsigned r = tt / (unsigned)nx;
+        j = (int)(r % (unsigned)ny);
+        k = (int)(r / (unsigned)ny);
+    } else {  // a block is 64 consecutive i x 4 consecutive faces along AX, so its waves share cells in L1
+        i = blockIdx.x * 64 + (threadIdx.x & 63);
+        const int fa = blockIdx.z * 4 + (threadIdx.x >> 6);
+        if (AX == 1) { j = fa; k = blockIdx.y; } else { k = fa; j = blockIdx.y; }
+        if (i >= Nx || fa > (AX == 1 ? Ny : Nz)) return;
+    }
+    const size_t t = ((size_t)k * ny + j) * nx + i;
     const int f = AX == 0 ? i : (AX == 1 ? j : k);
     auto cell = [&](int o) {  // index of the cell o steps from the face along the axis
         int ii = i, jj = j, kk = k;
@@ -297,11 +307,12 @@ __global__ __launch_bounds__(256) void k_face3d(Geom3D g, EOS64 e64, EOS32 e32,
 template <typename R>
 __global__ __launch_bounds__(256) void k_update3d(Geom3D g, Eos3D e, State3D U0, FaceFlux<R> FX, FaceFlux<R> FY,
                                                   FaceFlux<R> FZ, const Scalars* __restrict__ sc, State3D Us) {
-    const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (q >= (size_t)g.Nx * g.Ny * g.Nz) return;
-    const int i = (int)(q % g.Nx);
-    const size_t r = q / g.Nx;
-    const int j = (int)(r % g.Ny), k = (int)(r / g.Ny);
+    const unsigned qq = blockIdx.x * 256u + threadIdx.x;  // cells < 2^31 (checked at create)
+    if (qq >= (unsigned)g.Nx * (unsigned)g.Ny * (unsigned)g.Nz) return;
+    const size_t q = qq;
+    const int i = (int)(qq % (unsigned)g.Nx);
+    const unsigned r = qq / (unsigned)g.Nx;
+    const int j = (int)(r % (unsigned)g.Ny), k = (int)(r / (unsigned)g.Ny);
     const size_t xL = fx_(g, i, j, k), yD = fy_(g, i, j, k), zB = fz_(g, i, j, k);
     const size_t xR = xL + 1, yU = yD + g.Nx, zF = zB + (size_t)g.Nx * g.Ny;
     const double invdx = 1.0 / g.dx, invdy = 1.0 / g.dy, invdz = 1.0 / g.dz, dt = sc->dt;
@@ -318,10 +329,10 @@ __global__ __launch_bounds__(256) void k_update3d(Geom3D g, Eos3D e, State3D U0,
 
 // ------------------------------------------------------------------ edge EMFs (M3:836-939)
 template <typename R>
-__global__ void k_edge3d(Geom3D g, EmfArr<R> ef, State3D B0, Edges ed) {
-    const int ia = blockIdx.x * blockDim.x + threadIdx.x, ja = blockIdx.y, ka = blockIdx.z;
+__global__ __launch_bounds__(256) void k_edge3d(Geom3D g, EmfArr<R> ef, State3D B0, Edges ed) {
+    const int ia = blockIdx.x * 64 + (threadIdx.x & 63), ja = blockIdx.y, ka = blockIdx.z * 4 + (threadIdx.x >> 6);
     const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
-    if (ia > Nx) return;
+    if (ia > Nx || ka > Nz) return;
     const bool px = g.px, py = g.py, pz = g.pz;
     auto exy = [&](int i, int j, int k) { return (double)ef.Ex_y[fy_(g, bmap(i, Nx, px), fmap(j, Ny, py), bmap(k, Nz, pz))]; };
     auto ezx = [&](int i, int j, int k) { return (double)ef.Ez_x[fx_(g, fmap(i, Nx, px), bmap(j, Ny, py), bmap(k, Nz, pz))]; };
@@ -366,10 +377,10 @@ __global__ void k_edge3d(Geom3D g, EmfArr<R> ef, State3D B0, Edges ed) {
 }
 
 // ------------------------------------------------------------------ CT (M3:941-1015)
-__global__ void k_ct3d(Geom3D g, Edges ed, State3D B0, State3D B1, const Scalars* __restrict__ sc) {
-    const int ia = blockIdx.x * blockDim.x + threadIdx.x, ja = blockIdx.y, ka = blockIdx.z;
+__global__ __launch_bounds__(256) void k_ct3d(Geom3D g, Edges ed, State3D B0, State3D B1, const Scalars* __restrict__ sc) {
+    const int ia = blockIdx.x * 64 + (threadIdx.x & 63), ja = blockIdx.y, ka = blockIdx.z * 4 + (threadIdx.x >> 6);
     const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
-    if (ia > Nx) return;
+    if (ia > Nx || ka > Nz) return;
     const double dt = sc->dt;
     auto EX = [&](int i, int jf, int kf) { return ed.Ex[((size_t)kf * (Ny + 1) + jf) * Nx + i]; };
     auto EY = [&](int i, int j, int kf) { return ed.Ey[((size_t)kf * Ny + j) * (Nx + 1) + i]; };
@@ -397,10 +408,11 @@ template <bool FINAL>
 __global__ __launch_bounds__(256) void k_joule3d(Geom3D g, Eos3D e, State3D Us, State3D B1,
                                                  const Scalars* __restrict__ sc, State3D Un, State3D Out,
                                                  double* __restrict__ spd) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    // 64 consecutive i x 4 consecutive k of one j row per block (the z neighbours then share L1)
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y, k = blockIdx.z * 4 + (threadIdx.x >> 6);
     const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
     double speed = 0.0;
-    if (i < Nx) {
+    if (i < Nx && k < Nz) {
         const size_t q = ((size_t)k * Ny + j) * Nx + i;
         double rho = Us.rho[q], mx = Us.mx[q], my = Us.my[q], mz = Us.mz[q], E = Us.E[q];
         const double dt = sc->dt;
@@ -446,10 +458,18 @@ __global__ __launch_bounds__(256) void k_joule3d(Geom3D g, Eos3D e, State3D Us, 
 }
 
 __global__ __launch_bounds__(256) void k_speed_max3d(Geom3D g, Eos3D e, State3D U, double* __restrict__ spd) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y, k = blockIdx.z * 4 + (threadIdx.x >> 6);
     double speed = 0.0;
-    if (i < g.Nx) speed = signal_speed3d(e, prim_at(g, e, U, i, j, k));
+    if (i < g.Nx && k < g.Nz) speed = signal_speed3d(e, prim_at(g, e, U, i, j, k));
     block_max_to(speed, spd);
+}
+
+// First level of the dt reduction: nspd per-block maxima -> kDtPart3 values.
+constexpr int kDtPart3 = 256;
+__global__ __launch_bounds__(256) void k_partmax3d(const double* __restrict__ part, int n, double* __restrict__ out) {
+    double m = 0.0;
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) m = smax(m, part[k]);
+    block_max_to(m, out);
 }
 
 __global__ __launch_bounds__(256) void k_dt_finalize3d(Geom3D g, Scalars* sc, double* dts, const double* spd, int nspd) {
@@ -458,9 +478,9 @@ __global__ __launch_bounds__(256) void k_dt_finalize3d(Geom3D g, Scalars* sc, do
 }
 
 __global__ __launch_bounds__(256) void k_divb_max3d(Geom3D g, State3D U, double* __restrict__ spd) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y, k = blockIdx.z * 4 + (threadIdx.x >> 6);
     double v = 0.0;
-    if (i < g.Nx)  // divB_cc, M3:1070-1078 (face i+1 <= Nx etc. need no wrapping)
+    if (i < g.Nx && k < g.Nz)  // divB_cc, M3:1070-1078 (face i+1 <= Nx etc. need no wrapping)
         v = fabs((U.Bx[fx_(g, i + 1, j, k)] - U.Bx[fx_(g, i, j, k)]) / g.dx +
                  (U.By[fy_(g, i, j + 1, k)] - U.By[fy_(g, i, j, k)]) / g.dy +
                  (U.Bz[fz_(g, i, j, k + 1)] - U.Bz[fz_(g, i, j, k)]) / g.dz);
@@ -552,6 +572,7 @@ struct cc_mhd3d {
     Scalars* sc = nullptr;
     double* dts_dev = nullptr;
     double* spd = nullptr;
+    double* spd2 = nullptr;  // kDtPart3 second-level maxima
     double* red = nullptr;
     int nspd = 0;
     Profiler prof;
@@ -570,6 +591,8 @@ struct cc_mhd3d {
         return CC_OK;
     }
     size_t NFx() const { return (size_t)(p.Nx + 1) * p.Ny * p.Nz; }
+    // grid of the cell kernels that reduce per block: 64 i x 4 k of one j row per 256-thread block
+    dim3 cell_grid() const { return dim3((p.Nx + 63) / 64, p.Ny, (p.Nz + 3) / 4); }
     size_t NFy() const { return (size_t)p.Nx * (p.Ny + 1) * p.Nz; }
     size_t NFz() const { return (size_t)p.Nx * p.Ny * (p.Nz + 1); }
     Profiler* pr() { return prof.on ? &prof : nullptr; }
@@ -647,13 +670,15 @@ struct cc_mhd3d {
         }
         {
             ProfScope ps(pr(), stream, "face");
-            hipLaunchKernelGGL((k_face3d<1, MIXED, EOS64, EOS32>), dim3(blocks(NFy())), dim3(256), 0, stream, g, e64, e32,
+            hipLaunchKernelGGL((k_face3d<1, MIXED, EOS64, EOS32>), dim3((p.Nx + 63) / 64, p.Nz, (p.Ny + 1 + 3) / 4),
+                               dim3(256), 0, stream, g, e64, e32,
                                W<R>(), in.By, FF<R>(1), EF<R>());
             CC_CHECK_LAUNCH();
         }
         {
             ProfScope ps(pr(), stream, "face");
-            hipLaunchKernelGGL((k_face3d<2, MIXED, EOS64, EOS32>), dim3(blocks(NFz())), dim3(256), 0, stream, g, e64, e32,
+            hipLaunchKernelGGL((k_face3d<2, MIXED, EOS64, EOS32>), dim3((p.Nx + 63) / 64, p.Ny, (p.Nz + 1 + 3) / 4),
+                               dim3(256), 0, stream, g, e64, e32,
                                W<R>(), in.Bz, FF<R>(2), EF<R>());
             CC_CHECK_LAUNCH();
         }
@@ -685,22 +710,22 @@ struct cc_mhd3d {
         {
             ProfScope ps(pr(), stream, "edge");
             if (p.precision == CC_MIXED)
-                hipLaunchKernelGGL(k_edge3d<float>, dim3((Nx + 1 + 63) / 64, Ny + 1, Nz + 1), dim3(64), 0, stream, g,
+                hipLaunchKernelGGL(k_edge3d<float>, dim3((Nx + 1 + 63) / 64, Ny + 1, (Nz + 1 + 3) / 4), dim3(256), 0, stream, g,
                                    EF<float>(), in, ed);
             else
-                hipLaunchKernelGGL(k_edge3d<double>, dim3((Nx + 1 + 63) / 64, Ny + 1, Nz + 1), dim3(64), 0, stream, g,
+                hipLaunchKernelGGL(k_edge3d<double>, dim3((Nx + 1 + 63) / 64, Ny + 1, (Nz + 1 + 3) / 4), dim3(256), 0, stream, g,
                                    EF<double>(), in, ed);
             CC_CHECK_LAUNCH();
         }
         const State3D& bdst = FINAL ? Bt : out;
         {
             ProfScope ps(pr(), stream, "ct");
-            hipLaunchKernelGGL(k_ct3d, dim3((Nx + 1 + 63) / 64, Ny + 1, Nz + 1), dim3(64), 0, stream, g, ed, in, bdst, sc);
+            hipLaunchKernelGGL(k_ct3d, dim3((Nx + 1 + 63) / 64, Ny + 1, (Nz + 1 + 3) / 4), dim3(256), 0, stream, g, ed, in, bdst, sc);
             CC_CHECK_LAUNCH();
         }
         {
             ProfScope ps(pr(), stream, "joule");
-            hipLaunchKernelGGL(k_joule3d<FINAL>, dim3((Nx + 255) / 256, Ny, Nz), dim3(256), 0, stream, g, e, Us, bdst, sc, hn,
+            hipLaunchKernelGGL(k_joule3d<FINAL>, cell_grid(), dim3(256), 0, stream, g, e, Us, bdst, sc, hn,
                                out, spd);
             CC_CHECK_LAUNCH();
         }
@@ -710,7 +735,8 @@ struct cc_mhd3d {
     cc_status heun(int a, int b, int c, bool driver, double* dts) {
         if (driver) {
             ProfScope ps(pr(), stream, "dt");
-            hipLaunchKernelGGL(k_dt_finalize3d, dim3(1), dim3(256), 0, stream, g, sc, dts, spd, nspd);
+            hipLaunchKernelGGL(k_partmax3d, dim3(kDtPart3), dim3(256), 0, stream, spd, nspd, spd2);
+            hipLaunchKernelGGL(k_dt_finalize3d, dim3(1), dim3(256), 0, stream, g, sc, dts, spd2, kDtPart3);
             CC_CHECK_LAUNCH();
         }
         CC_TRY(stage<false>(S[a], S[b], S[a]));
@@ -719,7 +745,7 @@ struct cc_mhd3d {
     }
 
     cc_status speed_max(const State3D& s) {
-        hipLaunchKernelGGL(k_speed_max3d, dim3((p.Nx + 255) / 256, p.Ny, p.Nz), dim3(256), 0, stream, g, e, s, spd);
+        hipLaunchKernelGGL(k_speed_max3d, cell_grid(), dim3(256), 0, stream, g, e, s, spd);
         CC_CHECK_LAUNCH();
         return CC_OK;
     }
@@ -775,8 +801,12 @@ cc_status cc_mhd3d_create(const cc_mhd3d_params* p, cc_mhd3d** out) {
     if (st == CC_OK) st = h->alloc(&h->ed.Ey, (size_t)(p->Nx + 1) * p->Ny * (p->Nz + 1));
     if (st == CC_OK) st = h->alloc(&h->ed.Ez, (size_t)(p->Nx + 1) * (p->Ny + 1) * p->Nz);
     if (st == CC_OK) st = h->alloc(&h->dts_dev, kDtLog3);
-    h->nspd = ((p->Nx + 255) / 256) * p->Ny * p->Nz;
+    {
+        const dim3 cg = h->cell_grid();  // the per-block maxima of the cell kernels (speed, Joule, divB)
+        h->nspd = (int)(cg.x * cg.y * cg.z);
+    }
     if (st == CC_OK) st = h->alloc(&h->spd, (size_t)h->nspd);
+    if (st == CC_OK) st = h->alloc(&h->spd2, (size_t)kDtPart3);
     if (st == CC_OK) st = h->alloc(&h->red, 4);
     if (st == CC_OK && hipMalloc((void**)&h->sc, sizeof(Scalars)) != hipSuccess) {
         set_error(CC_ERR_OOM, "scalar allocation failed");
@@ -916,7 +946,7 @@ cc_status cc_mhd3d_run(cc_mhd3d* h, int nsteps, double* t_inout, double t_end, d
 cc_status cc_mhd3d_divb_max(cc_mhd3d* h, double* out) {
     CHECK_H3();
     if (!out) { set_error(CC_ERR_INVALID_ARG, "null output"); return CC_ERR_INVALID_ARG; }
-    hipLaunchKernelGGL(k_divb_max3d, dim3((h->p.Nx + 255) / 256, h->p.Ny, h->p.Nz), dim3(256), 0, h->stream, h->g,
+    hipLaunchKernelGGL(k_divb_max3d, h->cell_grid(), dim3(256), 0, h->stream, h->g,
                        h->S[h->cur], h->spd);
     hipLaunchKernelGGL(k_reduce_max_to, dim3(1), dim3(256), 0, h->stream, h->spd, h->nspd, h->red);
     CC_CHECK_LAUNCH();
