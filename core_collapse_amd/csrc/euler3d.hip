@@ -89,14 +89,27 @@ __global__ __launch_bounds__(256) void k_e3_prim(GeomE3 g, StateE3 U, PrimE3<R> 
 }
 
 // Face between cells q - e_AX and q: states from cells q - 2e, q - e, q, q + e (periodic).
+// x: consecutive threads take consecutive faces; y, z: a block is 64 consecutive i x 4 consecutive faces
+// along the axis (its waves then share the stencil cells in L1).
 template <typename R, int AX>
 __global__ __launch_bounds__(256) void k_e3_face(GeomE3 g, PrimE3<R> W, FluxE3<R> F, size_t n) {
-    const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (q >= n) return;
     const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
-    const int i = (int)(q & (size_t)(Nx - 1));
-    const size_t line = q >> g.lx;
-    const int j = (int)(line & (size_t)(Ny - 1)), k = (int)(line >> g.ly);
+    int i, j, k;
+    size_t q;
+    if (AX == 0) {
+        q = (size_t)blockIdx.x * 256 + threadIdx.x;
+        if (q >= n) return;
+        i = (int)(q & (size_t)(Nx - 1));
+        const size_t line = q >> g.lx;
+        j = (int)(line & (size_t)(Ny - 1));
+        k = (int)(line >> g.ly);
+    } else {
+        i = blockIdx.x * 64 + (threadIdx.x & 63);
+        const int fa = blockIdx.z * 4 + (threadIdx.x >> 6);
+        if (AX == 1) { j = fa; k = blockIdx.y; } else { k = fa; j = blockIdx.y; }
+        if (i >= Nx || fa >= (AX == 1 ? Ny : Nz)) return;
+        q = ((size_t)k * Ny + j) * Nx + i;
+    }
     size_t qm2, qm1, qp1;
     if (AX == 0) {
         const size_t r = q - i;
@@ -283,14 +296,15 @@ struct cc_euler3d {
         }
         {
             ProfScope ps(pr(), stream, "face");
+            const dim3 gy((p.Nx + 63) / 64, p.Nz, (p.Ny + 3) / 4), gz((p.Nx + 63) / 64, p.Ny, (p.Nz + 3) / 4);
             if (mixed) {
                 hipLaunchKernelGGL((k_e3_face<float, 0>), dim3(blocks()), dim3(256), 0, stream, g, W32, F32, n);
-                hipLaunchKernelGGL((k_e3_face<float, 1>), dim3(blocks()), dim3(256), 0, stream, g, W32, F32, n);
-                hipLaunchKernelGGL((k_e3_face<float, 2>), dim3(blocks()), dim3(256), 0, stream, g, W32, F32, n);
+                hipLaunchKernelGGL((k_e3_face<float, 1>), gy, dim3(256), 0, stream, g, W32, F32, n);
+                hipLaunchKernelGGL((k_e3_face<float, 2>), gz, dim3(256), 0, stream, g, W32, F32, n);
             } else {
                 hipLaunchKernelGGL((k_e3_face<double, 0>), dim3(blocks()), dim3(256), 0, stream, g, W64, F64, n);
-                hipLaunchKernelGGL((k_e3_face<double, 1>), dim3(blocks()), dim3(256), 0, stream, g, W64, F64, n);
-                hipLaunchKernelGGL((k_e3_face<double, 2>), dim3(blocks()), dim3(256), 0, stream, g, W64, F64, n);
+                hipLaunchKernelGGL((k_e3_face<double, 1>), gy, dim3(256), 0, stream, g, W64, F64, n);
+                hipLaunchKernelGGL((k_e3_face<double, 2>), gz, dim3(256), 0, stream, g, W64, F64, n);
             }
             CC_CHECK_LAUNCH();
         }
