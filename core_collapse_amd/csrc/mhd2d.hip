@@ -883,8 +883,10 @@ cc_status cc_mhd2d_batch_run(cc_mhd2d_batch* b, int nsteps, double* t_inout, con
     if (!b || nsteps < 0 || !t_inout || !t_end) { set_error(CC_ERR_INVALID_ARG, "cc_mhd2d_batch_run: bad arguments"); return CC_ERR_INVALID_ARG; }
     if (nsteps > kDtLog) { set_error(CC_ERR_INVALID_ARG, "cc_mhd2d_batch_run: at most 65536 steps per call"); return CC_ERR_INVALID_ARG; }
     const int a0 = b->h[0]->cur;
-    for (auto* h : b->h)
+    for (auto* h : b->h) {
         if (h->cur != a0) { set_error(CC_ERR_INVALID_ARG, "cc_mhd2d_batch_run: instances at different rotation phases"); return CC_ERR_INVALID_ARG; }
+        if (h->prof.on) { set_error(CC_ERR_INVALID_ARG, "cc_mhd2d_batch_run: profiling is on for an instance (the batch graph cannot time kernels)"); return CC_ERR_INVALID_ARG; }
+    }
     if (nsteps == 0) return CC_OK;
     // per-instance start: t, t_end, the signal speeds of the current state (as cc_mhd2d_run)
     for (size_t i = 0; i < b->h.size(); i++) {

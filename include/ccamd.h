@@ -130,9 +130,9 @@ cc_status cc_mhd2d_compare(cc_mhd2d* a, cc_mhd2d* b, double* out);
 /* Batches of independent 2D MHD runs (no reference counterpart: the benchmark's batch of collapse
  * instances of uneven size).  The instances' Heun steps run as the parallel branches of one HIP
  * graph -- one graph launch per step for the whole batch -- each with its own dt on the device.
- * The handles stay owned by the caller; all must be at the same point of their state rotation
- * (e.g. freshly uploaded, or stepped only through the batch) and must not be used by other calls
- * while the batch runs. */
+ * The handles stay owned by the caller and must outlive the batch; all must be at the same point of
+ * their state rotation (e.g. freshly uploaded, or stepped only through the batch), have profiling
+ * off, and must not be used by other calls while the batch runs.  At most 65536 steps per call. */
 typedef struct cc_mhd2d_batch cc_mhd2d_batch;
 cc_status cc_mhd2d_batch_create(cc_mhd2d* const* handles, int n, cc_mhd2d_batch** out);
 cc_status cc_mhd2d_batch_destroy(cc_mhd2d_batch* b);
