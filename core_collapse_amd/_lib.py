@@ -201,7 +201,7 @@ _PROTOS = {
     "cc_euler3d_download": (C.c_int, [C.c_void_p] + [DP] * 5),
     "cc_euler3d_gravity": (C.c_int, [C.c_void_p] + [DP] * 4),
     "cc_euler3d_poisson_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
-                                           DP, DP, C.POINTER(C.c_int)]),
+                                           C.POINTER(C.c_int), DP, DP, C.POINTER(C.c_int)]),
     "cc_euler3d_rhs": (C.c_int, [C.c_void_p] + [DP] * 5),
     "cc_euler3d_compute_dt": (C.c_int, [C.c_void_p, DP]),
     "cc_euler3d_rk2_step": (C.c_int, [C.c_void_p, C.c_double]),

@@ -486,8 +486,8 @@ cc_status cc_euler3d_gravity(cc_euler3d* h, double* gx, double* gy, double* gz, 
     return CC_OK;
 }
 
-cc_status cc_euler3d_poisson_stats(cc_euler3d* h, int* solves, int* iters_last, int* iters_max, double* resid_last,
-                                   double* resid_worst, int* unconverged) {
+cc_status cc_euler3d_poisson_stats(cc_euler3d* h, int* solves, int* iters_last, int* iters_max, int* iters_total,
+                                   double* resid_last, double* resid_worst, int* unconverged) {
     CHECK_E3();
     Poisson3DStatus s{};
     if (h->p.gravity_on) {
@@ -497,6 +497,7 @@ cc_status cc_euler3d_poisson_stats(cc_euler3d* h, int* solves, int* iters_last, 
     if (solves) *solves = s.solves;
     if (iters_last) *iters_last = s.iters_last;
     if (iters_max) *iters_max = s.iters_max;
+    if (iters_total) *iters_total = s.iters_total;
     if (resid_last) *resid_last = s.resid;
     if (resid_worst) *resid_worst = s.resid_worst;
     if (unconverged) *unconverged = s.unconverged;

@@ -238,6 +238,7 @@ __global__ __launch_bounds__(kThreads) void k_p3_check(P3Args a, const double* _
         if (conv || it >= a.max_iter) {
             st->done = 1;
             st->iters_last = it;
+            st->iters_total += it;
             st->solves += 1;
             st->iters_max = it > st->iters_max ? it : st->iters_max;
             if (!conv) st->unconverged += 1;

@@ -32,7 +32,7 @@ struct Poisson3DStatus {
     int iters, done;               // corrections applied in the current solve; solve finished
     int solves, iters_max;         // solves run; most corrections any solve needed
     int unconverged, iters_last;   // solves that stopped at max_iter; corrections of the last solve
-    int pad[2];
+    int iters_total, pad;          // corrections over all solves
 };
 
 class Poisson3D {

@@ -112,11 +112,12 @@ class Euler3D:
         return tuple(out)
 
     def poisson_stats(self):
-        s, il, im, u = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        s, il, im, it, u = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
         rl, rw = C.c_double(), C.c_double()
-        check(_lib.lib().cc_euler3d_poisson_stats(self._h, C.byref(s), C.byref(il), C.byref(im), C.byref(rl),
-                                                  C.byref(rw), C.byref(u)))
-        return {"solves": s.value, "iters_last": il.value, "iters_max": im.value, "resid_last": rl.value,
+        check(_lib.lib().cc_euler3d_poisson_stats(self._h, C.byref(s), C.byref(il), C.byref(im), C.byref(it),
+                                                  C.byref(rl), C.byref(rw), C.byref(u)))
+        return {"solves": s.value, "iters_last": il.value, "iters_max": im.value, "iters_total": it.value,
+                "resid_last": rl.value,
                 "resid_worst": rw.value, "unconverged": u.value}
 
     def rhs(self):

@@ -359,10 +359,10 @@ cc_status cc_euler3d_upload(cc_euler3d* h, const double* rho, const double* mx, 
 cc_status cc_euler3d_download(cc_euler3d* h, double* rho, double* mx, double* my, double* mz, double* E);
 /* Gravity of the current state (any output may be null; phi has zero mean).  Syncs. */
 cc_status cc_euler3d_gravity(cc_euler3d* h, double* gx, double* gy, double* gz, double* phi);
-/* Refinement record over all solves so far: solves run, corrections of the last solve and the most
- * any needed, the last and worst final relative residual, solves that stopped at max_iter. */
-cc_status cc_euler3d_poisson_stats(cc_euler3d* h, int* solves, int* iters_last, int* iters_max, double* resid_last,
-                                   double* resid_worst, int* unconverged);
+/* Refinement record over all solves so far: solves run, corrections of the last solve, the most any
+ * needed and their total, the last and worst final relative residual, solves that stopped at max_iter. */
+cc_status cc_euler3d_poisson_stats(cc_euler3d* h, int* solves, int* iters_last, int* iters_max, int* iters_total,
+                                   double* resid_last, double* resid_worst, int* unconverged);
 /* rhs (gravity included) of the current state.  Syncs. */
 cc_status cc_euler3d_rhs(cc_euler3d* h, double* drho, double* dmx, double* dmy, double* dmz, double* dE);
 cc_status cc_euler3d_compute_dt(cc_euler3d* h, double* dt);

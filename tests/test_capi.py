@@ -45,3 +45,38 @@ def test_library_loads_and_reports_errors_without_a_device():
     assert isinstance(L.cc_last_error(), bytes)
     h = C.c_void_p()
     assert L.cc_mhd2d_create(None, C.byref(h)) == 1  # CC_ERR_INVALID_ARG before any device call
+
+
+def test_new_entry_points_reject_bad_arguments_before_touching_a_device():
+    """Argument errors of the batch, 3D Euler, SOR-autotune and sampler entry points come back as status
+    codes with a message, before any HIP call (so they hold on a machine without a GPU)."""
+    import ctypes as C
+    import numpy as np
+    from core_collapse_amd import _lib
+    L = _lib.lib()
+    b = C.c_void_p()
+    assert L.cc_mhd2d_batch_create(None, 4, C.byref(b)) == 1
+    arr = (C.c_void_p * 2)(None, None)
+    assert L.cc_mhd2d_batch_create(arr, 2, C.byref(b)) == 1 and b"null handle" in L.cc_last_error()
+    assert L.cc_mhd2d_batch_create(arr, 0, C.byref(b)) == 1
+    t = np.zeros(1)
+    assert L.cc_mhd2d_batch_run(None, 1, t.ctypes.data_as(_lib.DP), t.ctypes.data_as(_lib.DP)) == 1
+    assert L.cc_mhd2d_batch_destroy(None) == 0
+    h = C.c_void_p()
+    for Nx, Ny, Nz in ((24, 16, 16), (16, 4, 16), (16, 16, 2048)):
+        p = _lib.Euler3dParams(Nx=Nx, Ny=Ny, Nz=Nz, x1=1.0, y1=1.0, z1=1.0, gamma=5.0 / 3.0, cfl=0.4,
+                               poisson_tol=1e-10, poisson_max_iter=8)
+        assert L.cc_euler3d_create(C.byref(p), C.byref(h)) == 2, (Nx, Ny, Nz)  # CC_ERR_NON_POW2
+    p = _lib.Euler3dParams(Nx=16, Ny=16, Nz=16, x1=1.0, y1=1.0, z1=1.0, gamma=5.0 / 3.0, cfl=0.4,
+                           poisson_tol=1e-10, poisson_max_iter=99)
+    assert L.cc_euler3d_create(C.byref(p), C.byref(h)) == 1  # max_iter out of [0, 64]
+    p.poisson_max_iter, p.gamma = 8, 1.0
+    assert L.cc_euler3d_create(C.byref(p), C.byref(h)) == 1  # gamma <= 1
+    assert L.cc_euler3d_destroy(None) == 0
+    S, WJ = C.c_int(), C.c_int()
+    assert L.cc_sor_tile_choice(128, 128, 600, 1, C.byref(S), C.byref(WJ)) == 0 and (S.value, WJ.value) == (16, 64)
+    assert L.cc_sor_tile_choice(128, 128, 600, 1, None, C.byref(WJ)) == 1
+    N = (C.c_int * 1)()
+    k = (_lib.Collapse2dIc * 1)()
+    e = np.zeros(1)
+    assert L.cc_collapse2d_batch_sample(1, 1, 256, 128, N, k, e.ctypes.data_as(_lib.DP)) == 1  # nmax < nmin
