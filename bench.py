@@ -338,9 +338,10 @@ def bench_sphere3d(N, warmup, steps, with_cpu):
         if n == 0 or ms <= 0:
             return None
         # the stage update, one launch per stage (half of them the final stage): per cell it reads 15 face
-        # fluxes (3 axes x 5, in the flux precision), the state (5 FP64), gravity (3 FP64), on the final
-        # stage also U^n (5 FP64), and writes the state: 15 w + 104 B or 15 w + 144 B, 15 w + 124 B on average
-        bpl = cells * (15 * word + 124)
+        # fluxes (3 axes x 5, in the flux precision), the state (5 FP64), the potential (1 FP64, differenced
+        # for g), on the final stage also U^n (5 FP64), and writes the state: 15 w + 88 B or 15 w + 128 B,
+        # 15 w + 108 B on average
+        bpl = cells * (15 * word + 108)
         avg = ms / n * 1e-3
         t1, t2 = load_traffic("cc::k_e3_update<1, %s>" % tname), load_traffic("cc::k_e3_update<2, %s>" % tname)
         return {"bound": "hbm", "achieved": round(bpl / avg / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

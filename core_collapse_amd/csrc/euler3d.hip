@@ -6,8 +6,9 @@
 //   k_e3_prim       floored primitives (rho, u, v, w, P) of every cell (EH:415-425), FP64 or FP32
 //   k_e3_face<AX>   one thread per face along AX (N faces per axis, periodic): 4-point MUSCL-MC
 //                   face states (EH:437-455) and the EH HLLC with two transverse velocities
-//   k_e3_update     flux divergence + rho g source (EH:505-524) and the Heun stage with floors
-//                   (EH:529-550); the last stage also leaves the block maxima of the signal speed
+//   k_e3_update     flux divergence + rho g source (EH:505-524), g differenced from the potential on
+//                   the fly, and the Heun stage with floors (EH:529-550); the last stage also leaves
+//                   the block maxima of the signal speed
 // Heun: dt (device) -> g(U) -> U1 = U + dt rhs(U) -> g(U1) -> U = U/2 + (U1 + dt rhs(U1))/2, the
 // final update in place (each thread reads U only at its own cell).  Mixed precision keeps the
 // primitives, reconstruction, HLLC and face fluxes in FP32; state, update and gravity source FP64.
@@ -144,10 +145,11 @@ __global__ __launch_bounds__(256) void k_e3_face(GeomE3 g, PrimE3<R> W, FluxE3<R
 
 // MODE 0: out = rhs(Uin).  MODE 1: out = Uin + dt rhs, floors.  MODE 2: out = Uold/2 + (Uin + dt
 // rhs)/2, floors, and the block maxima of the new state's signal speed.
+// The gravity g = -central grad phi is differenced here from the potential (the expression of the
+// solver's own gradient kernel), so g is never stored.
 template <int MODE, typename R>
 __global__ __launch_bounds__(256) void k_e3_update(GeomE3 g, FluxE3<R> F, StateE3 Uin, StateE3 Uold, StateE3 out,
-                                                   const double* __restrict__ gx, const double* __restrict__ gy,
-                                                   const double* __restrict__ gz, const Scalars* __restrict__ sc,
+                                                   const double* __restrict__ phi, const Scalars* __restrict__ sc,
                                                    double* __restrict__ spd, size_t n) {
     const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
     double speed = 0.0;
@@ -172,7 +174,12 @@ __global__ __launch_bounds__(256) void k_e3_update(GeomE3 g, FluxE3<R> F, StateE
         if (g.gravity_on) {
             const double rho = smax(r0, g.floor_rho);
             const double u = m0 / rho, v = n0 / rho, w = o0 / rho;
-            const double ax = gx[q], ay = gy[q], az = gz[q];
+            const size_t qxm = q - i + ((i - 1) & (Nx - 1));
+            const size_t qym = q - (size_t)j * Nx + (size_t)((j - 1) & (Ny - 1)) * Nx;
+            const size_t qzm = q - (size_t)k * sxy + (size_t)((k - 1) & (Nz - 1)) * sxy;
+            const double ax = -(phi[qx] - phi[qxm]) / (2.0 * g.dx);
+            const double ay = -(phi[qy] - phi[qym]) / (2.0 * g.dy);
+            const double az = -(phi[qz] - phi[qzm]) / (2.0 * g.dz);
             d[1] += rho * ax;
             d[2] += rho * ay;
             d[3] += rho * az;
@@ -276,9 +283,10 @@ struct cc_euler3d {
         return CC_OK;
     }
 
-    cc_status gravity_of(const double* rho) {
-        if (!p.gravity_on) return CC_OK;  // g stays zero
-        return pois.solve(rho, gx, gy, gz, stream, pr());
+    // phi of rho (the stage update differences it); with arrays, also g into gx, gy, gz
+    cc_status gravity_of(const double* rho, bool arrays = false) {
+        if (!p.gravity_on) return CC_OK;
+        return arrays ? pois.solve(rho, gx, gy, gz, stream, pr()) : pois.solve(rho, nullptr, nullptr, nullptr, stream, pr());
     }
 
     // rhs of `in` and the stage update (MODE as in k_e3_update)
@@ -312,10 +320,10 @@ struct cc_euler3d {
             ProfScope ps(pr(), stream, "update");
             if (mixed)
                 hipLaunchKernelGGL((k_e3_update<MODE, float>), dim3(blocks()), dim3(256), 0, stream, g, F32, in, old, out,
-                                   gx, gy, gz, sc, spd, n);
+                                   pois.phi, sc, spd, n);
             else
                 hipLaunchKernelGGL((k_e3_update<MODE, double>), dim3(blocks()), dim3(256), 0, stream, g, F64, in, old,
-                                   out, gx, gy, gz, sc, spd, n);
+                                   out, pois.phi, sc, spd, n);
             CC_CHECK_LAUNCH();
         }
         return CC_OK;
@@ -470,7 +478,7 @@ cc_status cc_euler3d_download(cc_euler3d* h, double* rho, double* mx, double* my
 
 cc_status cc_euler3d_gravity(cc_euler3d* h, double* gx, double* gy, double* gz, double* phi) {
     CHECK_E3();
-    CC_TRY(h->gravity_of(h->S.rho));
+    CC_TRY(h->gravity_of(h->S.rho, true));
     const size_t b = h->N() * sizeof(double);
     if (!h->p.gravity_on) {
         CC_HIP(hipStreamSynchronize(h->stream));

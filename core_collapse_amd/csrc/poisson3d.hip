@@ -452,7 +452,7 @@ cc_status solve_t(Poisson3D& P, const double* rho, double* gx, double* gy, doubl
             CC_CHECK_LAUNCH();
         }
     }
-    {
+    if (gx) {  // without g arrays the caller takes g from phi itself
         ProfScope ps(prof, s, "grad");
         hipLaunchKernelGGL(k_p3_grad, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, a, P.phi,
                            gx, gy, gz, P.dx, P.dy, P.dz);

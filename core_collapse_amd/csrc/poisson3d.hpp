@@ -40,7 +40,8 @@ class Poisson3D {
     cc_status init(int Nx, int Ny, int Nz, double dx, double dy, double dz, double G, double floor_rho, double tol,
                    int max_iter, int precision, hipStream_t s);
     void destroy();
-    // g = -grad phi(rho); rho, gx, gy, gz are device arrays of Nx*Ny*Nz cells.
+    // phi(rho) and g = -central grad phi; rho, gx, gy, gz are device arrays of Nx*Ny*Nz cells (gx == nullptr:
+    // phi only, for callers that difference phi themselves).
     cc_status solve(const double* rho, double* gx, double* gy, double* gz, hipStream_t s, Profiler* prof);
     ~Poisson3D() { destroy(); }
 
