@@ -76,8 +76,11 @@ def test_hydro_rhs_and_steps_bitwise_without_gravity(shape):
 
 
 @pytest.mark.parametrize("prec", [CC_FP64, CC_MIXED])
-@pytest.mark.parametrize("shape", [(16, 32, 8), (32, 32, 32), (64, 16, 32)])
+@pytest.mark.parametrize("shape", [(16, 32, 8), (32, 32, 32), (64, 16, 32), (1024, 16, 8), (16, 1024, 8), (8, 16, 1024),
+                                   (128, 8, 256)])
 def test_fft_gravity_matches_oracle(prec, shape):
+    """Every radix-stage combination of the Stockham passes: N = 8 (8), 16 (8, 2), 32 (8, 4), 64 (8, 8),
+    128 (8, 8, 2), 256 (8, 8, 4), 1024 (8, 8, 8, 2), on each axis."""
     Nx, Ny, Nz = shape
     p = E.Params(Nx=Nx, Ny=Ny, Nz=Nz, x1=2.0, G=0.7, precision=prec, poisson_tol=1e-10, poisson_max_iter=12)
     rng = np.random.default_rng(2)
