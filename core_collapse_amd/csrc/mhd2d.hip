@@ -15,6 +15,7 @@
 // and CT stay FP64.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -810,21 +811,104 @@ struct cc_mhd2d_batch {
     hipEvent_t fork = nullptr;
     std::vector<hipEvent_t> join;
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};  // one step of every instance, by rotation phase
+    // Merged SOR (every instance with gravity, clamped boundaries, one precision and tile shape): the
+    // tile levels of all instances run as one launch per level on the root stream.
+    bool merged = false;
+    bool mixed = false;
+    SorTile tile{};
+    int* tiles4 = nullptr;           // (instance, TS, TJ, TI) per tile, grouped by level
+    std::vector<int> level_off;      // level offsets, in tiles
+    SorBatchEntry* tab[3][2] = {};   // per rotation phase and solve (U^n, U^1): each instance's SOR arguments
+
+    // Fork from the root stream to every instance stream, run fn(instance) there, join back.
+    template <typename F>
+    cc_status fan(F&& fn) {
+        if (hipEventRecord(fork, root) != hipSuccess) return CC_ERR_HIP;
+        for (size_t i = 0; i < h.size(); i++) {
+            if (hipStreamWaitEvent(h[i]->stream, fork, 0) != hipSuccess) return CC_ERR_HIP;
+            CC_TRY(fn(h[i]));
+            if (hipEventRecord(join[i], h[i]->stream) != hipSuccess || hipStreamWaitEvent(root, join[i], 0) != hipSuccess)
+                return CC_ERR_HIP;
+        }
+        return CC_OK;
+    }
+    cc_status merged_sweeps(int a, int q) {
+        for (size_t l = 0; l + 1 < level_off.size(); ++l)
+            CC_TRY(launch_tile_batch(tile, mixed, level_off[l + 1] - level_off[l], root, tab[a][q], tiles4 + 4 * level_off[l]));
+        return CC_OK;
+    }
+    // Can the instances share merged SOR launches?  If so build the merged schedule and the tables.
+    cc_status setup_merged() {
+        const cc_mhd2d* h0 = h[0];
+        for (const cc_mhd2d* x : h) {
+            const Poisson2D& P = x->pois;
+            if (!x->p.gravity_on || !P.tiled() || P.iters <= 0 || x->p.precision != h0->p.precision ||
+                P.tile_.S != h0->pois.tile_.S || P.tile_.WJ != h0->pois.tile_.WJ || P.tile_.rhs_lds != h0->pois.tile_.rhs_lds)
+                return CC_OK;  // not mergeable: the instances keep their own SOR launches
+        }
+        tile = h0->pois.tile_;
+        mixed = h0->p.precision == CC_MIXED;
+        std::vector<int> flat;
+        level_off.assign(1, 0);
+        for (int L = 0;; ++L) {
+            bool any = false;
+            for (size_t i = 0; i < h.size(); i++) {
+                const Poisson2D& P = h[i]->pois;
+                if (L >= P.levels()) continue;
+                any = true;
+                const std::vector<int>& sh = P.schedule_host();
+                for (int t = P.level_off_[L]; t < P.level_off_[L + 1]; ++t) {
+                    flat.push_back((int)i);
+                    flat.push_back(sh[3 * t]);
+                    flat.push_back(sh[3 * t + 1]);
+                    flat.push_back(sh[3 * t + 2]);
+                }
+            }
+            if (!any) break;
+            level_off.push_back((int)flat.size() / 4);
+        }
+        if (hipMalloc(&tiles4, flat.size() * sizeof(int)) != hipSuccess) { set_error(CC_ERR_OOM, "batch schedule allocation failed"); return CC_ERR_OOM; }
+        CC_HIP(hipMemcpy(tiles4, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice));
+        for (int a = 0; a < 3; ++a)
+            for (int q = 0; q < 2; ++q) {
+                std::vector<SorBatchEntry> e(h.size());
+                for (size_t i = 0; i < h.size(); i++) {
+                    Poisson2D& P = h[i]->pois;
+                    e[i] = SorBatchEntry{P.args, P.phi, h[i]->S[q == 0 ? a : (a + 1) % 3].rho, P.rhobar};
+                }
+                if (hipMalloc(&tab[a][q], e.size() * sizeof(SorBatchEntry)) != hipSuccess) { set_error(CC_ERR_OOM, "batch table allocation failed"); return CC_ERR_OOM; }
+                CC_HIP(hipMemcpy(tab[a][q], e.data(), e.size() * sizeof(SorBatchEntry), hipMemcpyHostToDevice));
+            }
+        merged = true;
+        return CC_OK;
+    }
+    // One Heun step of every instance (heun(a, b, c, true) of each) with the SOR sweeps merged:
+    // per instance mean + phi = 0 | merged sweeps | g, dt, stage 1, mean + phi = 0 | merged sweeps | g, stage 2.
+    cc_status capture_merged_body(int a) {
+        const int b = (a + 1) % 3, c = (a + 2) % 3;
+        CC_TRY(fan([&](cc_mhd2d* x) { return x->pois.prologue(x->S[a].rho, x->stream, nullptr); }));
+        CC_TRY(merged_sweeps(a, 0));
+        CC_TRY(fan([&](cc_mhd2d* x) {
+            CC_TRY(x->pois.epilogue(x->gx, x->gy, x->stream, nullptr));
+            hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(256), 0, x->stream, x->g, x->sc, x->dts_dev, x->spd, x->nspd);
+            CC_CHECK_LAUNCH();
+            CC_TRY(x->stage<false>(x->S[a], x->S[b], x->gx, x->gy, x->S[a]));
+            return x->pois.prologue(x->S[b].rho, x->stream, nullptr);
+        }));
+        CC_TRY(merged_sweeps(a, 1));
+        return fan([&](cc_mhd2d* x) {
+            CC_TRY(x->pois.epilogue(x->g1x, x->g1y, x->stream, nullptr));
+            return x->stage<true>(x->S[b], x->S[c], x->g1x, x->g1y, x->S[a]);
+        });
+    }
 
     // One step of every instance as parallel branches of one graph: the root stream forks to the
     // instances' streams and joins them back.
     cc_status capture(int a) {
         const int b = (a + 1) % 3, c = (a + 2) % 3;
         CC_HIP(hipStreamBeginCapture(root, hipStreamCaptureModeGlobal));
-        cc_status st = CC_OK;
-        if (hipEventRecord(fork, root) != hipSuccess) st = CC_ERR_HIP;
-        for (size_t i = 0; i < h.size() && st == CC_OK; i++) {
-            if (hipStreamWaitEvent(h[i]->stream, fork, 0) != hipSuccess) { st = CC_ERR_HIP; break; }
-            st = h[i]->heun(a, b, c, true, h[i]->dts_dev);
-            if (st == CC_OK && (hipEventRecord(join[i], h[i]->stream) != hipSuccess ||
-                                hipStreamWaitEvent(root, join[i], 0) != hipSuccess))
-                st = CC_ERR_HIP;
-        }
+        cc_status st = merged ? capture_merged_body(a)
+                              : fan([&](cc_mhd2d* x) { return x->heun(a, b, c, true, x->dts_dev); });
         hipGraph_t gr = nullptr;
         const hipError_t e = hipStreamEndCapture(root, &gr);
         if (st != CC_OK || e != hipSuccess) {
@@ -843,6 +927,10 @@ struct cc_mhd2d_batch {
     ~cc_mhd2d_batch() {
         for (auto& g : graph)
             if (g) hipGraphExecDestroy(g);
+        if (tiles4) hipFree(tiles4);
+        for (auto& ta : tab)
+            for (auto* t : ta)
+                if (t) hipFree(t);
         for (auto e : join)
             if (e) hipEventDestroy(e);
         if (fork) hipEventDestroy(fork);
@@ -867,6 +955,12 @@ cc_status cc_mhd2d_batch_create(cc_mhd2d* const* handles, int n, cc_mhd2d_batch*
         delete b;
         set_error(CC_ERR_HIP, "cc_mhd2d_batch_create: stream / event creation failed");
         return CC_ERR_HIP;
+    }
+    for (int i = 0; i < n; i++)  // the instances' set-up copies (schedules, scalars) are done before any batch launch
+        if (hipStreamSynchronize(handles[i]->stream) != hipSuccess) { delete b; set_error(CC_ERR_HIP, "sync failed"); return CC_ERR_HIP; }
+    if (std::getenv("CC_BATCH_SEPARATE_SOR") == nullptr) {
+        const cc_status st = b->setup_merged();
+        if (st != CC_OK) { delete b; return st; }
     }
     *out = b;
     return CC_OK;

@@ -85,9 +85,9 @@ struct TileBox {
 };
 
 template <int S, int WJ>
-__device__ __forceinline__ TileBox tile_box(const SorArgs& a, const int* __restrict__ tiles) {
+__device__ __forceinline__ TileBox tile_box(const SorArgs& a, const int* __restrict__ tile) {
     TileBox b;
-    b.TS = tiles[3 * blockIdx.x]; b.TJ = tiles[3 * blockIdx.x + 1]; b.TI = tiles[3 * blockIdx.x + 2];
+    b.TS = tile[0]; b.TJ = tile[1]; b.TI = tile[2];
     b.s0 = S * b.TS; b.s1 = min(b.s0 + S, a.nsw);
     b.ib0 = max(0, kTileW * b.TI - (b.s1 - 1) - 1); b.ib1 = min(a.Nx - 1, kTileW * b.TI - b.s0 + kTileW);
     b.jb0 = max(0, WJ * b.TJ - (b.s1 - 1) - 1); b.jb1 = min(a.Ny - 1, WJ * b.TJ - b.s0 + WJ);
@@ -163,14 +163,13 @@ __device__ __forceinline__ void tile_steps(T* __restrict__ x, const SorArgs& a, 
 // LDS, one tile per CU) or it is loaded from global memory one step ahead (half the LDS, two tiles
 // per CU).
 template <int S, int WJ, bool RHS_LDS>
-__global__ __launch_bounds__(1024) void k_sor_tile_fp64(SorArgs a, const int* __restrict__ tiles,
-                                                         double* __restrict__ phi, const double* __restrict__ rho,
-                                                         const double* __restrict__ rhobar) {
+__device__ __forceinline__ void sor_tile_fp64(const SorArgs& a, const int* __restrict__ tile, double* __restrict__ phi,
+                                              const double* __restrict__ rho, const double* __restrict__ rhobar) {
     using G = TileGeom<S, WJ>;
     constexpr int SPW = S / 16, P = G::P;
     __shared__ double sphi[G::R * P];
     __shared__ double srhs[RHS_LDS ? G::R * P : 1];
-    const TileBox b = tile_box<S, WJ>(a, tiles);
+    const TileBox b = tile_box<S, WJ>(a, tile);
     const double rb = rhobar[0];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     for (int c = tid; c < (b.ni + 2) * (b.nj + 2); c += 1024) {
@@ -231,14 +230,13 @@ __global__ __launch_bounds__(1024) void k_sor_tile_fp64(SorArgs a, const int* __
 // c = (stencil(phi_hi) - rhs)/denom - phi_hi evaluated once in FP64, one SOR update is
 // delta += omega * (c + kx (dE + dW) + ky (dN + dS) - delta), exactly the FP64 update rearranged.
 template <int S, int WJ>
-__global__ __launch_bounds__(1024) void k_sor_tile_mixed(SorArgs a, const int* __restrict__ tiles,
-                                                          double* __restrict__ phi, const double* __restrict__ rho,
-                                                          const double* __restrict__ rhobar) {
+__device__ __forceinline__ void sor_tile_mixed(const SorArgs& a, const int* __restrict__ tile, double* __restrict__ phi,
+                                               const double* __restrict__ rho, const double* __restrict__ rhobar) {
     using G = TileGeom<S, WJ>;
     constexpr int SPW = S / 16, P = G::P;
     __shared__ float sd[G::R * P];
     __shared__ float sc[G::R * P];
-    const TileBox b = tile_box<S, WJ>(a, tiles);
+    const TileBox b = tile_box<S, WJ>(a, tile);
     const double rb = rhobar[0];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     for (int c = tid; c < (b.ni + 2) * (b.nj + 2); c += 1024) {
@@ -278,6 +276,36 @@ __global__ __launch_bounds__(1024) void k_sor_tile_mixed(SorArgs a, const int* _
             phi[g] = phi[g] + (double)sd[(y + 1) * P + x + 1];
         }
     }
+}
+
+// One tile per block: the tiles of one level of one grid, (TS, TJ, TI) per tile.
+template <int S, int WJ, bool RHS_LDS>
+__global__ __launch_bounds__(1024) void k_sor_tile_fp64(SorArgs a, const int* __restrict__ tiles,
+                                                         double* __restrict__ phi, const double* __restrict__ rho,
+                                                         const double* __restrict__ rhobar) {
+    sor_tile_fp64<S, WJ, RHS_LDS>(a, tiles + 3 * blockIdx.x, phi, rho, rhobar);
+}
+template <int S, int WJ>
+__global__ __launch_bounds__(1024) void k_sor_tile_mixed(SorArgs a, const int* __restrict__ tiles,
+                                                          double* __restrict__ phi, const double* __restrict__ rho,
+                                                          const double* __restrict__ rhobar) {
+    sor_tile_mixed<S, WJ>(a, tiles + 3 * blockIdx.x, phi, rho, rhobar);
+}
+// One tile per block of the tiles of one level of many grids: (instance, TS, TJ, TI) per tile; the
+// instance's arguments and arrays come from the table.
+template <int S, int WJ, bool RHS_LDS>
+__global__ __launch_bounds__(1024) void k_sor_tile_fp64_batch(const SorBatchEntry* __restrict__ tab,
+                                                               const int* __restrict__ tiles) {
+    const int* t = tiles + 4 * blockIdx.x;
+    const SorBatchEntry e = tab[t[0]];
+    sor_tile_fp64<S, WJ, RHS_LDS>(e.a, t + 1, e.phi, e.rho, e.rhobar);
+}
+template <int S, int WJ>
+__global__ __launch_bounds__(1024) void k_sor_tile_mixed_batch(const SorBatchEntry* __restrict__ tab,
+                                                                const int* __restrict__ tiles) {
+    const int* t = tiles + 4 * blockIdx.x;
+    const SorBatchEntry e = tab[t[0]];
+    sor_tile_mixed<S, WJ>(e.a, t + 1, e.phi, e.rho, e.rhobar);
 }
 
 // ------------------------------------------------------------------ periodic / mixed-BC SOR
@@ -464,6 +492,28 @@ static cc_status launch_tile(const SorTile& t, bool mixed, int nt, hipStream_t s
     return CC_ERR_INVALID_ARG;
 }
 
+cc_status launch_tile_batch(const SorTile& t, bool mixed, int nt, hipStream_t s, const SorBatchEntry* tab,
+                            const int* tiles4) {
+#define CC_SOR_BLAUNCH64(S_, WJ_, R_)                                                                         \
+    if (!mixed && t.S == S_ && t.WJ == WJ_ && t.rhs_lds == R_) {                                             \
+        hipLaunchKernelGGL((k_sor_tile_fp64_batch<S_, WJ_, R_>), dim3(nt), dim3(1024), 0, s, tab, tiles4); \
+        CC_CHECK_LAUNCH();                                                                                   \
+        return CC_OK;                                                                                        \
+    }
+#define CC_SOR_BLAUNCH32(S_, WJ_)                                                                           \
+    if (mixed && t.S == S_ && t.WJ == WJ_) {                                                                \
+        hipLaunchKernelGGL((k_sor_tile_mixed_batch<S_, WJ_>), dim3(nt), dim3(1024), 0, s, tab, tiles4);   \
+        CC_CHECK_LAUNCH();                                                                                  \
+        return CC_OK;                                                                                       \
+    }
+    CC_SOR_TILES_FP64(CC_SOR_BLAUNCH64)
+    CC_SOR_TILES_MIXED(CC_SOR_BLAUNCH32)
+#undef CC_SOR_BLAUNCH64
+#undef CC_SOR_BLAUNCH32
+    set_error(CC_ERR_INVALID_ARG, "SOR tile shape not compiled in");
+    return CC_ERR_INVALID_ARG;
+}
+
 // ------------------------------------------------------------------ host side
 cc_status Poisson2D::init(int Nx_, int Ny_, bool px_, bool py_, double dx_, double dy_, double G_, int iters_,
                           double omega_, int precision_) {
@@ -503,7 +553,7 @@ cc_status Poisson2D::init(int Nx_, int Ny_, bool px_, bool py_, double dx_, doub
             return CC_ERR_INVALID_ARG;
         }
         tile_ = t;
-        std::vector<int> flat;
+        std::vector<int>& flat = sched_host_;
         sor_levels(t, Nx, Ny, iters, flat, level_off_);
         if (hipMalloc(&sched, flat.size() * sizeof(int)) != hipSuccess) {
             set_error(CC_ERR_OOM, "Poisson2D: schedule allocation failed");
@@ -634,6 +684,12 @@ void Poisson2D::destroy() {
 }
 
 cc_status Poisson2D::solve(const double* rho, double* gx, double* gy, hipStream_t s, Profiler* prof) {
+    CC_TRY(prologue(rho, s, prof));
+    CC_TRY(sweeps(rho, s, prof));
+    return epilogue(gx, gy, s, prof);
+}
+
+cc_status Poisson2D::prologue(const double* rho, hipStream_t s, Profiler* prof) {
     const size_t N = (size_t)Nx * Ny;
     {
         ProfScope ps(prof, s, "mean");
@@ -641,6 +697,10 @@ cc_status Poisson2D::solve(const double* rho, double* gx, double* gy, hipStream_
     }
     hipLaunchKernelGGL(k_fill, dim3(std::min<size_t>((N + 255) / 256, 4096)), dim3(256), 0, s, phi, N, 0.0);
     CC_CHECK_LAUNCH();
+    return CC_OK;
+}
+
+cc_status Poisson2D::sweeps(const double* rho, hipStream_t s, Profiler* prof) {
     if (tiled_) {
         const int nl = levels();
         for (int l = 0; l < nl; ++l) {
@@ -668,6 +728,10 @@ cc_status Poisson2D::solve(const double* rho, double* gx, double* gy, hipStream_
             hipLaunchKernelGGL(k_sor_wavefront_1wg, dim3(1), dim3(1024), 0, s, args, phi, rho, rhobar);
         CC_CHECK_LAUNCH();
     }
+    return CC_OK;
+}
+
+cc_status Poisson2D::epilogue(double* gx, double* gy, hipStream_t s, Profiler* prof) {
     {
         ProfScope ps(prof, s, "grad");
         hipLaunchKernelGGL(k_grad2d, dim3((Nx + 255) / 256, Ny), dim3(256), 0, s, Nx, Ny, (int)px, (int)py, dx, dy,

@@ -39,6 +39,14 @@ struct SorArgs {
     float kxf, kyf, omegaf;  // FP32 correction-sweep coefficients: idx2/denom, idy2/denom, omega
 };
 
+// One grid of a batched tiled SOR launch: its arguments and arrays.
+struct SorBatchEntry {
+    SorArgs a;
+    double* phi;
+    const double* rho;
+    const double* rhobar;
+};
+
 class Poisson2D {
    public:
     cc_status init(int Nx, int Ny, bool px, bool py, double dx, double dy, double G, int iters, double omega,
@@ -46,8 +54,13 @@ class Poisson2D {
     // The fastest compiled tile shape for this grid (timed once per grid in the process, see cc_sor_autotune).
     cc_status autotuned_tile(SorTile& out);
     void destroy();
-    // g = -grad phi(rho); rho, gx, gy are device arrays of Nx*Ny.
+    // g = -grad phi(rho); rho, gx, gy are device arrays of Nx*Ny.  solve = prologue + sweeps + epilogue;
+    // a batch of tiled grids runs the three separately, with the sweeps of all grids merged per tile level.
     cc_status solve(const double* rho, double* gx, double* gy, hipStream_t s, Profiler* prof);
+    cc_status prologue(const double* rho, hipStream_t s, Profiler* prof);  // mean of rho, phi = 0
+    cc_status sweeps(const double* rho, hipStream_t s, Profiler* prof);
+    cc_status epilogue(double* gx, double* gy, hipStream_t s, Profiler* prof);  // g = -central grad phi
+    const std::vector<int>& schedule_host() const { return sched_host_; }
     // phi with its mean removed (after solve), device array.
     cc_status phi_mean_removed(double* out_dev, hipStream_t s);
     bool tiled() const { return tiled_; }
@@ -65,8 +78,14 @@ class Poisson2D {
     int* sched = nullptr;       // (TS,TJ,TI) per tile, grouped by level
     SorTile tile_{};            // tile shape of the tiled schedule
     std::vector<int> level_off_;
+    std::vector<int> sched_host_;  // the schedule's (TS, TJ, TI) triples, grouped by level
     SorArgs args{};
 };
+
+// Launch one level of a batched tiled SOR: nt tiles given as (instance, TS, TJ, TI), every instance
+// with tile shape t.
+cc_status launch_tile_batch(const SorTile& t, bool mixed, int nt, hipStream_t s, const SorBatchEntry* tab,
+                            const int* tiles4);
 
 // The tile levels of the tiled schedule for a tile shape (flat (TS, TJ, TI) triples, level offsets).
 void sor_levels(const SorTile& t, int Nx, int Ny, int iters, std::vector<int>& flat, std::vector<int>& level_off);

@@ -223,13 +223,19 @@ def test_mixed_vs_fp64_l1_density():
     assert e <= 1e-4, e
 
 
+@pytest.mark.parametrize("separate_sor", [False, True])
 @pytest.mark.parametrize("prec", [CC_FP64, CC_MIXED])
-def test_batch_equals_instances_run_alone(prec):
-    """cc_mhd2d_batch_run: independent collapse instances of uneven sizes, stepped as parallel
-    branches of one graph with per-instance dt, give bit for bit the times, dts and states of the
-    same instances run one by one -- over two calls (the second continuing the first), with one
-    instance stopping at its own t_end inside the run."""
+def test_batch_equals_instances_run_alone(prec, separate_sor, monkeypatch):
+    """cc_mhd2d_batch_run: independent collapse instances of uneven sizes, stepped together in one
+    graph with per-instance dt -- their SOR tile levels merged into one launch per level (default) or
+    each instance's step a separate branch (CC_BATCH_SEPARATE_SOR) -- give bit for bit the times, dts
+    and states of the same instances run one by one, over two calls (the second continuing the
+    first), with one instance stopping at its own t_end inside the run."""
     from core_collapse_amd.mhd2d import Mhd2D, Mhd2DBatch
+    if separate_sor:
+        monkeypatch.setenv("CC_BATCH_SEPARATE_SOR", "1")
+    else:
+        monkeypatch.delenv("CC_BATCH_SEPARATE_SOR", raising=False)
     sample = ic.collapse2d_batch_sample(5, seed=9, nmin=24, nmax=80)
 
     def make():
