@@ -232,7 +232,8 @@ def bench3d(N, warmup, steps, with_cpu):
 def bench_batch2d(n_inst, steps, warmup, nmin, nmax, seq_steps):
     """The batched collapse: n_inst independent instances from the batch sampler (grids log-uniform in
     [nmin, nmax]^2, contrast, eta, B0 and IC seed per instance), mixed precision, all advanced together
-    by one graph launch per step (parallel branches, per-instance dt).  Also times each instance run
+    by one graph launch per step (per-instance dt; the SOR tile levels of all instances merged into one
+    launch per level, the rest of each instance's step a parallel branch).  Also times each instance run
     alone for seq_steps steps, to set the batch against running the instances one after another."""
     from core_collapse_amd import CC_MIXED
     from core_collapse_amd import ic
@@ -249,6 +250,7 @@ def bench_batch2d(n_inst, steps, warmup, nmin, nmax, seq_steps):
     t_setup = time.perf_counter() - t_setup
     tiles = {str(e["N"]): M.sor_tile_choice(e["N"], e["N"], 600, CC_MIXED) for e in sample}
     b = M.Mhd2DBatch(sims)
+    binfo = b.info()
     t = b.run(warmup) if warmup > 0 else None
     t0 = time.perf_counter()
     t = b.run(steps, t)
@@ -277,7 +279,8 @@ def bench_batch2d(n_inst, steps, warmup, nmin, nmax, seq_steps):
                                 "ms_per_step": 1e3 * seq / steps if seq > 0 else None},
         "batch_over_sequential": seq / elapsed if seq > 0 else None,
         "sor_alg_GBps": cells * steps * 2 * 600 * 12 / elapsed / 1e9,
-        "setup_s": round(t_setup, 2), "sor_tiles": tiles,
+        "setup_s": round(t_setup, 2), "sor_merged": binfo["merged"], "sor_merged_tile": binfo["tile"],
+        "sor_tiles_alone": tiles,
         "max_divB": divb, "t_min": float(np.min(t)), "t_max": float(np.max(t)),
     }
 

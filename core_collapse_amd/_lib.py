@@ -219,6 +219,7 @@ _PROTOS = {
     "cc_sor_tile_choice": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "cc_mhd2d_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
     "cc_mhd2d_batch_destroy": (C.c_int, [C.c_void_p]),
+    "cc_mhd2d_batch_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "cc_mhd2d_batch_run": (C.c_int, [C.c_void_p, C.c_int, DP, DP]),
     "cc_collapse2d_batch_sample": (C.c_int, [C.c_uint, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int),
                                              C.POINTER(Collapse2dIc), DP]),

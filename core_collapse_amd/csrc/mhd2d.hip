@@ -846,29 +846,7 @@ struct cc_mhd2d_batch {
                 P.tile_.S != h0->pois.tile_.S || P.tile_.WJ != h0->pois.tile_.WJ || P.tile_.rhs_lds != h0->pois.tile_.rhs_lds)
                 return CC_OK;  // not mergeable: the instances keep their own SOR launches
         }
-        tile = h0->pois.tile_;
         mixed = h0->p.precision == CC_MIXED;
-        std::vector<int> flat;
-        level_off.assign(1, 0);
-        for (int L = 0;; ++L) {
-            bool any = false;
-            for (size_t i = 0; i < h.size(); i++) {
-                const Poisson2D& P = h[i]->pois;
-                if (L >= P.levels()) continue;
-                any = true;
-                const std::vector<int>& sh = P.schedule_host();
-                for (int t = P.level_off_[L]; t < P.level_off_[L + 1]; ++t) {
-                    flat.push_back((int)i);
-                    flat.push_back(sh[3 * t]);
-                    flat.push_back(sh[3 * t + 1]);
-                    flat.push_back(sh[3 * t + 2]);
-                }
-            }
-            if (!any) break;
-            level_off.push_back((int)flat.size() / 4);
-        }
-        if (hipMalloc(&tiles4, flat.size() * sizeof(int)) != hipSuccess) { set_error(CC_ERR_OOM, "batch schedule allocation failed"); return CC_ERR_OOM; }
-        CC_HIP(hipMemcpy(tiles4, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice));
         for (int a = 0; a < 3; ++a)
             for (int q = 0; q < 2; ++q) {
                 std::vector<SorBatchEntry> e(h.size());
@@ -879,8 +857,75 @@ struct cc_mhd2d_batch {
                 if (hipMalloc(&tab[a][q], e.size() * sizeof(SorBatchEntry)) != hipSuccess) { set_error(CC_ERR_OOM, "batch table allocation failed"); return CC_ERR_OOM; }
                 CC_HIP(hipMemcpy(tab[a][q], e.data(), e.size() * sizeof(SorBatchEntry), hipMemcpyHostToDevice));
             }
+        // the tile shape: the instances' own, or with autotuning the compiled shape whose merged solve is fastest
+        std::vector<SorTile> cands{h0->pois.tile_};
+        if (sor_autotune_enabled()) {
+#define CC_BCAND64(S_, WJ_, R_) if (!mixed) cands.push_back(SorTile{S_, WJ_, R_});
+#define CC_BCAND32(S_, WJ_) if (mixed) cands.push_back(SorTile{S_, WJ_, true});
+            CC_SOR_TILES_FP64(CC_BCAND64)
+            CC_SOR_TILES_MIXED(CC_BCAND32)
+#undef CC_BCAND64
+#undef CC_BCAND32
+        }
+        float best = 3.0e38f;
+        for (size_t ci = 0; ci < cands.size(); ++ci) {
+            std::vector<int> flat, off;
+            build_merged(cands[ci], flat, off);
+            int* t4 = nullptr;
+            if (hipMalloc(&t4, flat.size() * sizeof(int)) != hipSuccess) { set_error(CC_ERR_OOM, "batch schedule allocation failed"); return CC_ERR_OOM; }
+            if (hipMemcpy(t4, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) { hipFree(t4); set_error(CC_ERR_HIP, "copy failed"); return CC_ERR_HIP; }
+            float ms = 0.f;
+            if (cands.size() > 1) {  // time one merged solve (the first run pays first-launch costs); phi is scratch
+                hipEvent_t e0, e1;
+                hipEventCreate(&e0);
+                hipEventCreate(&e1);
+                cc_status st = CC_OK;
+                for (int rep = 0; rep < 2 && st == CC_OK; ++rep) {
+                    hipEventRecord(e0, root);
+                    for (size_t l = 0; l + 1 < off.size() && st == CC_OK; ++l)
+                        st = launch_tile_batch(cands[ci], mixed, off[l + 1] - off[l], root, tab[0][0], t4 + 4 * off[l]);
+                    hipEventRecord(e1, root);
+                    hipEventSynchronize(e1);
+                    hipEventElapsedTime(&ms, e0, e1);
+                }
+                hipEventDestroy(e0);
+                hipEventDestroy(e1);
+                if (st != CC_OK) { hipFree(t4); return st; }
+            }
+            if (ci == 0 || ms < best) {
+                best = ms;
+                if (tiles4) hipFree(tiles4);
+                tiles4 = t4;
+                level_off = off;
+                tile = cands[ci];
+            } else {
+                hipFree(t4);
+            }
+        }
         merged = true;
         return CC_OK;
+    }
+    // The merged schedule for tile shape t: level L holds level L of every instance, (instance, TS, TJ, TI).
+    void build_merged(const SorTile& t, std::vector<int>& flat, std::vector<int>& off) {
+        std::vector<std::vector<int>> fl(h.size()), lo(h.size());
+        for (size_t i = 0; i < h.size(); i++) sor_levels(t, h[i]->pois.Nx, h[i]->pois.Ny, h[i]->pois.iters, fl[i], lo[i]);
+        flat.clear();
+        off.assign(1, 0);
+        for (int L = 0;; ++L) {
+            bool any = false;
+            for (size_t i = 0; i < h.size(); i++) {
+                if (L + 1 >= (int)lo[i].size()) continue;
+                any = true;
+                for (int k = lo[i][L]; k < lo[i][L + 1]; ++k) {
+                    flat.push_back((int)i);
+                    flat.push_back(fl[i][3 * k]);
+                    flat.push_back(fl[i][3 * k + 1]);
+                    flat.push_back(fl[i][3 * k + 2]);
+                }
+            }
+            if (!any) break;
+            off.push_back((int)flat.size() / 4);
+        }
     }
     // One Heun step of every instance (heun(a, b, c, true) of each) with the SOR sweeps merged:
     // per instance mean + phi = 0 | merged sweeps | g, dt, stage 1, mean + phi = 0 | merged sweeps | g, stage 2.
@@ -963,6 +1008,14 @@ cc_status cc_mhd2d_batch_create(cc_mhd2d* const* handles, int n, cc_mhd2d_batch*
         if (st != CC_OK) { delete b; return st; }
     }
     *out = b;
+    return CC_OK;
+}
+
+cc_status cc_mhd2d_batch_info(cc_mhd2d_batch* b, int* merged, int* S, int* WJ) {
+    if (!b || !merged || !S || !WJ) { set_error(CC_ERR_INVALID_ARG, "cc_mhd2d_batch_info: null argument"); return CC_ERR_INVALID_ARG; }
+    *merged = b->merged ? 1 : 0;
+    *S = b->merged ? b->tile.S : 0;
+    *WJ = b->merged ? b->tile.WJ : 0;
     return CC_OK;
 }
 

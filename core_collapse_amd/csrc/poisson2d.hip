@@ -453,10 +453,6 @@ __global__ void k_sub_mean(const double* __restrict__ x, size_t n, const double*
 }
 
 // ------------------------------------------------------------------ tile dispatch
-// Compiled shapes (S, WJ[, source term in LDS]); FP64 with the source term in LDS needs 16 B per
-// box cell, which fits 160 KB of LDS only at WJ = 64.
-#define CC_SOR_TILES_FP64(X) X(16, 64, true) X(32, 64, true) X(32, 64, false) X(16, 128, false) X(32, 128, false)
-#define CC_SOR_TILES_MIXED(X) X(16, 64) X(32, 64) X(16, 128) X(32, 128)
 
 static bool sor_tile_supported(const SorTile& t, bool mixed) {
 #define CC_SOR_MATCH64(S_, WJ_, R_) if (!mixed && t.S == S_ && t.WJ == WJ_ && t.rhs_lds == R_) return true;

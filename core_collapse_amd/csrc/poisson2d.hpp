@@ -39,6 +39,11 @@ struct SorArgs {
     float kxf, kyf, omegaf;  // FP32 correction-sweep coefficients: idx2/denom, idy2/denom, omega
 };
 
+// The compiled tile shapes (S, WJ[, source term in LDS]); FP64 with the source term in LDS needs
+// 16 B per box cell, which fits 160 KB of LDS only at WJ = 64.
+#define CC_SOR_TILES_FP64(X) X(16, 64, true) X(32, 64, true) X(32, 64, false) X(16, 128, false) X(32, 128, false)
+#define CC_SOR_TILES_MIXED(X) X(16, 64) X(32, 64) X(16, 128) X(32, 128)
+
 // One grid of a batched tiled SOR launch: its arguments and arrays.
 struct SorBatchEntry {
     SorArgs a;

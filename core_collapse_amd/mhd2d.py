@@ -207,6 +207,12 @@ class Mhd2DBatch:
         check(_lib.lib().cc_mhd2d_batch_run(self._b, int(nsteps), _p(tt), _p(te)))
         return tt
 
+    def info(self):
+        """{"merged": SOR tile levels merged across instances, "tile": (S, WJ) of the merged launches}."""
+        m, S, WJ = C.c_int(), C.c_int(), C.c_int()
+        check(_lib.lib().cc_mhd2d_batch_info(self._b, C.byref(m), C.byref(S), C.byref(WJ)))
+        return {"merged": bool(m.value), "tile": (S.value, WJ.value)}
+
     def close(self):
         if self._b:
             _lib.lib().cc_mhd2d_batch_destroy(self._b)

@@ -136,6 +136,11 @@ cc_status cc_mhd2d_compare(cc_mhd2d* a, cc_mhd2d* b, double* out);
 typedef struct cc_mhd2d_batch cc_mhd2d_batch;
 cc_status cc_mhd2d_batch_create(cc_mhd2d* const* handles, int n, cc_mhd2d_batch** out);
 cc_status cc_mhd2d_batch_destroy(cc_mhd2d_batch* b);
+/* merged = 1 when the instances' SOR tile levels run as merged launches (every instance with
+ * gravity, clamped boundaries and one precision), with the tile shape (S sweeps, WJ rows) the batch
+ * uses -- the instances' own, or with cc_sor_autotune the fastest merged shape; merged = 0 when each
+ * instance's step is a separate graph branch (or CC_BATCH_SEPARATE_SOR is set). */
+cc_status cc_mhd2d_batch_info(cc_mhd2d_batch* b, int* merged, int* S, int* WJ);
 /* nsteps steps of every instance (the loop of cc_mhd2d_run: gravity, dt, clamp to t_end[i], Heun).
  * t_inout[i] holds instance i's t on entry and return.  Syncs. */
 cc_status cc_mhd2d_batch_run(cc_mhd2d_batch* b, int nsteps, double* t_inout, const double* t_end);

@@ -257,7 +257,8 @@ def test_batch_equals_instances_run_alone(prec, separate_sor, monkeypatch):
     t_alone = np.array([h.run(7, t_alone[i], t_end[i]) for i, h in enumerate(alone)])
     batch_sims = make()
     b = Mhd2DBatch(batch_sims)
-    print("batch sizes", [e["N"] for e in sample], flush=True)
+    print("batch sizes", [e["N"] for e in sample], b.info(), flush=True)
+    assert b.info()["merged"] == (not separate_sor)
     t = b.run(12, None, t_end)
     t = b.run(7, t, t_end)
     b.close()
