@@ -11,8 +11,9 @@ A zone-update is one cell advanced one full Heun step including its two Poisson 
 `value` is the mixed-precision zone-updates/s with all state resident in HBM; the FP64 figure,
 the ratio, the mixed-vs-FP64 relative L1 density error and max|div B| ride along.
 
-roofline: dominant kernel = the tiled SOR kernel (k_sor_tile_*), timed with HIP events on the
-handle's stream around every launch inside the timed region.  Algorithmic bytes per launch =
+roofline: dominant kernel = the tiled SOR kernel (k_sor_flow_*: one dataflow launch per 600-sweep
+solve; k_sor_tile_*, one launch per tile level, when CC_SOR_LEVELS is set), timed with HIP events
+on the handle's stream around every launch inside the timed region.  Algorithmic bytes per launch =
 (cell-sweeps in the launch) x 24 B (FP64: read phi, read rhs, write phi) or x 12 B (mixed: FP32
 phi and rhs) -- the per-sweep figure of the survey's B_alg.  The kernel keeps its tile in LDS for
 16 sweeps, so its algorithmic rate can exceed the HBM peak; `traffic` (measured HBM bytes per
@@ -492,15 +493,16 @@ def main():
         ach = bytes_per_launch / avg_s / 1e9
         return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_sor_tile", "launches": n, "avg_launch_us": round(avg_s * 1e6, 2),
+                "kernel": sor_kernel, "launches": n, "avg_launch_us": round(avg_s * 1e6, 2),
                 "alg_bytes_per_launch": bytes_per_launch}
 
+    sor_kernel = "k_sor_tile" if os.environ.get("CC_SOR_LEVELS") else "k_sor_flow"
     roof_mx = roof(res[CC_MIXED], 12.0)
     roof_64 = roof(res[CC_FP64], 24.0)
     if roof_mx:
-        roof_mx["traffic"] = load_traffic("cc::k_sor_tile_mixed<%d, %d>" % tuple(res[CC_MIXED]["tile"]))
+        roof_mx["traffic"] = load_traffic("cc::%s_mixed<%d, %d>" % ((sor_kernel,) + tuple(res[CC_MIXED]["tile"])))
     if roof_64:
-        roof_64["traffic"] = load_traffic("cc::k_sor_tile_fp64<%d, %d," % tuple(res[CC_FP64]["tile"]))
+        roof_64["traffic"] = load_traffic("cc::%s_fp64<%d, %d," % ((sor_kernel,) + tuple(res[CC_FP64]["tile"])))
     kernels = {}
     for prec, tag in ((CC_FP64, "fp64"), (CC_MIXED, "mixed")):
         kernels[tag] = {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in res[prec]["prof"].items()

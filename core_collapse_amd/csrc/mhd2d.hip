@@ -720,6 +720,7 @@ cc_status cc_mhd2d_run(cc_mhd2d* h, int nsteps, double* t_inout, double t_end, d
     if (dts) CC_HIP(hipMemcpyAsync(dts, h->dts_dev, (size_t)nsteps * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     CC_HIP(hipStreamSynchronize(h->stream));
     *t_inout = s.t;
+    CC_TRY(h->pois.flow_error(h->stream));
     if (!std::isfinite(s.dt) || s.dt < 0.0) {
         set_error(CC_ERR_BAD_DT, "non-finite or negative dt on device");
         return CC_ERR_BAD_DT;
@@ -1058,6 +1059,7 @@ cc_status cc_mhd2d_batch_run(cc_mhd2d_batch* b, int nsteps, double* t_inout, con
         Scalars sc;
         CC_HIP(hipMemcpy(&sc, h->sc, sizeof(Scalars), hipMemcpyDeviceToHost));
         t_inout[i] = sc.t;
+        if (h->pois.flow_error(h->stream) != CC_OK) st = CC_ERR_HIP;
         if (!std::isfinite(sc.dt) || sc.dt < 0.0) {
             set_error(CC_ERR_BAD_DT, "non-finite or negative dt on device in batch instance " + std::to_string(i));
             st = CC_ERR_BAD_DT;

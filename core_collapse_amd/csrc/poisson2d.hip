@@ -159,10 +159,24 @@ __device__ __forceinline__ void tile_steps(T* __restrict__ x, const SorArgs& a, 
     }
 }
 
+// phi accesses of a tile: plain, or (COH, the dataflow schedule) system-coherent -- loads that bypass
+// the caches and write-through stores -- so tiles on different XCDs see each other's results without
+// cache maintenance.
+template <bool COH>
+__device__ __forceinline__ double phi_ld(const double* p) {
+    if constexpr (COH) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    else return *p;
+}
+template <bool COH>
+__device__ __forceinline__ void phi_st(double* p, double v) {
+    if constexpr (COH) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    else *p = v;
+}
+
 // FP64: phi is staged in LDS; the source term 4 pi G (rho - mean) either too (RHS_LDS: twice the
 // LDS, one tile per CU) or it is loaded from global memory one step ahead (half the LDS, two tiles
 // per CU).
-template <int S, int WJ, bool RHS_LDS>
+template <int S, int WJ, bool RHS_LDS, bool COH = false>
 __device__ __forceinline__ void sor_tile_fp64(const SorArgs& a, const int* __restrict__ tile, double* __restrict__ phi,
                                               const double* __restrict__ rho, const double* __restrict__ rhobar) {
     using G = TileGeom<S, WJ>;
@@ -177,7 +191,7 @@ __device__ __forceinline__ void sor_tile_fp64(const SorArgs& a, const int* __res
         double v = 0.0, r = 0.0;  // margin cells are never read
         if (x > 0 && x <= b.ni && y > 0 && y <= b.nj) {
             const size_t g = (size_t)(b.jb0 - 1 + y) * a.Nx + (b.ib0 - 1 + x);
-            v = phi[g];
+            v = phi_ld<COH>(&phi[g]);
             if (RHS_LDS) r = a.c4piG * (rho[g] - rb);
         }
         sphi[y * P + x] = v;
@@ -222,14 +236,14 @@ __device__ __forceinline__ void sor_tile_fp64(const SorArgs& a, const int* __res
     for (int c = tid; c < b.ni * b.nj; c += 1024) {
         const int x = c % b.ni, y = c / b.ni;
         const int i = b.ib0 + x, j = b.jb0 + y;
-        if (tile_updates<WJ>(i, j, b.s0, b.s1, b.TI, b.TJ)) phi[(size_t)j * a.Nx + i] = sphi[(y + 1) * P + x + 1];
+        if (tile_updates<WJ>(i, j, b.s0, b.s1, b.TI, b.TJ)) phi_st<COH>(&phi[(size_t)j * a.Nx + i], sphi[(y + 1) * P + x + 1]);
     }
 }
 
 // Mixed precision: phi = phi_hi (FP64, fixed for the tile) + delta (FP32 in LDS).  With
 // c = (stencil(phi_hi) - rhs)/denom - phi_hi evaluated once in FP64, one SOR update is
 // delta += omega * (c + kx (dE + dW) + ky (dN + dS) - delta), exactly the FP64 update rearranged.
-template <int S, int WJ>
+template <int S, int WJ, bool COH = false>
 __device__ __forceinline__ void sor_tile_mixed(const SorArgs& a, const int* __restrict__ tile, double* __restrict__ phi,
                                                const double* __restrict__ rho, const double* __restrict__ rhobar) {
     using G = TileGeom<S, WJ>;
@@ -245,11 +259,11 @@ __device__ __forceinline__ void sor_tile_mixed(const SorArgs& a, const int* __re
         float cv = 0.f;
         if (x > 0 && x <= b.ni && y > 0 && y <= b.nj && tile_updates<WJ>(i, j, b.s0, b.s1, b.TI, b.TJ)) {
             const size_t g = (size_t)j * a.Nx + i;
-            const double pc = phi[g];
-            const double pE = phi[(size_t)j * a.Nx + min(i + 1, a.Nx - 1)];
-            const double pW = phi[(size_t)j * a.Nx + max(i - 1, 0)];
-            const double pN = phi[(size_t)min(j + 1, a.Ny - 1) * a.Nx + i];
-            const double pS = phi[(size_t)max(j - 1, 0) * a.Nx + i];
+            const double pc = phi_ld<COH>(&phi[g]);
+            const double pE = phi_ld<COH>(&phi[(size_t)j * a.Nx + min(i + 1, a.Nx - 1)]);
+            const double pW = phi_ld<COH>(&phi[(size_t)j * a.Nx + max(i - 1, 0)]);
+            const double pN = phi_ld<COH>(&phi[(size_t)min(j + 1, a.Ny - 1) * a.Nx + i]);
+            const double pS = phi_ld<COH>(&phi[(size_t)max(j - 1, 0) * a.Nx + i]);
             const double rhs = a.c4piG * (rho[g] - rb);
             cv = (float)(((pE + pW) * a.idx2 + (pN + pS) * a.idy2 - rhs) * a.inv_denom - pc);
         }
@@ -273,7 +287,7 @@ __device__ __forceinline__ void sor_tile_mixed(const SorArgs& a, const int* __re
         const int i = b.ib0 + x, j = b.jb0 + y;
         if (tile_updates<WJ>(i, j, b.s0, b.s1, b.TI, b.TJ)) {
             const size_t g = (size_t)j * a.Nx + i;
-            phi[g] = phi[g] + (double)sd[(y + 1) * P + x + 1];
+            phi_st<COH>(&phi[g], phi_ld<COH>(&phi[g]) + (double)sd[(y + 1) * P + x + 1]);
         }
     }
 }
@@ -306,6 +320,88 @@ __global__ __launch_bounds__(1024) void k_sor_tile_mixed_batch(const SorBatchEnt
     const int* t = tiles + 4 * blockIdx.x;
     const SorBatchEntry e = tab[t[0]];
     sor_tile_mixed<S, WJ>(e.a, t + 1, e.phi, e.rho, e.rhobar);
+}
+
+// ------------------------------------------------------------------ dataflow tiled SOR
+// The tiles of a whole solve in one persistent launch: blocks take tiles in level order from an
+// atomic counter and start one as soon as the tiles it depends on have finished (per tile, the
+// nearest non-empty tiles below it along TS, TJ and TI, through empty positions), instead of
+// waiting for the whole previous level.  Any grid size is safe: a block holds a tile only while it
+// runs, and the lowest unfinished tile's dependencies are always finished.  A wait is bounded:
+// past kFlowSpin polls the block records an error and goes on (results are then wrong, the host
+// reports it), so a fault in the schedule cannot hang the device.
+constexpr int kFlowSpin = 1 << 16;
+
+struct FlowSched {
+    const int* tiles;     // (TS, TJ, TI) per tile, level order
+    const int* dep_off;   // CSR: dependencies of tile t are dep_idx[dep_off[t] .. dep_off[t + 1])
+    const int* dep_idx;
+    int ntiles;
+    int* state;           // [0] tile counter, [1] error flag, [2 ..] done flag per tile
+};
+
+__device__ __forceinline__ int flow_next_tile(const FlowSched& f) {
+    __shared__ int s_tile;
+    __syncthreads();  // the previous tile is finished by every thread of the block
+    if (threadIdx.x == 0) s_tile = atomicAdd(&f.state[0], 1);
+    __syncthreads();
+    const int t = s_tile;
+    if (t < f.ntiles) {
+        const int d0 = f.dep_off[t], d1 = f.dep_off[t + 1];
+        for (int d = d0 + (int)threadIdx.x; d < d1; d += (int)blockDim.x) {
+            int* flag = &f.state[2 + f.dep_idx[d]];
+            int spins = 0;
+            while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) == 0) {
+                if (++spins > kFlowSpin) {
+                    __hip_atomic_store(&f.state[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __syncthreads();  // the tile's phi loads (system-coherent) follow the observed flags
+    }
+    return t;
+}
+
+__device__ __forceinline__ void flow_finish_tile(const FlowSched& f, int t) {
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's write-through phi stores have completed
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(&f.state[2 + t], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+template <int S, int WJ, bool RHS_LDS>
+__global__ __launch_bounds__(1024) void k_sor_flow_fp64(SorArgs a, FlowSched f, double* __restrict__ phi,
+                                                         const double* __restrict__ rho,
+                                                         const double* __restrict__ rhobar) {
+    for (;;) {
+        const int t = flow_next_tile(f);
+        if (t >= f.ntiles) break;
+        sor_tile_fp64<S, WJ, RHS_LDS, true>(a, f.tiles + 3 * t, phi, rho, rhobar);
+        flow_finish_tile(f, t);
+    }
+}
+template <int S, int WJ>
+__global__ __launch_bounds__(1024) void k_sor_flow_mixed(SorArgs a, FlowSched f, double* __restrict__ phi,
+                                                          const double* __restrict__ rho,
+                                                          const double* __restrict__ rhobar) {
+    for (;;) {
+        const int t = flow_next_tile(f);
+        if (t >= f.ntiles) break;
+        sor_tile_mixed<S, WJ, true>(a, f.tiles + 3 * t, phi, rho, rhobar);
+        flow_finish_tile(f, t);
+    }
+}
+
+__global__ void k_fill_int(int* __restrict__ x, int n, int v) {
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) x[q] = v;
+}
+
+// Clears a dataflow state of ntiles tiles for the next solve: the tile counter and the done flags;
+// the error flag (state[1]) stays set until the host reads it.
+__global__ void k_flow_reset(int* __restrict__ state, int ntiles) {
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ntiles + 2; q += gridDim.x * blockDim.x)
+        if (q != 1) state[q] = 0;
 }
 
 // ------------------------------------------------------------------ periodic / mixed-BC SOR
@@ -488,6 +584,79 @@ static cc_status launch_tile(const SorTile& t, bool mixed, int nt, hipStream_t s
     return CC_ERR_INVALID_ARG;
 }
 
+static cc_status launch_flow(const SorTile& t, bool mixed, int nblk, hipStream_t s, const SorArgs& args,
+                             const FlowSched& f, double* phi, const double* rho, const double* rhobar) {
+#define CC_SOR_FLAUNCH64(S_, WJ_, R_)                                                                         \
+    if (!mixed && t.S == S_ && t.WJ == WJ_ && t.rhs_lds == R_) {                                             \
+        hipLaunchKernelGGL((k_sor_flow_fp64<S_, WJ_, R_>), dim3(nblk), dim3(1024), 0, s, args, f, phi, rho, \
+                           rhobar);                                                                          \
+        CC_CHECK_LAUNCH();                                                                                   \
+        return CC_OK;                                                                                        \
+    }
+#define CC_SOR_FLAUNCH32(S_, WJ_)                                                                              \
+    if (mixed && t.S == S_ && t.WJ == WJ_) {                                                                   \
+        hipLaunchKernelGGL((k_sor_flow_mixed<S_, WJ_>), dim3(nblk), dim3(1024), 0, s, args, f, phi, rho, rhobar); \
+        CC_CHECK_LAUNCH();                                                                                     \
+        return CC_OK;                                                                                          \
+    }
+    CC_SOR_TILES_FP64(CC_SOR_FLAUNCH64)
+    CC_SOR_TILES_MIXED(CC_SOR_FLAUNCH32)
+#undef CC_SOR_FLAUNCH64
+#undef CC_SOR_FLAUNCH32
+    set_error(CC_ERR_INVALID_ARG, "SOR tile shape not compiled in");
+    return CC_ERR_INVALID_ARG;
+}
+
+// The tiled sweeps run as one dataflow launch unless CC_SOR_LEVELS asks for one launch per level.
+bool sor_flow_enabled(int ntiles) { return std::getenv("CC_SOR_LEVELS") == nullptr && ntiles > 0 && ntiles < 10000000; }
+
+// Dependencies of the dataflow schedule: for each tile of `flat` (level order), the nearest non-empty
+// tiles below it along TS, TJ and TI, reached through empty tile positions (CSR).
+void sor_flow_deps(const std::vector<int>& flat, std::vector<int>& off, std::vector<int>& idx) {
+    const int n = (int)flat.size() / 3;
+    int mTS = 0, mTJ = 0, mTI = 0;
+    for (int t = 0; t < n; ++t) {
+        mTS = std::max(mTS, flat[3 * t] + 1); mTJ = std::max(mTJ, flat[3 * t + 1] + 1); mTI = std::max(mTI, flat[3 * t + 2] + 1);
+    }
+    auto key = [&](int TS, int TJ, int TI) { return ((size_t)TS * mTJ + TJ) * mTI + TI; };
+    std::vector<int> at((size_t)mTS * mTJ * mTI, -1);  // tile index at a position, -1 if empty
+    for (int t = 0; t < n; ++t) at[key(flat[3 * t], flat[3 * t + 1], flat[3 * t + 2])] = t;
+    // deps of a position, memoised; positions are visited in increasing (TS, TJ, TI) so the lower ones are done
+    std::vector<std::vector<int>> memo((size_t)mTS * mTJ * mTI);
+    for (int TS = 0; TS < mTS; ++TS)
+        for (int TJ = 0; TJ < mTJ; ++TJ)
+            for (int TI = 0; TI < mTI; ++TI) {
+                std::vector<int> d;
+                const int nb[3][3] = {{TS - 1, TJ, TI}, {TS, TJ - 1, TI}, {TS, TJ, TI - 1}};
+                for (const auto& q : nb) {
+                    if (q[0] < 0 || q[1] < 0 || q[2] < 0) continue;
+                    const size_t kq = key(q[0], q[1], q[2]);
+                    if (at[kq] >= 0) d.push_back(at[kq]);
+                    else d.insert(d.end(), memo[kq].begin(), memo[kq].end());
+                }
+                std::sort(d.begin(), d.end());
+                d.erase(std::unique(d.begin(), d.end()), d.end());
+                // keep the maximal ones: a tile below another dependency (coordinate-wise) is waited for
+                // through it (every tile waits, transitively, for all non-empty tiles below it)
+                std::vector<int> m;
+                for (int x : d) {
+                    bool dominated = false;
+                    for (int y : d)
+                        if (y != x && flat[3 * x] <= flat[3 * y] && flat[3 * x + 1] <= flat[3 * y + 1] &&
+                            flat[3 * x + 2] <= flat[3 * y + 2]) { dominated = true; break; }
+                    if (!dominated) m.push_back(x);
+                }
+                memo[key(TS, TJ, TI)] = std::move(m);
+            }
+    off.assign(1, 0);
+    idx.clear();
+    for (int t = 0; t < n; ++t) {
+        const auto& d = memo[key(flat[3 * t], flat[3 * t + 1], flat[3 * t + 2])];
+        idx.insert(idx.end(), d.begin(), d.end());
+        off.push_back((int)idx.size());
+    }
+}
+
 cc_status launch_tile_batch(const SorTile& t, bool mixed, int nt, hipStream_t s, const SorBatchEntry* tab,
                             const int* tiles4) {
 #define CC_SOR_BLAUNCH64(S_, WJ_, R_)                                                                         \
@@ -556,6 +725,35 @@ cc_status Poisson2D::init(int Nx_, int Ny_, bool px_, bool py_, double dx_, doub
             return CC_ERR_OOM;
         }
         CC_HIP(hipMemcpy(sched, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice));
+        // the dataflow form of the same schedule (CC_SOR_LEVELS keeps one launch per level)
+        const int ntiles = (int)flat.size() / 3;
+        if (sor_flow_enabled(ntiles)) {
+            std::vector<int> off, idx;
+            sor_flow_deps(flat, off, idx);
+            if (hipMalloc(&dep_off, off.size() * sizeof(int)) != hipSuccess ||
+                hipMalloc(&dep_idx, std::max<size_t>(idx.size(), 1) * sizeof(int)) != hipSuccess ||
+                hipMalloc(&flow_state, ((size_t)ntiles + 2) * sizeof(int)) != hipSuccess) {
+                set_error(CC_ERR_OOM, "Poisson2D: dataflow schedule allocation failed");
+                return CC_ERR_OOM;
+            }
+            CC_HIP(hipMemcpy(dep_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+            if (!idx.empty()) CC_HIP(hipMemcpy(dep_idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
+            CC_HIP(hipMemset(flow_state, 0, ((size_t)ntiles + 2) * sizeof(int)));
+            flow_ = true;
+        }
+    }
+    return CC_OK;
+}
+
+cc_status Poisson2D::flow_error(hipStream_t s) {
+    if (!flow_) return CC_OK;
+    int e = 0;
+    CC_HIP(hipMemcpyAsync(&e, flow_state + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    CC_HIP(hipStreamSynchronize(s));
+    if (e != 0) {
+        CC_HIP(hipMemsetAsync(flow_state + 1, 0, sizeof(int), s));
+        set_error(CC_ERR_HIP, "SOR dataflow: a tile's dependency wait timed out");
+        return CC_ERR_HIP;
     }
     return CC_OK;
 }
@@ -640,23 +838,43 @@ cc_status Poisson2D::autotuned_tile(SorTile& out) {
     SorTile bt = out;
     for (const SorTile& c : cands) {
         if (st != CC_OK) break;
-        std::vector<int> flat, off;
+        std::vector<int> flat, off, doff, didx;
         sor_levels(c, Nx, Ny, iters, flat, off);
+        const int ntiles = (int)flat.size() / 3;
+        const bool flow = sor_flow_enabled(ntiles);
+        if (flow) sor_flow_deps(flat, doff, didx);
         int* sc = nullptr;
-        if (hipMalloc(&sc, flat.size() * sizeof(int)) != hipSuccess) { st = CC_ERR_OOM; set_error(st, "SOR autotune: allocation failed"); break; }
+        int *d_off = nullptr, *d_idx = nullptr, *d_st = nullptr;
+        if (hipMalloc(&sc, flat.size() * sizeof(int)) != hipSuccess ||
+            (flow && (hipMalloc(&d_off, doff.size() * sizeof(int)) != hipSuccess ||
+                      hipMalloc(&d_idx, std::max<size_t>(didx.size(), 1) * sizeof(int)) != hipSuccess ||
+                      hipMalloc(&d_st, ((size_t)ntiles + 2) * sizeof(int)) != hipSuccess))) {
+            st = CC_ERR_OOM; set_error(st, "SOR autotune: allocation failed");
+        }
         float ms = std::numeric_limits<float>::max();
-        if (hipMemcpy(sc, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) st = CC_ERR_HIP;
+        if (st == CC_OK && hipMemcpy(sc, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) st = CC_ERR_HIP;
+        if (st == CC_OK && flow && (hipMemcpy(d_off, doff.data(), doff.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+                                    (!didx.empty() && hipMemcpy(d_idx, didx.data(), didx.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)))
+            st = CC_ERR_HIP;
         for (int rep = 0; rep < 2 && st == CC_OK; ++rep) {  // the first run pays the first-launch costs
             hipEventRecord(e0, 0);
-            for (size_t l = 0; l + 1 < off.size() && st == CC_OK; ++l)
-                st = launch_tile(c, mixed, off[l + 1] - off[l], 0, args, sc + 3 * off[l], phi, rz, rhobar);
+            if (flow) {
+                hipLaunchKernelGGL(k_fill_int, dim3(std::min(64, (ntiles + 2 + 255) / 256)), dim3(256), 0, 0, d_st, ntiles + 2, 0);
+                st = launch_flow(c, mixed, std::min(ntiles, 512), 0, args, FlowSched{sc, d_off, d_idx, ntiles, d_st}, phi, rz, rhobar);
+            } else {
+                for (size_t l = 0; l + 1 < off.size() && st == CC_OK; ++l)
+                    st = launch_tile(c, mixed, off[l + 1] - off[l], 0, args, sc + 3 * off[l], phi, rz, rhobar);
+            }
             hipEventRecord(e1, 0);
             float t = 0.f;
             if (st == CC_OK && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&t, e0, e1) == hipSuccess)
                 ms = rep ? std::min(ms, t) : ms;
             else if (st == CC_OK) { st = CC_ERR_HIP; set_error(st, "SOR autotune: timing failed"); }
         }
-        hipFree(sc);
+        if (sc) hipFree(sc);
+        if (d_off) hipFree(d_off);
+        if (d_idx) hipFree(d_idx);
+        if (d_st) hipFree(d_st);
         if (st == CC_OK && ms < best) { best = ms; bt = c; }
     }
     if (e0) hipEventDestroy(e0);
@@ -674,8 +892,13 @@ void Poisson2D::destroy() {
     if (partial) hipFree(partial);
     if (rhobar) hipFree(rhobar);
     if (sched) hipFree(sched);
+    if (dep_off) hipFree(dep_off);
+    if (dep_idx) hipFree(dep_idx);
+    if (flow_state) hipFree(flow_state);
     phi = partial = rhobar = nullptr;
     sched = nullptr;
+    dep_off = dep_idx = flow_state = nullptr;
+    flow_ = false;
     level_off_.clear();
 }
 
@@ -697,7 +920,16 @@ cc_status Poisson2D::prologue(const double* rho, hipStream_t s, Profiler* prof) 
 }
 
 cc_status Poisson2D::sweeps(const double* rho, hipStream_t s, Profiler* prof) {
-    if (tiled_) {
+    if (tiled_ && flow_) {
+        const int ntiles = tiles_total();
+        ProfScope ps(prof, s, "sor");
+        hipLaunchKernelGGL(k_flow_reset, dim3(std::min(64, (ntiles + 2 + 255) / 256)), dim3(256), 0, s, flow_state,
+                           ntiles);
+        CC_CHECK_LAUNCH();
+        FlowSched f{sched, dep_off, dep_idx, ntiles, flow_state};
+        // two 1024-thread tiles fit a CU: a grid of 2 per CU keeps every block resident
+        CC_TRY(launch_flow(tile_, precision == CC_MIXED, std::min(ntiles, 512), s, args, f, phi, rho, rhobar));
+    } else if (tiled_) {
         const int nl = levels();
         for (int l = 0; l < nl; ++l) {
             const int nt = level_off_[l + 1] - level_off_[l];

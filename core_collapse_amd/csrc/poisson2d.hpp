@@ -66,6 +66,8 @@ class Poisson2D {
     cc_status sweeps(const double* rho, hipStream_t s, Profiler* prof);
     cc_status epilogue(double* gx, double* gy, hipStream_t s, Profiler* prof);  // g = -central grad phi
     const std::vector<int>& schedule_host() const { return sched_host_; }
+    // The dataflow schedule hit its bounded wait (a fault in the schedule; results are wrong).  Syncs s.
+    cc_status flow_error(hipStream_t s);
     // phi with its mean removed (after solve), device array.
     cc_status phi_mean_removed(double* out_dev, hipStream_t s);
     bool tiled() const { return tiled_; }
@@ -84,6 +86,10 @@ class Poisson2D {
     SorTile tile_{};            // tile shape of the tiled schedule
     std::vector<int> level_off_;
     std::vector<int> sched_host_;  // the schedule's (TS, TJ, TI) triples, grouped by level
+    bool flow_ = false;            // tiled sweeps as one dataflow launch (else one launch per level)
+    int* dep_off = nullptr;        // dataflow dependencies (CSR over the tiles of sched)
+    int* dep_idx = nullptr;
+    int* flow_state = nullptr;     // [0] tile counter, [1] error flag, [2 ..] done flags
     SorArgs args{};
 };
 
@@ -91,6 +97,11 @@ class Poisson2D {
 // with tile shape t.
 cc_status launch_tile_batch(const SorTile& t, bool mixed, int nt, hipStream_t s, const SorBatchEntry* tab,
                             const int* tiles4);
+
+// Whether a tiled solve of ntiles tiles runs as one dataflow launch (else one launch per level).
+bool sor_flow_enabled(int ntiles);
+// Dependencies of the dataflow schedule of `flat` (level order): CSR offsets and tile indices.
+void sor_flow_deps(const std::vector<int>& flat, std::vector<int>& off, std::vector<int>& idx);
 
 // The tile levels of the tiled schedule for a tile shape (flat (TS, TJ, TI) triples, level offsets).
 void sor_levels(const SorTile& t, int Nx, int Ny, int iters, std::vector<int>& flat, std::vector<int>& level_off);
