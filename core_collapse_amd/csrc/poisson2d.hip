@@ -760,8 +760,8 @@ cc_status Poisson2D::flow_error(hipStream_t s) {
 
 // The tile levels of the tiled schedule: (TS, TJ, TI) of every tile that updates at least one cell,
 // grouped by level TS + TJ + TI (flat, 3 ints per tile), and the level offsets (in tiles).
-void sor_levels(const SorTile& t, int Nx, int Ny, int iters, std::vector<int>& flat, std::vector<int>& level_off) {
-    const int W = kTileW, WJ = t.WJ, S = t.S;
+// W is the tile width in cells (kTileW in the kernels).
+void sor_levels_w(int W, int WJ, int S, int Nx, int Ny, int iters, std::vector<int>& flat, std::vector<int>& level_off) {
     const int nTS = (iters + S - 1) / S;
     const int nTI = (Nx + iters - 1 + W - 1) / W;
     const int nTJ = (Ny + iters - 1 + WJ - 1) / WJ;
@@ -788,6 +788,10 @@ void sor_levels(const SorTile& t, int Nx, int Ny, int iters, std::vector<int>& f
         flat.insert(flat.end(), v.begin(), v.end());
         level_off.push_back((int)flat.size() / 3);
     }
+}
+
+void sor_levels(const SorTile& t, int Nx, int Ny, int iters, std::vector<int>& flat, std::vector<int>& level_off) {
+    sor_levels_w(kTileW, t.WJ, t.S, Nx, Ny, iters, flat, level_off);
 }
 
 // ------------------------------------------------------------------ tile autotuner
@@ -998,6 +1002,31 @@ cc_status cc_sor_tile_choice(int Nx, int Ny, int sor_iters, int precision, int* 
     }
     *S = t.S;
     *WJ = t.WJ;
+    return CC_OK;
+}
+
+cc_status cc_sor_flow_schedule(int Nx, int Ny, int iters, int W, int WJ, int S, int* ntiles, int* ndeps, int* tiles,
+                               int* dep_off, int* dep_idx) {
+    if (Nx < 1 || Ny < 1 || iters < 1 || W < 1 || WJ < 1 || S < 1 || !ntiles || !ndeps ||
+        (double)(Nx + iters) * (Ny + iters) * iters / ((double)W * WJ * S) > 1.0e7) {
+        cc::set_error(CC_ERR_INVALID_ARG, "cc_sor_flow_schedule: bad grid, tile or output");
+        return CC_ERR_INVALID_ARG;
+    }
+    std::vector<int> flat, off, doff, didx;
+    cc::sor_levels_w(W, WJ, S, Nx, Ny, iters, flat, off);
+    cc::sor_flow_deps(flat, doff, didx);
+    const int n = (int)flat.size() / 3, nd = (int)didx.size();
+    if (tiles && dep_off && dep_idx) {
+        if (*ntiles < n || *ndeps < nd) {
+            cc::set_error(CC_ERR_INVALID_ARG, "cc_sor_flow_schedule: output arrays too small");
+            return CC_ERR_INVALID_ARG;
+        }
+        std::copy(flat.begin(), flat.end(), tiles);
+        std::copy(doff.begin(), doff.end(), dep_off);
+        std::copy(didx.begin(), didx.end(), dep_idx);
+    }
+    *ntiles = n;
+    *ndeps = nd;
     return CC_OK;
 }
 

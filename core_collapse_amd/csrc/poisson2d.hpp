@@ -104,6 +104,7 @@ bool sor_flow_enabled(int ntiles);
 void sor_flow_deps(const std::vector<int>& flat, std::vector<int>& off, std::vector<int>& idx);
 
 // The tile levels of the tiled schedule for a tile shape (flat (TS, TJ, TI) triples, level offsets).
+void sor_levels_w(int W, int WJ, int S, int Nx, int Ny, int iters, std::vector<int>& flat, std::vector<int>& level_off);
 void sor_levels(const SorTile& t, int Nx, int Ny, int iters, std::vector<int>& flat, std::vector<int>& level_off);
 bool sor_autotune_enabled();
 

@@ -190,6 +190,22 @@ def sor_tile_choice(Nx, Ny, sor_iters=600, precision=CC_FP64):
     return S.value, WJ.value
 
 
+def sor_flow_schedule(Nx, Ny, iters, W=64, WJ=64, S=16):
+    """The tiled SOR's dataflow schedule (cc_sor_flow_schedule; host only): a list of tiles (TS, TJ, TI)
+    in launch order and, per tile, the indices of the tiles it waits for."""
+    lib = _lib.lib()
+    n, nd = C.c_int(), C.c_int()
+    args = [int(v) for v in (Nx, Ny, iters, W, WJ, S)]
+    check(lib.cc_sor_flow_schedule(*args, C.byref(n), C.byref(nd), None, None, None))
+    tiles = np.zeros(3 * n.value, dtype=np.int32)
+    off = np.zeros(n.value + 1, dtype=np.int32)
+    idx = np.zeros(max(nd.value, 1), dtype=np.int32)
+    check(lib.cc_sor_flow_schedule(*args, C.byref(n), C.byref(nd), tiles.ctypes.data, off.ctypes.data,
+                                   idx.ctypes.data))
+    tl = [tuple(int(v) for v in tiles[3 * k:3 * k + 3]) for k in range(n.value)]
+    return tl, [[int(v) for v in idx[off[k]:off[k + 1]]] for k in range(n.value)]
+
+
 class Mhd2DBatch:
     """Independent Mhd2D instances stepped together: one HIP graph launch per step runs every
     instance's Heun step as a parallel branch, each with its own dt (cc_mhd2d_batch_*)."""

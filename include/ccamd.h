@@ -226,6 +226,15 @@ cc_status cc_hlld3d_batch(const cc_mhd3d_params* p, int n, const double* WL, con
 cc_status cc_sor_autotune(int enable);
 cc_status cc_sor_tile_choice(int Nx, int Ny, int sor_iters, int precision, int* S, int* WJ);
 
+/* The dataflow schedule of the tiled SOR (no reference counterpart; host only, touches no GPU): for an
+ * Nx x Ny clamped grid, `iters` sweeps and tiles of S sweeps x W x WJ cells (W = 64 in the kernels),
+ * the tiles (TS, TJ, TI) that update at least one cell, in launch order (3 ints each), and per tile the
+ * tiles it waits for before it starts, as CSR (dep_off: ntiles + 1 offsets into dep_idx).  Call with
+ * tiles, dep_off or dep_idx null to get *ntiles and *ndeps; then with arrays of 3 *ntiles, *ntiles + 1
+ * and *ndeps ints and *ntiles, *ndeps set to their capacities. */
+cc_status cc_sor_flow_schedule(int Nx, int Ny, int iters, int W, int WJ, int S, int* ntiles, int* ndeps, int* tiles,
+                               int* dep_off, int* dep_idx);
+
 /* ============================================================================================
  * 2D Euler HLLC + SOR self-gravity (EH).
  * ========================================================================================== */

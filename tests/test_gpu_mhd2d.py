@@ -223,6 +223,33 @@ def test_mixed_vs_fp64_l1_density():
     assert e <= 1e-4, e
 
 
+@pytest.mark.parametrize("prec", [CC_FP64, CC_MIXED])
+def test_sor_dataflow_equals_level_launches_bitwise(prec, monkeypatch):
+    """The default tiled solve (one persistent launch, tiles started as their waits clear) and the
+    one-launch-per-level solve (CC_SOR_LEVELS) give the same bits: a standalone solve on an uneven grid,
+    and collapse steps run through the captured step graph."""
+    rng = np.random.default_rng(17)
+    Nx, Ny = 300, 170
+    rho = 1.0 + rng.uniform(0, 1, (Ny, Nx))
+    p = ic.collapse2d_params(160, precision=prec)
+    st = ic.collapse2d_ic(p)
+    out = {}
+    for levels in (False, True):
+        if levels:
+            monkeypatch.setenv("CC_SOR_LEVELS", "1")
+        else:
+            monkeypatch.delenv("CC_SOR_LEVELS", raising=False)
+        gx, gy = M.poisson_sor(rho, 1.0 / Nx, 1.0 / Ny, 1.0, 600, 1.8, False, False, precision=prec)
+        with M.Mhd2D(p) as h:
+            h.upload(st)
+            t = h.run(3)
+            out[levels] = (gx, gy, t, h.download())
+    a, b = out[False], out[True]
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
+    for f in M.FIELDS:
+        assert np.array_equal(a[3][f], b[3][f]), f
+
+
 @pytest.mark.parametrize("separate_sor", [False, True])
 @pytest.mark.parametrize("prec", [CC_FP64, CC_MIXED])
 def test_batch_equals_instances_run_alone(prec, separate_sor, monkeypatch):

@@ -10,6 +10,8 @@ so the order in which the GPU runs the tiles of a level cannot change the result
 import numpy as np
 import pytest
 
+from core_collapse_amd import mhd2d as M
+
 
 def cell_update(phi, rhs, i, j, Nx, Ny, idx2, idy2, inv_denom, omega):
     pE = phi[j, min(i + 1, Nx - 1)]
@@ -73,30 +75,36 @@ def tile_updates(i, j, s0, s1, TI, TJ, W, WJ):
     return lo <= hi
 
 
+def run_tile(phi, rhs, tile, nsw, idx2, idy2, omega, W, WJ, S):
+    """One tile of the kernel: copy the tile's box, update the copy in step order, write back the
+    cells the predicate selects."""
+    Ny, Nx = rhs.shape
+    inv = 1.0 / (2.0 * (idx2 + idy2))
+    TS, TJ, TI = tile
+    s0, s1 = S * TS, min(S * TS + S, nsw)
+    box = phi.copy()
+    for (_, _, i, j) in tile_cells(tile, Nx, Ny, nsw, W, WJ, S):
+        cell_update(box, rhs, i, j, Nx, Ny, idx2, idy2, inv, omega)
+    ib0, ib1 = max(0, W * TI - (s1 - 1) - 1), min(Nx - 1, W * TI - s0 + W)
+    jb0, jb1 = max(0, WJ * TJ - (s1 - 1) - 1), min(Ny - 1, WJ * TJ - s0 + WJ)
+    changed = box != phi
+    for j in range(jb0, jb1 + 1):
+        for i in range(ib0, ib1 + 1):
+            if tile_updates(i, j, s0, s1, TI, TJ, W, WJ):
+                phi[j, i] = box[j, i]
+    # every cell the tile changed lies in its box and passes the predicate
+    cj, ci = np.nonzero(changed)
+    assert all(ib0 <= i <= ib1 and jb0 <= j <= jb1 and tile_updates(i, j, s0, s1, TI, TJ, W, WJ)
+               for i, j in zip(ci, cj))
+
+
 def tiled(rhs, nsw, idx2, idy2, omega, W, WJ, S, reverse_levels=False):
-    """The kernel: copy the tile's box, update the copy in step order, write back the cells the
-    predicate selects."""
+    """The kernel's tiles in level order (each level's tiles in either order)."""
     Ny, Nx = rhs.shape
     phi = np.zeros_like(rhs)
-    inv = 1.0 / (2.0 * (idx2 + idy2))
     for lev in tile_levels(Nx, Ny, nsw, W, WJ, S):
         for tile in (reversed(lev) if reverse_levels else lev):
-            TS, TJ, TI = tile
-            s0, s1 = S * TS, min(S * TS + S, nsw)
-            box = phi.copy()
-            for (_, _, i, j) in tile_cells(tile, Nx, Ny, nsw, W, WJ, S):
-                cell_update(box, rhs, i, j, Nx, Ny, idx2, idy2, inv, omega)
-            ib0, ib1 = max(0, W * TI - (s1 - 1) - 1), min(Nx - 1, W * TI - s0 + W)
-            jb0, jb1 = max(0, WJ * TJ - (s1 - 1) - 1), min(Ny - 1, WJ * TJ - s0 + WJ)
-            changed = box != phi
-            for j in range(jb0, jb1 + 1):
-                for i in range(ib0, ib1 + 1):
-                    if tile_updates(i, j, s0, s1, TI, TJ, W, WJ):
-                        phi[j, i] = box[j, i]
-            # every cell the tile changed lies in its box and passes the predicate
-            cj, ci = np.nonzero(changed)
-            assert all(ib0 <= i <= ib1 and jb0 <= j <= jb1 and tile_updates(i, j, s0, s1, TI, TJ, W, WJ)
-                       for i, j in zip(ci, cj))
+            run_tile(phi, rhs, tile, nsw, idx2, idy2, omega, W, WJ, S)
     return phi
 
 
@@ -352,3 +360,76 @@ def test_row_thread_kernel_equals_lexicographic_bitwise(Nx, Ny, px, py, nsw, PD)
     for loads_first in (False, True):
         got = skew_rows(rhs, nsw, float(Nx * Nx), float(Ny * Ny), 1.8, px, py, PD, loads_first)
         assert np.array_equal(got, want), loads_first
+
+
+# ------------------------------------------------------------------ dataflow schedule
+# The default tiled solve is one persistent launch: workgroups take tiles in launch (level) order from
+# an atomic counter and start a tile once every tile it waits for has set its done flag.  The waits
+# come from the library itself (cc_sor_flow_schedule, the host code that builds the device tables).
+
+
+def _footprint(tile, Nx, Ny, nsw, W, WJ, S):
+    cells = tile_cells(tile, Nx, Ny, nsw, W, WJ, S)
+    w = {(i, j) for (_, _, i, j) in cells}
+    r = set(w)
+    for (i, j) in w:
+        r |= {(min(i + 1, Nx - 1), j), (max(i - 1, 0), j), (i, min(j + 1, Ny - 1)), (i, max(j - 1, 0))}
+    return w, r
+
+
+FLOW_CASES = [(9, 7, 9, 4, 3, 2), (13, 11, 12, 4, 4, 3), (11, 13, 9, 4, 4, 3), (20, 6, 30, 5, 2, 7),
+              (6, 20, 17, 3, 6, 5), (150, 140, 40, 64, 64, 16), (100, 200, 70, 64, 128, 32)]
+
+
+@pytest.mark.parametrize("Nx,Ny,nsw,W,WJ,S", FLOW_CASES)
+def test_dataflow_waits_order_every_conflicting_pair(Nx, Ny, nsw, W, WJ, S):
+    """Two tiles of which one writes a cell the other reads are never in flight together: the later
+    one (in level order) waits for the earlier one, directly or through a chain of waits.  Pairs
+    without such a cell commute, so every execution the waits allow gives the level order's result."""
+    tiles, deps = M.sor_flow_schedule(Nx, Ny, nsw, W, WJ, S)
+    assert tiles == [t for lev in tile_levels(Nx, Ny, nsw, W, WJ, S) for t in lev]
+    n = len(tiles)
+    anc = []
+    for k in range(n):
+        a = set()
+        for d in deps[k]:
+            # a wait points to a tile below in every coordinate, hence earlier in launch order: the
+            # tiles a workgroup waits for were handed out before its own, so the waits cannot deadlock
+            assert d < k and tiles[d] != tiles[k] and all(tiles[d][c] <= tiles[k][c] for c in range(3))
+            a |= {d} | anc[d]
+        anc.append(a)
+    fp = [_footprint(t, Nx, Ny, nsw, W, WJ, S) for t in tiles]
+    for b in range(n):
+        for a in range(b):
+            if fp[a][0] & fp[b][1] or fp[b][0] & fp[a][1]:
+                assert a in anc[b], (tiles[a], tiles[b])
+    # the waits are minimal: no wait is implied by another one
+    for k in range(n):
+        for d in deps[k]:
+            assert not any(d in anc[e] for e in deps[k] if e != d)
+
+
+@pytest.mark.parametrize("Nx,Ny,nsw,W,WJ,S", FLOW_CASES[:5])
+def test_dataflow_random_orders_equal_lexicographic_bitwise(Nx, Ny, nsw, W, WJ, S):
+    tiles, deps = M.sor_flow_schedule(Nx, Ny, nsw, W, WJ, S)
+    rng = np.random.default_rng(Nx * 31 + Ny)
+    rhs = rng.standard_normal((Ny, Nx))
+    idx2, idy2 = float(Nx * Nx), float(Ny * Ny)
+    ref = lexicographic(rhs, nsw, idx2, idy2, 1.8)
+    for _ in range(3):
+        phi = np.zeros_like(rhs)
+        done = [False] * len(tiles)
+        while not all(done):
+            ready = [k for k in range(len(tiles)) if not done[k] and all(done[d] for d in deps[k])]
+            k = ready[rng.integers(len(ready))]
+            run_tile(phi, rhs, tiles[k], nsw, idx2, idy2, 1.8, W, WJ, S)
+            done[k] = True
+        assert np.array_equal(phi, ref)
+
+
+def test_dataflow_schedule_rejects_bad_arguments():
+    from core_collapse_amd._lib import CCError
+    with pytest.raises(CCError):
+        M.sor_flow_schedule(0, 8, 10)
+    with pytest.raises(CCError):
+        M.sor_flow_schedule(8, 8, 10, W=0)
