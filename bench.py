@@ -493,16 +493,23 @@ def main():
         ach = bytes_per_launch / avg_s / 1e9
         return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": sor_kernel, "launches": n, "avg_launch_us": round(avg_s * 1e6, 2),
+                "kernel": None, "launches": n, "avg_launch_us": round(avg_s * 1e6, 2),
                 "alg_bytes_per_launch": bytes_per_launch}
 
-    sor_kernel = "k_sor_tile" if os.environ.get("CC_SOR_LEVELS") else "k_sor_flow"
+    def sor_kernel(tile):
+        S, WJ, sheared = tile
+        return ("k_sor_s" if sheared else "k_sor_") + ("tile" if os.environ.get("CC_SOR_LEVELS") else "flow")
+
     roof_mx = roof(res[CC_MIXED], 12.0)
     roof_64 = roof(res[CC_FP64], 24.0)
     if roof_mx:
-        roof_mx["traffic"] = load_traffic("cc::%s_mixed<%d, %d>" % ((sor_kernel,) + tuple(res[CC_MIXED]["tile"])))
+        tl = res[CC_MIXED]["tile"]
+        roof_mx["kernel"] = sor_kernel(tl)
+        roof_mx["traffic"] = load_traffic("cc::%s_mixed<%d, %d>" % (sor_kernel(tl), tl[0], tl[1]))
     if roof_64:
-        roof_64["traffic"] = load_traffic("cc::%s_fp64<%d, %d," % ((sor_kernel,) + tuple(res[CC_FP64]["tile"])))
+        tl = res[CC_FP64]["tile"]
+        roof_64["kernel"] = sor_kernel(tl)
+        roof_64["traffic"] = load_traffic("cc::%s_fp64<%d, %d%s" % (sor_kernel(tl), tl[0], tl[1], ">" if tl[2] else ","))
     kernels = {}
     for prec, tag in ((CC_FP64, "fp64"), (CC_MIXED, "mixed")):
         kernels[tag] = {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in res[prec]["prof"].items()

@@ -843,8 +843,8 @@ struct cc_mhd2d_batch {
         const cc_mhd2d* h0 = h[0];
         for (const cc_mhd2d* x : h) {
             const Poisson2D& P = x->pois;
-            if (!x->p.gravity_on || !P.tiled() || P.iters <= 0 || x->p.precision != h0->p.precision ||
-                P.tile_.S != h0->pois.tile_.S || P.tile_.WJ != h0->pois.tile_.WJ || P.tile_.rhs_lds != h0->pois.tile_.rhs_lds)
+            // the merged launches run one tile shape of their own; the instances' shapes may differ
+            if (!x->p.gravity_on || !P.tiled() || P.iters <= 0 || x->p.precision != h0->p.precision)
                 return CC_OK;  // not mergeable: the instances keep their own SOR launches
         }
         mixed = h0->p.precision == CC_MIXED;
@@ -863,10 +863,16 @@ struct cc_mhd2d_batch {
         if (sor_autotune_enabled()) {
 #define CC_BCAND64(S_, WJ_, R_) if (!mixed) cands.push_back(SorTile{S_, WJ_, R_});
 #define CC_BCAND32(S_, WJ_) if (mixed) cands.push_back(SorTile{S_, WJ_, true});
+#define CC_BSCAND64(S_, K_) if (!mixed) cands.push_back(SorTile{S_, K_, true, true});
+#define CC_BSCAND32(S_, K_) if (mixed) cands.push_back(SorTile{S_, K_, true, true});
             CC_SOR_TILES_FP64(CC_BCAND64)
             CC_SOR_TILES_MIXED(CC_BCAND32)
+            CC_SOR_SHEARED_FP64(CC_BSCAND64)
+            CC_SOR_SHEARED_MIXED(CC_BSCAND32)
 #undef CC_BCAND64
 #undef CC_BCAND32
+#undef CC_BSCAND64
+#undef CC_BSCAND32
         }
         float best = 3.0e38f;
         for (size_t ci = 0; ci < cands.size(); ++ci) {
@@ -1012,11 +1018,12 @@ cc_status cc_mhd2d_batch_create(cc_mhd2d* const* handles, int n, cc_mhd2d_batch*
     return CC_OK;
 }
 
-cc_status cc_mhd2d_batch_info(cc_mhd2d_batch* b, int* merged, int* S, int* WJ) {
+cc_status cc_mhd2d_batch_info(cc_mhd2d_batch* b, int* merged, int* S, int* WJ, int* sheared) {
     if (!b || !merged || !S || !WJ) { set_error(CC_ERR_INVALID_ARG, "cc_mhd2d_batch_info: null argument"); return CC_ERR_INVALID_ARG; }
     *merged = b->merged ? 1 : 0;
     *S = b->merged ? b->tile.S : 0;
     *WJ = b->merged ? b->tile.WJ : 0;
+    if (sheared) *sheared = b->merged && b->tile.sheared ? 1 : 0;
     return CC_OK;
 }
 

@@ -172,6 +172,10 @@ __device__ __forceinline__ void phi_st(double* p, double v) {
     if constexpr (COH) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     else *p = v;
 }
+template <bool COH>
+__device__ __forceinline__ void phi_add(double* p, double v) {
+    (void)__hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, COH ? __HIP_MEMORY_SCOPE_SYSTEM : __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // FP64: phi is staged in LDS; the source term 4 pi G (rho - mean) either too (RHS_LDS: twice the
 // LDS, one tile per CU) or it is loaded from global memory one step ahead (half the LDS, two tiles
@@ -292,6 +296,287 @@ __device__ __forceinline__ void sor_tile_mixed(const SorArgs& a, const int* __re
     }
 }
 
+// ------------------------------------------------------------------ sheared tiles
+// Cell (i, j) of sweep s sits at k = i + j + 2s on the skewed time axis and reads only values written
+// at k - 1 and k - 2 (its W and S neighbours of sweep s, its E and N neighbours and itself of sweep
+// s - 1).  A sheared tile is a box of S sweeps x 64 values of i' = i + s x K values of k: at step d
+// (k = K TK + d) lane l of sweep s updates i = 64 TI + l - s, j = k - 64 TI - l - s.  Every lane of
+// an interior tile works at every step (a rectangular tile's diagonal wavefront keeps half of them
+// idle), and a tile needs K steps.  Cells written at one step have i + j of one parity and read only
+// cells of the other, so one barrier per step suffices.  The LDS box is kept in sheared coordinates
+// x = i - ib + 1, y = i + j - ub + 1 (ib, ub: the smallest i and i + j of the tile): a lane's cell
+// moves one row of the box per step, E/W neighbours are (x +- 1, y +- 1) and N/S are (x, y +- 1).
+// Tiles that touch a domain edge clamp the neighbours (M2:197-198).
+template <int S, int K>
+struct ShGeom {
+    static constexpr int BX = kTileW + S + 1;               // x range, margins included
+    static constexpr int BY = K + 2 * S;                    // y range, margins included
+    static constexpr int P = (BX % 2 == 0) ? BX : BX + 1;   // even pitch
+    static constexpr int NR = BX + BY - 1;                  // grid rows that cross the box
+};
+
+struct ShBox {
+    int TS, TK, TI, s0, s1, ib, ub;
+    bool interior;  // no clamped neighbour and no idle lane in the tile's active sweeps
+};
+
+template <int S, int K>
+__device__ __forceinline__ ShBox sh_box(const SorArgs& a, const int* __restrict__ tile) {
+    ShBox b;
+    b.TS = tile[0]; b.TK = tile[1]; b.TI = tile[2];
+    b.s0 = S * b.TS; b.s1 = min(b.s0 + S, a.nsw);
+    const int sl = b.s0 + S - 1;
+    b.ib = kTileW * b.TI - sl;
+    b.ub = K * b.TK - 2 * sl;
+    b.interior = b.ib >= 1 && b.ib + kTileW + S - 2 <= a.Nx - 2 && K * b.TK - (kTileW * b.TI + kTileW - 1) - sl >= 1 &&
+                 K * b.TK + K - 1 - kTileW * b.TI - b.s0 <= a.Ny - 2;
+    return b;
+}
+
+// Does the sheared tile update cell (i, j) at some sweep s in [s0, s1)?
+template <int K>
+__device__ __forceinline__ bool sh_updates(int i, int j, const ShBox& b) {
+    const int u = i + j;
+    const int lo = max(b.s0, max(kTileW * b.TI - i, (K * b.TK - u + 1) >> 1));
+    const int hi = min(b.s1 - 1, min(kTileW * b.TI + kTileW - 1 - i, (K * b.TK + K - 1 - u) >> 1));
+    return lo <= hi;
+}
+
+template <int S, int K>
+__device__ __forceinline__ LaneWalk sh_walk(const SorArgs& a, const ShBox& b, int s, int lane) {
+    using G = ShGeom<S, K>;
+    LaneWalk w;
+    w.ci = kTileW * b.TI + lane - s;
+    w.cj0 = K * b.TK - kTileW * b.TI - lane - s;  // cj at step d is cj0 + d
+    const bool ok = s < b.s1 && w.ci >= 0 && w.ci < a.Nx;
+    w.lo = ok ? max(0, -w.cj0) : 1 << 30;
+    w.hi = ok ? min(K, a.Ny - w.cj0) : 0;
+    w.base = (w.ci + w.cj0 - b.ub + 1) * G::P + (w.ci - b.ib + 1);
+    return w;
+}
+
+template <int S, int K, typename T, typename F>
+__device__ __forceinline__ void sh_steps(T* __restrict__ x, const SorArgs& a, const ShBox& b,
+                                         const LaneWalk (&w)[S / 16], F&& update) {
+    using G = ShGeom<S, K>;
+    constexpr int SPW = S / 16, P = G::P, EO = G::P + 1;
+    if (b.interior) {
+        for (int d = 0; d < K; ++d) {
+#pragma unroll
+            for (int m = 0; m < SPW; ++m) {
+                if (d >= w[m].lo && d < w[m].hi) {
+                    const int idx = w[m].base + d * P;
+                    x[idx] = update(idx, x[idx], x[idx + EO], x[idx - EO], x[idx + P], x[idx - P]);
+                }
+            }
+            __syncthreads();
+        }
+    } else {
+        int oE[SPW], oW[SPW];
+#pragma unroll
+        for (int m = 0; m < SPW; ++m) {
+            oE[m] = w[m].ci < a.Nx - 1 ? EO : 0;
+            oW[m] = w[m].ci > 0 ? -EO : 0;
+        }
+        for (int d = 0; d < K; ++d) {
+#pragma unroll
+            for (int m = 0; m < SPW; ++m) {
+                if (d >= w[m].lo && d < w[m].hi) {
+                    const int idx = w[m].base + d * P, cj = w[m].cj0 + d;
+                    const int oN = cj < a.Ny - 1 ? P : 0, oS = cj > 0 ? -P : 0;
+                    x[idx] = update(idx, x[idx], x[idx + oE[m]], x[idx + oW[m]], x[idx + oN], x[idx + oS]);
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// The box is staged and written back along the grid rows that cross it (coalesced along i): slot c
+// of thread t's walk is c = t + 1024 it, it < IT; sh_slot gives its LDS index (-1 when the slot lies
+// outside the box or the domain; cells outside the domain are never read), its cell and global index.
+// All of a thread's slots are loaded before any is stored, so the loads of a tile overlap.
+template <int S, int K>
+struct ShSlots {
+    static constexpr int N = ShGeom<S, K>::NR * ShGeom<S, K>::BX;
+    static constexpr int IT = (N + 1023) / 1024;
+};
+
+template <int S, int K>
+__device__ __forceinline__ int sh_slot(const SorArgs& a, const ShBox& b, int c, size_t& g, int& i, int& j) {
+    using G = ShGeom<S, K>;
+    const int i0 = b.ib - 1, u0 = b.ub - 1, jlo = u0 - i0 - (G::BX - 1);
+    const int x = c % G::BX, r = c / G::BX;
+    i = i0 + x;
+    j = jlo + r;
+    const int y = i + j - u0;
+    const bool in = c < ShSlots<S, K>::N && y >= 0 && y < G::BY && i >= 0 && i < a.Nx && j >= 0 && j < a.Ny;
+    g = in ? (size_t)j * a.Nx + i : 0;
+    return in ? y * G::P + x : -1;
+}
+
+template <int S, int K, bool COH = false>
+__device__ __forceinline__ void sor_sheared_fp64(const SorArgs& a, const int* __restrict__ tile,
+                                                 double* __restrict__ phi, const double* __restrict__ rho,
+                                                 const double* __restrict__ rhobar) {
+    using G = ShGeom<S, K>;
+    constexpr int SPW = S / 16, P = G::P, IT = ShSlots<S, K>::IT, CH = 4;
+    __shared__ double sphi[G::BY * P];
+    __shared__ double srhs[G::BY * P];
+    const ShBox b = sh_box<S, K>(a, tile);
+    const double rb = rhobar[0];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+#pragma unroll 1
+    for (int it0 = 0; it0 < IT; it0 += CH) {  // CH slots in flight per thread
+        int li[CH];
+        double vp[CH], vr[CH];
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            size_t g = 0;
+            int i, j;
+            li[q] = it0 + q < IT ? sh_slot<S, K>(a, b, tid + 1024 * (it0 + q), g, i, j) : -1;
+            vp[q] = li[q] >= 0 ? phi_ld<COH>(&phi[g]) : 0.0;
+            vr[q] = li[q] >= 0 ? rho[g] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            if (li[q] >= 0) {
+                sphi[li[q]] = vp[q];
+                srhs[li[q]] = a.c4piG * (vr[q] - rb);
+            }
+        }
+    }
+    LaneWalk w[SPW];
+#pragma unroll
+    for (int m = 0; m < SPW; ++m) w[m] = sh_walk<S, K>(a, b, b.s0 + wv * SPW + m, lane);
+    const double idx2 = a.idx2, idy2 = a.idy2, inv_denom = a.inv_denom, omega = a.omega;
+    __syncthreads();
+    sh_steps<S, K>(sphi, a, b, w, [&](int idx, double pc, double pE, double pW, double pN, double pS) {
+        const double pnew = ((pE + pW) * idx2 + (pN + pS) * idy2 - srhs[idx]) * inv_denom;
+        return pc + omega * (pnew - pc);
+    });
+#pragma unroll 1
+    for (int it = 0; it < IT; ++it) {
+        size_t g;
+        int i, j;
+        const int l = sh_slot<S, K>(a, b, tid + 1024 * it, g, i, j);
+        if (l >= 0 && sh_updates<K>(i, j, b)) phi_st<COH>(&phi[g], sphi[l]);
+    }
+}
+
+// Mixed precision on a sheared tile: the FP32 correction form of sor_tile_mixed.  phi_hi is staged in
+// FP64 (each cell loaded once) into LDS whose first half then holds the FP32 correction; the FP32
+// residual c, formed from the staged box, has its own LDS array.
+template <int S, int K, bool COH = false>
+__device__ __forceinline__ void sor_sheared_mixed(const SorArgs& a, const int* __restrict__ tile,
+                                                  double* __restrict__ phi, const double* __restrict__ rho,
+                                                  const double* __restrict__ rhobar) {
+    using G = ShGeom<S, K>;
+    constexpr int SPW = S / 16, P = G::P, IT = ShSlots<S, K>::IT, EO = G::P + 1, CH = 4;
+    __shared__ double sbuf[G::BY * P];
+    __shared__ float sc[G::BY * P];
+    float* sd = reinterpret_cast<float*>(sbuf);
+    const ShBox b = sh_box<S, K>(a, tile);
+    const double rb = rhobar[0];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+#pragma unroll 1
+    for (int it0 = 0; it0 < IT; it0 += CH) {  // CH slots in flight per thread
+        int li[CH];
+        double vp[CH];
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            size_t g = 0;
+            int i, j;
+            li[q] = it0 + q < IT ? sh_slot<S, K>(a, b, tid + 1024 * (it0 + q), g, i, j) : -1;
+            vp[q] = li[q] >= 0 ? phi_ld<COH>(&phi[g]) : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < CH; ++q)
+            if (li[q] >= 0) sbuf[li[q]] = vp[q];
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int it0 = 0; it0 < IT; it0 += CH) {
+        int li[CH], ii[CH], jj[CH];
+        double vr[CH];
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            size_t g = 0;
+            ii[q] = jj[q] = 0;
+            li[q] = it0 + q < IT ? sh_slot<S, K>(a, b, tid + 1024 * (it0 + q), g, ii[q], jj[q]) : -1;
+            vr[q] = li[q] >= 0 ? rho[g] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            const int l = li[q], i = ii[q], j = jj[q];
+            if (l >= 0) {
+                float cv = 0.f;
+                if (sh_updates<K>(i, j, b)) {
+                    const double pc = sbuf[l];
+                    const double pE = sbuf[l + (i < a.Nx - 1 ? EO : 0)], pW = sbuf[l - (i > 0 ? EO : 0)];
+                    const double pN = sbuf[l + (j < a.Ny - 1 ? P : 0)], pS = sbuf[l - (j > 0 ? P : 0)];
+                    const double rhs = a.c4piG * (vr[q] - rb);
+                    cv = (float)(((pE + pW) * a.idx2 + (pN + pS) * a.idy2 - rhs) * a.inv_denom - pc);
+                }
+                sc[l] = cv;
+            }
+        }
+    }
+    __syncthreads();  // phi_hi is read; its first half becomes the correction
+#pragma unroll 1
+    for (int it = 0; it < IT; ++it) {
+        size_t g;
+        int i, j;
+        const int l = sh_slot<S, K>(a, b, tid + 1024 * it, g, i, j);
+        if (l >= 0) sd[l] = 0.f;
+    }
+    LaneWalk w[SPW];
+#pragma unroll
+    for (int m = 0; m < SPW; ++m) w[m] = sh_walk<S, K>(a, b, b.s0 + wv * SPW + m, lane);
+    const float kx = a.kxf, ky = a.kyf, om = a.omegaf;
+    __syncthreads();
+    sh_steps<S, K>(sd, a, b, w, [&](int idx, float dc, float dE, float dW, float dN, float dS) {
+        const float gs = sc[idx] + (dE + dW) * kx + (dN + dS) * ky;
+        return dc + om * (gs - dc);
+    });
+    // phi += correction, as a no-return atomic add (the tile is the only writer of these cells while it
+    // runs, so the sum is the plain FP64 sum): no phi_hi reload, nothing to wait for before the next tile
+#pragma unroll 1
+    for (int it = 0; it < IT; ++it) {
+        size_t g;
+        int i, j;
+        const int l = sh_slot<S, K>(a, b, tid + 1024 * it, g, i, j);
+        if (l >= 0 && sh_updates<K>(i, j, b)) phi_add<COH>(&phi[g], (double)sd[l]);
+    }
+}
+
+template <int S, int K>
+__global__ __launch_bounds__(1024) void k_sor_stile_fp64(SorArgs a, const int* __restrict__ tiles,
+                                                          double* __restrict__ phi, const double* __restrict__ rho,
+                                                          const double* __restrict__ rhobar) {
+    sor_sheared_fp64<S, K>(a, tiles + 3 * blockIdx.x, phi, rho, rhobar);
+}
+template <int S, int K>
+__global__ __launch_bounds__(1024) void k_sor_stile_mixed(SorArgs a, const int* __restrict__ tiles,
+                                                           double* __restrict__ phi, const double* __restrict__ rho,
+                                                           const double* __restrict__ rhobar) {
+    sor_sheared_mixed<S, K>(a, tiles + 3 * blockIdx.x, phi, rho, rhobar);
+}
+template <int S, int K>
+__global__ __launch_bounds__(1024) void k_sor_stile_fp64_batch(const SorBatchEntry* __restrict__ tab,
+                                                                const int* __restrict__ tiles) {
+    const int* t = tiles + 4 * blockIdx.x;
+    const SorBatchEntry e = tab[t[0]];
+    sor_sheared_fp64<S, K>(e.a, t + 1, e.phi, e.rho, e.rhobar);
+}
+template <int S, int K>
+__global__ __launch_bounds__(1024) void k_sor_stile_mixed_batch(const SorBatchEntry* __restrict__ tab,
+                                                                 const int* __restrict__ tiles) {
+    const int* t = tiles + 4 * blockIdx.x;
+    const SorBatchEntry e = tab[t[0]];
+    sor_sheared_mixed<S, K>(e.a, t + 1, e.phi, e.rho, e.rhobar);
+}
+
 // One tile per block: the tiles of one level of one grid, (TS, TJ, TI) per tile.
 template <int S, int WJ, bool RHS_LDS>
 __global__ __launch_bounds__(1024) void k_sor_tile_fp64(SorArgs a, const int* __restrict__ tiles,
@@ -389,6 +674,29 @@ __global__ __launch_bounds__(1024) void k_sor_flow_mixed(SorArgs a, FlowSched f,
         const int t = flow_next_tile(f);
         if (t >= f.ntiles) break;
         sor_tile_mixed<S, WJ, true>(a, f.tiles + 3 * t, phi, rho, rhobar);
+        flow_finish_tile(f, t);
+    }
+}
+
+template <int S, int K>
+__global__ __launch_bounds__(1024) void k_sor_sflow_fp64(SorArgs a, FlowSched f, double* __restrict__ phi,
+                                                          const double* __restrict__ rho,
+                                                          const double* __restrict__ rhobar) {
+    for (;;) {
+        const int t = flow_next_tile(f);
+        if (t >= f.ntiles) break;
+        sor_sheared_fp64<S, K, true>(a, f.tiles + 3 * t, phi, rho, rhobar);
+        flow_finish_tile(f, t);
+    }
+}
+template <int S, int K>
+__global__ __launch_bounds__(1024) void k_sor_sflow_mixed(SorArgs a, FlowSched f, double* __restrict__ phi,
+                                                           const double* __restrict__ rho,
+                                                           const double* __restrict__ rhobar) {
+    for (;;) {
+        const int t = flow_next_tile(f);
+        if (t >= f.ntiles) break;
+        sor_sheared_mixed<S, K, true>(a, f.tiles + 3 * t, phi, rho, rhobar);
         flow_finish_tile(f, t);
     }
 }
@@ -551,26 +859,32 @@ __global__ void k_sub_mean(const double* __restrict__ x, size_t n, const double*
 // ------------------------------------------------------------------ tile dispatch
 
 static bool sor_tile_supported(const SorTile& t, bool mixed) {
-#define CC_SOR_MATCH64(S_, WJ_, R_) if (!mixed && t.S == S_ && t.WJ == WJ_ && t.rhs_lds == R_) return true;
-#define CC_SOR_MATCH32(S_, WJ_) if (mixed && t.S == S_ && t.WJ == WJ_) return true;
+#define CC_SOR_MATCH64(S_, WJ_, R_) if (!t.sheared && !mixed && t.S == S_ && t.WJ == WJ_ && t.rhs_lds == R_) return true;
+#define CC_SOR_MATCH32(S_, WJ_) if (!t.sheared && mixed && t.S == S_ && t.WJ == WJ_) return true;
+#define CC_SOR_SMATCH64(S_, K_) if (t.sheared && !mixed && t.S == S_ && t.WJ == K_) return true;
+#define CC_SOR_SMATCH32(S_, K_) if (t.sheared && mixed && t.S == S_ && t.WJ == K_) return true;
     CC_SOR_TILES_FP64(CC_SOR_MATCH64)
     CC_SOR_TILES_MIXED(CC_SOR_MATCH32)
+    CC_SOR_SHEARED_FP64(CC_SOR_SMATCH64)
+    CC_SOR_SHEARED_MIXED(CC_SOR_SMATCH32)
 #undef CC_SOR_MATCH64
 #undef CC_SOR_MATCH32
+#undef CC_SOR_SMATCH64
+#undef CC_SOR_SMATCH32
     return false;
 }
 
 static cc_status launch_tile(const SorTile& t, bool mixed, int nt, hipStream_t s, const SorArgs& args, const int* sched,
                              double* phi, const double* rho, const double* rhobar) {
 #define CC_SOR_LAUNCH64(S_, WJ_, R_)                                                                          \
-    if (!mixed && t.S == S_ && t.WJ == WJ_ && t.rhs_lds == R_) {                                              \
+    if (!t.sheared && !mixed && t.S == S_ && t.WJ == WJ_ && t.rhs_lds == R_) {                                              \
         hipLaunchKernelGGL((k_sor_tile_fp64<S_, WJ_, R_>), dim3(nt), dim3(1024), 0, s, args, sched, phi, rho, \
                            rhobar);                                                                           \
         CC_CHECK_LAUNCH();                                                                                    \
         return CC_OK;                                                                                         \
     }
 #define CC_SOR_LAUNCH32(S_, WJ_)                                                                           \
-    if (mixed && t.S == S_ && t.WJ == WJ_) {                                                               \
+    if (!t.sheared && mixed && t.S == S_ && t.WJ == WJ_) {                                                               \
         hipLaunchKernelGGL((k_sor_tile_mixed<S_, WJ_>), dim3(nt), dim3(1024), 0, s, args, sched, phi, rho, \
                            rhobar);                                                                        \
         CC_CHECK_LAUNCH();                                                                                 \
@@ -580,22 +894,58 @@ static cc_status launch_tile(const SorTile& t, bool mixed, int nt, hipStream_t s
     CC_SOR_TILES_MIXED(CC_SOR_LAUNCH32)
 #undef CC_SOR_LAUNCH64
 #undef CC_SOR_LAUNCH32
+#define CC_SOR_SL64(S_, K_) \
+    if (t.sheared && !mixed && t.S == S_ && t.WJ == K_) { \
+        hipLaunchKernelGGL((k_sor_stile_fp64<S_, K_>), dim3(nt), dim3(1024), 0, s, args, sched, phi, rho, rhobar); \
+        CC_CHECK_LAUNCH(); \
+        return CC_OK; \
+    }
+#define CC_SOR_SL32(S_, K_) \
+    if (t.sheared && mixed && t.S == S_ && t.WJ == K_) { \
+        hipLaunchKernelGGL((k_sor_stile_mixed<S_, K_>), dim3(nt), dim3(1024), 0, s, args, sched, phi, rho, rhobar); \
+        CC_CHECK_LAUNCH(); \
+        return CC_OK; \
+    }
+    CC_SOR_SHEARED_FP64(CC_SOR_SL64)
+    CC_SOR_SHEARED_MIXED(CC_SOR_SL32)
+#undef CC_SOR_SL64
+#undef CC_SOR_SL32
     set_error(CC_ERR_INVALID_ARG, "SOR tile shape not compiled in");
     return CC_ERR_INVALID_ARG;
 }
 
+// Grid of the dataflow launch: `ntiles` tiles, at most CC_SOR_FLOW_BLOCKS blocks (default 512, two per
+// CU); CC_SOR_FLOW_LDS reserves that many more bytes of LDS per block (fewer blocks per CU).
+static int flow_grid(int ntiles) {
+    static const int cap = [] {
+        const char* e = std::getenv("CC_SOR_FLOW_BLOCKS");
+        const int v = e ? std::atoi(e) : 0;
+        return v > 0 ? std::min(v, 65536) : 512;
+    }();
+    return std::min(ntiles, cap);
+}
+static unsigned flow_lds_pad() {
+    static const unsigned pad = [] {
+        const char* e = std::getenv("CC_SOR_FLOW_LDS");
+        const int v = e ? std::atoi(e) : 0;
+        return (unsigned)std::max(0, std::min(v, 65536));
+    }();
+    return pad;
+}
+
 static cc_status launch_flow(const SorTile& t, bool mixed, int nblk, hipStream_t s, const SorArgs& args,
                              const FlowSched& f, double* phi, const double* rho, const double* rhobar) {
+    const unsigned pad = flow_lds_pad();
 #define CC_SOR_FLAUNCH64(S_, WJ_, R_)                                                                         \
-    if (!mixed && t.S == S_ && t.WJ == WJ_ && t.rhs_lds == R_) {                                             \
-        hipLaunchKernelGGL((k_sor_flow_fp64<S_, WJ_, R_>), dim3(nblk), dim3(1024), 0, s, args, f, phi, rho, \
+    if (!t.sheared && !mixed && t.S == S_ && t.WJ == WJ_ && t.rhs_lds == R_) {                                             \
+        hipLaunchKernelGGL((k_sor_flow_fp64<S_, WJ_, R_>), dim3(nblk), dim3(1024), pad, s, args, f, phi, rho, \
                            rhobar);                                                                          \
         CC_CHECK_LAUNCH();                                                                                   \
         return CC_OK;                                                                                        \
     }
 #define CC_SOR_FLAUNCH32(S_, WJ_)                                                                              \
-    if (mixed && t.S == S_ && t.WJ == WJ_) {                                                                   \
-        hipLaunchKernelGGL((k_sor_flow_mixed<S_, WJ_>), dim3(nblk), dim3(1024), 0, s, args, f, phi, rho, rhobar); \
+    if (!t.sheared && mixed && t.S == S_ && t.WJ == WJ_) {                                                                   \
+        hipLaunchKernelGGL((k_sor_flow_mixed<S_, WJ_>), dim3(nblk), dim3(1024), pad, s, args, f, phi, rho, rhobar); \
         CC_CHECK_LAUNCH();                                                                                     \
         return CC_OK;                                                                                          \
     }
@@ -603,6 +953,22 @@ static cc_status launch_flow(const SorTile& t, bool mixed, int nblk, hipStream_t
     CC_SOR_TILES_MIXED(CC_SOR_FLAUNCH32)
 #undef CC_SOR_FLAUNCH64
 #undef CC_SOR_FLAUNCH32
+#define CC_SOR_SL64(S_, K_) \
+    if (t.sheared && !mixed && t.S == S_ && t.WJ == K_) { \
+        hipLaunchKernelGGL((k_sor_sflow_fp64<S_, K_>), dim3(nblk), dim3(1024), pad, s, args, f, phi, rho, rhobar); \
+        CC_CHECK_LAUNCH(); \
+        return CC_OK; \
+    }
+#define CC_SOR_SL32(S_, K_) \
+    if (t.sheared && mixed && t.S == S_ && t.WJ == K_) { \
+        hipLaunchKernelGGL((k_sor_sflow_mixed<S_, K_>), dim3(nblk), dim3(1024), pad, s, args, f, phi, rho, rhobar); \
+        CC_CHECK_LAUNCH(); \
+        return CC_OK; \
+    }
+    CC_SOR_SHEARED_FP64(CC_SOR_SL64)
+    CC_SOR_SHEARED_MIXED(CC_SOR_SL32)
+#undef CC_SOR_SL64
+#undef CC_SOR_SL32
     set_error(CC_ERR_INVALID_ARG, "SOR tile shape not compiled in");
     return CC_ERR_INVALID_ARG;
 }
@@ -660,13 +1026,13 @@ void sor_flow_deps(const std::vector<int>& flat, std::vector<int>& off, std::vec
 cc_status launch_tile_batch(const SorTile& t, bool mixed, int nt, hipStream_t s, const SorBatchEntry* tab,
                             const int* tiles4) {
 #define CC_SOR_BLAUNCH64(S_, WJ_, R_)                                                                         \
-    if (!mixed && t.S == S_ && t.WJ == WJ_ && t.rhs_lds == R_) {                                             \
+    if (!t.sheared && !mixed && t.S == S_ && t.WJ == WJ_ && t.rhs_lds == R_) {                                             \
         hipLaunchKernelGGL((k_sor_tile_fp64_batch<S_, WJ_, R_>), dim3(nt), dim3(1024), 0, s, tab, tiles4); \
         CC_CHECK_LAUNCH();                                                                                   \
         return CC_OK;                                                                                        \
     }
 #define CC_SOR_BLAUNCH32(S_, WJ_)                                                                           \
-    if (mixed && t.S == S_ && t.WJ == WJ_) {                                                                \
+    if (!t.sheared && mixed && t.S == S_ && t.WJ == WJ_) {                                                                \
         hipLaunchKernelGGL((k_sor_tile_mixed_batch<S_, WJ_>), dim3(nt), dim3(1024), 0, s, tab, tiles4);   \
         CC_CHECK_LAUNCH();                                                                                  \
         return CC_OK;                                                                                       \
@@ -675,6 +1041,22 @@ cc_status launch_tile_batch(const SorTile& t, bool mixed, int nt, hipStream_t s,
     CC_SOR_TILES_MIXED(CC_SOR_BLAUNCH32)
 #undef CC_SOR_BLAUNCH64
 #undef CC_SOR_BLAUNCH32
+#define CC_SOR_SL64(S_, K_) \
+    if (t.sheared && !mixed && t.S == S_ && t.WJ == K_) { \
+        hipLaunchKernelGGL((k_sor_stile_fp64_batch<S_, K_>), dim3(nt), dim3(1024), 0, s, tab, tiles4); \
+        CC_CHECK_LAUNCH(); \
+        return CC_OK; \
+    }
+#define CC_SOR_SL32(S_, K_) \
+    if (t.sheared && mixed && t.S == S_ && t.WJ == K_) { \
+        hipLaunchKernelGGL((k_sor_stile_mixed_batch<S_, K_>), dim3(nt), dim3(1024), 0, s, tab, tiles4); \
+        CC_CHECK_LAUNCH(); \
+        return CC_OK; \
+    }
+    CC_SOR_SHEARED_FP64(CC_SOR_SL64)
+    CC_SOR_SHEARED_MIXED(CC_SOR_SL32)
+#undef CC_SOR_SL64
+#undef CC_SOR_SL32
     set_error(CC_ERR_INVALID_ARG, "SOR tile shape not compiled in");
     return CC_ERR_INVALID_ARG;
 }
@@ -710,7 +1092,7 @@ cc_status Poisson2D::init(int Nx_, int Ny_, bool px_, bool py_, double dx_, doub
             int S_ = 0, WJ_ = 0;
             char g = 0;
             const int n = std::sscanf(e, "%d,%d,%c", &S_, &WJ_, &g);
-            if (n >= 2) { t = SorTile{S_, WJ_, !(n == 3 && g == 'g')}; from_env = true; }
+            if (n >= 2) { t = SorTile{S_, WJ_, !(n == 3 && g == 'g'), n == 3 && g == 's'}; from_env = true; }
         }
         if (!from_env && sor_autotune_enabled()) CC_TRY(autotuned_tile(t));
         if (!sor_tile_supported(t, mixed)) {
@@ -790,8 +1172,43 @@ void sor_levels_w(int W, int WJ, int S, int Nx, int Ny, int iters, std::vector<i
     }
 }
 
+// The sheared schedule: (TS, TK, TI) of every sheared tile (S sweeps x W values of i + s x K values
+// of i + j + 2s) that updates at least one cell, grouped by level TS + TK + TI.
+void sor_levels_sheared(int W, int K, int S, int Nx, int Ny, int iters, std::vector<int>& flat,
+                        std::vector<int>& level_off) {
+    const int nTS = (iters + S - 1) / S;
+    const int nTI = (Nx + iters - 1 + W - 1) / W;
+    const int nTK = (Nx + Ny + 2 * iters - 3 + K - 1) / K;
+    std::vector<std::vector<int>> lv(nTS + nTK + nTI);
+    for (int TS = 0; TS < nTS; ++TS) {
+        const int s0 = S * TS, s1 = std::min(s0 + S, iters);
+        for (int TK = 0; TK < nTK; ++TK)
+            for (int TI = 0; TI < nTI; ++TI) {
+                bool nonempty = false;
+                for (int s = s0; s < s1 && !nonempty; ++s) {
+                    const int imin = std::max(0, W * TI - s), imax = std::min(Nx - 1, W * TI + W - 1 - s);
+                    if (imin > imax) continue;
+                    const int jmin = std::max(0, K * TK - imax - 2 * s), jmax = std::min(Ny - 1, K * TK + K - 1 - imin - 2 * s);
+                    nonempty = jmin <= jmax;
+                }
+                if (nonempty) {
+                    auto& v = lv[TS + TK + TI];
+                    v.push_back(TS); v.push_back(TK); v.push_back(TI);
+                }
+            }
+    }
+    flat.clear();
+    level_off.assign(1, 0);
+    for (auto& v : lv) {
+        if (v.empty()) continue;
+        flat.insert(flat.end(), v.begin(), v.end());
+        level_off.push_back((int)flat.size() / 3);
+    }
+}
+
 void sor_levels(const SorTile& t, int Nx, int Ny, int iters, std::vector<int>& flat, std::vector<int>& level_off) {
-    sor_levels_w(kTileW, t.WJ, t.S, Nx, Ny, iters, flat, level_off);
+    if (t.sheared) sor_levels_sheared(kTileW, t.WJ, t.S, Nx, Ny, iters, flat, level_off);
+    else sor_levels_w(kTileW, t.WJ, t.S, Nx, Ny, iters, flat, level_off);
 }
 
 // ------------------------------------------------------------------ tile autotuner
@@ -820,10 +1237,16 @@ cc_status Poisson2D::autotuned_tile(SorTile& out) {
     std::vector<SorTile> cands;
 #define CC_SOR_CAND64(S_, WJ_, R_) if (!mixed) cands.push_back(SorTile{S_, WJ_, R_});
 #define CC_SOR_CAND32(S_, WJ_) if (mixed) cands.push_back(SorTile{S_, WJ_, true});
+#define CC_SOR_SCAND64(S_, K_) if (!mixed) cands.push_back(SorTile{S_, K_, true, true});
+#define CC_SOR_SCAND32(S_, K_) if (mixed) cands.push_back(SorTile{S_, K_, true, true});
     CC_SOR_TILES_FP64(CC_SOR_CAND64)
     CC_SOR_TILES_MIXED(CC_SOR_CAND32)
+    CC_SOR_SHEARED_FP64(CC_SOR_SCAND64)
+    CC_SOR_SHEARED_MIXED(CC_SOR_SCAND32)
 #undef CC_SOR_CAND64
 #undef CC_SOR_CAND32
+#undef CC_SOR_SCAND64
+#undef CC_SOR_SCAND32
     const size_t N = (size_t)Nx * Ny;
     double* rz = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -864,7 +1287,7 @@ cc_status Poisson2D::autotuned_tile(SorTile& out) {
             hipEventRecord(e0, 0);
             if (flow) {
                 hipLaunchKernelGGL(k_fill_int, dim3(std::min(64, (ntiles + 2 + 255) / 256)), dim3(256), 0, 0, d_st, ntiles + 2, 0);
-                st = launch_flow(c, mixed, std::min(ntiles, 512), 0, args, FlowSched{sc, d_off, d_idx, ntiles, d_st}, phi, rz, rhobar);
+                st = launch_flow(c, mixed, flow_grid(ntiles), 0, args, FlowSched{sc, d_off, d_idx, ntiles, d_st}, phi, rz, rhobar);
             } else {
                 for (size_t l = 0; l + 1 < off.size() && st == CC_OK; ++l)
                     st = launch_tile(c, mixed, off[l + 1] - off[l], 0, args, sc + 3 * off[l], phi, rz, rhobar);
@@ -879,6 +1302,9 @@ cc_status Poisson2D::autotuned_tile(SorTile& out) {
         if (d_off) hipFree(d_off);
         if (d_idx) hipFree(d_idx);
         if (d_st) hipFree(d_st);
+        if (st == CC_OK && std::getenv("CC_SOR_AUTOTUNE_LOG"))
+            std::fprintf(stderr, "sor autotune %dx%d %s: S %d, %s %d%s: %.3f ms\n", Nx, Ny, mixed ? "mixed" : "fp64", c.S,
+                         c.sheared ? "K" : "WJ", c.WJ, c.sheared ? " sheared" : (c.rhs_lds ? "" : " rhs-global"), ms);
         if (st == CC_OK && ms < best) { best = ms; bt = c; }
     }
     if (e0) hipEventDestroy(e0);
@@ -932,7 +1358,7 @@ cc_status Poisson2D::sweeps(const double* rho, hipStream_t s, Profiler* prof) {
         CC_CHECK_LAUNCH();
         FlowSched f{sched, dep_off, dep_idx, ntiles, flow_state};
         // two 1024-thread tiles fit a CU: a grid of 2 per CU keeps every block resident
-        CC_TRY(launch_flow(tile_, precision == CC_MIXED, std::min(ntiles, 512), s, args, f, phi, rho, rhobar));
+        CC_TRY(launch_flow(tile_, precision == CC_MIXED, flow_grid(ntiles), s, args, f, phi, rho, rhobar));
     } else if (tiled_) {
         const int nl = levels();
         for (int l = 0; l < nl; ++l) {
@@ -992,7 +1418,7 @@ cc_status cc_sor_autotune(int enable) {
     return CC_OK;
 }
 
-cc_status cc_sor_tile_choice(int Nx, int Ny, int sor_iters, int precision, int* S, int* WJ) {
+cc_status cc_sor_tile_choice(int Nx, int Ny, int sor_iters, int precision, int* S, int* WJ, int* sheared) {
     if (!S || !WJ) { cc::set_error(CC_ERR_INVALID_ARG, "null output"); return CC_ERR_INVALID_ARG; }
     cc::SorTile t = precision == CC_MIXED ? cc::kTileMixedDefault : cc::kTileFp64Default;
     {
@@ -1002,18 +1428,20 @@ cc_status cc_sor_tile_choice(int Nx, int Ny, int sor_iters, int precision, int* 
     }
     *S = t.S;
     *WJ = t.WJ;
+    if (sheared) *sheared = t.sheared ? 1 : 0;
     return CC_OK;
 }
 
-cc_status cc_sor_flow_schedule(int Nx, int Ny, int iters, int W, int WJ, int S, int* ntiles, int* ndeps, int* tiles,
-                               int* dep_off, int* dep_idx) {
+cc_status cc_sor_flow_schedule(int Nx, int Ny, int iters, int W, int WJ, int S, int sheared, int* ntiles, int* ndeps,
+                               int* tiles, int* dep_off, int* dep_idx) {
     if (Nx < 1 || Ny < 1 || iters < 1 || W < 1 || WJ < 1 || S < 1 || !ntiles || !ndeps ||
         (double)(Nx + iters) * (Ny + iters) * iters / ((double)W * WJ * S) > 1.0e7) {
         cc::set_error(CC_ERR_INVALID_ARG, "cc_sor_flow_schedule: bad grid, tile or output");
         return CC_ERR_INVALID_ARG;
     }
     std::vector<int> flat, off, doff, didx;
-    cc::sor_levels_w(W, WJ, S, Nx, Ny, iters, flat, off);
+    if (sheared) cc::sor_levels_sheared(W, WJ, S, Nx, Ny, iters, flat, off);
+    else cc::sor_levels_w(W, WJ, S, Nx, Ny, iters, flat, off);
     cc::sor_flow_deps(flat, doff, didx);
     const int n = (int)flat.size() / 3, nd = (int)didx.size();
     if (tiles && dep_off && dep_idx) {

@@ -28,8 +28,9 @@ namespace cc {
 // Tile shape of the tiled schedule: S sweeps per tile (16 waves x S/16), WJ rows; FP64 keeps the
 // source term in LDS (rhs_lds) or loads it from global memory one step ahead.
 struct SorTile {
-    int S = 32, WJ = 64;
+    int S = 32, WJ = 64;    // sweeps per tile; rows (rectangular) or k values (sheared) per tile
     bool rhs_lds = true;
+    bool sheared = false;   // boxes in (sweep, i + j + 2 sweep, i + sweep) instead of (sweep, j + sweep, i + sweep)
 };
 
 struct SorArgs {
@@ -43,6 +44,9 @@ struct SorArgs {
 // 16 B per box cell, which fits 160 KB of LDS only at WJ = 64.
 #define CC_SOR_TILES_FP64(X) X(16, 64, true) X(32, 64, true) X(32, 64, false) X(16, 128, false) X(32, 128, false)
 #define CC_SOR_TILES_MIXED(X) X(16, 64) X(32, 64) X(16, 128) X(32, 128)
+// The compiled sheared tile shapes (S, K); the source term is in LDS.
+#define CC_SOR_SHEARED_FP64(X) X(16, 64) X(16, 32) X(32, 32)
+#define CC_SOR_SHEARED_MIXED(X) X(16, 64) X(16, 48) X(16, 32) X(16, 128)
 
 // One grid of a batched tiled SOR launch: its arguments and arrays.
 struct SorBatchEntry {
@@ -105,6 +109,7 @@ void sor_flow_deps(const std::vector<int>& flat, std::vector<int>& off, std::vec
 
 // The tile levels of the tiled schedule for a tile shape (flat (TS, TJ, TI) triples, level offsets).
 void sor_levels_w(int W, int WJ, int S, int Nx, int Ny, int iters, std::vector<int>& flat, std::vector<int>& level_off);
+void sor_levels_sheared(int W, int K, int S, int Nx, int Ny, int iters, std::vector<int>& flat, std::vector<int>& level_off);
 void sor_levels(const SorTile& t, int Nx, int Ny, int iters, std::vector<int>& flat, std::vector<int>& level_off);
 bool sor_autotune_enabled();
 

@@ -184,18 +184,21 @@ def sor_autotune(enable=True):
 
 
 def sor_tile_choice(Nx, Ny, sor_iters=600, precision=CC_FP64):
-    """(sweeps per tile, rows per tile) of the SOR schedule a grid gets (cc_sor_tile_choice)."""
-    S, WJ = C.c_int(), C.c_int()
-    check(_lib.lib().cc_sor_tile_choice(int(Nx), int(Ny), int(sor_iters), int(precision), C.byref(S), C.byref(WJ)))
-    return S.value, WJ.value
+    """(sweeps per tile, rows or k values per tile, sheared) of the SOR schedule a grid gets
+    (cc_sor_tile_choice)."""
+    S, WJ, sh = C.c_int(), C.c_int(), C.c_int()
+    check(_lib.lib().cc_sor_tile_choice(int(Nx), int(Ny), int(sor_iters), int(precision), C.byref(S), C.byref(WJ),
+                                        C.byref(sh)))
+    return S.value, WJ.value, bool(sh.value)
 
 
-def sor_flow_schedule(Nx, Ny, iters, W=64, WJ=64, S=16):
+def sor_flow_schedule(Nx, Ny, iters, W=64, WJ=64, S=16, sheared=False):
     """The tiled SOR's dataflow schedule (cc_sor_flow_schedule; host only): a list of tiles (TS, TJ, TI)
-    in launch order and, per tile, the indices of the tiles it waits for."""
+    -- (TS, TK, TI) for sheared tiles -- in launch order and, per tile, the indices of the tiles it
+    waits for."""
     lib = _lib.lib()
     n, nd = C.c_int(), C.c_int()
-    args = [int(v) for v in (Nx, Ny, iters, W, WJ, S)]
+    args = [int(v) for v in (Nx, Ny, iters, W, WJ, S, bool(sheared))]
     check(lib.cc_sor_flow_schedule(*args, C.byref(n), C.byref(nd), None, None, None))
     tiles = np.zeros(3 * n.value, dtype=np.int32)
     off = np.zeros(n.value + 1, dtype=np.int32)
@@ -224,10 +227,11 @@ class Mhd2DBatch:
         return tt
 
     def info(self):
-        """{"merged": SOR tile levels merged across instances, "tile": (S, WJ) of the merged launches}."""
-        m, S, WJ = C.c_int(), C.c_int(), C.c_int()
-        check(_lib.lib().cc_mhd2d_batch_info(self._b, C.byref(m), C.byref(S), C.byref(WJ)))
-        return {"merged": bool(m.value), "tile": (S.value, WJ.value)}
+        """{"merged": SOR tile levels merged across instances, "tile": (S, WJ, sheared) of the merged
+        launches}."""
+        m, S, WJ, sh = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(_lib.lib().cc_mhd2d_batch_info(self._b, C.byref(m), C.byref(S), C.byref(WJ), C.byref(sh)))
+        return {"merged": bool(m.value), "tile": (S.value, WJ.value, bool(sh.value))}
 
     def close(self):
         if self._b:

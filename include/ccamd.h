@@ -137,10 +137,11 @@ typedef struct cc_mhd2d_batch cc_mhd2d_batch;
 cc_status cc_mhd2d_batch_create(cc_mhd2d* const* handles, int n, cc_mhd2d_batch** out);
 cc_status cc_mhd2d_batch_destroy(cc_mhd2d_batch* b);
 /* merged = 1 when the instances' SOR tile levels run as merged launches (every instance with
- * gravity, clamped boundaries and one precision), with the tile shape (S sweeps, WJ rows) the batch
- * uses -- the instances' own, or with cc_sor_autotune the fastest merged shape; merged = 0 when each
- * instance's step is a separate graph branch (or CC_BATCH_SEPARATE_SOR is set). */
-cc_status cc_mhd2d_batch_info(cc_mhd2d_batch* b, int* merged, int* S, int* WJ);
+ * gravity, clamped boundaries and one precision), with the tile shape (S sweeps, WJ rows or, with
+ * *sheared = 1, WJ values of i + j + 2 sweep) the batch uses -- the first instance's, or with
+ * cc_sor_autotune the fastest merged shape; merged = 0 when each instance's step is a separate graph
+ * branch (or CC_BATCH_SEPARATE_SOR is set).  sheared may be null. */
+cc_status cc_mhd2d_batch_info(cc_mhd2d_batch* b, int* merged, int* S, int* WJ, int* sheared);
 /* nsteps steps of every instance (the loop of cc_mhd2d_run: gravity, dt, clamp to t_end[i], Heun).
  * t_inout[i] holds instance i's t on entry and return.  Syncs. */
 cc_status cc_mhd2d_batch_run(cc_mhd2d_batch* b, int nsteps, double* t_inout, const double* t_end);
@@ -222,18 +223,21 @@ cc_status cc_hlld3d_batch(const cc_mhd3d_params* p, int n, const double* WL, con
  * afterwards with clamped boundaries times the compiled SOR tile shapes (sweeps per tile x rows) on
  * its grid once -- one full solve each, HIP events -- and keeps the fastest; the choice is cached per
  * (Nx, Ny, sor_iters, precision) for the process.  The environment variable CC_SOR_TILE overrides it.
- * cc_sor_tile_choice reports the shape a grid gets (the default shape if it was never tuned). */
+ * cc_sor_tile_choice reports the shape a grid gets (the default shape if it was never tuned): S sweeps per
+ * tile, WJ rows (rectangular tile) or WJ values of i + j + 2 sweep (sheared tile, *sheared = 1; sheared
+ * may be null). */
 cc_status cc_sor_autotune(int enable);
-cc_status cc_sor_tile_choice(int Nx, int Ny, int sor_iters, int precision, int* S, int* WJ);
+cc_status cc_sor_tile_choice(int Nx, int Ny, int sor_iters, int precision, int* S, int* WJ, int* sheared);
 
 /* The dataflow schedule of the tiled SOR (no reference counterpart; host only, touches no GPU): for an
- * Nx x Ny clamped grid, `iters` sweeps and tiles of S sweeps x W x WJ cells (W = 64 in the kernels),
+ * Nx x Ny clamped grid, `iters` sweeps and tiles of S sweeps x W x WJ cells (W = 64 in the kernels;
+ * sheared != 0: S sweeps x W values of i + sweep x WJ values of i + j + 2 sweep),
  * the tiles (TS, TJ, TI) that update at least one cell, in launch order (3 ints each), and per tile the
  * tiles it waits for before it starts, as CSR (dep_off: ntiles + 1 offsets into dep_idx).  Call with
  * tiles, dep_off or dep_idx null to get *ntiles and *ndeps; then with arrays of 3 *ntiles, *ntiles + 1
  * and *ndeps ints and *ntiles, *ndeps set to their capacities. */
-cc_status cc_sor_flow_schedule(int Nx, int Ny, int iters, int W, int WJ, int S, int* ntiles, int* ndeps, int* tiles,
-                               int* dep_off, int* dep_idx);
+cc_status cc_sor_flow_schedule(int Nx, int Ny, int iters, int W, int WJ, int S, int sheared, int* ntiles, int* ndeps,
+                               int* tiles, int* dep_off, int* dep_idx);
 
 /* ============================================================================================
  * 2D Euler HLLC + SOR self-gravity (EH).

@@ -223,6 +223,64 @@ def test_mixed_vs_fp64_l1_density():
     assert e <= 1e-4, e
 
 
+SHEARED_SHAPES = [(CC_FP64, 16, 64), (CC_FP64, 16, 32), (CC_FP64, 32, 32), (CC_MIXED, 16, 64), (CC_MIXED, 16, 48),
+                  (CC_MIXED, 16, 32), (CC_MIXED, 16, 128)]
+
+
+@pytest.mark.parametrize("prec,S,K", SHEARED_SHAPES)
+def test_sheared_tiles_match_oracle_and_rectangular_tiles(prec, S, K, monkeypatch):
+    """Sheared SOR tiles (CC_SOR_TILE="S,K,s"): the lexicographic iterate on grids with interior and edge
+    tiles and a partial last sweep group -- FP64 against the oracle at 1e-9 and against the rectangular
+    tiles at 1e-12, mixed at the 1e-5 bar of test_poisson_sor_mixed_close_to_oracle; one launch per level
+    and the dataflow launch give the same bits."""
+    for Nx, Ny, iters in ((300, 170, 600), (130, 67, 37), (64, 64, 600), (200, 90, 45)):
+        rng = np.random.default_rng(Nx + Ny + iters)
+        p = M.Params(Nx=Nx, Ny=Ny, periodic_x=False, periodic_y=False, sor_iters=iters, G=1.0)
+        rho = 1.0 + rng.uniform(0, 1, (Ny, Nx)) + 5.0 * np.exp(-((np.arange(Nx)[None, :] - Nx / 2) ** 2 +
+                                                               (np.arange(Ny)[:, None] - Ny / 3) ** 2) / 20.0)
+        out = {}
+        for form in ("flow", "levels", "rect"):
+            monkeypatch.setenv("CC_SOR_TILE", "16,64" if form == "rect" else "%d,%d,s" % (S, K))
+            if form == "levels":
+                monkeypatch.setenv("CC_SOR_LEVELS", "1")
+            else:
+                monkeypatch.delenv("CC_SOR_LEVELS", raising=False)
+            out[form] = M.poisson_sor(rho, p.dx, p.dy, p.G, iters, p.sor_omega, False, False, precision=prec)
+        ox, oy = _oracle_poisson(p, rho)
+        gx, gy = out["flow"]
+        assert np.array_equal(gx, out["levels"][0]) and np.array_equal(gy, out["levels"][1])
+        tol = 1e-9 if prec == CC_FP64 else 1e-5
+        assert rel_err(gx, ox) < tol and rel_err(gy, oy) < tol, (Nx, Ny, iters, rel_err(gx, ox), rel_err(gy, oy))
+        if prec == CC_FP64:
+            rx, ry = out["rect"]
+            assert rel_err(gx, rx) < 1e-12 and rel_err(gy, ry) < 1e-12, (rel_err(gx, rx), rel_err(gy, ry))
+
+
+@pytest.mark.parametrize("prec", [CC_FP64, CC_MIXED])
+def test_sheared_collapse_steps_match_oracle(prec, monkeypatch):
+    """Collapse steps through the captured step graph with sheared SOR tiles (the dataflow launch)."""
+    monkeypatch.setenv("CC_SOR_TILE", "16,64,s")
+    monkeypatch.delenv("CC_SOR_LEVELS", raising=False)
+    N, nsteps = 96, 3
+    p = ic.collapse2d_params(N, precision=prec)
+    st = ic.collapse2d_ic(p)
+    with M.Mhd2D(p) as h:
+        h.upload(st)
+        t = h.run(nsteps)
+        got = h.download()
+    o = oracle()
+    S_ = m2_arrays(p, st)
+    ot = o.or_m2_run(C.byref(m2_oparams(p)), C.byref(S_.cstate()), C.c_double(p.dx), C.c_double(p.dy), nsteps,
+                     C.c_double(0.0), C.c_double(1e300), None)
+    want = m2_dict(p, S_)
+    if prec == CC_FP64:
+        assert abs(t - ot) < 1e-11 * ot
+        for f in M.FIELDS:
+            assert rel_err(got[f], want[f]) < 1e-9, f
+    else:
+        assert rel_l1(got["rho"], want["rho"]) <= 1e-4
+
+
 @pytest.mark.parametrize("prec", [CC_FP64, CC_MIXED])
 def test_sor_dataflow_equals_level_launches_bitwise(prec, monkeypatch):
     """The default tiled solve (one persistent launch, tiles started as their waits clear) and the
@@ -250,15 +308,21 @@ def test_sor_dataflow_equals_level_launches_bitwise(prec, monkeypatch):
         assert np.array_equal(a[3][f], b[3][f]), f
 
 
-@pytest.mark.parametrize("separate_sor", [False, True])
+@pytest.mark.parametrize("mode", ["merged", "separate", "merged_sheared"])
 @pytest.mark.parametrize("prec", [CC_FP64, CC_MIXED])
-def test_batch_equals_instances_run_alone(prec, separate_sor, monkeypatch):
+def test_batch_equals_instances_run_alone(prec, mode, monkeypatch):
     """cc_mhd2d_batch_run: independent collapse instances of uneven sizes, stepped together in one
     graph with per-instance dt -- their SOR tile levels merged into one launch per level (default) or
     each instance's step a separate branch (CC_BATCH_SEPARATE_SOR) -- give bit for bit the times, dts
     and states of the same instances run one by one, over two calls (the second continuing the
-    first), with one instance stopping at its own t_end inside the run."""
+    first), with one instance stopping at its own t_end inside the run.  merged_sheared: the instances
+    and the merged launches use sheared SOR tiles."""
     from core_collapse_amd.mhd2d import Mhd2D, Mhd2DBatch
+    separate_sor = mode == "separate"
+    if mode == "merged_sheared":
+        monkeypatch.setenv("CC_SOR_TILE", "16,64,s")
+    else:
+        monkeypatch.delenv("CC_SOR_TILE", raising=False)
     if separate_sor:
         monkeypatch.setenv("CC_BATCH_SEPARATE_SOR", "1")
     else:
@@ -286,6 +350,7 @@ def test_batch_equals_instances_run_alone(prec, separate_sor, monkeypatch):
     b = Mhd2DBatch(batch_sims)
     print("batch sizes", [e["N"] for e in sample], b.info(), flush=True)
     assert b.info()["merged"] == (not separate_sor)
+    assert b.info()["tile"][2] == (mode == "merged_sheared")
     t = b.run(12, None, t_end)
     t = b.run(7, t, t_end)
     b.close()
@@ -313,9 +378,12 @@ def test_sor_autotune_picks_a_compiled_shape_and_keeps_parity(prec):
             h.upload(st)
             t = h.run(nsteps)
             got = h.download()
-        S, WJ = M.sor_tile_choice(N, N, p.sor_iters, prec)
-        print("autotuned SOR tile", S, WJ, flush=True)
-        assert (S, WJ) in {(16, 64), (32, 64), (16, 128), (32, 128)}
+        S, WJ, sheared = M.sor_tile_choice(N, N, p.sor_iters, prec)
+        print("autotuned SOR tile", S, WJ, sheared, flush=True)
+        if sheared:
+            assert (S, WJ) in ({(16, 64), (16, 32), (32, 32)} if prec == CC_FP64 else {(16, 64), (16, 48), (16, 32), (16, 128)})
+        else:
+            assert (S, WJ) in {(16, 64), (32, 64), (16, 128), (32, 128)}
     finally:
         M.sor_autotune(False)
     o = oracle()

@@ -368,8 +368,7 @@ def test_row_thread_kernel_equals_lexicographic_bitwise(Nx, Ny, px, py, nsw, PD)
 # come from the library itself (cc_sor_flow_schedule, the host code that builds the device tables).
 
 
-def _footprint(tile, Nx, Ny, nsw, W, WJ, S):
-    cells = tile_cells(tile, Nx, Ny, nsw, W, WJ, S)
+def _footprint(cells, Nx, Ny):
     w = {(i, j) for (_, _, i, j) in cells}
     r = set(w)
     for (i, j) in w:
@@ -381,13 +380,22 @@ FLOW_CASES = [(9, 7, 9, 4, 3, 2), (13, 11, 12, 4, 4, 3), (11, 13, 9, 4, 4, 3), (
               (6, 20, 17, 3, 6, 5), (150, 140, 40, 64, 64, 16), (100, 200, 70, 64, 128, 32)]
 
 
+def _geometry(sheared):
+    """(levels, cells of a tile, the kernel's tile body) of rectangular or sheared tiles."""
+    if sheared:
+        return sheared_levels, sheared_cells, sheared_run_tile
+    return tile_levels, tile_cells, run_tile
+
+
+@pytest.mark.parametrize("sheared", [False, True])
 @pytest.mark.parametrize("Nx,Ny,nsw,W,WJ,S", FLOW_CASES)
-def test_dataflow_waits_order_every_conflicting_pair(Nx, Ny, nsw, W, WJ, S):
+def test_dataflow_waits_order_every_conflicting_pair(Nx, Ny, nsw, W, WJ, S, sheared):
     """Two tiles of which one writes a cell the other reads are never in flight together: the later
     one (in level order) waits for the earlier one, directly or through a chain of waits.  Pairs
     without such a cell commute, so every execution the waits allow gives the level order's result."""
-    tiles, deps = M.sor_flow_schedule(Nx, Ny, nsw, W, WJ, S)
-    assert tiles == [t for lev in tile_levels(Nx, Ny, nsw, W, WJ, S) for t in lev]
+    levels, cells_of, _ = _geometry(sheared)
+    tiles, deps = M.sor_flow_schedule(Nx, Ny, nsw, W, WJ, S, sheared=sheared)
+    assert tiles == [t for lev in levels(Nx, Ny, nsw, W, WJ, S) for t in lev]
     n = len(tiles)
     anc = []
     for k in range(n):
@@ -398,7 +406,7 @@ def test_dataflow_waits_order_every_conflicting_pair(Nx, Ny, nsw, W, WJ, S):
             assert d < k and tiles[d] != tiles[k] and all(tiles[d][c] <= tiles[k][c] for c in range(3))
             a |= {d} | anc[d]
         anc.append(a)
-    fp = [_footprint(t, Nx, Ny, nsw, W, WJ, S) for t in tiles]
+    fp = [_footprint(cells_of(t, Nx, Ny, nsw, W, WJ, S), Nx, Ny) for t in tiles]
     for b in range(n):
         for a in range(b):
             if fp[a][0] & fp[b][1] or fp[b][0] & fp[a][1]:
@@ -409,9 +417,11 @@ def test_dataflow_waits_order_every_conflicting_pair(Nx, Ny, nsw, W, WJ, S):
             assert not any(d in anc[e] for e in deps[k] if e != d)
 
 
+@pytest.mark.parametrize("sheared", [False, True])
 @pytest.mark.parametrize("Nx,Ny,nsw,W,WJ,S", FLOW_CASES[:5])
-def test_dataflow_random_orders_equal_lexicographic_bitwise(Nx, Ny, nsw, W, WJ, S):
-    tiles, deps = M.sor_flow_schedule(Nx, Ny, nsw, W, WJ, S)
+def test_dataflow_random_orders_equal_lexicographic_bitwise(Nx, Ny, nsw, W, WJ, S, sheared):
+    _, _, body = _geometry(sheared)
+    tiles, deps = M.sor_flow_schedule(Nx, Ny, nsw, W, WJ, S, sheared=sheared)
     rng = np.random.default_rng(Nx * 31 + Ny)
     rhs = rng.standard_normal((Ny, Nx))
     idx2, idy2 = float(Nx * Nx), float(Ny * Ny)
@@ -422,7 +432,7 @@ def test_dataflow_random_orders_equal_lexicographic_bitwise(Nx, Ny, nsw, W, WJ, 
         while not all(done):
             ready = [k for k in range(len(tiles)) if not done[k] and all(done[d] for d in deps[k])]
             k = ready[rng.integers(len(ready))]
-            run_tile(phi, rhs, tiles[k], nsw, idx2, idy2, 1.8, W, WJ, S)
+            body(phi, rhs, tiles[k], nsw, idx2, idy2, 1.8, W, WJ, S)
             done[k] = True
         assert np.array_equal(phi, ref)
 
@@ -433,3 +443,137 @@ def test_dataflow_schedule_rejects_bad_arguments():
         M.sor_flow_schedule(0, 8, 10)
     with pytest.raises(CCError):
         M.sor_flow_schedule(8, 8, 10, W=0)
+
+
+# ------------------------------------------------------------------ sheared tiles
+# A cell (i, j) of sweep s sits at k = i + j + 2s on the skewed time axis; every value it reads was
+# written at k - 1 or k - 2.  A sheared tile is a box of S sweeps x W values of i' = i + s x K values
+# of k: at step d (k = K TK + d) lane l of sweep s updates i = W TI + l - s, j = k - W TI - l - s, so
+# every lane works at every step of an interior tile.  Its LDS box is kept in sheared coordinates
+# (x, y) = (i - ib + 1, i + j - ub + 1): E/W neighbours are (x +- 1, y +- 1), N/S are (x, y +- 1).
+
+
+def sheared_levels(Nx, Ny, nsw, W, K, S):
+    """Non-empty sheared tiles (TS, TK, TI) grouped by TS + TK + TI (the host schedule)."""
+    nTS = (nsw + S - 1) // S
+    nTI = (Nx + nsw - 1 + W - 1) // W
+    nTK = (Nx + Ny + 2 * nsw - 3 + K - 1) // K
+    levels = {}
+    for TS in range(nTS):
+        s0, s1 = S * TS, min(S * TS + S, nsw)
+        for TK in range(nTK):
+            for TI in range(nTI):
+                ok = False
+                for s in range(s0, s1):
+                    imin, imax = max(0, W * TI - s), min(Nx - 1, W * TI + W - 1 - s)
+                    if imin > imax:
+                        continue
+                    jmin, jmax = max(0, K * TK - imax - 2 * s), min(Ny - 1, K * TK + K - 1 - imin - 2 * s)
+                    if jmin <= jmax:
+                        ok = True
+                        break
+                if ok:
+                    levels.setdefault(TS + TK + TI, []).append((TS, TK, TI))
+    return [levels[k] for k in sorted(levels)]
+
+
+def sheared_cells(tile, Nx, Ny, nsw, W, K, S):
+    TS, TK, TI = tile
+    s0, s1 = S * TS, min(S * TS + S, nsw)
+    out = []
+    for d in range(K):
+        for s in range(s0, s1):
+            for lane in range(W):
+                i, j = W * TI + lane - s, K * TK + d - W * TI - lane - s
+                if 0 <= i < Nx and 0 <= j < Ny:
+                    out.append((d, s, i, j))
+    return out
+
+
+def sheared_updates(i, j, s0, s1, TI, TK, W, K):
+    """The kernel's write-back predicate: some sweep s in [s0, s1) puts (i, j) inside the tile."""
+    u = i + j
+    lo = max(s0, W * TI - i, (K * TK - u + 1) >> 1)
+    hi = min(s1 - 1, W * TI + W - 1 - i, (K * TK + K - 1 - u) >> 1)
+    return lo <= hi
+
+
+def sheared_run_tile(phi, rhs, tile, nsw, idx2, idy2, omega, W, K, S):
+    """One sheared tile as the kernel runs it: stage the in-domain cells of the box, K steps (all
+    reads of a step before its writes; no cell written at a step is read by another lane at that
+    step), write back the cells the predicate selects."""
+    Ny, Nx = rhs.shape
+    inv = 1.0 / (2.0 * (idx2 + idy2))
+    TS, TK, TI = tile
+    s0, s1 = S * TS, min(S * TS + S, nsw)
+    ib, ub = W * TI - (s0 + S - 1), K * TK - 2 * (s0 + S - 1)
+    BX, BY = W + S + 1, K + 2 * S
+    box = np.full((BY, BX), np.nan)
+    for y in range(BY):
+        for x in range(BX):
+            i, u = ib - 1 + x, ub - 1 + y
+            j = u - i
+            if 0 <= i < Nx and 0 <= j < Ny:
+                box[y, x] = phi[j, i]
+    interior = ib >= 1 and ib + W + S - 2 <= Nx - 2 and \
+        K * TK - (W * TI + W - 1) - (s0 + S - 1) >= 1 and K * TK + K - 1 - W * TI - s0 <= Ny - 2
+    for d in range(K):
+        new, reads, writes = [], set(), set()
+        for s in range(s0, s1):
+            for lane in range(W):
+                i, j = W * TI + lane - s, K * TK + d - W * TI - lane - s
+                if not (0 <= i < Nx and 0 <= j < Ny):
+                    assert not interior
+                    continue
+                x, y = i - ib + 1, i + j - ub + 1
+                assert 1 <= x <= BX - 2 and 1 <= y <= BY - 2
+                e = (y + 1, x + 1) if i < Nx - 1 else (y, x)
+                w = (y - 1, x - 1) if i > 0 else (y, x)
+                n = (y + 1, x) if j < Ny - 1 else (y, x)
+                so = (y - 1, x) if j > 0 else (y, x)
+                if interior:
+                    assert i < Nx - 1 and i > 0 and j < Ny - 1 and j > 0
+                pc = box[y, x]
+                pnew = ((box[e] + box[w]) * idx2 + (box[n] + box[so]) * idy2 - rhs[j, i]) * inv
+                v = pc + omega * (pnew - pc)
+                assert np.isfinite(v)
+                new.append(((y, x), v))
+                reads |= {e, w, n, so} - {(y, x)}  # a clamped neighbour is the cell itself
+                writes.add((y, x))
+        assert len(writes) == len(new) and not (reads & writes), (tile, d)
+        for c, v in new:
+            box[c] = v
+    for y in range(BY):
+        for x in range(BX):
+            i, u = ib - 1 + x, ub - 1 + y
+            j = u - i
+            if 0 <= i < Nx and 0 <= j < Ny:
+                upd = sheared_updates(i, j, s0, s1, TI, TK, W, K)
+                if upd:
+                    phi[j, i] = box[y, x]
+                else:
+                    assert box[y, x] == phi[j, i] or np.isnan(box[y, x])
+
+
+SHEARED_CASES = [(9, 7, 5, 4, 3, 2), (11, 13, 9, 4, 4, 3), (6, 10, 7, 3, 5, 4), (13, 11, 12, 4, 6, 2),
+                 (20, 6, 30, 5, 2, 7), (70, 66, 18, 16, 8, 4)]
+
+
+@pytest.mark.parametrize("Nx,Ny,nsw,W,K,S", SHEARED_CASES)
+def test_sheared_tiles_equal_lexicographic_bitwise(Nx, Ny, nsw, W, K, S):
+    rng = np.random.default_rng(Nx * 100 + Ny + 7)
+    rhs = rng.standard_normal((Ny, Nx))
+    idx2, idy2 = float(Nx * Nx), float(Ny * Ny)
+    ref = lexicographic(rhs, nsw, idx2, idy2, 1.8)
+    count = {}
+    for lev in sheared_levels(Nx, Ny, nsw, W, K, S):
+        for tile in lev:
+            for (_, s, i, j) in sheared_cells(tile, Nx, Ny, nsw, W, K, S):
+                count[(s, i, j)] = count.get((s, i, j), 0) + 1
+    assert len(count) == nsw * Nx * Ny and set(count.values()) == {1}
+    for rev in (False, True):
+        phi = np.zeros_like(rhs)
+        for lev in sheared_levels(Nx, Ny, nsw, W, K, S):
+            for tile in (reversed(lev) if rev else lev):
+                sheared_run_tile(phi, rhs, tile, nsw, idx2, idy2, 1.8, W, K, S)
+        assert np.array_equal(phi, ref)
