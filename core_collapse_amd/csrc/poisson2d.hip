@@ -355,18 +355,46 @@ __device__ __forceinline__ LaneWalk sh_walk(const SorArgs& a, const ShBox& b, in
     return w;
 }
 
+// Lane l receives lane l - 1's value (a wave-wide DPP shift right by one lane; lane 0 gets 0).
+__device__ __forceinline__ float lane_shr1(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ double lane_shr1(double v) {
+    const long long u = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(u & 0xffffffffll), 0x138, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), 0x138, 0xf, 0xf, true);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+
 template <int S, int K, typename T, typename F>
 __device__ __forceinline__ void sh_steps(T* __restrict__ x, const SorArgs& a, const ShBox& b,
                                          const LaneWalk (&w)[S / 16], F&& update) {
     using G = ShGeom<S, K>;
     constexpr int SPW = S / 16, P = G::P, EO = G::P + 1;
     if (b.interior) {
-        for (int d = 0; d < K; ++d) {
+        // Every lane of an active sweep works at every step, so from step 1 on a lane's S neighbour is
+        // its own result of the previous step and its W neighbour lane l - 1's (lane 0 reads it).
+        const int lane = threadIdx.x & 63;
+        T prev[SPW];
+#pragma unroll
+        for (int m = 0; m < SPW; ++m) {
+            prev[m] = T(0);
+            if (0 >= w[m].lo && 0 < w[m].hi) {
+                const int idx = w[m].base;
+                prev[m] = update(idx, x[idx], x[idx + EO], x[idx - EO], x[idx + P], x[idx - P]);
+                x[idx] = prev[m];
+            }
+        }
+        __syncthreads();
+        for (int d = 1; d < K; ++d) {
 #pragma unroll
             for (int m = 0; m < SPW; ++m) {
-                if (d >= w[m].lo && d < w[m].hi) {
+                if (d >= w[m].lo && d < w[m].hi) {  // uniform over the wave in an interior tile
                     const int idx = w[m].base + d * P;
-                    x[idx] = update(idx, x[idx], x[idx + EO], x[idx - EO], x[idx + P], x[idx - P]);
+                    T pW = lane_shr1(prev[m]);
+                    if (lane == 0) pW = x[idx - EO];
+                    prev[m] = update(idx, x[idx], x[idx + EO], pW, x[idx + P], prev[m]);
+                    x[idx] = prev[m];
                 }
             }
             __syncthreads();
