@@ -820,6 +820,11 @@ struct cc_mhd2d_batch {
     int* tiles4 = nullptr;           // (instance, TS, TJ, TI) per tile, grouped by level
     std::vector<int> level_off;      // level offsets, in tiles
     SorBatchEntry* tab[3][2] = {};   // per rotation phase and solve (U^n, U^1): each instance's SOR arguments
+    // The merged solve as one dataflow launch (unless CC_SOR_LEVELS): per tile the tiles it waits for,
+    // as CSR over merged indices, and the launch state (counter, error flag, done flags).
+    bool flow = false;
+    int ntiles = 0;
+    int *dep_off = nullptr, *dep_idx = nullptr, *fstate = nullptr;
 
     // Fork from the root stream to every instance stream, run fn(instance) there, join back.
     template <typename F>
@@ -834,8 +839,38 @@ struct cc_mhd2d_batch {
         return CC_OK;
     }
     cc_status merged_sweeps(int a, int q) {
+        if (flow) return launch_flow_batch(tile, mixed, ntiles, root, tab[a][q], tiles4, dep_off, dep_idx, fstate);
         for (size_t l = 0; l + 1 < level_off.size(); ++l)
             CC_TRY(launch_tile_batch(tile, mixed, level_off[l + 1] - level_off[l], root, tab[a][q], tiles4 + 4 * level_off[l]));
+        return CC_OK;
+    }
+    // Device copies of a merged schedule: tiles, and with waits the CSR arrays and a launch state.
+    struct MergedDev {
+        int *t4 = nullptr, *doff = nullptr, *didx = nullptr, *st = nullptr;
+        void free_all() {
+            for (int* p : {t4, doff, didx, st})
+                if (p) hipFree(p);
+            t4 = doff = didx = st = nullptr;
+        }
+    };
+    static cc_status upload_merged(const std::vector<int>& flat, const std::vector<int>* doff,
+                                   const std::vector<int>* didx, MergedDev& d) {
+        const size_t nt = flat.size() / 4;
+        bool ok = hipMalloc(&d.t4, flat.size() * sizeof(int)) == hipSuccess &&
+                  hipMemcpy(d.t4, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+        if (ok && doff) {
+            ok = hipMalloc(&d.doff, doff->size() * sizeof(int)) == hipSuccess &&
+                 hipMalloc(&d.didx, std::max<size_t>(didx->size(), 1) * sizeof(int)) == hipSuccess &&
+                 hipMalloc(&d.st, (nt + 2) * sizeof(int)) == hipSuccess &&
+                 hipMemcpy(d.doff, doff->data(), doff->size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+                 (didx->empty() || hipMemcpy(d.didx, didx->data(), didx->size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess) &&
+                 hipMemset(d.st, 0, (nt + 2) * sizeof(int)) == hipSuccess;
+        }
+        if (!ok) {
+            d.free_all();
+            set_error(CC_ERR_OOM, "batch schedule allocation failed");
+            return CC_ERR_OOM;
+        }
         return CC_OK;
     }
     // Can the instances share merged SOR launches?  If so build the merged schedule and the tables.
@@ -875,47 +910,77 @@ struct cc_mhd2d_batch {
 #undef CC_BSCAND32
         }
         float best = 3.0e38f;
+        MergedDev keep;
         for (size_t ci = 0; ci < cands.size(); ++ci) {
-            std::vector<int> flat, off;
-            build_merged(cands[ci], flat, off);
-            int* t4 = nullptr;
-            if (hipMalloc(&t4, flat.size() * sizeof(int)) != hipSuccess) { set_error(CC_ERR_OOM, "batch schedule allocation failed"); return CC_ERR_OOM; }
-            if (hipMemcpy(t4, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) { hipFree(t4); set_error(CC_ERR_HIP, "copy failed"); return CC_ERR_HIP; }
-            float ms = 0.f;
-            if (cands.size() > 1) {  // time one merged solve (the first run pays first-launch costs); phi is scratch
+            std::vector<int> flat, off, doff, didx;
+            build_merged(cands[ci], flat, off, &doff, &didx);
+            const int nt = (int)flat.size() / 4;
+            const bool can_flow = sor_flow_enabled(nt);
+            MergedDev d;
+            CC_TRY(upload_merged(flat, can_flow ? &doff : nullptr, can_flow ? &didx : nullptr, d));
+            // with several candidates, each shape is timed in both launch forms: the levels of a merged
+            // schedule are already full, so one launch per level can beat the dataflow launch
+            float ms_form[2] = {0.f, 3.0e38f};  // [0] one launch per level, [1] one dataflow launch
+            for (int fl = 0; fl < (can_flow ? 2 : 1) && cands.size() > 1; ++fl) {
                 hipEvent_t e0, e1;
                 hipEventCreate(&e0);
                 hipEventCreate(&e1);
                 cc_status st = CC_OK;
-                for (int rep = 0; rep < 2 && st == CC_OK; ++rep) {
+                for (int rep = 0; rep < 2 && st == CC_OK; ++rep) {  // the first run pays first-launch costs; phi is scratch
                     hipEventRecord(e0, root);
-                    for (size_t l = 0; l + 1 < off.size() && st == CC_OK; ++l)
-                        st = launch_tile_batch(cands[ci], mixed, off[l + 1] - off[l], root, tab[0][0], t4 + 4 * off[l]);
+                    if (fl) {
+                        st = launch_flow_batch(cands[ci], mixed, nt, root, tab[0][0], d.t4, d.doff, d.didx, d.st);
+                    } else {
+                        for (size_t l = 0; l + 1 < off.size() && st == CC_OK; ++l)
+                            st = launch_tile_batch(cands[ci], mixed, off[l + 1] - off[l], root, tab[0][0], d.t4 + 4 * off[l]);
+                    }
                     hipEventRecord(e1, root);
                     hipEventSynchronize(e1);
-                    hipEventElapsedTime(&ms, e0, e1);
+                    hipEventElapsedTime(&ms_form[fl], e0, e1);
                 }
                 hipEventDestroy(e0);
                 hipEventDestroy(e1);
-                if (st != CC_OK) { hipFree(t4); return st; }
+                if (st == CC_OK && fl) st = sor_flow_error(d.st, root);
+                if (st != CC_OK) { d.free_all(); keep.free_all(); return st; }
+                if (std::getenv("CC_SOR_AUTOTUNE_LOG"))
+                    std::fprintf(stderr, "sor autotune batch of %zu %s: S %d, %s %d%s, %s: %.3f ms\n", h.size(),
+                                 mixed ? "mixed" : "fp64", cands[ci].S, cands[ci].sheared ? "K" : "WJ", cands[ci].WJ,
+                                 cands[ci].sheared ? " sheared" : (cands[ci].rhs_lds ? "" : " rhs-global"),
+                                 fl ? "dataflow" : "levels", ms_form[fl]);
             }
+            // a single candidate (no autotuning) runs as the single-grid solves do: dataflow unless CC_SOR_LEVELS
+            const bool use_flow = cands.size() > 1 ? ms_form[1] < ms_form[0] : can_flow;
+            const float ms = ms_form[use_flow ? 1 : 0];
             if (ci == 0 || ms < best) {
                 best = ms;
-                if (tiles4) hipFree(tiles4);
-                tiles4 = t4;
+                keep.free_all();
+                keep = d;
                 level_off = off;
                 tile = cands[ci];
+                flow = use_flow;
+                ntiles = nt;
             } else {
-                hipFree(t4);
+                d.free_all();
             }
         }
+        tiles4 = keep.t4;
+        dep_off = keep.doff;
+        dep_idx = keep.didx;
+        fstate = keep.st;
         merged = true;
         return CC_OK;
     }
     // The merged schedule for tile shape t: level L holds level L of every instance, (instance, TS, TJ, TI).
-    void build_merged(const SorTile& t, std::vector<int>& flat, std::vector<int>& off) {
-        std::vector<std::vector<int>> fl(h.size()), lo(h.size());
-        for (size_t i = 0; i < h.size(); i++) sor_levels(t, h[i]->pois.Nx, h[i]->pois.Ny, h[i]->pois.iters, fl[i], lo[i]);
+    // The merged schedule of tile shape t: level L of every instance, then level L + 1; with doff/didx also
+    // each tile's waits (the instance's own, sor_flow_deps) as merged indices.
+    void build_merged(const SorTile& t, std::vector<int>& flat, std::vector<int>& off, std::vector<int>* doff = nullptr,
+                      std::vector<int>* didx = nullptr) {
+        std::vector<std::vector<int>> fl(h.size()), lo(h.size()), pos(h.size());
+        for (size_t i = 0; i < h.size(); i++) {
+            sor_levels(t, h[i]->pois.Nx, h[i]->pois.Ny, h[i]->pois.iters, fl[i], lo[i]);
+            pos[i].assign(fl[i].size() / 3, -1);
+        }
+        std::vector<std::pair<int, int>> who;  // (instance, local tile) of each merged tile
         flat.clear();
         off.assign(1, 0);
         for (int L = 0;; ++L) {
@@ -924,6 +989,8 @@ struct cc_mhd2d_batch {
                 if (L + 1 >= (int)lo[i].size()) continue;
                 any = true;
                 for (int k = lo[i][L]; k < lo[i][L + 1]; ++k) {
+                    pos[i][k] = (int)flat.size() / 4;
+                    who.emplace_back((int)i, k);
                     flat.push_back((int)i);
                     flat.push_back(fl[i][3 * k]);
                     flat.push_back(fl[i][3 * k + 1]);
@@ -932,6 +999,17 @@ struct cc_mhd2d_batch {
             }
             if (!any) break;
             off.push_back((int)flat.size() / 4);
+        }
+        if (doff && didx) {
+            std::vector<std::vector<int>> loff(h.size()), lidx(h.size());
+            for (size_t i = 0; i < h.size(); i++) sor_flow_deps(fl[i], loff[i], lidx[i]);
+            doff->assign(1, 0);
+            didx->clear();
+            for (const auto& w : who) {
+                const int i = w.first, k = w.second;
+                for (int q = loff[i][k]; q < loff[i][k + 1]; ++q) didx->push_back(pos[i][lidx[i][q]]);
+                doff->push_back((int)didx->size());
+            }
         }
     }
     // One Heun step of every instance (heun(a, b, c, true) of each) with the SOR sweeps merged:
@@ -980,6 +1058,8 @@ struct cc_mhd2d_batch {
         for (auto& g : graph)
             if (g) hipGraphExecDestroy(g);
         if (tiles4) hipFree(tiles4);
+        for (int* p : {dep_off, dep_idx, fstate})
+            if (p) hipFree(p);
         for (auto& ta : tab)
             for (auto* t : ta)
                 if (t) hipFree(t);
@@ -1060,6 +1140,7 @@ cc_status cc_mhd2d_batch_run(cc_mhd2d_batch* b, int nsteps, double* t_inout, con
     }
     CC_HIP(hipStreamSynchronize(b->root));
     cc_status st = CC_OK;
+    if (b->merged && b->flow && sor_flow_error(b->fstate, b->root) != CC_OK) st = CC_ERR_HIP;
     for (size_t i = 0; i < b->h.size(); i++) {
         cc_mhd2d* h = b->h[i];
         h->cur = cur;

@@ -729,6 +729,53 @@ __global__ __launch_bounds__(1024) void k_sor_sflow_mixed(SorArgs a, FlowSched f
     }
 }
 
+// The dataflow launch over the merged schedule of many grids: (instance, TS, TJ|TK, TI) per tile, the
+// instance's arguments and arrays from the table.
+template <int S, int WJ, bool RHS_LDS>
+__global__ __launch_bounds__(1024) void k_sor_flow_fp64_batch(const SorBatchEntry* __restrict__ tab, FlowSched f) {
+    for (;;) {
+        const int t = flow_next_tile(f);
+        if (t >= f.ntiles) break;
+        const int* tt = f.tiles + 4 * t;
+        const SorBatchEntry e = tab[tt[0]];
+        sor_tile_fp64<S, WJ, RHS_LDS, true>(e.a, tt + 1, e.phi, e.rho, e.rhobar);
+        flow_finish_tile(f, t);
+    }
+}
+template <int S, int WJ>
+__global__ __launch_bounds__(1024) void k_sor_flow_mixed_batch(const SorBatchEntry* __restrict__ tab, FlowSched f) {
+    for (;;) {
+        const int t = flow_next_tile(f);
+        if (t >= f.ntiles) break;
+        const int* tt = f.tiles + 4 * t;
+        const SorBatchEntry e = tab[tt[0]];
+        sor_tile_mixed<S, WJ, true>(e.a, tt + 1, e.phi, e.rho, e.rhobar);
+        flow_finish_tile(f, t);
+    }
+}
+template <int S, int K>
+__global__ __launch_bounds__(1024) void k_sor_sflow_fp64_batch(const SorBatchEntry* __restrict__ tab, FlowSched f) {
+    for (;;) {
+        const int t = flow_next_tile(f);
+        if (t >= f.ntiles) break;
+        const int* tt = f.tiles + 4 * t;
+        const SorBatchEntry e = tab[tt[0]];
+        sor_sheared_fp64<S, K, true>(e.a, tt + 1, e.phi, e.rho, e.rhobar);
+        flow_finish_tile(f, t);
+    }
+}
+template <int S, int K>
+__global__ __launch_bounds__(1024) void k_sor_sflow_mixed_batch(const SorBatchEntry* __restrict__ tab, FlowSched f) {
+    for (;;) {
+        const int t = flow_next_tile(f);
+        if (t >= f.ntiles) break;
+        const int* tt = f.tiles + 4 * t;
+        const SorBatchEntry e = tab[tt[0]];
+        sor_sheared_mixed<S, K, true>(e.a, tt + 1, e.phi, e.rho, e.rhobar);
+        flow_finish_tile(f, t);
+    }
+}
+
 __global__ void k_fill_int(int* __restrict__ x, int n, int v) {
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) x[q] = v;
 }
@@ -1087,6 +1134,62 @@ cc_status launch_tile_batch(const SorTile& t, bool mixed, int nt, hipStream_t s,
 #undef CC_SOR_SL32
     set_error(CC_ERR_INVALID_ARG, "SOR tile shape not compiled in");
     return CC_ERR_INVALID_ARG;
+}
+
+cc_status launch_flow_batch(const SorTile& t, bool mixed, int ntiles, hipStream_t s, const SorBatchEntry* tab,
+                            const int* tiles4, const int* dep_off, const int* dep_idx, int* state) {
+    hipLaunchKernelGGL(k_flow_reset, dim3(std::min(64, (ntiles + 2 + 255) / 256)), dim3(256), 0, s, state, ntiles);
+    CC_CHECK_LAUNCH();
+    const FlowSched f{tiles4, dep_off, dep_idx, ntiles, state};
+    const int nblk = flow_grid(ntiles);
+    const unsigned pad = flow_lds_pad();
+#define CC_SOR_BFL64(S_, WJ_, R_)                                                                          \
+    if (!t.sheared && !mixed && t.S == S_ && t.WJ == WJ_ && t.rhs_lds == R_) {                             \
+        hipLaunchKernelGGL((k_sor_flow_fp64_batch<S_, WJ_, R_>), dim3(nblk), dim3(1024), pad, s, tab, f); \
+        CC_CHECK_LAUNCH();                                                                                \
+        return CC_OK;                                                                                     \
+    }
+#define CC_SOR_BFL32(S_, WJ_)                                                                           \
+    if (!t.sheared && mixed && t.S == S_ && t.WJ == WJ_) {                                              \
+        hipLaunchKernelGGL((k_sor_flow_mixed_batch<S_, WJ_>), dim3(nblk), dim3(1024), pad, s, tab, f); \
+        CC_CHECK_LAUNCH();                                                                             \
+        return CC_OK;                                                                                  \
+    }
+#define CC_SOR_BSFL64(S_, K_)                                                                          \
+    if (t.sheared && !mixed && t.S == S_ && t.WJ == K_) {                                              \
+        hipLaunchKernelGGL((k_sor_sflow_fp64_batch<S_, K_>), dim3(nblk), dim3(1024), pad, s, tab, f); \
+        CC_CHECK_LAUNCH();                                                                            \
+        return CC_OK;                                                                                 \
+    }
+#define CC_SOR_BSFL32(S_, K_)                                                                           \
+    if (t.sheared && mixed && t.S == S_ && t.WJ == K_) {                                                \
+        hipLaunchKernelGGL((k_sor_sflow_mixed_batch<S_, K_>), dim3(nblk), dim3(1024), pad, s, tab, f); \
+        CC_CHECK_LAUNCH();                                                                             \
+        return CC_OK;                                                                                  \
+    }
+    CC_SOR_TILES_FP64(CC_SOR_BFL64)
+    CC_SOR_TILES_MIXED(CC_SOR_BFL32)
+    CC_SOR_SHEARED_FP64(CC_SOR_BSFL64)
+    CC_SOR_SHEARED_MIXED(CC_SOR_BSFL32)
+#undef CC_SOR_BFL64
+#undef CC_SOR_BFL32
+#undef CC_SOR_BSFL64
+#undef CC_SOR_BSFL32
+    set_error(CC_ERR_INVALID_ARG, "SOR tile shape not compiled in");
+    return CC_ERR_INVALID_ARG;
+}
+
+cc_status sor_flow_error(const int* state, hipStream_t s) {
+    int e = 0;
+    CC_HIP(hipMemcpyAsync(&e, state + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    CC_HIP(hipStreamSynchronize(s));
+    if (e != 0) {
+        int z = 0;
+        CC_HIP(hipMemcpy(const_cast<int*>(state) + 1, &z, sizeof(int), hipMemcpyHostToDevice));
+        set_error(CC_ERR_HIP, "SOR dataflow: a tile's dependency wait timed out");
+        return CC_ERR_HIP;
+    }
+    return CC_OK;
 }
 
 // ------------------------------------------------------------------ host side

@@ -106,6 +106,12 @@ cc_status launch_tile_batch(const SorTile& t, bool mixed, int nt, hipStream_t s,
 bool sor_flow_enabled(int ntiles);
 // Dependencies of the dataflow schedule of `flat` (level order): CSR offsets and tile indices.
 void sor_flow_deps(const std::vector<int>& flat, std::vector<int>& off, std::vector<int>& idx);
+// The dataflow launch over a merged schedule (instance, TS, TJ|TK, TI per tile, level order) with its waits
+// (CSR over merged indices) and a state of ntiles + 2 ints, which it clears first.
+cc_status launch_flow_batch(const SorTile& t, bool mixed, int ntiles, hipStream_t s, const SorBatchEntry* tab,
+                            const int* tiles4, const int* dep_off, const int* dep_idx, int* state);
+// Reports (and clears) a timed-out wait recorded in a dataflow state; synchronises s.
+cc_status sor_flow_error(const int* state, hipStream_t s);
 
 // The tile levels of the tiled schedule for a tile shape (flat (TS, TJ, TI) triples, level offsets).
 void sor_levels_w(int W, int WJ, int S, int Nx, int Ny, int iters, std::vector<int>& flat, std::vector<int>& level_off);
