@@ -232,16 +232,76 @@ __device__ __forceinline__ void face_hlld3d(const EOS64& e64, const EOS32& e32, 
     }
 }
 
-// One face of axis AX per thread (x: M3:727-750, y: M3:752-775 via the (u,v)/(Bx,By) swap of
-// M3:577-595, z: M3:777-800 via the (u,v,w)->(w,u,v) map of M3:604-622).  PLM-MC on the four cells
-// f-2 .. f+1 along the axis (boundary-mapped), HLLD, then the five fluxes in reference order and the
-// two face EMFs (Ey_x = F(Bz), Ez_x = -F(By); Ez_y = G(Bx), Ex_y = -G(Bz); Ex_z = H(By), Ey_z = -H(Bx)).
+// PLM-MC on the four cells f-2 .. f+1 of face (i, j, k) of axis AX (boundary-mapped) and the face
+// states in the axis frame of hlld_x (y: the (u,v)/(Bx,By) swap of M3:577-595, z: the (u,v,w)->(w,u,v)
+// map of M3:604-622).
+template <int AX, typename R>
+__device__ __forceinline__ void face_states3d(const Geom3D& g, const PrimArrays<R>& W, double Bn, int i, int j, int k,
+                                              Prim<R>& L, Prim<R>& Rr) {
+    const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+    const int f = AX == 0 ? i : (AX == 1 ? j : k);
+    auto cell = [&](int o) {  // index of the cell o steps from the face along the axis
+        int ii = i, jj = j, kk = k;
+        if (AX == 0) ii = bmap(f + o, Nx, g.px);
+        if (AX == 1) jj = bmap(f + o, Ny, g.py);
+        if (AX == 2) kk = bmap(f + o, Nz, g.pz);
+        return ((size_t)kk * Ny + jj) * Nx + ii;
+    };
+    const Prim<R> w0 = load_prim(W, cell(-2)), w1 = load_prim(W, cell(-1)), w2 = load_prim(W, cell(0)),
+                  w3 = load_prim(W, cell(1));
+    const R frho = (R)g.floor_rho, fP = (R)g.floor_P;
+    Prim<R> a, b, WL, WR;
+    plm_mc<R>(w0, w1, w2, AX, frho, fP, a, WL);  // right state of cell f-1
+    plm_mc<R>(w1, w2, w3, AX, frho, fP, WR, b);  // left state of cell f
+    L = WL;
+    Rr = WR;
+    if (AX == 1) {
+        L = Prim<R>{WL.rho, WL.v, WL.u, WL.w, WL.P, (R)Bn, WL.Bx, WL.Bz};
+        Rr = Prim<R>{WR.rho, WR.v, WR.u, WR.w, WR.P, (R)Bn, WR.Bx, WR.Bz};
+    } else if (AX == 2) {
+        L = Prim<R>{WL.rho, WL.w, WL.u, WL.v, WL.P, (R)Bn, WL.Bx, WL.By};
+        Rr = Prim<R>{WR.rho, WR.w, WR.u, WR.v, WR.P, (R)Bn, WR.Bx, WR.By};
+    }
+}
+
+// The five fluxes in reference order and the two face EMFs of face t (Ey_x = F(Bz), Ez_x = -F(By);
+// Ez_y = G(Bx), Ex_y = -G(Bz); Ex_z = H(By), Ey_z = -H(Bx)).
+template <int AX, typename R>
+__device__ __forceinline__ void face_store3d(FaceFlux<R>& F, EmfArr<R>& ef, size_t t, const R o[7]) {
+    if (AX == 0) {
+        F.f[0][t] = o[0]; F.f[1][t] = o[1]; F.f[2][t] = o[2]; F.f[3][t] = o[3]; F.f[4][t] = o[4];
+        ef.Ey_x[t] = o[6];
+        ef.Ez_x[t] = -o[5];
+    } else if (AX == 1) {
+        F.f[0][t] = o[0]; F.f[1][t] = o[2]; F.f[2][t] = o[1]; F.f[3][t] = o[3]; F.f[4][t] = o[4];
+        ef.Ez_y[t] = o[5];
+        ef.Ex_y[t] = -o[6];
+    } else {
+        F.f[0][t] = o[0]; F.f[1][t] = o[2]; F.f[2][t] = o[3]; F.f[3][t] = o[1]; F.f[4][t] = o[4];
+        ef.Ex_z[t] = o[6];
+        ef.Ey_z[t] = -o[5];
+    }
+}
+
+// Faces whose FP32 HLLD flagged the Alfven degeneracy, per axis, for the FP64 recomputation pass.
+struct DegQueue {
+    int* count;   // [3]
+    int* face[3]; // face indices, capacity = the axis' face count
+};
+
+// One face of axis AX per thread (x: M3:727-750, y: M3:752-775, z: M3:777-800): PLM-MC, HLLD, the
+// fluxes and EMFs.  On x and y faces the mixed form runs the FP32 HLLD only and queues the faces it
+// flags as near the Alfven degeneracy; k_face3d_fp64 then recomputes those in FP64 (the FP64 solver's
+// registers would otherwise halve this kernel's occupancy).  z faces, which the field along z of the
+// turbulence benchmark keeps near the degeneracy, recompute inline (a queue of most faces would cost
+// more than the occupancy gains).
 template <int AX, bool MIXED, typename EOS64, typename EOS32>
 __global__ __launch_bounds__(256) void k_face3d(Geom3D g, EOS64 e64, EOS32 e32,
                                                 PrimArrays<typename std::conditional<MIXED, float, double>::type> W,
                                                 const double* __restrict__ Bface,
                                                 FaceFlux<typename std::conditional<MIXED, float, double>::type> F,
-                                                EmfArr<typename std::conditional<MIXED, float, double>::type> ef) {
+                                                EmfArr<typename std::conditional<MIXED, float, double>::type> ef,
+                                                DegQueue dq) {
     using R = typename std::conditional<MIXED, float, double>::type;
     const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
     const int nx = Nx + (AX == 0), ny = Ny + (AX == 1);
@@ -260,47 +320,58 @@ __global__ __launch_bounds__(256) void k_face3d(Geom3D g, EOS64 e64, EOS32 e32,
         if (i >= Nx || fa > (AX == 1 ? Ny : Nz)) return;
     }
     const size_t t = ((size_t)k * ny + j) * nx + i;
-    const int f = AX == 0 ? i : (AX == 1 ? j : k);
-    auto cell = [&](int o) {  // index of the cell o steps from the face along the axis
-        int ii = i, jj = j, kk = k;
-        if (AX == 0) ii = bmap(f + o, Nx, g.px);
-        if (AX == 1) jj = bmap(f + o, Ny, g.py);
-        if (AX == 2) kk = bmap(f + o, Nz, g.pz);
-        return ((size_t)kk * Ny + jj) * Nx + ii;
-    };
-    const Prim<R> w0 = load_prim(W, cell(-2)), w1 = load_prim(W, cell(-1)), w2 = load_prim(W, cell(0)),
-                  w3 = load_prim(W, cell(1));
-    const R frho = (R)g.floor_rho, fP = (R)g.floor_P;
-    Prim<R> a, b, WL, WR;
-    plm_mc<R>(w0, w1, w2, AX, frho, fP, a, WL);  // right state of cell f-1
-    plm_mc<R>(w1, w2, w3, AX, frho, fP, WR, b);  // left state of cell f
     const double Bn = Bface[t];
-    Prim<R> L = WL, Rr = WR;
-    if (AX == 1) {
-        L = Prim<R>{WL.rho, WL.v, WL.u, WL.w, WL.P, (R)Bn, WL.Bx, WL.Bz};
-        Rr = Prim<R>{WR.rho, WR.v, WR.u, WR.w, WR.P, (R)Bn, WR.Bx, WR.Bz};
-    } else if (AX == 2) {
-        L = Prim<R>{WL.rho, WL.w, WL.u, WL.v, WL.P, (R)Bn, WL.Bx, WL.By};
-        Rr = Prim<R>{WR.rho, WR.w, WR.u, WR.v, WR.P, (R)Bn, WR.Bx, WR.By};
-    }
+    Prim<R> L, Rr;
+    face_states3d<AX, R>(g, W, Bn, i, j, k, L, Rr);
     R o[7];
-    if constexpr (MIXED)
+    if constexpr (MIXED && AX != 2) {
+        bool deg = false;
+        hlld_x<float>(e32, L, Rr, (float)Bn, fabs(Bn) < 1e-12, o, &deg, g.tau);
+        if (deg) dq.face[AX][atomicAdd(&dq.count[AX], 1)] = (int)t;
+    } else if constexpr (MIXED) {  // z faces: often at the degeneracy in the benchmark field, recomputed inline
         hlld_x_mixed32(e32, e64, L, Rr, Bn, o, g.tau);
-    else
-        hlld_x<double>(e64, L, Rr, Bn, fabs(Bn) < 1e-12, o);
-    if (AX == 0) {
-        F.f[0][t] = o[0]; F.f[1][t] = o[1]; F.f[2][t] = o[2]; F.f[3][t] = o[3]; F.f[4][t] = o[4];
-        ef.Ey_x[t] = o[6];
-        ef.Ez_x[t] = -o[5];
-    } else if (AX == 1) {
-        F.f[0][t] = o[0]; F.f[1][t] = o[2]; F.f[2][t] = o[1]; F.f[3][t] = o[3]; F.f[4][t] = o[4];
-        ef.Ez_y[t] = o[5];
-        ef.Ex_y[t] = -o[6];
     } else {
-        F.f[0][t] = o[0]; F.f[1][t] = o[2]; F.f[2][t] = o[3]; F.f[3][t] = o[1]; F.f[4][t] = o[4];
-        ef.Ex_z[t] = o[6];
-        ef.Ey_z[t] = -o[5];
+        hlld_x<double>(e64, L, Rr, Bn, fabs(Bn) < 1e-12, o);
     }
+    face_store3d<AX, R>(F, ef, t, o);
+}
+
+template <int AX, typename EOS64>
+__device__ __forceinline__ void face_fp64_one(const Geom3D& g, const EOS64& e64, const PrimArrays<float>& W,
+                                              const double* __restrict__ Bface, FaceFlux<float>& F, EmfArr<float>& ef,
+                                              int tf) {
+    const int nx = g.Nx + (AX == 0), ny = g.Ny + (AX == 1);
+    const int i = tf % nx, r = tf / nx, j = r % ny, k = r / ny;
+    const size_t t = (size_t)tf;
+    const double Bn = Bface[t];
+    Prim<float> L, Rr;
+    face_states3d<AX, float>(g, W, Bn, i, j, k, L, Rr);
+    const Prim<double> l{L.rho, L.u, L.v, L.w, L.P, L.Bx, L.By, L.Bz};
+    const Prim<double> rr{Rr.rho, Rr.u, Rr.v, Rr.w, Rr.P, Rr.Bx, Rr.By, Rr.Bz};
+    double o64[7];
+    hlld_x<double>(e64, l, rr, Bn, fabs(Bn) < 1e-12, o64);
+    float o[7];
+#pragma unroll
+    for (int c = 0; c < 7; ++c) o[c] = (float)o64[c];
+    face_store3d<AX, float>(F, ef, t, o);
+}
+
+// The FP64 HLLD of the queued faces of all three axes (the mixed path's fallback of hlld_x_mixed32).
+template <typename EOS64>
+__global__ __launch_bounds__(256) void k_face3d_fp64(Geom3D g, EOS64 e64, PrimArrays<float> W, const double* __restrict__ Bx,
+                                                     const double* __restrict__ By, const double* __restrict__ Bz,
+                                                     FaceFlux<float> F0, FaceFlux<float> F1, FaceFlux<float> F2,
+                                                     EmfArr<float> ef, DegQueue dq) {
+    const int n0 = dq.count[0], n1 = dq.count[1], n2 = dq.count[2];
+    for (int q = blockIdx.x * 256 + (int)threadIdx.x; q < n0 + n1 + n2; q += gridDim.x * 256) {
+        if (q < n0) face_fp64_one<0>(g, e64, W, Bx, F0, ef, dq.face[0][q]);
+        else if (q < n0 + n1) face_fp64_one<1>(g, e64, W, By, F1, ef, dq.face[1][q - n0]);
+        else face_fp64_one<2>(g, e64, W, Bz, F2, ef, dq.face[2][q - n0 - n1]);
+    }
+}
+
+__global__ void k_zero_count3(int* __restrict__ c) {
+    if (threadIdx.x < 3) c[threadIdx.x] = 0;
 }
 
 // Flux difference, conserved update and floors with the old faces (M3:802-834), in FP64.
@@ -568,6 +639,7 @@ struct cc_mhd3d {
     void* prim[8] = {};     // primitives, flux precision
     void* flx[3][5] = {};   // face fluxes per axis, flux precision
     void* emf[6] = {};      // face EMFs Ey_x, Ez_x, Ez_y, Ex_y, Ex_z, Ey_z, flux precision
+    DegQueue dq{};          // mixed: faces queued for the FP64 HLLD
     Edges ed{};
     Scalars* sc = nullptr;
     double* dts_dev = nullptr;
@@ -633,6 +705,10 @@ struct cc_mhd3d {
             for (auto& a : flx[ax]) CC_TRY(alloc_bytes(&a, nf[ax] * s));
         const size_t ne[6] = {NFx(), NFx(), NFy(), NFy(), NFz(), NFz()};
         for (int c = 0; c < 6; ++c) CC_TRY(alloc_bytes(&emf[c], ne[c] * s));
+        if (p.precision == CC_MIXED) {
+            CC_TRY(alloc_bytes((void**)&dq.count, 3 * sizeof(int)));
+            for (int ax = 0; ax < 3; ++ax) CC_TRY(alloc_bytes((void**)&dq.face[ax], nf[ax] * sizeof(int)));
+        }
         return CC_OK;
     }
     template <typename R>
@@ -662,24 +738,34 @@ struct cc_mhd3d {
             hipLaunchKernelGGL(k_prim3d<R>, dim3(blocks(N())), dim3(256), 0, stream, g, e, in, W<R>());
             CC_CHECK_LAUNCH();
         }
+        if (MIXED) {
+            hipLaunchKernelGGL(k_zero_count3, dim3(1), dim3(64), 0, stream, dq.count);
+            CC_CHECK_LAUNCH();
+        }
         {
             ProfScope ps(pr(), stream, "face");
             hipLaunchKernelGGL((k_face3d<0, MIXED, EOS64, EOS32>), dim3(blocks(NFx())), dim3(256), 0, stream, g, e64, e32,
-                               W<R>(), in.Bx, FF<R>(0), EF<R>());
+                               W<R>(), in.Bx, FF<R>(0), EF<R>(), dq);
             CC_CHECK_LAUNCH();
         }
         {
             ProfScope ps(pr(), stream, "face");
             hipLaunchKernelGGL((k_face3d<1, MIXED, EOS64, EOS32>), dim3((p.Nx + 63) / 64, p.Nz, (p.Ny + 1 + 3) / 4),
                                dim3(256), 0, stream, g, e64, e32,
-                               W<R>(), in.By, FF<R>(1), EF<R>());
+                               W<R>(), in.By, FF<R>(1), EF<R>(), dq);
             CC_CHECK_LAUNCH();
         }
         {
             ProfScope ps(pr(), stream, "face");
             hipLaunchKernelGGL((k_face3d<2, MIXED, EOS64, EOS32>), dim3((p.Nx + 63) / 64, p.Ny, (p.Nz + 1 + 3) / 4),
                                dim3(256), 0, stream, g, e64, e32,
-                               W<R>(), in.Bz, FF<R>(2), EF<R>());
+                               W<R>(), in.Bz, FF<R>(2), EF<R>(), dq);
+            CC_CHECK_LAUNCH();
+        }
+        if constexpr (MIXED) {
+            ProfScope ps(pr(), stream, "face");
+            hipLaunchKernelGGL((k_face3d_fp64<EOS64>), dim3(2048), dim3(256), 0, stream, g, e64, W<float>(), in.Bx, in.By,
+                               in.Bz, FF<float>(0), FF<float>(1), FF<float>(2), EF<float>(), dq);
             CC_CHECK_LAUNCH();
         }
         {
@@ -765,8 +851,8 @@ cc_status cc_mhd3d_create(const cc_mhd3d_params* p, cc_mhd3d** out) {
     if (!out || !p) { set_error(CC_ERR_INVALID_ARG, "null argument"); return CC_ERR_INVALID_ARG; }
     *out = nullptr;
     if (p->Nx < 2 || p->Ny < 2 || p->Nz < 2 || p->Nx > 4096 || p->Ny > 4096 || p->Nz > 4096 ||
-        (size_t)p->Nx * p->Ny * p->Nz > (size_t)1 << 31) {
-        set_error(CC_ERR_INVALID_ARG, "grid must be within [2, 4096]^3 and at most 2^31 cells");
+        (size_t)(p->Nx + 1) * (p->Ny + 1) * (p->Nz + 1) >= (size_t)1 << 31) {
+        set_error(CC_ERR_INVALID_ARG, "grid must be within [2, 4096]^3 with (Nx+1)(Ny+1)(Nz+1) < 2^31");
         return CC_ERR_INVALID_ARG;
     }
     if (p->precision != CC_FP64 && p->precision != CC_MIXED) { set_error(CC_ERR_INVALID_ARG, "bad precision"); return CC_ERR_INVALID_ARG; }
