@@ -178,8 +178,17 @@ struct EmfArr {
     R *Ey_x, *Ez_x, *Ez_y, *Ex_y, *Ex_z, *Ey_z;
 };
 
-template <typename R>
-__global__ __launch_bounds__(256) void k_prim3d(Geom3D g, Eos3D e, State3D U0, PrimArrays<R> W) {
+// The cell kernels take the EOS kind as a template argument (IDEAL), so the other EOS's code and
+// registers drop out of the instance that runs.
+template <bool IDEAL>
+__device__ __forceinline__ Eos3D eos_of(Eos3D e) {
+    e.ideal = IDEAL ? 1 : 0;
+    return e;
+}
+
+template <typename R, bool IDEAL>
+__global__ __launch_bounds__(256) void k_prim3d(Geom3D g, Eos3D e_in, State3D U0, PrimArrays<R> W) {
+    const Eos3D e = eos_of<IDEAL>(e_in);
     const unsigned qq = blockIdx.x * 256u + threadIdx.x;  // cells < 2^31 (checked at create)
     if (qq >= (unsigned)g.Nx * (unsigned)g.Ny * (unsigned)g.Nz) return;
     const size_t q = qq;
@@ -375,9 +384,10 @@ __global__ void k_zero_count3(int* __restrict__ c) {
 }
 
 // Flux difference, conserved update and floors with the old faces (M3:802-834), in FP64.
-template <typename R>
-__global__ __launch_bounds__(256) void k_update3d(Geom3D g, Eos3D e, State3D U0, FaceFlux<R> FX, FaceFlux<R> FY,
+template <typename R, bool IDEAL>
+__global__ __launch_bounds__(256) void k_update3d(Geom3D g, Eos3D e_in, State3D U0, FaceFlux<R> FX, FaceFlux<R> FY,
                                                   FaceFlux<R> FZ, const Scalars* __restrict__ sc, State3D Us) {
+    const Eos3D e = eos_of<IDEAL>(e_in);
     const unsigned qq = blockIdx.x * 256u + threadIdx.x;  // cells < 2^31 (checked at create)
     if (qq >= (unsigned)g.Nx * (unsigned)g.Ny * (unsigned)g.Nz) return;
     const size_t q = qq;
@@ -475,10 +485,11 @@ __global__ __launch_bounds__(256) void k_ct3d(Geom3D g, Edges ed, State3D B0, St
 }
 
 // ------------------------------------------------------------------ Joule + floors (+ Heun)
-template <bool FINAL>
-__global__ __launch_bounds__(256) void k_joule3d(Geom3D g, Eos3D e, State3D Us, State3D B1,
+template <bool FINAL, bool IDEAL>
+__global__ __launch_bounds__(256) void k_joule3d(Geom3D g, Eos3D e_in, State3D Us, State3D B1,
                                                  const Scalars* __restrict__ sc, State3D Un, State3D Out,
                                                  double* __restrict__ spd) {
+    const Eos3D e = eos_of<IDEAL>(e_in);
     // 64 consecutive i x 4 consecutive k of one j row per block (the z neighbours then share L1)
     const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y, k = blockIdx.z * 4 + (threadIdx.x >> 6);
     const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
@@ -528,7 +539,9 @@ __global__ __launch_bounds__(256) void k_joule3d(Geom3D g, Eos3D e, State3D Us, 
     if (FINAL) block_max_to(speed, spd);
 }
 
-__global__ __launch_bounds__(256) void k_speed_max3d(Geom3D g, Eos3D e, State3D U, double* __restrict__ spd) {
+template <bool IDEAL>
+__global__ __launch_bounds__(256) void k_speed_max3d(Geom3D g, Eos3D e_in, State3D U, double* __restrict__ spd) {
+    const Eos3D e = eos_of<IDEAL>(e_in);
     const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y, k = blockIdx.z * 4 + (threadIdx.x >> 6);
     double speed = 0.0;
     if (i < g.Nx && k < g.Nz) speed = signal_speed3d(e, prim_at(g, e, U, i, j, k));
@@ -735,7 +748,8 @@ struct cc_mhd3d {
         using R = typename std::conditional<MIXED, float, double>::type;
         {
             ProfScope ps(pr(), stream, "prim");
-            hipLaunchKernelGGL(k_prim3d<R>, dim3(blocks(N())), dim3(256), 0, stream, g, e, in, W<R>());
+            if (e.ideal) hipLaunchKernelGGL((k_prim3d<R, true>), dim3(blocks(N())), dim3(256), 0, stream, g, e, in, W<R>());
+            else hipLaunchKernelGGL((k_prim3d<R, false>), dim3(blocks(N())), dim3(256), 0, stream, g, e, in, W<R>());
             CC_CHECK_LAUNCH();
         }
         if (MIXED) {
@@ -770,7 +784,7 @@ struct cc_mhd3d {
         }
         {
             ProfScope ps(pr(), stream, "update");
-            hipLaunchKernelGGL(k_update3d<R>, dim3(blocks(N())), dim3(256), 0, stream, g, e, in, FF<R>(0), FF<R>(1),
+            hipLaunchKernelGGL((e.ideal ? k_update3d<R, true> : k_update3d<R, false>), dim3(blocks(N())), dim3(256), 0, stream, g, e, in, FF<R>(0), FF<R>(1),
                                FF<R>(2), sc, Us);
             CC_CHECK_LAUNCH();
         }
@@ -811,7 +825,7 @@ struct cc_mhd3d {
         }
         {
             ProfScope ps(pr(), stream, "joule");
-            hipLaunchKernelGGL(k_joule3d<FINAL>, cell_grid(), dim3(256), 0, stream, g, e, Us, bdst, sc, hn,
+            hipLaunchKernelGGL((e.ideal ? k_joule3d<FINAL, true> : k_joule3d<FINAL, false>), cell_grid(), dim3(256), 0, stream, g, e, Us, bdst, sc, hn,
                                out, spd);
             CC_CHECK_LAUNCH();
         }
@@ -831,7 +845,7 @@ struct cc_mhd3d {
     }
 
     cc_status speed_max(const State3D& s) {
-        hipLaunchKernelGGL(k_speed_max3d, cell_grid(), dim3(256), 0, stream, g, e, s, spd);
+        hipLaunchKernelGGL((e.ideal ? k_speed_max3d<true> : k_speed_max3d<false>), cell_grid(), dim3(256), 0, stream, g, e, s, spd);
         CC_CHECK_LAUNCH();
         return CC_OK;
     }
