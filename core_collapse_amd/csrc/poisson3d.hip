@@ -23,6 +23,7 @@ template <typename T> __device__ __forceinline__ T cconj(T a) { return T{a.x, -a
 
 struct P3Args {
     int Nx, Ny, Nz, lx, ly, lz;  // sizes and their log2
+    int nh;                      // Nx/2 + 1: the x frequencies kept (the spectrum of real data is Hermitian)
     double idx2, idy2, idz2, c4piG, floor_rho, tol, inv_n;
     int max_iter;
 };
@@ -221,8 +222,11 @@ __global__ __launch_bounds__(1024) void k_p3_xfwd(P3Args a, int lnl, const doubl
         x[m] = T{(R)r, (R)0};
     }
     fft_line(x, j, lt, a.lx, tw, s + ((size_t)l << a.lx), 1);
+    // the line is real, so X(Nx - kx) = conj X(kx): keep kx = 0 .. Nx/2 (elements m < 4, and m = 4 of j = 0)
+    const size_t rowh = line * (size_t)a.nh;
 #pragma unroll
-    for (int m = 0; m < 8; ++m) buf[row + j + (m << lt)] = x[m];
+    for (int m = 0; m < 5; ++m)
+        if (m < 4 || j == 0) buf[rowh + j + (m << lt)] = x[m];
     block_max_any(rmax, part);
 }
 
@@ -258,19 +262,25 @@ __global__ __launch_bounds__(1024) void k_p3_y(P3Args a, int lc, typename Cx<R>:
     if (st->done) return;
     extern __shared__ unsigned char smem_raw[];
     T* s = reinterpret_cast<T*>(smem_raw);
-    const int Nx = a.Nx, lt = a.ly - 3, C = 1 << lc;
+    const int nh = a.nh, lt = a.ly - 3, C = 1 << lc, nbc = (nh + C - 1) >> lc;
     const int c = threadIdx.x & (C - 1), j = threadIdx.x >> lc;
-    const int xb = blockIdx.x & ((Nx >> lc) - 1), k = blockIdx.x >> (a.lx - lc);
-    T* base = buf + (size_t)k * Nx * a.Ny + (xb << lc) + c;
+    const int xb = blockIdx.x % nbc, k = blockIdx.x / nbc;
+    // columns past nh (the last block of a plane) transform a copy of the last column and store nothing:
+    // every thread takes part in the block's barriers
+    const int col = (xb << lc) + c;
+    const bool valid = col < nh;
+    T* base = buf + (size_t)k * nh * a.Ny + (valid ? col : nh - 1);
     T x[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-        const T v = base[(size_t)(j + (m << lt)) * Nx];
+        const T v = base[(size_t)(j + (m << lt)) * nh];
         x[m] = INV ? cconj(v) : v;
     }
     fft_line(x, j, lt, a.ly, tw, s + c, C);
+    if (valid) {
 #pragma unroll
-    for (int m = 0; m < 8; ++m) base[(size_t)(j + (m << lt)) * Nx] = INV ? cconj(x[m]) : x[m];
+        for (int m = 0; m < 8; ++m) base[(size_t)(j + (m << lt)) * nh] = INV ? cconj(x[m]) : x[m];
+    }
 }
 
 // z pass: forward transform, spectral division by the discrete Laplacian (and 1/N), inverse
@@ -283,16 +293,18 @@ __global__ __launch_bounds__(1024) void k_p3_z(P3Args a, int lc, typename Cx<R>:
     if (st->done) return;
     extern __shared__ unsigned char smem_raw[];
     T* s = reinterpret_cast<T*>(smem_raw);
-    const int Nx = a.Nx, lt = a.lz - 3, C = 1 << lc;
+    const int Nx = a.Nx, nh = a.nh, lt = a.lz - 3, C = 1 << lc, nbc = (nh + C - 1) >> lc;
     const int c = threadIdx.x & (C - 1), j = threadIdx.x >> lc;
-    const int xb = blockIdx.x & ((Nx >> lc) - 1), jy = blockIdx.x >> (a.lx - lc);
-    const size_t zs = (size_t)Nx * a.Ny;
-    T* base = buf + (size_t)jy * Nx + (xb << lc) + c;
+    const int xb = blockIdx.x % nbc, jy = blockIdx.x / nbc;
+    const int col = (xb << lc) + c;
+    const bool valid = col < nh;  // as in the y pass
+    const size_t zs = (size_t)nh * a.Ny;
+    T* base = buf + (size_t)jy * nh + (valid ? col : nh - 1);
     T x[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) x[m] = base[(size_t)(j + (m << lt)) * zs];
     fft_line(x, j, lt, a.lz, tw, s + c, C);
-    const int kx = (xb << lc) + c;
+    const int kx = valid ? col : nh - 1;
     const double lxy = lam[kx] + lam[Nx + jy];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
@@ -302,8 +314,10 @@ __global__ __launch_bounds__(1024) void k_p3_z(P3Args a, int lc, typename Cx<R>:
         x[m] = T{x[m].x * f, -(x[m].y * f)};  // scaled and conjugated for the inverse
     }
     fft_line(x, j, lt, a.lz, tw, s + c, C);
+    if (valid) {
 #pragma unroll
-    for (int m = 0; m < 8; ++m) base[(size_t)(j + (m << lt)) * zs] = cconj(x[m]);
+        for (int m = 0; m < 8; ++m) base[(size_t)(j + (m << lt)) * zs] = cconj(x[m]);
+    }
 }
 
 // inverse x pass and the FP64 correction phi += Re
@@ -315,12 +329,15 @@ __global__ __launch_bounds__(1024) void k_p3_xinv(P3Args a, int lnl, const typen
     if (st->done) return;
     extern __shared__ unsigned char smem_raw[];
     T* s = reinterpret_cast<T*>(smem_raw);
-    const int lt = a.lx - 3;
+    const int lt = a.lx - 3, Nx = a.Nx;
     const int j = threadIdx.x & ((1 << lt) - 1), l = threadIdx.x >> lt;
-    const size_t row = (((size_t)blockIdx.x << lnl) + l) * a.Nx;
+    const size_t line = ((size_t)blockIdx.x << lnl) + l, row = line * Nx, rowh = line * (size_t)a.nh;
     T x[8];
 #pragma unroll
-    for (int m = 0; m < 8; ++m) x[m] = cconj(buf[row + j + (m << lt)]);
+    for (int m = 0; m < 8; ++m) {  // kx > Nx/2 from the Hermitian symmetry X(kx) = conj X(Nx - kx)
+        const int p = j + (m << lt);
+        x[m] = p <= (Nx >> 1) ? cconj(buf[rowh + p]) : buf[rowh + (Nx - p)];
+    }
     fft_line(x, j, lt, a.lx, tw, s + ((size_t)l << a.lx), 1);
 #pragma unroll
     for (int m = 0; m < 8; ++m) phi[row + j + (m << lt)] += (double)x[m].x;  // Re(conj(.)) = Re(.)
@@ -373,8 +390,9 @@ Geo geometry(int Nx, int Ny, int Nz, bool mixed) {
     g.thr_y = (1 << g.lc_y) << (ly - 3);
     g.thr_z = (1 << g.lc_z) << (lz - 3);
     g.nblk_x = (Ny * Nz) >> g.lnl_x;
-    g.nblk_y = (Nx >> g.lc_y) * Nz;
-    g.nblk_z = (Nx >> g.lc_z) * Ny;
+    const int nh = Nx / 2 + 1;
+    g.nblk_y = ((nh + (1 << g.lc_y) - 1) >> g.lc_y) * Nz;
+    g.nblk_z = ((nh + (1 << g.lc_z) - 1) >> g.lc_z) * Ny;
     g.lds_x = ((size_t)Nx << g.lnl_x) * tsz;
     g.lds_y = ((size_t)Ny << g.lc_y) * tsz;
     g.lds_z = ((size_t)Nz << g.lc_z) * tsz;
@@ -385,6 +403,7 @@ P3Args make_args(const Poisson3D& P) {
     P3Args a;
     a.Nx = P.Nx; a.Ny = P.Ny; a.Nz = P.Nz;
     a.lx = ilog2(P.Nx); a.ly = ilog2(P.Ny); a.lz = ilog2(P.Nz);
+    a.nh = P.Nx / 2 + 1;
     a.idx2 = 1.0 / (P.dx * P.dx); a.idy2 = 1.0 / (P.dy * P.dy); a.idz2 = 1.0 / (P.dz * P.dz);
     a.c4piG = 4.0 * M_PI * P.G;
     a.floor_rho = P.floor_rho;
@@ -482,7 +501,8 @@ cc_status Poisson3D::init(int Nx_, int Ny_, int Nz_, double dx_, double dy_, dou
     const size_t n = (size_t)Nx * Ny * Nz, tsz = mixed ? 8 : 16;
     const Geo g = geometry(Nx, Ny, Nz, mixed);
     npart = std::max(g.nblk_x, 3 * kStatBlocks);
-    bool ok = hipMalloc(&phi, n * sizeof(double)) == hipSuccess && hipMalloc(&buf, n * tsz) == hipSuccess &&
+    const size_t nspec = (size_t)(Nx / 2 + 1) * Ny * Nz;  // the half spectrum along x
+    bool ok = hipMalloc(&phi, n * sizeof(double)) == hipSuccess && hipMalloc(&buf, nspec * tsz) == hipSuccess &&
               hipMalloc(&part, (size_t)npart * sizeof(double)) == hipSuccess &&
               hipMalloc(&st, sizeof(Poisson3DStatus)) == hipSuccess &&
               hipMalloc(&lam, (size_t)(Nx + Ny + Nz) * sizeof(double)) == hipSuccess;

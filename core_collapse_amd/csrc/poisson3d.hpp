@@ -6,7 +6,9 @@
 //   r = rhs - L_h phi                          FP64, fused into the forward x pass (also max|r|)
 //   c = L_h^-1 r                               FFT in R (FP32 or FP64): forward x, forward y,
 //                                              z forward * 1/(lambda_x + lambda_y + lambda_z) z inverse
-//                                              (one kernel), inverse y, inverse x
+//                                              (one kernel), inverse y, inverse x; r is real, so only
+//                                              kx = 0 .. Nx/2 of the x transform is kept and passed on
+//                                              (X(Nx - kx) = conj X(kx) rebuilds the rest in the inverse)
 //   phi += c                                   FP64, fused into the inverse x pass
 // where lambda_a(k) = -(4/h_a^2) sin^2(pi k/N_a) are the discrete Laplacian's eigenvalues, so one
 // FP64 iteration is an exact solve and an FP32 one gains about seven digits.  Iterations stop
@@ -48,7 +50,7 @@ class Poisson3D {
     int Nx = 0, Ny = 0, Nz = 0, max_iter = 0, precision = 0;
     double dx = 1, dy = 1, dz = 1, G = 1, floor_rho = 0, tol = 1e-10;
     double* phi = nullptr;        // N, FP64, mean zero
-    void* buf = nullptr;          // N complex<R> spectrum
+    void* buf = nullptr;          // (Nx/2+1) Ny Nz complex<R>: the half spectrum along x
     double* part = nullptr;       // block partials
     int npart = 0;
     Poisson3DStatus* st = nullptr;
