@@ -45,8 +45,8 @@ struct SorArgs {
 #define CC_SOR_TILES_FP64(X) X(16, 64, true) X(32, 64, true) X(32, 64, false) X(16, 128, false) X(32, 128, false)
 #define CC_SOR_TILES_MIXED(X) X(16, 64) X(32, 64) X(16, 128) X(32, 128)
 // The compiled sheared tile shapes (S, K); the source term is in LDS.
-#define CC_SOR_SHEARED_FP64(X) X(16, 64) X(16, 32) X(32, 32)
-#define CC_SOR_SHEARED_MIXED(X) X(16, 64) X(16, 48) X(16, 32) X(16, 128) X(32, 64) X(32, 32)
+#define CC_SOR_SHEARED_FP64(X) X(16, 64) X(16, 32) X(32, 32) X(16, 80)
+#define CC_SOR_SHEARED_MIXED(X) X(16, 64) X(16, 48) X(16, 32) X(16, 128) X(32, 64) X(32, 32) X(16, 80)
 
 // One grid of a batched tiled SOR launch: its arguments and arrays.
 struct SorBatchEntry {

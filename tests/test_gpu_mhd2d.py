@@ -223,8 +223,9 @@ def test_mixed_vs_fp64_l1_density():
     assert e <= 1e-4, e
 
 
-SHEARED_SHAPES = [(CC_FP64, 16, 64), (CC_FP64, 16, 32), (CC_FP64, 32, 32), (CC_MIXED, 16, 64), (CC_MIXED, 16, 48),
-                  (CC_MIXED, 16, 32), (CC_MIXED, 16, 128), (CC_MIXED, 32, 64), (CC_MIXED, 32, 32)]
+SHEARED_SHAPES = [(CC_FP64, 16, 64), (CC_FP64, 16, 32), (CC_FP64, 32, 32), (CC_FP64, 16, 80), (CC_MIXED, 16, 64),
+                  (CC_MIXED, 16, 48), (CC_MIXED, 16, 32), (CC_MIXED, 16, 128), (CC_MIXED, 32, 64), (CC_MIXED, 32, 32),
+                  (CC_MIXED, 16, 80)]
 
 
 @pytest.mark.parametrize("prec,S,K", SHEARED_SHAPES)
@@ -381,7 +382,8 @@ def test_sor_autotune_picks_a_compiled_shape_and_keeps_parity(prec):
         S, WJ, sheared = M.sor_tile_choice(N, N, p.sor_iters, prec)
         print("autotuned SOR tile", S, WJ, sheared, flush=True)
         if sheared:
-            assert (S, WJ) in ({(16, 64), (16, 32), (32, 32)} if prec == CC_FP64 else {(16, 64), (16, 48), (16, 32), (16, 128), (32, 64), (32, 32)})
+            assert (S, WJ) in ({(16, 64), (16, 32), (32, 32), (16, 80)} if prec == CC_FP64 else
+                               {(16, 64), (16, 48), (16, 32), (16, 128), (32, 64), (32, 32), (16, 80)})
         else:
             assert (S, WJ) in {(16, 64), (32, 64), (16, 128), (32, 128)}
     finally:
