@@ -493,17 +493,22 @@ __device__ __forceinline__ void sor_sheared_fp64(const SorArgs& a, const int* __
 }
 
 // Mixed precision on a sheared tile: the FP32 correction form of sor_tile_mixed.  phi_hi is staged in
-// FP64 (each cell loaded once) into LDS whose first half then holds the FP32 correction; the FP32
-// residual c, formed from the staged box, has its own LDS array.
+// FP64 (each cell loaded once); the FP32 residual c, formed from the staged box, has its own LDS array,
+// and so does the FP32 correction where that costs no occupancy (16 B per box cell, one tile per CU
+// either way), else it reuses the first half of the phi_hi box once c is formed (one more pass and
+// barrier).
 template <int S, int K, bool COH = false>
 __device__ __forceinline__ void sor_sheared_mixed(const SorArgs& a, const int* __restrict__ tile,
                                                   double* __restrict__ phi, const double* __restrict__ rho,
                                                   const double* __restrict__ rhobar) {
     using G = ShGeom<S, K>;
     constexpr int SPW = S / 16, P = G::P, IT = ShSlots<S, K>::IT, EO = G::P + 1, CH = 4;
+    // separate only when the shared layout already allows just one tile per CU
+    constexpr bool SEP = (size_t)G::BY * P * 12 > 80 * 1024 && (size_t)G::BY * P * 16 <= 150 * 1024;
     __shared__ double sbuf[G::BY * P];
     __shared__ float sc[G::BY * P];
-    float* sd = reinterpret_cast<float*>(sbuf);
+    __shared__ float sdsep[SEP ? G::BY * P : 1];
+    float* sd = SEP ? sdsep : reinterpret_cast<float*>(sbuf);
     const ShBox b = sh_box<S, K>(a, tile);
     const double rb = rhobar[0];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -547,16 +552,19 @@ __device__ __forceinline__ void sor_sheared_mixed(const SorArgs& a, const int* _
                     cv = (float)(((pE + pW) * a.idx2 + (pN + pS) * a.idy2 - rhs) * a.inv_denom - pc);
                 }
                 sc[l] = cv;
+                if (SEP) sd[l] = 0.f;
             }
         }
     }
-    __syncthreads();  // phi_hi is read; its first half becomes the correction
+    if (!SEP) {
+        __syncthreads();  // phi_hi is read; its first half becomes the correction
 #pragma unroll 1
-    for (int it = 0; it < IT; ++it) {
-        size_t g;
-        int i, j;
-        const int l = sh_slot<S, K>(a, b, tid + 1024 * it, g, i, j);
-        if (l >= 0) sd[l] = 0.f;
+        for (int it = 0; it < IT; ++it) {
+            size_t g;
+            int i, j;
+            const int l = sh_slot<S, K>(a, b, tid + 1024 * it, g, i, j);
+            if (l >= 0) sd[l] = 0.f;
+        }
     }
     LaneWalk w[SPW];
 #pragma unroll
