@@ -448,7 +448,9 @@ __device__ __forceinline__ void sor_sheared_fp64(const SorArgs& a, const int* __
                                                  double* __restrict__ phi, const double* __restrict__ rho,
                                                  const double* __restrict__ rhobar) {
     using G = ShGeom<S, K>;
-    constexpr int SPW = S / 16, P = G::P, IT = ShSlots<S, K>::IT, CH = 4;
+    // one tile per CU (the box and source term fill the LDS), so 128 VGPRs per lane: a thread's slots
+    // are loaded in two rounds
+    constexpr int SPW = S / 16, P = G::P, IT = ShSlots<S, K>::IT, CH = (IT + 1) / 2;
     __shared__ double sphi[G::BY * P];
     __shared__ double srhs[G::BY * P];
     const ShBox b = sh_box<S, K>(a, tile);
@@ -502,9 +504,12 @@ __device__ __forceinline__ void sor_sheared_mixed(const SorArgs& a, const int* _
                                                   double* __restrict__ phi, const double* __restrict__ rho,
                                                   const double* __restrict__ rhobar) {
     using G = ShGeom<S, K>;
-    constexpr int SPW = S / 16, P = G::P, IT = ShSlots<S, K>::IT, EO = G::P + 1, CH = 4;
+    constexpr int SPW = S / 16, P = G::P, IT = ShSlots<S, K>::IT, EO = G::P + 1;
     // separate only when the shared layout already allows just one tile per CU
     constexpr bool SEP = (size_t)G::BY * P * 12 > 80 * 1024 && (size_t)G::BY * P * 16 <= 150 * 1024;
+    // slots in flight per thread: half of them at one tile per CU (128 VGPRs per lane), else 4
+    constexpr bool ONE = (size_t)G::BY * P * 12 > 80 * 1024;
+    constexpr int CH = ONE ? (IT + 1) / 2 : 4;
     __shared__ double sbuf[G::BY * P];
     __shared__ float sc[G::BY * P];
     __shared__ float sdsep[SEP ? G::BY * P : 1];
