@@ -580,14 +580,18 @@ __device__ __forceinline__ void sor_sheared_mixed(const SorArgs& a, const int* _
         const float gs = sc[idx] + (dE + dW) * kx + (dN + dS) * ky;
         return dc + om * (gs - dc);
     });
-    // phi += correction, as a no-return atomic add (the tile is the only writer of these cells while it
-    // runs, so the sum is the plain FP64 sum): no phi_hi reload, nothing to wait for before the next tile
+    // phi_hi + correction: from the staged phi_hi where it is still in LDS (separate correction array),
+    // else as a no-return atomic add (the tile is the only writer of these cells while it runs, so the
+    // sum is the plain FP64 sum)
 #pragma unroll 1
     for (int it = 0; it < IT; ++it) {
         size_t g;
         int i, j;
         const int l = sh_slot<S, K>(a, b, tid + 1024 * it, g, i, j);
-        if (l >= 0 && sh_updates<K>(i, j, b)) phi_add<COH>(&phi[g], (double)sd[l]);
+        if (l >= 0 && sh_updates<K>(i, j, b)) {
+            if (SEP) phi_st<COH>(&phi[g], sbuf[l] + (double)sd[l]);
+            else phi_add<COH>(&phi[g], (double)sd[l]);
+        }
     }
 }
 
