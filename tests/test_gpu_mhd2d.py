@@ -322,6 +322,8 @@ def test_batch_equals_instances_run_alone(prec, mode, monkeypatch):
     separate_sor = mode == "separate"
     if mode == "merged_sheared":
         monkeypatch.setenv("CC_SOR_TILE", "16,64,s")
+    elif mode == "merged":
+        monkeypatch.setenv("CC_SOR_TILE", "16,64")  # rectangular tiles
     else:
         monkeypatch.delenv("CC_SOR_TILE", raising=False)
     if separate_sor:
@@ -351,7 +353,7 @@ def test_batch_equals_instances_run_alone(prec, mode, monkeypatch):
     b = Mhd2DBatch(batch_sims)
     print("batch sizes", [e["N"] for e in sample], b.info(), flush=True)
     assert b.info()["merged"] == (not separate_sor)
-    assert b.info()["tile"][2] == (mode == "merged_sheared")
+    assert b.info()["tile"][2] == (mode != "merged") or separate_sor
     t = b.run(12, None, t_end)
     t = b.run(7, t, t_end)
     b.close()
