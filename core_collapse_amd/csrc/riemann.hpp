@@ -153,7 +153,8 @@ __host__ __device__ __forceinline__ void flux_ideal_x(const Prim<R>& W, R E, R F
 // bn_small carries that fallback test evaluated on the FP64 face field.
 // If `degenerate` is non-null, it is set when either side is within a relative margin `tau` of the
 // Alfven degeneracy dL = 0 / dR = 0 (M2:419-442), where the star states are ill-conditioned and a
-// low-precision evaluation cannot be trusted; the caller then recomputes the face in FP64.
+// low-precision evaluation cannot be trusted; the caller then recomputes the face in FP64 (the outputs
+// of a degenerate face are zeros: the solver stops there).
 template <typename R, typename EOS>
 __host__ __device__ __forceinline__ void hlld_x(const EOS& eos, Prim<R> WL, Prim<R> WR, R Bn, bool bn_small, R out[7],
                                                 bool* degenerate = nullptr, R tau = R(0)) {
@@ -205,6 +206,11 @@ __host__ __device__ __forceinline__ void hlld_x(const EOS& eos, Prim<R> WL, Prim
     if (degenerate) {
         const R tL = fabs(rhoL * sL * (SL - SM)) + Bn2, tR = fabs(rhoR * sR * (SR - SM)) + Bn2;
         *degenerate = fabs(dL) < tau * tL || fabs(dR) < tau * tR || fabs(denom) < tau * (fabs(rhoL * (SL - uL)) + fabs(rhoR * (SR - uR)));
+        if (*degenerate) {  // the caller recomputes this face in FP64: the rest would be discarded
+#pragma unroll
+            for (int k = 0; k < 7; k++) out[k] = R(0);
+            return;
+        }
     }
 
     R ByL_s = WL.By, BzL_s = WL.Bz, ByR_s = WR.By, BzR_s = WR.Bz;
