@@ -11,11 +11,11 @@ A zone-update is one cell advanced one full Heun step including its two Poisson 
 `value` is the mixed-precision zone-updates/s with all state resident in HBM; the FP64 figure,
 the ratio, the mixed-vs-FP64 relative L1 density error and max|div B| ride along.
 
-roofline: dominant kernel = the tiled SOR kernel (k_sor_flow_*: one dataflow launch per 600-sweep
-solve; k_sor_tile_*, one launch per tile level, when CC_SOR_LEVELS is set), timed with HIP events
-on the handle's stream around every launch inside the timed region.  Algorithmic bytes per launch =
-(cell-sweeps in the launch) x 24 B (FP64: read phi, read rhs, write phi) or x 12 B (mixed: FP32
-phi and rhs) -- the per-sweep figure of the survey's B_alg.  The kernel keeps its tile in LDS for
+roofline: dominant kernel = the tiled SOR kernel (k_sor_sflow_* / k_sor_flow_*: one dataflow launch
+per 600-sweep solve on sheared or rectangular tiles; k_sor_(s)tile_*, one launch per tile level, when
+CC_SOR_LEVELS is set), timed with HIP events on the handle's stream around every launch inside the
+timed region.  Algorithmic bytes per launch = (cell-sweeps in the launch) x 24 B (FP64: read phi,
+read rhs, write phi) or x 12 B (mixed: FP32 phi and rhs), the compulsory traffic of one sweep.  The kernel keeps its tile in LDS for
 16 sweeps, so its algorithmic rate can exceed the HBM peak; `traffic` (measured HBM bytes per
 launch, from rocprofv3 FETCH_SIZE/WRITE_SIZE in profiles/hbm_traffic.json) shows the real HBM load.
 
@@ -426,8 +426,8 @@ def load_traffic(prefix):
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py: this build runs on one GPU only (the solver step is single-GPU; see DESIGN.md "
-              "for how it would shard); got --gpus %d" % a.gpus, file=sys.stderr)
+        print("bench.py: runs on one GPU only (the solver step is built and measured on one GPU; sharding "
+              "across GPUs is not built); got --gpus %d" % a.gpus, file=sys.stderr)
         sys.exit(2)
     from core_collapse_amd import CC_FP64, CC_MIXED
     if a.copy_calibration:
