@@ -225,6 +225,7 @@ def bench3d(N, warmup, steps, with_cpu):
                   "update": ["cc::k_update3d<%s>" % R], "edge": ["cc::k_edge3d<%s>" % R], "ct": ["cc::k_ct3d"],
                   "joule": ["cc::k_joule3d<true>", "cc::k_joule3d<false>"]}
         out.setdefault("hbm_measured", {})[tag] = measured_hbm(res[prec]["prof"], groups)
+        out.setdefault("fusion_factor", {})[tag] = fusion_factor(out["hbm_measured"][tag], zu, 320)
     if with_cpu:
         out["cpu_baseline"] = cpu_baseline3d()
     return out
@@ -380,6 +381,8 @@ def bench_sphere3d(N, warmup, steps, with_cpu):
         groups = {"prim": ["cc::k_e3_prim<%s>" % R], "face": tuple("cc::k_e3_face<%s, %d>" % (R, a) for a in range(3)),
                   "update": ["cc::k_e3_update<1, %s>" % R, "cc::k_e3_update<2, %s>" % R], "grad": ["cc::k_p3_grad"]}
         out.setdefault("hbm_measured", {})[tag] = measured_hbm(res[prec]["prof"], groups)
+        # hydro kernels only: B_alg's hydro share is 5 state passes of 40 B plus g, 224 B per zone-update
+        out.setdefault("fusion_factor_hydro", {})[tag] = fusion_factor(out["hbm_measured"][tag], zu, 224)
     if with_cpu:
         out["cpu_baseline"] = cpu_baseline_sphere()
     return out
@@ -404,8 +407,15 @@ def measured_hbm(prof, groups):
         tb += b * n
         tt += ms * 1e-3
     if tt > 0:
-        out["all"] = {"GBps": round(tb / tt / 1e9, 1), "frac": round(tb / tt / 1e9 / HBM_PEAK_GBS, 4)}
+        out["all"] = {"GBps": round(tb / tt / 1e9, 1), "frac": round(tb / tt / 1e9 / HBM_PEAK_GBS, 4),
+                      "bytes_total": tb}
     return out
+
+
+def fusion_factor(hbm, zone_updates, balg):
+    """Measured HBM bytes per zone-update of the listed kernels over the algorithmic bytes B_alg."""
+    a = hbm.get("all")
+    return round(a["bytes_total"] / zone_updates / balg, 3) if a and zone_updates > 0 else None
 
 
 def load_traffic(prefix):
