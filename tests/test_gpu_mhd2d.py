@@ -81,7 +81,9 @@ def _oracle_poisson(p, rho):
 
 
 @pytest.mark.parametrize("Nx,Ny,periodic,iters", [(32, 32, False, 600), (72, 40, False, 600), (130, 67, False, 37),
-                                                   (24, 24, True, 200), (20, 28, True, 600)])
+                                                   (24, 24, True, 200), (20, 28, True, 600), (2, 2, False, 7),
+                                                   (3, 2, False, 40), (2, 97, False, 33), (97, 2, False, 33),
+                                                   (5, 17, False, 600), (65, 3, False, 1), (129, 130, False, 17)])
 def test_poisson_sor_fp64_matches_oracle(Nx, Ny, periodic, iters):
     rng = np.random.default_rng(Nx * 1000 + Ny)
     p = M.Params(Nx=Nx, Ny=Ny, periodic_x=periodic, periodic_y=periodic, sor_iters=iters, G=1.0)
