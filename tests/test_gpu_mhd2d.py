@@ -112,12 +112,12 @@ def test_poisson_sor_periodic_schedules_match_oracle(Nx, Ny, px, py, iters):
     assert rel_err(gx, ox) < 1e-9 and rel_err(gy, oy) < 1e-9, (rel_err(gx, ox), rel_err(gy, oy))
 
 
-def test_poisson_sor_mixed_close_to_oracle():
+@pytest.mark.parametrize("Nx,Ny,iters", [(96, 80, 600), (2, 2, 7), (5, 17, 600), (97, 2, 33), (130, 67, 37)])
+def test_poisson_sor_mixed_close_to_oracle(Nx, Ny, iters):
     rng = np.random.default_rng(5)
-    Nx, Ny = 96, 80
-    p = M.Params(Nx=Nx, Ny=Ny, periodic_x=False, periodic_y=False, sor_iters=600, G=1.0)
+    p = M.Params(Nx=Nx, Ny=Ny, periodic_x=False, periodic_y=False, sor_iters=iters, G=1.0)
     rho = 1.0 + rng.uniform(0, 1, (Ny, Nx))
-    gx, gy = M.poisson_sor(rho, p.dx, p.dy, p.G, 600, 1.8, False, False, precision=CC_MIXED)
+    gx, gy = M.poisson_sor(rho, p.dx, p.dy, p.G, iters, 1.8, False, False, precision=CC_MIXED)
     ox, oy = _oracle_poisson(p, rho)
     assert rel_err(gx, ox) < 1e-5 and rel_err(gy, oy) < 1e-5, (rel_err(gx, ox), rel_err(gy, oy))
 
