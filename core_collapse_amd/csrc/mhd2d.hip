@@ -1147,7 +1147,8 @@ cc_status cc_mhd2d_batch_run(cc_mhd2d_batch* b, int nsteps, double* t_inout, con
         Scalars sc;
         CC_HIP(hipMemcpy(&sc, h->sc, sizeof(Scalars), hipMemcpyDeviceToHost));
         t_inout[i] = sc.t;
-        if (h->pois.flow_error(h->stream) != CC_OK) st = CC_ERR_HIP;
+        // the instances' own dataflow SOR runs only when the batch does not merge their SOR
+        if (!b->merged && h->pois.flow_error(h->stream) != CC_OK) st = CC_ERR_HIP;
         if (!std::isfinite(sc.dt) || sc.dt < 0.0) {
             set_error(CC_ERR_BAD_DT, "non-finite or negative dt on device in batch instance " + std::to_string(i));
             st = CC_ERR_BAD_DT;
