@@ -658,17 +658,25 @@ __global__ __launch_bounds__(1024) void k_sor_tile_mixed_batch(const SorBatchEnt
 // atomic counter and start one as soon as the tiles it depends on have finished (per tile, the
 // nearest non-empty tiles below it along TS, TJ and TI, through empty positions), instead of
 // waiting for the whole previous level.  Any grid size is safe: a block holds a tile only while it
-// runs, and the lowest unfinished tile's dependencies are always finished.  A wait is bounded:
-// past kFlowSpin polls the block records an error and goes on (results are then wrong, the host
-// reports it), so a fault in the schedule cannot hang the device.
-constexpr int kFlowSpin = 1 << 16;
-
+// runs, and the lowest unfinished tile's dependencies are always finished.  A wait is bounded in
+// time (wait_ticks of the device's constant-rate wall clock, set by the host to 100 ms, far beyond
+// any legitimate wait): past it the block records an error and goes on (results are then wrong, the
+// host reports it), so a fault in the schedule cannot hang the device.
+//
+// Hand-off: a finished tile's phi stores (system-scope atomics, written through to memory) are
+// drained by every wave (s_waitcnt 0) before the workgroup barrier; thread 0 then publishes the done
+// flag with an agent-scope release store.  A waiting block polls the flags with relaxed loads and,
+// once all are set, issues an agent-scope acquire fence before the barrier that releases the tile's
+// phi loads, so the loads are ordered after the producer's stores by the HIP memory model (release
+// store -> acquire fence synchronisation) and not only by the coherence of the cache-bypassing
+// accesses.
 struct FlowSched {
     const int* tiles;     // (TS, TJ, TI) per tile, level order
     const int* dep_off;   // CSR: dependencies of tile t are dep_idx[dep_off[t] .. dep_off[t + 1])
     const int* dep_idx;
     int ntiles;
     int* state;           // [0] tile counter, [1] error flag, [2 ..] done flag per tile
+    unsigned long long wait_ticks;  // bound of one wait, in wall-clock ticks
 };
 
 __device__ __forceinline__ int flow_next_tile(const FlowSched& f) {
@@ -679,18 +687,21 @@ __device__ __forceinline__ int flow_next_tile(const FlowSched& f) {
     const int t = s_tile;
     if (t < f.ntiles) {
         const int d0 = f.dep_off[t], d1 = f.dep_off[t + 1];
-        for (int d = d0 + (int)threadIdx.x; d < d1; d += (int)blockDim.x) {
-            int* flag = &f.state[2 + f.dep_idx[d]];
-            int spins = 0;
-            while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) == 0) {
-                if (++spins > kFlowSpin) {
-                    __hip_atomic_store(&f.state[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
+        if (d0 + (int)threadIdx.x < d1) {  // one waiting thread per dependency
+            const unsigned long long t0 = wall_clock64();
+            for (int d = d0 + (int)threadIdx.x; d < d1; d += (int)blockDim.x) {
+                const int* flag = &f.state[2 + f.dep_idx[d]];
+                while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+                    if (wall_clock64() - t0 > f.wait_ticks) {
+                        __hip_atomic_store(&f.state[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
                 }
-                __builtin_amdgcn_s_sleep(2);
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
-        __syncthreads();  // the tile's phi loads (system-coherent) follow the observed flags
+        __syncthreads();  // the tile's phi loads follow the acquired flags
     }
     return t;
 }
@@ -698,7 +709,7 @@ __device__ __forceinline__ int flow_next_tile(const FlowSched& f) {
 __device__ __forceinline__ void flow_finish_tile(const FlowSched& f, int t) {
     __builtin_amdgcn_s_waitcnt(0);  // this wave's write-through phi stores have completed
     __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(&f.state[2 + t], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (threadIdx.x == 0) __hip_atomic_store(&f.state[2 + t], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <int S, int WJ, bool RHS_LDS>
@@ -1009,6 +1020,16 @@ static cc_status launch_tile(const SorTile& t, bool mixed, int nt, hipStream_t s
 
 // Grid of the dataflow launch: `ntiles` tiles, at most CC_SOR_FLOW_BLOCKS blocks (default 512, two per
 // CU); CC_SOR_FLOW_LDS reserves that many more bytes of LDS per block (fewer blocks per CU).
+// 100 ms of the device's constant-rate wall clock (hipDeviceAttributeWallClockRate, kHz), the bound
+// of one dataflow wait.
+static unsigned long long sor_flow_wait_ticks() {
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
+        khz <= 0)
+        khz = 100000;  // the 100 MHz reference clock of CDNA devices
+    return (unsigned long long)khz * 100ull;
+}
+
 static int flow_grid(int ntiles) {
     static const int cap = [] {
         const char* e = std::getenv("CC_SOR_FLOW_BLOCKS");
@@ -1158,7 +1179,7 @@ cc_status launch_flow_batch(const SorTile& t, bool mixed, int ntiles, hipStream_
                             const int* tiles4, const int* dep_off, const int* dep_idx, int* state) {
     hipLaunchKernelGGL(k_flow_reset, dim3(std::min(64, (ntiles + 2 + 255) / 256)), dim3(256), 0, s, state, ntiles);
     CC_CHECK_LAUNCH();
-    const FlowSched f{tiles4, dep_off, dep_idx, ntiles, state};
+    const FlowSched f{tiles4, dep_off, dep_idx, ntiles, state, sor_flow_wait_ticks()};
     const int nblk = flow_grid(ntiles);
     const unsigned pad = flow_lds_pad();
 #define CC_SOR_BFL64(S_, WJ_, R_)                                                                          \
@@ -1436,7 +1457,7 @@ cc_status Poisson2D::autotuned_tile(SorTile& out) {
             hipEventRecord(e0, 0);
             if (flow) {
                 hipLaunchKernelGGL(k_fill_int, dim3(std::min(64, (ntiles + 2 + 255) / 256)), dim3(256), 0, 0, d_st, ntiles + 2, 0);
-                st = launch_flow(c, mixed, flow_grid(ntiles), 0, args, FlowSched{sc, d_off, d_idx, ntiles, d_st}, phi, rz, rhobar);
+                st = launch_flow(c, mixed, flow_grid(ntiles), 0, args, FlowSched{sc, d_off, d_idx, ntiles, d_st, sor_flow_wait_ticks()}, phi, rz, rhobar);
             } else {
                 for (size_t l = 0; l + 1 < off.size() && st == CC_OK; ++l)
                     st = launch_tile(c, mixed, off[l + 1] - off[l], 0, args, sc + 3 * off[l], phi, rz, rhobar);
@@ -1505,7 +1526,7 @@ cc_status Poisson2D::sweeps(const double* rho, hipStream_t s, Profiler* prof) {
         hipLaunchKernelGGL(k_flow_reset, dim3(std::min(64, (ntiles + 2 + 255) / 256)), dim3(256), 0, s, flow_state,
                            ntiles);
         CC_CHECK_LAUNCH();
-        FlowSched f{sched, dep_off, dep_idx, ntiles, flow_state};
+        FlowSched f{sched, dep_off, dep_idx, ntiles, flow_state, sor_flow_wait_ticks()};
         // two 1024-thread tiles fit a CU: a grid of 2 per CU keeps every block resident
         CC_TRY(launch_flow(tile_, precision == CC_MIXED, flow_grid(ntiles), s, args, f, phi, rho, rhobar));
     } else if (tiled_) {

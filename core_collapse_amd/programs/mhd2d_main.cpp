@@ -70,6 +70,9 @@ int main(int argc, char** argv) {
     cli.add("B0", &col.B0, "collapse IC: uniform Bx");
     cli.add("precision", &precision, "fp64 (reference arithmetic) | mixed");
     cli.add("out", &out, "output file");
+    DtGuard guard;
+    cli.add("dt_guard", &guard.ratio,
+            "stop with status 3 when dt falls below this fraction of the last 64 steps' largest dt (0: never)");
     if (int r = cli.parse(argc, argv)) return r == 1 ? 0 : 2;
     p.precision = parse_precision(precision);
     if (p.precision < 0 || (ic_name != "reference" && ic_name != "collapse") || !(dt_store > 0.0)) {
@@ -116,6 +119,22 @@ int main(int argc, char** argv) {
     while (t < t_end) {
         double dt = 0.0;
         check(cc_mhd2d_run(h, 1, &t, t_end, &dt), "cc_mhd2d_run");
+        if (guard.trip(dt, t, t_end)) {
+            std::vector<double> f(w.buffer().size());
+            check(cc_mhd2d_frame(h, f.data()), "cc_mhd2d_frame");
+            const size_t n = (size_t)Nx * Ny;
+            const size_t q = fastest_cell(&f[0], &f[n], &f[5 * n], &f[6 * n], &f[7 * n], &f[8 * n], n, p.gamma);
+            std::fprintf(stderr,
+                         "dt collapsed at t=%.9g: dt=%.4g after steps of up to %.4g (--dt_guard %g); fastest cell "
+                         "i=%zu j=%zu: rho=%.6g P=%.6g u=%.6g v=%.6g Bx=%.6g By=%.6g Bz=%.6g -- an HLLD star state "
+                         "near the Alfven degeneracy (the reference guards its denominator only by 1e-14); stopping\n",
+                         t, dt, guard.scale(), guard.ratio, q % Nx, q / Nx, f[q], f[n + q], f[3 * n + q], f[4 * n + q],
+                         f[6 * n + q], f[7 * n + q], f[8 * n + q]);
+            if (!w.finish({{"/x", centres(p.x0, p.x1, Nx)}, {"/y", centres(p.y0, p.y1, Ny)}}))
+                std::fprintf(stderr, "%s\n", w.error().c_str());
+            cc_mhd2d_destroy(h);
+            return 3;
+        }
         if (t >= next_store - 1e-15) {
             double maxdiv = 0.0;
             check(cc_mhd2d_divb_max(h, &maxdiv), "cc_mhd2d_divb_max");

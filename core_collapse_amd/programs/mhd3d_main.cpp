@@ -79,6 +79,9 @@ int main(int argc, char** argv) {
     cli.add("precision", &precision, "fp64 (reference arithmetic) | mixed");
     cli.add("degen_tau", &p.degen_tau, "mixed: FP64 fallback margin near the Alfven degeneracy (<= 0: 0.02)");
     cli.add("out", &out, "output file");
+    DtGuard guard;
+    cli.add("dt_guard", &guard.ratio,
+            "stop with status 3 when dt falls below this fraction of the last 64 steps' largest dt (0: never)");
     if (int r = cli.parse(argc, argv)) return r == 1 ? 0 : 2;
     p.precision = parse_precision(precision);
     p.eos = (eos == "hybrid") ? 0 : (eos == "ideal") ? 1 : -1;
@@ -114,6 +117,23 @@ int main(int argc, char** argv) {
     while (t < t_end) {
         double dt = 0.0;
         check(cc_mhd3d_run(h, 1, &t, t_end, &dt), "cc_mhd3d_run");
+        if (guard.trip(dt, t, t_end)) {
+            std::vector<double> f(w.buffer().size());
+            check(cc_mhd3d_frame(h, f.data()), "cc_mhd3d_frame");
+            const size_t n = (size_t)Nx * Ny * Nz;
+            const double gam = p.eos == 1 ? p.gamma : p.gamma2;
+            const size_t q = fastest_cell(&f[0], &f[n], &f[6 * n], &f[7 * n], &f[8 * n], &f[9 * n], n, gam);
+            std::fprintf(stderr,
+                         "dt collapsed at t=%.9g: dt=%.4g after steps of up to %.4g (--dt_guard %g); fastest cell "
+                         "i=%zu j=%zu k=%zu: rho=%.6g P=%.6g -- an HLLD star state near the Alfven degeneracy (the "
+                         "reference guards its denominator only by 1e-14); stopping\n",
+                         t, dt, guard.scale(), guard.ratio, q % Nx, (q / Nx) % Ny, q / ((size_t)Nx * Ny), f[q], f[n + q]);
+            if (!w.finish({{"/x", centres(p.x0, p.x1, Nx)}, {"/y", centres(p.y0, p.y1, Ny)},
+                           {"/z", centres(p.z0, p.z1, Nz)}}))
+                std::fprintf(stderr, "%s\n", w.error().c_str());
+            cc_mhd3d_destroy(h);
+            return 3;
+        }
         if (t >= tnext - 1e-12) {
             std::vector<double> f = w.buffer();
             check(cc_mhd3d_frame(h, f.data()), "cc_mhd3d_frame");

@@ -64,11 +64,27 @@ def test_hlld_batch_mixed_close_to_oracle(axis):
     n = 4000
     WL, WR = _random_prims(rng, n), _random_prims(rng, n)
     Bn = rng.normal(0, 1.0, n)
+    # faces placed at the HLL switch (|Bn| < 1e-12) and at the Alfven degeneracy of the left star state
+    # (B along the normal, c_s < v_A: rho (S_L - u)(S_L - S_M) - Bn^2 ~ 0), where the mixed path recomputes in FP64
+    Bn[::13] = 0.0
+    deg = np.arange(5, n, 17)
+    WL[deg, 6:8] = 1e-3 * rng.normal(0, 1.0, (len(deg), 2))
+    WR[deg] = WL[deg]
+    WR[deg, 1] += 1e-3
+    WL[deg, 4] = 0.3 * WL[deg, 0]
+    WR[deg, 4] = WL[deg, 4]
+    Bn[deg] = 1.5 * np.sqrt(WL[deg, 0])
     got = M.hlld_batch(axis, WL, WR, Bn, precision=CC_MIXED)
     want = _oracle_hlld(axis, WL, WR, Bn)
     scale = np.abs(want).max(axis=1, keepdims=True) + 1e-12
     err = np.abs(got - want) / scale
-    assert np.median(err) < 1e-5, np.median(err)
+    # FP32 Riemann solver on FP32-rounded states: ~1e-7 typical, a few hundred ulps where the HLLD's
+    # subtractions cancel; the bound on the worst face is what catches a wrong branch on a minority of faces
+    print("mixed HLLD rel err: median %.2e, 99.9%% %.2e, max %.2e" % (np.median(err), np.quantile(err, 0.999), err.max()))
+    assert np.median(err) < 1e-6, np.median(err)
+    assert np.quantile(err, 0.999) < 1e-4, np.quantile(err, 0.999)
+    assert err.max() < 1e-2, err.max()
+    assert err[::13].max() < 1e-4, err[::13].max()  # HLL faces: no cancellation-prone star states
 
 
 def _oracle_poisson(p, rho):
@@ -365,7 +381,64 @@ def test_batch_equals_instances_run_alone(prec, mode, monkeypatch):
         s1, s2 = h1.download(), h2.download()
         for f in M.FIELDS:
             assert np.array_equal(s1[f], s2[f]), f
+    # every instance against the oracle's run of the same loop (M2:1111-1130), both calls, its own t_end:
+    # FP64 to 1e-9 relative per field, the mixed form at the 1e-4 relative L1 density bar
+    o = oracle()
+    for i, (e, h) in enumerate(zip(sample, batch_sims)):
+        p = h.params
+        S_ = m2_arrays(p, ic.collapse2d_ic(p, **e["ic"]))
+        op = m2_oparams(p)
+        ot = o.or_m2_run(C.byref(op), C.byref(S_.cstate()), C.c_double(p.dx), C.c_double(p.dy), 12, C.c_double(0.0),
+                         C.c_double(t_end[i]), None)
+        ot = o.or_m2_run(C.byref(op), C.byref(S_.cstate()), C.c_double(p.dx), C.c_double(p.dy), 7, C.c_double(ot),
+                         C.c_double(t_end[i]), None)
+        got, want = h.download(), m2_dict(p, S_)
+        if prec == CC_FP64:
+            assert abs(t[i] - ot) <= 1e-11 * ot, (i, t[i], ot)
+            for f in M.FIELDS:
+                assert rel_err(got[f], want[f]) < 1e-9, (i, f, rel_err(got[f], want[f]))
+        else:
+            assert abs(t[i] - ot) <= 1e-6 * ot, (i, t[i], ot)
+            assert rel_l1(got["rho"], want["rho"]) <= 1e-4, (i, rel_l1(got["rho"], want["rho"]))
     for h in alone + batch_sims:
+        h.close()
+
+
+def test_batch_of_64_sampled_collapses_equals_instances_run_alone():
+    """The 64-instance batch itself (collapse2d_batch_sample(64, seed 42, N in [128, 1024]), mixed
+    precision, 600 SOR sweeps): 3 batch steps give bit for bit the times and states of the instances
+    stepped alone, and the face field stays divergence-free at round-off (max|div B| <= 1e-12)."""
+    from core_collapse_amd.mhd2d import Mhd2D, Mhd2DBatch
+    sample = ic.collapse2d_batch_sample(64, seed=42, nmin=128, nmax=1024)
+    assert len(sample) == 64 and min(e["N"] for e in sample) >= 128 and max(e["N"] for e in sample) <= 1024
+
+    def make():
+        sims = []
+        for e in sample:
+            p = ic.collapse2d_params(e["N"], eta=e["eta"], precision=CC_MIXED)
+            h = Mhd2D(p)
+            h.upload(ic.collapse2d_ic(p, **e["ic"]))
+            sims.append(h)
+        return sims
+
+    batch_sims = make()
+    b = Mhd2DBatch(batch_sims)
+    print("batch of 64: cells %d, %s" % (sum(e["N"] ** 2 for e in sample), b.info()), flush=True)
+    t = b.run(3, None, np.full(64, 1e300))
+    b.close()
+    divb = max(h.divB_max() for h in batch_sims)
+    print("batch max|divB| after 3 steps: %.3e" % divb, flush=True)
+    assert divb <= 1e-12, divb
+    for i, h in enumerate(batch_sims):
+        p = h.params
+        with Mhd2D(p) as a:
+            a.upload(ic.collapse2d_ic(p, **sample[i]["ic"]))
+            ta = a.run(3)
+            sa = a.download()
+        assert ta == t[i], (i, ta, t[i])
+        sb = h.download()
+        for f in M.FIELDS:
+            assert np.array_equal(sa[f], sb[f]), (i, f)
         h.close()
 
 
