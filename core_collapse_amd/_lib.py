@@ -172,6 +172,7 @@ _PROTOS = {
     "cc_ns2d_kernel_time": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int), DP]),
     "cc_ns2d_reset_profiling": (C.c_int, [C.c_void_p]),
     "cc_fft2d": (C.c_int, [C.c_int, C.c_int, C.c_int, DP]),
+    "cc_fft2d_noalloc": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "cc_copy_calibration": (C.c_int, [C.c_size_t, C.c_int, DP]),
     "cc_poisson2d_sor": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                    C.c_int, C.c_double, C.c_int, DP, DP, DP, DP]),

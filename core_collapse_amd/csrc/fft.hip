@@ -1,5 +1,7 @@
 // fft.hip -- batched radix-2 FFT lines in LDS; see fft.hpp.
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "fft.hpp"
@@ -104,8 +106,7 @@ static int ilog2(int n) {
     return l;
 }
 
-cc_status Fft2D::init(int Nx_, int Ny_) {
-    destroy();
+cc_status Fft2D::plan(int Nx_, int Ny_) {
     Nx = Nx_;
     Ny = Ny_;
     auto pow2 = [](int n) { return n > 0 && (n & (n - 1)) == 0; };
@@ -117,13 +118,23 @@ cc_status Fft2D::init(int Nx_, int Ny_) {
         set_error(CC_ERR_INVALID_ARG, "FFT line length above 4096 is not supported");
         return CC_ERR_INVALID_ARG;
     }
-    std::vector<double> t;
     ax.n = Nx; ax.logn = ilog2(Nx); ax.lpb = std::max(1, std::min(16, 2048 / Nx));
     ay.n = Ny; ay.logn = ilog2(Ny); ay.lpb = std::max(1, std::min(16, 2048 / Ny));
-    ax.tw_fwd = t.size() / 2; build_twiddles(Nx, false, t);
-    ax.tw_inv = t.size() / 2; build_twiddles(Nx, true, t);
-    ay.tw_fwd = t.size() / 2; build_twiddles(Ny, false, t);
-    ay.tw_inv = t.size() / 2; build_twiddles(Ny, true, t);
+    ax.tw_fwd = 0;
+    ax.tw_inv = ax.tw_fwd + (size_t)(Nx - 1);
+    ay.tw_fwd = ax.tw_inv + (size_t)(Nx - 1);
+    ay.tw_inv = ay.tw_fwd + (size_t)(Ny - 1);
+    return CC_OK;
+}
+
+cc_status Fft2D::init(int Nx_, int Ny_) {
+    destroy();
+    CC_TRY(plan(Nx_, Ny_));
+    std::vector<double> t;
+    build_twiddles(Nx, false, t);
+    build_twiddles(Nx, true, t);
+    build_twiddles(Ny, false, t);
+    build_twiddles(Ny, true, t);
     t.push_back(0.0); t.push_back(0.0);  // keeps the table non-empty for 1-point axes
     if (hipMalloc(&tw, t.size() * sizeof(double)) != hipSuccess) {
         set_error(CC_ERR_OOM, "FFT twiddle allocation failed");
@@ -175,5 +186,105 @@ extern "C" cc_status cc_fft2d(int Nx, int Ny, int inverse, double* data) {
         st = CC_ERR_HIP;
     }
     hipFree(d);
+    return st;
+}
+
+// ------------------------------------------------------------------ allocation-free entry point
+// The twiddle tables of cc_fft2d_noalloc live in `scratch`, a file-scope device array: every GPU the
+// library runs on holds one copy until the process ends.  Host code never passes the array's name to
+// a kernel (on the host it is only a stand-in symbol); it uses the per-device address that
+// hipGetSymbolAddress returns on the entry point's first call on that device.
+namespace cc {
+
+constexpr int kFftScratchEntries = 4 * kFftMaxLen;  // complex FP64 entries: 4 tables of < 4096 (256 KiB)
+static __device__ __align__(256) double2 scratch[kFftScratchEntries];
+
+constexpr int kFftMaxStages = 12;  // log2(4096)
+// Per table (x fwd, x inv, y fwd, y inv): length n, offset, and w_len = (cos, sin)(+-2 pi / len) of
+// every stage from the host's libm (NS:164).
+struct FftTwiddlePlan {
+    int n[4], off[4];
+    double wr[4][kFftMaxStages], wi[4][kFftMaxStages];
+};
+
+// One thread per (table, stage): w_j = w_len^j by the same sequential complex products as
+// Fft2D::build_twiddles (NS:162-171), compiled without contraction, so the tables are bit-identical.
+__global__ void k_fft_twiddles(FftTwiddlePlan pl, double2* __restrict__ tw) {
+    const int tab = (int)threadIdx.x / kFftMaxStages, st = (int)threadIdx.x % kFftMaxStages;
+    if (tab >= 4) return;
+    const int len = 2 << st;
+    if (len > pl.n[tab]) return;
+    const double lr = pl.wr[tab][st], li = pl.wi[tab][st];
+    double wr = 1.0, wi = 0.0;
+    double2* out = tw + pl.off[tab] + (len / 2 - 1);
+    for (int j = 0; j < len / 2; j++) {
+        out[j] = make_double2(wr, wi);
+        const double nr = wr * lr - wi * li;
+        const double ni = wr * li + wi * lr;
+        wr = nr;
+        wi = ni;
+    }
+}
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double2* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (looked up once per device).
+static cc_status fft_scratch(double2** out) {
+    int dev = 0;
+    CC_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error(CC_ERR_INVALID_ARG, "cc_fft2d_noalloc: device index out of range");
+        return CC_ERR_INVALID_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_error(CC_ERR_HIP, std::string("cc_fft2d_noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return CC_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = static_cast<double2*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return CC_OK;
+}
+
+}  // namespace cc
+
+extern "C" cc_status cc_fft2d_noalloc(int Nx, int Ny, int inverse, double* data, void* stream) {
+    using namespace cc;
+    if (!data) { set_error(CC_ERR_INVALID_ARG, "null data"); return CC_ERR_INVALID_ARG; }
+    Fft2D f;
+    CC_TRY(f.plan(Nx, Ny));  // powers of two <= 4096: the tables fit kFftScratchEntries
+    if (f.twiddle_entries() > (size_t)kFftScratchEntries) {
+        set_error(CC_ERR_INVALID_ARG, "cc_fft2d_noalloc: twiddle tables exceed the scratch array");
+        return CC_ERR_INVALID_ARG;
+    }
+    double2* tw = nullptr;
+    CC_TRY(fft_scratch(&tw));
+    FftTwiddlePlan pl{};
+    const int ns[4] = {Nx, Nx, Ny, Ny};
+    const size_t offs[4] = {f.ax.tw_fwd, f.ax.tw_inv, f.ay.tw_fwd, f.ay.tw_inv};
+    for (int t = 0; t < 4; ++t) {
+        pl.n[t] = ns[t];
+        pl.off[t] = (int)offs[t];
+        for (int st = 0; st < kFftMaxStages; ++st) {
+            const int len = 2 << st;
+            const double ang = 2.0 * M_PI / len * ((t & 1) ? +1.0 : -1.0);
+            pl.wr[t][st] = std::cos(ang);
+            pl.wi[t][st] = std::sin(ang);
+        }
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_fft_twiddles, dim3(1), dim3(64), 0, s, pl, tw);
+    CC_CHECK_LAUNCH();
+    f.tw = reinterpret_cast<double*>(tw);
+    cc_status st = f.pass(false, inverse != 0, FFT_IN_COMPLEX, FFT_OUT_COMPLEX, data, nullptr, nullptr, nullptr, nullptr, s);
+    if (st == CC_OK)
+        st = f.pass(true, inverse != 0, FFT_IN_COMPLEX, FFT_OUT_COMPLEX, data, nullptr, nullptr, nullptr, nullptr, s);
+    f.tw = nullptr;  // the plan does not own the scratch array
     return st;
 }

@@ -32,6 +32,8 @@ struct FftGreen {  // A *= (-(4 pi G)) / k^2 with continuous k = 2 pi n / L, k =
 
 class Fft2D {
    public:
+    // Line lengths, launch shapes and twiddle-table offsets only (no allocation).
+    cc_status plan(int Nx, int Ny);
     cc_status init(int Nx, int Ny);
     void destroy();
     ~Fft2D() { destroy(); }
@@ -42,6 +44,9 @@ class Fft2D {
     int Nx = 0, Ny = 0;
     FftAxis ax, ay;
     double* tw = nullptr;  // device twiddles, complex interleaved
+
+    // Number of complex twiddle entries of the plan (all four tables).
+    size_t twiddle_entries() const { return 2 * (size_t)(Nx - 1) + 2 * (size_t)(Ny - 1) + 1; }
 
    private:
     static void build_twiddles(int n, bool inverse, std::vector<double>& out);

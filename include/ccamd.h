@@ -350,6 +350,16 @@ cc_status cc_ns2d_compare(cc_ns2d* a, cc_ns2d* b, double* out);
  * then columns, inverse scaled by 1/n per axis.  Powers of two <= 4096. */
 cc_status cc_fft2d(int Nx, int Ny, int inverse, double* data);
 
+/* cc_fft2d on device memory, allocating nothing (no counterpart in the reference: for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight serving loops).  `data` is a
+ * device pointer to Nx*Ny interleaved complex values, transformed in place; `stream` is a hipStream_t
+ * (null: the default stream) on which the call is asynchronous.  The twiddle tables go to a fixed
+ * device array of 256 KiB that each GPU holds until the process ends: Nx and Ny must be powers of two
+ * <= 4096 (else CC_ERR_NON_POW2 / CC_ERR_INVALID_ARG, before any device work).  Every call on one GPU
+ * rebuilds the tables in that one array, so calls that overlap on a GPU (other streams, other threads)
+ * race; serialise them.  Results are bit-identical to cc_fft2d. */
+cc_status cc_fft2d_noalloc(int Nx, int Ny, int inverse, double* data, void* stream);
+
 /* ============================================================================================
  * 3D Euler HLLC + FFT self-gravity, periodic (the uniform-sphere collapse benchmark).  No reference
  * program has this step: it is the EH step (EH:406-550: floored primitives, 4-point MUSCL-MC, the
