@@ -1,34 +1,31 @@
 #!/usr/bin/env python3
 """Benchmark of the solver step on one MI355X.
 
-Workload: 2D resistive MHD, HLLD + constrained transport + SOR self-gravity (two 600-sweep Poisson
-solves per Heun step), Gaussian collapse on a 2048 x 2048 grid over [-0.5,0.5]^2 with clamped
-boundaries, gamma 5/3, G 1, eta 1e-4, CFL 0.4 (the 2D MHD program's step with the collapse IC).
-The same K steps run once in plain FP64 and once in the mixed form (FP32 reconstruction, Riemann
-solver and SOR sweeps; FP64 update, B/CT and Poisson residual), from the same seeded IC.
+Headline workload (config 2): 2D resistive MHD, HLLD + constrained transport + SOR self-gravity (two
+600-sweep Poisson solves per Heun step), Gaussian collapse on a 2048 x 2048 grid over [-0.5,0.5]^2 with
+clamped boundaries, gamma 5/3, G 1, eta 1e-4, CFL 0.4, compensated face update.  --steps steps run once in
+plain FP64 and once in the mixed form (FP32 reconstruction, Riemann solver and SOR sweeps; FP64 update,
+B/CT and Poisson residual) from the same seeded IC, each timed on the captured-graph path with no
+per-launch events.  A zone-update is one cell advanced one full Heun step including its two Poisson
+solves; `value` is the mixed zone-updates/s with all state resident in HBM.
 
-A zone-update is one cell advanced one full Heun step including its two Poisson solves.
-`value` is the mixed-precision zone-updates/s with all state resident in HBM; the FP64 figure,
-the ratio, the mixed-vs-FP64 relative L1 density error and max|div B| ride along.
+After the timed steps, each precision runs --profile-steps more with HIP events around every launch on
+the handle's stream: `roofline` (dominant kernel: the SOR dataflow launch, one per 600-sweep solve) and
+`kernel_times` come from that pass.  The SOR's algorithmic bytes per launch are cell-sweeps x 24 B (FP64:
+read phi, read rhs, write phi) or 12 B (mixed); the tile keeps 16 sweeps in LDS, so this rate can exceed
+the HBM peak -- `traffic` (rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, profiles/hbm_traffic.json) and
+`traffic_frac` give the HBM load itself.
 
-roofline: dominant kernel = the tiled SOR kernel (k_sor_sflow_* / k_sor_flow_*: one dataflow launch
-per 600-sweep solve on sheared or rectangular tiles; k_sor_(s)tile_*, one launch per tile level, when
-CC_SOR_LEVELS is set), timed with HIP events on the handle's stream around every launch inside the
-timed region.  Algorithmic bytes per launch = (cell-sweeps in the launch) x 24 B (FP64: read phi,
-read rhs, write phi) or x 12 B (mixed: FP32 phi and rhs), the compulsory traffic of one sweep.  The kernel keeps its tile in LDS for
-16 sweeps, so its algorithmic rate can exceed the HBM peak; `traffic` (measured HBM bytes per
-launch, from rocprofv3 FETCH_SIZE/WRITE_SIZE in profiles/hbm_traffic.json) shows the real HBM load.
+`accuracy`: both precisions run --accuracy-steps (1000) steps from the IC (timed too); per frame field
+the relative L1 and L-inf error of mixed against FP64 and max|div B| of both.
 
-cpu_baseline: the reference program's own step functions (compiled from its sources in
-oracle/_ref, single thread) on a bounded sample of the same problem (see "sample").
+cpu_baseline: the oracle's C restatement of the reference step (bit-identical to the reference's step
+functions on the golden vectors), single thread, on a bounded sample of the same problem.
 
-The line also carries "turb3d": the 3D resistive MHD step (HLLD + CT, no gravity) on the 256^3
-periodic turbulence problem (ideal gas gamma 5/3, beta 1 field along z, k^-2 solenoidal velocity at
-rms Mach 1, eta 5e-5, CFL 0.4), FP64 and mixed, with the roofline of its dominant kernel (the face
-kernel, one launch per axis: per face it reads one cell's 8 primitives and the face B and writes 5
-fluxes + 2 EMFs, 128 B in FP64 and 68 B with FP32 primitives and fluxes)
-and the CPU restatement's rate on a 64^3 sample (the reference has no ideal-gas EOS, so the CPU
-leg is the oracle's port of its step with that EOS).  --workload selects one of the two problems.
+The line also carries "turb3d" (config 3: 3D HLLD+CT turbulence at 256^3, 500 steps per precision,
+per-field errors, face-kernel roofline without the FP64 pass over queued faces), "batch2d" (config 4:
+64 collapse instances in one graph per step) and "sphere3d" (config 5: 3D Euler + FFT gravity at 512^3,
+300 steps per precision).  --workload selects one problem.
 """
 import argparse
 import ctypes as C
@@ -50,14 +47,16 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=20, help="timed steps of the headline (2048^2 collapse)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--N", type=int, default=2048)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound on the CPU-baseline sample")
+    ap.add_argument("--accuracy-steps", type=int, default=1000,
+                    help="length of the FP64 and mixed 2048^2 runs compared field by field (0: skip)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound on each CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=("all", "collapse2d", "turb3d", "batch2d", "sphere3d"), default="all")
     ap.add_argument("--N5", type=int, default=512, help="grid of the 3D Euler sphere problem")
-    ap.add_argument("--steps5", type=int, default=10)
+    ap.add_argument("--steps5", type=int, default=300)
     ap.add_argument("--batch", type=int, default=64, help="instances of the batched collapse")
     ap.add_argument("--steps-batch", type=int, default=200)
     ap.add_argument("--sor-autotune", type=int, default=1,
@@ -65,51 +64,113 @@ def parse():
     ap.add_argument("--seq-steps", type=int, default=10,
                     help="steps per instance of the one-after-another comparison (0: skip)")
     ap.add_argument("--N3", type=int, default=256, help="grid of the 3D turbulence problem")
-    ap.add_argument("--steps3", type=int, default=10)
+    ap.add_argument("--steps3", type=int, default=500)
+    ap.add_argument("--profile-steps", type=int, default=5,
+                    help="steps of the separate profiled pass (HIP events per launch) that gives kernel times")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (for counter calibration) and exit")
     return ap.parse_args()
 
 
-def run_gpu(N, precision, warmup, steps):
-    from core_collapse_amd import ic
-    from core_collapse_amd import mhd2d as M
-    p = ic.collapse2d_params(N, precision=precision)
-    st = ic.collapse2d_ic(p)
-    h = M.Mhd2D(p)
-    h.upload(st)
-    t = h.run(warmup) if warmup > 0 else 0.0
-    h.set_profiling(True)
-    h.reset_profiling()
+def log(msg):
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
+def timed_run(h, steps, t=0.0):
+    """steps of the handle's driver loop (captured HIP graphs, no per-launch events), wall time around them."""
     h.sync()
     t0 = time.perf_counter()
     t = h.run(steps, t)          # synchronises at the end
-    elapsed = time.perf_counter() - t0
-    prof = {k: h.kernel_time(k) for k in ("sor", "flux", "corner", "ct", "grad", "mean", "dt")}
+    return t, time.perf_counter() - t0
+
+
+def profiled_pass(h, steps, t, names):
+    """steps more with per-launch HIP events on the handle's stream: {kernel scope: (launches, total ms)}."""
+    h.set_profiling(True)
+    h.reset_profiling()
+    t = h.run(steps, t)
+    prof = {k: h.kernel_time(k) for k in names}
     h.set_profiling(False)
-    out = h.download()
-    divb = h.divB_max()
-    h.close()
-    tile = M.sor_tile_choice(N, N, p.sor_iters, precision)
-    return {"elapsed": elapsed, "t": t, "rho": out["rho"], "divb": divb, "prof": prof, "params": p, "tile": tile}
+    return t, prof
+
+
+def field_errors(cmp, ref_frame, names):
+    """Per frame field: relative L1 (sum|a-b| / sum|b|) and L-inf relative to the field's max |b|."""
+    out = {}
+    for f in names:
+        mean, rl1, mx = cmp[f]
+        scale = float(np.abs(ref_frame[f]).max())
+        out[f] = {"l1_rel": rl1, "linf_rel": mx / scale if scale > 0 else mx, "linf_abs": mx}
+    return out
+
+
+def bench_collapse2d(N, warmup, steps, acc_steps, prof_steps):
+    """Config 2: the 2D collapse at N^2, plain FP64 and mixed.  Headline: `steps` steps per precision timed
+    on the graph path; a profiled pass of prof_steps gives the kernel times; then both precisions run
+    acc_steps from the IC and are compared field by field on the device."""
+    from core_collapse_amd import CC_FP64, CC_MIXED
+    from core_collapse_amd import ic
+    from core_collapse_amd import mhd2d as M
+    res = {}
+    for prec in (CC_FP64, CC_MIXED):
+        p = ic.collapse2d_params(N, precision=prec)
+        h = M.Mhd2D(p)
+        h.upload(ic.collapse2d_ic(p))
+        t = h.run(warmup) if warmup > 0 else 0.0
+        t, el = timed_run(h, steps, t)
+        t, prof = profiled_pass(h, prof_steps, t, ("sor", "flux", "corner", "ct", "grad", "mean", "dt"))
+        res[prec] = {"elapsed": el, "prof": prof, "tile": M.sor_tile_choice(N, N, p.sor_iters, prec)}
+        h.close()
+        log("collapse2d %s %.3f ms/step" % ("fp64" if prec == CC_FP64 else "mixed", 1e3 * el / steps))
+    acc = None
+    if acc_steps > 0:
+        hs, tacc = {}, {}
+        for prec in (CC_FP64, CC_MIXED):
+            p = ic.collapse2d_params(N, precision=prec)
+            hs[prec] = M.Mhd2D(p)
+            hs[prec].upload(ic.collapse2d_ic(p))
+            _, tacc[prec] = timed_run(hs[prec], acc_steps)
+            log("collapse2d %d-step %s run %.1f s" % (acc_steps, "fp64" if prec == CC_FP64 else "mixed", tacc[prec]))
+        cmp = hs[CC_MIXED].compare(hs[CC_FP64])
+        ref = hs[CC_FP64].frame()
+        zu = float(N) * N * acc_steps
+        acc = {"steps": acc_steps,
+               "fp64": {"value": zu / tacc[CC_FP64], "ms_per_step": 1e3 * tacc[CC_FP64] / acc_steps,
+                        "max_divB": hs[CC_FP64].divB_max()},
+               "mixed": {"value": zu / tacc[CC_MIXED], "ms_per_step": 1e3 * tacc[CC_MIXED] / acc_steps,
+                         "max_divB": hs[CC_MIXED].divB_max()},
+               "mixed_over_fp64": tacc[CC_FP64] / tacc[CC_MIXED],
+               "errors": field_errors(cmp, ref, ("rho", "P", "u", "v", "vabs", "Bx", "By", "Bz"))}
+        for h in hs.values():
+            h.close()
+    return res, acc
 
 
 def sor_cell_sweeps_per_solve(N, iters=600):
     return float(N) * N * iters
 
 
+def cpu_info():
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return "%s, nproc=%d" % (cpu, os.cpu_count() or 0)
+
+
 def cpu_baseline(N, budget_s):
-    """Time the reference's own compiled step (driver loop M2:1111-1117) on a sample grid."""
+    """The oracle's restatement of the 2D step (bit-identical to the reference's step functions on the
+    golden vectors) on a sample grid of the same problem, single thread."""
     from core_collapse_amd import ic
     from oracle import bind as OB
-    kind = "reference"
-    try:
-        lib = OB.Lib("ref_m2")
-    except FileNotFoundError:
-        lib = OB.Lib("oracle")
-        kind = "port"
+    lib = OB.Lib("oracle")
     n = 512 if N >= 512 else N
-    p = ic.collapse2d_params(n)
+    p = ic.collapse2d_params(n, ct_compensated=False)
     st = ic.collapse2d_ic(p)
     op = OB.m2_params(gamma=p.gamma, cfl=p.cfl, x0=p.x0, x1=p.x1, y0=p.y0, y1=p.y1, Nx=n, Ny=n,
                       floor_rho=p.floor_rho, floor_P=p.floor_P, periodic_x=0, periodic_y=0, eta=p.eta,
@@ -122,40 +183,40 @@ def cpu_baseline(N, budget_s):
                           C.c_double(1e300), None)
         tsum += time.perf_counter() - t0
         steps += 1
-    zups = n * n * steps / tsum
-    cpu = platform.processor() or platform.machine()
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
-    return {"value": zups, "unit": "zone-updates/s", "cores": 1, "kind": kind,
-            "sample": "%d Heun step(s) of the same collapse problem at %dx%d (FP64, 600-sweep SOR x2 per step, "
-                      "%.1f s), single thread, %s, nproc=%d" % (steps, n, n, tsum, cpu, os.cpu_count() or 0)}
+    return {"value": n * n * steps / tsum, "unit": "zone-updates/s", "cores": 1, "kind": "port",
+            "sample": "%d Heun step(s) of the same collapse problem at %dx%d (FP64 reference arithmetic, 600-sweep "
+                      "SOR x2 per step, %.1f s), single thread, the oracle's C restatement of the reference step "
+                      "(bit-identical to it on the golden vectors), %s" % (steps, n, n, tsum, cpu_info())}
 
 
-def run_gpu3d(N, precision, warmup, steps):
+def bench_turb3d(N, warmup, steps, prof_steps):
+    """Config 3: 3D resistive HLLD+CT, turbulence at N^3, FP64 and mixed, `steps` steps each from the IC
+    timed on the graph path, compared field by field on the device; then a profiled pass."""
+    from core_collapse_amd import CC_FP64, CC_MIXED
     from core_collapse_amd import ic
     from core_collapse_amd import mhd3d as M3
-    p = ic.turbulence3d_params(N, precision=precision)
-    h = M3.Mhd3D(p)
-    h.upload(ic.turbulence3d_ic(p))
-    t = h.run(warmup) if warmup > 0 else 0.0
-    h.set_profiling(True)
-    h.reset_profiling()
-    h.sync()
-    t0 = time.perf_counter()
-    t = h.run(steps, t)
-    elapsed = time.perf_counter() - t0
-    prof = {k: h.kernel_time(k) for k in ("prim", "face", "update", "edge", "ct", "joule", "dt")}
-    h.set_profiling(False)
-    out = h.download()
-    divb = h.divB_max()
-    h.close()
-    return {"elapsed": elapsed, "rho": out["rho"], "divb": divb, "prof": prof}
+    hs, el, prof = {}, {}, {}
+    names = ("prim", "face", "face_fp64", "update", "edge", "ct", "joule", "dt")
+    for prec in (CC_FP64, CC_MIXED):
+        p = ic.turbulence3d_params(N, precision=prec)
+        h = M3.Mhd3D(p)
+        h.upload(ic.turbulence3d_ic(p))
+        if warmup > 0:  # warm the graphs on a copy of the IC, then restart from it
+            h.run(warmup)
+            h.upload(ic.turbulence3d_ic(p))
+        _, el[prec] = timed_run(h, steps)
+        hs[prec] = h
+        log("turb3d %s %.3f ms/step over %d steps" % ("fp64" if prec == CC_FP64 else "mixed", 1e3 * el[prec] / steps,
+                                                        steps))
+    cmp = hs[CC_MIXED].compare(hs[CC_FP64])
+    ref = hs[CC_FP64].frame()
+    errs = field_errors(cmp, ref, ("rho", "P", "u", "v", "w", "vabs", "Bx", "By", "Bz"))
+    divb = {prec: hs[prec].divB_max() for prec in hs}
+    emax = float(ref["P"].max())
+    for prec in (CC_FP64, CC_MIXED):
+        _, prof[prec] = profiled_pass(hs[prec], prof_steps, 0.0, names)
+        hs[prec].close()
+    return el, prof, errs, divb, emax
 
 
 def cpu_baseline3d(n=128, budget_s=12.0):
@@ -165,7 +226,8 @@ def cpu_baseline3d(n=128, budget_s=12.0):
     lib = OB.Lib("oracle")
     p = ic.turbulence3d_params(n)
     st = ic.turbulence3d_ic(p)
-    op = OB.m3_params(Nx=n, Ny=n, Nz=n, cfl=p.cfl, eta=p.eta, ideal_eos=1, gamma_ideal=p.gamma)
+    op = OB.m3_params(Nx=n, Ny=n, Nz=n, cfl=p.cfl, eta=p.eta, ideal_eos=1, gamma_ideal=p.gamma, hlld_guard=p.hlld_guard,
+                      ct_comp=int(p.ct_compensated))
     S = OB.M3Arrays(n, n, n, {k: v.reshape(-1) for k, v in st.items()})
     steps, tsum = 0, 0.0
     while steps < 20 and (steps == 0 or tsum < budget_s):
@@ -176,47 +238,50 @@ def cpu_baseline3d(n=128, budget_s=12.0):
         steps += 1
     return {"value": n ** 3 * steps / tsum, "unit": "zone-updates/s", "cores": 1, "kind": "port",
             "sample": "%d Heun step(s) of the same turbulence problem at %d^3 (FP64, %.1f s), single thread, "
-                      "oracle restatement (the reference program has no ideal-gas EOS)" % (steps, n, tsum)}
+                      "oracle restatement (the reference program has no ideal-gas EOS), %s" % (steps, n, tsum, cpu_info())}
 
 
-def bench3d(N, warmup, steps, with_cpu):
+def bench3d(N, warmup, steps, prof_steps, with_cpu):
     from core_collapse_amd import CC_FP64, CC_MIXED
-    res = {prec: run_gpu3d(N, prec, warmup, steps) for prec in (CC_FP64, CC_MIXED)}
+    el, prof, errs, divb, pmax = bench_turb3d(N, warmup, steps, prof_steps)
     zu = float(N) ** 3 * steps
 
-    def roof(r, word):
-        n, ms = r["prof"]["face"]
+    def roof(pr, word):
+        n, ms = pr["face"]
         if n == 0 or ms <= 0:
             return None
-        # per face: read the 8 primitives of one cell and the face B (FP64), write 5 fluxes + 2 EMFs
+        # per face: read the 8 primitives of one cell and the face B (FP64), write 5 fluxes + 2 EMFs; the
+        # mixed path's FP64 pass over queued faces is timed apart ("face_fp64") and not included here
         bpl = float(N) ** 3 * (8 * word + 8 + 7 * word)
         avg = ms / n * 1e-3
         return {"bound": "hbm", "achieved": round(bpl / avg / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(bpl / avg / 1e9 / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_face3d",
                 "launches": n, "avg_launch_us": round(avg * 1e6, 2), "alg_bytes_per_launch": bpl}
 
-    a, b = res[CC_MIXED]["rho"], res[CC_FP64]["rho"]
-    r64, rmx = roof(res[CC_FP64], 8), roof(res[CC_MIXED], 4)
+    r64, rmx = roof(prof[CC_FP64], 8), roof(prof[CC_MIXED], 4)
     if r64:
         r64["traffic"] = load_traffic("cc::k_face3d<0, false")
     if rmx:
         rmx["traffic"] = load_traffic("cc::k_face3d<0, true")
     out = {
         "workload": "3D resistive MHD HLLD+CT, periodic turbulence %d^3, ideal gas gamma 5/3, beta 1 along z, "
-                    "rms Mach 1, eta 5e-5, CFL 0.4" % N,
+                    "rms Mach 1, eta 5e-5, CFL 0.4, HLLD degeneracy guard 1e-2, compensated face update" % N,
         "steps": steps, "warmup": warmup,
-        "value": zu / res[CC_MIXED]["elapsed"], "unit": "zone-updates/s",
-        "ms_per_step": 1e3 * res[CC_MIXED]["elapsed"] / steps,
-        "fp64": {"value": zu / res[CC_FP64]["elapsed"], "ms_per_step": 1e3 * res[CC_FP64]["elapsed"] / steps,
-                 "roofline": r64, "max_divB": res[CC_FP64]["divb"]},
-        "mixed_over_fp64": res[CC_FP64]["elapsed"] / res[CC_MIXED]["elapsed"],
-        "l1_rho_rel_err": float(np.abs(a - b).sum() / np.abs(b).sum()),
-        "linf_rho_rel_err": float(np.abs(a - b).max() / np.abs(b).max()),
-        "max_divB": res[CC_MIXED]["divb"],
+        "value": zu / el[CC_MIXED], "unit": "zone-updates/s",
+        "ms_per_step": 1e3 * el[CC_MIXED] / steps,
+        "fp64": {"value": zu / el[CC_FP64], "ms_per_step": 1e3 * el[CC_FP64] / steps, "roofline": r64,
+                 "max_divB": divb[CC_FP64]},
+        "mixed_over_fp64": el[CC_FP64] / el[CC_MIXED],
+        "errors": errs,
+        "l1_rho_rel_err": errs["rho"]["l1_rel"],
+        "linf_rho_rel_err": errs["rho"]["linf_rel"],
+        "max_divB": divb[CC_MIXED],
+        "max_P_fp64": pmax,
         "roofline": rmx,
         "zone_update_alg_bytes": 320,
-        "kernel_times": {tag: {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in res[prec]["prof"].items()
+        "kernel_times": {tag: {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in prof[prec].items()
                                if v[0] > 0} for prec, tag in ((CC_FP64, "fp64"), (CC_MIXED, "mixed"))},
+        "profiled_steps": prof_steps,
     }
     out["hbm_frac_alg"] = round(out["value"] * 320 / 1e9 / HBM_PEAK_GBS, 4)
     for prec, tag, R, mx in ((CC_FP64, "fp64", "double", "false"), (CC_MIXED, "mixed", "float", "true")):
@@ -224,8 +289,10 @@ def bench3d(N, warmup, steps, with_cpu):
                   "face": ["cc::k_face3d<%d, %s," % (a, mx) for a in range(3)],
                   "update": ["cc::k_update3d<%s>" % R], "edge": ["cc::k_edge3d<%s>" % R], "ct": ["cc::k_ct3d"],
                   "joule": ["cc::k_joule3d<true>", "cc::k_joule3d<false>"]}
-        out.setdefault("hbm_measured", {})[tag] = measured_hbm(res[prec]["prof"], groups)
-        out.setdefault("fusion_factor", {})[tag] = fusion_factor(out["hbm_measured"][tag], zu, 320)
+        hbm = measured_hbm(prof[prec], groups)
+        out.setdefault("hbm_measured", {})[tag] = hbm
+        zu_p = float(N) ** 3 * prof_steps
+        out.setdefault("fusion_factor", {})[tag] = fusion_factor(hbm, zu_p, 320)
     if with_cpu:
         out["cpu_baseline"] = cpu_baseline3d()
     return out
@@ -287,27 +354,23 @@ def bench_batch2d(n_inst, steps, warmup, nmin, nmax, seq_steps):
     }
 
 
-def run_gpu_sphere(N, precision, warmup, steps):
+def run_gpu_sphere(N, precision, warmup, steps, prof_steps):
     from core_collapse_amd import euler3d as E3
     from core_collapse_amd import ic
     p = ic.sphere3d_params(N, precision=precision)
     h = E3.Euler3D(p)
     h.upload(ic.sphere3d_ic(p))
-    t = h.run(warmup) if warmup > 0 else 0.0
-    h.set_profiling(True)
-    h.reset_profiling()
-    h.sync()
-    t0 = time.perf_counter()
-    t = h.run(steps, t)
-    elapsed = time.perf_counter() - t0
-    prof = {k: h.kernel_time(k) for k in ("prim", "face", "update", "mean", "fft_x", "fft_y", "fft_z", "check",
-                                          "grad", "dt")}
-    h.set_profiling(False)
+    if warmup > 0:  # warm the graphs, then restart from the IC
+        h.run(warmup)
+        h.upload(ic.sphere3d_ic(p))
+    t, elapsed = timed_run(h, steps)
     stats = h.poisson_stats()
     out = h.download()
     tot = h.totals()
+    _, prof = profiled_pass(h, prof_steps, t, ("prim", "face", "update", "mean", "fft_x", "fft_y", "fft_z", "check",
+                                               "grad", "dt"))
     h.close()
-    return {"elapsed": elapsed, "rho": out["rho"], "prof": prof, "poisson": stats, "mass": tot["mass"]}
+    return {"elapsed": elapsed, "state": out, "prof": prof, "poisson": stats, "mass": tot["mass"]}
 
 
 def cpu_baseline_sphere(n=128, budget_s=12.0):
@@ -330,11 +393,15 @@ def cpu_baseline_sphere(n=128, budget_s=12.0):
                       "single thread, oracle restatement (no reference program has this step)" % (steps, n, tsum)}
 
 
-def bench_sphere3d(N, warmup, steps, with_cpu):
+def bench_sphere3d(N, warmup, steps, prof_steps, with_cpu):
     """3D Euler HLLC + FFT self-gravity, uniform sphere, FP64 and mixed (FP32 primitives, fluxes and FFT;
-    FP64 state, update and Poisson residual)."""
+    FP64 state, update and Poisson residual), `steps` steps each from the IC."""
     from core_collapse_amd import CC_FP64, CC_MIXED
-    res = {prec: run_gpu_sphere(N, prec, warmup, steps) for prec in (CC_FP64, CC_MIXED)}
+    res = {}
+    for prec in (CC_FP64, CC_MIXED):
+        res[prec] = run_gpu_sphere(N, prec, warmup, steps, prof_steps)
+        log("sphere3d %s %.3f ms/step over %d steps" % ("fp64" if prec == CC_FP64 else "mixed",
+                                                         1e3 * res[prec]["elapsed"] / steps, steps))
     zu = float(N) ** 3 * steps
     cells = float(N) ** 3
 
@@ -358,7 +425,12 @@ def bench_sphere3d(N, warmup, steps, with_cpu):
         k = max(r["poisson"]["iters_last"], 1)
         return 392 + 192 * k
 
-    a, b = res[CC_MIXED]["rho"], res[CC_FP64]["rho"]
+    errs = {}
+    for f in ("rho", "mx", "my", "mz", "E"):
+        a, b = res[CC_MIXED]["state"][f], res[CC_FP64]["state"][f]
+        d = np.abs(a - b)
+        errs[f] = {"l1_rel": float(d.sum() / max(np.abs(b).sum(), 1e-300)),
+                   "linf_rel": float(d.max() / max(np.abs(b).max(), 1e-300))}
     out = {
         "workload": "3D Euler HLLC + FFT Poisson self-gravity, periodic %d^3, gamma 5/3, G 1, uniform sphere rho 10 "
                     "(r < 0.2) in 0.1, 1%% seeded noise, p 0.01, at rest; FFT refined in FP64 to 1e-10" % N,
@@ -368,23 +440,29 @@ def bench_sphere3d(N, warmup, steps, with_cpu):
         "fp64": {"value": zu / res[CC_FP64]["elapsed"], "ms_per_step": 1e3 * res[CC_FP64]["elapsed"] / steps,
                  "roofline": roof(res[CC_FP64], 8, "double"), "poisson": res[CC_FP64]["poisson"]},
         "mixed_over_fp64": res[CC_FP64]["elapsed"] / res[CC_MIXED]["elapsed"],
-        "l1_rho_rel_err": float(np.abs(a - b).sum() / np.abs(b).sum()),
-        "linf_rho_rel_err": float(np.abs(a - b).max() / np.abs(b).max()),
+        "errors": errs,
+        "l1_rho_rel_err": errs["rho"]["l1_rel"],
+        "linf_rho_rel_err": errs["rho"]["linf_rel"],
+        "mass_fp64": res[CC_FP64]["mass"], "mass_mixed": res[CC_MIXED]["mass"],
         "roofline": roof(res[CC_MIXED], 4, "float"),
         "poisson": res[CC_MIXED]["poisson"],
         "zone_update_alg_bytes": balg(res[CC_MIXED]),
         "kernel_times": {tag: {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in res[prec]["prof"].items()
                                if v[0] > 0} for prec, tag in ((CC_FP64, "fp64"), (CC_MIXED, "mixed"))},
+        "profiled_steps": prof_steps,
     }
     out["hbm_frac_alg"] = round(out["value"] * balg(res[CC_MIXED]) / 1e9 / HBM_PEAK_GBS, 4)
     for prec, tag, R in ((CC_FP64, "fp64", "double"), (CC_MIXED, "mixed", "float")):
         groups = {"prim": ["cc::k_e3_prim<%s>" % R], "face": tuple("cc::k_e3_face<%s, %d>" % (R, a) for a in range(3)),
                   "update": ["cc::k_e3_update<1, %s>" % R, "cc::k_e3_update<2, %s>" % R], "grad": ["cc::k_p3_grad"]}
-        out.setdefault("hbm_measured", {})[tag] = measured_hbm(res[prec]["prof"], groups)
+        hbm = measured_hbm(res[prec]["prof"], groups)
+        out.setdefault("hbm_measured", {})[tag] = hbm
         # hydro kernels only: B_alg's hydro share is 5 state passes of 40 B plus g, 224 B per zone-update
-        out.setdefault("fusion_factor_hydro", {})[tag] = fusion_factor(out["hbm_measured"][tag], zu, 224)
+        out.setdefault("fusion_factor_hydro", {})[tag] = fusion_factor(hbm, cells * prof_steps, 224)
     if with_cpu:
         out["cpu_baseline"] = cpu_baseline_sphere()
+    for prec in res:
+        res[prec]["state"] = None
     return out
 
 
@@ -454,72 +532,59 @@ def main():
         M2.sor_autotune(True)
     t3d = None
     if a.workload in ("all", "turb3d"):
-        print("bench: %d^3 turbulence, warmup %d, timed %d steps" % (a.N3, 2, a.steps3), file=sys.stderr, flush=True)
-        t3d = bench3d(a.N3, 2, a.steps3, not a.no_cpu)
-        print("bench: turb3d mixed %.3f ms/step, fp64 %.3f ms/step" % (t3d["ms_per_step"], t3d["fp64"]["ms_per_step"]),
-              file=sys.stderr, flush=True)
+        log("%d^3 turbulence, %d steps per precision" % (a.N3, a.steps3))
+        t3d = bench3d(a.N3, 2, a.steps3, a.profile_steps, not a.no_cpu)
         if a.workload == "turb3d":
             print(json.dumps({"turb3d": t3d}))
             return
     sp = None
     if a.workload in ("all", "sphere3d"):
-        print("bench: %d^3 sphere, warmup %d, timed %d steps" % (a.N5, 2, a.steps5), file=sys.stderr, flush=True)
-        sp = bench_sphere3d(a.N5, 2, a.steps5, not a.no_cpu)
-        print("bench: sphere3d mixed %.3f ms/step, fp64 %.3f ms/step" % (sp["ms_per_step"], sp["fp64"]["ms_per_step"]),
-              file=sys.stderr, flush=True)
+        log("%d^3 sphere, %d steps per precision" % (a.N5, a.steps5))
+        sp = bench_sphere3d(a.N5, 2, a.steps5, a.profile_steps, not a.no_cpu)
         if a.workload == "sphere3d":
             print(json.dumps({"sphere3d": sp}))
             return
     bt = None
     if a.workload in ("all", "batch2d"):
-        print("bench: batch of %d collapses, timed %d steps" % (a.batch, a.steps_batch), file=sys.stderr, flush=True)
+        log("batch of %d collapses, timed %d steps" % (a.batch, a.steps_batch))
         bt = bench_batch2d(a.batch, a.steps_batch, 3, 128, 1024, a.seq_steps)
-        print("bench: batch2d %.3f ms/step (sequential %s ms/step)" % (bt["ms_per_step"],
-              bt["sequential_estimate"]["ms_per_step"]), file=sys.stderr, flush=True)
+        log("batch2d %.3f ms/step (sequential %s ms/step)" % (bt["ms_per_step"], bt["sequential_estimate"]["ms_per_step"]))
         if a.workload == "batch2d":
             print(json.dumps({"batch2d": bt}))
             return
-    print("bench: %dx%d collapse, warmup %d, timed %d steps" % (N, N, W, K), file=sys.stderr, flush=True)
-    res = {}
-    for prec in (CC_FP64, CC_MIXED):
-        res[prec] = run_gpu(N, prec, W, K)
-        r = res[prec]
-        print("bench: %s %.3f ms/step" % ("fp64" if prec == CC_FP64 else "mixed", 1e3 * r["elapsed"] / K),
-              file=sys.stderr, flush=True)
+    log("%dx%d collapse, warmup %d, timed %d steps per precision, then %d-step runs compared" % (N, N, W, K,
+                                                                                                 a.accuracy_steps))
+    res, acc = bench_collapse2d(N, W, K, a.accuracy_steps, a.profile_steps)
     zone_updates = float(N) * N * K
     v64 = zone_updates / res[CC_FP64]["elapsed"]
     vmx = zone_updates / res[CC_MIXED]["elapsed"]
-    rho64, rhomx = res[CC_FP64]["rho"], res[CC_MIXED]["rho"]
-    l1 = float(np.abs(rhomx - rho64).sum() / np.abs(rho64).sum())
-    linf = float(np.abs(rhomx - rho64).max() / np.abs(rho64).max())
 
-    def roof(r, bytes_per_sweep):
+    def roof(r, bytes_per_sweep, traffic_name):
         n, ms = r["prof"]["sor"]
         if n == 0 or ms <= 0:
             return None
-        sweeps = sor_cell_sweeps_per_solve(N) * 2 * K  # two solves per step
+        sweeps = sor_cell_sweeps_per_solve(N) * 2 * a.profile_steps  # two solves per step
         bytes_per_launch = sweeps * bytes_per_sweep / n
         avg_s = ms / n * 1e-3
         ach = bytes_per_launch / avg_s / 1e9
+        tr = load_traffic(traffic_name)
         return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": None, "launches": n, "avg_launch_us": round(avg_s * 1e6, 2),
-                "alg_bytes_per_launch": bytes_per_launch}
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": tr,
+                "traffic_frac": round(tr / avg_s / 1e9 / HBM_PEAK_GBS, 4) if tr else None,
+                "kernel": traffic_name.replace("cc::", ""), "launches": n, "avg_launch_us": round(avg_s * 1e6, 2),
+                "alg_bytes_per_launch": bytes_per_launch,
+                "note": "algorithmic rate of a temporally blocked solver (16 sweeps per tile pass through LDS): "
+                        "achieved can exceed the HBM peak; traffic_frac is the counter-measured HBM fraction"}
 
-    def sor_kernel(tile):
+    def sor_name(tile, prec):
         S, WJ, sheared = tile
-        return ("k_sor_s" if sheared else "k_sor_") + ("tile" if os.environ.get("CC_SOR_LEVELS") else "flow")
+        form = ("k_sor_s" if sheared else "k_sor_") + ("tile" if os.environ.get("CC_SOR_LEVELS") else "flow")
+        if prec == CC_MIXED:
+            return "cc::%s_mixed<%d, %d>" % (form, S, WJ)
+        return "cc::%s_fp64<%d, %d%s" % (form, S, WJ, ">" if sheared else ",")
 
-    roof_mx = roof(res[CC_MIXED], 12.0)
-    roof_64 = roof(res[CC_FP64], 24.0)
-    if roof_mx:
-        tl = res[CC_MIXED]["tile"]
-        roof_mx["kernel"] = sor_kernel(tl)
-        roof_mx["traffic"] = load_traffic("cc::%s_mixed<%d, %d>" % (sor_kernel(tl), tl[0], tl[1]))
-    if roof_64:
-        tl = res[CC_FP64]["tile"]
-        roof_64["kernel"] = sor_kernel(tl)
-        roof_64["traffic"] = load_traffic("cc::%s_fp64<%d, %d%s" % (sor_kernel(tl), tl[0], tl[1], ">" if tl[2] else ","))
+    roof_mx = roof(res[CC_MIXED], 12.0, sor_name(res[CC_MIXED]["tile"], CC_MIXED))
+    roof_64 = roof(res[CC_FP64], 24.0, sor_name(res[CC_FP64]["tile"], CC_FP64))
     kernels = {}
     for prec, tag in ((CC_FP64, "fp64"), (CC_MIXED, "mixed")):
         kernels[tag] = {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in res[prec]["prof"].items()
@@ -539,19 +604,20 @@ def main():
         "dtype": "f32 fluxes/SOR sweeps + f64 update/CT/residual",
         "data": "synthetic",
         "config": {"workload": "2D resistive MHD HLLD+CT+SOR self-gravity, Gaussian collapse %dx%d, clamped "
-                               "boundaries, 600 SOR sweeps x 2 solves per Heun step" % (N, N),
+                               "boundaries, 600 SOR sweeps x 2 solves per Heun step, compensated face update" % (N, N),
                    "grid": [N, N], "gamma": 5.0 / 3.0, "G": 1.0, "eta": 1e-4, "cfl": 0.4, "sor_iters": 600,
                    "sor_omega": 1.8, "seed": 42},
-        "fp64": {"value": v64, "ms_per_step": 1e3 * res[CC_FP64]["elapsed"] / K, "roofline": roof_64,
-                 "max_divB": res[CC_FP64]["divb"]},
+        "fp64": {"value": v64, "ms_per_step": 1e3 * res[CC_FP64]["elapsed"] / K, "roofline": roof_64},
         "mixed_over_fp64": vmx / v64,
-        "l1_rho_rel_err": l1,
-        "linf_rho_rel_err": linf,
-        "max_divB": res[CC_MIXED]["divb"],
+        "accuracy": acc,
+        "l1_rho_rel_err": acc["errors"]["rho"]["l1_rel"] if acc else None,
+        "linf_rho_rel_err": acc["errors"]["rho"]["linf_rel"] if acc else None,
+        "max_divB": acc["mixed"]["max_divB"] if acc else None,
         "roofline": roof_mx,
         "cpu_baseline": cpu,
         "sor_tile": {"fp64": res[CC_FP64]["tile"], "mixed": res[CC_MIXED]["tile"]},
         "kernel_times": kernels,
+        "profiled_steps": a.profile_steps,
         "turb3d": t3d,
         "batch2d": bt,
         "sphere3d": sp,

@@ -30,7 +30,8 @@ class Mhd2dParams(C.Structure):
                 ("y0", C.c_double), ("y1", C.c_double), ("Nx", C.c_int), ("Ny", C.c_int),
                 ("floor_rho", C.c_double), ("floor_P", C.c_double), ("periodic_x", C.c_int),
                 ("periodic_y", C.c_int), ("eta", C.c_double), ("gravity_on", C.c_int), ("G", C.c_double),
-                ("sor_iters", C.c_int), ("sor_omega", C.c_double), ("precision", C.c_int)]
+                ("sor_iters", C.c_int), ("sor_omega", C.c_double), ("precision", C.c_int),
+                ("hlld_guard", C.c_double), ("ct_compensated", C.c_int)]
 
 
 class Mhd3dParams(C.Structure):
@@ -40,7 +41,8 @@ class Mhd3dParams(C.Structure):
                [("Nx", C.c_int), ("Ny", C.c_int), ("Nz", C.c_int), ("floor_rho", C.c_double),
                 ("floor_P", C.c_double), ("periodic_x", C.c_int), ("periodic_y", C.c_int),
                 ("periodic_z", C.c_int), ("eta", C.c_double), ("eos", C.c_int), ("gamma", C.c_double),
-                ("precision", C.c_int), ("degen_tau", C.c_double)]
+                ("precision", C.c_int), ("degen_tau", C.c_double), ("hlld_guard", C.c_double),
+                ("ct_compensated", C.c_int)]
 
 
 class Euler2dParams(C.Structure):

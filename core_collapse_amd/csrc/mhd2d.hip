@@ -30,11 +30,14 @@ struct Geom2D {
     int Nx, Ny, px, py;
     double dx, dy, gamma, gm1, floor_rho, floor_P, eta, cfl;
     int gravity_on;
+    int ct_comp;        // compensated face update (FP32 low words of the face field)
+    double hlld_guard;  // HLLD degeneracy guard (0: the reference's 1e-14 only)
 };
 
 struct State2D {
     double *rho, *mx, *my, *mz, *E, *Bz;  // Nx*Ny
     double *Bx, *By;                      // (Nx+1)*Ny, Nx*(Ny+1)
+    float *Bx_lo, *By_lo;                 // rounding errors of Bx, By (compensated face update only)
 };
 
 // ------------------------------------------------------------------ cell helpers (FP64)
@@ -81,9 +84,9 @@ __device__ __forceinline__ void face_hlld2d(const Geom2D& g, const Prim<double>&
     const IdealEOS2D<double> e64{g.gm1, g.gamma, g.floor_rho};
     if (MIXED) {
         const IdealEOS2D<float> e32{(float)g.gm1, (float)g.gamma, (float)g.floor_rho};
-        hlld_x_mixed(e32, e64, WL, WR, Bn, F);
+        hlld_x_mixed(e32, e64, WL, WR, Bn, F, kDegenerateTau, g.hlld_guard);
     } else {
-        hlld_x<double>(e64, WL, WR, Bn, fabs(Bn) < 1e-12, F);
+        hlld_x<double>(e64, WL, WR, Bn, fabs(Bn) < 1e-12, F, nullptr, 0.0, g.hlld_guard);
     }
 }
 
@@ -197,19 +200,30 @@ __global__ void k_corner2d(Geom2D g, const double* __restrict__ ezx, const doubl
 }
 
 // ------------------------------------------------------------------ CT + sources (+ Heun) kernel
-// B1 of x-face (fi, fj) and y-face (fi, fj) from the corner EMFs (M2:807-830).
-__device__ __forceinline__ double ct_bx(const Geom2D& g, const double* Bx0, const double* ez, int fi, int fj, double dt) {
-    return Bx0[(size_t)fj * (g.Nx + 1) + fi] +
-           (-(dt / g.dy)) * (ez[(size_t)(fj + 1) * (g.Nx + 1) + fi] - ez[(size_t)fj * (g.Nx + 1) + fi]);
+// Increments of x-face (fi, fj) and y-face (fi, fj) from the corner EMFs (M2:807-830).
+__device__ __forceinline__ double ct_dbx(const Geom2D& g, const double* ez, int fi, int fj, double dt) {
+    return (-(dt / g.dy)) * (ez[(size_t)(fj + 1) * (g.Nx + 1) + fi] - ez[(size_t)fj * (g.Nx + 1) + fi]);
 }
-__device__ __forceinline__ double ct_by(const Geom2D& g, const double* By0, const double* ez, int fi, int fj, double dt) {
-    return By0[(size_t)fj * g.Nx + fi] +
-           (dt / g.dx) * (ez[(size_t)fj * (g.Nx + 1) + fi + 1] - ez[(size_t)fj * (g.Nx + 1) + fi]);
+__device__ __forceinline__ double ct_dby(const Geom2D& g, const double* ez, int fi, int fj, double dt) {
+    return (dt / g.dx) * (ez[(size_t)fj * (g.Nx + 1) + fi + 1] - ez[(size_t)fj * (g.Nx + 1) + fi]);
 }
+// B1 of a face: the reference's B0 + increment, or its compensated form (hi word; lo into *lo).
+__device__ __forceinline__ double ct_face(const Geom2D& g, const double* B0, const float* B0lo, size_t f, double d,
+                                          float* lo = nullptr) {
+    if (!g.ct_comp) return B0[f] + d;
+    double h;
+    float l;
+    ct_add(B0[f], B0lo[f], d, h, l);
+    if (lo) *lo = l;
+    return h;
+}
+__device__ __forceinline__ size_t fxi(const Geom2D& g, int fi, int fj) { return (size_t)fj * (g.Nx + 1) + fi; }
+__device__ __forceinline__ size_t fyi(const Geom2D& g, int fi, int fj) { return (size_t)fj * g.Nx + fi; }
 
 template <bool FINAL>
 __global__ __launch_bounds__(256) void k_ct_src2d(Geom2D g, State2D Us, const double* __restrict__ Bx0,
-                                                  const double* __restrict__ By0, const double* __restrict__ ez,
+                                                  const double* __restrict__ By0, const float* __restrict__ Bx0lo,
+                                                  const float* __restrict__ By0lo, const double* __restrict__ ez,
                                                   const double* __restrict__ gx, const double* __restrict__ gy,
                                                   Scalars* __restrict__ sc, State2D Un, State2D Out,
                                                   double* __restrict__ spd) {
@@ -220,8 +234,13 @@ __global__ __launch_bounds__(256) void k_ct_src2d(Geom2D g, State2D Us, const do
     if (active) {
         const size_t q = (size_t)j * Nx + i;
         const double dt = sc->dt;
-        const double bxL = ct_bx(g, Bx0, ez, i, j, dt), bxR = ct_bx(g, Bx0, ez, i + 1, j, dt);
-        const double byD = ct_by(g, By0, ez, i, j, dt), byU = ct_by(g, By0, ez, i, j + 1, dt);
+        const double dxL = ct_dbx(g, ez, i, j, dt), dxR = ct_dbx(g, ez, i + 1, j, dt);
+        const double dyD = ct_dby(g, ez, i, j, dt), dyU = ct_dby(g, ez, i, j + 1, dt);
+        float lxL = 0.f, lxR = 0.f, lyD = 0.f, lyU = 0.f;
+        const double bxL = ct_face(g, Bx0, Bx0lo, fxi(g, i, j), dxL, &lxL);
+        const double bxR = ct_face(g, Bx0, Bx0lo, fxi(g, i + 1, j), dxR, &lxR);
+        const double byD = ct_face(g, By0, By0lo, fyi(g, i, j), dyD, &lyD);
+        const double byU = ct_face(g, By0, By0lo, fyi(g, i, j + 1), dyU, &lyU);
         const double bx1 = 0.5 * (bxL + bxR), by1 = 0.5 * (byD + byU);
         double rho = Us.rho[q], mx = Us.mx[q], my = Us.my[q], mz = Us.mz[q], E = Us.E[q], Bz = Us.Bz[q];
         if (g.gravity_on) {  // add_gravity_sources, M2:602-629
@@ -239,8 +258,8 @@ __global__ __launch_bounds__(256) void k_ct_src2d(Geom2D g, State2D Us, const do
             const double BzN = Us.Bz[(size_t)jN * Nx + i], BzS = Us.Bz[(size_t)jS * Nx + i];
             const double dBzdy = (BzN - BzS) / (2 * g.dy);
             const double dBzdx = (BzE - BzW) / (2 * g.dx);
-            const double dBydx = (ct_by(g, By0, ez, iE, j, dt) - byD) / g.dx;
-            const double dBxdy = (ct_bx(g, Bx0, ez, i, jN, dt) - bxL) / g.dy;
+            const double dBydx = (ct_face(g, By0, By0lo, fyi(g, iE, j), ct_dby(g, ez, iE, j, dt)) - byD) / g.dx;
+            const double dBxdy = (ct_face(g, Bx0, Bx0lo, fxi(g, i, jN), ct_dbx(g, ez, i, jN, dt)) - bxL) / g.dy;
             const double Jx = dBzdy, Jy = -dBzdx, Jz = dBydx - dBxdy;
             const double J2 = Jx * Jx + Jy * Jy + Jz * Jz;
             E += dt * g.eta * J2;
@@ -255,6 +274,12 @@ __global__ __launch_bounds__(256) void k_ct_src2d(Geom2D g, State2D Us, const do
             if (i == Nx - 1) Out.Bx[(size_t)j * (Nx + 1) + Nx] = bxR;
             Out.By[(size_t)j * Nx + i] = byD;
             if (j == Ny - 1) Out.By[(size_t)Ny * Nx + i] = byU;
+            if (g.ct_comp) {
+                Out.Bx_lo[(size_t)j * (Nx + 1) + i] = lxL;
+                if (i == Nx - 1) Out.Bx_lo[(size_t)j * (Nx + 1) + Nx] = lxR;
+                Out.By_lo[(size_t)j * Nx + i] = lyD;
+                if (j == Ny - 1) Out.By_lo[(size_t)Ny * Nx + i] = lyU;
+            }
         } else {  // Heun average and floors, M2:848-868
             rho = 0.5 * (Un.rho[q] + rho);
             mx = 0.5 * (Un.mx[q] + mx);
@@ -262,10 +287,20 @@ __global__ __launch_bounds__(256) void k_ct_src2d(Geom2D g, State2D Us, const do
             mz = 0.5 * (Un.mz[q] + mz);
             E = 0.5 * (Un.E[q] + E);
             Bz = 0.5 * (Un.Bz[q] + Bz);
-            const double nbxL = 0.5 * (Un.Bx[(size_t)j * (Nx + 1) + i] + bxL);
-            const double nbxR = 0.5 * (Un.Bx[(size_t)j * (Nx + 1) + i + 1] + bxR);
-            const double nbyD = 0.5 * (Un.By[(size_t)j * Nx + i] + byD);
-            const double nbyU = 0.5 * (Un.By[(size_t)(j + 1) * Nx + i] + byU);
+            double nbxL, nbxR, nbyD, nbyU;
+            float nlxL = 0.f, nlxR = 0.f, nlyD = 0.f, nlyU = 0.f;
+            if (g.ct_comp) {  // 1/2 (B^n + B^2) of the compensated fields
+                const size_t fL = fxi(g, i, j), fR = fxi(g, i + 1, j), fD = fyi(g, i, j), fU = fyi(g, i, j + 1);
+                ct_avg(Un.Bx[fL], Un.Bx_lo[fL], bxL, lxL, nbxL, nlxL);
+                ct_avg(Un.Bx[fR], Un.Bx_lo[fR], bxR, lxR, nbxR, nlxR);
+                ct_avg(Un.By[fD], Un.By_lo[fD], byD, lyD, nbyD, nlyD);
+                ct_avg(Un.By[fU], Un.By_lo[fU], byU, lyU, nbyU, nlyU);
+            } else {
+                nbxL = 0.5 * (Un.Bx[(size_t)j * (Nx + 1) + i] + bxL);
+                nbxR = 0.5 * (Un.Bx[(size_t)j * (Nx + 1) + i + 1] + bxR);
+                nbyD = 0.5 * (Un.By[(size_t)j * Nx + i] + byD);
+                nbyU = 0.5 * (Un.By[(size_t)(j + 1) * Nx + i] + byU);
+            }
             const double nbx = 0.5 * (nbxL + nbxR), nby = 0.5 * (nbyD + nbyU);
             enforce_floors2d(g, rho, mx, my, mz, E, nbx, nby, Bz);
             Out.rho[q] = rho; Out.mx[q] = mx; Out.my[q] = my; Out.mz[q] = mz; Out.E[q] = E; Out.Bz[q] = Bz;
@@ -273,6 +308,12 @@ __global__ __launch_bounds__(256) void k_ct_src2d(Geom2D g, State2D Us, const do
             if (i == Nx - 1) Out.Bx[(size_t)j * (Nx + 1) + Nx] = nbxR;
             Out.By[(size_t)j * Nx + i] = nbyD;
             if (j == Ny - 1) Out.By[(size_t)Ny * Nx + i] = nbyU;
+            if (g.ct_comp) {
+                Out.Bx_lo[(size_t)j * (Nx + 1) + i] = nlxL;
+                if (i == Nx - 1) Out.Bx_lo[(size_t)j * (Nx + 1) + Nx] = nlxR;
+                Out.By_lo[(size_t)j * Nx + i] = nlyD;
+                if (j == Ny - 1) Out.By_lo[(size_t)Ny * Nx + i] = nlyU;
+            }
             // next step's compute_dt (M2:579-598) on the final state
             speed = signal_speed2d(g, cons_to_prim2d(g, rho, mx, my, mz, E, nbx, nby, Bz));
         }
@@ -416,10 +457,23 @@ struct cc_mhd2d {
         CC_HIP(hipMemsetAsync(*ptr, 0, n * sizeof(double), stream));
         return CC_OK;
     }
-    cc_status alloc_state(State2D& s) {
+    cc_status alloc_f(float** ptr, size_t n) {
+        if (hipMalloc((void**)ptr, n * sizeof(float)) != hipSuccess) {
+            set_error(CC_ERR_OOM, "cc_mhd2d: device allocation failed");
+            return CC_ERR_OOM;
+        }
+        allocs.push_back(*ptr);
+        CC_HIP(hipMemsetAsync(*ptr, 0, n * sizeof(float), stream));
+        return CC_OK;
+    }
+    cc_status alloc_state(State2D& s, bool lo) {
         CC_TRY(alloc(&s.rho, N())); CC_TRY(alloc(&s.mx, N())); CC_TRY(alloc(&s.my, N()));
         CC_TRY(alloc(&s.mz, N())); CC_TRY(alloc(&s.E, N())); CC_TRY(alloc(&s.Bz, N()));
         CC_TRY(alloc(&s.Bx, NFx())); CC_TRY(alloc(&s.By, NFy()));
+        if (lo) {
+            CC_TRY(alloc_f(&s.Bx_lo, NFx()));
+            CC_TRY(alloc_f(&s.By_lo, NFy()));
+        }
         return CC_OK;
     }
 
@@ -445,7 +499,7 @@ struct cc_mhd2d {
         {
             ProfScope ps(pr(), stream, "ct");
             hipLaunchKernelGGL(k_ct_src2d<FINAL>, dim3((Nx + 255) / 256, Ny), dim3(256), 0, stream, g, Us, in.Bx, in.By,
-                               ez, gxp, gyp, sc, hn, out, spd);
+                               in.Bx_lo, in.By_lo, ez, gxp, gyp, sc, hn, out, spd);
             CC_CHECK_LAUNCH();
         }
         return CC_OK;
@@ -511,6 +565,10 @@ cc_status check_params2d(const cc_mhd2d_params* p) {
         set_error(CC_ERR_INVALID_ARG, "precision must be CC_FP64 or CC_MIXED");
         return CC_ERR_INVALID_ARG;
     }
+    if (!(p->hlld_guard >= 0.0 && p->hlld_guard < 1.0)) {
+        set_error(CC_ERR_INVALID_ARG, "hlld_guard must be in [0, 1)");
+        return CC_ERR_INVALID_ARG;
+    }
     return CC_OK;
 }
 }  // namespace
@@ -533,14 +591,15 @@ cc_status cc_mhd2d_create(const cc_mhd2d_params* p, cc_mhd2d** out) {
     g.dx = (p->x1 - p->x0) / p->Nx; g.dy = (p->y1 - p->y0) / p->Ny;
     g.gamma = p->gamma; g.gm1 = p->gamma - 1.0; g.floor_rho = p->floor_rho; g.floor_P = p->floor_P;
     g.eta = p->eta; g.cfl = p->cfl; g.gravity_on = p->gravity_on != 0;
+    g.ct_comp = p->ct_compensated != 0; g.hlld_guard = p->hlld_guard;
     cc_status st = CC_OK;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         set_error(CC_ERR_HIP, "hipStreamCreate failed");
         delete h;
         return CC_ERR_HIP;
     }
-    for (int k = 0; k < 3 && st == CC_OK; k++) st = h->alloc_state(h->S[k]);
-    if (st == CC_OK) st = h->alloc_state(h->Us);
+    for (int k = 0; k < 3 && st == CC_OK; k++) st = h->alloc_state(h->S[k], g.ct_comp);
+    if (st == CC_OK) st = h->alloc_state(h->Us, false);
     if (st == CC_OK) st = h->alloc(&h->ezx, h->NFx());
     if (st == CC_OK) st = h->alloc(&h->ezy, h->NFy());
     if (st == CC_OK) st = h->alloc(&h->ez, (size_t)(p->Nx + 1) * (p->Ny + 1));
@@ -604,6 +663,10 @@ cc_status cc_mhd2d_upload(cc_mhd2d* h, const double* rho, const double* mx, cons
     CC_HIP(hipMemcpyAsync(s.Bz, Bz, n, hipMemcpyHostToDevice, h->stream));
     CC_HIP(hipMemcpyAsync(s.Bx, Bx_f, h->NFx() * sizeof(double), hipMemcpyHostToDevice, h->stream));
     CC_HIP(hipMemcpyAsync(s.By, By_f, h->NFy() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if (h->g.ct_comp) {  // an uploaded face field is exact as given
+        CC_HIP(hipMemsetAsync(s.Bx_lo, 0, h->NFx() * sizeof(float), h->stream));
+        CC_HIP(hipMemsetAsync(s.By_lo, 0, h->NFy() * sizeof(float), h->stream));
+    }
     CC_HIP(hipStreamSynchronize(h->stream));
     return CC_OK;
 }

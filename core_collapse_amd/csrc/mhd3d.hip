@@ -29,7 +29,9 @@ namespace cc {
 struct Geom3D {
     int Nx, Ny, Nz, px, py, pz;
     double dx, dy, dz, floor_rho, floor_P, eta, cfl;
-    float tau;  // mixed precision: relative Alfven-degeneracy margin for the FP64 fallback
+    float tau;          // mixed precision: relative Alfven-degeneracy margin for the FP64 fallback
+    int ct_comp;        // compensated face update (FP32 low words of the face field)
+    double hlld_guard;  // HLLD degeneracy guard (0: the reference's 1e-14 only)
 };
 
 // FP64 EOS for pressure recovery and floors: hybrid (M3:114-212) or ideal.
@@ -42,6 +44,7 @@ struct Eos3D {
 struct State3D {
     double *rho, *mx, *my, *mz, *E;  // Nx*Ny*Nz, (k*Ny+j)*Nx+i
     double *Bx, *By, *Bz;            // (Nx+1)Ny Nz, Nx(Ny+1)Nz, Nx Ny(Nz+1)
+    float *Bx_lo, *By_lo, *Bz_lo;    // rounding errors of the faces (compensated face update only)
 };
 
 struct Edges {
@@ -210,15 +213,16 @@ __device__ __forceinline__ Prim<R> load_prim(const PrimArrays<R>& W, size_t q) {
 // is within tau of the Alfven degeneracy.
 template <typename EOS32, typename EOS64>
 __device__ __forceinline__ void hlld_x_mixed32(const EOS32& e32, const EOS64& e64, const Prim<float>& L,
-                                               const Prim<float>& R, double Bn, float out[7], float tau) {
+                                               const Prim<float>& R, double Bn, float out[7], float tau,
+                                               double guard) {
     const bool bn_small = fabs(Bn) < 1e-12;
     bool deg = false;
-    hlld_x<float>(e32, L, R, (float)Bn, bn_small, out, &deg, tau);
+    hlld_x<float>(e32, L, R, (float)Bn, bn_small, out, &deg, tau, (float)guard);
     if (deg) {
         const Prim<double> l{L.rho, L.u, L.v, L.w, L.P, L.Bx, L.By, L.Bz};
         const Prim<double> r{R.rho, R.u, R.v, R.w, R.P, R.Bx, R.By, R.Bz};
         double o[7];
-        hlld_x<double>(e64, l, r, Bn, bn_small, o);
+        hlld_x<double>(e64, l, r, Bn, bn_small, o, nullptr, 0.0, guard);
 #pragma unroll
         for (int c = 0; c < 7; ++c) out[c] = (float)o[c];
     }
@@ -228,16 +232,16 @@ __device__ __forceinline__ void hlld_x_mixed32(const EOS32& e32, const EOS64& e6
 // the mixed path's FP32 states and FP32 HLLD with its FP64 fallback.
 template <bool MIXED, typename EOS64, typename EOS32>
 __device__ __forceinline__ void face_hlld3d(const EOS64& e64, const EOS32& e32, const Prim<double>& WL,
-                                            const Prim<double>& WR, double Bn, double F[7], float tau) {
+                                            const Prim<double>& WR, double Bn, double F[7], float tau, double guard) {
     if constexpr (MIXED) {
         const Prim<float> l{(float)WL.rho, (float)WL.u, (float)WL.v, (float)WL.w, (float)WL.P, (float)WL.Bx, (float)WL.By, (float)WL.Bz};
         const Prim<float> r{(float)WR.rho, (float)WR.u, (float)WR.v, (float)WR.w, (float)WR.P, (float)WR.Bx, (float)WR.By, (float)WR.Bz};
         float o[7];
-        hlld_x_mixed32(e32, e64, l, r, Bn, o, tau);
+        hlld_x_mixed32(e32, e64, l, r, Bn, o, tau, guard);
 #pragma unroll
         for (int c = 0; c < 7; ++c) F[c] = o[c];
     } else {
-        hlld_x<double>(e64, WL, WR, Bn, fabs(Bn) < 1e-12, F);
+        hlld_x<double>(e64, WL, WR, Bn, fabs(Bn) < 1e-12, F, nullptr, 0.0, guard);
     }
 }
 
@@ -335,12 +339,12 @@ __global__ __launch_bounds__(256) void k_face3d(Geom3D g, EOS64 e64, EOS32 e32,
     R o[7];
     if constexpr (MIXED && AX != 2) {
         bool deg = false;
-        hlld_x<float>(e32, L, Rr, (float)Bn, fabs(Bn) < 1e-12, o, &deg, g.tau);
+        hlld_x<float>(e32, L, Rr, (float)Bn, fabs(Bn) < 1e-12, o, &deg, g.tau, (float)g.hlld_guard);
         if (deg) dq.face[AX][atomicAdd(&dq.count[AX], 1)] = (int)t;
     } else if constexpr (MIXED) {  // z faces: often at the degeneracy in the benchmark field, recomputed inline
-        hlld_x_mixed32(e32, e64, L, Rr, Bn, o, g.tau);
+        hlld_x_mixed32(e32, e64, L, Rr, Bn, o, g.tau, g.hlld_guard);
     } else {
-        hlld_x<double>(e64, L, Rr, Bn, fabs(Bn) < 1e-12, o);
+        hlld_x<double>(e64, L, Rr, Bn, fabs(Bn) < 1e-12, o, nullptr, 0.0, g.hlld_guard);
     }
     face_store3d<AX, R>(F, ef, t, o);
 }
@@ -358,7 +362,7 @@ __device__ __forceinline__ void face_fp64_one(const Geom3D& g, const EOS64& e64,
     const Prim<double> l{L.rho, L.u, L.v, L.w, L.P, L.Bx, L.By, L.Bz};
     const Prim<double> rr{Rr.rho, Rr.u, Rr.v, Rr.w, Rr.P, Rr.Bx, Rr.By, Rr.Bz};
     double o64[7];
-    hlld_x<double>(e64, l, rr, Bn, fabs(Bn) < 1e-12, o64);
+    hlld_x<double>(e64, l, rr, Bn, fabs(Bn) < 1e-12, o64, nullptr, 0.0, g.hlld_guard);
     float o[7];
 #pragma unroll
     for (int c = 0; c < 7; ++c) o[c] = (float)o64[c];
@@ -467,20 +471,32 @@ __global__ __launch_bounds__(256) void k_ct3d(Geom3D g, Edges ed, State3D B0, St
     auto EY = [&](int i, int j, int kf) { return ed.Ey[((size_t)kf * Ny + j) * (Nx + 1) + i]; };
     auto EZ = [&](int i, int jf, int k) { return ed.Ez[((size_t)k * (Ny + 1) + jf) * (Nx + 1) + i]; };
     // face indices j+1 <= Ny and k+1 <= Nz need no wrapping (imod(n, n+1) = n, clamp(n, 0, n) = n)
+    // B1 = B0 + increment: the reference's sum, or (compensated) the exact sum carried as hi + FP32 lo
+    auto put = [&](double* B1v, float* B1lo, const double* B0v, const float* B0lo, size_t f, double d) {
+        if (g.ct_comp) {
+            double hi;
+            float lo;
+            ct_add(B0v[f], B0lo[f], d, hi, lo);
+            B1v[f] = hi;
+            B1lo[f] = lo;
+        } else {
+            B1v[f] = B0v[f] + d;
+        }
+    };
     if (ja < Ny && ka < Nz) {
         const int i = ia, j = ja, k = ka;
-        const size_t f = fx_(g, i, j, k);
-        B1.Bx[f] = B0.Bx[f] + (-(dt / g.dy) * (EZ(i, j + 1, k) - EZ(i, j, k)) + (dt / g.dz) * (EY(i, j, k + 1) - EY(i, j, k)));
+        put(B1.Bx, B1.Bx_lo, B0.Bx, B0.Bx_lo, fx_(g, i, j, k),
+            -(dt / g.dy) * (EZ(i, j + 1, k) - EZ(i, j, k)) + (dt / g.dz) * (EY(i, j, k + 1) - EY(i, j, k)));
     }
     if (ia < Nx && ka < Nz) {
         const int i = ia, j = ja, k = ka;
-        const size_t f = fy_(g, i, j, k);
-        B1.By[f] = B0.By[f] + (+(dt / g.dx) * (EZ(i + 1, j, k) - EZ(i, j, k)) - (dt / g.dz) * (EX(i, j, k + 1) - EX(i, j, k)));
+        put(B1.By, B1.By_lo, B0.By, B0.By_lo, fy_(g, i, j, k),
+            +(dt / g.dx) * (EZ(i + 1, j, k) - EZ(i, j, k)) - (dt / g.dz) * (EX(i, j, k + 1) - EX(i, j, k)));
     }
     if (ia < Nx && ja < Ny) {
         const int i = ia, j = ja, k = ka;
-        const size_t f = fz_(g, i, j, k);
-        B1.Bz[f] = B0.Bz[f] + (-(dt / g.dx) * (EY(i + 1, j, k) - EY(i, j, k)) + (dt / g.dy) * (EX(i, j + 1, k) - EX(i, j, k)));
+        put(B1.Bz, B1.Bz_lo, B0.Bz, B0.Bz_lo, fz_(g, i, j, k),
+            -(dt / g.dx) * (EY(i + 1, j, k) - EY(i, j, k)) + (dt / g.dy) * (EX(i, j + 1, k) - EX(i, j, k)));
     }
 }
 
@@ -521,9 +537,22 @@ __global__ __launch_bounds__(256) void k_joule3d(Geom3D g, Eos3D e_in, State3D U
             my = 0.5 * (Un.my[q] + my);
             mz = 0.5 * (Un.mz[q] + mz);
             E = 0.5 * (Un.E[q] + E);
-            const double nxL = 0.5 * (Un.Bx[fx_(g, i, j, k)] + bxL), nxR = 0.5 * (Un.Bx[fx_(g, i + 1, j, k)] + bxR);
-            const double nyD = 0.5 * (Un.By[fy_(g, i, j, k)] + byD), nyU = 0.5 * (Un.By[fy_(g, i, j + 1, k)] + byU);
-            const double nzB = 0.5 * (Un.Bz[fz_(g, i, j, k)] + bzB), nzF = 0.5 * (Un.Bz[fz_(g, i, j, k + 1)] + bzF);
+            double nxL, nxR, nyD, nyU, nzB, nzF;
+            float lxL = 0.f, lxR = 0.f, lyD = 0.f, lyU = 0.f, lzB = 0.f, lzF = 0.f;
+            const size_t fL = fx_(g, i, j, k), fR = fx_(g, i + 1, j, k), fD = fy_(g, i, j, k), fU = fy_(g, i, j + 1, k);
+            const size_t fB = fz_(g, i, j, k), fF = fz_(g, i, j, k + 1);
+            if (g.ct_comp) {  // 1/2 (B^n + B^2) of the compensated fields
+                ct_avg(Un.Bx[fL], Un.Bx_lo[fL], bxL, B1.Bx_lo[fL], nxL, lxL);
+                ct_avg(Un.Bx[fR], Un.Bx_lo[fR], bxR, B1.Bx_lo[fR], nxR, lxR);
+                ct_avg(Un.By[fD], Un.By_lo[fD], byD, B1.By_lo[fD], nyD, lyD);
+                ct_avg(Un.By[fU], Un.By_lo[fU], byU, B1.By_lo[fU], nyU, lyU);
+                ct_avg(Un.Bz[fB], Un.Bz_lo[fB], bzB, B1.Bz_lo[fB], nzB, lzB);
+                ct_avg(Un.Bz[fF], Un.Bz_lo[fF], bzF, B1.Bz_lo[fF], nzF, lzF);
+            } else {
+                nxL = 0.5 * (Un.Bx[fL] + bxL); nxR = 0.5 * (Un.Bx[fR] + bxR);
+                nyD = 0.5 * (Un.By[fD] + byD); nyU = 0.5 * (Un.By[fU] + byU);
+                nzB = 0.5 * (Un.Bz[fB] + bzB); nzF = 0.5 * (Un.Bz[fF] + bzF);
+            }
             const double bx = 0.5 * (nxL + nxR), by = 0.5 * (nyD + nyU), bz = 0.5 * (nzB + nzF);
             floors3d(e, rho, mx, my, mz, E, bx, by, bz);
             Out.rho[q] = rho; Out.mx[q] = mx; Out.my[q] = my; Out.mz[q] = mz; Out.E[q] = E;
@@ -533,6 +562,14 @@ __global__ __launch_bounds__(256) void k_joule3d(Geom3D g, Eos3D e_in, State3D U
             if (j == Ny - 1) Out.By[fy_(g, i, Ny, k)] = nyU;
             Out.Bz[fz_(g, i, j, k)] = nzB;
             if (k == Nz - 1) Out.Bz[fz_(g, i, j, Nz)] = nzF;
+            if (g.ct_comp) {
+                Out.Bx_lo[fL] = lxL;
+                if (i == Nx - 1) Out.Bx_lo[fR] = lxR;
+                Out.By_lo[fD] = lyD;
+                if (j == Ny - 1) Out.By_lo[fU] = lyU;
+                Out.Bz_lo[fB] = lzB;
+                if (k == Nz - 1) Out.Bz_lo[fF] = lzF;
+            }
             speed = signal_speed3d(e, cons_to_prim3d(e, rho, mx, my, mz, E, bx, by, bz));
         }
     }
@@ -604,7 +641,7 @@ __global__ __launch_bounds__(256) void k_reduce_max_to(const double* spd, int ns
 }
 
 template <bool MIXED, typename EOS64, typename EOS32>
-__global__ void k_hlld3d_batch(EOS64 e64, EOS32 e32, float tau, int n, const double* __restrict__ WLa,
+__global__ void k_hlld3d_batch(EOS64 e64, EOS32 e32, float tau, double guard, int n, const double* __restrict__ WLa,
                                const double* __restrict__ WRa, const double* __restrict__ Bna, double* __restrict__ out) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -613,7 +650,7 @@ __global__ void k_hlld3d_batch(EOS64 e64, EOS32 e32, float tau, int n, const dou
     const Prim<double> WL{a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7]};
     const Prim<double> WR{b[0], b[1], b[2], b[3], b[4], b[5], b[6], b[7]};
     double F[7];
-    face_hlld3d<MIXED>(e64, e32, WL, WR, Bna[t], F, tau);
+    face_hlld3d<MIXED>(e64, e32, WL, WR, Bna[t], F, tau, guard);
     for (int c = 0; c < 7; c++) out[7 * (size_t)t + c] = F[c];
 }
 
@@ -698,6 +735,11 @@ struct cc_mhd3d {
     }
     cc_status alloc_faces(State3D& s) {
         CC_TRY(alloc(&s.Bx, NFx())); CC_TRY(alloc(&s.By, NFy())); CC_TRY(alloc(&s.Bz, NFz()));
+        if (g.ct_comp) {
+            CC_TRY(alloc_bytes((void**)&s.Bx_lo, NFx() * sizeof(float)));
+            CC_TRY(alloc_bytes((void**)&s.By_lo, NFy() * sizeof(float)));
+            CC_TRY(alloc_bytes((void**)&s.Bz_lo, NFz() * sizeof(float)));
+        }
         return CC_OK;
     }
 
@@ -777,7 +819,7 @@ struct cc_mhd3d {
             CC_CHECK_LAUNCH();
         }
         if constexpr (MIXED) {
-            ProfScope ps(pr(), stream, "face");
+            ProfScope ps(pr(), stream, "face_fp64");
             hipLaunchKernelGGL((k_face3d_fp64<EOS64>), dim3(2048), dim3(256), 0, stream, g, e64, W<float>(), in.Bx, in.By,
                                in.Bz, FF<float>(0), FF<float>(1), FF<float>(2), EF<float>(), dq);
             CC_CHECK_LAUNCH();
@@ -872,6 +914,7 @@ cc_status cc_mhd3d_create(const cc_mhd3d_params* p, cc_mhd3d** out) {
     if (p->precision != CC_FP64 && p->precision != CC_MIXED) { set_error(CC_ERR_INVALID_ARG, "bad precision"); return CC_ERR_INVALID_ARG; }
     if (p->eos != 0 && p->eos != 1) { set_error(CC_ERR_INVALID_ARG, "eos must be 0 (hybrid) or 1 (ideal)"); return CC_ERR_INVALID_ARG; }
     if (p->eos == 1 && !(p->gamma > 1.0)) { set_error(CC_ERR_INVALID_ARG, "ideal EOS needs gamma > 1"); return CC_ERR_INVALID_ARG; }
+    if (!(p->hlld_guard >= 0.0 && p->hlld_guard < 1.0)) { set_error(CC_ERR_INVALID_ARG, "hlld_guard must be in [0, 1)"); return CC_ERR_INVALID_ARG; }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { set_error(CC_ERR_NO_DEVICE, "no HIP device"); return CC_ERR_NO_DEVICE; }
     cc_mhd3d* h = new cc_mhd3d();
@@ -881,6 +924,8 @@ cc_status cc_mhd3d_create(const cc_mhd3d_params* p, cc_mhd3d** out) {
     g.dx = (p->x1 - p->x0) / p->Nx; g.dy = (p->y1 - p->y0) / p->Ny; g.dz = (p->z1 - p->z0) / p->Nz;
     g.floor_rho = p->floor_rho; g.floor_P = p->floor_P; g.eta = p->eta; g.cfl = p->cfl;
     g.tau = p->degen_tau > 0.0 ? (float)p->degen_tau : kDegenerateTau;
+    g.ct_comp = p->ct_compensated != 0;
+    g.hlld_guard = p->hlld_guard;
     h->e.ideal = p->eos == 1;
     h->e.gamma = p->gamma;
     h->e.hy = make_hybrid<double>(p);
@@ -947,6 +992,11 @@ cc_status cc_mhd3d_upload(cc_mhd3d* h, const double* rho, const double* mx, cons
     CC_HIP(hipMemcpyAsync(s.Bx, Bx_f, h->NFx() * 8, hipMemcpyHostToDevice, h->stream));
     CC_HIP(hipMemcpyAsync(s.By, By_f, h->NFy() * 8, hipMemcpyHostToDevice, h->stream));
     CC_HIP(hipMemcpyAsync(s.Bz, Bz_f, h->NFz() * 8, hipMemcpyHostToDevice, h->stream));
+    if (h->g.ct_comp) {  // an uploaded face field is exact as given
+        CC_HIP(hipMemsetAsync(s.Bx_lo, 0, h->NFx() * sizeof(float), h->stream));
+        CC_HIP(hipMemsetAsync(s.By_lo, 0, h->NFy() * sizeof(float), h->stream));
+        CC_HIP(hipMemsetAsync(s.Bz_lo, 0, h->NFz() * sizeof(float), h->stream));
+    }
     CC_HIP(hipStreamSynchronize(h->stream));
     return CC_OK;
 }
@@ -1136,16 +1186,16 @@ cc_status cc_hlld3d_batch(const cc_mhd3d_params* p, int n, const double* WL, con
         const IdealEOS3D<double> e64{p->gamma - 1.0, p->gamma, p->floor_rho};
         const IdealEOS3D<float> e32{(float)(p->gamma - 1.0), (float)p->gamma, (float)p->floor_rho};
         if (mixed)
-            hipLaunchKernelGGL((k_hlld3d_batch<true, IdealEOS3D<double>, IdealEOS3D<float>>), grid, dim3(256), 0, 0, e64, e32, tau, n, dL, dR, dB, dO);
+            hipLaunchKernelGGL((k_hlld3d_batch<true, IdealEOS3D<double>, IdealEOS3D<float>>), grid, dim3(256), 0, 0, e64, e32, tau, p->hlld_guard, n, dL, dR, dB, dO);
         else
-            hipLaunchKernelGGL((k_hlld3d_batch<false, IdealEOS3D<double>, IdealEOS3D<float>>), grid, dim3(256), 0, 0, e64, e32, tau, n, dL, dR, dB, dO);
+            hipLaunchKernelGGL((k_hlld3d_batch<false, IdealEOS3D<double>, IdealEOS3D<float>>), grid, dim3(256), 0, 0, e64, e32, tau, p->hlld_guard, n, dL, dR, dB, dO);
     } else {
         const HybridEOS<double> e64 = make_hybrid<double>(p);
         const HybridEOS<float> e32 = make_hybrid<float>(p);
         if (mixed)
-            hipLaunchKernelGGL((k_hlld3d_batch<true, HybridEOS<double>, HybridEOS<float>>), grid, dim3(256), 0, 0, e64, e32, tau, n, dL, dR, dB, dO);
+            hipLaunchKernelGGL((k_hlld3d_batch<true, HybridEOS<double>, HybridEOS<float>>), grid, dim3(256), 0, 0, e64, e32, tau, p->hlld_guard, n, dL, dR, dB, dO);
         else
-            hipLaunchKernelGGL((k_hlld3d_batch<false, HybridEOS<double>, HybridEOS<float>>), grid, dim3(256), 0, 0, e64, e32, tau, n, dL, dR, dB, dO);
+            hipLaunchKernelGGL((k_hlld3d_batch<false, HybridEOS<double>, HybridEOS<float>>), grid, dim3(256), 0, 0, e64, e32, tau, p->hlld_guard, n, dL, dR, dB, dO);
     }
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpy(out, dO, (size_t)n * 7 * 8, hipMemcpyDeviceToHost);

@@ -155,9 +155,13 @@ __host__ __device__ __forceinline__ void flux_ideal_x(const Prim<R>& W, R E, R F
 // Alfven degeneracy dL = 0 / dR = 0 (M2:419-442), where the star states are ill-conditioned and a
 // low-precision evaluation cannot be trusted; the caller then recomputes the face in FP64 (the outputs
 // of a degenerate face are zeros: the solver stops there).
+// `guard` > 0 (no reference counterpart; 0 is the reference) sends a face whose d_L or d_R is below
+// guard * max(Bn^2, |rho s (S - S_M)|) to the reference's own HLL flux (the |Bn| < 1e-12 branch):
+// the reference divides the star states by d with only |d| > 1e-14 as a guard, so a face whose outer
+// wave speed nearly meets the star-region Alfven speed gets star states of O(100) and more.
 template <typename R, typename EOS>
 __host__ __device__ __forceinline__ void hlld_x(const EOS& eos, Prim<R> WL, Prim<R> WR, R Bn, bool bn_small, R out[7],
-                                                bool* degenerate = nullptr, R tau = R(0)) {
+                                                bool* degenerate = nullptr, R tau = R(0), R guard = R(0)) {
     WL.Bx = Bn;
     WR.Bx = Bn;
     const R rhoL = WL.rho, rhoR = WR.rho, uL = WL.u, uR = WR.u;
@@ -180,12 +184,15 @@ __host__ __device__ __forceinline__ void hlld_x(const EOS& eos, Prim<R> WL, Prim
         for (int k = 0; k < 7; k++) out[k] = FR[k];
         return;
     }
-    if (bn_small) {  // M2:449-467 (HLL)
+    auto hll = [&]() {  // M2:449-467 (HLL)
         R UL[7] = {WL.rho, WL.rho * WL.u, WL.rho * WL.v, WL.rho * WL.w, EL, WL.By, WL.Bz};
         R UR[7] = {WR.rho, WR.rho * WR.u, WR.rho * WR.v, WR.rho * WR.w, ER, WR.By, WR.Bz};
         R inv = R(1) / (SR - SL);
 #pragma unroll
         for (int k = 0; k < 7; k++) out[k] = (SR * FL[k] - SL * FR[k] + SL * SR * (UR[k] - UL[k])) * inv;
+    };
+    if (bn_small) {
+        hll();
         return;
     }
     const R BL2 = WL.Bx * WL.Bx + WL.By * WL.By + WL.Bz * WL.Bz;
@@ -211,6 +218,11 @@ __host__ __device__ __forceinline__ void hlld_x(const EOS& eos, Prim<R> WL, Prim
             for (int k = 0; k < 7; k++) out[k] = R(0);
             return;
         }
+    }
+    if (guard > R(0) && (fabs(dL) < guard * smax(Bn2, R(fabs(rhoL * sL * (SL - SM)))) ||
+                         fabs(dR) < guard * smax(Bn2, R(fabs(rhoR * sR * (SR - SM)))))) {
+        hll();
+        return;
     }
 
     R ByL_s = WL.By, BzL_s = WL.Bz, ByR_s = WR.By, BzR_s = WR.Bz;
@@ -309,15 +321,15 @@ constexpr float kDegenerateTau = 2e-2f;
 template <typename EOS32, typename EOS64>
 __host__ __device__ __forceinline__ void hlld_x_mixed(const EOS32& e32, const EOS64& e64, const Prim<double>& WL,
                                                       const Prim<double>& WR, double Bn, double out[7],
-                                                      float tau = kDegenerateTau) {
+                                                      float tau = kDegenerateTau, double guard = 0.0) {
     const bool bn_small = fabs(Bn) < 1e-12;
     const Prim<float> L{(float)WL.rho, (float)WL.u, (float)WL.v, (float)WL.w, (float)WL.P, (float)WL.Bx, (float)WL.By, (float)WL.Bz};
     const Prim<float> R{(float)WR.rho, (float)WR.u, (float)WR.v, (float)WR.w, (float)WR.P, (float)WR.Bx, (float)WR.By, (float)WR.Bz};
     float F[7];
     bool deg = false;
-    hlld_x<float>(e32, L, R, (float)Bn, bn_small, F, &deg, tau);
+    hlld_x<float>(e32, L, R, (float)Bn, bn_small, F, &deg, tau, (float)guard);
     if (deg) {
-        hlld_x<double>(e64, WL, WR, Bn, bn_small, out);
+        hlld_x<double>(e64, WL, WR, Bn, bn_small, out, nullptr, 0.0, guard);
     } else {
 #pragma unroll
         for (int k = 0; k < 7; k++) out[k] = (double)F[k];

@@ -53,4 +53,33 @@ __device__ __forceinline__ void finalize_dt(Scalars* sc, double maxs, double h, 
 
 __global__ void k_set_scalars(Scalars* sc, double dt, double t, double t_end, int reset_step);
 
+// ------------------------------------------------------------------ compensated face-field update
+// With compensation on, each face carries the rounding error of its stored FP64 value as an FP32 low
+// word (hi + lo is the field to about 1e-24 relative), so the discrete divergence of the stored field
+// no longer accumulates one rounding per face per step: hi stays the correctly rounded face value and
+// max |div B| stays at the size of one rounding (<= 2 ulp(B) / h per axis) over any number of steps.
+// The increments are the reference's (M2:807-830, M3:941-1015); only the additions are exact.
+// TwoSum (Knuth): s + e == a + b exactly, s = fl(a + b).  Compiled code must not reassociate these.
+__host__ __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+// One CT stage: (hi, lo) + delta.
+__host__ __device__ __forceinline__ void ct_add(double hi, float lo, double delta, double& hi1, float& lo1) {
+    double e;
+    two_sum(hi, (double)lo + delta, hi1, e);
+    lo1 = (float)e;
+}
+// The Heun average 1/2 (B^n + B^2) of the face field (rk2_step M2:850-866, M3:1042-1065) from the two
+// compensated fields B^n = (hn, ln) and B^2 = (h2, l2), exact up to the rounding of the low words.
+__host__ __device__ __forceinline__ void ct_avg(double hn, float ln, double h2, float l2, double& hi, float& lo) {
+    double s, e, h, l;
+    two_sum(hn, h2, s, e);
+    const double t = e + ((double)ln + (double)l2);
+    two_sum(s, t, h, l);
+    hi = 0.5 * h;
+    lo = (float)(0.5 * l);
+}
+
 }  // namespace cc

@@ -45,6 +45,10 @@ class Params:
     sor_iters: int = 600
     sor_omega: float = 1.8
     precision: int = CC_FP64
+    # no reference counterpart (0 keeps the reference's arithmetic): HLL for faces whose HLLD star-state
+    # denominator is below hlld_guard of its scale; FP32 low words carried through CT and the Heun average
+    hlld_guard: float = 0.0
+    ct_compensated: bool = False
 
     @property
     def dx(self):

@@ -47,6 +47,8 @@ class Params:
     gamma: float = 5.0 / 3.0
     precision: int = CC_FP64
     degen_tau: float = 0.0   # mixed precision: FP64 fallback margin near the Alfven degeneracy (0: default)
+    hlld_guard: float = 0.0  # HLL for faces whose HLLD star-state denominator is below this fraction of its scale
+    ct_compensated: bool = False  # FP32 low words of the faces carried through CT and the Heun average
 
     @property
     def dx(self):

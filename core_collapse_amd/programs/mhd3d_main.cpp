@@ -79,6 +79,9 @@ int main(int argc, char** argv) {
     cli.add("precision", &precision, "fp64 (reference arithmetic) | mixed");
     cli.add("degen_tau", &p.degen_tau, "mixed: FP64 fallback margin near the Alfven degeneracy (<= 0: 0.02)");
     cli.add("out", &out, "output file");
+    cli.add("hlld_guard", &p.hlld_guard,
+            "HLL for faces whose HLLD star-state denominator is below this fraction of its scale (0: the reference)");
+    cli.add_bool("ct_compensated", &p.ct_compensated, "carry the faces' rounding errors through CT (0: the reference)");
     DtGuard guard;
     cli.add("dt_guard", &guard.ratio,
             "stop with status 3 when dt falls below this fraction of the last 64 steps' largest dt (0: never)");

@@ -63,6 +63,16 @@ typedef struct cc_mhd2d_params {
     int sor_iters;    /* M2:71 */
     double sor_omega; /* M2:72 */
     int precision;    /* cc_precision */
+    /* No reference counterpart (0 keeps the reference's arithmetic):
+     * hlld_guard: an HLLD face whose star-state denominator rho (S - u)(S - S_M) - Bn^2 (M2:419-442) on
+     *   either side is below hlld_guard * max(Bn^2, |rho (S - u)(S - S_M)|) takes the reference's own HLL
+     *   flux (its |Bn| < 1e-12 branch, M2:449) instead of star states divided by a near-zero d (the
+     *   reference guards d only by 1e-14).  In [0, 1).
+     * ct_compensated: the face update of CT (M2:807-830) and the Heun average of the faces (M2:850-866)
+     *   carry each face's rounding error in an FP32 low word, so max|div B| of the stored field stays at
+     *   one rounding instead of growing with the number of steps. */
+    double hlld_guard;
+    int ct_compensated;
 } cc_mhd2d_params;
 
 typedef struct cc_mhd2d cc_mhd2d;
@@ -178,6 +188,11 @@ typedef struct cc_mhd3d_params {
     int precision;    /* cc_precision */
     double degen_tau; /* CC_MIXED only: faces whose HLLD lies within this relative margin of the Alfven
                          degeneracy are recomputed in FP64; <= 0 selects the default 0.02 */
+    /* As in cc_mhd2d_params (no reference counterpart; 0 keeps the reference's arithmetic): the HLLD
+     * degeneracy guard on rho (S - u)(S - S_M) - Bn^2 (M3:463-484, HLL branch M3:491) and the compensated
+     * face update of CT (M3:941-1015) and of the Heun average (M3:1042-1065). */
+    double hlld_guard;
+    int ct_compensated;
 } cc_mhd3d_params;
 
 typedef struct cc_mhd3d cc_mhd3d;
@@ -202,7 +217,8 @@ cc_status cc_mhd3d_run(cc_mhd3d* h, int nsteps, double* t_inout, double t_end, d
 /* max |divB_cc| (M3:1070-1078).  Syncs. */
 cc_status cc_mhd3d_divb_max(cc_mhd3d* h, double* out);
 cc_status cc_mhd3d_sync(cc_mhd3d* h);
-/* Per-kernel HIP-event timing ("prim", "face", "update", "edge", "ct", "joule", "dt"). */
+/* Per-kernel HIP-event timing ("prim", "face" (x, y, z face kernels), "face_fp64" (the mixed path's FP64 pass over
+ * faces queued near the Alfven degeneracy), "update", "edge", "ct", "joule", "dt"). */
 cc_status cc_mhd3d_set_profiling(cc_mhd3d* h, int on);
 cc_status cc_mhd3d_kernel_time(cc_mhd3d* h, const char* kernel, int* count, double* total_ms);
 cc_status cc_mhd3d_reset_profiling(cc_mhd3d* h);

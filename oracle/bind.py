@@ -18,11 +18,16 @@ class M2Params(C.Structure):
                 ("y0", C.c_double), ("y1", C.c_double), ("Nx", C.c_int), ("Ny", C.c_int),
                 ("floor_rho", C.c_double), ("floor_P", C.c_double), ("periodic_x", C.c_int),
                 ("periodic_y", C.c_int), ("eta", C.c_double), ("gravity_on", C.c_int), ("G", C.c_double),
-                ("sor_iters", C.c_int), ("sor_omega", C.c_double)]
+                ("sor_iters", C.c_int), ("sor_omega", C.c_double), ("hlld_guard", C.c_double),
+                ("ct_comp", C.c_int)]
+
+
+FP = C.POINTER(C.c_float)
 
 
 class M2State(C.Structure):
-    _fields_ = [(n, DP) for n in ("rho", "mx", "my", "mz", "E", "Bz", "Bx_f", "By_f")]
+    _fields_ = [(n, DP) for n in ("rho", "mx", "my", "mz", "E", "Bz", "Bx_f", "By_f")] + \
+               [(n, FP) for n in ("Bx_lo", "By_lo")]
 
 
 class M3Params(C.Structure):
@@ -32,11 +37,12 @@ class M3Params(C.Structure):
                [("Nx", C.c_int), ("Ny", C.c_int), ("Nz", C.c_int), ("floor_rho", C.c_double),
                 ("floor_P", C.c_double), ("periodic_x", C.c_int), ("periodic_y", C.c_int),
                 ("periodic_z", C.c_int), ("eta", C.c_double), ("ideal_eos", C.c_int),
-                ("gamma_ideal", C.c_double)]
+                ("gamma_ideal", C.c_double), ("hlld_guard", C.c_double), ("ct_comp", C.c_int)]
 
 
 class M3State(C.Structure):
-    _fields_ = [(n, DP) for n in ("rho", "mx", "my", "mz", "E", "Bx_f", "By_f", "Bz_f")]
+    _fields_ = [(n, DP) for n in ("rho", "mx", "my", "mz", "E", "Bx_f", "By_f", "Bz_f")] + \
+               [(n, FP) for n in ("Bx_lo", "By_lo", "Bz_lo")]
 
 
 class EHParams(C.Structure):
@@ -72,7 +78,7 @@ def m2_params(**kw):
     """Reference defaults of M2:44-86 (mhd2d_resistive_hlld_ct_gravity.cpp), overridable."""
     d = dict(gamma=5.0 / 3.0, cfl=0.45, x0=0.0, x1=1.0, y0=0.0, y1=1.0, Nx=192, Ny=192, floor_rho=1e-10,
              floor_P=1e-10, periodic_x=1, periodic_y=1, eta=5e-4, gravity_on=1, G=8.0, sor_iters=600,
-             sor_omega=1.8)
+             sor_omega=1.8, hlld_guard=0.0, ct_comp=0)
     d.update(kw)
     return M2Params(**d)
 
@@ -82,7 +88,7 @@ def m3_params(**kw):
     d = dict(gamma1=1.30, gamma2=2.50, K1=0.05, rho_nuc=80.0, gamma_th=1.50, rho_bounce=120.0, gamma_bounce=3.0,
              Kbounce=0.02, gamma_init=5.0 / 3.0, cfl=0.4, x0=0.0, x1=1.0, y0=0.0, y1=1.0, z0=0.0, z1=1.0,
              Nx=48, Ny=48, Nz=48, floor_rho=1e-10, floor_P=1e-10, periodic_x=1, periodic_y=1, periodic_z=1,
-             eta=5e-4, ideal_eos=0, gamma_ideal=5.0 / 3.0)
+             eta=5e-4, ideal_eos=0, gamma_ideal=5.0 / 3.0, hlld_guard=0.0, ct_comp=0)
     d.update(kw)
     return M3Params(**d)
 
@@ -116,6 +122,11 @@ def ptr(a):
     return a.ctypes.data_as(DP)
 
 
+def fptr(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(FP)
+
+
 class Lib:
     """Either the restatement (kind='oracle') or a reference harness (kind='ref_m2', 'ref_m3', ...)."""
 
@@ -146,7 +157,8 @@ class Lib:
 
 
 class M2Arrays:
-    """Reference-layout 2D MHD state held in numpy arrays (M2:88-95)."""
+    """Reference-layout 2D MHD state held in numpy arrays (M2:88-95), plus the FP32 low words of the faces
+    that the compensated face update carries (zeros unless ct_comp is on)."""
 
     FIELDS = ("rho", "mx", "my", "mz", "E", "Bz", "Bx_f", "By_f")
 
@@ -161,12 +173,19 @@ class M2Arrays:
                 assert self.a[f].size == n, (f, self.a[f].size, n)
             else:
                 self.a[f] = np.zeros(n)
+        self.lo = {"Bx_lo": np.zeros((Nx + 1) * Ny, np.float32), "By_lo": np.zeros(Nx * (Ny + 1), np.float32)}
+        if arrays is not None:
+            for f in self.lo:
+                if f in arrays:
+                    self.lo[f] = np.ascontiguousarray(arrays[f], dtype=np.float32).reshape(-1).copy()
 
     def copy(self):
-        return M2Arrays(self.Nx, self.Ny, self.a)
+        c = M2Arrays(self.Nx, self.Ny, self.a)
+        c.lo = {k: v.copy() for k, v in self.lo.items()}
+        return c
 
     def cstate(self):
-        return M2State(*[ptr(self.a[f]) for f in self.FIELDS])
+        return M2State(*([ptr(self.a[f]) for f in self.FIELDS] + [fptr(self.lo[f]) for f in ("Bx_lo", "By_lo")]))
 
     def __getitem__(self, k):
         return self.a[k]
@@ -186,12 +205,15 @@ class M3Arrays:
                 assert self.a[f].size == n, (f, self.a[f].size, n)
             else:
                 self.a[f] = np.zeros(n)
+        self.lo = {k.replace("_f", "_lo"): np.zeros(shapes[k], np.float32) for k in ("Bx_f", "By_f", "Bz_f")}
 
     def copy(self):
-        return M3Arrays(self.Nx, self.Ny, self.Nz, self.a)
+        c = M3Arrays(self.Nx, self.Ny, self.Nz, self.a)
+        c.lo = {k: v.copy() for k, v in self.lo.items()}
+        return c
 
     def cstate(self):
-        return M3State(*[ptr(self.a[f]) for f in self.FIELDS])
+        return M3State(*([ptr(self.a[f]) for f in self.FIELDS] + [fptr(self.lo[f]) for f in ("Bx_lo", "By_lo", "Bz_lo")]))
 
     def __getitem__(self, k):
         return self.a[k]

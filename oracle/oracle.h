@@ -17,6 +17,27 @@
 extern "C" {
 #endif
 
+/* Compensated face update (ct_comp; no reference counterpart).  TwoSum (Knuth, TAOCP vol. 2, 4.2.2):
+ * s = fl(a + b) and s + e = a + b exactly.  A face is hi + lo with lo the FP32 rounding error of hi; CT adds
+ * its increment to the pair, the Heun average halves the exact sum of two pairs. */
+static inline void or_two_sum(double a, double b, double* s, double* e) {
+    const double ss = a + b, bb = ss - a;
+    *s = ss;
+    *e = (a - (ss - bb)) + (b - bb);
+}
+static inline void or_ct_add(double hi, float lo, double d, double* hi1, float* lo1) {
+    double e;
+    or_two_sum(hi, (double)lo + d, hi1, &e);
+    *lo1 = (float)e;
+}
+static inline void or_ct_avg(double hn, float ln, double h2, float l2, double* hi, float* lo) {
+    double s, e, h, l;
+    or_two_sum(hn, h2, &s, &e);
+    or_two_sum(s, e + ((double)ln + (double)l2), &h, &l);
+    *hi = 0.5 * h;
+    *lo = (float)(0.5 * l);
+}
+
 /* ---- 2D resistive MHD (mhd2d_resistive_hlld_ct_gravity.cpp, Params M2:44-86) ---- */
 typedef struct {
     double gamma, cfl, x0, x1, y0, y1;
@@ -28,12 +49,19 @@ typedef struct {
     double G;
     int sor_iters;
     double sor_omega;
+    /* No reference counterpart (0 keeps the reference's arithmetic; the reference harness ignores them):
+     * hlld_guard > 0 sends an HLLD face whose d_L or d_R (M2:419-442) is below
+     * hlld_guard * max(Bn^2, |rho s (S - S_M)|) to the HLL branch (M2:449); ct_comp carries each face's
+     * rounding error in an FP32 low word through CT and the Heun average (state Bx_lo, By_lo). */
+    double hlld_guard;
+    int ct_comp;
 } or_m2_params;
 
 typedef struct {
     double *rho, *mx, *my, *mz, *E, *Bz; /* Nx*Ny, q = j*Nx+i        */
     double *Bx_f;                        /* (Nx+1)*Ny, j*(Nx+1)+i_f */
     double *By_f;                        /* Nx*(Ny+1), j_f*Nx+i     */
+    float *Bx_lo, *By_lo;                /* low words of the faces (ct_comp only; else unused) */
 } or_m2_state;
 
 void or_m2_hlld_x(const or_m2_params* p, const double* WL, const double* WR, double Bn, double* out8);
@@ -66,6 +94,8 @@ typedef struct {
     double eta;
     int ideal_eos;      /* 0: the reference hybrid EOS (M3:114-212); 1: ideal gas with gamma_ideal */
     double gamma_ideal; /* used when ideal_eos = 1 (no reference counterpart; see the EOS note in oracle_mhd3d.c) */
+    double hlld_guard;  /* as in or_m2_params (M3:463-484, HLL branch M3:491) */
+    int ct_comp;        /* as in or_m2_params (CT M3:941-1015, Heun M3:1042-1065; state B*_lo) */
 } or_m3_params;
 
 typedef struct {
@@ -73,6 +103,7 @@ typedef struct {
     double *Bx_f;                   /* (Nx+1)*Ny*Nz, (k*Ny+j)*(Nx+1)+i_f */
     double *By_f;                   /* Nx*(Ny+1)*Nz, (k*(Ny+1)+j_f)*Nx+i */
     double *Bz_f;                   /* Nx*Ny*(Nz+1), (k_f*Ny+j)*Nx+i     */
+    float *Bx_lo, *By_lo, *Bz_lo;   /* low words of the faces (ct_comp only; else unused) */
 } or_m3_state;
 
 void or_m3_hlld_x(const or_m3_params* p, const double* WL, const double* WR, double Bn, double* out7);

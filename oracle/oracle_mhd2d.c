@@ -368,7 +368,9 @@ static FluxX hlld_x(const or_m2_params* p, Prim WL, Prim WR, double Bn) {
     double SLL = SM - fabs(Bn) / sqrt_rhoL;
     double SRR = SM + fabs(Bn) / sqrt_rhoR;
 
-    if (fabs(Bn) < 1e-12) {
+    const double gd = p->hlld_guard;
+    if (fabs(Bn) < 1e-12 || (gd > 0.0 && (fabs(dL) < gd * fmax(Bn * Bn, fabs(rhoL * sL * (SL - SM))) ||
+                                          fabs(dR) < gd * fmax(Bn * Bn, fabs(rhoR * sR * (SR - SM)))))) {
         g_last_branch = 2;
         double inv = 1.0 / (SR - SL);
         double FUL[7] = {FL.frho, FL.fmx, FL.fmy, FL.fmz, FL.fE, FL.fBy, FL.fBz};
@@ -667,7 +669,9 @@ void or_m2_advance_stage(const or_m2_params* p, const or_m2_state* U0, or_m2_sta
             size_t f = (size_t)j * (size_t)(Nx + 1) + (size_t)i;
             double ezU = Ez_edge[(size_t)je1 * (size_t)(Nx + 1) + (size_t)i];
             double ezD = Ez_edge[(size_t)je0 * (size_t)(Nx + 1) + (size_t)i];
-            U1->Bx_f[f] += -(dt / dy) * (ezU - ezD);
+            const double d = -(dt / dy) * (ezU - ezD);
+            if (p->ct_comp) or_ct_add(U0->Bx_f[f], U0->Bx_lo[f], d, &U1->Bx_f[f], &U1->Bx_lo[f]);
+            else U1->Bx_f[f] += d;
         }
     for (int j = 0; j <= Ny; j++)
         for (int i = 0; i < Nx; i++) {
@@ -677,7 +681,9 @@ void or_m2_advance_stage(const or_m2_params* p, const or_m2_state* U0, or_m2_sta
             size_t f = (size_t)j * (size_t)Nx + (size_t)i;
             double ezR = Ez_edge[(size_t)j * (size_t)(Nx + 1) + (size_t)ie1];
             double ezL = Ez_edge[(size_t)j * (size_t)(Nx + 1) + (size_t)ie0];
-            U1->By_f[f] += +(dt / dx) * (ezR - ezL);
+            const double d = +(dt / dx) * (ezR - ezL);
+            if (p->ct_comp) or_ct_add(U0->By_f[f], U0->By_lo[f], d, &U1->By_f[f], &U1->By_lo[f]);
+            else U1->By_f[f] += d;
         }
     free(Ez_edge);
     free(Fx); free(Gy); free(Ez_x); free(Ez_y);
@@ -688,6 +694,8 @@ void or_m2_advance_stage(const or_m2_params* p, const or_m2_state* U0, or_m2_sta
 
 static void state_alloc(or_m2_state* S, int Nx, int Ny) {
     size_t N = (size_t)Nx * (size_t)Ny;
+    S->Bx_lo = (float*)calloc((size_t)(Nx + 1) * (size_t)Ny, sizeof(float));
+    S->By_lo = (float*)calloc((size_t)Nx * (size_t)(Ny + 1), sizeof(float));
     S->rho = (double*)malloc(N * sizeof(double)); S->mx = (double*)malloc(N * sizeof(double));
     S->my = (double*)malloc(N * sizeof(double));  S->mz = (double*)malloc(N * sizeof(double));
     S->E = (double*)malloc(N * sizeof(double));   S->Bz = (double*)malloc(N * sizeof(double));
@@ -696,6 +704,7 @@ static void state_alloc(or_m2_state* S, int Nx, int Ny) {
 }
 static void state_free(or_m2_state* S) {
     free(S->rho); free(S->mx); free(S->my); free(S->mz); free(S->E); free(S->Bz); free(S->Bx_f); free(S->By_f);
+    free(S->Bx_lo); free(S->By_lo);
 }
 
 /* M2:838-869 */
@@ -720,8 +729,13 @@ void or_m2_rk2_step(const or_m2_params* p, or_m2_state* U, const double* gx, con
         U->Bz[q] = 0.5 * (U->Bz[q] + U2.Bz[q]);
     }
     size_t NFx = (size_t)(Nx + 1) * (size_t)Ny, NFy = (size_t)Nx * (size_t)(Ny + 1);
-    for (size_t k = 0; k < NFx; k++) U->Bx_f[k] = 0.5 * (U->Bx_f[k] + U2.Bx_f[k]);
-    for (size_t k = 0; k < NFy; k++) U->By_f[k] = 0.5 * (U->By_f[k] + U2.By_f[k]);
+    if (p->ct_comp) {
+        for (size_t k = 0; k < NFx; k++) or_ct_avg(U->Bx_f[k], U->Bx_lo[k], U2.Bx_f[k], U2.Bx_lo[k], &U->Bx_f[k], &U->Bx_lo[k]);
+        for (size_t k = 0; k < NFy; k++) or_ct_avg(U->By_f[k], U->By_lo[k], U2.By_f[k], U2.By_lo[k], &U->By_f[k], &U->By_lo[k]);
+    } else {
+        for (size_t k = 0; k < NFx; k++) U->Bx_f[k] = 0.5 * (U->Bx_f[k] + U2.Bx_f[k]);
+        for (size_t k = 0; k < NFy; k++) U->By_f[k] = 0.5 * (U->By_f[k] + U2.By_f[k]);
+    }
     for (int j = 0; j < Ny; j++)
         for (int i = 0; i < Nx; i++) {
             size_t q = idx2(i, j, Nx);

@@ -272,7 +272,9 @@ static void hlld_x(const or_m3_params* p, Prim WL, Prim WR, double Bn, double ou
     }
     double sqrt_rhoL = sqrt(dmax(rhoL_s, 1e-300)), sqrt_rhoR = sqrt(dmax(rhoR_s, 1e-300));
     double SLL = SM - fabs(Bn) / sqrt_rhoL, SRR = SM + fabs(Bn) / sqrt_rhoR;
-    if (fabs(Bn) < 1e-12) {
+    const double gd = p->hlld_guard;
+    if (fabs(Bn) < 1e-12 || (gd > 0.0 && (fabs(dL) < gd * fmax(Bn * Bn, fabs(rhoL * sL * (SL - SM))) ||
+                                          fabs(dR) < gd * fmax(Bn * Bn, fabs(rhoR * sR * (SR - SM)))))) {
         double inv = 1.0 / (SR - SL);
         for (int c = 0; c < 7; c++) out[c] = (SR * FL[c] - SL * FR[c] + SL * SR * (UR[c] - UL[c])) * inv;
         return;
@@ -528,7 +530,10 @@ void or_m3_advance_stage(const or_m3_params* p, const or_m3_state* U0, or_m3_sta
                 double ezD = Ez_e[((size_t)k * (Ny + 1) + jf0) * (size_t)(Nx + 1) + i];
                 double eyF = Ey_e[((size_t)kf1 * Ny + j) * (size_t)(Nx + 1) + i];
                 double eyB = Ey_e[((size_t)kf0 * Ny + j) * (size_t)(Nx + 1) + i];
-                U1->Bx_f[FX(i, j, k)] += -(dt / dy) * (ezU - ezD) + (dt / dz) * (eyF - eyB);
+                const double d = -(dt / dy) * (ezU - ezD) + (dt / dz) * (eyF - eyB);
+                const size_t f = FX(i, j, k);
+                if (p->ct_comp) or_ct_add(U0->Bx_f[f], U0->Bx_lo[f], d, &U1->Bx_f[f], &U1->Bx_lo[f]);
+                else U1->Bx_f[f] += d;
             }
     for (int k = 0; k < Nz; k++)
         for (int j = 0; j <= Ny; j++)
@@ -540,7 +545,10 @@ void or_m3_advance_stage(const or_m3_params* p, const or_m3_state* U0, or_m3_sta
                 double ezL = Ez_e[((size_t)k * (Ny + 1) + j) * (size_t)(Nx + 1) + if0];
                 double exF = Ex_e[((size_t)kf1 * (Ny + 1) + j) * (size_t)Nx + i];
                 double exB = Ex_e[((size_t)kf0 * (Ny + 1) + j) * (size_t)Nx + i];
-                U1->By_f[FY(i, j, k)] += +(dt / dx) * (ezR - ezL) - (dt / dz) * (exF - exB);
+                const double d = +(dt / dx) * (ezR - ezL) - (dt / dz) * (exF - exB);
+                const size_t f = FY(i, j, k);
+                if (p->ct_comp) or_ct_add(U0->By_f[f], U0->By_lo[f], d, &U1->By_f[f], &U1->By_lo[f]);
+                else U1->By_f[f] += d;
             }
     for (int k = 0; k <= Nz; k++)
         for (int j = 0; j < Ny; j++)
@@ -552,7 +560,10 @@ void or_m3_advance_stage(const or_m3_params* p, const or_m3_state* U0, or_m3_sta
                 double eyL = Ey_e[((size_t)k * Ny + j) * (size_t)(Nx + 1) + if0];
                 double exU = Ex_e[((size_t)k * (Ny + 1) + jf1) * (size_t)Nx + i];
                 double exD = Ex_e[((size_t)k * (Ny + 1) + jf0) * (size_t)Nx + i];
-                U1->Bz_f[FZ(i, j, k)] += -(dt / dx) * (eyR - eyL) + (dt / dy) * (exU - exD);
+                const double d = -(dt / dx) * (eyR - eyL) + (dt / dy) * (exU - exD);
+                const size_t f = FZ(i, j, k);
+                if (p->ct_comp) or_ct_add(U0->Bz_f[f], U0->Bz_lo[f], d, &U1->Bz_f[f], &U1->Bz_lo[f]);
+                else U1->Bz_f[f] += d;
             }
     add_joule_heating(p, U1, dx, dy, dz, dt);
     for (int k = 0; k < Nz; k++)
@@ -575,9 +586,13 @@ static void state_alloc(const or_m3_params* p, or_m3_state* S) {
     S->Bx_f = malloc((size_t)(p->Nx + 1) * p->Ny * p->Nz * 8);
     S->By_f = malloc((size_t)p->Nx * (p->Ny + 1) * p->Nz * 8);
     S->Bz_f = malloc((size_t)p->Nx * p->Ny * (p->Nz + 1) * 8);
+    S->Bx_lo = calloc((size_t)(p->Nx + 1) * p->Ny * p->Nz, sizeof(float));
+    S->By_lo = calloc((size_t)p->Nx * (p->Ny + 1) * p->Nz, sizeof(float));
+    S->Bz_lo = calloc((size_t)p->Nx * p->Ny * (p->Nz + 1), sizeof(float));
 }
 static void state_free(or_m3_state* S) {
     free(S->rho); free(S->mx); free(S->my); free(S->mz); free(S->E); free(S->Bx_f); free(S->By_f); free(S->Bz_f);
+    free(S->Bx_lo); free(S->By_lo); free(S->Bz_lo);
 }
 
 /* M3:1034-1066 */
@@ -597,9 +612,15 @@ void or_m3_rk2_step(const or_m3_params* p, or_m3_state* U, double dx, double dy,
     }
     size_t NFx = (size_t)(p->Nx + 1) * p->Ny * p->Nz, NFy = (size_t)p->Nx * (p->Ny + 1) * p->Nz,
            NFz = (size_t)p->Nx * p->Ny * (p->Nz + 1);
-    for (size_t f = 0; f < NFx; f++) U->Bx_f[f] = 0.5 * (U->Bx_f[f] + U2.Bx_f[f]);
-    for (size_t f = 0; f < NFy; f++) U->By_f[f] = 0.5 * (U->By_f[f] + U2.By_f[f]);
-    for (size_t f = 0; f < NFz; f++) U->Bz_f[f] = 0.5 * (U->Bz_f[f] + U2.Bz_f[f]);
+    if (p->ct_comp) {
+        for (size_t f = 0; f < NFx; f++) or_ct_avg(U->Bx_f[f], U->Bx_lo[f], U2.Bx_f[f], U2.Bx_lo[f], &U->Bx_f[f], &U->Bx_lo[f]);
+        for (size_t f = 0; f < NFy; f++) or_ct_avg(U->By_f[f], U->By_lo[f], U2.By_f[f], U2.By_lo[f], &U->By_f[f], &U->By_lo[f]);
+        for (size_t f = 0; f < NFz; f++) or_ct_avg(U->Bz_f[f], U->Bz_lo[f], U2.Bz_f[f], U2.Bz_lo[f], &U->Bz_f[f], &U->Bz_lo[f]);
+    } else {
+        for (size_t f = 0; f < NFx; f++) U->Bx_f[f] = 0.5 * (U->Bx_f[f] + U2.Bx_f[f]);
+        for (size_t f = 0; f < NFy; f++) U->By_f[f] = 0.5 * (U->By_f[f] + U2.By_f[f]);
+        for (size_t f = 0; f < NFz; f++) U->Bz_f[f] = 0.5 * (U->Bz_f[f] + U2.Bz_f[f]);
+    }
     for (int k = 0; k < p->Nz; k++)
         for (int j = 0; j < p->Ny; j++)
             for (int i = 0; i < p->Nx; i++) {

@@ -277,12 +277,37 @@ def ns_vectors(ref):
     np.savez_compressed(os.path.join(HERE, "ns_step.npz"), **res)
 
 
+def config1_vectors(ref):
+    """The 2D collapse benchmark's plain FP64 run (128^2, 200 steps, reference arithmetic) through the
+    reference's own step loop: the final state, every dt and t (m2_config1.npz)."""
+    from core_collapse_amd import ic
+    p = ic.collapse2d_params(128, ct_compensated=False)
+    st = ic.collapse2d_ic(p)
+    op = OB.m2_params(gamma=p.gamma, cfl=p.cfl, x0=p.x0, x1=p.x1, y0=p.y0, y1=p.y1, Nx=128, Ny=128, periodic_x=0,
+                      periodic_y=0, eta=p.eta, G=p.G, sor_iters=p.sor_iters, sor_omega=p.sor_omega)
+    S = OB.M2Arrays(128, 128, {k: v.reshape(-1) for k, v in st.items()})
+    res = {"in_rho_sum": np.array([S.a["rho"].sum()])}
+    dts = np.zeros(200)
+    t = ref.or_m2_run(C.byref(op), C.byref(S.cstate()), C.c_double(p.dx), C.c_double(p.dy), 200, C.c_double(0.0),
+                      C.c_double(1e300), OB.ptr(dts))
+    for f in S.FIELDS:
+        res["out_" + f] = S.a[f]
+    res["dts"] = dts
+    res["t"] = np.array([t])
+    np.savez_compressed(os.path.join(HERE, "m2_config1.npz"), **res)
+
+
 if __name__ == "__main__":
+    import sys
+    if sys.argv[1:] == ["config1"]:
+        config1_vectors(OB.Lib("ref_m2"))
+        sys.exit(0)
     orc = OB.Lib("oracle")
     m2_vectors(OB.Lib("ref_m2"), orc)
     m3_vectors(OB.Lib("ref_m3"))
     eh_vectors(OB.Lib("ref_eh"))
     ns_vectors(OB.Lib("ref_ns"))
+    config1_vectors(OB.Lib("ref_m2"))
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

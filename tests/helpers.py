@@ -34,7 +34,8 @@ def m2_oparams(p):
     return OB.m2_params(gamma=p.gamma, cfl=p.cfl, x0=p.x0, x1=p.x1, y0=p.y0, y1=p.y1, Nx=p.Nx, Ny=p.Ny,
                         floor_rho=p.floor_rho, floor_P=p.floor_P, periodic_x=int(p.periodic_x),
                         periodic_y=int(p.periodic_y), eta=p.eta, gravity_on=int(p.gravity_on), G=p.G,
-                        sor_iters=p.sor_iters, sor_omega=p.sor_omega)
+                        sor_iters=p.sor_iters, sor_omega=p.sor_omega, hlld_guard=p.hlld_guard,
+                        ct_comp=int(p.ct_compensated))
 
 
 def m2_arrays(p, state):
@@ -100,7 +101,8 @@ def m3_oparams(p):
                         x0=p.x0, x1=p.x1, y0=p.y0, y1=p.y1, z0=p.z0, z1=p.z1, Nx=p.Nx, Ny=p.Ny, Nz=p.Nz,
                         floor_rho=p.floor_rho, floor_P=p.floor_P, periodic_x=int(p.periodic_x),
                         periodic_y=int(p.periodic_y), periodic_z=int(p.periodic_z), eta=p.eta,
-                        ideal_eos=int(p.eos == 1), gamma_ideal=p.gamma)
+                        ideal_eos=int(p.eos == 1), gamma_ideal=p.gamma, hlld_guard=p.hlld_guard,
+                        ct_comp=int(p.ct_compensated))
 
 
 def m3_arrays(p, state):

@@ -140,23 +140,25 @@ def test_default_run_matches_reference_program(tag, tmp_path):
 
 
 def test_m2_default_run_matches_reference_log(tmp_path):
-    """The M2 default configuration to t = 0.18 against the reference's log of its default run: the
+    """The M2 default configuration to t = 0.15 against the reference's log of its default run: the
     init line digit for digit, then t and dt of every stored frame (7 digits printed; the GPU step
-    contracts to FMA, so a last digit may differ).  The two runs print the same t and dt for the
-    first 181 frames; at t = 0.1811 the reference takes a 14 % shorter step than the GPU run (a
-    threshold decision in some cell goes the other way at round-off level) and the trajectories
-    part, so the comparison stops at 0.18.  The first frame is the oracle's frame of the IC bit for
-    bit, and the file has the reference layout."""
+    contracts to FMA and removes the SOR mean once per solve, so a last digit may differ).  Later in this
+    run the trajectory is chaotic at the level of rounding: the reference HLLD divides its star states by
+    rho (S - u)(S - S_M) - Bn^2 with only |.| > 1e-14 as a guard, and a face that comes close to that
+    degeneracy (|d| ~ 1e-9 is seen) produces star states of O(100) and a short step.  Whether a given
+    trajectory meets such a face first at t = 0.168 (the GPU run), 0.181, 0.215 or 0.231 (one-ulp changes of
+    the initial state on the CPU) is decided by rounding, so the comparison stops at 0.15.  The first frame
+    is the oracle's frame of the IC bit for bit, and the file has the reference layout."""
     from core_collapse_amd.mhd2d import Params
     meta, _ = D.fixture("m2")
     ref = meta["stdout"]
-    r, wall = _run("m2", ["--t_end", "0.18"], str(tmp_path))
+    r, wall = _run("m2", ["--t_end", "0.15"], str(tmp_path))
     out = r.stdout.splitlines()
-    print("m2 to t=0.18: %.1f s, %d lines" % (wall, len(out)))
+    print("m2 to t=0.15: %.1f s, %d lines" % (wall, len(out)))
     assert out[0] == ref[0] and out[-1] == "Wrote out2d_mhd.h5"
     pat = re.compile(r"t=(\S+) dt=(\S+) max\|divB\|=(\S+)$")
     got = [pat.match(x).groups() for x in out[1:-1]]
-    want = [pat.match(x).groups() for x in ref[1:] if pat.match(x) and float(pat.match(x).group(1)) < 0.18]
+    want = [pat.match(x).groups() for x in ref[1:] if pat.match(x) and float(pat.match(x).group(1)) < 0.15]
     assert len(got) in (len(want), len(want) + 1)  # + the frame at t_end when no reference frame falls there
     for g, w in zip(got, want):
         assert abs(float(g[0]) / float(w[0]) - 1) < 2e-6 and abs(float(g[1]) / float(w[1]) - 1) < 2e-6, (g, w)

@@ -195,11 +195,13 @@ def test_compute_dt_matches_oracle():
     assert abs(dt - want) <= 1e-14 * want, (dt, want)
 
 
+@pytest.mark.parametrize("comp", [False, True])
 @pytest.mark.parametrize("N,nsteps", [(48, 8)])
-def test_collapse_run_matches_oracle(N, nsteps):
+def test_collapse_run_matches_oracle(N, nsteps, comp):
     """The benchmark's Gaussian-collapse problem (clamped boundaries, gravity, resistivity),
-    nsteps of the driver loop with device-side dt, against the oracle's loop."""
-    p = ic.collapse2d_params(N)
+    nsteps of the driver loop with device-side dt, against the oracle's loop; with the reference's face
+    update and with the compensated one."""
+    p = ic.collapse2d_params(N, ct_compensated=comp)
     st = ic.collapse2d_ic(p)
     with M.Mhd2D(p) as h:
         h.upload(st)
@@ -215,6 +217,52 @@ def test_collapse_run_matches_oracle(N, nsteps):
     assert abs(t - ot) < 1e-11 * ot
     for f in M.FIELDS:
         assert rel_err(got[f], want[f]) < 1e-9, (f, rel_err(got[f], want[f]))
+
+
+def test_config1_full_run_matches_the_reference_step_code():
+    """Config 1 itself: the 2D collapse at 128^2 in plain FP64 (the reference's face update) for its 200
+    steps, against the same run through the reference program's own step functions (compiled from its
+    source; tests/golden/m2_config1.npz): every dt to 1e-10 relative, every field to 1e-8 relative to its
+    maximum.  The GPU differs from the reference only by FMA contraction and its reduction order."""
+    import os
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "m2_config1.npz"))
+    p = ic.collapse2d_params(128, ct_compensated=False)
+    st = ic.collapse2d_ic(p)
+    assert st["rho"].sum() == d["in_rho_sum"][0]
+    with M.Mhd2D(p) as h:
+        h.upload(st)
+        t, dts = h.run(200, 0.0, 1e300, return_dts=True)
+        got = h.download()
+        divb = h.divB_max()
+    e_dt = rel_err(dts, d["dts"])
+    errs = {f: rel_err(got[f].reshape(-1), d["out_" + f]) for f in M.FIELDS}
+    print("config 1 (128^2, 200 steps): t %.9g vs %.9g, dt err %.2e, fields %s, max|divB| %.2e"
+          % (t, d["t"][0], e_dt, {k: "%.1e" % v for k, v in errs.items()}, divb), flush=True)
+    assert e_dt < 1e-10, e_dt
+    assert abs(t - d["t"][0]) <= 1e-10 * d["t"][0]
+    for f, e in errs.items():
+        assert e < 1e-8, (f, e)
+
+
+@pytest.mark.parametrize("prec", [CC_FP64, CC_MIXED])
+def test_compensated_face_update_keeps_divb_at_one_rounding(prec):
+    """The collapse at 256^2 for 300 steps: with the compensated face update max|div B| stays at one
+    rounding of the face values (<= 2 ulp(max|B|) (1/dx + 1/dy), far below 1e-12); the reference's update,
+    the same run otherwise, accumulates roundings."""
+    out = {}
+    for comp in (False, True):
+        p = ic.collapse2d_params(256, precision=prec, ct_compensated=comp)
+        with M.Mhd2D(p) as h:
+            h.upload(ic.collapse2d_ic(p))
+            h.run(300)
+            s = h.download()
+            out[comp] = (h.divB_max(), max(np.abs(s["Bx_f"]).max(), np.abs(s["By_f"]).max()), p)
+    dc, bmax, p = out[True]
+    bound = 2 * np.spacing(bmax) * (1 / p.dx + 1 / p.dy)
+    print("max|divB| after 300 steps at 256^2: compensated %.2e (bound %.2e), reference update %.2e"
+          % (dc, bound, out[False][0]), flush=True)
+    assert dc <= bound and dc < 1e-12, (dc, bound)
+    assert out[False][0] > dc
 
 
 def test_divb_stays_at_roundoff():

@@ -113,13 +113,15 @@ def _turb_state(p, tilted):
     return st
 
 
-@pytest.mark.parametrize("tilted", [True, False])
-def test_turbulence_run_matches_oracle_and_divb(tilted):
-    """Turbulence IC at 16^3, 5 steps of the driver loop.  With the field along z only, x/y faces
-    start at Bn = 0 (the HLL branch of M3:491) and z faces at the Alfven degeneracy, so roundoff
-    decides a few branch choices: the bar there is relative L1 <= 1e-8; with a tilted field the
-    GPU reproduces the oracle to 1e-12."""
-    p = ic.turbulence3d_params(16)
+@pytest.mark.parametrize("tilted,guard", [(True, 1e-2), (False, 1e-2), (True, 0.0), (False, 0.0)])
+def test_turbulence_run_matches_oracle_and_divb(tilted, guard):
+    """Turbulence IC at 16^3, 5 steps of the driver loop, compensated face update.  With the benchmark's
+    HLLD guard (1e-2) the run is well-conditioned and the GPU reproduces the oracle to 1e-10 in every field,
+    with the field along z (the benchmark's own IC) or tilted.  With the reference's HLLD (guard 0) and the
+    field along z only, z faces sit at the star-state degeneracy and x/y faces at Bn = 0 (the HLL branch of
+    M3:491), so rounding decides a few branch choices: the bar there is relative L1 <= 1e-7 on rho and E;
+    tilted, the GPU reproduces the oracle to 1e-12."""
+    p = ic.turbulence3d_params(16, hlld_guard=guard)
     st = _turb_state(p, tilted)
     nsteps = 5
     with M.Mhd3D(p) as h:
@@ -133,10 +135,11 @@ def test_turbulence_run_matches_oracle_and_divb(tilted):
     o.or_m3_run(C.byref(m3_oparams(p)), C.byref(S.cstate()), C.c_double(p.dx), C.c_double(p.dy), C.c_double(p.dz),
                 nsteps, C.c_double(0.0), C.c_double(1e300), OB.ptr(odts))
     want = m3_dict(p, S)
-    if tilted:
-        assert rel_err(dts, odts) < 1e-12
+    if tilted or guard > 0:
+        tol = 1e-12 if guard == 0 else 1e-10
+        assert rel_err(dts, odts) < tol
         for f in M.FIELDS:
-            assert rel_err(got[f], want[f]) < 1e-12, (f, rel_err(got[f], want[f]))
+            assert rel_err(got[f], want[f]) < tol, (f, rel_err(got[f], want[f]))
     else:
         assert rel_err(dts, odts) < 1e-6
         for f in ("rho", "E"):
@@ -146,10 +149,7 @@ def test_turbulence_run_matches_oracle_and_divb(tilted):
 
 def test_mixed_vs_fp64_turbulence_tilted():
     """Mixed vs FP64 at 32^3 over 10 steps with a tilted field (no face near |Bn| = 1e-12 or the Alfven
-    degeneracy): relative L1 density error <= 1e-4.  With the field along z only,
-    the reference scheme amplifies any 1e-7 perturbation (FP32 fluxes) to ~1e-3 within 10 steps --
-    the FP64 GPU path against the FP64 oracle (FMA rounding only) already differs by ~1e-7 pointwise
-    there -- so that IC is reported by the benchmark, not held to 1e-4 here."""
+    degeneracy): relative L1 density error <= 1e-4."""
     out = {}
     for prec in (CC_FP64, CC_MIXED):
         p = ic.turbulence3d_params(32, precision=prec)
@@ -159,6 +159,32 @@ def test_mixed_vs_fp64_turbulence_tilted():
             out[prec] = (h.download(), h.divB_max())
     e = rel_l1(out[CC_MIXED][0]["rho"], out[CC_FP64][0]["rho"])
     assert e <= 1e-4, e
+    assert out[CC_MIXED][1] < 1e-12 and out[CC_FP64][1] < 1e-12
+
+
+def test_mixed_vs_fp64_on_the_turbulence_benchmark_ic():
+    """The 3D benchmark's own IC (beta 1 field along z) at 32^3 for 20 steps with its HLLD guard: the FP64
+    run stays finite and smooth (max E < 10), and mixed agrees with FP64 to the 1e-4 relative L1 bar in
+    every field (measured: 2e-6 for rho, <= 2e-5 for every field).  The largest pointwise difference,
+    relative to each field's maximum, is held below 5e-2 (measured 1.5e-2 in the momenta): a face whose
+    star-state denominator lies at the guard threshold takes HLL in one precision and HLLD in the other."""
+    out = {}
+    for prec in (CC_FP64, CC_MIXED):
+        p = ic.turbulence3d_params(32, precision=prec)
+        with M.Mhd3D(p) as h:
+            h.upload(ic.turbulence3d_ic(p))
+            h.run(20)
+            out[prec] = (h.download(), h.divB_max())
+    a, b = out[CC_MIXED][0], out[CC_FP64][0]
+    assert b["E"].max() < 10, b["E"].max()
+    l1 = {f: rel_l1(a[f], b[f]) for f in M.FIELDS}
+    linf = {f: rel_err(a[f], b[f]) for f in M.FIELDS}
+    print("config-3 IC 32^3 x 20 mixed vs fp64: L1 %s Linf %s divB %.1e/%.1e" % (
+        {k: "%.1e" % v for k, v in l1.items()}, {k: "%.1e" % v for k, v in linf.items()}, out[CC_MIXED][1],
+        out[CC_FP64][1]), flush=True)
+    for f in M.FIELDS:
+        assert l1[f] <= 1e-4, (f, l1[f])
+        assert linf[f] <= 5e-2, (f, linf[f])
     assert out[CC_MIXED][1] < 1e-12 and out[CC_FP64][1] < 1e-12
 
 
