@@ -941,8 +941,10 @@ struct cc_mhd2d_batch {
         const cc_mhd2d* h0 = h[0];
         for (const cc_mhd2d* x : h) {
             const Poisson2D& P = x->pois;
-            // the merged launches run one tile shape of their own; the instances' shapes may differ
-            if (!x->p.gravity_on || !P.tiled() || P.iters <= 0 || x->p.precision != h0->p.precision)
+            // the merged launches run one LDS tile shape of their own; the instances' shapes may differ.
+            // Wave-streamed solves stay per instance (each a branch of the batch graph).
+            if (!x->p.gravity_on || !P.tiled() || P.iters <= 0 || x->p.precision != h0->p.precision ||
+                P.wave_streamed())
                 return CC_OK;  // not mergeable: the instances keep their own SOR launches
         }
         mixed = h0->p.precision == CC_MIXED;
@@ -1267,6 +1269,7 @@ cc_status cc_poisson2d_sor(int Nx, int Ny, int periodic_x, int periodic_y, doubl
     cc_status st = P.solve(drho, dgx, dgy, 0, nullptr);
     if (st == CC_OK && phi) st = P.phi_mean_removed(P.phi, 0);
     hipError_t e = hipDeviceSynchronize();
+    if (st == CC_OK && e == hipSuccess) st = P.flow_error(0);
     if (st == CC_OK && e == hipSuccess) {
         if (gx) hipMemcpy(gx, dgx, N * 8, hipMemcpyDeviceToHost);
         if (gy) hipMemcpy(gy, dgy, N * 8, hipMemcpyDeviceToHost);

@@ -1264,7 +1264,12 @@ cc_status Poisson2D::init(int Nx_, int Ny_, bool px_, bool py_, double dx_, doub
             const int n = std::sscanf(e, "%d,%d,%c", &S_, &WJ_, &g);
             if (n >= 2) { t = SorTile{S_, WJ_, !(n == 3 && g == 'g'), n == 3 && g == 's'}; from_env = true; }
         }
-        if (!from_env && sor_autotune_enabled()) CC_TRY(autotuned_tile(t));
+        const bool ws = sor_ws_enabled();
+        if (ws) {
+            ws_ = new SorWs();
+            CC_TRY(ws_->init(Nx, Ny, iters, dx, dy, G, omega, precision));
+        }
+        if (!from_env && !ws && sor_autotune_enabled()) CC_TRY(autotuned_tile(t));
         if (!sor_tile_supported(t, mixed)) {
             set_error(CC_ERR_INVALID_ARG, "Poisson2D: unsupported SOR tile shape");
             return CC_ERR_INVALID_ARG;
@@ -1298,6 +1303,7 @@ cc_status Poisson2D::init(int Nx_, int Ny_, bool px_, bool py_, double dx_, doub
 }
 
 cc_status Poisson2D::flow_error(hipStream_t s) {
+    if (ws_) CC_TRY(ws_->error(s));
     if (!flow_) return CC_OK;
     int e = 0;
     CC_HIP(hipMemcpyAsync(&e, flow_state + 1, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1495,6 +1501,8 @@ void Poisson2D::destroy() {
     if (dep_off) hipFree(dep_off);
     if (dep_idx) hipFree(dep_idx);
     if (flow_state) hipFree(flow_state);
+    delete ws_;
+    ws_ = nullptr;
     phi = partial = rhobar = nullptr;
     sched = nullptr;
     dep_off = dep_idx = flow_state = nullptr;
@@ -1520,7 +1528,9 @@ cc_status Poisson2D::prologue(const double* rho, hipStream_t s, Profiler* prof) 
 }
 
 cc_status Poisson2D::sweeps(const double* rho, hipStream_t s, Profiler* prof) {
-    if (tiled_ && flow_) {
+    if (ws_) {
+        CC_TRY(ws_->solve(rho, rhobar, phi, s, prof));
+    } else if (tiled_ && flow_) {
         const int ntiles = tiles_total();
         ProfScope ps(prof, s, "sor");
         hipLaunchKernelGGL(k_flow_reset, dim3(std::min(64, (ntiles + 2 + 255) / 256)), dim3(256), 0, s, flow_state,

@@ -6,8 +6,10 @@
 // with the sweep (the stencil weights sum to the diagonal), so removing it once at the end
 // gives the same iterate and g does not depend on it at all.
 //
-// Two schedules reproduce the lexicographic iterate exactly (up to rounding):
-//  * clamped ("outflow") boundaries on both axes: a time-skewed, LDS-tiled wavefront.  Cell
+// Three schedules reproduce the lexicographic iterate exactly (up to rounding):
+//  * clamped ("outflow") boundaries on both axes: a time-skewed, LDS-tiled wavefront; or (CC_SOR_WS=1)
+//    wave-streamed tiles (sorws.hpp), one wave of 64 rows x SW sweeps streaming over the columns with its
+//    neighbours in registers.  Cell
 //    (i,j) of sweep s runs at wavefront step i+j+2s, i.e. every step updates one colour of a
 //    red-black checkerboard inside a moving band; tiles of 64 x WJ cells x S sweeps are
 //    parallelograms in (i+s, j+s, s) and run level by level (one launch per tile level).
@@ -22,6 +24,7 @@
 
 #include "common.hpp"
 #include "profiler.hpp"
+#include "sorws.hpp"
 
 namespace cc {
 
@@ -95,6 +98,8 @@ class Poisson2D {
     int* dep_idx = nullptr;
     int* flow_state = nullptr;     // [0] tile counter, [1] error flag, [2 ..] done flags
     SorArgs args{};
+    SorWs* ws_ = nullptr;          // the wave-streamed solve of a clamped grid (else the LDS tiles)
+    bool wave_streamed() const { return ws_ != nullptr; }
 };
 
 // Launch one level of a batched tiled SOR: nt tiles given as (instance, TS, TJ, TI), every instance

@@ -300,6 +300,7 @@ def test_sheared_tiles_match_oracle_and_rectangular_tiles(prec, S, K, monkeypatc
     tiles and a partial last sweep group -- FP64 against the oracle at 1e-9 and against the rectangular
     tiles at 1e-12, mixed at the 1e-5 bar of test_poisson_sor_mixed_close_to_oracle; one launch per level
     and the dataflow launch give the same bits."""
+    monkeypatch.setenv("CC_SOR_WS", "0")
     for Nx, Ny, iters in ((300, 170, 600), (130, 67, 37), (64, 64, 600), (200, 90, 45)):
         rng = np.random.default_rng(Nx + Ny + iters)
         p = M.Params(Nx=Nx, Ny=Ny, periodic_x=False, periodic_y=False, sor_iters=iters, G=1.0)
@@ -326,6 +327,7 @@ def test_sheared_tiles_match_oracle_and_rectangular_tiles(prec, S, K, monkeypatc
 @pytest.mark.parametrize("prec", [CC_FP64, CC_MIXED])
 def test_sheared_collapse_steps_match_oracle(prec, monkeypatch):
     """Collapse steps through the captured step graph with sheared SOR tiles (the dataflow launch)."""
+    monkeypatch.setenv("CC_SOR_WS", "0")
     monkeypatch.setenv("CC_SOR_TILE", "16,64,s")
     monkeypatch.delenv("CC_SOR_LEVELS", raising=False)
     N, nsteps = 96, 3
@@ -353,6 +355,7 @@ def test_sor_dataflow_equals_level_launches_bitwise(prec, monkeypatch):
     """The default tiled solve (one persistent launch, tiles started as their waits clear) and the
     one-launch-per-level solve (CC_SOR_LEVELS) give the same bits: a standalone solve on an uneven grid,
     and collapse steps run through the captured step graph."""
+    monkeypatch.setenv("CC_SOR_WS", "0")
     rng = np.random.default_rng(17)
     Nx, Ny = 300, 170
     rho = 1.0 + rng.uniform(0, 1, (Ny, Nx))
@@ -386,6 +389,7 @@ def test_batch_equals_instances_run_alone(prec, mode, monkeypatch):
     and the merged launches use sheared SOR tiles."""
     from core_collapse_amd.mhd2d import Mhd2D, Mhd2DBatch
     separate_sor = mode == "separate"
+    monkeypatch.delenv("CC_SOR_WS", raising=False)  # LDS tiles (wave-streamed solves are never merged)
     if mode == "merged_sheared":
         monkeypatch.setenv("CC_SOR_TILE", "16,64,s")
     elif mode == "merged":
@@ -491,10 +495,11 @@ def test_batch_of_64_sampled_collapses_equals_instances_run_alone():
 
 
 @pytest.mark.parametrize("prec", [CC_FP64, CC_MIXED])
-def test_sor_autotune_picks_a_compiled_shape_and_keeps_parity(prec):
+def test_sor_autotune_picks_a_compiled_shape_and_keeps_parity(prec, monkeypatch):
     """cc_sor_autotune: the handle times every compiled tile shape once for its grid and runs with the
     fastest; the solve is the same lexicographic iterate whatever the shape (FP64 against the oracle at
     the tolerance of test_collapse_run_matches_oracle, mixed at the 1e-4 L1 bar)."""
+    monkeypatch.setenv("CC_SOR_WS", "0")
     N, nsteps = 80, 4
     p = ic.collapse2d_params(N, precision=prec)
     st = ic.collapse2d_ic(p)

@@ -1,579 +1,300 @@
-"""CPU proof of the GPU SOR schedules (core_collapse_amd/csrc/poisson2d.hip).
+"""The wave-streamed SOR schedule of core_collapse_amd/csrc/sorws.hip, modelled on the CPU.
 
-The tiled kernel claims to reproduce lexicographic Gauss-Seidel SOR (M2:207-218) with clamped
-boundaries; the periodic / mixed-boundary kernels (time skew c = max(Nx, Ny), one thread per row)
-claim the same with wrap-around neighbours (the last two tests).  Here the kernel's schedule is emulated cell by cell in Python (same tile geometry,
-same level order, same per-cell formula) and compared bit for bit with the plain lexicographic
-loop; a second check shows that no two tiles of one level touch a cell that the other writes,
-so the order in which the GPU runs the tiles of a level cannot change the result.
+The model follows the kernel step for step: the same register rings and slots, the same loads (issued when
+the kernel issues them, kWsPD steps ahead), stores, records, waits and publications, with each tile a
+generator that a random scheduler advances one action at a time whenever the wait before that action is
+satisfied.  Memory is checked for races the way the hardware could expose them: a store stays private to
+its tile until the tile publishes; a load of a location that another tile has stored to and not yet
+published returns NaN (a race whose value the sweep would then carry into the result), and a store to a
+location another tile has loaded and not yet published (its load may still be in flight) fails the test.
+
+Pinned against a direct lexicographic sweep written from the reference's loop (clamped neighbours,
+M2:196-223) -- FP64: bit for bit; mixed: against the same sweep written as per-block iterative refinement
+(phi = phi_b + d, FP64 residual of phi_b, FP32 sweeps of d), bit for bit, and within FP32 rounding of the
+FP64 sweep.  Grids cover interior and edge tiles, a partial last sweep block, widths below the tile
+height, one-row and one-column grids.
 """
 import numpy as np
 import pytest
 
-from core_collapse_amd import mhd2d as M
+W = 64
+PD, PUB = 6, 12   # kWsPD, steps per publication
+RS, RP = 12, 9    # kWsRS, kWsRP: record ring slots, steps a record is loaded ahead
 
 
-def cell_update(phi, rhs, i, j, Nx, Ny, idx2, idy2, inv_denom, omega):
-    pE = phi[j, min(i + 1, Nx - 1)]
-    pW = phi[j, max(i - 1, 0)]
-    pN = phi[min(j + 1, Ny - 1), i]
-    pS = phi[max(j - 1, 0), i]
-    pc = phi[j, i]
-    pnew = ((pE + pW) * idx2 + (pN + pS) * idy2 - rhs[j, i]) * inv_denom
-    phi[j, i] = pc + omega * (pnew - pc)
-
-
-def lexicographic(rhs, nsw, idx2, idy2, omega):
-    Ny, Nx = rhs.shape
-    phi = np.zeros_like(rhs)
-    inv = 1.0 / (2.0 * (idx2 + idy2))
+def lex_sor(src, nsw, kx, ky, om, dtype=np.float64, x0=None):
+    """Lexicographic SOR from x0 (zero), clamped neighbours, the kernel's operation order."""
+    Ny, Nx = src.shape
+    x = np.zeros((Ny, Nx), dtype) if x0 is None else x0.astype(dtype).copy()
+    kx, ky, om = dtype(kx), dtype(ky), dtype(om)
     for _ in range(nsw):
-        for j in range(Ny):
-            for i in range(Nx):
-                cell_update(phi, rhs, i, j, Nx, Ny, idx2, idy2, inv, omega)
+        for r in range(Ny):
+            for c in range(Nx):
+                s = x[r, c]
+                Wv = x[r, c - 1] if c > 0 else s
+                Ev = x[r, c + 1] if c < Nx - 1 else s
+                Sv = x[r - 1, c] if r > 0 else s
+                Nv = x[r + 1, c] if r < Ny - 1 else s
+                x[r, c] = s + om * (((src[r, c] + kx * (Ev + Wv)) + ky * (Nv + Sv)) - s)
+    return x
+
+
+def block_refined_sor(src, nsw, SW, kx, ky, om):
+    """The mixed form: per block of SW sweeps, the FP64 residual of the block's base phi_b (clamped), SOR
+    sweeps of an FP32 correction d from zero with that residual as source, then phi = phi_b + d in FP64."""
+    Ny, Nx = src.shape
+    phi = np.zeros((Ny, Nx))
+    for s0 in range(0, nsw, SW):
+        b = phi
+        E = np.concatenate([b[:, 1:], b[:, -1:]], 1)
+        Wb = np.concatenate([b[:, :1], b[:, :-1]], 1)
+        N = np.concatenate([b[1:], b[-1:]], 0)
+        S = np.concatenate([b[:1], b[:-1]], 0)
+        res = (((src + kx * (E + Wb)) + ky * (N + S)) - b).astype(np.float32)
+        d = lex_sor(res, min(SW, nsw - s0), kx, ky, om, np.float32)
+        phi = b + d.astype(np.float64)
     return phi
 
 
-def tile_levels(Nx, Ny, nsw, W, WJ, S):
-    """The host-side schedule of Poisson2D::init: non-empty tiles grouped by TS+TJ+TI."""
-    nTS = (nsw + S - 1) // S
-    nTI = (Nx + nsw - 1 + W - 1) // W
-    nTJ = (Ny + nsw - 1 + WJ - 1) // WJ
-    levels = {}
-    for TS in range(nTS):
-        s0, s1 = S * TS, min(S * TS + S, nsw)
-        for TJ in range(nTJ):
-            for TI in range(nTI):
-                ok = any(W * TI - s + W - 1 >= 0 and W * TI - s <= Nx - 1 and WJ * TJ - s + WJ - 1 >= 0 and
-                         WJ * TJ - s <= Ny - 1 for s in range(s0, s1))
-                if ok:
-                    levels.setdefault(TS + TJ + TI, []).append((TS, TJ, TI))
-    return [levels[k] for k in sorted(levels)]
+class Mem:
+    """A flat array with private (unpublished) stores and in-flight loads per tile."""
+
+    def __init__(self, n, ntiles, dtype):
+        self.vis = np.zeros(n, dtype)
+        self.cur = np.zeros(n, dtype)
+        self.writer = np.full(n, -1)
+        self.inflight = np.zeros((ntiles, n), bool)
+
+    def load(self, t, idx):
+        idx = np.asarray(idx)
+        assert idx.min() >= 0 and idx.max() < self.vis.size, "load out of bounds"
+        w = self.writer[idx]
+        out = np.where(w == t, self.cur[idx], self.vis[idx])
+        out = np.where((w >= 0) & (w != t), np.nan, out)
+        self.inflight[t, idx] = True
+        return out
+
+    def store(self, t, idx, val):
+        idx = np.asarray(idx)
+        if idx.size == 0:
+            return
+        assert idx.min() >= 0 and idx.max() < self.vis.size, "store out of bounds"
+        others = np.delete(self.inflight[:, idx], t, axis=0)
+        assert not others.any(), "store to a location with another tile's load in flight"
+        assert not ((self.writer[idx] >= 0) & (self.writer[idx] != t)).any(), "two unpublished stores"
+        self.cur[idx] = val
+        self.writer[idx] = t
+
+    def publish(self, t):
+        m = self.writer == t
+        self.vis[m] = self.cur[m]
+        self.writer[m] = -1
+        self.inflight[t] = False
 
 
-def tile_cells(tile, Nx, Ny, nsw, W, WJ, S):
-    """(step, sweep, i, j) of every update of a tile, in the kernel's step order."""
-    TS, TJ, TI = tile
-    s0, s1 = S * TS, min(S * TS + S, nsw)
-    out = []
-    for d in range(W + WJ - 1):
-        for s in range(s0, s1):
-            for lane in range(W):
-                lj = d - lane
-                if not (0 <= lj < WJ):
-                    continue
-                i, j = W * TI - s + lane, WJ * TJ - s + lj
-                if 0 <= i < Nx and 0 <= j < Ny:
-                    out.append((d, s, i, j))
-    return out
+class Model:
+    def __init__(self, src, nsw, SW, kx, ky, om, mixed):
+        Ny, Nx = src.shape
+        self.Nx, self.Ny, self.nsw, self.SW = Nx, Ny, nsw, SW
+        self.kx, self.ky, self.om, self.mixed = kx, ky, om, mixed
+        self.R = np.float32 if mixed else np.float64
+        self.NB = (Ny + SW - 1 + 63) // 64
+        self.pitch = 64 * self.NB + 2
+        self.K = Nx + 64 * self.NB + SW + 16
+        self.ntiles = self.NB * ((nsw + SW - 1) // SW)
+        self.RECN = 2 * SW + (4 if mixed else 0)  # record: SW values, SW source terms, (mixed) FP64 base, pad
+        self.rec_stride = (Nx + SW + 160) * self.RECN
+        n = self.K * self.pitch
+        self.D = Mem(n, self.ntiles, np.float64)
+        self.C = np.zeros(n)
+        for r in range(Ny):
+            for c in range(Nx):
+                self.C[(c + r) * self.pitch + r] = src[r, c]
+        self.rec = Mem(self.ntiles * self.rec_stride, self.ntiles, self.R)
+        self.progress = np.zeros(self.ntiles, np.int64)
 
+    def publish(self, t, g):
+        self.D.publish(t)
+        self.rec.publish(t)
+        self.progress[t] = g
 
-def tile_updates(i, j, s0, s1, TI, TJ, W, WJ):
-    """The kernel's write-back predicate (tile_updates in poisson2d.hip)."""
-    lo = max(s0, W * TI - i, WJ * TJ - j)
-    hi = min(s1 - 1, W * TI - i + W - 1, WJ * TJ - j + WJ - 1)
-    return lo <= hi
+    # -------------------------------------------------------------- one tile, as the kernel runs it
+    def tile(self, tile):
+        SW, R, P = self.SW, self.R, self.pitch
+        q, b = divmod(tile, self.NB)
+        nm = min(SW, self.nsw - q * SW)
+        gbase = 64 * b + 2 * SW * q
+        pB = tile - 1 if b > 0 else -1
+        pL = tile - self.NB if q > 0 else -1
+        pD = tile - self.NB + 1 if (q > 0 and b + 1 < self.NB) else -1
+        Tend = (self.Nx + 63 + nm + 11) // 12 * 12
+        lane = np.arange(W)
+        r0 = 64 * b + lane
+        X = np.zeros((3, SW, W), R)
+        Sr = np.zeros((3, SW, W), R)
+        RX = np.zeros((RS, SW, W), R)   # record ring (broadcast form; the kernel keeps it spread over lanes)
+        RSr = np.zeros((RS, SW, W), R)
+        E, N, Cc, Bw = (np.zeros((PD, W)) for _ in range(4))
+        RB = np.zeros((RS, W))
+        RECN = self.RECN
 
+        def prefetch(T):
+            e = T % PD
+            k = T + 64 * b
+            E[e] = self.D.load(tile, (k + 1) * P + r0)
+            N[e] = self.D.load(tile, (k + 1) * P + r0 + 1)
+            Cc[e] = self.C[k * P + r0]
+            if self.mixed:
+                kw, rw = k - 2 * (nm - 1), r0 - (nm - 1)
+                Bw[e] = self.D.load(tile, np.where((rw >= 0) & (kw >= 0), kw * P + rw, k * P + r0))
 
-def run_tile(phi, rhs, tile, nsw, idx2, idy2, omega, W, WJ, S):
-    """One tile of the kernel: copy the tile's box, update the copy in step order, write back the
-    cells the predicate selects."""
-    Ny, Nx = rhs.shape
-    inv = 1.0 / (2.0 * (idx2 + idy2))
-    TS, TJ, TI = tile
-    s0, s1 = S * TS, min(S * TS + S, nsw)
-    box = phi.copy()
-    for (_, _, i, j) in tile_cells(tile, Nx, Ny, nsw, W, WJ, S):
-        cell_update(box, rhs, i, j, Nx, Ny, idx2, idy2, inv, omega)
-    ib0, ib1 = max(0, W * TI - (s1 - 1) - 1), min(Nx - 1, W * TI - s0 + W)
-    jb0, jb1 = max(0, WJ * TJ - (s1 - 1) - 1), min(Ny - 1, WJ * TJ - s0 + WJ)
-    changed = box != phi
-    for j in range(jb0, jb1 + 1):
-        for i in range(ib0, ib1 + 1):
-            if tile_updates(i, j, s0, s1, TI, TJ, W, WJ):
-                phi[j, i] = box[j, i]
-    # every cell the tile changed lies in its box and passes the predicate
-    cj, ci = np.nonzero(changed)
-    assert all(ib0 <= i <= ib1 and jb0 <= j <= jb1 and tile_updates(i, j, s0, s1, TI, TJ, W, WJ)
-               for i, j in zip(ci, cj))
+        def load_record(T, slot):  # the producer's local step T (uniform address: every lane the same)
+            assert (T + 1) * RECN <= self.rec_stride, "record outside the producer's"
+            base = pB * self.rec_stride + T * RECN
+            v = self.rec.load(tile, np.arange(base, base + RECN))
+            RX[slot] = v[:SW, None]
+            RSr[slot] = v[SW:2 * SW, None]
+            if self.mixed:
+                RB[slot] = v[2 * SW:2 * SW + 2].view(np.float64)[0]
 
+        def shr(v, old):
+            o = np.empty(W, v.dtype)
+            o[1:] = v[:-1]
+            o[0] = old[0]
+            return o
 
-def tiled(rhs, nsw, idx2, idy2, omega, W, WJ, S, reverse_levels=False):
-    """The kernel's tiles in level order (each level's tiles in either order)."""
-    Ny, Nx = rhs.shape
-    phi = np.zeros_like(rhs)
-    for lev in tile_levels(Nx, Ny, nsw, W, WJ, S):
-        for tile in (reversed(lev) if reverse_levels else lev):
-            run_tile(phi, rhs, tile, nsw, idx2, idy2, omega, W, WJ, S)
-    return phi
+        state = {"Eprev": None, "Eprev2": np.zeros(W)}
 
-
-@pytest.mark.parametrize("Nx,Ny,nsw,W,WJ,S", [(9, 7, 5, 4, 3, 2), (11, 13, 9, 4, 4, 3), (6, 10, 7, 3, 5, 4),
-                                               (70, 66, 18, 64, 64, 16)])
-def test_tiled_schedule_equals_lexicographic_bitwise(Nx, Ny, nsw, W, WJ, S):
-    rng = np.random.default_rng(Nx * 100 + Ny)
-    rhs = rng.standard_normal((Ny, Nx))
-    idx2, idy2 = float(Nx * Nx), float(Ny * Ny)
-    ref = lexicographic(rhs, nsw, idx2, idy2, 1.8)
-    for rev in (False, True):
-        got = tiled(rhs, nsw, idx2, idy2, 1.8, W, WJ, S, reverse_levels=rev)
-        assert np.array_equal(got, ref)
-
-
-@pytest.mark.parametrize("Nx,Ny,nsw,W,WJ,S", [(9, 7, 9, 4, 3, 2), (13, 11, 12, 4, 4, 3), (150, 140, 40, 64, 64, 16),
-                                               (150, 300, 70, 64, 64, 32), (150, 300, 70, 64, 128, 32)])
-def test_no_conflicts_within_a_level(Nx, Ny, nsw, W, WJ, S):
-    for lev in tile_levels(Nx, Ny, nsw, W, WJ, S):
-        writes, reads = [], []
-        for tile in lev:
-            cells = tile_cells(tile, Nx, Ny, nsw, W, WJ, S)
-            w = {(i, j) for (_, _, i, j) in cells}
-            r = set(w)
-            for (_, _, i, j) in cells:
-                r |= {(min(i + 1, Nx - 1), j), (max(i - 1, 0), j), (i, min(j + 1, Ny - 1)), (i, max(j - 1, 0))}
-            writes.append(w)
-            reads.append(r)
-        for a in range(len(lev)):
-            for b in range(len(lev)):
-                if a != b:
-                    assert not (writes[a] & reads[b]), (lev[a], lev[b])
-
-
-def test_every_update_happens_once():
-    Nx, Ny, nsw, W, WJ, S = 37, 29, 23, 8, 6, 4
-    count = {}
-    for lev in tile_levels(Nx, Ny, nsw, W, WJ, S):
-        for tile in lev:
-            for (_, s, i, j) in tile_cells(tile, Nx, Ny, nsw, W, WJ, S):
-                count[(s, i, j)] = count.get((s, i, j), 0) + 1
-    assert len(count) == nsw * Nx * Ny and set(count.values()) == {1}
-
-
-def tiled_ghost(rhs, nsw, idx2, idy2, omega, W, WJ, S, reverse_levels=False):
-    """The kernel as written since the ghost-margin form: the tile's box is copied with a one-cell
-    margin; margin cells on a domain edge mirror the boundary cell (clamped neighbour) and are
-    rewritten together with it, margin cells inside the domain are never read (NaN here, so a read
-    would show); every stencil read is an unclamped box offset.  Each lane walks its column over the
-    steps [lane + ljlo, lane + ljhi) that the kernel computes once per lane."""
-    Ny, Nx = rhs.shape
-    phi = np.zeros_like(rhs)
-    inv = 1.0 / (2.0 * (idx2 + idy2))
-    for lev in tile_levels(Nx, Ny, nsw, W, WJ, S):
-        for tile in (reversed(lev) if reverse_levels else lev):
-            TS, TJ, TI = tile
-            s0, s1 = S * TS, min(S * TS + S, nsw)
-            ib0, ib1 = max(0, W * TI - (s1 - 1) - 1), min(Nx - 1, W * TI - s0 + W)
-            jb0, jb1 = max(0, WJ * TJ - (s1 - 1) - 1), min(Ny - 1, WJ * TJ - s0 + WJ)
-            ni, nj = ib1 - ib0 + 1, jb1 - jb0 + 1
-            assert ni + 2 <= W + S + 3 and nj + 2 <= WJ + S + 3
-            box = np.full((nj + 2, ni + 2), np.nan)
-            box[1:-1, 1:-1] = phi[jb0:jb1 + 1, ib0:ib1 + 1]
-            if ib0 == 0:
-                box[1:-1, 0] = box[1:-1, 1]
-            if ib1 == Nx - 1:
-                box[1:-1, -1] = box[1:-1, -2]
-            if jb0 == 0:
-                box[0, 1:-1] = box[1, 1:-1]
-            if jb1 == Ny - 1:
-                box[-1, 1:-1] = box[-2, 1:-1]
-            for d in range(W + WJ - 1):
-                for s in range(s0, s1):
-                    for lane in range(W):
-                        ci = W * TI - s + lane
-                        if not (0 <= ci < Nx):
-                            continue
-                        ljlo, ljhi = max(0, s - WJ * TJ), min(WJ, Ny - WJ * TJ + s)
-                        if not (lane + ljlo <= d < lane + ljhi):
-                            continue
-                        cj = WJ * TJ - s + d - lane
-                        assert 0 <= cj < Ny
-                        x, y = ci - ib0 + 1, cj - jb0 + 1
-                        assert 1 <= x <= ni and 1 <= y <= nj  # LDS reads stay inside the box + margin
-                        pc = box[y, x]
-                        pnew = ((box[y, x + 1] + box[y, x - 1]) * idx2 + (box[y + 1, x] + box[y - 1, x]) * idy2
-                                - rhs[cj, ci]) * inv
-                        v = pc + omega * (pnew - pc)
-                        assert np.isfinite(v)
-                        box[y, x] = v
-                        if ci == 0:
-                            box[y, x - 1] = v
-                        if ci == Nx - 1:
-                            box[y, x + 1] = v
-                        if cj == 0:
-                            box[y - 1, x] = v
-                        if cj == Ny - 1:
-                            box[y + 1, x] = v
-            for j in range(jb0, jb1 + 1):
-                for i in range(ib0, ib1 + 1):
-                    if tile_updates(i, j, s0, s1, TI, TJ, W, WJ):
-                        phi[j, i] = box[j - jb0 + 1, i - ib0 + 1]
-    return phi
-
-
-@pytest.mark.parametrize("Nx,Ny,nsw,W,WJ,S", [(9, 7, 5, 4, 3, 2), (11, 13, 9, 4, 4, 3), (10, 9, 12, 4, 8, 4),
-                                               (21, 19, 13, 8, 16, 8), (70, 40, 9, 64, 64, 4)])
-def test_ghost_margin_kernel_equals_lexicographic_bitwise(Nx, Ny, nsw, W, WJ, S):
-    rng = np.random.default_rng(Nx * 100 + Ny + 7)
-    rhs = rng.standard_normal((Ny, Nx))
-    idx2, idy2 = float(Nx * Nx), float(Ny * Ny)
-    ref = lexicographic(rhs, nsw, idx2, idy2, 1.8)
-    for rev in (False, True):
-        assert np.array_equal(tiled_ghost(rhs, nsw, idx2, idy2, 1.8, W, WJ, S, reverse_levels=rev), ref)
-
-
-# ------------------------------------------------------------------ periodic / mixed boundaries
-# k_sor_wavefront_1wg and k_sor_skew_rows: cell (i, j) of sweep s at step i + j + c s, c = max(Nx, Ny).
-
-def lexicographic_bc(rhs, nsw, idx2, idy2, omega, px, py):
-    Ny, Nx = rhs.shape
-    phi = np.zeros_like(rhs)
-    inv = 1.0 / (2.0 * (idx2 + idy2))
-    bm = lambda i, n, p: (i % n) if p else min(max(i, 0), n - 1)  # noqa: E731
-    for _ in range(nsw):
-        for j in range(Ny):
-            for i in range(Nx):
-                pE, pW = phi[j, bm(i + 1, Nx, px)], phi[j, bm(i - 1, Nx, px)]
-                pN, pS = phi[bm(j + 1, Ny, py), i], phi[bm(j - 1, Ny, py), i]
-                pc = phi[j, i]
-                pnew = ((pE + pW) * idx2 + (pN + pS) * idy2 - rhs[j, i]) * inv
-                phi[j, i] = pc + omega * (pnew - pc)
-    return phi
-
-
-def skewed_bc(rhs, nsw, idx2, idy2, omega, px, py, order):
-    """The time-skewed schedule; within a step the cells are updated in `order` (0: rows up,
-    1: rows down, 2: all reads before all writes) -- equal results show they are independent."""
-    Ny, Nx = rhs.shape
-    phi = np.zeros_like(rhs)
-    inv = 1.0 / (2.0 * (idx2 + idy2))
-    bm = lambda i, n, p: (i % n) if p else min(max(i, 0), n - 1)  # noqa: E731
-    c = max(Nx, Ny)
-    for T in range(c * (nsw - 1) + Nx + Ny - 1):
-        pending = []
-        for j in (range(Ny) if order != 1 else range(Ny - 1, -1, -1)):
-            s, i = divmod(T - j, c)
-            if T - j < 0 or s >= nsw or i >= Nx:
-                continue
-            pE, pW = phi[j, bm(i + 1, Nx, px)], phi[j, bm(i - 1, Nx, px)]
-            pN, pS = phi[bm(j + 1, Ny, py), i], phi[bm(j - 1, Ny, py), i]
-            pc = phi[j, i]
-            v = pc + omega * (((pE + pW) * idx2 + (pN + pS) * idy2 - rhs[j, i]) * inv - pc)
-            if order == 2:
-                pending.append((j, i, v))
-            else:
-                phi[j, i] = v
-        for j, i, v in pending:
-            phi[j, i] = v
-    return phi
-
-
-def skew_rows(rhs, nsw, idx2, idy2, omega, px, py, PD, loads_first=False):
-    """k_sor_skew_rows step by step: one thread per row, west from the thread's previous result,
-    south through the double-buffered LDS exchange, periodic seams from registers and the rings of
-    rows 0 and Ny-1, the rest loaded PD steps ahead (a load sees memory as it is when issued).
-    loads_first issues a step's loads before the step's stores: the kernel orders them only within
-    a thread, so equal results show no load reads a cell stored in the same step."""
-    Ny, Nx = rhs.shape
-    N, RING = Nx * Ny, 2048
-    phi = np.zeros(N + 64)
-    r = rhs.reshape(-1)
-    inv = 1.0 / (2.0 * (idx2 + idy2))
-    c = max(Nx, Ny)
-    steps = c * (nsw - 1) + Nx + Ny - 1
-    xs = np.zeros((2, 64)); ring0 = np.zeros(RING); ringN = np.zeros(RING)
-    out = np.zeros(64); first = np.zeros(64); wlast = np.zeros(64)
-    pos = [[-j, 0] for j in range(64)]
-
-    def issue(j, ii, ss):
-        act = j < Ny and ss < nsw and 0 <= ii < Nx
-        k = j * Nx + (ii if act else 0)
-        dump = N + j
-        return (phi[k + 1 if act and ii + 1 < Nx else dump], phi[k + Nx if act and j + 1 < Ny else dump],
-                phi[k if act else dump], r[k if act else 0])
-
-    q = [[None] * 64 for _ in range(PD)]
-    pre = [[-j, 0] for j in range(64)]
-    for u in range(PD):
-        for j in range(64):
-            q[u][j] = issue(j, *pre[j])
-            pre[j][0] += 1
-            if pre[j][0] == c:
-                pre[j] = [0, pre[j][1] + 1]
-    for T in range(steps):
-        u, par = T % PD, T & 1
-        stores = []
-        for j in range(64):
-            i, s = pos[j]
-            E0, N0, pc, rr = q[u][j]
-            act = j < Ny and s < nsw and 0 <= i < Nx
-            Sr = ((0.0 if s == 0 else ringN[(T - (c - Ny + 1)) % RING]) if py else pc)
-            S = xs[par ^ 1][j - 1] if j > 0 else Sr
-            Nb = N0 if j < Ny - 1 else (ring0[(T - (Ny - 1)) % RING] if py else pc)
-            E = E0 if i < Nx - 1 else (first[j] if px else pc)
-            W = out[j] if i > 0 else ((0.0 if s == 0 else wlast[j]) if px else pc)
-            with np.errstate(all="ignore"):
-                v = pc + omega * (((E + W) * idx2 + (Nb + S) * idy2 - rr) * inv - pc)
-            stores.append((j * Nx + i if act else N + j, v))
-            xs[par][j] = v
-            if j == 0:
-                ring0[T % RING] = v
-            if j == Ny - 1:
-                ringN[T % RING] = v
-            first[j] = v if i == 0 else first[j]
-            wlast[j] = v if i == Nx - 1 else wlast[j]
-            out[j] = v
-        if not loads_first:
-            for k, v in stores:
-                phi[k] = v
-        for j in range(64):
-            q[u][j] = issue(j, *pre[j])
-            pre[j][0] += 1
-            if pre[j][0] == c:
-                pre[j] = [0, pre[j][1] + 1]
-            pos[j][0] += 1
-            if pos[j][0] == c:
-                pos[j] = [0, pos[j][1] + 1]
-        if loads_first:
-            for k, v in stores:
-                phi[k] = v
-    return phi[:N].reshape(Ny, Nx)
-
-
-@pytest.mark.parametrize("Nx,Ny,px,py,nsw", [(12, 12, 1, 1, 5), (10, 17, 1, 1, 4), (19, 8, 1, 1, 4), (15, 9, 1, 0, 4),
-                                             (9, 14, 0, 1, 4), (13, 13, 0, 0, 3), (2, 5, 1, 1, 3)])
-def test_time_skewed_periodic_schedule_equals_lexicographic_bitwise(Nx, Ny, px, py, nsw):
-    rng = np.random.default_rng(Nx * 31 + Ny)
-    rhs = 4.0 * np.pi * (rng.uniform(0, 1, (Ny, Nx)) - 0.5)
-    want = lexicographic_bc(rhs, nsw, float(Nx * Nx), float(Ny * Ny), 1.8, px, py)
-    for order in (0, 1, 2):
-        assert np.array_equal(skewed_bc(rhs, nsw, float(Nx * Nx), float(Ny * Ny), 1.8, px, py, order), want)
-
-
-@pytest.mark.parametrize("Nx,Ny,px,py,nsw,PD", [(18, 18, 1, 1, 4, 8), (20, 30, 1, 0, 3, 8), (40, 7, 1, 1, 3, 8),
-                                                (12, 12, 1, 1, 5, 2), (9, 14, 0, 1, 4, 4), (25, 19, 0, 0, 3, 8)])
-def test_row_thread_kernel_equals_lexicographic_bitwise(Nx, Ny, px, py, nsw, PD):
-    if max(Nx, Ny) < 2 * PD + 2:
-        pytest.skip("the kernel needs c >= 2 PD + 2")
-    rng = np.random.default_rng(Nx * 7 + Ny)
-    rhs = 4.0 * np.pi * (rng.uniform(0, 1, (Ny, Nx)) - 0.5)
-    want = lexicographic_bc(rhs, nsw, float(Nx * Nx), float(Ny * Ny), 1.8, px, py)
-    for loads_first in (False, True):
-        got = skew_rows(rhs, nsw, float(Nx * Nx), float(Ny * Ny), 1.8, px, py, PD, loads_first)
-        assert np.array_equal(got, want), loads_first
-
-
-# ------------------------------------------------------------------ dataflow schedule
-# The default tiled solve is one persistent launch: workgroups take tiles in launch (level) order from
-# an atomic counter and start a tile once every tile it waits for has set its done flag.  The waits
-# come from the library itself (cc_sor_flow_schedule, the host code that builds the device tables).
-
-
-def _footprint(cells, Nx, Ny):
-    w = {(i, j) for (_, _, i, j) in cells}
-    r = set(w)
-    for (i, j) in w:
-        r |= {(min(i + 1, Nx - 1), j), (max(i - 1, 0), j), (i, min(j + 1, Ny - 1)), (i, max(j - 1, 0))}
-    return w, r
-
-
-FLOW_CASES = [(9, 7, 9, 4, 3, 2), (13, 11, 12, 4, 4, 3), (11, 13, 9, 4, 4, 3), (20, 6, 30, 5, 2, 7),
-              (6, 20, 17, 3, 6, 5), (150, 140, 40, 64, 64, 16), (100, 200, 70, 64, 128, 32)]
-
-
-def _geometry(sheared):
-    """(levels, cells of a tile, the kernel's tile body) of rectangular or sheared tiles."""
-    if sheared:
-        return sheared_levels, sheared_cells, sheared_run_tile
-    return tile_levels, tile_cells, run_tile
-
-
-@pytest.mark.parametrize("sheared", [False, True])
-@pytest.mark.parametrize("Nx,Ny,nsw,W,WJ,S", FLOW_CASES)
-def test_dataflow_waits_order_every_conflicting_pair(Nx, Ny, nsw, W, WJ, S, sheared):
-    """Two tiles of which one writes a cell the other reads are never in flight together: the later
-    one (in level order) waits for the earlier one, directly or through a chain of waits.  Pairs
-    without such a cell commute, so every execution the waits allow gives the level order's result."""
-    levels, cells_of, _ = _geometry(sheared)
-    tiles, deps = M.sor_flow_schedule(Nx, Ny, nsw, W, WJ, S, sheared=sheared)
-    assert tiles == [t for lev in levels(Nx, Ny, nsw, W, WJ, S) for t in lev]
-    n = len(tiles)
-    anc = []
-    for k in range(n):
-        a = set()
-        for d in deps[k]:
-            # a wait points to a tile below in every coordinate, hence earlier in launch order: the
-            # tiles a workgroup waits for were handed out before its own, so the waits cannot deadlock
-            assert d < k and tiles[d] != tiles[k] and all(tiles[d][c] <= tiles[k][c] for c in range(3))
-            a |= {d} | anc[d]
-        anc.append(a)
-    fp = [_footprint(cells_of(t, Nx, Ny, nsw, W, WJ, S), Nx, Ny) for t in tiles]
-    for b in range(n):
-        for a in range(b):
-            if fp[a][0] & fp[b][1] or fp[b][0] & fp[a][1]:
-                assert a in anc[b], (tiles[a], tiles[b])
-    # the waits are minimal: no wait is implied by another one
-    for k in range(n):
-        for d in deps[k]:
-            assert not any(d in anc[e] for e in deps[k] if e != d)
-
-
-@pytest.mark.parametrize("sheared", [False, True])
-@pytest.mark.parametrize("Nx,Ny,nsw,W,WJ,S", FLOW_CASES[:5])
-def test_dataflow_random_orders_equal_lexicographic_bitwise(Nx, Ny, nsw, W, WJ, S, sheared):
-    _, _, body = _geometry(sheared)
-    tiles, deps = M.sor_flow_schedule(Nx, Ny, nsw, W, WJ, S, sheared=sheared)
-    rng = np.random.default_rng(Nx * 31 + Ny)
-    rhs = rng.standard_normal((Ny, Nx))
-    idx2, idy2 = float(Nx * Nx), float(Ny * Ny)
-    ref = lexicographic(rhs, nsw, idx2, idy2, 1.8)
-    for _ in range(3):
-        phi = np.zeros_like(rhs)
-        done = [False] * len(tiles)
-        while not all(done):
-            ready = [k for k in range(len(tiles)) if not done[k] and all(done[d] for d in deps[k])]
-            k = ready[rng.integers(len(ready))]
-            body(phi, rhs, tiles[k], nsw, idx2, idy2, 1.8, W, WJ, S)
-            done[k] = True
-        assert np.array_equal(phi, ref)
-
-
-def test_dataflow_schedule_rejects_bad_arguments():
-    from core_collapse_amd._lib import CCError
-    with pytest.raises(CCError):
-        M.sor_flow_schedule(0, 8, 10)
-    with pytest.raises(CCError):
-        M.sor_flow_schedule(8, 8, 10, W=0)
-
-
-# ------------------------------------------------------------------ sheared tiles
-# A cell (i, j) of sweep s sits at k = i + j + 2s on the skewed time axis; every value it reads was
-# written at k - 1 or k - 2.  A sheared tile is a box of S sweeps x W values of i' = i + s x K values
-# of k: at step d (k = K TK + d) lane l of sweep s updates i = W TI + l - s, j = k - W TI - l - s, so
-# every lane works at every step of an interior tile.  Its LDS box is kept in sheared coordinates
-# (x, y) = (i - ib + 1, i + j - ub + 1): E/W neighbours are (x +- 1, y +- 1), N/S are (x, y +- 1).
-
-
-def sheared_levels(Nx, Ny, nsw, W, K, S):
-    """Non-empty sheared tiles (TS, TK, TI) grouped by TS + TK + TI (the host schedule)."""
-    nTS = (nsw + S - 1) // S
-    nTI = (Nx + nsw - 1 + W - 1) // W
-    nTK = (Nx + Ny + 2 * nsw - 3 + K - 1) // K
-    levels = {}
-    for TS in range(nTS):
-        s0, s1 = S * TS, min(S * TS + S, nsw)
-        for TK in range(nTK):
-            for TI in range(nTI):
-                ok = False
-                for s in range(s0, s1):
-                    imin, imax = max(0, W * TI - s), min(Nx - 1, W * TI + W - 1 - s)
-                    if imin > imax:
-                        continue
-                    jmin, jmax = max(0, K * TK - imax - 2 * s), min(Ny - 1, K * TK + K - 1 - imin - 2 * s)
-                    if jmin <= jmax:
-                        ok = True
-                        break
-                if ok:
-                    levels.setdefault(TS + TK + TI, []).append((TS, TK, TI))
-    return [levels[k] for k in sorted(levels)]
-
-
-def sheared_cells(tile, Nx, Ny, nsw, W, K, S):
-    TS, TK, TI = tile
-    s0, s1 = S * TS, min(S * TS + S, nsw)
-    out = []
-    for d in range(K):
-        for s in range(s0, s1):
-            for lane in range(W):
-                i, j = W * TI + lane - s, K * TK + d - W * TI - lane - s
-                if 0 <= i < Nx and 0 <= j < Ny:
-                    out.append((d, s, i, j))
-    return out
-
-
-def sheared_updates(i, j, s0, s1, TI, TK, W, K):
-    """The kernel's write-back predicate: some sweep s in [s0, s1) puts (i, j) inside the tile."""
-    u = i + j
-    lo = max(s0, W * TI - i, (K * TK - u + 1) >> 1)
-    hi = min(s1 - 1, W * TI + W - 1 - i, (K * TK + K - 1 - u) >> 1)
-    return lo <= hi
-
-
-def sheared_run_tile(phi, rhs, tile, nsw, idx2, idy2, omega, W, K, S):
-    """One sheared tile as the kernel runs it: stage the in-domain cells of the box, K steps (all
-    reads of a step before its writes; no cell written at a step is read by another lane at that
-    step), write back the cells the predicate selects."""
-    Ny, Nx = rhs.shape
-    inv = 1.0 / (2.0 * (idx2 + idy2))
-    TS, TK, TI = tile
-    s0, s1 = S * TS, min(S * TS + S, nsw)
-    ib, ub = W * TI - (s0 + S - 1), K * TK - 2 * (s0 + S - 1)
-    BX, BY = W + S + 1, K + 2 * S
-    box = np.full((BY, BX), np.nan)
-    for y in range(BY):
-        for x in range(BX):
-            i, u = ib - 1 + x, ub - 1 + y
-            j = u - i
-            if 0 <= i < Nx and 0 <= j < Ny:
-                box[y, x] = phi[j, i]
-    interior = ib >= 1 and ib + W + S - 2 <= Nx - 2 and \
-        K * TK - (W * TI + W - 1) - (s0 + S - 1) >= 1 and K * TK + K - 1 - W * TI - s0 <= Ny - 2
-    for d in range(K):
-        new, reads, writes = [], set(), set()
-        for s in range(s0, s1):
-            for lane in range(W):
-                i, j = W * TI + lane - s, K * TK + d - W * TI - lane - s
-                if not (0 <= i < Nx and 0 <= j < Ny):
-                    assert not interior
-                    continue
-                x, y = i - ib + 1, i + j - ub + 1
-                assert 1 <= x <= BX - 2 and 1 <= y <= BY - 2
-                e = (y + 1, x + 1) if i < Nx - 1 else (y, x)
-                w = (y - 1, x - 1) if i > 0 else (y, x)
-                n = (y + 1, x) if j < Ny - 1 else (y, x)
-                so = (y - 1, x) if j > 0 else (y, x)
-                if interior:
-                    assert i < Nx - 1 and i > 0 and j < Ny - 1 and j > 0
-                pc = box[y, x]
-                pnew = ((box[e] + box[w]) * idx2 + (box[n] + box[so]) * idy2 - rhs[j, i]) * inv
-                v = pc + omega * (pnew - pc)
-                assert np.isfinite(v)
-                new.append(((y, x), v))
-                reads |= {e, w, n, so} - {(y, x)}  # a clamped neighbour is the cell itself
-                writes.add((y, x))
-        assert len(writes) == len(new) and not (reads & writes), (tile, d)
-        for c, v in new:
-            box[c] = v
-    for y in range(BY):
-        for x in range(BX):
-            i, u = ib - 1 + x, ub - 1 + y
-            j = u - i
-            if 0 <= i < Nx and 0 <= j < Ny:
-                upd = sheared_updates(i, j, s0, s1, TI, TK, W, K)
-                if upd:
-                    phi[j, i] = box[y, x]
+        def step(T):
+            cur, p1, p2, e = T % 3, (T + 2) % 3, (T + 1) % 3, T % PD
+            r1, r2 = (T + 63) % RS, (T + 62) % RS
+            kx, ky, om = R(self.kx), R(self.ky), R(self.om)
+            for m in range(SW):
+                c, r = T - lane - m, r0 - m
+                if m == 0:
+                    if self.mixed:
+                        sb, eb, nb, wb = state["Eprev"], E[e].copy(), N[e].copy(), state["Eprev2"].copy()
+                        s_b = shr(state["Eprev2"], RB[r1])  # lane l-1's base of the previous step
+                        eb = np.where(c == self.Nx - 1, sb, eb)
+                        wb = np.where(c == 0, sb, wb)
+                        s_b = np.where(r == 0, sb, s_b)
+                        nb = np.where(r == self.Ny - 1, sb, nb)
+                        with np.errstate(invalid="ignore"):
+                            src = (((Cc[e] + self.kx * (eb + wb)) + self.ky * (nb + s_b)) - sb).astype(R)
+                        selfv = Ev = Nv = np.zeros(W, R)
+                    else:
+                        selfv, Ev, Nv, src = state["Eprev"].copy(), E[e].copy(), N[e].copy(), Cc[e].copy()
                 else:
-                    assert box[y, x] == phi[j, i] or np.isnan(box[y, x])
+                    selfv = shr(X[p2][m - 1], RX[r2][m - 1])
+                    Ev = shr(X[p1][m - 1], RX[r1][m - 1])
+                    Nv = X[p1][m - 1].copy()
+                    src = shr(Sr[p2][m - 1], RSr[r2][m - 1])
+                Sv = shr(X[p1][m], RX[r1][m])
+                Wv = X[p1][m].copy()
+                if m == 0:
+                    Ev = np.where(c == self.Nx - 1, selfv, Ev)
+                Sv = np.where(r == 0, selfv, Sv)
+                Nv = np.where(r == self.Ny - 1, selfv, Nv)
+                with np.errstate(invalid="ignore"):
+                    xn = selfv + om * (((src + kx * (Ev + Wv)) + ky * (Nv + Sv)) - selfv)
+                active = (c >= 0) & (c < self.Nx) & (r >= 0) & (r < self.Ny) & (m < nm)
+                X[cur][m] = np.where(active, xn, np.where(c < 0, Ev, Wv))
+                Sr[cur][m] = src
+                st = active & (m == nm - 1)
+                if st.any():
+                    val = Bw[e] + xn.astype(np.float64) if self.mixed else xn
+                    self.D.store(tile, ((c + r) * P + r)[st], val[st])
+            assert (T + 1) * RECN <= self.rec_stride, "record outside the tile's"
+            base = tile * self.rec_stride + T * RECN
+            v = np.concatenate([X[cur][:, 63], Sr[cur][:, 63]])
+            if self.mixed:
+                v = np.concatenate([v, np.array([state["Eprev"][63], 0.0]).view(np.float32)])
+            self.rec.store(tile, np.arange(base, base + RECN), v)
+            state["Eprev2"] = state["Eprev"]
+            state["Eprev"] = E[e].copy()
+
+        # ---- the kernel's sequence of waits and actions
+        yield [(pL, gbase + PD), (pD, gbase + PD)]
+        for T in range(PD):
+            prefetch(T)
+        state["Eprev"] = self.D.load(tile, (64 * b) * P + r0)
+        if not self.mixed:  # W of sweep 0 at column 0: the value held at step -1, the E input of that step
+            X[2][0] = state["Eprev"]
+        if pB >= 0:
+            yield [(pB, gbase + RP)]
+            for Tp in range(62, 64 + RP):
+                load_record(Tp, Tp % RS)
+        for T in range(Tend):
+            if pB >= 0:
+                yield [(pB, gbase + T + RP + 1)]
+                load_record(T + 64 + RP, (T + 64 + RP) % RS)
+            step(T)
+            yield [(pL, gbase + T + PD + 1), (pD, gbase + T + PD + 1)]
+            prefetch(T + PD)
+            if (T + 1) % PUB == 0:
+                self.publish(tile, gbase + T + 1)
+        self.publish(tile, 1 << 60)
+
+    def run(self, rng):
+        gens = {t: self.tile(t) for t in range(self.ntiles)}
+        waits = {t: next(g) for t, g in gens.items()}
+        while gens:
+            ready = [t for t, w in waits.items() if all(p < 0 or self.progress[p] >= need for p, need in w)]
+            assert ready, "deadlock"
+            t = ready[rng.integers(len(ready))]
+            try:
+                waits[t] = next(gens[t])
+            except StopIteration:
+                del gens[t], waits[t]
+        x = np.empty((self.Ny, self.Nx))
+        for r in range(self.Ny):
+            for c in range(self.Nx):
+                x[r, c] = self.D.vis[(c + r) * self.pitch + r]
+        return x
 
 
-SHEARED_CASES = [(9, 7, 5, 4, 3, 2), (11, 13, 9, 4, 4, 3), (6, 10, 7, 3, 5, 4), (13, 11, 12, 4, 6, 2),
-                 (20, 6, 30, 5, 2, 7), (70, 66, 18, 16, 8, 4)]
+CASES = [(20, 9, 7, 4), (70, 130, 11, 4), (33, 65, 17, 8), (130, 70, 9, 8), (5, 3, 5, 4), (1, 70, 6, 4),
+         (70, 1, 9, 8), (2, 2, 3, 8), (64, 64, 16, 4)]
 
 
-@pytest.mark.parametrize("Nx,Ny,nsw,W,K,S", SHEARED_CASES)
-def test_sheared_tiles_equal_lexicographic_bitwise(Nx, Ny, nsw, W, K, S):
-    rng = np.random.default_rng(Nx * 100 + Ny + 7)
-    rhs = rng.standard_normal((Ny, Nx))
-    idx2, idy2 = float(Nx * Nx), float(Ny * Ny)
-    ref = lexicographic(rhs, nsw, idx2, idy2, 1.8)
-    count = {}
-    for lev in sheared_levels(Nx, Ny, nsw, W, K, S):
-        for tile in lev:
-            for (_, s, i, j) in sheared_cells(tile, Nx, Ny, nsw, W, K, S):
-                count[(s, i, j)] = count.get((s, i, j), 0) + 1
-    assert len(count) == nsw * Nx * Ny and set(count.values()) == {1}
-    for rev in (False, True):
-        phi = np.zeros_like(rhs)
-        for lev in sheared_levels(Nx, Ny, nsw, W, K, S):
-            for tile in (reversed(lev) if rev else lev):
-                sheared_run_tile(phi, rhs, tile, nsw, idx2, idy2, 1.8, W, K, S)
-        assert np.array_equal(phi, ref)
+def _problem(Nx, Ny, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal((Ny, Nx)), 0.3, 0.2, 1.8
+
+
+@pytest.mark.parametrize("Nx,Ny,nsw,SW", CASES)
+def test_fp64_schedule_is_the_lexicographic_sweep(Nx, Ny, nsw, SW):
+    src, kx, ky, om = _problem(Nx, Ny, Nx * 7 + Ny)
+    want = lex_sor(src, nsw, kx, ky, om)
+    for seed in range(2):
+        got = Model(src, nsw, SW, kx, ky, om, mixed=False).run(np.random.default_rng(seed))
+        assert np.array_equal(got, want), (seed, np.nanmax(np.abs(got - want)))
+
+
+@pytest.mark.parametrize("Nx,Ny,nsw,SW", CASES)
+def test_mixed_schedule_is_block_refined_sor(Nx, Ny, nsw, SW):
+    src, kx, ky, om = _problem(Nx, Ny, Nx * 5 + Ny)
+    want = block_refined_sor(src, nsw, SW, kx, ky, om)
+    got = Model(src, nsw, SW, kx, ky, om, mixed=True).run(np.random.default_rng(3))
+    assert np.array_equal(got, want), np.nanmax(np.abs(got - want))
+    # and it is the FP64 sweep to FP32 rounding of the corrections
+    ref = lex_sor(src, nsw, kx, ky, om)
+    assert np.abs(got - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
+def test_the_model_catches_a_missing_wait():
+    """Dropping the wait on the (b+1, q-1) producer lets a tile read cells before they are published."""
+    src, kx, ky, om = _problem(70, 130, 1)
+    m = Model(src, 11, 4, kx, ky, om, mixed=False)
+    tile_gen = m.tile
+
+    def no_diag_wait(t):
+        for w in tile_gen(t):
+            yield [pw for pw in w if pw[0] != t - m.NB + 1 or t % m.NB == m.NB - 1]
+    m.tile = no_diag_wait
+    got = None
+    try:
+        got = m.run(np.random.default_rng(0))
+    except AssertionError:
+        return
+    assert not np.array_equal(got, lex_sor(src, 11, kx, ky, om))
