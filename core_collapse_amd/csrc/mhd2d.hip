@@ -84,7 +84,7 @@ __device__ __forceinline__ void face_hlld2d(const Geom2D& g, const Prim<double>&
     const IdealEOS2D<double> e64{g.gm1, g.gamma, g.floor_rho};
     if (MIXED) {
         const IdealEOS2D<float> e32{(float)g.gm1, (float)g.gamma, (float)g.floor_rho};
-        hlld_x_mixed(e32, e64, WL, WR, Bn, F, kDegenerateTau, g.hlld_guard);
+        hlld_x_mixed(e32, e64, WL, WR, Bn, F, default_degen_tau(g.hlld_guard), g.hlld_guard);
     } else {
         hlld_x<double>(e64, WL, WR, Bn, fabs(Bn) < 1e-12, F, nullptr, 0.0, g.hlld_guard);
     }

@@ -303,12 +303,13 @@ struct DegQueue {
 };
 
 // One face of axis AX per thread (x: M3:727-750, y: M3:752-775, z: M3:777-800): PLM-MC, HLLD, the
-// fluxes and EMFs.  On x and y faces the mixed form runs the FP32 HLLD only and queues the faces it
-// flags as near the Alfven degeneracy; k_face3d_fp64 then recomputes those in FP64 (the FP64 solver's
-// registers would otherwise halve this kernel's occupancy).  z faces, which the field along z of the
-// turbulence benchmark keeps near the degeneracy, recompute inline (a queue of most faces would cost
-// more than the occupancy gains).
-template <int AX, bool MIXED, typename EOS64, typename EOS32>
+// fluxes and EMFs.  The mixed form runs the FP32 HLLD only and queues the faces it flags (near the
+// Alfven degeneracy, or with the degeneracy guard on, near the guard's threshold); k_face3d_fp64 then
+// recomputes those in FP64 (the FP64 solver's registers would otherwise halve this kernel's occupancy).
+// Without the guard the field along z of the turbulence benchmark keeps most z faces near the
+// degeneracy: INLINE z faces then recompute inline (a queue of most faces would cost more than the
+// occupancy gains).
+template <int AX, bool MIXED, typename EOS64, typename EOS32, bool INLINE = false>
 __global__ __launch_bounds__(256) void k_face3d(Geom3D g, EOS64 e64, EOS32 e32,
                                                 PrimArrays<typename std::conditional<MIXED, float, double>::type> W,
                                                 const double* __restrict__ Bface,
@@ -337,11 +338,11 @@ __global__ __launch_bounds__(256) void k_face3d(Geom3D g, EOS64 e64, EOS32 e32,
     Prim<R> L, Rr;
     face_states3d<AX, R>(g, W, Bn, i, j, k, L, Rr);
     R o[7];
-    if constexpr (MIXED && AX != 2) {
+    if constexpr (MIXED && !INLINE) {
         bool deg = false;
         hlld_x<float>(e32, L, Rr, (float)Bn, fabs(Bn) < 1e-12, o, &deg, g.tau, (float)g.hlld_guard);
         if (deg) dq.face[AX][atomicAdd(&dq.count[AX], 1)] = (int)t;
-    } else if constexpr (MIXED) {  // z faces: often at the degeneracy in the benchmark field, recomputed inline
+    } else if constexpr (MIXED) {  // recomputed inline
         hlld_x_mixed32(e32, e64, L, Rr, Bn, o, g.tau, g.hlld_guard);
     } else {
         hlld_x<double>(e64, L, Rr, Bn, fabs(Bn) < 1e-12, o, nullptr, 0.0, g.hlld_guard);
@@ -813,9 +814,12 @@ struct cc_mhd3d {
         }
         {
             ProfScope ps(pr(), stream, "face");
-            hipLaunchKernelGGL((k_face3d<2, MIXED, EOS64, EOS32>), dim3((p.Nx + 63) / 64, p.Ny, (p.Nz + 1 + 3) / 4),
-                               dim3(256), 0, stream, g, e64, e32,
-                               W<R>(), in.Bz, FF<R>(2), EF<R>(), dq);
+            if (MIXED && p.hlld_guard <= 0.0)
+                hipLaunchKernelGGL((k_face3d<2, MIXED, EOS64, EOS32, true>), dim3((p.Nx + 63) / 64, p.Ny, (p.Nz + 1 + 3) / 4),
+                                   dim3(256), 0, stream, g, e64, e32, W<R>(), in.Bz, FF<R>(2), EF<R>(), dq);
+            else
+                hipLaunchKernelGGL((k_face3d<2, MIXED, EOS64, EOS32>), dim3((p.Nx + 63) / 64, p.Ny, (p.Nz + 1 + 3) / 4),
+                                   dim3(256), 0, stream, g, e64, e32, W<R>(), in.Bz, FF<R>(2), EF<R>(), dq);
             CC_CHECK_LAUNCH();
         }
         if constexpr (MIXED) {
@@ -923,7 +927,7 @@ cc_status cc_mhd3d_create(const cc_mhd3d_params* p, cc_mhd3d** out) {
     g.Nx = p->Nx; g.Ny = p->Ny; g.Nz = p->Nz; g.px = p->periodic_x != 0; g.py = p->periodic_y != 0; g.pz = p->periodic_z != 0;
     g.dx = (p->x1 - p->x0) / p->Nx; g.dy = (p->y1 - p->y0) / p->Ny; g.dz = (p->z1 - p->z0) / p->Nz;
     g.floor_rho = p->floor_rho; g.floor_P = p->floor_P; g.eta = p->eta; g.cfl = p->cfl;
-    g.tau = p->degen_tau > 0.0 ? (float)p->degen_tau : kDegenerateTau;
+    g.tau = p->degen_tau > 0.0 ? (float)p->degen_tau : default_degen_tau(p->hlld_guard);
     g.ct_comp = p->ct_compensated != 0;
     g.hlld_guard = p->hlld_guard;
     h->e.ideal = p->eos == 1;
@@ -1181,7 +1185,7 @@ cc_status cc_hlld3d_batch(const cc_mhd3d_params* p, int n, const double* WL, con
     hipMemcpy(dB, Bn, (size_t)n * 8, hipMemcpyHostToDevice);
     const dim3 grid((n + 255) / 256);
     const bool mixed = p->precision == CC_MIXED;
-    const float tau = p->degen_tau > 0.0 ? (float)p->degen_tau : kDegenerateTau;
+    const float tau = p->degen_tau > 0.0 ? (float)p->degen_tau : default_degen_tau(p->hlld_guard);
     if (p->eos == 1) {
         const IdealEOS3D<double> e64{p->gamma - 1.0, p->gamma, p->floor_rho};
         const IdealEOS3D<float> e32{(float)(p->gamma - 1.0), (float)p->gamma, (float)p->floor_rho};

@@ -151,10 +151,14 @@ __host__ __device__ __forceinline__ void flux_ideal_x(const Prim<R>& W, R E, R F
 // non-textbook v*, w* denominators, the single-star energy formula reused for the double-star
 // states, the double-star flux through S_L / S_R, and the HLL fallback when |Bn| < 1e-12.
 // bn_small carries that fallback test evaluated on the FP64 face field.
-// If `degenerate` is non-null, it is set when either side is within a relative margin `tau` of the
-// Alfven degeneracy dL = 0 / dR = 0 (M2:419-442), where the star states are ill-conditioned and a
-// low-precision evaluation cannot be trusted; the caller then recomputes the face in FP64 (the outputs
-// of a degenerate face are zeros: the solver stops there).
+// If `degenerate` is non-null, it is set for a face whose low-precision evaluation cannot be trusted;
+// the caller then recomputes the face in FP64 (the outputs of a degenerate face are zeros: the solver
+// stops there).  Without the guard that is a face within the relative margin `tau` of the Alfven
+// degeneracy dL = 0 / dR = 0 (M2:419-442), where the star states are ill-conditioned.  With the guard
+// (below), faces with |d| under guard * m take HLL, which does not divide by d, and the others have star
+// states whose condition number in d is at most 1 / guard: the one decision that rounding can flip is
+// the guard test itself, so only a face with |d| within the relative band `tau` of guard * m (either
+// side) is recomputed.
 // `guard` > 0 (no reference counterpart; 0 is the reference) sends a face whose d_L or d_R is below
 // guard * max(Bn^2, |rho s (S - S_M)|) to the reference's own HLL flux (the |Bn| < 1e-12 branch):
 // the reference divides the star states by d with only |d| > 1e-14 as a guard, so a face whose outer
@@ -211,8 +215,15 @@ __host__ __device__ __forceinline__ void hlld_x(const EOS& eos, Prim<R> WL, Prim
     const R dL = rhoL * sL * (SL - SM) - Bn2;
     const R dR = rhoR * sR * (SR - SM) - Bn2;
     if (degenerate) {
-        const R tL = fabs(rhoL * sL * (SL - SM)) + Bn2, tR = fabs(rhoR * sR * (SR - SM)) + Bn2;
-        *degenerate = fabs(dL) < tau * tL || fabs(dR) < tau * tR || fabs(denom) < tau * (fabs(rhoL * (SL - uL)) + fabs(rhoR * (SR - uR)));
+        if (guard > R(0)) {
+            const R mL = guard * smax(Bn2, R(fabs(rhoL * sL * (SL - SM)))), mR = guard * smax(Bn2, R(fabs(rhoR * sR * (SR - SM))));
+            *degenerate = fabs(fabs(dL) - mL) <= tau * mL || fabs(fabs(dR) - mR) <= tau * mR ||
+                          fabs(denom) < tau * guard * (fabs(rhoL * (SL - uL)) + fabs(rhoR * (SR - uR)));
+        } else {
+            const R tL = fabs(rhoL * sL * (SL - SM)) + Bn2, tR = fabs(rhoR * sR * (SR - SM)) + Bn2;
+            *degenerate = fabs(dL) < tau * tL || fabs(dR) < tau * tR ||
+                          fabs(denom) < tau * (fabs(rhoL * (SL - uL)) + fabs(rhoR * (SR - uR)));
+        }
         if (*degenerate) {  // the caller recomputes this face in FP64: the rest would be discarded
 #pragma unroll
             for (int k = 0; k < 7; k++) out[k] = R(0);
@@ -317,11 +328,18 @@ namespace cc {
 // Mixed-precision HLLD: FP32 evaluation from FP64 face states, with the face recomputed in FP64 when
 // it lies within the relative margin tau of the Alfven degeneracy (tau >= 1 recomputes every face).
 constexpr float kDegenerateTau = 2e-2f;
+// With the degeneracy guard on: the relative band around the guard threshold inside which the FP32
+// decision may differ from the FP64 one.  The FP32 error of d is a few tens of ulps of
+// max(Bn^2, |rho s (S - S_M)|), i.e. below 1e-5 of it, against a band half-width of 1e-2 x guard of it.
+constexpr float kGuardBand = 1e-2f;
+// The fallback margin of hlld_x's degenerate test for a guard value.
+__host__ __device__ inline float default_degen_tau(double guard) { return guard > 0.0 ? kGuardBand : kDegenerateTau; }
 
 template <typename EOS32, typename EOS64>
 __host__ __device__ __forceinline__ void hlld_x_mixed(const EOS32& e32, const EOS64& e64, const Prim<double>& WL,
                                                       const Prim<double>& WR, double Bn, double out[7],
-                                                      float tau = kDegenerateTau, double guard = 0.0) {
+                                                      float tau = -1.f, double guard = 0.0) {
+    if (tau < 0.f) tau = default_degen_tau(guard);
     const bool bn_small = fabs(Bn) < 1e-12;
     const Prim<float> L{(float)WL.rho, (float)WL.u, (float)WL.v, (float)WL.w, (float)WL.P, (float)WL.Bx, (float)WL.By, (float)WL.Bz};
     const Prim<float> R{(float)WR.rho, (float)WR.u, (float)WR.v, (float)WR.w, (float)WR.P, (float)WR.Bx, (float)WR.By, (float)WR.Bz};
