@@ -28,6 +28,8 @@
 
 namespace cc {
 
+constexpr int kSorTileW = 64;  // tile width along i of the LDS-tiled schedules (one wave64: lane = column)
+
 // Tile shape of the tiled schedule: S sweeps per tile (16 waves x S/16), WJ rows; FP64 keeps the
 // source term in LDS (rhs_lds) or loads it from global memory one step ahead.
 struct SorTile {

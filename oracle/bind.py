@@ -132,7 +132,8 @@ class Lib:
 
     def __init__(self, kind="oracle"):
         if kind == "oracle":
-            path = os.path.join(HERE, "liboracle.so")
+            # CC_ORACLE_LIB: another build of the same sources (the sanitizer build of the CPU suite)
+            path = os.environ.get("CC_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
         else:
             path = os.path.join(HERE, "_ref", "lib%s.so" % kind)
         if not os.path.exists(path):

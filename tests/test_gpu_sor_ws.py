@@ -99,3 +99,36 @@ def test_collapse_steps_with_wave_streamed_sor_match_oracle(prec, monkeypatch):
             assert rel_err(got[f], want[f]) < 1e-9, f
     else:
         assert rel_l1(got["rho"], want["rho"]) <= 1e-4
+
+
+def test_batch_with_wave_streamed_sor_equals_instances_run_alone(monkeypatch):
+    """cc_mhd2d_batch_run with wave-streamed solves: the instances' SOR launches stay separate (one branch
+    of the batch graph each, never merged), and three batch steps give bit for bit the times and states
+    of the same instances stepped alone."""
+    from core_collapse_amd.mhd2d import Mhd2D, Mhd2DBatch
+    monkeypatch.setenv("CC_SOR_WS", "1")
+    sample = ic.collapse2d_batch_sample(4, seed=5, nmin=40, nmax=140)
+
+    def make():
+        sims = []
+        for e in sample:
+            p = ic.collapse2d_params(e["N"], eta=e["eta"], precision=CC_MIXED, sor_iters=120)
+            h = Mhd2D(p)
+            h.upload(ic.collapse2d_ic(p, **e["ic"]))
+            sims.append(h)
+        return sims
+
+    alone = make()
+    t_alone = [h.run(3) for h in alone]
+    sims = make()
+    b = Mhd2DBatch(sims)
+    assert not b.info()["merged"]
+    t = b.run(3, None, np.full(len(sample), 1e300))
+    b.close()
+    assert list(t) == t_alone, (t, t_alone)
+    for h1, h2 in zip(alone, sims):
+        s1, s2 = h1.download(), h2.download()
+        for f in M.FIELDS:
+            assert np.array_equal(s1[f], s2[f]), f
+        h1.close()
+        h2.close()
