@@ -186,8 +186,11 @@ typedef struct cc_mhd3d_params {
     int eos;
     double gamma;
     int precision;    /* cc_precision */
-    double degen_tau; /* CC_MIXED only: faces whose HLLD lies within this relative margin of the Alfven
-                         degeneracy are recomputed in FP64; <= 0 selects the default 0.02 */
+    double degen_tau; /* CC_MIXED only: faces that the FP32 HLLD cannot be trusted on are recomputed in FP64:
+                         without the guard, faces within this relative margin of the Alfven degeneracy;
+                         with hlld_guard > 0, faces whose |d| lies within this relative band of the guard
+                         threshold (the one decision FP32 rounding can flip); <= 0 selects 0.02 without,
+                         0.01 with the guard */
     /* As in cc_mhd2d_params (no reference counterpart; 0 keeps the reference's arithmetic): the HLLD
      * degeneracy guard on rho (S - u)(S - S_M) - Bn^2 (M3:463-484, HLL branch M3:491) and the compensated
      * face update of CT (M3:941-1015) and of the Heun average (M3:1042-1065). */
