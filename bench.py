@@ -196,7 +196,7 @@ def bench_turb3d(N, warmup, steps, prof_steps):
     from core_collapse_amd import ic
     from core_collapse_amd import mhd3d as M3
     hs, el, prof = {}, {}, {}
-    names = ("prim", "face", "face_fp64", "update", "edge", "ct", "joule", "dt")
+    names = ("prim", "face", "face_fp64", "update", "ect", "edge", "ct", "joule", "dt")
     for prec in (CC_FP64, CC_MIXED):
         p = ic.turbulence3d_params(N, precision=prec)
         h = M3.Mhd3D(p)

@@ -15,6 +15,7 @@
 // ill-conditioned); face fluxes and EMFs stored in FP32; flux differences, the conserved update,
 // EMF averages and CT in FP64.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <type_traits>
 #include <vector>
@@ -414,55 +415,92 @@ __global__ __launch_bounds__(256) void k_update3d(Geom3D g, Eos3D e_in, State3D 
 }
 
 // ------------------------------------------------------------------ edge EMFs (M3:836-939)
+// Face EMFs at boundary-mapped indices (the reference's wrap of face and cell indices).
+template <typename R>
+struct EmfAt {
+    const Geom3D& g;
+    const EmfArr<R>& ef;
+    __device__ __forceinline__ double exy(int i, int j, int k) const { return (double)ef.Ex_y[fy_(g, bmap(i, g.Nx, g.px), fmap(j, g.Ny, g.py), bmap(k, g.Nz, g.pz))]; }
+    __device__ __forceinline__ double ezx(int i, int j, int k) const { return (double)ef.Ez_x[fx_(g, fmap(i, g.Nx, g.px), bmap(j, g.Ny, g.py), bmap(k, g.Nz, g.pz))]; }
+    __device__ __forceinline__ double eyx(int i, int j, int k) const { return (double)ef.Ey_x[fx_(g, fmap(i, g.Nx, g.px), bmap(j, g.Ny, g.py), bmap(k, g.Nz, g.pz))]; }
+    __device__ __forceinline__ double exz(int i, int j, int k) const { return (double)ef.Ex_z[fz_(g, bmap(i, g.Nx, g.px), bmap(j, g.Ny, g.py), fmap(k, g.Nz, g.pz))]; }
+    __device__ __forceinline__ double eyz(int i, int j, int k) const { return (double)ef.Ey_z[fz_(g, bmap(i, g.Nx, g.px), bmap(j, g.Ny, g.py), fmap(k, g.Nz, g.pz))]; }
+    __device__ __forceinline__ double ezy(int i, int j, int k) const { return (double)ef.Ez_y[fy_(g, bmap(i, g.Nx, g.px), fmap(j, g.Ny, g.py), bmap(k, g.Nz, g.pz))]; }
+};
+
+// The three edge EMFs: 1/4 of the four adjacent face EMFs + eta * curl of the cell-centred B0, with the
+// reference's index handling (M3:836-939).  The resistive parts take the cell-centred differences as
+// arguments so that callers holding the cell-centred field can share it.
+__device__ __forceinline__ double edge_sum(double ideal, double eta, double d1, double d2) {
+    double res = 0.0;
+    if (eta > 0.0) res = eta * (d1 - d2);
+    return ideal + res;
+}
+template <typename R>
+__device__ __forceinline__ double edge_ex(const Geom3D& g, const EmfArr<R>& ef, const State3D& B0, int i, int jf, int kf) {
+    const EmfAt<R> m{g, ef};
+    const double ex_ideal = 0.25 * (m.exy(i, jf, kf - 1) + m.exy(i, jf, kf) + m.exz(i, jf - 1, kf) + m.exz(i, jf, kf));
+    double dBzdy = 0.0, dBydz = 0.0;
+    if (g.eta > 0.0) {
+        dBzdy = (bzc(g, B0.Bz, i, jf, kf) - bzc(g, B0.Bz, i, jf - 1, kf)) / g.dy;
+        dBydz = (byc(g, B0.By, i, jf, kf) - byc(g, B0.By, i, jf, kf - 1)) / g.dz;
+    }
+    return edge_sum(ex_ideal, g.eta, dBzdy, dBydz);
+}
+template <typename R>
+__device__ __forceinline__ double edge_ey(const Geom3D& g, const EmfArr<R>& ef, const State3D& B0, int i, int j, int kf) {
+    const EmfAt<R> m{g, ef};
+    const double ey_ideal = 0.25 * (m.eyx(i, j, kf - 1) + m.eyx(i, j, kf) + m.eyz(i - 1, j, kf) + m.eyz(i, j, kf));
+    double dBxdz = 0.0, dBzdx = 0.0;
+    if (g.eta > 0.0) {
+        dBxdz = (bxc(g, B0.Bx, i, j, kf) - bxc(g, B0.Bx, i, j, kf - 1)) / g.dz;
+        dBzdx = (bzc(g, B0.Bz, i, j, kf) - bzc(g, B0.Bz, i - 1, j, kf)) / g.dx;
+    }
+    return edge_sum(ey_ideal, g.eta, dBxdz, dBzdx);
+}
+template <typename R>
+__device__ __forceinline__ double edge_ez(const Geom3D& g, const EmfArr<R>& ef, const State3D& B0, int i, int jf, int k) {
+    const EmfAt<R> m{g, ef};
+    const double ez_ideal = 0.25 * (m.ezx(i, jf - 1, k) + m.ezx(i, jf, k) + m.ezy(i - 1, jf, k) + m.ezy(i, jf, k));
+    double dBydx = 0.0, dBxdy = 0.0;
+    if (g.eta > 0.0) {
+        dBydx = (byc(g, B0.By, i, jf, k) - byc(g, B0.By, i - 1, jf, k)) / g.dx;
+        dBxdy = (bxc(g, B0.Bx, i, jf, k) - bxc(g, B0.Bx, i, jf - 1, k)) / g.dy;
+    }
+    return edge_sum(ez_ideal, g.eta, dBydx, dBxdy);
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_edge3d(Geom3D g, EmfArr<R> ef, State3D B0, Edges ed) {
     const int ia = blockIdx.x * 64 + (threadIdx.x & 63), ja = blockIdx.y, ka = blockIdx.z * 4 + (threadIdx.x >> 6);
     const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
     if (ia > Nx || ka > Nz) return;
-    const bool px = g.px, py = g.py, pz = g.pz;
-    auto exy = [&](int i, int j, int k) { return (double)ef.Ex_y[fy_(g, bmap(i, Nx, px), fmap(j, Ny, py), bmap(k, Nz, pz))]; };
-    auto ezx = [&](int i, int j, int k) { return (double)ef.Ez_x[fx_(g, fmap(i, Nx, px), bmap(j, Ny, py), bmap(k, Nz, pz))]; };
-    auto eyx = [&](int i, int j, int k) { return (double)ef.Ey_x[fx_(g, fmap(i, Nx, px), bmap(j, Ny, py), bmap(k, Nz, pz))]; };
-    auto exz = [&](int i, int j, int k) { return (double)ef.Ex_z[fz_(g, bmap(i, Nx, px), bmap(j, Ny, py), fmap(k, Nz, pz))]; };
-    auto eyz = [&](int i, int j, int k) { return (double)ef.Ey_z[fz_(g, bmap(i, Nx, px), bmap(j, Ny, py), fmap(k, Nz, pz))]; };
-    auto ezy = [&](int i, int j, int k) { return (double)ef.Ez_y[fy_(g, bmap(i, Nx, px), fmap(j, Ny, py), bmap(k, Nz, pz))]; };
-    const double eta = g.eta;
-    if (ia < Nx) {  // Ex(i, jf, kf)
-        const int i = ia, jf = ja, kf = ka;
-        const double ex_ideal = 0.25 * (exy(i, jf, kf - 1) + exy(i, jf, kf) + exz(i, jf - 1, kf) + exz(i, jf, kf));
-        double ex_res = 0.0;
-        if (eta > 0.0) {
-            const double dBzdy = (bzc(g, B0.Bz, i, jf, kf) - bzc(g, B0.Bz, i, jf - 1, kf)) / g.dy;
-            const double dBydz = (byc(g, B0.By, i, jf, kf) - byc(g, B0.By, i, jf, kf - 1)) / g.dz;
-            ex_res = eta * (dBzdy - dBydz);
-        }
-        ed.Ex[((size_t)kf * (Ny + 1) + jf) * Nx + i] = ex_ideal + ex_res;
-    }
-    if (ja < Ny) {  // Ey(if, j, kf)
-        const int i = ia, j = ja, kf = ka;
-        const double ey_ideal = 0.25 * (eyx(i, j, kf - 1) + eyx(i, j, kf) + eyz(i - 1, j, kf) + eyz(i, j, kf));
-        double ey_res = 0.0;
-        if (eta > 0.0) {
-            const double dBxdz = (bxc(g, B0.Bx, i, j, kf) - bxc(g, B0.Bx, i, j, kf - 1)) / g.dz;
-            const double dBzdx = (bzc(g, B0.Bz, i, j, kf) - bzc(g, B0.Bz, i - 1, j, kf)) / g.dx;
-            ey_res = eta * (dBxdz - dBzdx);
-        }
-        ed.Ey[((size_t)kf * Ny + j) * (Nx + 1) + i] = ey_ideal + ey_res;
-    }
-    if (ka < Nz) {  // Ez(if, jf, k)
-        const int i = ia, jf = ja, k = ka;
-        const double ez_ideal = 0.25 * (ezx(i, jf - 1, k) + ezx(i, jf, k) + ezy(i - 1, jf, k) + ezy(i, jf, k));
-        double ez_res = 0.0;
-        if (eta > 0.0) {
-            const double dBydx = (byc(g, B0.By, i, jf, k) - byc(g, B0.By, i - 1, jf, k)) / g.dx;
-            const double dBxdy = (bxc(g, B0.Bx, i, jf, k) - bxc(g, B0.Bx, i, jf - 1, k)) / g.dy;
-            ez_res = eta * (dBydx - dBxdy);
-        }
-        ed.Ez[((size_t)k * (Ny + 1) + jf) * (Nx + 1) + i] = ez_ideal + ez_res;
-    }
+    if (ia < Nx) ed.Ex[((size_t)ka * (Ny + 1) + ja) * Nx + ia] = edge_ex(g, ef, B0, ia, ja, ka);          // Ex(i, jf, kf)
+    if (ja < Ny) ed.Ey[((size_t)ka * Ny + ja) * (Nx + 1) + ia] = edge_ey(g, ef, B0, ia, ja, ka);          // Ey(if, j, kf)
+    if (ka < Nz) ed.Ez[((size_t)ka * (Ny + 1) + ja) * (Nx + 1) + ia] = edge_ez(g, ef, B0, ia, ja, ka);    // Ez(if, jf, k)
 }
 
 // ------------------------------------------------------------------ CT (M3:941-1015)
+// Face-field increments from the edge EMFs around the face.
+__device__ __forceinline__ double ct_dbx(const Geom3D& g, double dt, double ez_j, double ez_j1, double ey_k, double ey_k1) {
+    return -(dt / g.dy) * (ez_j1 - ez_j) + (dt / g.dz) * (ey_k1 - ey_k);
+}
+__device__ __forceinline__ double ct_dby(const Geom3D& g, double dt, double ez_i, double ez_i1, double ex_k, double ex_k1) {
+    return +(dt / g.dx) * (ez_i1 - ez_i) - (dt / g.dz) * (ex_k1 - ex_k);
+}
+__device__ __forceinline__ double ct_dbz(const Geom3D& g, double dt, double ey_i, double ey_i1, double ex_j, double ex_j1) {
+    return -(dt / g.dx) * (ey_i1 - ey_i) + (dt / g.dy) * (ex_j1 - ex_j);
+}
+// B1 = B0 + increment: the reference's sum, or (compensated) the exact sum carried as hi + FP32 lo.
+__device__ __forceinline__ void ct_new(bool comp, double b0, float b0lo, double d, double& hi, float& lo) {
+    if (comp) {
+        ct_add(b0, b0lo, d, hi, lo);
+    } else {
+        hi = b0 + d;
+        lo = 0.f;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_ct3d(Geom3D g, Edges ed, State3D B0, State3D B1, const Scalars* __restrict__ sc) {
     const int ia = blockIdx.x * 64 + (threadIdx.x & 63), ja = blockIdx.y, ka = blockIdx.z * 4 + (threadIdx.x >> 6);
     const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
@@ -472,117 +510,337 @@ __global__ __launch_bounds__(256) void k_ct3d(Geom3D g, Edges ed, State3D B0, St
     auto EY = [&](int i, int j, int kf) { return ed.Ey[((size_t)kf * Ny + j) * (Nx + 1) + i]; };
     auto EZ = [&](int i, int jf, int k) { return ed.Ez[((size_t)k * (Ny + 1) + jf) * (Nx + 1) + i]; };
     // face indices j+1 <= Ny and k+1 <= Nz need no wrapping (imod(n, n+1) = n, clamp(n, 0, n) = n)
-    // B1 = B0 + increment: the reference's sum, or (compensated) the exact sum carried as hi + FP32 lo
     auto put = [&](double* B1v, float* B1lo, const double* B0v, const float* B0lo, size_t f, double d) {
-        if (g.ct_comp) {
-            double hi;
-            float lo;
-            ct_add(B0v[f], B0lo[f], d, hi, lo);
-            B1v[f] = hi;
-            B1lo[f] = lo;
-        } else {
-            B1v[f] = B0v[f] + d;
-        }
+        double hi;
+        float lo;
+        ct_new(g.ct_comp, B0v[f], g.ct_comp ? B0lo[f] : 0.f, d, hi, lo);
+        B1v[f] = hi;
+        if (g.ct_comp) B1lo[f] = lo;
     };
     if (ja < Ny && ka < Nz) {
         const int i = ia, j = ja, k = ka;
-        put(B1.Bx, B1.Bx_lo, B0.Bx, B0.Bx_lo, fx_(g, i, j, k),
-            -(dt / g.dy) * (EZ(i, j + 1, k) - EZ(i, j, k)) + (dt / g.dz) * (EY(i, j, k + 1) - EY(i, j, k)));
+        put(B1.Bx, B1.Bx_lo, B0.Bx, B0.Bx_lo, fx_(g, i, j, k), ct_dbx(g, dt, EZ(i, j, k), EZ(i, j + 1, k), EY(i, j, k), EY(i, j, k + 1)));
     }
     if (ia < Nx && ka < Nz) {
         const int i = ia, j = ja, k = ka;
-        put(B1.By, B1.By_lo, B0.By, B0.By_lo, fy_(g, i, j, k),
-            +(dt / g.dx) * (EZ(i + 1, j, k) - EZ(i, j, k)) - (dt / g.dz) * (EX(i, j, k + 1) - EX(i, j, k)));
+        put(B1.By, B1.By_lo, B0.By, B0.By_lo, fy_(g, i, j, k), ct_dby(g, dt, EZ(i, j, k), EZ(i + 1, j, k), EX(i, j, k), EX(i, j, k + 1)));
     }
     if (ia < Nx && ja < Ny) {
         const int i = ia, j = ja, k = ka;
-        put(B1.Bz, B1.Bz_lo, B0.Bz, B0.Bz_lo, fz_(g, i, j, k),
-            -(dt / g.dx) * (EY(i + 1, j, k) - EY(i, j, k)) + (dt / g.dy) * (EX(i, j + 1, k) - EX(i, j, k)));
+        put(B1.Bz, B1.Bz_lo, B0.Bz, B0.Bz_lo, fz_(g, i, j, k), ct_dbz(g, dt, EY(i, j, k), EY(i + 1, j, k), EX(i, j, k), EX(i, j + 1, k)));
+    }
+}
+
+// ------------------------------------------------------------------ edge EMFs + CT in one pass
+// The edge EMFs and CT (M3:836-1015) without the edge arrays.  A block owns a TX x TY column of cells and
+// marches along z over a chunk of planes; position (xs, ys) in [0, TX+2) x [0, TY+2) holds (a, b) =
+// (i0-1+xs, j0-1+ys).  Per plane s of the march:
+//   (1) every position evaluates the cell-centred B0 (bxc, byc, bzc) at raw (a, b, s+1) with the
+//       reference's index maps and shares it through LDS;
+//   (2) positions xs, ys >= 1 evaluate Ex, Ey at z-face s+1 and Ez at plane s with the reference's
+//       expressions: the -x / -y cell-centred values come from LDS, those of plane s from registers;
+//   (3) positions owning faces (the tile's, and those of index Nx / Ny / Nz on the grid's high sides)
+//       form Bx, By of plane s and Bz of z-face s+1 from their own edges and those of the +x / +y
+//       neighbour (LDS), and write them.
+// Every edge is evaluated once per tile plus a one-position halo, and neither the edges nor the face
+// EMFs of a plane are read from memory twice.
+template <int TX, int TY>
+struct EctShape {
+    static constexpr int PX = TX + 2, PY = TY + 2, NP = PX * PY, NT = NP <= 512 ? 512 : 1024;
+    static_assert(NP <= 1024, "tile positions exceed the block");
+};
+
+template <typename R, int TX, int TY>
+__global__ __launch_bounds__((EctShape<TX, TY>::NT)) void k_ect3d(Geom3D g, EmfArr<R> ef, State3D B0, State3D B1,
+                                                                const Scalars* __restrict__ sc, int zchunk) {
+    constexpr int PX = EctShape<TX, TY>::PX, NP = EctShape<TX, TY>::NP;
+    __shared__ double s_cbx[NP], s_cby[NP], s_cbz[NP];  // bxc, byc of plane s; bzc of plane s+1
+    __shared__ double s_ez[NP], s_ey[NP], s_ex[NP];     // Ez of plane s; Ey, Ex of z-face s+1
+    const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+    const bool comp = g.ct_comp != 0;
+    const int ntx = (Nx + TX - 1) / TX;
+    const int i0 = (int)(blockIdx.x % (unsigned)ntx) * TX, j0 = (int)(blockIdx.x / (unsigned)ntx) * TY;
+    const int c0 = (int)blockIdx.y * zchunk, c1 = min(Nz, c0 + zchunk);
+    const bool act = threadIdx.x < (unsigned)NP;
+    const int p = act ? (int)threadIdx.x : 0;
+    const int xs = p % PX, ys = p / PX;
+    const int a = i0 - 1 + xs, b = j0 - 1 + ys;
+    const bool has_edge = act && xs >= 1 && ys >= 1;
+    const bool own_bx = act && ys >= 1 && ys <= TY && b < Ny && xs >= 1 && a <= Nx && (xs <= TX || a == Nx);
+    const bool own_by = act && xs >= 1 && xs <= TX && a < Nx && ys >= 1 && b <= Ny && (ys <= TY || b == Ny);
+    const bool own_bz = act && xs >= 1 && xs <= TX && a < Nx && ys >= 1 && ys <= TY && b < Ny;
+    const double dt = sc->dt, eta = g.eta;
+    // Boundary-mapped indices split into an x/y part per position (fixed along the march) and a z part per
+    // plane (uniform over the block): fx_ = (k Ny + j)(Nx+1) + i with i face-mapped and j, k cell-mapped,
+    // fy_ = (k (Ny+1) + j) Nx + i with j face-mapped, fz_ = (k Ny + j) Nx + i with k face-mapped.
+    const int NX1 = Nx + 1, NY1 = Ny + 1;
+    const int fa = fmap(a, Nx, g.px), fa1 = fmap(a + 1, Nx, g.px), ca = bmap(a, Nx, g.px), cam = bmap(a - 1, Nx, g.px);
+    const int fb = fmap(b, Ny, g.py), fb1 = fmap(b + 1, Ny, g.py), cb = bmap(b, Ny, g.py), cbm = bmap(b - 1, Ny, g.py);
+    const int ox_ab = cb * NX1 + fa, ox_a1b = cb * NX1 + fa1, ox_abm = cbm * NX1 + fa;
+    const int oy_ab = fb * Nx + ca, oy_ab1 = fb1 * Nx + ca, oy_amb = fb * Nx + cam;
+    const int oz_ab = cb * Nx + ca, oz_abm = cbm * Nx + ca, oz_amb = cb * Nx + cam;
+    auto zx = [&](int k) { return (size_t)bmap(k, Nz, g.pz) * ((size_t)Ny * NX1); };
+    auto zy = [&](int k) { return (size_t)bmap(k, Nz, g.pz) * ((size_t)NY1 * Nx); };
+    auto zz = [&](int kf) { return (size_t)fmap(kf, Nz, g.pz) * ((size_t)Ny * Nx); };
+
+    // Global values of one march step s: B0 faces of plane s+1 (the cell-centred field), the face EMFs
+    // Ex_y, Ey_x of plane s+1, Ex_z, Ey_z of z-face s+1, Ez_x, Ez_y of plane s, and the low words of the
+    // faces this position updates.  Loaded one step ahead.
+    struct StepIn {
+        double bxa, bxb, bya, byb, bza, bzb, exy, eyx, exzm, exz, eyzm, eyz, ezxm, ezx, ezym, ezy;
+        float lbx, lby, lbz;
+    };
+    auto load_step = [&](int s, StepIn& L) {
+        const int pn = s + 1;
+        const size_t xp = zx(pn), yp = zy(pn), zp = zz(pn), zp1 = zz(pn + 1), xs0 = zx(s), ys0 = zy(s);
+        L.bxa = B0.Bx[xp + ox_ab]; L.bxb = B0.Bx[xp + ox_a1b];
+        L.bya = B0.By[yp + oy_ab]; L.byb = B0.By[yp + oy_ab1];
+        L.bza = B0.Bz[zp + oz_ab]; L.bzb = B0.Bz[zp1 + oz_ab];
+        L.exy = (double)ef.Ex_y[yp + oy_ab]; L.eyx = (double)ef.Ey_x[xp + ox_ab];
+        L.exzm = (double)ef.Ex_z[zp + oz_abm]; L.exz = (double)ef.Ex_z[zp + oz_ab];
+        L.eyzm = (double)ef.Ey_z[zp + oz_amb]; L.eyz = (double)ef.Ey_z[zp + oz_ab];
+        L.ezxm = (double)ef.Ez_x[xs0 + ox_abm]; L.ezx = (double)ef.Ez_x[xs0 + ox_ab];
+        L.ezym = (double)ef.Ez_y[ys0 + oy_amb]; L.ezy = (double)ef.Ez_y[ys0 + oy_ab];
+        L.lbx = L.lby = L.lbz = 0.f;
+        if (comp) {  // own faces are in range: the maps are the identity there
+            if (own_bx && s >= c0) L.lbx = B0.Bx_lo[xs0 + ox_ab];
+            if (own_by && s >= c0) L.lby = B0.By_lo[ys0 + oy_ab];
+            if (own_bz) L.lbz = B0.Bz_lo[zp + oz_ab];
+        }
+    };
+
+    // carried along z: cell-centred B0 and the face values of plane s, the face EMFs Ex_y, Ey_x of plane
+    // s, the edges Ex, Ey of z-face s
+    double cbx_o, cby_o, bxa_o, bya_o, exy_o, eyx_o;
+    {
+        const int q = c0 - 1;
+        const size_t xq = zx(q), yq = zy(q);
+        bxa_o = B0.Bx[xq + ox_ab];
+        bya_o = B0.By[yq + oy_ab];
+        cbx_o = 0.5 * (bxa_o + B0.Bx[xq + ox_a1b]);
+        cby_o = 0.5 * (bya_o + B0.By[yq + oy_ab1]);
+        exy_o = (double)ef.Ex_y[yq + oy_ab];
+        eyx_o = (double)ef.Ey_x[xq + ox_ab];
+    }
+    double ex_c = 0.0, ey_c = 0.0;
+    StepIn cur;
+    load_step(c0 - 1, cur);
+
+    for (int s = c0 - 1; s < c1; ++s) {
+        const int pn = s + 1;  // z-face and cell plane s+1 (<= c1 <= Nz)
+        // (1) cell-centred B0 of plane s+1
+        const double cbx_n = 0.5 * (cur.bxa + cur.bxb), cby_n = 0.5 * (cur.bya + cur.byb),
+                     cbz_n = 0.5 * (cur.bza + cur.bzb);
+        if (act) {
+            s_cbx[p] = cbx_o;
+            s_cby[p] = cby_o;
+            s_cbz[p] = cbz_n;
+        }
+        __syncthreads();
+        StepIn nxt;
+        if (s + 1 < c1) load_step(s + 1, nxt);
+        // (2) edges: Ex, Ey of z-face s+1, Ez of plane s
+        double ex_n = 0.0, ey_n = 0.0, ez = 0.0;
+        if (has_edge) {
+            const double exi = 0.25 * (exy_o + cur.exy + cur.exzm + cur.exz);
+            const double eyi = 0.25 * (eyx_o + cur.eyx + cur.eyzm + cur.eyz);
+            double dBzdy = 0.0, dBydz = 0.0, dBxdz = 0.0, dBzdx = 0.0;
+            if (eta > 0.0) {
+                dBzdy = (cbz_n - s_cbz[p - PX]) / g.dy;
+                dBydz = (cby_n - cby_o) / g.dz;
+                dBxdz = (cbx_n - cbx_o) / g.dz;
+                dBzdx = (cbz_n - s_cbz[p - 1]) / g.dx;
+            }
+            ex_n = edge_sum(exi, eta, dBzdy, dBydz);
+            ey_n = edge_sum(eyi, eta, dBxdz, dBzdx);
+            if (s >= c0) {
+                const double ezi = 0.25 * (cur.ezxm + cur.ezx + cur.ezym + cur.ezy);
+                double dBydx = 0.0, dBxdy = 0.0;
+                if (eta > 0.0) {
+                    dBydx = (cby_o - s_cby[p - 1]) / g.dx;
+                    dBxdy = (cbx_o - s_cbx[p - PX]) / g.dy;
+                }
+                ez = edge_sum(ezi, eta, dBydx, dBxdy);
+            }
+        }
+        if (act) {
+            s_ez[p] = ez;
+            s_ey[p] = ey_n;
+            s_ex[p] = ex_n;
+        }
+        __syncthreads();
+        // (3) the face field: Bx, By of plane s, Bz of z-face s+1 (z-face c1 belongs to the next chunk)
+        if (s >= c0) {
+            if (own_bx) {
+                const size_t f = zx(s) + ox_ab;
+                double hi;
+                float lo;
+                ct_new(comp, bxa_o, cur.lbx, ct_dbx(g, dt, ez, s_ez[p + PX], ey_c, ey_n), hi, lo);
+                B1.Bx[f] = hi;
+                if (comp) B1.Bx_lo[f] = lo;
+            }
+            if (own_by) {
+                const size_t f = zy(s) + oy_ab;
+                double hi;
+                float lo;
+                ct_new(comp, bya_o, cur.lby, ct_dby(g, dt, ez, s_ez[p + 1], ex_c, ex_n), hi, lo);
+                B1.By[f] = hi;
+                if (comp) B1.By_lo[f] = lo;
+            }
+        }
+        if (own_bz && (pn < c1 || pn == Nz)) {
+            const size_t f = zz(pn) + oz_ab;
+            double hi;
+            float lo;
+            ct_new(comp, cur.bza, cur.lbz, ct_dbz(g, dt, ey_n, s_ey[p + 1], ex_n, s_ex[p + PX]), hi, lo);
+            B1.Bz[f] = hi;
+            if (comp) B1.Bz_lo[f] = lo;
+        }
+        cbx_o = cbx_n; cby_o = cby_n; bxa_o = cur.bxa; bya_o = cur.bya;
+        exy_o = cur.exy; eyx_o = cur.eyx; ex_c = ex_n; ey_c = ey_n;
+        cur = nxt;
     }
 }
 
 // ------------------------------------------------------------------ Joule + floors (+ Heun)
-template <bool FINAL, bool IDEAL>
-__global__ __launch_bounds__(256) void k_joule3d(Geom3D g, Eos3D e_in, State3D Us, State3D B1,
+// The cell kernels that end a stage or reduce over cells march along z: one thread per (i, j) column of a
+// chunk of zchunk planes, 64 i x 4 j per block.  Their per-block maxima therefore number
+// col_blocks = ceil(Nx/64) ceil(Ny/4) ceil(Nz/zchunk) for every writer (speed, Joule, divB).
+struct ColGrid {
+    int i, j, c0, c1;
+    __device__ ColGrid(const Geom3D& g, int zchunk)
+        : i((int)blockIdx.x * 64 + (int)(threadIdx.x & 63)), j((int)blockIdx.y * 4 + (int)(threadIdx.x >> 6)),
+          c0((int)blockIdx.z * zchunk), c1(min(g.Nz, (int)blockIdx.z * zchunk + zchunk)) {}
+    __device__ bool valid(const Geom3D& g) const { return i < g.Nx && j < g.Ny; }
+};
+
+// Joule heating from the central differences of the cell-centred B1 (add_joule_heating, M3:652-692),
+// floors (M3:1019-1029) and in stage 2 the Heun average with its floors (M3:1042-1065) and the next
+// step's signal-speed maximum.  Along the column the cell-centred B1 of planes k-1, k, k+1 and the z faces
+// k, k+1 of the column and of its four x/y neighbours stay in registers; the boundary-mapped face indices
+// are split into a per-thread x/y part and a per-plane z part, so a plane costs 17 face loads and no index
+// maps.  Same expressions as the reference, in the same order.
+// The primitives of the result (cons_to_prim on its face-averaged B, M3:298-308) are stored in the flux
+// precision for the next stage's face kernels, which k_prim3d would otherwise recompute from the result.
+// The stage-2 instance holds more live values (Heun average, compensated faces): it runs at 2 waves per SIMD
+// without spilling rather than at 3 with.
+template <typename R, bool FINAL, bool IDEAL>
+__global__ __launch_bounds__(256, FINAL ? 2 : 4) void k_joule3d(Geom3D g, Eos3D e_in, State3D Us, State3D B1,
                                                  const Scalars* __restrict__ sc, State3D Un, State3D Out,
-                                                 double* __restrict__ spd) {
+                                                 double* __restrict__ spd, int zchunk, PrimArrays<R> W) {
     const Eos3D e = eos_of<IDEAL>(e_in);
-    // 64 consecutive i x 4 consecutive k of one j row per block (the z neighbours then share L1)
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y, k = blockIdx.z * 4 + (threadIdx.x >> 6);
-    const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+    const ColGrid cg(g, zchunk);
+    const int i = cg.i, j = cg.j, Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+    const bool comp = g.ct_comp != 0;
     double speed = 0.0;
-    if (i < Nx && k < Nz) {
-        const size_t q = ((size_t)k * Ny + j) * Nx + i;
-        double rho = Us.rho[q], mx = Us.mx[q], my = Us.my[q], mz = Us.mz[q], E = Us.E[q];
+    if (cg.valid(g)) {
         const double dt = sc->dt;
-        if (g.eta > 0.0) {  // add_joule_heating, M3:652-692
-            const double dBzdy = (bzc(g, B1.Bz, i, j + 1, k) - bzc(g, B1.Bz, i, j - 1, k)) / (2 * g.dy);
-            const double dBydz = (byc(g, B1.By, i, j, k + 1) - byc(g, B1.By, i, j, k - 1)) / (2 * g.dz);
-            const double dBxdz = (bxc(g, B1.Bx, i, j, k + 1) - bxc(g, B1.Bx, i, j, k - 1)) / (2 * g.dz);
-            const double dBzdx = (bzc(g, B1.Bz, i + 1, j, k) - bzc(g, B1.Bz, i - 1, j, k)) / (2 * g.dx);
-            const double dBydx = (byc(g, B1.By, i + 1, j, k) - byc(g, B1.By, i - 1, j, k)) / (2 * g.dx);
-            const double dBxdy = (bxc(g, B1.Bx, i, j + 1, k) - bxc(g, B1.Bx, i, j - 1, k)) / (2 * g.dy);
-            const double Jx = dBzdy - dBydz, Jy = dBxdz - dBzdx, Jz = dBydx - dBxdy;
-            const double J2 = Jx * Jx + Jy * Jy + Jz * Jz;
-            E += dt * g.eta * J2;
-        }
-        const double bxL = B1.Bx[fx_(g, i, j, k)], bxR = B1.Bx[fx_(g, i + 1, j, k)];
-        const double byD = B1.By[fy_(g, i, j, k)], byU = B1.By[fy_(g, i, j + 1, k)];
-        const double bzB = B1.Bz[fz_(g, i, j, k)], bzF = B1.Bz[fz_(g, i, j, k + 1)];
-        floors3d(e, rho, mx, my, mz, E, 0.5 * (bxL + bxR), 0.5 * (byD + byU), 0.5 * (bzB + bzF));  // M3:1019-1029
-        if (!FINAL) {
-            Out.rho[q] = rho; Out.mx[q] = mx; Out.my[q] = my; Out.mz[q] = mz; Out.E[q] = E;
-        } else {  // Heun average and floors, M3:1042-1065
-            rho = 0.5 * (Un.rho[q] + rho);
-            mx = 0.5 * (Un.mx[q] + mx);
-            my = 0.5 * (Un.my[q] + my);
-            mz = 0.5 * (Un.mz[q] + mz);
-            E = 0.5 * (Un.E[q] + E);
-            double nxL, nxR, nyD, nyU, nzB, nzF;
-            float lxL = 0.f, lxR = 0.f, lyD = 0.f, lyU = 0.f, lzB = 0.f, lzF = 0.f;
-            const size_t fL = fx_(g, i, j, k), fR = fx_(g, i + 1, j, k), fD = fy_(g, i, j, k), fU = fy_(g, i, j + 1, k);
-            const size_t fB = fz_(g, i, j, k), fF = fz_(g, i, j, k + 1);
-            if (g.ct_comp) {  // 1/2 (B^n + B^2) of the compensated fields
-                ct_avg(Un.Bx[fL], Un.Bx_lo[fL], bxL, B1.Bx_lo[fL], nxL, lxL);
-                ct_avg(Un.Bx[fR], Un.Bx_lo[fR], bxR, B1.Bx_lo[fR], nxR, lxR);
-                ct_avg(Un.By[fD], Un.By_lo[fD], byD, B1.By_lo[fD], nyD, lyD);
-                ct_avg(Un.By[fU], Un.By_lo[fU], byU, B1.By_lo[fU], nyU, lyU);
-                ct_avg(Un.Bz[fB], Un.Bz_lo[fB], bzB, B1.Bz_lo[fB], nzB, lzB);
-                ct_avg(Un.Bz[fF], Un.Bz_lo[fF], bzF, B1.Bz_lo[fF], nzF, lzF);
-            } else {
-                nxL = 0.5 * (Un.Bx[fL] + bxL); nxR = 0.5 * (Un.Bx[fR] + bxR);
-                nyD = 0.5 * (Un.By[fD] + byD); nyU = 0.5 * (Un.By[fU] + byU);
-                nzB = 0.5 * (Un.Bz[fB] + bzB); nzF = 0.5 * (Un.Bz[fF] + bzF);
+        const int NX1 = Nx + 1, NY1 = Ny + 1;
+        const int im = bmap(i - 1, Nx, g.px), ip = bmap(i + 1, Nx, g.px), jm = bmap(j - 1, Ny, g.py), jp = bmap(j + 1, Ny, g.py);
+        // x faces (fx_, j cell-mapped): rows j, j-1, j+1 at faces i and i+1; y faces (fy_, i cell-mapped):
+        // columns i, i-1, i+1 at faces j and j+1; z faces (fz_): (i, j) and its four x/y neighbours
+        const int ox0 = j * NX1 + i, oxm = jm * NX1 + i, oxp = jp * NX1 + i;
+        const int oy0 = j * Nx + i, oym = j * Nx + im, oyp = j * Nx + ip;
+        const int oz0 = j * Nx + i, ozjm = jm * Nx + i, ozjp = jp * Nx + i, ozim = j * Nx + im, ozip = j * Nx + ip;
+        const size_t sx = (size_t)Ny * NX1, sy = (size_t)NY1 * Nx, sz = (size_t)Ny * Nx;
+        auto bxc_at = [&](int k) {
+            const size_t o = (size_t)bmap(k, Nz, g.pz) * sx + ox0;
+            return 0.5 * (B1.Bx[o] + B1.Bx[o + 1]);
+        };
+        auto byc_at = [&](int k) {
+            const size_t o = (size_t)bmap(k, Nz, g.pz) * sy + oy0;
+            return 0.5 * (B1.By[o] + B1.By[o + Nx]);
+        };
+        const int c0 = cg.c0, c1 = cg.c1;
+        double bxc_m = bxc_at(c0 - 1), byc_m = byc_at(c0 - 1), bxc_0 = bxc_at(c0), byc_0 = byc_at(c0);
+        size_t zf = (size_t)c0 * sz;  // z face k (face indices k, k+1 <= Nz need no map)
+        double bz0 = B1.Bz[zf + oz0], bzjm = B1.Bz[zf + ozjm], bzjp = B1.Bz[zf + ozjp], bzim = B1.Bz[zf + ozim],
+               bzip = B1.Bz[zf + ozip];
+        for (int k = c0; k < c1; ++k) {
+            const size_t zf1 = zf + sz;
+            const double bxc_p = bxc_at(k + 1), byc_p = byc_at(k + 1);
+            const double bz1 = B1.Bz[zf1 + oz0], bz1jm = B1.Bz[zf1 + ozjm], bz1jp = B1.Bz[zf1 + ozjp],
+                         bz1im = B1.Bz[zf1 + ozim], bz1ip = B1.Bz[zf1 + ozip];
+            const size_t xk = (size_t)k * sx, yk = (size_t)k * sy, q = zf + oz0;  // q = ((k Ny + j) Nx + i)
+            double rho = Us.rho[q], mx = Us.mx[q], my = Us.my[q], mz = Us.mz[q], E = Us.E[q];
+            if (g.eta > 0.0) {  // add_joule_heating, M3:652-692
+                const double dBzdy = (0.5 * (bzjp + bz1jp) - 0.5 * (bzjm + bz1jm)) / (2 * g.dy);
+                const double dBydz = (byc_p - byc_m) / (2 * g.dz);
+                const double dBxdz = (bxc_p - bxc_m) / (2 * g.dz);
+                const double dBzdx = (0.5 * (bzip + bz1ip) - 0.5 * (bzim + bz1im)) / (2 * g.dx);
+                const double dBydx = (0.5 * (B1.By[yk + oyp] + B1.By[yk + oyp + Nx]) -
+                                      0.5 * (B1.By[yk + oym] + B1.By[yk + oym + Nx])) / (2 * g.dx);
+                const double dBxdy = (0.5 * (B1.Bx[xk + oxp] + B1.Bx[xk + oxp + 1]) -
+                                      0.5 * (B1.Bx[xk + oxm] + B1.Bx[xk + oxm + 1])) / (2 * g.dy);
+                const double Jx = dBzdy - dBydz, Jy = dBxdz - dBzdx, Jz = dBydx - dBxdy;
+                const double J2 = Jx * Jx + Jy * Jy + Jz * Jz;
+                E += dt * g.eta * J2;
             }
-            const double bx = 0.5 * (nxL + nxR), by = 0.5 * (nyD + nyU), bz = 0.5 * (nzB + nzF);
-            floors3d(e, rho, mx, my, mz, E, bx, by, bz);
-            Out.rho[q] = rho; Out.mx[q] = mx; Out.my[q] = my; Out.mz[q] = mz; Out.E[q] = E;
-            Out.Bx[fx_(g, i, j, k)] = nxL;
-            if (i == Nx - 1) Out.Bx[fx_(g, Nx, j, k)] = nxR;
-            Out.By[fy_(g, i, j, k)] = nyD;
-            if (j == Ny - 1) Out.By[fy_(g, i, Ny, k)] = nyU;
-            Out.Bz[fz_(g, i, j, k)] = nzB;
-            if (k == Nz - 1) Out.Bz[fz_(g, i, j, Nz)] = nzF;
-            if (g.ct_comp) {
-                Out.Bx_lo[fL] = lxL;
-                if (i == Nx - 1) Out.Bx_lo[fR] = lxR;
-                Out.By_lo[fD] = lyD;
-                if (j == Ny - 1) Out.By_lo[fU] = lyU;
-                Out.Bz_lo[fB] = lzB;
-                if (k == Nz - 1) Out.Bz_lo[fF] = lzF;
+            const size_t fL = xk + ox0, fR = fL + 1, fD = yk + oy0, fU = fD + Nx, fB = q, fF = zf1 + oz0;
+            const double bxL = B1.Bx[fL], bxR = B1.Bx[fR], byD = B1.By[fD], byU = B1.By[fU], bzB = bz0, bzF = bz1;
+            floors3d(e, rho, mx, my, mz, E, 0.5 * (bxL + bxR), 0.5 * (byD + byU), 0.5 * (bzB + bzF));  // M3:1019-1029
+            Prim<double> P;
+            if (!FINAL) {
+                Out.rho[q] = rho; Out.mx[q] = mx; Out.my[q] = my; Out.mz[q] = mz; Out.E[q] = E;
+                P = cons_to_prim3d(e, rho, mx, my, mz, E, 0.5 * (bxL + bxR), 0.5 * (byD + byU), 0.5 * (bzB + bzF));
+            } else {  // Heun average and floors, M3:1042-1065
+                rho = 0.5 * (Un.rho[q] + rho);
+                mx = 0.5 * (Un.mx[q] + mx);
+                my = 0.5 * (Un.my[q] + my);
+                mz = 0.5 * (Un.mz[q] + mz);
+                E = 0.5 * (Un.E[q] + E);
+                double nxL, nxR, nyD, nyU, nzB, nzF;
+                float lxL = 0.f, lxR = 0.f, lyD = 0.f, lyU = 0.f, lzB = 0.f, lzF = 0.f;
+                if (comp) {  // 1/2 (B^n + B^2) of the compensated fields
+                    ct_avg(Un.Bx[fL], Un.Bx_lo[fL], bxL, B1.Bx_lo[fL], nxL, lxL);
+                    ct_avg(Un.Bx[fR], Un.Bx_lo[fR], bxR, B1.Bx_lo[fR], nxR, lxR);
+                    ct_avg(Un.By[fD], Un.By_lo[fD], byD, B1.By_lo[fD], nyD, lyD);
+                    ct_avg(Un.By[fU], Un.By_lo[fU], byU, B1.By_lo[fU], nyU, lyU);
+                    ct_avg(Un.Bz[fB], Un.Bz_lo[fB], bzB, B1.Bz_lo[fB], nzB, lzB);
+                    ct_avg(Un.Bz[fF], Un.Bz_lo[fF], bzF, B1.Bz_lo[fF], nzF, lzF);
+                } else {
+                    nxL = 0.5 * (Un.Bx[fL] + bxL); nxR = 0.5 * (Un.Bx[fR] + bxR);
+                    nyD = 0.5 * (Un.By[fD] + byD); nyU = 0.5 * (Un.By[fU] + byU);
+                    nzB = 0.5 * (Un.Bz[fB] + bzB); nzF = 0.5 * (Un.Bz[fF] + bzF);
+                }
+                const double bx = 0.5 * (nxL + nxR), by = 0.5 * (nyD + nyU), bz = 0.5 * (nzB + nzF);
+                floors3d(e, rho, mx, my, mz, E, bx, by, bz);
+                Out.rho[q] = rho; Out.mx[q] = mx; Out.my[q] = my; Out.mz[q] = mz; Out.E[q] = E;
+                Out.Bx[fL] = nxL;
+                if (i == Nx - 1) Out.Bx[fR] = nxR;
+                Out.By[fD] = nyD;
+                if (j == Ny - 1) Out.By[fU] = nyU;
+                Out.Bz[fB] = nzB;
+                if (k == Nz - 1) Out.Bz[fF] = nzF;
+                if (comp) {
+                    Out.Bx_lo[fL] = lxL;
+                    if (i == Nx - 1) Out.Bx_lo[fR] = lxR;
+                    Out.By_lo[fD] = lyD;
+                    if (j == Ny - 1) Out.By_lo[fU] = lyU;
+                    Out.Bz_lo[fB] = lzB;
+                    if (k == Nz - 1) Out.Bz_lo[fF] = lzF;
+                }
+                P = cons_to_prim3d(e, rho, mx, my, mz, E, bx, by, bz);
+                speed = smax(speed, signal_speed3d(e, P));
             }
-            speed = signal_speed3d(e, cons_to_prim3d(e, rho, mx, my, mz, E, bx, by, bz));
+            W.c[0][q] = (R)P.rho; W.c[1][q] = (R)P.u; W.c[2][q] = (R)P.v; W.c[3][q] = (R)P.w;
+            W.c[4][q] = (R)P.P; W.c[5][q] = (R)P.Bx; W.c[6][q] = (R)P.By; W.c[7][q] = (R)P.Bz;
+            bxc_m = bxc_0; bxc_0 = bxc_p; byc_m = byc_0; byc_0 = byc_p;
+            bz0 = bz1; bzjm = bz1jm; bzjp = bz1jp; bzim = bz1im; bzip = bz1ip;
+            zf = zf1;
         }
     }
     if (FINAL) block_max_to(speed, spd);
 }
 
 template <bool IDEAL>
-__global__ __launch_bounds__(256) void k_speed_max3d(Geom3D g, Eos3D e_in, State3D U, double* __restrict__ spd) {
+__global__ __launch_bounds__(256) void k_speed_max3d(Geom3D g, Eos3D e_in, State3D U, double* __restrict__ spd,
+                                                     int zchunk) {
     const Eos3D e = eos_of<IDEAL>(e_in);
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y, k = blockIdx.z * 4 + (threadIdx.x >> 6);
+    const ColGrid cg(g, zchunk);
     double speed = 0.0;
-    if (i < g.Nx && k < g.Nz) speed = signal_speed3d(e, prim_at(g, e, U, i, j, k));
+    if (cg.valid(g))
+        for (int k = cg.c0; k < cg.c1; ++k) speed = smax(speed, signal_speed3d(e, prim_at(g, e, U, cg.i, cg.j, k)));
     block_max_to(speed, spd);
 }
 
@@ -599,13 +857,15 @@ __global__ __launch_bounds__(256) void k_dt_finalize3d(Geom3D g, Scalars* sc, do
     if (threadIdx.x == 0) finalize_dt(sc, maxs, smin(smin(g.dx, g.dy), g.dz), g.cfl, g.eta, dts);
 }
 
-__global__ __launch_bounds__(256) void k_divb_max3d(Geom3D g, State3D U, double* __restrict__ spd) {
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y, k = blockIdx.z * 4 + (threadIdx.x >> 6);
+__global__ __launch_bounds__(256) void k_divb_max3d(Geom3D g, State3D U, double* __restrict__ spd, int zchunk) {
+    const ColGrid cg(g, zchunk);
+    const int i = cg.i, j = cg.j;
     double v = 0.0;
-    if (i < g.Nx && k < g.Nz)  // divB_cc, M3:1070-1078 (face i+1 <= Nx etc. need no wrapping)
-        v = fabs((U.Bx[fx_(g, i + 1, j, k)] - U.Bx[fx_(g, i, j, k)]) / g.dx +
-                 (U.By[fy_(g, i, j + 1, k)] - U.By[fy_(g, i, j, k)]) / g.dy +
-                 (U.Bz[fz_(g, i, j, k + 1)] - U.Bz[fz_(g, i, j, k)]) / g.dz);
+    if (cg.valid(g))
+        for (int k = cg.c0; k < cg.c1; ++k)  // divB_cc, M3:1070-1078 (face i+1 <= Nx etc. need no wrapping)
+            v = smax(v, fabs((U.Bx[fx_(g, i + 1, j, k)] - U.Bx[fx_(g, i, j, k)]) / g.dx +
+                             (U.By[fy_(g, i, j + 1, k)] - U.By[fy_(g, i, j, k)]) / g.dy +
+                             (U.Bz[fz_(g, i, j, k + 1)] - U.Bz[fz_(g, i, j, k)]) / g.dz));
     block_max_to(v, spd);
 }
 
@@ -698,6 +958,10 @@ struct cc_mhd3d {
     double* spd2 = nullptr;  // kDtPart3 second-level maxima
     double* red = nullptr;
     int nspd = 0;
+    // edge EMFs + CT in one pass (k_ect3d, tile shape ect_shape); k_edge3d + k_ct3d otherwise
+    bool fused = true;
+    int ect_shape = 1;
+    int zchunk = 32;
     Profiler prof;
     FrameBuffer frame;
     FieldStats stats;
@@ -714,8 +978,9 @@ struct cc_mhd3d {
         return CC_OK;
     }
     size_t NFx() const { return (size_t)(p.Nx + 1) * p.Ny * p.Nz; }
-    // grid of the cell kernels that reduce per block: 64 i x 4 k of one j row per 256-thread block
-    dim3 cell_grid() const { return dim3((p.Nx + 63) / 64, p.Ny, (p.Nz + 3) / 4); }
+    // grid of the column-marching cell kernels (Joule, speed, divB): 64 i x 4 j columns per block, z chunks
+    int jchunk = 16;
+    dim3 col_grid() const { return dim3((p.Nx + 63) / 64, (p.Ny + 3) / 4, (p.Nz + jchunk - 1) / jchunk); }
     size_t NFy() const { return (size_t)p.Nx * (p.Ny + 1) * p.Nz; }
     size_t NFz() const { return (size_t)p.Nx * p.Ny * (p.Nz + 1); }
     Profiler* pr() { return prof.on ? &prof : nullptr; }
@@ -789,12 +1054,6 @@ struct cc_mhd3d {
     template <bool MIXED, typename EOS64, typename EOS32>
     cc_status launch_flux(const EOS64& e64, const EOS32& e32, const State3D& in) {
         using R = typename std::conditional<MIXED, float, double>::type;
-        {
-            ProfScope ps(pr(), stream, "prim");
-            if (e.ideal) hipLaunchKernelGGL((k_prim3d<R, true>), dim3(blocks(N())), dim3(256), 0, stream, g, e, in, W<R>());
-            else hipLaunchKernelGGL((k_prim3d<R, false>), dim3(blocks(N())), dim3(256), 0, stream, g, e, in, W<R>());
-            CC_CHECK_LAUNCH();
-        }
         if (MIXED) {
             hipLaunchKernelGGL(k_zero_count3, dim3(1), dim3(64), 0, stream, dq.count);
             CC_CHECK_LAUNCH();
@@ -853,7 +1112,11 @@ struct cc_mhd3d {
     cc_status stage(const State3D& in, const State3D& out, const State3D& hn) {
         const int Nx = p.Nx, Ny = p.Ny, Nz = p.Nz;
         CC_TRY(flux(in));
-        {
+        const State3D& bdst = FINAL ? Bt : out;
+        if (fused) {
+            ProfScope ps(pr(), stream, "ect");
+            CC_TRY(p.precision == CC_MIXED ? launch_ect<float>(in, bdst) : launch_ect<double>(in, bdst));
+        } else {
             ProfScope ps(pr(), stream, "edge");
             if (p.precision == CC_MIXED)
                 hipLaunchKernelGGL(k_edge3d<float>, dim3((Nx + 1 + 63) / 64, Ny + 1, (Nz + 1 + 3) / 4), dim3(256), 0, stream, g,
@@ -863,19 +1126,40 @@ struct cc_mhd3d {
                                    EF<double>(), in, ed);
             CC_CHECK_LAUNCH();
         }
-        const State3D& bdst = FINAL ? Bt : out;
-        {
+        if (!fused) {
             ProfScope ps(pr(), stream, "ct");
             hipLaunchKernelGGL(k_ct3d, dim3((Nx + 1 + 63) / 64, Ny + 1, (Nz + 1 + 3) / 4), dim3(256), 0, stream, g, ed, in, bdst, sc);
             CC_CHECK_LAUNCH();
         }
         {
             ProfScope ps(pr(), stream, "joule");
-            hipLaunchKernelGGL((e.ideal ? k_joule3d<FINAL, true> : k_joule3d<FINAL, false>), cell_grid(), dim3(256), 0, stream, g, e, Us, bdst, sc, hn,
-                               out, spd);
+            if (p.precision == CC_MIXED)
+                hipLaunchKernelGGL((e.ideal ? k_joule3d<float, FINAL, true> : k_joule3d<float, FINAL, false>), col_grid(), dim3(256), 0,
+                                   stream, g, e, Us, bdst, sc, hn, out, spd, jchunk, W<float>());
+            else
+                hipLaunchKernelGGL((e.ideal ? k_joule3d<double, FINAL, true> : k_joule3d<double, FINAL, false>), col_grid(), dim3(256), 0,
+                                   stream, g, e, Us, bdst, sc, hn, out, spd, jchunk, W<double>());
             CC_CHECK_LAUNCH();
         }
         return CC_OK;
+    }
+
+    template <typename R, int TX, int TY>
+    cc_status launch_ect_shape(const State3D& in, const State3D& B1) {
+        const dim3 grid((unsigned)(((p.Nx + TX - 1) / TX) * ((p.Ny + TY - 1) / TY)),
+                        (unsigned)((p.Nz + zchunk - 1) / zchunk));
+        hipLaunchKernelGGL((k_ect3d<R, TX, TY>), grid, dim3(EctShape<TX, TY>::NT), 0, stream, g, EF<R>(), in, B1, sc,
+                           zchunk);
+        CC_CHECK_LAUNCH();
+        return CC_OK;
+    }
+    template <typename R>
+    cc_status launch_ect(const State3D& in, const State3D& B1) {
+        switch (ect_shape) {
+            case 1: return launch_ect_shape<R, 32, 12>(in, B1);
+            case 2: return launch_ect_shape<R, 64, 5>(in, B1);
+            default: return launch_ect_shape<R, 32, 28>(in, B1);
+        }
     }
 
     cc_status heun(int a, int b, int c, bool driver, double* dts) {
@@ -890,8 +1174,21 @@ struct cc_mhd3d {
         return CC_OK;
     }
 
+    // Primitives of s in the flux precision for the first stage's face kernels; every stage's Joule pass
+    // leaves those of its result for the next stage, so a step sequence needs this once at its start.
+    cc_status launch_prim(const State3D& s) {
+        ProfScope ps(pr(), stream, "prim");
+        const bool mixed = p.precision == CC_MIXED;
+        if (mixed && e.ideal) hipLaunchKernelGGL((k_prim3d<float, true>), dim3(blocks(N())), dim3(256), 0, stream, g, e, s, W<float>());
+        else if (mixed) hipLaunchKernelGGL((k_prim3d<float, false>), dim3(blocks(N())), dim3(256), 0, stream, g, e, s, W<float>());
+        else if (e.ideal) hipLaunchKernelGGL((k_prim3d<double, true>), dim3(blocks(N())), dim3(256), 0, stream, g, e, s, W<double>());
+        else hipLaunchKernelGGL((k_prim3d<double, false>), dim3(blocks(N())), dim3(256), 0, stream, g, e, s, W<double>());
+        CC_CHECK_LAUNCH();
+        return CC_OK;
+    }
+
     cc_status speed_max(const State3D& s) {
-        hipLaunchKernelGGL((e.ideal ? k_speed_max3d<true> : k_speed_max3d<false>), cell_grid(), dim3(256), 0, stream, g, e, s, spd);
+        hipLaunchKernelGGL((e.ideal ? k_speed_max3d<true> : k_speed_max3d<false>), col_grid(), dim3(256), 0, stream, g, e, s, spd, jchunk);
         CC_CHECK_LAUNCH();
         return CC_OK;
     }
@@ -946,12 +1243,16 @@ cc_status cc_mhd3d_create(const cc_mhd3d_params* p, cc_mhd3d** out) {
     if (st == CC_OK) st = h->alloc_cells(h->Us);
     if (st == CC_OK) st = h->alloc_faces(h->Bt);
     if (st == CC_OK) st = h->alloc_flux_path();
-    if (st == CC_OK) st = h->alloc(&h->ed.Ex, (size_t)p->Nx * (p->Ny + 1) * (p->Nz + 1));
-    if (st == CC_OK) st = h->alloc(&h->ed.Ey, (size_t)(p->Nx + 1) * p->Ny * (p->Nz + 1));
-    if (st == CC_OK) st = h->alloc(&h->ed.Ez, (size_t)(p->Nx + 1) * (p->Ny + 1) * p->Nz);
+    if (const char* v = std::getenv("CC_M3_FUSED")) h->fused = std::atoi(v) != 0;
+    if (const char* v = std::getenv("CC_M3_ECT_SHAPE")) h->ect_shape = std::atoi(v);
+    if (const char* v = std::getenv("CC_M3_ZCHUNK")) h->zchunk = std::max(1, std::atoi(v));
+    if (const char* v = std::getenv("CC_M3_JCHUNK")) h->jchunk = std::max(1, std::atoi(v));
+    if (st == CC_OK && !h->fused) st = h->alloc(&h->ed.Ex, (size_t)p->Nx * (p->Ny + 1) * (p->Nz + 1));
+    if (st == CC_OK && !h->fused) st = h->alloc(&h->ed.Ey, (size_t)(p->Nx + 1) * p->Ny * (p->Nz + 1));
+    if (st == CC_OK && !h->fused) st = h->alloc(&h->ed.Ez, (size_t)(p->Nx + 1) * (p->Ny + 1) * p->Nz);
     if (st == CC_OK) st = h->alloc(&h->dts_dev, kDtLog3);
     {
-        const dim3 cg = h->cell_grid();  // the per-block maxima of the cell kernels (speed, Joule, divB)
+        const dim3 cg = h->col_grid();  // the per-block maxima of the cell kernels (speed, Joule, divB)
         h->nspd = (int)(cg.x * cg.y * cg.z);
     }
     if (st == CC_OK) st = h->alloc(&h->spd, (size_t)h->nspd);
@@ -1037,6 +1338,7 @@ cc_status cc_mhd3d_stage(cc_mhd3d* h, double dt) {
     CHECK_H3();
     hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, h->stream, h->sc, dt, 0.0, 1e300, 0);
     const int a = h->cur, b = (a + 1) % 3;
+    CC_TRY(h->launch_prim(h->S[a]));
     CC_TRY(h->stage<false>(h->S[a], h->S[b], h->S[a]));
     h->cur = b;
     return CC_OK;
@@ -1047,6 +1349,7 @@ cc_status cc_mhd3d_rk2_step(cc_mhd3d* h, double dt) {
     if (!std::isfinite(dt) || dt < 0.0) { set_error(CC_ERR_BAD_DT, "dt must be finite and >= 0"); return CC_ERR_BAD_DT; }
     hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, h->stream, h->sc, dt, 0.0, 1e300, 0);
     const int a = h->cur, b = (a + 1) % 3, c = (a + 2) % 3;
+    CC_TRY(h->launch_prim(h->S[a]));
     CC_TRY(h->heun(a, b, c, false, nullptr));
     h->cur = c;
     return CC_OK;
@@ -1068,6 +1371,7 @@ cc_status cc_mhd3d_run(cc_mhd3d* h, int nsteps, double* t_inout, double t_end, d
     }
     hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, h->stream, h->sc, 0.0, *t_inout, t_end, 1);
     CC_TRY(h->speed_max(h->S[h->cur]));
+    CC_TRY(h->launch_prim(h->S[h->cur]));
     const bool use_graph = !h->prof.on;
     for (int s = 0; s < nsteps; ++s) {
         const int a = h->cur, b = (a + 1) % 3, c = (a + 2) % 3;
@@ -1100,8 +1404,8 @@ cc_status cc_mhd3d_run(cc_mhd3d* h, int nsteps, double* t_inout, double t_end, d
 cc_status cc_mhd3d_divb_max(cc_mhd3d* h, double* out) {
     CHECK_H3();
     if (!out) { set_error(CC_ERR_INVALID_ARG, "null output"); return CC_ERR_INVALID_ARG; }
-    hipLaunchKernelGGL(k_divb_max3d, h->cell_grid(), dim3(256), 0, h->stream, h->g,
-                       h->S[h->cur], h->spd);
+    hipLaunchKernelGGL(k_divb_max3d, h->col_grid(), dim3(256), 0, h->stream, h->g,
+                       h->S[h->cur], h->spd, h->jchunk);
     hipLaunchKernelGGL(k_reduce_max_to, dim3(1), dim3(256), 0, h->stream, h->spd, h->nspd, h->red);
     CC_CHECK_LAUNCH();
     CC_HIP(hipMemcpyAsync(out, h->red, 8, hipMemcpyDeviceToHost, h->stream));
