@@ -1,14 +1,14 @@
 // mhd2d.hip -- gfx950 solver step for 2D resistive MHD with HLLD + constrained transport + SOR
 // self-gravity (restating mhd2d_resistive_hlld_ct_gravity.cpp, cited as M2:<line>).
 //
-// One forward-Euler stage (advance_stage, M2:690-834) is three kernels:
+// One forward-Euler stage (advance_stage, M2:690-834) is two kernels:
 //   k_flux2d      PLM-MC reconstruction + HLLD on x- and y-faces of a 32x8 tile from primitives
 //                 staged once per cell in LDS, flux difference + floors (old faces B0)  -> U*,
 //                 face EMFs Ez_x, Ez_y
-//   k_corner2d    corner Ez = 1/4 sum of face EMFs + eta Jz(B0)                          -> Ez
-//   k_ct_src2d    CT face update, gravity source, floors (B1), Joule heating, Bz diffusion,
-//                 floors (B1); the stage-2 instance also forms the Heun average, its floors
-//                 and the next step's CFL signal-speed maximum (wave64 shuffles + one atomic).
+//   k_ct_src2d    corner Ez = 1/4 sum of face EMFs + eta Jz(B0) of a 64x4 tile into LDS, CT face
+//                 update, gravity source, floors (B1), Joule heating, Bz diffusion, floors (B1); the
+//                 stage-2 instance also forms the Heun average, its floors and the next step's CFL
+//                 signal-speed maximum (wave64 shuffles, one partial per block).
 // Heun (rk2_step, M2:838-869) = stage(U^n, g) -> U1; g1 = Poisson(U1.rho); stage(U1, g1) + average.
 // In the mixed-precision form the HLLD runs in FP32 (recomputed in FP64 for faces near the Alfven
 // degeneracy) on FP64 primitives and PLM states; flux differences, conserved updates, EMF averages
@@ -181,31 +181,52 @@ __global__ __launch_bounds__(256) void k_flux2d(Geom2D g, State2D U0, const Scal
 }
 
 // ------------------------------------------------------------------ corner EMF (M2:781-804)
-__global__ void k_corner2d(Geom2D g, const double* __restrict__ ezx, const double* __restrict__ ezy,
-                           const double* __restrict__ Bx, const double* __restrict__ By, double* __restrict__ ez) {
-    const int ie = blockIdx.x * blockDim.x + threadIdx.x, je = blockIdx.y;
+// Corner (ie, je), ie in [0, Nx], je in [0, Ny]: 1/4 of the four face EMFs + eta Jz of B0 (Jz_edge,
+// M2:565-575), with the reference's index maps.
+__device__ __forceinline__ double corner_ez(const Geom2D& g, const double* __restrict__ ezx, const double* __restrict__ ezy,
+                                           const double* __restrict__ Bx, const double* __restrict__ By, int ie, int je) {
     const int Nx = g.Nx, Ny = g.Ny;
-    if (ie > Nx) return;
     const int jm = bmap(je - 1, Ny, g.py), jc = bmap(je, Ny, g.py);
     const int im = bmap(ie - 1, Nx, g.px), ic = bmap(ie, Nx, g.px);
     const double ez_ideal = 0.25 * (ezx[(size_t)jm * (Nx + 1) + ie] + ezx[(size_t)jc * (Nx + 1) + ie] +
                                     ezy[(size_t)je * Nx + im] + ezy[(size_t)je * Nx + ic]);
     double ez_res = 0.0;
-    if (g.eta > 0.0) {  // Jz_edge, M2:565-575
+    if (g.eta > 0.0) {
         const double dBydx = (By[(size_t)je * Nx + ic] - By[(size_t)je * Nx + im]) / g.dx;
         const double dBxdy = (Bx[(size_t)jc * (Nx + 1) + ie] - Bx[(size_t)jm * (Nx + 1) + ie]) / g.dy;
         ez_res = g.eta * (dBydx - dBxdy);
     }
-    ez[(size_t)je * (Nx + 1) + ie] = ez_ideal + ez_res;
+    return ez_ideal + ez_res;
 }
+
+// The corner EMFs a 64 x 4 cell tile reads: corners (i0 .. i0+65, j0 .. j0+5) in LDS (the CT of the
+// tile's faces and the Joule term's B1 of the +x / +y neighbour faces), any other one (across a periodic
+// boundary) evaluated where it is read.
+struct CornerTile {
+    static constexpr int TX = 64, TY = 4, CX = TX + 2, CY = TY + 2;
+    const Geom2D& g;
+    const double* ezx;
+    const double* ezy;
+    const double* Bx;
+    const double* By;
+    const double* lds;
+    int i0, j0;
+    __device__ double operator()(int ie, int je) const {
+        const int x = ie - i0, y = je - j0;
+        if (x >= 0 && x < CX && y >= 0 && y < CY && ie <= g.Nx && je <= g.Ny) return lds[y * CX + x];
+        return corner_ez(g, ezx, ezy, Bx, By, ie, je);
+    }
+};
 
 // ------------------------------------------------------------------ CT + sources (+ Heun) kernel
 // Increments of x-face (fi, fj) and y-face (fi, fj) from the corner EMFs (M2:807-830).
-__device__ __forceinline__ double ct_dbx(const Geom2D& g, const double* ez, int fi, int fj, double dt) {
-    return (-(dt / g.dy)) * (ez[(size_t)(fj + 1) * (g.Nx + 1) + fi] - ez[(size_t)fj * (g.Nx + 1) + fi]);
+template <typename EZ>
+__device__ __forceinline__ double ct_dbx(const Geom2D& g, const EZ& ez, int fi, int fj, double dt) {
+    return (-(dt / g.dy)) * (ez(fi, fj + 1) - ez(fi, fj));
 }
-__device__ __forceinline__ double ct_dby(const Geom2D& g, const double* ez, int fi, int fj, double dt) {
-    return (dt / g.dx) * (ez[(size_t)fj * (g.Nx + 1) + fi + 1] - ez[(size_t)fj * (g.Nx + 1) + fi]);
+template <typename EZ>
+__device__ __forceinline__ double ct_dby(const Geom2D& g, const EZ& ez, int fi, int fj, double dt) {
+    return (dt / g.dx) * (ez(fi + 1, fj) - ez(fi, fj));
 }
 // B1 of a face: the reference's B0 + increment, or its compensated form (hi word; lo into *lo).
 __device__ __forceinline__ double ct_face(const Geom2D& g, const double* B0, const float* B0lo, size_t f, double d,
@@ -220,16 +241,28 @@ __device__ __forceinline__ double ct_face(const Geom2D& g, const double* B0, con
 __device__ __forceinline__ size_t fxi(const Geom2D& g, int fi, int fj) { return (size_t)fj * (g.Nx + 1) + fi; }
 __device__ __forceinline__ size_t fyi(const Geom2D& g, int fi, int fj) { return (size_t)fj * g.Nx + fi; }
 
+// One kernel for corner EMFs, CT and the cell sources of a stage: a 64 x 4 tile evaluates the corner EMFs
+// it reads into LDS (1.5 per cell), then updates its faces and cells.
 template <bool FINAL>
 __global__ __launch_bounds__(256) void k_ct_src2d(Geom2D g, State2D Us, const double* __restrict__ Bx0,
                                                   const double* __restrict__ By0, const float* __restrict__ Bx0lo,
-                                                  const float* __restrict__ By0lo, const double* __restrict__ ez,
+                                                  const float* __restrict__ By0lo, const double* __restrict__ ezx,
+                                                  const double* __restrict__ ezy,
                                                   const double* __restrict__ gx, const double* __restrict__ gy,
                                                   Scalars* __restrict__ sc, State2D Un, State2D Out,
                                                   double* __restrict__ spd) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    constexpr int CX = CornerTile::CX, CY = CornerTile::CY;
+    __shared__ double s_ez[CX * CY];
+    const int i0 = blockIdx.x * CornerTile::TX, j0 = blockIdx.y * CornerTile::TY;
+    const int i = i0 + (int)(threadIdx.x & 63), j = j0 + (int)(threadIdx.x >> 6);
     const int Nx = g.Nx, Ny = g.Ny;
-    const bool active = i < Nx;
+    for (int t = threadIdx.x; t < CX * CY; t += 256) {
+        const int ie = i0 + t % CX, je = j0 + t / CX;
+        if (ie <= Nx && je <= Ny) s_ez[t] = corner_ez(g, ezx, ezy, Bx0, By0, ie, je);
+    }
+    __syncthreads();
+    const CornerTile ez{g, ezx, ezy, Bx0, By0, s_ez, i0, j0};
+    const bool active = i < Nx && j < Ny;
     double speed = 0.0;
     if (active) {
         const size_t q = (size_t)j * Nx + i;
@@ -322,10 +355,11 @@ __global__ __launch_bounds__(256) void k_ct_src2d(Geom2D g, State2D Us, const do
 }
 
 // ------------------------------------------------------------------ dt, diagnostics, small kernels
+// (the grid of k_ct_src2d, 64 x 4 cells per block, so both leave the same number of per-block maxima)
 __global__ __launch_bounds__(256) void k_speed_max2d(Geom2D g, State2D U, double* __restrict__ spd) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    const int i = blockIdx.x * 64 + (int)(threadIdx.x & 63), j = blockIdx.y * 4 + (int)(threadIdx.x >> 6);
     double speed = 0.0;
-    if (i < g.Nx) {
+    if (i < g.Nx && j < g.Ny) {
         const size_t q = (size_t)j * g.Nx + i;
         const double bx = 0.5 * (U.Bx[(size_t)j * (g.Nx + 1) + i] + U.Bx[(size_t)j * (g.Nx + 1) + i + 1]);
         const double by = 0.5 * (U.By[(size_t)j * g.Nx + i] + U.By[(size_t)(j + 1) * g.Nx + i]);
@@ -429,12 +463,14 @@ struct cc_mhd2d {
     State2D S[3]{};  // state sets; cur/aux roles rotate
     int cur = 0;
     State2D Us{};    // post-flux cells (only the 6 cell arrays are used)
-    double *ezx = nullptr, *ezy = nullptr, *ez = nullptr;
+    double *ezx = nullptr, *ezy = nullptr;
     double *gx = nullptr, *gy = nullptr, *g1x = nullptr, *g1y = nullptr;
     Scalars* sc = nullptr;
     double* dts_dev = nullptr;  // device log of per-step dt (kDtLog entries)
     double* spd = nullptr;      // per-block signal-speed maxima of the last speed pass
     int nspd = 0;
+    // grid of the CT + sources kernel and of the signal-speed kernel: 64 x 4 cells per 256-thread block
+    dim3 ct_grid() const { return dim3((unsigned)((p.Nx + 63) / 64), (unsigned)((p.Ny + 3) / 4)); }
     Poisson2D pois;
     Profiler prof;
     FrameBuffer frame;
@@ -491,15 +527,9 @@ struct cc_mhd2d {
             CC_CHECK_LAUNCH();
         }
         {
-            ProfScope ps(pr(), stream, "corner");
-            hipLaunchKernelGGL(k_corner2d, dim3((Nx + 1 + 127) / 128, Ny + 1), dim3(128), 0, stream, g, ezx, ezy, in.Bx,
-                               in.By, ez);
-            CC_CHECK_LAUNCH();
-        }
-        {
             ProfScope ps(pr(), stream, "ct");
-            hipLaunchKernelGGL(k_ct_src2d<FINAL>, dim3((Nx + 255) / 256, Ny), dim3(256), 0, stream, g, Us, in.Bx, in.By,
-                               in.Bx_lo, in.By_lo, ez, gxp, gyp, sc, hn, out, spd);
+            hipLaunchKernelGGL(k_ct_src2d<FINAL>, ct_grid(), dim3(256), 0, stream, g, Us, in.Bx, in.By,
+                               in.Bx_lo, in.By_lo, ezx, ezy, gxp, gyp, sc, hn, out, spd);
             CC_CHECK_LAUNCH();
         }
         return CC_OK;
@@ -535,7 +565,7 @@ struct cc_mhd2d {
 
     cc_status speed_max(const State2D& s) {
         ProfScope ps(pr(), stream, "dt");
-        hipLaunchKernelGGL(k_speed_max2d, dim3((p.Nx + 255) / 256, p.Ny), dim3(256), 0, stream, g, s, spd);
+        hipLaunchKernelGGL(k_speed_max2d, ct_grid(), dim3(256), 0, stream, g, s, spd);
         CC_CHECK_LAUNCH();
         return CC_OK;
     }
@@ -602,13 +632,12 @@ cc_status cc_mhd2d_create(const cc_mhd2d_params* p, cc_mhd2d** out) {
     if (st == CC_OK) st = h->alloc_state(h->Us, false);
     if (st == CC_OK) st = h->alloc(&h->ezx, h->NFx());
     if (st == CC_OK) st = h->alloc(&h->ezy, h->NFy());
-    if (st == CC_OK) st = h->alloc(&h->ez, (size_t)(p->Nx + 1) * (p->Ny + 1));
     if (st == CC_OK) st = h->alloc(&h->gx, h->N());
     if (st == CC_OK) st = h->alloc(&h->gy, h->N());
     if (st == CC_OK) st = h->alloc(&h->g1x, h->N());
     if (st == CC_OK) st = h->alloc(&h->g1y, h->N());
     if (st == CC_OK) st = h->alloc(&h->dts_dev, kDtLog);
-    h->nspd = ((p->Nx + 255) / 256) * p->Ny;
+    h->nspd = (int)(h->ct_grid().x * h->ct_grid().y);
     if (st == CC_OK) st = h->alloc(&h->spd, (size_t)h->nspd);
     if (st == CC_OK && hipMalloc((void**)&h->sc, sizeof(Scalars)) != hipSuccess) {
         set_error(CC_ERR_OOM, "scalar allocation failed");

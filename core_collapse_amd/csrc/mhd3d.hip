@@ -3,13 +3,16 @@
 // cold + thermal + bounce EOS or an ideal gas.
 //
 // One forward-Euler stage (advance_stage, M3:694-1030) is:
-//   k_prim3d    primitives of every cell (FP64 recovery), stored in the flux precision
 //   k_face3d    x, y and z faces, one face per thread: PLM-MC + HLLD -> 5 face fluxes, 2 face EMFs
+//               (from the primitives in the flux precision, left by the previous stage's k_joule3d or,
+//               at the start of a call, by k_prim3d)
 //   k_update3d  flux difference + floors (old faces), FP64                      -> U*
-//   k_edge3d    Ex, Ey, Ez edges = 1/4 sum of face EMFs + eta * curl of cell-centred B0
-//   k_ct3d      face-field induction on the three face arrays                  -> B1
-//   k_joule3d   Joule heating from cell-centred B1 differences + floors; the stage-2 instance also
-//               forms the Heun average, its floors and the next step's signal-speed maximum.
+//   k_ect3d     edge EMFs (1/4 sum of face EMFs + eta * curl of cell-centred B0) and the face-field
+//               induction, in one z-marching pass over LDS tiles              -> B1
+//   k_joule3d   Joule heating from cell-centred B1 differences + floors, marching along z per column;
+//               the stage-2 instance also forms the Heun average, its floors and the next step's
+//               signal-speed maximum; both store the primitives of their result for the next stage.
+// k_edge3d + k_ct3d (edge arrays in memory) replace k_ect3d when CC_M3_FUSED=0.
 // Mixed precision: primitives recovered in FP64 and stored in FP32; PLM and HLLD in FP32 (HLLD
 // recomputed in FP64 for faces near the Alfven degeneracy, where the reference's star states are
 // ill-conditioned); face fluxes and EMFs stored in FP32; flux differences, the conserved update,
@@ -1156,9 +1159,9 @@ struct cc_mhd3d {
     template <typename R>
     cc_status launch_ect(const State3D& in, const State3D& B1) {
         switch (ect_shape) {
-            case 1: return launch_ect_shape<R, 32, 12>(in, B1);
+            case 0: return launch_ect_shape<R, 32, 28>(in, B1);
             case 2: return launch_ect_shape<R, 64, 5>(in, B1);
-            default: return launch_ect_shape<R, 32, 28>(in, B1);
+            default: return launch_ect_shape<R, 32, 12>(in, B1);
         }
     }
 
