@@ -118,7 +118,7 @@ def bench_collapse2d(N, warmup, steps, acc_steps, prof_steps):
         h.upload(ic.collapse2d_ic(p))
         t = h.run(warmup) if warmup > 0 else 0.0
         t, el = timed_run(h, steps, t)
-        t, prof = profiled_pass(h, prof_steps, t, ("sor", "flux", "corner", "ct", "grad", "mean", "dt"))
+        t, prof = profiled_pass(h, prof_steps, t, ("sor", "flux", "ct", "grad", "mean", "dt"))
         res[prec] = {"elapsed": el, "prof": prof, "tile": M.sor_tile_choice(N, N, p.sor_iters, prec)}
         h.close()
         log("collapse2d %s %.3f ms/step" % ("fp64" if prec == CC_FP64 else "mixed", 1e3 * el / steps))
@@ -287,8 +287,9 @@ def bench3d(N, warmup, steps, prof_steps, with_cpu):
     for prec, tag, R, mx in ((CC_FP64, "fp64", "double", "false"), (CC_MIXED, "mixed", "float", "true")):
         groups = {"prim": ["cc::k_prim3d<%s>" % R],
                   "face": ["cc::k_face3d<%d, %s," % (a, mx) for a in range(3)],
-                  "update": ["cc::k_update3d<%s>" % R], "edge": ["cc::k_edge3d<%s>" % R], "ct": ["cc::k_ct3d"],
-                  "joule": ["cc::k_joule3d<true>", "cc::k_joule3d<false>"]}
+                  "update": ["cc::k_update3d<%s>" % R], "ect": ["cc::k_ect3d<%s, 32, 12>" % R],
+                  "edge": ["cc::k_edge3d<%s>" % R], "ct": ["cc::k_ct3d"],
+                  "joule": ["cc::k_joule3d<%s, true, true>" % R, "cc::k_joule3d<%s, false, true>" % R]}
         hbm = measured_hbm(prof[prec], groups)
         out.setdefault("hbm_measured", {})[tag] = hbm
         zu_p = float(N) ** 3 * prof_steps
