@@ -265,7 +265,7 @@ def bench3d(N, warmup, steps, prof_steps, with_cpu):
         rmx["traffic"] = load_traffic("cc::k_face3d<0, true")
     out = {
         "workload": "3D resistive MHD HLLD+CT, periodic turbulence %d^3, ideal gas gamma 5/3, beta 1 along z, "
-                    "rms Mach 1, eta 5e-5, CFL 0.4, HLLD degeneracy guard 1e-2, compensated face update" % N,
+                    "rms Mach 1, eta 5e-5, CFL 0.4, HLLD degeneracy guard 5e-2, compensated face update" % N,
         "steps": steps, "warmup": warmup,
         "value": zu / el[CC_MIXED], "unit": "zone-updates/s",
         "ms_per_step": 1e3 * el[CC_MIXED] / steps,

@@ -104,6 +104,36 @@ def test_guarded_hlld_is_the_reference_face_or_hll():
     assert 100 < changed < n // 2, changed
 
 
+# A z-face of the 256^3 turbulence benchmark run with guard 1e-2 (mixed precision, step 294; FP64 gives the
+# same flux on these states), in the x-normal frame of hlld_x: a strong normal field (Bn = 1.61) and a
+# transverse velocity jump of ~4.  d_L / max(Bn^2, |rho s (S - S_M)|) = 0.0103 passes a 1e-2 guard, the star
+# states amplify By by ~42 and the energy flux is 1.35e3 from O(1) states: the cell behind the face went to
+# P ~ 1e10 in the next step.  Faces of this kind left ~50 single-cell pressure spikes in the FP64 run as well.
+TURB_WL = np.array([0.33268, -0.17699, -1.73912, -3.19886, 0.27983, 1.6144, 0.50274, 0.75518])
+TURB_WR = np.array([0.53041, -0.17699, 0.89125, 0.7546, 3.56277, 1.6144, -0.12061, -0.05063])
+TURB_BN = 1.6144
+
+
+def _m3_hlld_ideal(guard, WL, WR, Bn):
+    op = OB.m3_params(ideal_eos=1, gamma_ideal=G53, hlld_guard=guard)
+    out = np.zeros(7)
+    oracle().or_m3_hlld_x(C.byref(op), OB.ptr(np.ascontiguousarray(WL, dtype=np.float64)),
+                          OB.ptr(np.ascontiguousarray(WR, dtype=np.float64)), C.c_double(Bn), OB.ptr(out))
+    return out
+
+
+def test_turbulence_face_amplified_at_guard_1e2_takes_hll_at_the_benchmark_guard():
+    ref = _m3_hlld_ideal(0.0, TURB_WL, TURB_WR, TURB_BN)
+    assert abs(ref[4]) > 1e3 and abs(ref[5]) > 40, ref                       # the reference's HLLD
+    assert np.array_equal(_m3_hlld_ideal(1e-2, TURB_WL, TURB_WR, TURB_BN), ref)  # a 1e-2 guard lets it through
+    guard = ic.turbulence3d_params(16).hlld_guard
+    assert guard >= 3e-2, guard
+    got = _m3_hlld_ideal(guard, TURB_WL, TURB_WR, TURB_BN)
+    want = _hll_textbook(TURB_WL, TURB_WR, TURB_BN)
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-14 * np.abs(want).max()), (got, want)
+    assert np.abs(got).max() < 5, got
+
+
 def test_turbulence_problem_blows_up_in_reference_arithmetic_and_not_with_the_guard():
     """The 3D turbulence IC (beta 1 field along z, rms Mach 1) at 16^3 for 12 steps: with the reference's
     HLLD the total energy density reaches ~1e11 and dt falls to ~1e-8; with the guard it stays O(1)."""
