@@ -553,8 +553,8 @@ struct EctShape {
     static_assert(NP <= 1024, "tile positions exceed the block");
 };
 
-template <typename R, int TX, int TY>
-__global__ __launch_bounds__((EctShape<TX, TY>::NT)) void k_ect3d(Geom3D g, EmfArr<R> ef, State3D B0, State3D B1,
+template <typename R, int TX, int TY, int MINW = 1>
+__global__ __launch_bounds__((EctShape<TX, TY>::NT), MINW) void k_ect3d(Geom3D g, EmfArr<R> ef, State3D B0, State3D B1,
                                                                 const Scalars* __restrict__ sc, int zchunk) {
     constexpr int PX = EctShape<TX, TY>::PX, NP = EctShape<TX, TY>::NP;
     __shared__ double s_cbx[NP], s_cby[NP], s_cbz[NP];  // bxc, byc of plane s; bzc of plane s+1
@@ -590,7 +590,8 @@ __global__ __launch_bounds__((EctShape<TX, TY>::NT)) void k_ect3d(Geom3D g, EmfA
     // Ex_y, Ey_x of plane s+1, Ex_z, Ey_z of z-face s+1, Ez_x, Ez_y of plane s, and the low words of the
     // faces this position updates.  Loaded one step ahead.
     struct StepIn {
-        double bxa, bxb, bya, byb, bza, bzb, exy, eyx, exzm, exz, eyzm, eyz, ezxm, ezx, ezym, ezy;
+        double bxa, bxb, bya, byb, bza, bzb;
+        R exy, eyx, exzm, exz, eyzm, eyz, ezxm, ezx, ezym, ezy;  // face EMFs in the flux precision
         float lbx, lby, lbz;
     };
     auto load_step = [&](int s, StepIn& L) {
@@ -599,11 +600,11 @@ __global__ __launch_bounds__((EctShape<TX, TY>::NT)) void k_ect3d(Geom3D g, EmfA
         L.bxa = B0.Bx[xp + ox_ab]; L.bxb = B0.Bx[xp + ox_a1b];
         L.bya = B0.By[yp + oy_ab]; L.byb = B0.By[yp + oy_ab1];
         L.bza = B0.Bz[zp + oz_ab]; L.bzb = B0.Bz[zp1 + oz_ab];
-        L.exy = (double)ef.Ex_y[yp + oy_ab]; L.eyx = (double)ef.Ey_x[xp + ox_ab];
-        L.exzm = (double)ef.Ex_z[zp + oz_abm]; L.exz = (double)ef.Ex_z[zp + oz_ab];
-        L.eyzm = (double)ef.Ey_z[zp + oz_amb]; L.eyz = (double)ef.Ey_z[zp + oz_ab];
-        L.ezxm = (double)ef.Ez_x[xs0 + ox_abm]; L.ezx = (double)ef.Ez_x[xs0 + ox_ab];
-        L.ezym = (double)ef.Ez_y[ys0 + oy_amb]; L.ezy = (double)ef.Ez_y[ys0 + oy_ab];
+        L.exy = ef.Ex_y[yp + oy_ab]; L.eyx = ef.Ey_x[xp + ox_ab];
+        L.exzm = ef.Ex_z[zp + oz_abm]; L.exz = ef.Ex_z[zp + oz_ab];
+        L.eyzm = ef.Ey_z[zp + oz_amb]; L.eyz = ef.Ey_z[zp + oz_ab];
+        L.ezxm = ef.Ez_x[xs0 + ox_abm]; L.ezx = ef.Ez_x[xs0 + ox_ab];
+        L.ezym = ef.Ez_y[ys0 + oy_amb]; L.ezy = ef.Ez_y[ys0 + oy_ab];
         L.lbx = L.lby = L.lbz = 0.f;
         if (comp) {  // own faces are in range: the maps are the identity there
             if (own_bx && s >= c0) L.lbx = B0.Bx_lo[xs0 + ox_ab];
@@ -645,8 +646,8 @@ __global__ __launch_bounds__((EctShape<TX, TY>::NT)) void k_ect3d(Geom3D g, EmfA
         // (2) edges: Ex, Ey of z-face s+1, Ez of plane s
         double ex_n = 0.0, ey_n = 0.0, ez = 0.0;
         if (has_edge) {
-            const double exi = 0.25 * (exy_o + cur.exy + cur.exzm + cur.exz);
-            const double eyi = 0.25 * (eyx_o + cur.eyx + cur.eyzm + cur.eyz);
+            const double exi = 0.25 * (exy_o + (double)cur.exy + (double)cur.exzm + (double)cur.exz);
+            const double eyi = 0.25 * (eyx_o + (double)cur.eyx + (double)cur.eyzm + (double)cur.eyz);
             double dBzdy = 0.0, dBydz = 0.0, dBxdz = 0.0, dBzdx = 0.0;
             if (eta > 0.0) {
                 dBzdy = (cbz_n - s_cbz[p - PX]) / g.dy;
@@ -657,7 +658,7 @@ __global__ __launch_bounds__((EctShape<TX, TY>::NT)) void k_ect3d(Geom3D g, EmfA
             ex_n = edge_sum(exi, eta, dBzdy, dBydz);
             ey_n = edge_sum(eyi, eta, dBxdz, dBzdx);
             if (s >= c0) {
-                const double ezi = 0.25 * (cur.ezxm + cur.ezx + cur.ezym + cur.ezy);
+                const double ezi = 0.25 * ((double)cur.ezxm + (double)cur.ezx + (double)cur.ezym + (double)cur.ezy);
                 double dBydx = 0.0, dBxdy = 0.0;
                 if (eta > 0.0) {
                     dBydx = (cby_o - s_cby[p - 1]) / g.dx;
@@ -700,7 +701,7 @@ __global__ __launch_bounds__((EctShape<TX, TY>::NT)) void k_ect3d(Geom3D g, EmfA
             if (comp) B1.Bz_lo[f] = lo;
         }
         cbx_o = cbx_n; cby_o = cby_n; bxa_o = cur.bxa; bya_o = cur.bya;
-        exy_o = cur.exy; eyx_o = cur.eyx; ex_c = ex_n; ey_c = ey_n;
+        exy_o = (double)cur.exy; eyx_o = (double)cur.eyx; ex_c = ex_n; ey_c = ey_n;
         cur = nxt;
     }
 }
@@ -963,7 +964,7 @@ struct cc_mhd3d {
     int nspd = 0;
     // edge EMFs + CT in one pass (k_ect3d, tile shape ect_shape); k_edge3d + k_ct3d otherwise
     bool fused = true;
-    int ect_shape = 1;
+    int ect_shape = -1;  // -1: 3 for the mixed path (fits two blocks per CU), 1 for FP64 (spills at that bound)
     int zchunk = 32;
     Profiler prof;
     FrameBuffer frame;
@@ -1147,20 +1148,22 @@ struct cc_mhd3d {
         return CC_OK;
     }
 
-    template <typename R, int TX, int TY>
+    template <typename R, int TX, int TY, int MINW = 1>
     cc_status launch_ect_shape(const State3D& in, const State3D& B1) {
         const dim3 grid((unsigned)(((p.Nx + TX - 1) / TX) * ((p.Ny + TY - 1) / TY)),
                         (unsigned)((p.Nz + zchunk - 1) / zchunk));
-        hipLaunchKernelGGL((k_ect3d<R, TX, TY>), grid, dim3(EctShape<TX, TY>::NT), 0, stream, g, EF<R>(), in, B1, sc,
+        hipLaunchKernelGGL((k_ect3d<R, TX, TY, MINW>), grid, dim3(EctShape<TX, TY>::NT), 0, stream, g, EF<R>(), in, B1, sc,
                            zchunk);
         CC_CHECK_LAUNCH();
         return CC_OK;
     }
     template <typename R>
     cc_status launch_ect(const State3D& in, const State3D& B1) {
-        switch (ect_shape) {
+        const int shape = ect_shape >= 0 ? ect_shape : (std::is_same<R, float>::value ? 3 : 1);
+        switch (shape) {
             case 0: return launch_ect_shape<R, 32, 28>(in, B1);
             case 2: return launch_ect_shape<R, 64, 5>(in, B1);
+            case 3: return launch_ect_shape<R, 32, 12, 4>(in, B1);
             default: return launch_ect_shape<R, 32, 12>(in, B1);
         }
     }

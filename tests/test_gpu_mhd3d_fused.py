@@ -2,7 +2,7 @@
 span several tiles and z chunks, against the CPU oracle (mhd3d_resistive_hlld_ct.cpp restated,
 M3:694-1066) and against the split passes k_edge3d + k_ct3d of the same library.
 
-Tiles are 32 x 28, 32 x 12 or 64 x 5 cells (CC_M3_ECT_SHAPE 0, 1, 2), so a 70 x 53 x 11 grid has partial
+Tiles are 32 x 28, 32 x 12 or 64 x 5 cells (CC_M3_ECT_SHAPE 0, 1 and 3, 2), so a 70 x 53 x 11 grid has partial
 tiles on both axes; CC_M3_ZCHUNK = 3 splits z into four chunks (and CC_M3_JCHUNK the column-marching Joule
 pass k_joule3d likewise), so chunk seams, the index maps at both z ends and the faces of index Nx, Ny, Nz are
 all exercised, with periodic and clamped boundaries.  FP64 bars allow
@@ -29,7 +29,7 @@ def _params(eos, bc, comp, precision=CC_FP64, eta=2e-3):
                     eta=eta, ct_compensated=comp, precision=precision)
 
 
-@pytest.mark.parametrize("shape,zchunk", [(0, 3), (0, 64), (1, 3), (2, 5)])
+@pytest.mark.parametrize("shape,zchunk", [(0, 3), (0, 64), (1, 3), (2, 5), (3, 4)])
 @pytest.mark.parametrize("eos,bc,comp", [(1, "periodic", False), (1, "periodic", True), (0, "clamped", False),
                                          (1, "mixed", True), (0, "periodic", True)])
 def test_fused_stage_and_step_match_oracle(monkeypatch, eos, bc, comp, shape, zchunk):
