@@ -287,7 +287,7 @@ def bench3d(N, warmup, steps, prof_steps, with_cpu):
     for prec, tag, R, mx in ((CC_FP64, "fp64", "double", "false"), (CC_MIXED, "mixed", "float", "true")):
         groups = {"prim": ["cc::k_prim3d<%s>" % R],
                   "face": ["cc::k_face3d<%d, %s," % (a, mx) for a in range(3)],
-                  "update": ["cc::k_update3d<%s>" % R], "ect": ["cc::k_ect3d<%s, 32, 12>" % R],
+                  "update": ["cc::k_update3d<%s>" % R], "ect": ["cc::k_ect3d<%s, 32, 12," % R],
                   "edge": ["cc::k_edge3d<%s>" % R], "ct": ["cc::k_ct3d"],
                   "joule": ["cc::k_joule3d<%s, true, true>" % R, "cc::k_joule3d<%s, false, true>" % R]}
         hbm = measured_hbm(prof[prec], groups)
