@@ -213,37 +213,16 @@ __device__ __forceinline__ Prim<R> load_prim(const PrimArrays<R>& W, size_t q) {
     return Prim<R>{W.c[0][q], W.c[1][q], W.c[2][q], W.c[3][q], W.c[4][q], W.c[5][q], W.c[6][q], W.c[7][q]};
 }
 
-// Mixed precision on FP32 face states: FP32 HLLD, recomputed in FP64 on the same states when the face
-// is within tau of the Alfven degeneracy.
-template <typename EOS32, typename EOS64>
-__device__ __forceinline__ void hlld_x_mixed32(const EOS32& e32, const EOS64& e64, const Prim<float>& L,
-                                               const Prim<float>& R, double Bn, float out[7], float tau,
-                                               double guard) {
-    const bool bn_small = fabs(Bn) < 1e-12;
-    bool deg = false;
-    hlld_x<float>(e32, L, R, (float)Bn, bn_small, out, &deg, tau, (float)guard);
-    if (deg) {
-        const Prim<double> l{L.rho, L.u, L.v, L.w, L.P, L.Bx, L.By, L.Bz};
-        const Prim<double> r{R.rho, R.u, R.v, R.w, R.P, R.Bx, R.By, R.Bz};
-        double o[7];
-        hlld_x<double>(e64, l, r, Bn, bn_small, o, nullptr, 0.0, guard);
-#pragma unroll
-        for (int c = 0; c < 7; ++c) out[c] = (float)o[c];
-    }
-}
-
 // The face solver of the flux path on FP64 face states (for the batched entry point): FP64 HLLD, or
-// the mixed path's FP32 states and FP32 HLLD with its FP64 fallback.
+// the mixed path's FP32 HLLD on the rounded states with its FP64 fallback on the FP64 states, the result
+// rounded to FP32 as the mixed step stores its fluxes.
 template <bool MIXED, typename EOS64, typename EOS32>
 __device__ __forceinline__ void face_hlld3d(const EOS64& e64, const EOS32& e32, const Prim<double>& WL,
                                             const Prim<double>& WR, double Bn, double F[7], float tau, double guard) {
     if constexpr (MIXED) {
-        const Prim<float> l{(float)WL.rho, (float)WL.u, (float)WL.v, (float)WL.w, (float)WL.P, (float)WL.Bx, (float)WL.By, (float)WL.Bz};
-        const Prim<float> r{(float)WR.rho, (float)WR.u, (float)WR.v, (float)WR.w, (float)WR.P, (float)WR.Bx, (float)WR.By, (float)WR.Bz};
-        float o[7];
-        hlld_x_mixed32(e32, e64, l, r, Bn, o, tau, guard);
+        hlld_x_mixed(e32, e64, WL, WR, Bn, F, tau, guard);
 #pragma unroll
-        for (int c = 0; c < 7; ++c) F[c] = o[c];
+        for (int c = 0; c < 7; ++c) F[c] = (float)F[c];
     } else {
         hlld_x<double>(e64, WL, WR, Bn, fabs(Bn) < 1e-12, F, nullptr, 0.0, guard);
     }
@@ -251,21 +230,19 @@ __device__ __forceinline__ void face_hlld3d(const EOS64& e64, const EOS32& e32, 
 
 // PLM-MC on the four cells f-2 .. f+1 of face (i, j, k) of axis AX (boundary-mapped) and the face
 // states in the axis frame of hlld_x (y: the (u,v)/(Bx,By) swap of M3:577-595, z: the (u,v,w)->(w,u,v)
-// map of M3:604-622).
-template <int AX, typename R>
-__device__ __forceinline__ void face_states3d(const Geom3D& g, const PrimArrays<R>& W, double Bn, int i, int j, int k,
-                                              Prim<R>& L, Prim<R>& Rr) {
-    const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+// map of M3:604-622).  cell_prim(i, j, k) gives the primitives of an in-range cell.
+template <int AX, typename R, typename CellPrim>
+__device__ __forceinline__ void face_states3d_of(const Geom3D& g, const CellPrim& cell_prim, double Bn, int i, int j,
+                                                 int k, Prim<R>& L, Prim<R>& Rr) {
     const int f = AX == 0 ? i : (AX == 1 ? j : k);
-    auto cell = [&](int o) {  // index of the cell o steps from the face along the axis
+    auto cell = [&](int o) {  // the cell o steps from the face along the axis
         int ii = i, jj = j, kk = k;
-        if (AX == 0) ii = bmap(f + o, Nx, g.px);
-        if (AX == 1) jj = bmap(f + o, Ny, g.py);
-        if (AX == 2) kk = bmap(f + o, Nz, g.pz);
-        return ((size_t)kk * Ny + jj) * Nx + ii;
+        if (AX == 0) ii = bmap(f + o, g.Nx, g.px);
+        if (AX == 1) jj = bmap(f + o, g.Ny, g.py);
+        if (AX == 2) kk = bmap(f + o, g.Nz, g.pz);
+        return cell_prim(ii, jj, kk);
     };
-    const Prim<R> w0 = load_prim(W, cell(-2)), w1 = load_prim(W, cell(-1)), w2 = load_prim(W, cell(0)),
-                  w3 = load_prim(W, cell(1));
+    const Prim<R> w0 = cell(-2), w1 = cell(-1), w2 = cell(0), w3 = cell(1);
     const R frho = (R)g.floor_rho, fP = (R)g.floor_P;
     Prim<R> a, b, WL, WR;
     plm_mc<R>(w0, w1, w2, AX, frho, fP, a, WL);  // right state of cell f-1
@@ -279,6 +256,27 @@ __device__ __forceinline__ void face_states3d(const Geom3D& g, const PrimArrays<
         L = Prim<R>{WL.rho, WL.w, WL.u, WL.v, WL.P, (R)Bn, WL.Bx, WL.By};
         Rr = Prim<R>{WR.rho, WR.w, WR.u, WR.v, WR.P, (R)Bn, WR.Bx, WR.By};
     }
+}
+
+// Face states from the stored primitives of the stage.
+template <int AX, typename R>
+__device__ __forceinline__ void face_states3d(const Geom3D& g, const PrimArrays<R>& W, double Bn, int i, int j, int k,
+                                              Prim<R>& L, Prim<R>& Rr) {
+    face_states3d_of<AX, R>(
+        g, [&](int ii, int jj, int kk) { return load_prim(W, ((size_t)kk * g.Ny + jj) * g.Nx + ii); }, Bn, i, j, k, L,
+        Rr);
+}
+
+// The mixed path's FP64 solve of a face its FP32 HLLD flagged: the four cells' primitives recovered in FP64
+// from the conserved state U of the stage, PLM-MC and HLLD in FP64, i.e. the FP64 path's flux of that face.
+template <int AX, typename EOS64>
+__device__ __forceinline__ void face_hlld_fp64_from_cons(const Geom3D& g, const Eos3D& e, const EOS64& e64,
+                                                         const State3D& U, double Bn, int i, int j, int k,
+                                                         double o[7]) {
+    Prim<double> L, Rr;
+    face_states3d_of<AX, double>(
+        g, [&](int ii, int jj, int kk) { return prim_at(g, e, U, ii, jj, kk); }, Bn, i, j, k, L, Rr);
+    hlld_x<double>(e64, L, Rr, Bn, fabs(Bn) < 1e-12, o, nullptr, 0.0, g.hlld_guard);
 }
 
 // The five fluxes in reference order and the two face EMFs of face t (Ey_x = F(Bz), Ez_x = -F(By);
@@ -309,17 +307,17 @@ struct DegQueue {
 // One face of axis AX per thread (x: M3:727-750, y: M3:752-775, z: M3:777-800): PLM-MC, HLLD, the
 // fluxes and EMFs.  The mixed form runs the FP32 HLLD only and queues the faces it flags (near the
 // Alfven degeneracy, or with the degeneracy guard on, near the guard's threshold); k_face3d_fp64 then
-// recomputes those in FP64 (the FP64 solver's registers would otherwise halve this kernel's occupancy).
-// Without the guard the field along z of the turbulence benchmark keeps most z faces near the
-// degeneracy: INLINE z faces then recompute inline (a queue of most faces would cost more than the
-// occupancy gains).
+// recomputes those in FP64 from the conserved state U (the FP64 solver's registers would otherwise halve
+// this kernel's occupancy).  Without the guard a field along z (the turbulence problem's) keeps most z
+// faces near the degeneracy: INLINE z faces then recompute inline (a queue of most faces would cost more
+// than the occupancy gains).
 template <int AX, bool MIXED, typename EOS64, typename EOS32, bool INLINE = false>
 __global__ __launch_bounds__(256) void k_face3d(Geom3D g, EOS64 e64, EOS32 e32,
                                                 PrimArrays<typename std::conditional<MIXED, float, double>::type> W,
                                                 const double* __restrict__ Bface,
                                                 FaceFlux<typename std::conditional<MIXED, float, double>::type> F,
                                                 EmfArr<typename std::conditional<MIXED, float, double>::type> ef,
-                                                DegQueue dq) {
+                                                DegQueue dq, Eos3D e, State3D U) {
     using R = typename std::conditional<MIXED, float, double>::type;
     const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
     const int nx = Nx + (AX == 0), ny = Ny + (AX == 1);
@@ -347,7 +345,14 @@ __global__ __launch_bounds__(256) void k_face3d(Geom3D g, EOS64 e64, EOS32 e32,
         hlld_x<float>(e32, L, Rr, (float)Bn, fabs(Bn) < 1e-12, o, &deg, g.tau, (float)g.hlld_guard);
         if (deg) dq.face[AX][atomicAdd(&dq.count[AX], 1)] = (int)t;
     } else if constexpr (MIXED) {  // recomputed inline
-        hlld_x_mixed32(e32, e64, L, Rr, Bn, o, g.tau, g.hlld_guard);
+        bool deg = false;
+        hlld_x<float>(e32, L, Rr, (float)Bn, fabs(Bn) < 1e-12, o, &deg, g.tau, (float)g.hlld_guard);
+        if (deg) {
+            double o64[7];
+            face_hlld_fp64_from_cons<AX>(g, e, e64, U, Bn, i, j, k, o64);
+#pragma unroll
+            for (int c = 0; c < 7; ++c) o[c] = (float)o64[c];
+        }
     } else {
         hlld_x<double>(e64, L, Rr, Bn, fabs(Bn) < 1e-12, o, nullptr, 0.0, g.hlld_guard);
     }
@@ -355,36 +360,30 @@ __global__ __launch_bounds__(256) void k_face3d(Geom3D g, EOS64 e64, EOS32 e32,
 }
 
 template <int AX, typename EOS64>
-__device__ __forceinline__ void face_fp64_one(const Geom3D& g, const EOS64& e64, const PrimArrays<float>& W,
+__device__ __forceinline__ void face_fp64_one(const Geom3D& g, const Eos3D& e, const EOS64& e64, const State3D& U,
                                               const double* __restrict__ Bface, FaceFlux<float>& F, EmfArr<float>& ef,
                                               int tf) {
     const int nx = g.Nx + (AX == 0), ny = g.Ny + (AX == 1);
     const int i = tf % nx, r = tf / nx, j = r % ny, k = r / ny;
-    const size_t t = (size_t)tf;
-    const double Bn = Bface[t];
-    Prim<float> L, Rr;
-    face_states3d<AX, float>(g, W, Bn, i, j, k, L, Rr);
-    const Prim<double> l{L.rho, L.u, L.v, L.w, L.P, L.Bx, L.By, L.Bz};
-    const Prim<double> rr{Rr.rho, Rr.u, Rr.v, Rr.w, Rr.P, Rr.Bx, Rr.By, Rr.Bz};
     double o64[7];
-    hlld_x<double>(e64, l, rr, Bn, fabs(Bn) < 1e-12, o64, nullptr, 0.0, g.hlld_guard);
+    face_hlld_fp64_from_cons<AX>(g, e, e64, U, Bface[tf], i, j, k, o64);
     float o[7];
 #pragma unroll
     for (int c = 0; c < 7; ++c) o[c] = (float)o64[c];
-    face_store3d<AX, float>(F, ef, t, o);
+    face_store3d<AX, float>(F, ef, (size_t)tf, o);
 }
 
-// The FP64 HLLD of the queued faces of all three axes (the mixed path's fallback of hlld_x_mixed32).
+// The FP64 faces of the mixed path: the queued faces of all three axes, each solved from the stage's
+// conserved state U (the FP64 path's flux of the face, rounded to FP32).
 template <typename EOS64>
-__global__ __launch_bounds__(256) void k_face3d_fp64(Geom3D g, EOS64 e64, PrimArrays<float> W, const double* __restrict__ Bx,
-                                                     const double* __restrict__ By, const double* __restrict__ Bz,
-                                                     FaceFlux<float> F0, FaceFlux<float> F1, FaceFlux<float> F2,
-                                                     EmfArr<float> ef, DegQueue dq) {
+__global__ __launch_bounds__(256) void k_face3d_fp64(Geom3D g, Eos3D e, EOS64 e64, State3D U, FaceFlux<float> F0,
+                                                     FaceFlux<float> F1, FaceFlux<float> F2, EmfArr<float> ef,
+                                                     DegQueue dq) {
     const int n0 = dq.count[0], n1 = dq.count[1], n2 = dq.count[2];
     for (int q = blockIdx.x * 256 + (int)threadIdx.x; q < n0 + n1 + n2; q += gridDim.x * 256) {
-        if (q < n0) face_fp64_one<0>(g, e64, W, Bx, F0, ef, dq.face[0][q]);
-        else if (q < n0 + n1) face_fp64_one<1>(g, e64, W, By, F1, ef, dq.face[1][q - n0]);
-        else face_fp64_one<2>(g, e64, W, Bz, F2, ef, dq.face[2][q - n0 - n1]);
+        if (q < n0) face_fp64_one<0>(g, e, e64, U, U.Bx, F0, ef, dq.face[0][q]);
+        else if (q < n0 + n1) face_fp64_one<1>(g, e, e64, U, U.By, F1, ef, dq.face[1][q - n0]);
+        else face_fp64_one<2>(g, e, e64, U, U.Bz, F2, ef, dq.face[2][q - n0 - n1]);
     }
 }
 
@@ -1065,30 +1064,30 @@ struct cc_mhd3d {
         {
             ProfScope ps(pr(), stream, "face");
             hipLaunchKernelGGL((k_face3d<0, MIXED, EOS64, EOS32>), dim3(blocks(NFx())), dim3(256), 0, stream, g, e64, e32,
-                               W<R>(), in.Bx, FF<R>(0), EF<R>(), dq);
+                               W<R>(), in.Bx, FF<R>(0), EF<R>(), dq, e, in);
             CC_CHECK_LAUNCH();
         }
         {
             ProfScope ps(pr(), stream, "face");
             hipLaunchKernelGGL((k_face3d<1, MIXED, EOS64, EOS32>), dim3((p.Nx + 63) / 64, p.Nz, (p.Ny + 1 + 3) / 4),
                                dim3(256), 0, stream, g, e64, e32,
-                               W<R>(), in.By, FF<R>(1), EF<R>(), dq);
+                               W<R>(), in.By, FF<R>(1), EF<R>(), dq, e, in);
             CC_CHECK_LAUNCH();
         }
         {
             ProfScope ps(pr(), stream, "face");
             if (MIXED && p.hlld_guard <= 0.0)
                 hipLaunchKernelGGL((k_face3d<2, MIXED, EOS64, EOS32, true>), dim3((p.Nx + 63) / 64, p.Ny, (p.Nz + 1 + 3) / 4),
-                                   dim3(256), 0, stream, g, e64, e32, W<R>(), in.Bz, FF<R>(2), EF<R>(), dq);
+                                   dim3(256), 0, stream, g, e64, e32, W<R>(), in.Bz, FF<R>(2), EF<R>(), dq, e, in);
             else
                 hipLaunchKernelGGL((k_face3d<2, MIXED, EOS64, EOS32>), dim3((p.Nx + 63) / 64, p.Ny, (p.Nz + 1 + 3) / 4),
-                                   dim3(256), 0, stream, g, e64, e32, W<R>(), in.Bz, FF<R>(2), EF<R>(), dq);
+                                   dim3(256), 0, stream, g, e64, e32, W<R>(), in.Bz, FF<R>(2), EF<R>(), dq, e, in);
             CC_CHECK_LAUNCH();
         }
         if constexpr (MIXED) {
             ProfScope ps(pr(), stream, "face_fp64");
-            hipLaunchKernelGGL((k_face3d_fp64<EOS64>), dim3(2048), dim3(256), 0, stream, g, e64, W<float>(), in.Bx, in.By,
-                               in.Bz, FF<float>(0), FF<float>(1), FF<float>(2), EF<float>(), dq);
+            hipLaunchKernelGGL((k_face3d_fp64<EOS64>), dim3(2048), dim3(256), 0, stream, g, e, e64, in, FF<float>(0),
+                               FF<float>(1), FF<float>(2), EF<float>(), dq);
             CC_CHECK_LAUNCH();
         }
         {

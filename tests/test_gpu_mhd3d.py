@@ -66,6 +66,32 @@ def test_advance_stage3d_matches_oracle(eos, periodic):
         assert rel_err(got[f], want[f]) < 1e-11, (f, rel_err(got[f], want[f]))
 
 
+@pytest.mark.parametrize("eos,periodic", [(0, True), (0, False), (1, True), (1, False)])
+def test_mixed_stage_with_every_subsonic_face_on_the_fp64_fallback(eos, periodic):
+    """With degen_tau = 2 the mixed path's FP32 HLLD hands every subsonic face with |Bn| >= 1e-12 (nearly all
+    faces of this smooth state) to its FP64 pass, which recovers the four cells' primitives from the FP64
+    conserved state and runs PLM and HLLD in FP64: the stage then differs from the FP64 oracle stage by the
+    FP32 rounding of the stored fluxes times dt/dx (~1e-9 here).  A cell index off by one in that pass, at
+    the interior or at a mapped boundary, moves a flux by ~1e-2 and the stage by ~1e-4."""
+    p = M.Params(Nx=12, Ny=10, Nz=9, eos=eos, periodic_x=periodic, periodic_y=periodic, periodic_z=periodic,
+                 eta=2e-3, precision=CC_MIXED, degen_tau=2.0)
+    st = random_mhd3d_state(p, 5)
+    dt = 1e-3
+    with M.Mhd3D(p) as h:
+        h.upload(st)
+        h.advance_stage(dt)
+        got = h.download()
+    S0 = m3_arrays(p, st)
+    S1 = S0.copy()
+    oracle().or_m3_advance_stage(C.byref(m3_oparams(p)), C.byref(S0.cstate()), C.byref(S1.cstate()),
+                                 C.c_double(p.dx), C.c_double(p.dy), C.c_double(p.dz), C.c_double(dt))
+    want = m3_dict(p, S1)
+    err = {f: rel_err(got[f], want[f]) for f in M.FIELDS}
+    print("mixed stage, FP64 fallback on every subsonic face, vs oracle:", {f: "%.1e" % e for f, e in err.items()})
+    for f in M.FIELDS:
+        assert err[f] < 1e-7, (f, err[f])
+
+
 @pytest.mark.parametrize("eos", [0, 1])
 def test_rk2_step3d_matches_oracle(eos):
     p = M.Params(Nx=16, Ny=12, Nz=10, eos=eos, eta=1e-3)
@@ -115,8 +141,8 @@ def _turb_state(p, tilted):
 
 @pytest.mark.parametrize("tilted,guard", [(True, 1e-2), (False, 1e-2), (True, 0.0), (False, 0.0)])
 def test_turbulence_run_matches_oracle_and_divb(tilted, guard):
-    """Turbulence IC at 16^3, 5 steps of the driver loop, compensated face update.  With the benchmark's
-    HLLD guard (1e-2) the run is well-conditioned and the GPU reproduces the oracle to 1e-10 in every field,
+    """Turbulence IC at 16^3, 5 steps of the driver loop, compensated face update.  With the HLLD guard on
+    (1e-2) the run is well-conditioned and the GPU reproduces the oracle to 1e-10 in every field,
     with the field along z (the benchmark's own IC) or tilted.  With the reference's HLLD (guard 0) and the
     field along z only, z faces sit at the star-state degeneracy and x/y faces at Bn = 0 (the HLL branch of
     M3:491), so rounding decides a few branch choices: the bar there is relative L1 <= 1e-7 on rho and E;
@@ -163,11 +189,13 @@ def test_mixed_vs_fp64_turbulence_tilted():
 
 
 def test_mixed_vs_fp64_on_the_turbulence_benchmark_ic():
-    """The 3D benchmark's own IC (beta 1 field along z) at 32^3 for 20 steps with its HLLD guard: the FP64
-    run stays finite and smooth (max E < 10), and mixed agrees with FP64 to the 1e-4 relative L1 bar in
-    every field (measured: 2e-6 for rho, <= 2e-5 for every field).  The largest pointwise difference,
-    relative to each field's maximum, is held below 5e-2 (measured 1.5e-2 in the momenta): a face whose
-    star-state denominator lies at the guard threshold takes HLL in one precision and HLLD in the other."""
+    """The 3D benchmark's own IC (beta 1 field along z) at 32^3 for 20 steps with its HLLD guard (5e-2): the
+    FP64 run stays finite and smooth (max E < 10), and mixed agrees with FP64 to the 1e-4 relative L1 bar in
+    every field (measured: 1.1e-7 for rho, <= 5.1e-7 for every field).  The largest pointwise difference,
+    relative to each field's maximum, is held below 5e-2 (measured 1.1e-4): a few faces of this run lie
+    within ~1e-6 of the guard's threshold, so a state difference of 1e-7 (mixed against FP64) decides HLL or
+    HLLD there; FP64 runs from this IC with 1e-7 random noise on rho, m and E show the same jumps (2.0e-4 at
+    step 6 in 4 of 8 seeds, 4.9e-4 at step 10 in 2 of 8)."""
     out = {}
     for prec in (CC_FP64, CC_MIXED):
         p = ic.turbulence3d_params(32, precision=prec)
@@ -188,25 +216,24 @@ def test_mixed_vs_fp64_on_the_turbulence_benchmark_ic():
     assert out[CC_MIXED][1] < 1e-12 and out[CC_FP64][1] < 1e-12
 
 
-def test_mixed_fallback_tau_two_is_fp64_hlld_on_fp32_states():
-    """The mixed path carries FP32 face states and stores FP32 fluxes; with degen_tau >= 1 every
-    subsonic face takes the FP64 fallback, whose stored result must then be the FP64 HLLD of the
-    FP32-rounded states, rounded to FP32.  Supersonic and |Bn| < 1e-12 faces stay on the FP32 solver
-    (a third of these random faces), so the exact bar is on the median face and the rest are held to
-    FP32 accuracy."""
+def test_mixed_fallback_tau_two_is_fp64_hlld_of_the_fp64_states():
+    """The mixed path solves faces in FP32 on FP32-rounded states and stores FP32 fluxes; with degen_tau >= 1
+    every subsonic face takes the FP64 fallback, which solves the unrounded FP64 states, so the stored result
+    must be the FP64 HLLD of the given states rounded to FP32.  Supersonic and |Bn| < 1e-12 faces stay on the
+    FP32 solver (a third of these random faces), so the exact bar is on the median face and the rest are held
+    to FP32 accuracy."""
     rng = np.random.default_rng(77)
     n = 2000
     WL, WR = _prims(rng, n, False), _prims(rng, n, False)
     Bn = rng.normal(0, 1.0, n)
     p = M.Params(eos=1, precision=CC_MIXED, degen_tau=2.0)
     got = M.hlld_batch(p, WL, WR, Bn)
-    WLr, WRr = WL.astype(np.float32).astype(np.float64), WR.astype(np.float32).astype(np.float64)
     o = oracle()
     op = m3_oparams(M.Params(eos=1))
     want = np.zeros((n, 7))
     for k in range(n):
         r = np.zeros(7)
-        o.or_m3_hlld_x(C.byref(op), OB.ptr(np.ascontiguousarray(WLr[k])), OB.ptr(np.ascontiguousarray(WRr[k])),
+        o.or_m3_hlld_x(C.byref(op), OB.ptr(np.ascontiguousarray(WL[k])), OB.ptr(np.ascontiguousarray(WR[k])),
                        C.c_double(Bn[k]), OB.ptr(r))
         want[k] = r
     want = want.astype(np.float32).astype(np.float64)
