@@ -66,6 +66,21 @@ __host__ __device__ __forceinline__ int bmap(int i, int n, bool periodic) {
 }
 
 // Smallest positive value used where the reference writes 1e-300 (M2:155, M2:444); it underflows in FP32.
+// a / b in the arithmetic type R.  FP64 divides exactly (the reference's arithmetic).  FP32 in device code
+// multiplies by the hardware reciprocal (v_rcp_f32, 1 ulp), about a tenth of the instructions of a correctly
+// rounded FP32 quotient; the FP32 solvers only produce FP32-rounded fluxes whose ill-conditioned cases go to
+// an FP64 solver, so the extra ulp is immaterial.
+template <typename R>
+__host__ __device__ __forceinline__ R qdiv(R a, R b) { return a / b; }
+template <>
+__host__ __device__ __forceinline__ float qdiv<float>(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return a * __builtin_amdgcn_rcpf(b);
+#else
+    return a / b;
+#endif
+}
+
 template <typename R> struct Tiny;
 template <> struct Tiny<double> { static __host__ __device__ __forceinline__ double v() { return 1e-300; } };
 template <> struct Tiny<float> { static __host__ __device__ __forceinline__ float v() { return 1e-30f; } };

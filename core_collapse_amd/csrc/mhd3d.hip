@@ -3,20 +3,21 @@
 // cold + thermal + bounce EOS or an ideal gas.
 //
 // One forward-Euler stage (advance_stage, M3:694-1030) is:
-//   k_face3d    x, y and z faces, one face per thread: PLM-MC + HLLD -> 5 face fluxes, 2 face EMFs
+//   k_face3d    x, y and (mixed) z faces, one face per thread: PLM-MC + HLLD -> 5 face fluxes, 2 face EMFs
 //               (from the primitives in the flux precision, left by the previous stage's k_joule3d or,
 //               at the start of a call, by k_prim3d)
-//   k_update3d  flux difference + floors (old faces), FP64                      -> U*
+//   k_update3d  (mixed) flux difference + floors (old faces), FP64               -> U*
+//   k_zupd3d    (FP64) z faces + flux difference + floors in one column march    -> U*
 //   k_ect3d     edge EMFs (1/4 sum of face EMFs + eta * curl of cell-centred B0) and the face-field
 //               induction, in one z-marching pass over LDS tiles              -> B1
 //   k_joule3d   Joule heating from cell-centred B1 differences + floors, marching along z per column;
 //               the stage-2 instance also forms the Heun average, its floors and the next step's
 //               signal-speed maximum; both store the primitives of their result for the next stage.
 // k_edge3d + k_ct3d (edge arrays in memory) replace k_ect3d when CC_M3_FUSED=0.
-// Mixed precision: primitives recovered in FP64 and stored in FP32; PLM and HLLD in FP32 (HLLD
-// recomputed in FP64 for faces near the Alfven degeneracy, where the reference's star states are
-// ill-conditioned); face fluxes and EMFs stored in FP32; flux differences, the conserved update,
-// EMF averages and CT in FP64.
+// Mixed precision: primitives recovered in FP64 and stored in FP32; PLM and HLLD in FP32 (faces near
+// the Alfven degeneracy, where the reference's star states are ill-conditioned, or near the guard's
+// threshold are queued and solved again in FP64 from the conserved state by k_face3d_fp64); face fluxes
+// and EMFs stored in FP32; flux differences, the conserved update, EMF averages and CT in FP64.
 #include <algorithm>
 #include <cstdlib>
 #include <cmath>
@@ -279,30 +280,91 @@ __device__ __forceinline__ void face_hlld_fp64_from_cons(const Geom3D& g, const 
     hlld_x<double>(e64, L, Rr, Bn, fabs(Bn) < 1e-12, o, nullptr, 0.0, g.hlld_guard);
 }
 
-// The five fluxes in reference order and the two face EMFs of face t (Ey_x = F(Bz), Ez_x = -F(By);
-// Ez_y = G(Bx), Ex_y = -G(Bz); Ex_z = H(By), Ey_z = -H(Bx)).
+// The five fluxes of a face of axis AX in the conserved order (rho, mx, my, mz, E) from hlld_x's output in
+// the axis frame.
 template <int AX, typename R>
-__device__ __forceinline__ void face_store3d(FaceFlux<R>& F, EmfArr<R>& ef, size_t t, const R o[7]) {
+__device__ __forceinline__ void face_fluxes3d(const R o[7], R f[5]) {
+    f[0] = o[0];
+    f[4] = o[4];
+    if (AX == 0) { f[1] = o[1]; f[2] = o[2]; f[3] = o[3]; }
+    else if (AX == 1) { f[1] = o[2]; f[2] = o[1]; f[3] = o[3]; }
+    else { f[1] = o[2]; f[2] = o[3]; f[3] = o[1]; }
+}
+
+// The two face EMFs of face t (Ey_x = F(Bz), Ez_x = -F(By); Ez_y = G(Bx), Ex_y = -G(Bz); Ex_z = H(By),
+// Ey_z = -H(Bx)).
+template <int AX, typename R>
+__device__ __forceinline__ void face_emfs3d(EmfArr<R>& ef, size_t t, const R o[7]) {
     if (AX == 0) {
-        F.f[0][t] = o[0]; F.f[1][t] = o[1]; F.f[2][t] = o[2]; F.f[3][t] = o[3]; F.f[4][t] = o[4];
         ef.Ey_x[t] = o[6];
         ef.Ez_x[t] = -o[5];
     } else if (AX == 1) {
-        F.f[0][t] = o[0]; F.f[1][t] = o[2]; F.f[2][t] = o[1]; F.f[3][t] = o[3]; F.f[4][t] = o[4];
         ef.Ez_y[t] = o[5];
         ef.Ex_y[t] = -o[6];
     } else {
-        F.f[0][t] = o[0]; F.f[1][t] = o[2]; F.f[2][t] = o[3]; F.f[3][t] = o[1]; F.f[4][t] = o[4];
         ef.Ex_z[t] = o[6];
         ef.Ey_z[t] = -o[5];
     }
 }
 
-// Faces whose FP32 HLLD flagged the Alfven degeneracy, per axis, for the FP64 recomputation pass.
+// The five fluxes in reference order and the two face EMFs of face t.
+template <int AX, typename R>
+__device__ __forceinline__ void face_store3d(FaceFlux<R>& F, EmfArr<R>& ef, size_t t, const R o[7]) {
+    R f[5];
+    face_fluxes3d<AX, R>(o, f);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) F.f[c][t] = f[c];
+    face_emfs3d<AX, R>(ef, t, o);
+}
+
+// Faces whose FP32 HLLD flagged the Alfven degeneracy, per axis, for the FP64 recomputation pass, and (z faces +
+// update in one pass) the cells with such a z face, whose update waits for the FP64 flux.  Queue q (faces of axis
+// 0, 1, 2; cells) is split into kQSub sub-queues: sub-queue k takes the indices [k span[q], (k+1) span[q]) into
+// the slots of the same range, so its capacity is exact, and has its own counter on its own cache line.  One
+// counter serialises: in a turbulence run ~70 % of the z-face waves can hold a flagged face, and their returning
+// atomics on one address took the z-face kernel from 0.27 to 1.08 ms at 256^3.
+constexpr int kQSub = 256;    // sub-queues per queue
+constexpr int kQStride = 32;  // ints between counters (128 bytes)
 struct DegQueue {
-    int* count;   // [3]
-    int* face[3]; // face indices, capacity = the axis' face count
+    int* count;    // counter of sub-queue k of queue q at count[(q * kQSub + k) * kQStride]
+    int* face[3];  // face indices per axis (capacity: the axis' face count)
+    int* cell;     // cell indices (capacity: the cell count)
+    int span[4];   // indices per sub-queue: ceil(n / kQSub)
 };
+
+// Appends v to queue q for the lanes of the wave where `take` holds.  The lanes whose v falls in the first
+// taking lane's sub-queue share one atomic (their slots contiguous in lane order); the others (a wave straddling
+// a sub-queue boundary) take one each.  Every active lane of the wave must call it (uniform control flow).
+__device__ __forceinline__ void queue_push(const DegQueue& dq, int q, bool take, int v) {
+    const unsigned long long mask = __ballot(take);
+    if (mask == 0ull) return;
+    int* arr = q < 3 ? dq.face[q] : dq.cell;
+    const int span = dq.span[q];
+    const int k = v / span;
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    const int kl = __shfl(k, leader, 64);
+    const bool same = take && k == kl;
+    const unsigned long long ms = __ballot(same);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&dq.count[(q * kQSub + kl) * kQStride], __popcll(ms));
+    base = __shfl(base, leader, 64);
+    if (same) arr[kl * span + base + __popcll(ms & ((1ull << lane) - 1ull))] = v;
+    else if (take) arr[k * span + atomicAdd(&dq.count[(q * kQSub + k) * kQStride], 1)] = v;
+}
+
+// Visits the entries of queue q with the blocks of a grid-stride launch: block b takes sub-queue b mod kQSub,
+// the blocks with that remainder share its entries.
+template <typename F>
+__device__ __forceinline__ void queue_for_each(const DegQueue& dq, int q, F&& f) {
+    const int* arr = q < 3 ? dq.face[q] : dq.cell;
+    const int parts = max(1, (int)gridDim.x / kQSub);
+    for (int k = (int)blockIdx.x % kQSub; k < kQSub; k += (int)min(gridDim.x, (unsigned)kQSub)) {
+        const int n = dq.count[(q * kQSub + k) * kQStride];
+        for (int e = ((int)blockIdx.x / kQSub) * (int)blockDim.x + (int)threadIdx.x; e < n; e += parts * (int)blockDim.x)
+            f(arr[k * dq.span[q] + e]);
+    }
+}
 
 // One face of axis AX per thread (x: M3:727-750, y: M3:752-775, z: M3:777-800): PLM-MC, HLLD, the
 // fluxes and EMFs.  The mixed form runs the FP32 HLLD only and queues the faces it flags (near the
@@ -343,7 +405,7 @@ __global__ __launch_bounds__(256) void k_face3d(Geom3D g, EOS64 e64, EOS32 e32,
     if constexpr (MIXED && !INLINE) {
         bool deg = false;
         hlld_x<float>(e32, L, Rr, (float)Bn, fabs(Bn) < 1e-12, o, &deg, g.tau, (float)g.hlld_guard);
-        if (deg) dq.face[AX][atomicAdd(&dq.count[AX], 1)] = (int)t;
+        queue_push(dq, AX, deg, (int)t);
     } else if constexpr (MIXED) {  // recomputed inline
         bool deg = false;
         hlld_x<float>(e32, L, Rr, (float)Bn, fabs(Bn) < 1e-12, o, &deg, g.tau, (float)g.hlld_guard);
@@ -373,22 +435,56 @@ __device__ __forceinline__ void face_fp64_one(const Geom3D& g, const Eos3D& e, c
     face_store3d<AX, float>(F, ef, (size_t)tf, o);
 }
 
-// The FP64 faces of the mixed path: the queued faces of all three axes, each solved from the stage's
-// conserved state U (the FP64 path's flux of the face, rounded to FP32).
+// The FP64 faces of the mixed path: the queued faces of the axes in the mask `axes` (bit a: axis a), each
+// solved from the stage's conserved state U (the FP64 path's flux of the face, rounded to FP32).
 template <typename EOS64>
 __global__ __launch_bounds__(256) void k_face3d_fp64(Geom3D g, Eos3D e, EOS64 e64, State3D U, FaceFlux<float> F0,
                                                      FaceFlux<float> F1, FaceFlux<float> F2, EmfArr<float> ef,
-                                                     DegQueue dq) {
-    const int n0 = dq.count[0], n1 = dq.count[1], n2 = dq.count[2];
-    for (int q = blockIdx.x * 256 + (int)threadIdx.x; q < n0 + n1 + n2; q += gridDim.x * 256) {
-        if (q < n0) face_fp64_one<0>(g, e, e64, U, U.Bx, F0, ef, dq.face[0][q]);
-        else if (q < n0 + n1) face_fp64_one<1>(g, e, e64, U, U.By, F1, ef, dq.face[1][q - n0]);
-        else face_fp64_one<2>(g, e, e64, U, U.Bz, F2, ef, dq.face[2][q - n0 - n1]);
-    }
+                                                     DegQueue dq, int axes) {
+    if (axes & 1) queue_for_each(dq, 0, [&](int tf) { face_fp64_one<0>(g, e, e64, U, U.Bx, F0, ef, tf); });
+    if (axes & 2) queue_for_each(dq, 1, [&](int tf) { face_fp64_one<1>(g, e, e64, U, U.By, F1, ef, tf); });
+    if (axes & 4) queue_for_each(dq, 2, [&](int tf) { face_fp64_one<2>(g, e, e64, U, U.Bz, F2, ef, tf); });
 }
 
-__global__ void k_zero_count3(int* __restrict__ c) {
-    if (threadIdx.x < 3) c[threadIdx.x] = 0;
+__global__ void k_zero_counts(int* __restrict__ c) {  // the 4 x kQSub counters
+    for (int k = (int)threadIdx.x; k < 4 * kQSub; k += (int)blockDim.x) c[k * kQStride] = 0;
+}
+
+// U0 + dt * (flux difference) and the floors with the old faces' cell-centred field (M3:802-834), in FP64.
+// flux(ax, side, c) is component c (conserved order) of the flux through the cell's lower (side 0) or
+// upper (side 1) face of axis ax.
+template <typename FluxAt>
+__device__ __forceinline__ void cell_update3d(const Geom3D& g, const Eos3D& e, double dt, const double u0[5],
+                                              const FluxAt& flux, double bx, double by, double bz, double u[5]) {
+    const double invdx = 1.0 / g.dx, invdy = 1.0 / g.dy, invdz = 1.0 / g.dz;
+    double d[5];
+#pragma unroll
+    for (int c = 0; c < 5; c++)
+        d[c] = -(flux(0, 1, c) - flux(0, 0, c)) * invdx - (flux(1, 1, c) - flux(1, 0, c)) * invdy -
+               (flux(2, 1, c) - flux(2, 0, c)) * invdz;
+    double rho = u0[0] + dt * d[0], mx = u0[1] + dt * d[1], my = u0[2] + dt * d[2];
+    double mz = u0[3] + dt * d[3], E = u0[4] + dt * d[4];
+    floors3d(e, rho, mx, my, mz, E, bx, by, bz);
+    u[0] = rho; u[1] = mx; u[2] = my; u[3] = mz; u[4] = E;
+}
+
+// The update of cell (i, j, k) from the stored face fluxes.
+template <typename R>
+__device__ __forceinline__ void update_cell_at3d(const Geom3D& g, const Eos3D& e, const State3D& U0,
+                                                 const FaceFlux<R>& FX, const FaceFlux<R>& FY, const FaceFlux<R>& FZ,
+                                                 double dt, int i, int j, int k, State3D& Us) {
+    const size_t q = ((size_t)k * g.Ny + j) * g.Nx + i;
+    const size_t xL = fx_(g, i, j, k), yD = fy_(g, i, j, k), zB = fz_(g, i, j, k);
+    const size_t xo[2] = {xL, xL + 1}, yo[2] = {yD, yD + g.Nx}, zo[2] = {zB, zB + (size_t)g.Nx * g.Ny};
+    const double u0[5] = {U0.rho[q], U0.mx[q], U0.my[q], U0.mz[q], U0.E[q]};
+    double u[5];
+    cell_update3d(
+        g, e, dt, u0,
+        [&](int ax, int side, int c) {
+            return ax == 0 ? (double)FX.f[c][xo[side]] : ax == 1 ? (double)FY.f[c][yo[side]] : (double)FZ.f[c][zo[side]];
+        },
+        bxc(g, U0.Bx, i, j, k), byc(g, U0.By, i, j, k), bzc(g, U0.Bz, i, j, k), u);
+    Us.rho[q] = u[0]; Us.mx[q] = u[1]; Us.my[q] = u[2]; Us.mz[q] = u[3]; Us.E[q] = u[4];
 }
 
 // Flux difference, conserved update and floors with the old faces (M3:802-834), in FP64.
@@ -398,22 +494,131 @@ __global__ __launch_bounds__(256) void k_update3d(Geom3D g, Eos3D e_in, State3D 
     const Eos3D e = eos_of<IDEAL>(e_in);
     const unsigned qq = blockIdx.x * 256u + threadIdx.x;  // cells < 2^31 (checked at create)
     if (qq >= (unsigned)g.Nx * (unsigned)g.Ny * (unsigned)g.Nz) return;
-    const size_t q = qq;
     const int i = (int)(qq % (unsigned)g.Nx);
     const unsigned r = qq / (unsigned)g.Nx;
     const int j = (int)(r % (unsigned)g.Ny), k = (int)(r / (unsigned)g.Ny);
-    const size_t xL = fx_(g, i, j, k), yD = fy_(g, i, j, k), zB = fz_(g, i, j, k);
-    const size_t xR = xL + 1, yU = yD + g.Nx, zF = zB + (size_t)g.Nx * g.Ny;
-    const double invdx = 1.0 / g.dx, invdy = 1.0 / g.dy, invdz = 1.0 / g.dz, dt = sc->dt;
-    double d[5];
+    update_cell_at3d<R>(g, e, U0, FX, FY, FZ, sc->dt, i, j, k, Us);
+}
+
+// ------------------------------------------------------------------ z faces + update in one pass
+// One face of axis AX from PLM states in the flux precision R: the axis-frame rotation of face_states3d, the
+// HLLD (FP32: also whether the FP64 solver must redo the face), the fluxes in conserved order.
+template <int AX, typename R, typename EOSR>
+__device__ __forceinline__ bool solve_face3d(const Geom3D& g, const EOSR& eos, const Prim<R>& WL, const Prim<R>& WR,
+                                             double Bn, R o[7], R f[5]) {
+    Prim<R> L = WL, Rr = WR;
+    if (AX == 1) {
+        L = Prim<R>{WL.rho, WL.v, WL.u, WL.w, WL.P, (R)Bn, WL.Bx, WL.Bz};
+        Rr = Prim<R>{WR.rho, WR.v, WR.u, WR.w, WR.P, (R)Bn, WR.Bx, WR.Bz};
+    } else if (AX == 2) {
+        L = Prim<R>{WL.rho, WL.w, WL.u, WL.v, WL.P, (R)Bn, WL.Bx, WL.By};
+        Rr = Prim<R>{WR.rho, WR.w, WR.u, WR.v, WR.P, (R)Bn, WR.Bx, WR.By};
+    }
+    bool deg = false;
+    if constexpr (std::is_same<R, float>::value)
+        hlld_x<float>(eos, L, Rr, (float)Bn, fabs(Bn) < 1e-12, o, &deg, g.tau, (float)g.hlld_guard);
+    else
+        hlld_x<double>(eos, L, Rr, Bn, fabs(Bn) < 1e-12, o, nullptr, 0.0, g.hlld_guard);
+    face_fluxes3d<AX, R>(o, f);
+    return deg;
+}
+
+// The z faces and the update of a stage in one column march (z faces: M3:777-800, update: M3:802-834), in
+// place of k_face3d<2> + k_update3d.  One thread per (i, j) column marches over a chunk of planes: at plane s
+// it forms PLM-MC of cell s+1 from the primitives of cells s, s+1 (carried) and s+2 (loaded one plane ahead),
+// solves z face s+1 between the carried plus state of cell s and the minus state of cell s+1, and updates
+// cell s from its two z-face fluxes (registers) and the x and y face fluxes the face kernels stored (final:
+// in the mixed path the FP64 redo of flagged x and y faces runs before).  The z fluxes never reach memory.
+// The march starts two planes early, so that a chunk's bottom face (also the top face of the chunk below)
+// comes from the same code as every other z face.  Mixed: the chunk that owns a z face the FP32 HLLD flags
+// queues it; a cell with such a face stores its two z-face fluxes and is queued for k_update3d_cells, which
+// redoes it after k_face3d_fp64 has stored the FP64 flux.
+template <bool MIXED, typename EOSR, bool IDEAL>
+__global__ __launch_bounds__(256) void k_zupd3d(Geom3D g, Eos3D e_in, EOSR eos,
+                                                PrimArrays<typename std::conditional<MIXED, float, double>::type> W,
+                                                State3D U0, const Scalars* __restrict__ sc, State3D Us,
+                                                FaceFlux<typename std::conditional<MIXED, float, double>::type> FX,
+                                                FaceFlux<typename std::conditional<MIXED, float, double>::type> FY,
+                                                FaceFlux<typename std::conditional<MIXED, float, double>::type> FZ,
+                                                EmfArr<typename std::conditional<MIXED, float, double>::type> ef,
+                                                DegQueue dq, int zchunk) {
+    using R = typename std::conditional<MIXED, float, double>::type;
+    const Eos3D e = eos_of<IDEAL>(e_in);
+    const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+    const int i = (int)blockIdx.x * 64 + (int)(threadIdx.x & 63), j = (int)blockIdx.y * 4 + (int)(threadIdx.x >> 6);
+    if (i >= Nx || j >= Ny) return;  // no barriers below
+    const int c0 = (int)blockIdx.z * zchunk, c1 = min(Nz, c0 + zchunk);
+    const size_t NXY = (size_t)Nx * Ny, NX1Y = (size_t)(Nx + 1) * Ny, NXY1 = (size_t)Nx * (Ny + 1);
+    const size_t coff = (size_t)j * Nx + i, xoff = (size_t)j * (Nx + 1) + i;
+    const R frho = (R)g.floor_rho, fP = (R)g.floor_P;
+    const double dt = sc->dt;
+    auto prim = [&](int k) { return load_prim(W, (size_t)bmap(k, Nz, g.pz) * NXY + coff); };
+
+    Prim<R> wp{};   // plus state of cell s
+    R fzb[5] = {};  // flux of z face s
+    bool dzb = false;
+    double bzb = 0.0;
+    for (int s = c0 - 2; s < c1; ++s) {
+        // cells s and s+1 were loaded by the previous plane's step (cache hits): reloading them costs less
+        // than carrying three primitive sets through the solver
+        Prim<R> mn, wpn;
+        plm_mc<R>(prim(s), prim(s + 1), prim(s + 2), 2, frho, fP, mn, wpn);  // minus and plus states of cell s+1
+        if (s >= c0 - 1) {
+            const size_t zf = (size_t)(s + 1) * NXY;
+            const double bzt = U0.Bz[zf + coff];
+            R o[7], f[5];
+            const bool d = solve_face3d<2, R>(g, eos, wp, mn, bzt, o, f);
+            const bool own = s + 1 < c1 || s + 1 == Nz;  // z faces c0 .. c1-1 (and Nz) belong to this chunk
+            const bool defer = MIXED && s >= c0 && (dzb || d);
+            if constexpr (MIXED) queue_push(dq, 3, defer, (int)((size_t)s * NXY + coff));
+            if (s >= c0) {  // cell s
+                const size_t zc = (size_t)s * NXY, q = zc + coff;
+                if (defer) {
 #pragma unroll
-    for (int c = 0; c < 5; c++)
-        d[c] = -((double)FX.f[c][xR] - (double)FX.f[c][xL]) * invdx - ((double)FY.f[c][yU] - (double)FY.f[c][yD]) * invdy -
-               ((double)FZ.f[c][zF] - (double)FZ.f[c][zB]) * invdz;
-    double rho = U0.rho[q] + dt * d[0], mx = U0.mx[q] + dt * d[1], my = U0.my[q] + dt * d[2];
-    double mz = U0.mz[q] + dt * d[3], E = U0.E[q] + dt * d[4];
-    floors3d(e, rho, mx, my, mz, E, bxc(g, U0.Bx, i, j, k), byc(g, U0.By, i, j, k), bzc(g, U0.Bz, i, j, k));
-    Us.rho[q] = rho; Us.mx[q] = mx; Us.my[q] = my; Us.mz[q] = mz; Us.E[q] = E;
+                    for (int k = 0; k < 5; ++k) {
+                        FZ.f[k][q] = fzb[k];
+                        FZ.f[k][zf + coff] = f[k];
+                    }
+                } else {
+                    const size_t xL = (size_t)s * NX1Y + xoff, yD = (size_t)s * NXY1 + coff;
+                    const double u0[5] = {U0.rho[q], U0.mx[q], U0.my[q], U0.mz[q], U0.E[q]};
+                    double u[5];
+                    cell_update3d(
+                        g, e, dt, u0,
+                        [&](int ax, int side, int k) {
+                            return ax == 0 ? (double)FX.f[k][xL + side]
+                                           : ax == 1 ? (double)FY.f[k][yD + (side ? (size_t)Nx : 0)]
+                                                     : (double)(side ? f[k] : fzb[k]);
+                        },
+                        0.5 * (U0.Bx[xL] + U0.Bx[xL + 1]), 0.5 * (U0.By[yD] + U0.By[yD + Nx]), 0.5 * (bzb + bzt), u);
+                    Us.rho[q] = u[0]; Us.mx[q] = u[1]; Us.my[q] = u[2]; Us.mz[q] = u[3]; Us.E[q] = u[4];
+                }
+            }
+            if (own) face_emfs3d<2, R>(ef, zf + coff, o);  // after the update: before its loads it holds registers
+            if constexpr (MIXED) queue_push(dq, 2, own && d, (int)(zf + coff));
+#pragma unroll
+            for (int k = 0; k < 5; ++k) fzb[k] = f[k];
+            dzb = d;
+            bzb = bzt;
+        }
+        wp = wpn;
+    }
+}
+
+// The update of the cells k_zupd3d queued (those with a z face the FP32 HLLD flagged), after k_face3d_fp64 has
+// stored the FP64 fluxes of the flagged faces beside the FP32 ones the cell wrote.
+template <bool IDEAL>
+__global__ __launch_bounds__(256) void k_update3d_cells(Geom3D g, Eos3D e_in, State3D U0, FaceFlux<float> FX,
+                                                        FaceFlux<float> FY, FaceFlux<float> FZ,
+                                                        const Scalars* __restrict__ sc, State3D Us, DegQueue dq) {
+    const Eos3D e = eos_of<IDEAL>(e_in);
+    queue_for_each(dq, 3, [&](int c) {
+        const unsigned qq = (unsigned)c;
+        const int i = (int)(qq % (unsigned)g.Nx);
+        const unsigned r = qq / (unsigned)g.Nx;
+        const int j = (int)(r % (unsigned)g.Ny), k = (int)(r / (unsigned)g.Ny);
+        update_cell_at3d<float>(g, e, U0, FX, FY, FZ, sc->dt, i, j, k, Us);
+    });
 }
 
 // ------------------------------------------------------------------ edge EMFs (M3:836-939)
@@ -936,6 +1141,13 @@ HybridEOS<R> make_hybrid(const cc_mhd3d_params* p) {
 
 }  // namespace cc
 
+// The EOS in the flux precision: FP32 for the mixed path, FP64 otherwise.
+template <bool MIXED, typename EOS64, typename EOS32>
+const typename std::conditional<MIXED, EOS32, EOS64>::type& select_eos(const EOS64& e64, const EOS32& e32) {
+    if constexpr (MIXED) return e32;
+    else return e64;
+}
+
 // ==================================================================== host handle
 using namespace cc;
 
@@ -965,6 +1177,13 @@ struct cc_mhd3d {
     bool fused = true;
     int ect_shape = -1;  // -1: 3 for the mixed path (fits two blocks per CU), 1 for FP64 (spills at that bound)
     int zchunk = 32;
+    // z faces + update in one column march (k_zupd3d, z chunks of zuchunk planes): 1 always, 0 never, -1
+    // (default) for FP64 only.  In the mixed path every z face the FP32 HLLD flags defers its two cells to a
+    // second pass, which costs more than the saved z-flux traffic (256^3 turbulence: 5.5-6.1 against 5.5-5.7 ms
+    // per step); in FP64 the pass saves 0.3 ms per stage.
+    int zupd = -1;
+    int zuchunk = 32;
+    bool use_zupd() const { return zupd > 0 || (zupd < 0 && p.precision != CC_MIXED); }
     Profiler prof;
     FrameBuffer frame;
     FieldStats stats;
@@ -1030,8 +1249,13 @@ struct cc_mhd3d {
         const size_t ne[6] = {NFx(), NFx(), NFy(), NFy(), NFz(), NFz()};
         for (int c = 0; c < 6; ++c) CC_TRY(alloc_bytes(&emf[c], ne[c] * s));
         if (p.precision == CC_MIXED) {
-            CC_TRY(alloc_bytes((void**)&dq.count, 3 * sizeof(int)));
-            for (int ax = 0; ax < 3; ++ax) CC_TRY(alloc_bytes((void**)&dq.face[ax], nf[ax] * sizeof(int)));
+            CC_TRY(alloc_bytes((void**)&dq.count, (size_t)4 * kQSub * kQStride * sizeof(int)));
+            for (int ax = 0; ax < 3; ++ax) {
+                CC_TRY(alloc_bytes((void**)&dq.face[ax], nf[ax] * sizeof(int)));
+                dq.span[ax] = (int)((nf[ax] + kQSub - 1) / kQSub);
+            }
+            CC_TRY(alloc_bytes((void**)&dq.cell, N() * sizeof(int)));
+            dq.span[3] = (int)((N() + kQSub - 1) / kQSub);
         }
         return CC_OK;
     }
@@ -1058,8 +1282,48 @@ struct cc_mhd3d {
     cc_status launch_flux(const EOS64& e64, const EOS32& e32, const State3D& in) {
         using R = typename std::conditional<MIXED, float, double>::type;
         if (MIXED) {
-            hipLaunchKernelGGL(k_zero_count3, dim3(1), dim3(64), 0, stream, dq.count);
+            hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(256), 0, stream, dq.count);
             CC_CHECK_LAUNCH();
+        }
+        if (use_zupd()) {  // x and y faces; z faces + update in one column march
+            {
+                ProfScope ps(pr(), stream, "face");
+                hipLaunchKernelGGL((k_face3d<0, MIXED, EOS64, EOS32>), dim3(blocks(NFx())), dim3(256), 0, stream, g, e64,
+                                   e32, W<R>(), in.Bx, FF<R>(0), EF<R>(), dq, e, in);
+                CC_CHECK_LAUNCH();
+            }
+            {
+                ProfScope ps(pr(), stream, "face");
+                hipLaunchKernelGGL((k_face3d<1, MIXED, EOS64, EOS32>), dim3((p.Nx + 63) / 64, p.Nz, (p.Ny + 1 + 3) / 4),
+                                   dim3(256), 0, stream, g, e64, e32, W<R>(), in.By, FF<R>(1), EF<R>(), dq, e, in);
+                CC_CHECK_LAUNCH();
+            }
+            if constexpr (MIXED) {
+                ProfScope ps(pr(), stream, "face_fp64");
+                hipLaunchKernelGGL((k_face3d_fp64<EOS64>), dim3(2048), dim3(256), 0, stream, g, e, e64, in,
+                                   FF<float>(0), FF<float>(1), FF<float>(2), EF<float>(), dq, 3);
+                CC_CHECK_LAUNCH();
+            }
+            {
+                ProfScope ps(pr(), stream, "zupd");
+                const dim3 grid((p.Nx + 63) / 64, (p.Ny + 3) / 4, (p.Nz + zuchunk - 1) / zuchunk);
+                using EOSR = typename std::conditional<MIXED, EOS32, EOS64>::type;
+                const EOSR& er = select_eos<MIXED>(e64, e32);
+                hipLaunchKernelGGL((e.ideal ? k_zupd3d<MIXED, EOSR, true> : k_zupd3d<MIXED, EOSR, false>), grid,
+                                   dim3(256), 0, stream, g, e, er, W<R>(), in, sc, Us, FF<R>(0), FF<R>(1), FF<R>(2),
+                                   EF<R>(), dq, zuchunk);
+                CC_CHECK_LAUNCH();
+            }
+            if constexpr (MIXED) {
+                ProfScope ps(pr(), stream, "face_fp64");
+                hipLaunchKernelGGL((k_face3d_fp64<EOS64>), dim3(2048), dim3(256), 0, stream, g, e, e64, in,
+                                   FF<float>(0), FF<float>(1), FF<float>(2), EF<float>(), dq, 4);
+                hipLaunchKernelGGL((e.ideal ? k_update3d_cells<true> : k_update3d_cells<false>), dim3(1024),
+                                   dim3(256), 0, stream, g, e, in, FF<float>(0), FF<float>(1), FF<float>(2), sc, Us,
+                                   dq);
+                CC_CHECK_LAUNCH();
+            }
+            return CC_OK;
         }
         {
             ProfScope ps(pr(), stream, "face");
@@ -1087,7 +1351,7 @@ struct cc_mhd3d {
         if constexpr (MIXED) {
             ProfScope ps(pr(), stream, "face_fp64");
             hipLaunchKernelGGL((k_face3d_fp64<EOS64>), dim3(2048), dim3(256), 0, stream, g, e, e64, in, FF<float>(0),
-                               FF<float>(1), FF<float>(2), EF<float>(), dq);
+                               FF<float>(1), FF<float>(2), EF<float>(), dq, 7);
             CC_CHECK_LAUNCH();
         }
         {
@@ -1252,6 +1516,8 @@ cc_status cc_mhd3d_create(const cc_mhd3d_params* p, cc_mhd3d** out) {
     if (const char* v = std::getenv("CC_M3_ECT_SHAPE")) h->ect_shape = std::atoi(v);
     if (const char* v = std::getenv("CC_M3_ZCHUNK")) h->zchunk = std::max(1, std::atoi(v));
     if (const char* v = std::getenv("CC_M3_JCHUNK")) h->jchunk = std::max(1, std::atoi(v));
+    if (const char* v = std::getenv("CC_M3_ZUPD")) h->zupd = std::atoi(v);
+    if (const char* v = std::getenv("CC_M3_ZUCHUNK")) h->zuchunk = std::max(1, std::atoi(v));
     if (st == CC_OK && !h->fused) st = h->alloc(&h->ed.Ex, (size_t)p->Nx * (p->Ny + 1) * (p->Nz + 1));
     if (st == CC_OK && !h->fused) st = h->alloc(&h->ed.Ey, (size_t)(p->Nx + 1) * p->Ny * (p->Nz + 1));
     if (st == CC_OK && !h->fused) st = h->alloc(&h->ed.Ez, (size_t)(p->Nx + 1) * (p->Ny + 1) * p->Nz);

@@ -45,10 +45,10 @@ struct IdealEOS2D {
     __host__ __device__ __forceinline__ Aux aux(const Prim<R>&) const { return Aux{}; }
     __host__ __device__ __forceinline__ R total_energy(const Prim<R>& W, const Aux&) const {
         R B2 = W.Bx * W.Bx + W.By * W.By + W.Bz * W.Bz;
-        return W.P / gm1 + R(0.5) * W.rho * (W.u * W.u + W.v * W.v + W.w * W.w) + R(0.5) * B2;
+        return qdiv<R>(W.P, gm1) + R(0.5) * W.rho * (W.u * W.u + W.v * W.v + W.w * W.w) + R(0.5) * B2;
     }
     __host__ __device__ __forceinline__ R sound_speed2(const Prim<R>& W, const Aux&) const {  // M2:148-150
-        return gamma * W.P / smax(W.rho, floor_rho);
+        return qdiv<R>(gamma * W.P, smax(W.rho, floor_rho));
     }
 };
 
@@ -63,10 +63,10 @@ struct IdealEOS3D {
         const R rho = smax(W.rho, floor_rho);
         const R kin = R(0.5) * rho * (W.u * W.u + W.v * W.v + W.w * W.w);
         const R mag = R(0.5) * (W.Bx * W.Bx + W.By * W.By + W.Bz * W.Bz);
-        return kin + mag + W.P / gm1;
+        return kin + mag + qdiv<R>(W.P, gm1);
     }
     __host__ __device__ __forceinline__ R sound_speed2(const Prim<R>& W, const Aux&) const {
-        return gamma * W.P / smax(W.rho, floor_rho);
+        return qdiv<R>(gamma * W.P, smax(W.rho, floor_rho));
     }
 };
 
@@ -121,8 +121,8 @@ __host__ __device__ __forceinline__ R fast_speed(const EOS& eos, const Prim<R>& 
     R a2 = eos.sound_speed2(W, ax);
     R b2 = W.Bx * W.Bx + W.By * W.By + W.Bz * W.Bz;
     R rho = smax(W.rho, Tiny<R>::v());
-    R vA2 = b2 / rho;
-    R vAn2 = (Bn * Bn) / rho;
+    R vA2 = qdiv<R>(b2, rho);
+    R vAn2 = qdiv<R>(Bn * Bn, rho);
     R disc = (a2 + vA2) * (a2 + vA2) - R(4) * a2 * vAn2;
     disc = smax(R(0), disc);
     R cf2 = R(0.5) * ((a2 + vA2) + sqrt(disc));
@@ -191,7 +191,7 @@ __host__ __device__ __forceinline__ void hlld_x(const EOS& eos, Prim<R> WL, Prim
     auto hll = [&]() {  // M2:449-467 (HLL)
         R UL[7] = {WL.rho, WL.rho * WL.u, WL.rho * WL.v, WL.rho * WL.w, EL, WL.By, WL.Bz};
         R UR[7] = {WR.rho, WR.rho * WR.u, WR.rho * WR.v, WR.rho * WR.w, ER, WR.By, WR.Bz};
-        R inv = R(1) / (SR - SL);
+        R inv = qdiv<R>(R(1), SR - SL);
 #pragma unroll
         for (int k = 0; k < 7; k++) out[k] = (SR * FL[k] - SL * FR[k] + SL * SR * (UR[k] - UL[k])) * inv;
     };
@@ -205,10 +205,10 @@ __host__ __device__ __forceinline__ void hlld_x(const EOS& eos, Prim<R> WL, Prim
 
     R denom = rhoL * (SL - uL) - rhoR * (SR - uR);
     R SM = R(0);
-    if (fabs(denom) > R(1e-14)) SM = (ptR - ptL + rhoL * uL * (SL - uL) - rhoR * uR * (SR - uR)) / denom;
+    if (fabs(denom) > R(1e-14)) SM = qdiv<R>(ptR - ptL + rhoL * uL * (SL - uL) - rhoR * uR * (SR - uR), denom);
 
-    const R rhoL_s = rhoL * (SL - uL) / (SL - SM);
-    const R rhoR_s = rhoR * (SR - uR) / (SR - SM);
+    const R rhoL_s = qdiv<R>(rhoL * (SL - uL), SL - SM);
+    const R rhoR_s = qdiv<R>(rhoR * (SR - uR), SR - SM);
     const R pt_s = ptL + rhoL * (SL - uL) * (SM - uL);
     const R sL = SL - uL, sR = SR - uR;
     const R Bn2 = Bn * Bn;
@@ -240,24 +240,24 @@ __host__ __device__ __forceinline__ void hlld_x(const EOS& eos, Prim<R> WL, Prim
     R vL_s = WL.v, wL_s = WL.w, vR_s = WR.v, wR_s = WR.w;
     if (fabs(dL) > R(1e-14)) {
         R num = rhoL * sL * (SL - uL) - Bn2;
-        ByL_s = WL.By * num / dL;
-        BzL_s = WL.Bz * num / dL;
+        ByL_s = qdiv<R>(WL.By * num, dL);
+        BzL_s = qdiv<R>(WL.Bz * num, dL);
         R den = rhoL * (SL - uL) * (SL - SM);
-        vL_s = WL.v - Bn * (ByL_s - WL.By) / den;
-        wL_s = WL.w - Bn * (BzL_s - WL.Bz) / den;
+        vL_s = WL.v - qdiv<R>(Bn * (ByL_s - WL.By), den);
+        wL_s = WL.w - qdiv<R>(Bn * (BzL_s - WL.Bz), den);
     }
     if (fabs(dR) > R(1e-14)) {
         R num = rhoR * sR * (SR - uR) - Bn2;
-        ByR_s = WR.By * num / dR;
-        BzR_s = WR.Bz * num / dR;
+        ByR_s = qdiv<R>(WR.By * num, dR);
+        BzR_s = qdiv<R>(WR.Bz * num, dR);
         R den = rhoR * (SR - uR) * (SR - SM);
-        vR_s = WR.v - Bn * (ByR_s - WR.By) / den;
-        wR_s = WR.w - Bn * (BzR_s - WR.Bz) / den;
+        vR_s = WR.v - qdiv<R>(Bn * (ByR_s - WR.By), den);
+        wR_s = WR.w - qdiv<R>(Bn * (BzR_s - WR.Bz), den);
     }
     const R sqrt_rhoL = sqrt(smax(rhoL_s, Tiny<R>::v()));
     const R sqrt_rhoR = sqrt(smax(rhoR_s, Tiny<R>::v()));
-    const R SLL = SM - fabs(Bn) / sqrt_rhoL;
-    const R SRR = SM + fabs(Bn) / sqrt_rhoR;
+    const R SLL = SM - qdiv<R>(fabs(Bn), sqrt_rhoL);
+    const R SRR = SM + qdiv<R>(fabs(Bn), sqrt_rhoR);
 
     // Select the branch first (M2:526-529) and evaluate only its star state; the others are unused.
     const bool left = (R(0) < SM);
@@ -278,17 +278,17 @@ __host__ __device__ __forceinline__ void hlld_x(const EOS& eos, Prim<R> WL, Prim
     } else {  // M2:469-478
         const R signBn = (Bn >= R(0)) ? R(1) : R(-1);
         const R denom2 = sqrt_rhoL + sqrt_rhoR;
-        v_s = (sqrt_rhoL * vL_s + sqrt_rhoR * vR_s + (ByR_s - ByL_s) * signBn) / denom2;
-        w_s = (sqrt_rhoL * wL_s + sqrt_rhoR * wR_s + (BzR_s - BzL_s) * signBn) / denom2;
-        By_s = (sqrt_rhoL * ByR_s + sqrt_rhoR * ByL_s + sqrt_rhoL * sqrt_rhoR * (vR_s - vL_s) * signBn) / denom2;
-        Bz_s = (sqrt_rhoL * BzR_s + sqrt_rhoR * BzL_s + sqrt_rhoL * sqrt_rhoR * (wR_s - wL_s) * signBn) / denom2;
+        v_s = qdiv<R>(sqrt_rhoL * vL_s + sqrt_rhoR * vR_s + (ByR_s - ByL_s) * signBn, denom2);
+        w_s = qdiv<R>(sqrt_rhoL * wL_s + sqrt_rhoR * wR_s + (BzR_s - BzL_s) * signBn, denom2);
+        By_s = qdiv<R>(sqrt_rhoL * ByR_s + sqrt_rhoR * ByL_s + sqrt_rhoL * sqrt_rhoR * (vR_s - vL_s) * signBn, denom2);
+        Bz_s = qdiv<R>(sqrt_rhoL * BzR_s + sqrt_rhoR * BzL_s + sqrt_rhoL * sqrt_rhoR * (wR_s - wL_s) * signBn, denom2);
     }
     // U_star (M2:486-505) and flux_from_state (M2:507-519); Wr.Bx = Bn on both sides
     const R Br2 = Bn * Bn + Wr_By * Wr_By + Wr_Bz * Wr_Bz;
     const R ptr = Wr_P + R(0.5) * Br2;
     const R vdotB_s = SM * Bn + v_s * By_s + w_s * Bz_s;
     const R vdotB_w = Wr_u * Bn + Wr_v * Wr_By + Wr_w * Wr_Bz;
-    const R Es = ((S - Wr_u) * Er - (ptr * Wr_u - Bn * vdotB_w) + (pt_s * SM - Bn * vdotB_s)) / (S - SM);
+    const R Es = qdiv<R>((S - Wr_u) * Er - (ptr * Wr_u - Bn * vdotB_w) + (pt_s * SM - Bn * vdotB_s), S - SM);
     const R Ust[7] = {rho_s, rho_s * SM, rho_s * v_s, rho_s * w_s, Es, By_s, Bz_s};
     const R U0[7] = {Wr_rho, Wr_rho * Wr_u, Wr_rho * Wr_v, Wr_rho * Wr_w, Er, Wr_By, Wr_Bz};
 #pragma unroll

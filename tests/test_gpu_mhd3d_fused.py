@@ -1,6 +1,7 @@
 """GPU parity of the one-pass edge EMFs + CT of the 3D MHD stage (k_ect3d, z-marching tiles) on grids that
 span several tiles and z chunks, against the CPU oracle (mhd3d_resistive_hlld_ct.cpp restated,
-M3:694-1066) and against the split passes k_edge3d + k_ct3d of the same library.
+M3:694-1066) and against the split passes k_edge3d + k_ct3d of the same library; and of the column pass that
+solves the z faces and updates the cells (k_zupd3d, CC_M3_ZUPD) against the split k_face3d<2> + k_update3d.
 
 Tiles are 32 x 28, 32 x 12 or 64 x 5 cells (CC_M3_ECT_SHAPE 0, 1 and 3, 2), so a 70 x 53 x 11 grid has partial
 tiles on both axes; CC_M3_ZCHUNK = 3 splits z into four chunks (and CC_M3_JCHUNK the column-marching Joule
@@ -107,3 +108,61 @@ def test_fused_passes_on_tiny_grids_match_oracle(monkeypatch, grid, bc):
     want = m3_dict(p, S)
     for f in M.FIELDS:
         assert rel_err(got[f], want[f]) < 1e-10, (f, rel_err(got[f], want[f]))
+
+
+ZUPD_GRIDS = [(70, 53, 11), (2, 3, 2), (5, 2, 7), (3, 33, 4)]
+
+
+@pytest.mark.parametrize("grid", ZUPD_GRIDS)
+@pytest.mark.parametrize("bc", ["periodic", "clamped", "mixed"])
+@pytest.mark.parametrize("eos", [1, 0])
+@pytest.mark.parametrize("precision", [CC_FP64, CC_MIXED])
+def test_zface_update_pass_matches_split_passes(monkeypatch, grid, bc, eos, precision):
+    """One stage and one Heun step with the z faces + update in one column march (z chunks of 3 planes, so the
+    chunk seams and the two-plane start of the march are exercised) against the split passes.  FP64: the same
+    arithmetic, up to FMA contraction (1e-13).  Mixed: the FP32 solver of the two kernels may round a face flux
+    one ulp apart, which moves a stage by ~1e-9 of a field (measured <= 7.2e-9 after a Heun step); 2e-8 bounds
+    it.  An index or chunk-seam error moves a field by ~1e-3."""
+    monkeypatch.setenv("CC_M3_ZUCHUNK", "3")
+    px, py, pz = BCS[bc]
+    p = M.Params(Nx=grid[0], Ny=grid[1], Nz=grid[2], eos=eos, periodic_x=px, periodic_y=py, periodic_z=pz, eta=2e-3,
+                 ct_compensated=True, precision=precision, hlld_guard=1e-2)
+    st = random_mhd3d_state(p, 3 + eos)
+    tol = 1e-13 if precision == CC_FP64 else 2e-8
+    for what in ("stage", "rk2"):
+        out = {}
+        for zupd in (1, 0):
+            monkeypatch.setenv("CC_M3_ZUPD", str(zupd))
+            with M.Mhd3D(p) as h:
+                h.upload(st)
+                h.advance_stage(1e-3) if what == "stage" else h.rk2_step(1e-3)
+                out[zupd] = h.download()
+        for f in M.FIELDS:
+            e = rel_err(out[1][f], out[0][f])
+            assert e <= tol, (what, f, e)
+
+
+@pytest.mark.parametrize("eos,periodic", [(0, True), (1, False)])
+def test_zface_update_pass_deferred_cells_match_oracle(monkeypatch, eos, periodic):
+    """Mixed, with degen_tau = 2 and no HLLD guard: the FP32 solver flags every subsonic face, so nearly every
+    cell of the column pass is deferred (its two z-face fluxes stored, the cell queued) and redone after the
+    FP64 pass over the queued z faces; the stage then equals the FP64 oracle stage up to the FP32 rounding of
+    the stored fluxes (~1e-9 here, bar 1e-7).  z chunks of 3 planes, 12 x 10 x 9 cells."""
+    monkeypatch.setenv("CC_M3_ZUPD", "1")
+    monkeypatch.setenv("CC_M3_ZUCHUNK", "3")
+    p = M.Params(Nx=12, Ny=10, Nz=9, eos=eos, periodic_x=periodic, periodic_y=periodic, periodic_z=periodic,
+                 eta=2e-3, precision=CC_MIXED, degen_tau=2.0)
+    st = random_mhd3d_state(p, 5)
+    dt = 1e-3
+    with M.Mhd3D(p) as h:
+        h.upload(st)
+        h.advance_stage(dt)
+        got = h.download()
+    S0 = m3_arrays(p, st)
+    S1 = S0.copy()
+    oracle().or_m3_advance_stage(C.byref(m3_oparams(p)), C.byref(S0.cstate()), C.byref(S1.cstate()),
+                                 C.c_double(p.dx), C.c_double(p.dy), C.c_double(p.dz), C.c_double(dt))
+    want = m3_dict(p, S1)
+    for f in M.FIELDS:
+        e = rel_err(got[f], want[f])
+        assert e < 1e-7, (f, e)
