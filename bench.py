@@ -119,7 +119,7 @@ def bench_collapse2d(N, warmup, steps, acc_steps, prof_steps):
         t = h.run(warmup) if warmup > 0 else 0.0
         t, el = timed_run(h, steps, t)
         t, prof = profiled_pass(h, prof_steps, t, ("sor", "flux", "ct", "grad", "mean", "dt"))
-        res[prec] = {"elapsed": el, "prof": prof, "tile": M.sor_tile_choice(N, N, p.sor_iters, prec)}
+        res[prec] = {"elapsed": el, "prof": prof, "tile": M.sor_tile_choice(N, N, p.sor_iters, prec, with_kernels=True)}
         h.close()
         log("collapse2d %s %.3f ms/step" % ("fp64" if prec == CC_FP64 else "mixed", 1e3 * el / steps))
     acc = None
@@ -354,6 +354,7 @@ def bench_batch2d(n_inst, steps, warmup, nmin, nmax, seq_steps):
         "batch_over_sequential": seq / elapsed if seq > 0 else None,
         "sor_alg_GBps": cells * steps * 2 * 600 * 12 / elapsed / 1e9,
         "setup_s": round(t_setup, 2), "sor_merged": binfo["merged"], "sor_merged_tile": binfo["tile"],
+        "sor_merged_fp64_kernels": binfo["fp64_kernels"],
         "sor_tiles_alone": tiles,
         "max_divB": divb, "t_min": float(np.min(t)), "t_max": float(np.max(t)),
     }
@@ -581,15 +582,19 @@ def main():
                 "note": "algorithmic rate of a temporally blocked solver (16 sweeps per tile pass through LDS): "
                         "achieved can exceed the HBM peak; traffic_frac is the counter-measured HBM fraction"}
 
-    def sor_name(tile, prec):
-        S, WJ, sheared = tile
+    def sor_kernel(tile, prec):
+        """The SOR kernel a tile choice runs and its algorithmic bytes per cell-sweep (the mixed kernels sweep
+        an FP32 correction: 12 B; the FP64 kernels, also the choice of a mixed grid where they are faster: 24 B)."""
+        S, WJ, sheared, fp64k = tile
         form = ("k_sor_s" if sheared else "k_sor_") + ("tile" if os.environ.get("CC_SOR_LEVELS") else "flow")
-        if prec == CC_MIXED:
-            return "cc::%s_mixed<%d, %d>" % (form, S, WJ)
-        return "cc::%s_fp64<%d, %d%s" % (form, S, WJ, ">" if sheared else ",")
+        if prec == CC_MIXED and not fp64k:
+            return "cc::%s_mixed<%d, %d>" % (form, S, WJ), 12.0
+        return "cc::%s_fp64<%d, %d%s" % (form, S, WJ, ">" if sheared else ","), 24.0
 
-    roof_mx = roof(res[CC_MIXED], 12.0, sor_name(res[CC_MIXED]["tile"], CC_MIXED))
-    roof_64 = roof(res[CC_FP64], 24.0, sor_name(res[CC_FP64]["tile"], CC_FP64))
+    kmx, bmx = sor_kernel(res[CC_MIXED]["tile"], CC_MIXED)
+    k64, b64 = sor_kernel(res[CC_FP64]["tile"], CC_FP64)
+    roof_mx = roof(res[CC_MIXED], bmx, kmx)
+    roof_64 = roof(res[CC_FP64], b64, k64)
     kernels = {}
     for prec, tag in ((CC_FP64, "fp64"), (CC_MIXED, "mixed")):
         kernels[tag] = {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in res[prec]["prof"].items()
@@ -606,7 +611,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 fluxes/SOR sweeps + f64 update/CT/residual",
+        "dtype": ("f32 reconstruction/Riemann fluxes + f64 SOR sweeps/update/CT" if res[CC_MIXED]["tile"][3] else
+                  "f32 fluxes/SOR sweeps + f64 update/CT/residual"),
         "data": "synthetic",
         "config": {"workload": "2D resistive MHD HLLD+CT+SOR self-gravity, Gaussian collapse %dx%d, clamped "
                                "boundaries, 600 SOR sweeps x 2 solves per Heun step, compensated face update" % (N, N),

@@ -219,11 +219,11 @@ _PROTOS = {
     "cc_euler2d_ic_defaults": (None, [C.POINTER(Euler2dIc)]),
     "cc_euler2d_ic_reference": (C.c_int, [C.POINTER(Euler2dParams), C.POINTER(Euler2dIc)] + [DP] * 4),
     "cc_sor_autotune": (C.c_int, [C.c_int]),
-    "cc_sor_tile_choice": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int] + [C.POINTER(C.c_int)] * 3),
+    "cc_sor_tile_choice": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int] + [C.POINTER(C.c_int)] * 4),
     "cc_sor_flow_schedule": (C.c_int, [C.c_int] * 7 + [C.POINTER(C.c_int)] * 2 + [C.c_void_p] * 3),
     "cc_mhd2d_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
     "cc_mhd2d_batch_destroy": (C.c_int, [C.c_void_p]),
-    "cc_mhd2d_batch_info": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_int)] * 4),
+    "cc_mhd2d_batch_info": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_int)] * 5),
     "cc_mhd2d_batch_run": (C.c_int, [C.c_void_p, C.c_int, DP, DP]),
     "cc_collapse2d_batch_sample": (C.c_int, [C.c_uint, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int),
                                              C.POINTER(Collapse2dIc), DP]),

@@ -931,9 +931,9 @@ struct cc_mhd2d_batch {
         return CC_OK;
     }
     cc_status merged_sweeps(int a, int q) {
-        if (flow) return launch_flow_batch(tile, mixed, ntiles, root, tab[a][q], tiles4, dep_off, dep_idx, fstate);
+        if (flow) return launch_flow_batch(tile, sor_mixed_kernels(tile, mixed), ntiles, root, tab[a][q], tiles4, dep_off, dep_idx, fstate);
         for (size_t l = 0; l + 1 < level_off.size(); ++l)
-            CC_TRY(launch_tile_batch(tile, mixed, level_off[l + 1] - level_off[l], root, tab[a][q], tiles4 + 4 * level_off[l]));
+            CC_TRY(launch_tile_batch(tile, sor_mixed_kernels(tile, mixed), level_off[l + 1] - level_off[l], root, tab[a][q], tiles4 + 4 * level_off[l]));
         return CC_OK;
     }
     // Device copies of a merged schedule: tiles, and with waits the CSR arrays and a launch state.
@@ -990,9 +990,9 @@ struct cc_mhd2d_batch {
         // the tile shape: the instances' own, or with autotuning the compiled shape whose merged solve is fastest
         std::vector<SorTile> cands{h0->pois.tile_};
         if (sor_autotune_enabled()) {
-#define CC_BCAND64(S_, WJ_, R_) if (!mixed) cands.push_back(SorTile{S_, WJ_, R_});
+#define CC_BCAND64(S_, WJ_, R_) cands.push_back(SorTile{S_, WJ_, R_, false, mixed});
 #define CC_BCAND32(S_, WJ_) if (mixed) cands.push_back(SorTile{S_, WJ_, true});
-#define CC_BSCAND64(S_, K_) if (!mixed) cands.push_back(SorTile{S_, K_, true, true});
+#define CC_BSCAND64(S_, K_) cands.push_back(SorTile{S_, K_, true, true, mixed});
 #define CC_BSCAND32(S_, K_) if (mixed) cands.push_back(SorTile{S_, K_, true, true});
             CC_SOR_TILES_FP64(CC_BCAND64)
             CC_SOR_TILES_MIXED(CC_BCAND32)
@@ -1023,10 +1023,10 @@ struct cc_mhd2d_batch {
                 for (int rep = 0; rep < 2 && st == CC_OK; ++rep) {  // the first run pays first-launch costs; phi is scratch
                     hipEventRecord(e0, root);
                     if (fl) {
-                        st = launch_flow_batch(cands[ci], mixed, nt, root, tab[0][0], d.t4, d.doff, d.didx, d.st);
+                        st = launch_flow_batch(cands[ci], sor_mixed_kernels(cands[ci], mixed), nt, root, tab[0][0], d.t4, d.doff, d.didx, d.st);
                     } else {
                         for (size_t l = 0; l + 1 < off.size() && st == CC_OK; ++l)
-                            st = launch_tile_batch(cands[ci], mixed, off[l + 1] - off[l], root, tab[0][0], d.t4 + 4 * off[l]);
+                            st = launch_tile_batch(cands[ci], sor_mixed_kernels(cands[ci], mixed), off[l + 1] - off[l], root, tab[0][0], d.t4 + 4 * off[l]);
                     }
                     hipEventRecord(e1, root);
                     hipEventSynchronize(e1);
@@ -1037,10 +1037,10 @@ struct cc_mhd2d_batch {
                 if (st == CC_OK && fl) st = sor_flow_error(d.st, root);
                 if (st != CC_OK) { d.free_all(); keep.free_all(); return st; }
                 if (std::getenv("CC_SOR_AUTOTUNE_LOG"))
-                    std::fprintf(stderr, "sor autotune batch of %zu %s: S %d, %s %d%s, %s: %.3f ms\n", h.size(),
+                    std::fprintf(stderr, "sor autotune batch of %zu %s: S %d, %s %d%s%s, %s: %.3f ms\n", h.size(),
                                  mixed ? "mixed" : "fp64", cands[ci].S, cands[ci].sheared ? "K" : "WJ", cands[ci].WJ,
                                  cands[ci].sheared ? " sheared" : (cands[ci].rhs_lds ? "" : " rhs-global"),
-                                 fl ? "dataflow" : "levels", ms_form[fl]);
+                                 cands[ci].fp64 ? " fp64-kernels" : "", fl ? "dataflow" : "levels", ms_form[fl]);
             }
             // a single candidate (no autotuning) runs as the single-grid solves do: dataflow unless CC_SOR_LEVELS
             const bool use_flow = cands.size() > 1 ? ms_form[1] < ms_form[0] : can_flow;
@@ -1192,12 +1192,13 @@ cc_status cc_mhd2d_batch_create(cc_mhd2d* const* handles, int n, cc_mhd2d_batch*
     return CC_OK;
 }
 
-cc_status cc_mhd2d_batch_info(cc_mhd2d_batch* b, int* merged, int* S, int* WJ, int* sheared) {
+cc_status cc_mhd2d_batch_info(cc_mhd2d_batch* b, int* merged, int* S, int* WJ, int* sheared, int* fp64_kernels) {
     if (!b || !merged || !S || !WJ) { set_error(CC_ERR_INVALID_ARG, "cc_mhd2d_batch_info: null argument"); return CC_ERR_INVALID_ARG; }
     *merged = b->merged ? 1 : 0;
     *S = b->merged ? b->tile.S : 0;
     *WJ = b->merged ? b->tile.WJ : 0;
     if (sheared) *sheared = b->merged && b->tile.sheared ? 1 : 0;
+    if (fp64_kernels) *fp64_kernels = b->merged && b->tile.fp64 ? 1 : 0;
     return CC_OK;
 }
 

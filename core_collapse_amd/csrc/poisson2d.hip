@@ -1212,11 +1212,16 @@ cc_status Poisson2D::init(int Nx_, int Ny_, bool px_, bool py_, double dx_, doub
         const bool mixed = precision == CC_MIXED;
         SorTile t = mixed ? kTileMixedDefault : kTileFp64Default;
         bool from_env = false;
-        if (const char* e = std::getenv("CC_SOR_TILE")) {
-            int S_ = 0, WJ_ = 0;
-            char g = 0;
-            const int n = std::sscanf(e, "%d,%d,%c", &S_, &WJ_, &g);
-            if (n >= 2) { t = SorTile{S_, WJ_, !(n == 3 && g == 'g'), n == 3 && g == 's'}; from_env = true; }
+        if (const char* e = std::getenv("CC_SOR_TILE")) {  // "S,WJ[,flags]": g (source term from global), s
+            int S_ = 0, WJ_ = 0;                            // (sheared), d (FP64 kernels in a mixed solve)
+            char fl[8] = {0};
+            const int n = std::sscanf(e, "%d,%d,%7s", &S_, &WJ_, fl);
+            if (n >= 2) {
+                const std::string f = n == 3 ? fl : "";
+                t = SorTile{S_, WJ_, f.find('g') == std::string::npos, f.find('s') != std::string::npos};
+                t.fp64 = mixed && f.find('d') != std::string::npos;
+                from_env = true;
+            }
         }
         const bool ws = sor_ws_enabled();
         if (ws) {
@@ -1224,7 +1229,7 @@ cc_status Poisson2D::init(int Nx_, int Ny_, bool px_, bool py_, double dx_, doub
             CC_TRY(ws_->init(Nx, Ny, iters, dx, dy, G, omega, precision));
         }
         if (!from_env && !ws && sor_autotune_enabled()) CC_TRY(autotuned_tile(t));
-        if (!sor_tile_supported(t, mixed)) {
+        if (!sor_tile_supported(t, sor_mixed_kernels(t, mixed))) {
             set_error(CC_ERR_INVALID_ARG, "Poisson2D: unsupported SOR tile shape");
             return CC_ERR_INVALID_ARG;
         }
@@ -1297,9 +1302,11 @@ cc_status Poisson2D::autotuned_tile(SorTile& out) {
         if (it != g_tuned.end()) { out = it->second; return CC_OK; }
     }
     std::vector<SorTile> cands;
-#define CC_SOR_CAND64(S_, WJ_, R_) if (!mixed) cands.push_back(SorTile{S_, WJ_, R_});
+    // a mixed solve also times the FP64 kernels (fp64 = mixed): where the FP32 correction sweeps do not pay
+    // for their residual passes, full FP64 sweeps are both faster and exact
+#define CC_SOR_CAND64(S_, WJ_, R_) cands.push_back(SorTile{S_, WJ_, R_, false, mixed});
 #define CC_SOR_CAND32(S_, WJ_) if (mixed) cands.push_back(SorTile{S_, WJ_, true});
-#define CC_SOR_SCAND64(S_, K_) if (!mixed) cands.push_back(SorTile{S_, K_, true, true});
+#define CC_SOR_SCAND64(S_, K_) cands.push_back(SorTile{S_, K_, true, true, mixed});
 #define CC_SOR_SCAND32(S_, K_) if (mixed) cands.push_back(SorTile{S_, K_, true, true});
     CC_SOR_TILES_FP64(CC_SOR_CAND64)
     CC_SOR_TILES_MIXED(CC_SOR_CAND32)
@@ -1349,10 +1356,10 @@ cc_status Poisson2D::autotuned_tile(SorTile& out) {
             hipEventRecord(e0, 0);
             if (flow) {
                 hipLaunchKernelGGL(k_fill_int, dim3(std::min(64, (ntiles + 2 + 255) / 256)), dim3(256), 0, 0, d_st, ntiles + 2, 0);
-                st = launch_flow(c, mixed, flow_grid(ntiles), 0, args, FlowSched{sc, d_off, d_idx, ntiles, d_st, sor_flow_wait_ticks()}, phi, rz, rhobar);
+                st = launch_flow(c, sor_mixed_kernels(c, mixed), flow_grid(ntiles), 0, args, FlowSched{sc, d_off, d_idx, ntiles, d_st, sor_flow_wait_ticks()}, phi, rz, rhobar);
             } else {
                 for (size_t l = 0; l + 1 < off.size() && st == CC_OK; ++l)
-                    st = launch_tile(c, mixed, off[l + 1] - off[l], 0, args, sc + 3 * off[l], phi, rz, rhobar);
+                    st = launch_tile(c, sor_mixed_kernels(c, mixed), off[l + 1] - off[l], 0, args, sc + 3 * off[l], phi, rz, rhobar);
             }
             hipEventRecord(e1, 0);
             float t = 0.f;
@@ -1365,8 +1372,9 @@ cc_status Poisson2D::autotuned_tile(SorTile& out) {
         if (d_idx) hipFree(d_idx);
         if (d_st) hipFree(d_st);
         if (st == CC_OK && std::getenv("CC_SOR_AUTOTUNE_LOG"))
-            std::fprintf(stderr, "sor autotune %dx%d %s: S %d, %s %d%s: %.3f ms\n", Nx, Ny, mixed ? "mixed" : "fp64", c.S,
-                         c.sheared ? "K" : "WJ", c.WJ, c.sheared ? " sheared" : (c.rhs_lds ? "" : " rhs-global"), ms);
+            std::fprintf(stderr, "sor autotune %dx%d %s: S %d, %s %d%s%s: %.3f ms\n", Nx, Ny, mixed ? "mixed" : "fp64", c.S,
+                         c.sheared ? "K" : "WJ", c.WJ, c.sheared ? " sheared" : (c.rhs_lds ? "" : " rhs-global"),
+                         c.fp64 ? " fp64-kernels" : "", ms);
         if (st == CC_OK && ms < best) { best = ms; bt = c; }
     }
     if (e0) hipEventDestroy(e0);
@@ -1424,13 +1432,14 @@ cc_status Poisson2D::sweeps(const double* rho, hipStream_t s, Profiler* prof) {
         CC_CHECK_LAUNCH();
         FlowSched f{sched, dep_off, dep_idx, ntiles, flow_state, sor_flow_wait_ticks()};
         // two 1024-thread tiles fit a CU: a grid of 2 per CU keeps every block resident
-        CC_TRY(launch_flow(tile_, precision == CC_MIXED, flow_grid(ntiles), s, args, f, phi, rho, rhobar));
+        CC_TRY(launch_flow(tile_, sor_mixed_kernels(tile_, precision == CC_MIXED), flow_grid(ntiles), s, args, f, phi, rho,
+                           rhobar));
     } else if (tiled_) {
         const int nl = levels();
         for (int l = 0; l < nl; ++l) {
             const int nt = level_off_[l + 1] - level_off_[l];
             ProfScope ps(prof, s, "sor");
-            CC_TRY(launch_tile(tile_, precision == CC_MIXED, nt, s, args, sched + 3 * level_off_[l], phi, rho,
+            CC_TRY(launch_tile(tile_, sor_mixed_kernels(tile_, precision == CC_MIXED), nt, s, args, sched + 3 * level_off_[l], phi, rho,
                                rhobar));
             CC_CHECK_LAUNCH();
         }
@@ -1484,7 +1493,8 @@ cc_status cc_sor_autotune(int enable) {
     return CC_OK;
 }
 
-cc_status cc_sor_tile_choice(int Nx, int Ny, int sor_iters, int precision, int* S, int* WJ, int* sheared) {
+cc_status cc_sor_tile_choice(int Nx, int Ny, int sor_iters, int precision, int* S, int* WJ, int* sheared,
+                             int* fp64_kernels) {
     if (!S || !WJ) { cc::set_error(CC_ERR_INVALID_ARG, "null output"); return CC_ERR_INVALID_ARG; }
     cc::SorTile t = precision == CC_MIXED ? cc::kTileMixedDefault : cc::kTileFp64Default;
     {
@@ -1495,6 +1505,7 @@ cc_status cc_sor_tile_choice(int Nx, int Ny, int sor_iters, int precision, int* 
     *S = t.S;
     *WJ = t.WJ;
     if (sheared) *sheared = t.sheared ? 1 : 0;
+    if (fp64_kernels) *fp64_kernels = t.fp64 ? 1 : 0;
     return CC_OK;
 }
 

@@ -36,7 +36,11 @@ struct SorTile {
     int S = 32, WJ = 64;    // sweeps per tile; rows (rectangular) or k values (sheared) per tile
     bool rhs_lds = true;
     bool sheared = false;   // boxes in (sweep, i + j + 2 sweep, i + sweep) instead of (sweep, j + sweep, i + sweep)
+    bool fp64 = false;      // a mixed-precision solve running the FP64 kernels (full FP64 sweeps, the reference's
+                            // arithmetic): the autotuner takes them where they are faster than the mixed ones
 };
+// The kernels a solve of precision `mixed` runs with tile t: the mixed (FP32 correction) kernels or the FP64 ones.
+inline bool sor_mixed_kernels(const SorTile& t, bool mixed) { return mixed && !t.fp64; }
 
 struct SorArgs {
     int Nx, Ny, nsw;

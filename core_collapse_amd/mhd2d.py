@@ -187,13 +187,13 @@ def sor_autotune(enable=True):
     check(_lib.lib().cc_sor_autotune(int(bool(enable))))
 
 
-def sor_tile_choice(Nx, Ny, sor_iters=600, precision=CC_FP64):
+def sor_tile_choice(Nx, Ny, sor_iters=600, precision=CC_FP64, with_kernels=False):
     """(sweeps per tile, rows or k values per tile, sheared) of the SOR schedule a grid gets
-    (cc_sor_tile_choice)."""
-    S, WJ, sh = C.c_int(), C.c_int(), C.c_int()
+    (cc_sor_tile_choice); with_kernels=True appends whether a mixed grid runs the FP64 kernels."""
+    S, WJ, sh, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
     check(_lib.lib().cc_sor_tile_choice(int(Nx), int(Ny), int(sor_iters), int(precision), C.byref(S), C.byref(WJ),
-                                        C.byref(sh)))
-    return S.value, WJ.value, bool(sh.value)
+                                        C.byref(sh), C.byref(d)))
+    return (S.value, WJ.value, bool(sh.value)) + ((bool(d.value),) if with_kernels else ())
 
 
 def sor_flow_schedule(Nx, Ny, iters, W=64, WJ=64, S=16, sheared=False):
@@ -232,10 +232,10 @@ class Mhd2DBatch:
 
     def info(self):
         """{"merged": SOR tile levels merged across instances, "tile": (S, WJ, sheared) of the merged
-        launches}."""
-        m, S, WJ, sh = C.c_int(), C.c_int(), C.c_int(), C.c_int()
-        check(_lib.lib().cc_mhd2d_batch_info(self._b, C.byref(m), C.byref(S), C.byref(WJ), C.byref(sh)))
-        return {"merged": bool(m.value), "tile": (S.value, WJ.value, bool(sh.value))}
+        launches, "fp64_kernels": a mixed batch running the FP64 SOR kernels}."""
+        m, S, WJ, sh, d = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(_lib.lib().cc_mhd2d_batch_info(self._b, C.byref(m), C.byref(S), C.byref(WJ), C.byref(sh), C.byref(d)))
+        return {"merged": bool(m.value), "tile": (S.value, WJ.value, bool(sh.value)), "fp64_kernels": bool(d.value)}
 
     def close(self):
         if self._b:

@@ -149,9 +149,10 @@ cc_status cc_mhd2d_batch_destroy(cc_mhd2d_batch* b);
 /* merged = 1 when the instances' SOR tile levels run as merged launches (every instance with
  * gravity, clamped boundaries and one precision), with the tile shape (S sweeps, WJ rows or, with
  * *sheared = 1, WJ values of i + j + 2 sweep) the batch uses -- the first instance's, or with
- * cc_sor_autotune the fastest merged shape; merged = 0 when each instance's step is a separate graph
- * branch (or CC_BATCH_SEPARATE_SOR is set).  sheared may be null. */
-cc_status cc_mhd2d_batch_info(cc_mhd2d_batch* b, int* merged, int* S, int* WJ, int* sheared);
+ * cc_sor_autotune the fastest merged shape, *fp64_kernels = 1 when a mixed batch runs the FP64 SOR kernels;
+ * merged = 0 when each instance's step is a separate graph branch (or CC_BATCH_SEPARATE_SOR is set).
+ * sheared and fp64_kernels may be null. */
+cc_status cc_mhd2d_batch_info(cc_mhd2d_batch* b, int* merged, int* S, int* WJ, int* sheared, int* fp64_kernels);
 /* nsteps steps of every instance (the loop of cc_mhd2d_run: gravity, dt, clamp to t_end[i], Heun).
  * t_inout[i] holds instance i's t on entry and return.  Syncs. */
 cc_status cc_mhd2d_batch_run(cc_mhd2d_batch* b, int nsteps, double* t_inout, const double* t_end);
@@ -242,11 +243,14 @@ cc_status cc_hlld3d_batch(const cc_mhd3d_params* p, int n, const double* WL, con
  * afterwards with clamped boundaries times the compiled SOR tile shapes (sweeps per tile x rows) on
  * its grid once -- one full solve each, HIP events -- and keeps the fastest; the choice is cached per
  * (Nx, Ny, sor_iters, precision) for the process.  The environment variable CC_SOR_TILE overrides it.
- * cc_sor_tile_choice reports the shape a grid gets (the default shape if it was never tuned): S sweeps per
- * tile, WJ rows (rectangular tile) or WJ values of i + j + 2 sweep (sheared tile, *sheared = 1; sheared
- * may be null). */
+ * For a mixed-precision grid the FP64 kernels are candidates too (full FP64 sweeps: exact, and faster where
+ * the FP32 correction sweeps do not pay for their FP64 residual passes).  cc_sor_tile_choice reports the
+ * shape a grid gets (the default shape if it was never tuned): S sweeps per tile, WJ rows (rectangular
+ * tile) or WJ values of i + j + 2 sweep (sheared tile, *sheared = 1), and *fp64_kernels = 1 for a mixed
+ * grid that runs the FP64 kernels (sheared and fp64_kernels may be null). */
 cc_status cc_sor_autotune(int enable);
-cc_status cc_sor_tile_choice(int Nx, int Ny, int sor_iters, int precision, int* S, int* WJ, int* sheared);
+cc_status cc_sor_tile_choice(int Nx, int Ny, int sor_iters, int precision, int* S, int* WJ, int* sheared,
+                             int* fp64_kernels);
 
 /* The dataflow schedule of the tiled SOR (no reference counterpart; host only, touches no GPU): for an
  * Nx x Ny clamped grid, `iters` sweeps and tiles of S sweeps x W x WJ cells (W = 64 in the kernels;

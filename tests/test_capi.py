@@ -74,8 +74,8 @@ def test_new_entry_points_reject_bad_arguments_before_touching_a_device():
     assert L.cc_euler3d_create(C.byref(p), C.byref(h)) == 1  # gamma <= 1
     assert L.cc_euler3d_destroy(None) == 0
     S, WJ, sh = C.c_int(), C.c_int(), C.c_int()
-    assert L.cc_sor_tile_choice(128, 128, 600, 1, C.byref(S), C.byref(WJ), C.byref(sh)) == 0
-    assert L.cc_sor_tile_choice(128, 128, 600, 1, None, C.byref(WJ), None) == 1
+    assert L.cc_sor_tile_choice(128, 128, 600, 1, C.byref(S), C.byref(WJ), C.byref(sh), None) == 0
+    assert L.cc_sor_tile_choice(128, 128, 600, 1, None, C.byref(WJ), None, None) == 1
     N = (C.c_int * 1)()
     k = (_lib.Collapse2dIc * 1)()
     e = np.zeros(1)

@@ -138,6 +138,25 @@ def test_poisson_sor_mixed_close_to_oracle(Nx, Ny, iters):
     assert rel_err(gx, ox) < 1e-5 and rel_err(gy, oy) < 1e-5, (rel_err(gx, ox), rel_err(gy, oy))
 
 
+@pytest.mark.parametrize("Nx,Ny,iters", [(96, 80, 600), (130, 67, 37), (5, 17, 600)])
+@pytest.mark.parametrize("tile", ["16,64,s", "16,64"])
+def test_mixed_solve_on_the_fp64_kernels_is_the_fp64_solve(monkeypatch, Nx, Ny, iters, tile):
+    """A mixed-precision grid may run the FP64 SOR kernels (CC_SOR_TILE flag d; the autotuner's choice where
+    they are faster): its solve is then the FP64 solve, bit for bit."""
+    rng = np.random.default_rng(8)
+    p = M.Params(Nx=Nx, Ny=Ny, periodic_x=False, periodic_y=False, sor_iters=iters, G=1.0)
+    rho = 1.0 + rng.uniform(0, 1, (Ny, Nx))
+    monkeypatch.setenv("CC_SOR_WS", "0")
+    monkeypatch.setenv("CC_SOR_TILE", tile + ("d" if "," in tile[tile.index(",") + 1:] else ",d"))
+    gm = M.poisson_sor(rho, p.dx, p.dy, p.G, iters, 1.8, False, False, precision=CC_MIXED)
+    monkeypatch.setenv("CC_SOR_TILE", tile)
+    gd = M.poisson_sor(rho, p.dx, p.dy, p.G, iters, 1.8, False, False, precision=CC_FP64)
+    gr = M.poisson_sor(rho, p.dx, p.dy, p.G, iters, 1.8, False, False, precision=CC_MIXED)  # mixed kernels
+    for a, b in zip(gm, gd):
+        assert np.array_equal(a, b)
+    assert not all(np.array_equal(a, b) for a, b in zip(gr, gd))  # the flag does switch the kernels
+
+
 @pytest.mark.parametrize("periodic", [True, False])
 def test_advance_stage_matches_oracle(periodic):
     p = M.Params(Nx=40, Ny=24, periodic_x=periodic, periodic_y=periodic, eta=2e-3, G=1.0, sor_iters=50)
@@ -509,10 +528,11 @@ def test_sor_autotune_picks_a_compiled_shape_and_keeps_parity(prec, monkeypatch)
             h.upload(st)
             t = h.run(nsteps)
             got = h.download()
-        S, WJ, sheared = M.sor_tile_choice(N, N, p.sor_iters, prec)
-        print("autotuned SOR tile", S, WJ, sheared, flush=True)
+        S, WJ, sheared, fp64k = M.sor_tile_choice(N, N, p.sor_iters, prec, with_kernels=True)
+        print("autotuned SOR tile", S, WJ, sheared, "fp64 kernels" if fp64k else "", flush=True)
+        assert not (fp64k and prec == CC_FP64)
         if sheared:
-            assert (S, WJ) in ({(16, 64), (16, 32), (32, 32), (16, 80)} if prec == CC_FP64 else
+            assert (S, WJ) in ({(16, 64), (16, 32), (32, 32), (16, 80)} if prec == CC_FP64 or fp64k else
                                {(16, 64), (16, 48), (16, 32), (16, 128), (32, 64), (32, 32), (16, 80)})
         else:
             assert (S, WJ) in {(16, 64), (32, 64), (16, 128), (32, 128)}
