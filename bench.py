@@ -373,8 +373,8 @@ def run_gpu_sphere(N, precision, warmup, steps, prof_steps):
     stats = h.poisson_stats()
     out = h.download()
     tot = h.totals()
-    _, prof = profiled_pass(h, prof_steps, t, ("prim", "face", "update", "mean", "fft_x", "fft_y", "fft_z", "check",
-                                               "grad", "dt"))
+    _, prof = profiled_pass(h, prof_steps, t, ("prim", "face", "update", "fused", "mean", "fft_x", "fft_y", "fft_z",
+                                               "check", "grad", "dt"))
     h.close()
     return {"elapsed": elapsed, "state": out, "prof": prof, "poisson": stats, "mass": tot["mass"]}
 
@@ -412,19 +412,21 @@ def bench_sphere3d(N, warmup, steps, prof_steps, with_cpu):
     cells = float(N) ** 3
 
     def roof(r, word, tname):
-        n, ms = r["prof"]["update"]
+        # the faces + update pass (k_e3_fused), or with CC_E3_FUSED=0 the stage update (k_e3_update); one launch
+        # per stage, half of them the final stage.  Per cell the fused pass reads the 5 primitives (in the flux
+        # precision) and the update reads 15 face fluxes instead; both read the state (5 FP64), the potential
+        # (1 FP64, differenced for g), on the final stage also U^n (5 FP64), and write the state: 5 (or 15) w +
+        # 88 B or + 128 B, + 108 B on average
+        kern = "k_e3_fused" if r["prof"].get("fused", (0, 0.0))[0] > 0 else "k_e3_update"
+        n, ms = r["prof"]["fused" if kern == "k_e3_fused" else "update"]
         if n == 0 or ms <= 0:
             return None
-        # the stage update, one launch per stage (half of them the final stage): per cell it reads 15 face
-        # fluxes (3 axes x 5, in the flux precision), the state (5 FP64), the potential (1 FP64, differenced
-        # for g), on the final stage also U^n (5 FP64), and writes the state: 15 w + 88 B or 15 w + 128 B,
-        # 15 w + 108 B on average
-        bpl = cells * (15 * word + 108)
+        bpl = cells * ((5 if kern == "k_e3_fused" else 15) * word + 108)
         avg = ms / n * 1e-3
-        t1, t2 = load_traffic("cc::k_e3_update<1, %s>" % tname), load_traffic("cc::k_e3_update<2, %s>" % tname)
+        t1, t2 = load_traffic("cc::%s<1, %s>" % (kern, tname)), load_traffic("cc::%s<2, %s>" % (kern, tname))
         return {"bound": "hbm", "achieved": round(bpl / avg / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(bpl / avg / 1e9 / HBM_PEAK_GBS, 4),
-                "traffic": 0.5 * (t1 + t2) if t1 and t2 else None, "kernel": "k_e3_update",
+                "traffic": 0.5 * (t1 + t2) if t1 and t2 else None, "kernel": kern,
                 "launches": n, "avg_launch_us": round(avg * 1e6, 2), "alg_bytes_per_launch": bpl}
 
     def balg(r):
@@ -460,7 +462,8 @@ def bench_sphere3d(N, warmup, steps, prof_steps, with_cpu):
     out["hbm_frac_alg"] = round(out["value"] * balg(res[CC_MIXED]) / 1e9 / HBM_PEAK_GBS, 4)
     for prec, tag, R in ((CC_FP64, "fp64", "double"), (CC_MIXED, "mixed", "float")):
         groups = {"prim": ["cc::k_e3_prim<%s>" % R], "face": tuple("cc::k_e3_face<%s, %d>" % (R, a) for a in range(3)),
-                  "update": ["cc::k_e3_update<1, %s>" % R, "cc::k_e3_update<2, %s>" % R], "grad": ["cc::k_p3_grad"]}
+                  "update": ["cc::k_e3_update<1, %s>" % R, "cc::k_e3_update<2, %s>" % R],
+                  "fused": ["cc::k_e3_fused<1, %s>" % R, "cc::k_e3_fused<2, %s>" % R], "grad": ["cc::k_p3_grad"]}
         hbm = measured_hbm(res[prec]["prof"], groups)
         out.setdefault("hbm_measured", {})[tag] = hbm
         # hydro kernels only: B_alg's hydro share is 5 state passes of 40 B plus g, 224 B per zone-update

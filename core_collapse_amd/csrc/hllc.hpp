@@ -49,8 +49,8 @@ __host__ __device__ __forceinline__ void hllc_eh(R gamma, R floor_rho, const Hyd
         return;
     }
     const R S = use_left ? SL : SR;
-    const R rS = c0 * (S - un) / (S - SM);
-    const R ES = ((S - un) * cE - P * un + P * SM) / (S - SM);
+    const R rS = qdiv<R>(c0 * (S - un), S - SM);
+    const R ES = qdiv<R>((S - un) * cE - P * un + P * SM, S - SM);
     out[0] = F0 + S * (rS - c0);
     out[1] = F1 + S * (rS * SM - cn);
     out[2] = F2 + S * (rS * ut - ct);
@@ -98,21 +98,21 @@ template <typename R, int AX>
 __host__ __device__ __forceinline__ void hllc_e3(R gamma, R floor_rho, const R L[5], const R Rt[5], R out[5]) {
     const R gm1 = gamma - R(1);
     const R unL = L[1 + AX], unR = Rt[1 + AX];
-    const R aL = sqrt(gamma * L[4] / smax(L[0], floor_rho));
-    const R aR = sqrt(gamma * Rt[4] / smax(Rt[0], floor_rho));
+    const R aL = sqrt(qdiv<R>(gamma * L[4], smax(L[0], floor_rho)));
+    const R aR = sqrt(qdiv<R>(gamma * Rt[4], smax(Rt[0], floor_rho)));
     const R SL = smin(unL - aL, unR - aR);
     const R SR = smax(unL + aL, unR + aR);
     const R rL = smax(L[0], floor_rho), rR = smax(Rt[0], floor_rho);
-    const R EL = R(0.5) * rL * ((L[1] * L[1] + L[2] * L[2]) + L[3] * L[3]) + L[4] / gm1;
-    const R ER = R(0.5) * rR * ((Rt[1] * Rt[1] + Rt[2] * Rt[2]) + Rt[3] * Rt[3]) + Rt[4] / gm1;
+    const R EL = R(0.5) * rL * ((L[1] * L[1] + L[2] * L[2]) + L[3] * L[3]) + qdiv<R>(L[4], gm1);
+    const R ER = R(0.5) * rR * ((Rt[1] * Rt[1] + Rt[2] * Rt[2]) + Rt[3] * Rt[3]) + qdiv<R>(Rt[4], gm1);
     const R num = (Rt[4] - L[4]) + rL * unL * (SL - unL) - rR * unR * (SR - unR);
     const R den = rL * (SL - unL) - rR * (SR - unR);
-    const R SM = (fabs(den) > R(1e-14)) ? (num / den) : R(0);
+    const R SM = (fabs(den) > R(1e-14)) ? qdiv<R>(num, den) : R(0);
     const bool up_left = R(0) <= SL, up_right = R(0) >= SR, star_left = R(0) < SM;
     const bool use_left = up_left || (!up_right && star_left);
     const R* W = use_left ? L : Rt;
     const R rho = W[0], un = W[1 + AX], P = W[4];
-    const R Ef = P / gm1 + R(0.5) * rho * ((W[1] * W[1] + W[2] * W[2]) + W[3] * W[3]);
+    const R Ef = qdiv<R>(P, gm1) + R(0.5) * rho * ((W[1] * W[1] + W[2] * W[2]) + W[3] * W[3]);
     R F[5];
     F[0] = rho * un;
 #pragma unroll
@@ -129,8 +129,8 @@ __host__ __device__ __forceinline__ void hllc_e3(R gamma, R floor_rho, const R L
     }
     const R S = use_left ? SL : SR;
     const R c0 = use_left ? rL : rR, cE = use_left ? EL : ER;
-    const R rS = c0 * (S - un) / (S - SM);
-    const R ES = ((S - un) * cE - P * un + P * SM) / (S - SM);
+    const R rS = qdiv<R>(c0 * (S - un), S - SM);
+    const R ES = qdiv<R>((S - un) * cE - P * un + P * SM, S - SM);
     out[0] = F[0] + S * (rS - c0);
 #pragma unroll
     for (int m = 0; m < 3; ++m) {

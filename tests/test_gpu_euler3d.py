@@ -78,9 +78,11 @@ def test_hydro_rhs_and_steps_bitwise_without_gravity(shape):
 @pytest.mark.parametrize("prec", [CC_FP64, CC_MIXED])
 @pytest.mark.parametrize("shape", [(16, 32, 8), (32, 32, 32), (64, 16, 32), (1024, 16, 8), (16, 1024, 8), (8, 16, 1024),
                                    (128, 8, 256)])
-def test_fft_gravity_matches_oracle(prec, shape):
+def test_fft_gravity_matches_oracle(monkeypatch, prec, shape):
     """Every radix-stage combination of the Stockham passes: N = 8 (8), 16 (8, 2), 32 (8, 4), 64 (8, 8),
-    128 (8, 8, 2), 256 (8, 8, 4), 1024 (8, 8, 8, 2), on each axis."""
+    128 (8, 8, 2), 256 (8, 8, 4), 1024 (8, 8, 8, 2), on each axis.  The mixed case runs the FP32 FFT
+    (CC_E3_FFT=fp32; at this tolerance a mixed handle otherwise takes the FP64 one)."""
+    monkeypatch.setenv("CC_E3_FFT", "fp32")
     Nx, Ny, Nz = shape
     p = E.Params(Nx=Nx, Ny=Ny, Nz=Nz, x1=2.0, G=0.7, precision=prec, poisson_tol=1e-10, poisson_max_iter=12)
     rng = np.random.default_rng(2)
@@ -103,7 +105,9 @@ def test_fft_gravity_matches_oracle(prec, shape):
         assert _rel(a.reshape(-1), b) < 1e-9
 
 
-def test_refinement_warm_start_and_uniform_density():
+@pytest.mark.parametrize("fft", ["fp32", "fp64"])
+def test_refinement_warm_start_and_uniform_density(monkeypatch, fft):
+    monkeypatch.setenv("CC_E3_FFT", fft)
     p = E.Params(Nx=16, Ny=16, Nz=16, precision=CC_MIXED)
     rng = np.random.default_rng(3)
     st = _state(rng, p)
@@ -166,3 +170,46 @@ def test_bad_grid_is_rejected():
     from core_collapse_amd._lib import CCError
     with pytest.raises(CCError):
         E.Euler3D(E.Params(Nx=24, Ny=16, Nz=16))
+
+
+@pytest.mark.parametrize("prec", [CC_FP64, CC_MIXED])
+@pytest.mark.parametrize("shape", [(8, 8, 8), (16, 8, 32), (64, 16, 16), (32, 32, 8)])
+def test_fused_pass_equals_split_passes(monkeypatch, prec, shape):
+    """The faces + update pass (k_e3_fused: one z march per 32 x 7 column of cells, x/y fluxes through LDS, z fluxes
+    in registers) against the face kernels + k_e3_update, three steps with gravity, z chunks of 3 planes (seams,
+    the one-plane warm-up, tiles wider than an 8-cell axis): the same arithmetic in the same order without FP
+    contraction, so the states, dts and t are bit-identical in both precisions."""
+    Nx, Ny, Nz = shape
+    monkeypatch.setenv("CC_E3_ZCHUNK", "3")
+    p = E.Params(Nx=Nx, Ny=Ny, Nz=Nz, gravity_on=True, precision=prec)
+    st = _state(np.random.default_rng(4), p)
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("CC_E3_FUSED", fused)
+        with E.Euler3D(p) as h:
+            h.upload(st)
+            t, dts = h.run(3, 0.0, 1e300, return_dts=True)
+            out[fused] = (t, dts, h.download())
+    (ta, da, a), (tb, db, b) = out["1"], out["0"]
+    assert ta == tb and np.array_equal(da, db), (da, db)
+    for f in E.FIELDS:
+        assert np.array_equal(a[f], b[f]), (f, _rel(a[f], b[f]))
+
+
+def test_mixed_handle_runs_the_fp64_fft_at_tight_tolerance(monkeypatch):
+    """At poisson_tol < 1e-6 two FP32-FFT corrections cost more than the one FP64-FFT correction that reaches
+    the tolerance, so a mixed handle solves gravity with the FP64 FFT: phi and g equal an FP64 handle's bit for
+    bit, after one correction."""
+    monkeypatch.delenv("CC_E3_FFT", raising=False)
+    rng = np.random.default_rng(6)
+    out = {}
+    for prec in (CC_MIXED, CC_FP64):
+        p = E.Params(Nx=32, Ny=16, Nz=32, G=0.7, precision=prec, poisson_tol=1e-10, poisson_max_iter=12)
+        st = _state(rng if prec == CC_MIXED else np.random.default_rng(6), p)
+        with E.Euler3D(p) as h:
+            h.upload(st)
+            out[prec] = (h.gravity(), h.poisson_stats())
+    (gm, sm), (gd, sd) = out[CC_MIXED], out[CC_FP64]
+    assert sm["iters_last"] == sd["iters_last"] == 1, (sm, sd)
+    for a, b in zip(gm, gd):
+        assert np.array_equal(a, b)
