@@ -356,46 +356,73 @@ __device__ __forceinline__ LaneWalk sh_walk(const SorArgs& a, const ShBox& b, in
     return w;
 }
 
-// Lane l receives lane l - 1's value (a wave-wide DPP shift right by one lane; lane 0 gets 0).
-__device__ __forceinline__ float lane_shr1(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, true));
+// Lane l receives lane l + 1's value, lane 63 keeps `edge` (lane_shr1_or: lane l - 1's, lane 0 keeps
+// `edge`): wave-wide DPP shifts by one lane without bound_ctrl leave the lane without a source unwritten.
+__device__ __forceinline__ float lane_shl1_or(float edge, float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, edge), __builtin_bit_cast(int, v),
+                                                                 0x130, 0xf, 0xf, false));
 }
-__device__ __forceinline__ double lane_shr1(double v) {
-    const long long u = __builtin_bit_cast(long long, v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(u & 0xffffffffll), 0x138, 0xf, 0xf, true);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), 0x138, 0xf, 0xf, true);
+__device__ __forceinline__ float lane_shr1_or(float edge, float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, edge), __builtin_bit_cast(int, v),
+                                                                 0x138, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double lane_move_or(double edge, double v) {
+    const long long u = __builtin_bit_cast(long long, v), e = __builtin_bit_cast(long long, edge);
+    const int lo = __builtin_amdgcn_update_dpp((int)(e & 0xffffffffll), (int)(u & 0xffffffffll), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(e >> 32), (int)(u >> 32), CTRL, 0xf, 0xf, false);
     return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
 }
+__device__ __forceinline__ double lane_shl1_or(double edge, double v) { return lane_move_or<0x130>(edge, v); }
+__device__ __forceinline__ double lane_shr1_or(double edge, double v) { return lane_move_or<0x138>(edge, v); }
 
 template <int S, int K, typename T, typename F>
-__device__ __forceinline__ void sh_steps(T* __restrict__ x, const SorArgs& a, const ShBox& b,
-                                         const LaneWalk (&w)[S / 16], F&& update) {
+__device__ __forceinline__ void sh_steps(T* __restrict__ x, const T* __restrict__ src, const SorArgs& a,
+                                         const ShBox& b, const LaneWalk (&w)[S / 16], F&& update) {
     using G = ShGeom<S, K>;
     constexpr int SPW = S / 16, P = G::P, EO = G::P + 1;
     if (b.interior) {
-        // Every lane of an active sweep works at every step, so from step 1 on a lane's S neighbour is
-        // its own result of the previous step and its W neighbour lane l - 1's (lane 0 reads it).
+        // Every lane of an active sweep works at every step (from step 0 to K - 1), so from step 1 on a
+        // lane's S neighbour is its own result of the previous step and its W neighbour lane l - 1's; its
+        // own old value is the N neighbour it read at the previous step (the cell it moves to, which only
+        // it writes in between), and its E neighbour is lane l + 1's N neighbour.  Lane 63 reads its E
+        // neighbour (written by sweep s - 1 at the previous step).  Lane 0's W neighbour lies outside the
+        // cells the tile updates in sweeps <= s, so it holds its staged value until sweep s + 1 writes it
+        // at the next step: it is read, with the source term, one step ahead.  A step waits for one round
+        // of LDS reads (the N neighbour, and lane 63's E).
         const int lane = threadIdx.x & 63;
-        T prev[SPW];
+        T prev[SPW], nold[SPW], rn[SPW], wn[SPW];
+        bool act[SPW];  // a sweep of the tile (every lane active at every step), wave-uniform
 #pragma unroll
         for (int m = 0; m < SPW; ++m) {
-            prev[m] = T(0);
-            if (0 >= w[m].lo && 0 < w[m].hi) {
+            act[m] = __builtin_amdgcn_readfirstlane(w[m].lo) == 0;
+            prev[m] = nold[m] = rn[m] = wn[m] = T(0);
+            if (act[m]) {
                 const int idx = w[m].base;
-                prev[m] = update(idx, x[idx], x[idx + EO], x[idx - EO], x[idx + P], x[idx - P]);
+                nold[m] = x[idx + P];
+                prev[m] = update(src[idx], x[idx], x[idx + EO], x[idx - EO], nold[m], x[idx - P]);
                 x[idx] = prev[m];
+                rn[m] = src[idx + P];
+                if (lane == 0) wn[m] = x[idx + P - EO];
             }
         }
         __syncthreads();
         for (int d = 1; d < K; ++d) {
 #pragma unroll
             for (int m = 0; m < SPW; ++m) {
-                if (d >= w[m].lo && d < w[m].hi) {  // uniform over the wave in an interior tile
+                if (act[m]) {
                     const int idx = w[m].base + d * P;
-                    T pW = lane_shr1(prev[m]);
-                    if (lane == 0) pW = x[idx - EO];
-                    prev[m] = update(idx, x[idx], x[idx + EO], pW, x[idx + P], prev[m]);
+                    const T pN = x[idx + P];
+                    T eE = T(0);
+                    if (lane == 63) eE = x[idx + EO];
+                    const T pE = lane_shl1_or(eE, pN), pW = lane_shr1_or(wn[m], prev[m]);
+                    prev[m] = update(rn[m], nold[m], pE, pW, pN, prev[m]);
                     x[idx] = prev[m];
+                    nold[m] = pN;
+                    if (d + 1 < K) {
+                        rn[m] = src[idx + P];
+                        if (lane == 0) wn[m] = x[idx + P - EO];
+                    }
                 }
             }
             __syncthreads();
@@ -413,7 +440,7 @@ __device__ __forceinline__ void sh_steps(T* __restrict__ x, const SorArgs& a, co
                 if (d >= w[m].lo && d < w[m].hi) {
                     const int idx = w[m].base + d * P, cj = w[m].cj0 + d;
                     const int oN = cj < a.Ny - 1 ? P : 0, oS = cj > 0 ? -P : 0;
-                    x[idx] = update(idx, x[idx], x[idx + oE[m]], x[idx + oW[m]], x[idx + oN], x[idx + oS]);
+                    x[idx] = update(src[idx], x[idx], x[idx + oE[m]], x[idx + oW[m]], x[idx + oN], x[idx + oS]);
                 }
             }
             __syncthreads();
@@ -423,11 +450,17 @@ __device__ __forceinline__ void sh_steps(T* __restrict__ x, const SorArgs& a, co
 
 // The box is staged and written back along the grid rows that cross it (coalesced along i): slot c
 // of thread t's walk is c = t + 1024 it, it < IT; sh_slot gives its LDS index (-1 when the slot lies
-// outside the box or the domain; cells outside the domain are never read), its cell and global index.
-// All of a thread's slots are loaded before any is stored, so the loads of a tile overlap.
+// outside the domain; cells outside the domain are never read), its cell and global index.  Grid row
+// r of the box (r = j - j of its first row) holds x in [max(0, BX-1-r), min(BX-1, BX+BY-2-r)], so the
+// rows above BX - 1 and below BY are triangles of widths r + 1 and BX + BY - 1 - r; when BY >= BX - 1,
+// row v < BX - 1 shares a row of slots with row v + BY (widths summing to BX), so BY rows of BX slots
+// hold the box exactly.  Otherwise a slot row is one grid row and slots outside the box are skipped.
+// All of a thread's slots of a round are loaded before any is stored, so the loads of a tile overlap.
 template <int S, int K>
 struct ShSlots {
-    static constexpr int N = ShGeom<S, K>::NR * ShGeom<S, K>::BX;
+    using G = ShGeom<S, K>;
+    static constexpr bool PAIRED = G::BY >= G::BX - 1;
+    static constexpr int N = (PAIRED ? G::BY : G::NR) * G::BX;
     static constexpr int IT = (N + 1023) / 1024;
 };
 
@@ -435,10 +468,11 @@ template <int S, int K>
 __device__ __forceinline__ int sh_slot(const SorArgs& a, const ShBox& b, int c, size_t& g, int& i, int& j) {
     using G = ShGeom<S, K>;
     const int i0 = b.ib - 1, u0 = b.ub - 1, jlo = u0 - i0 - (G::BX - 1);
-    const int x = c % G::BX, r = c / G::BX;
+    const int x = c % G::BX, v = c / G::BX;
+    const int r = ShSlots<S, K>::PAIRED && v < G::BX - 1 && x < G::BX - 1 - v ? v + G::BY : v;
     i = i0 + x;
     j = jlo + r;
-    const int y = i + j - u0;
+    const int y = x + r - (G::BX - 1);
     const bool in = c < ShSlots<S, K>::N && y >= 0 && y < G::BY && i >= 0 && i < a.Nx && j >= 0 && j < a.Ny;
     g = in ? (size_t)j * a.Nx + i : 0;
     return in ? y * G::P + x : -1;
@@ -450,8 +484,8 @@ __device__ __forceinline__ void sor_sheared_fp64(const SorArgs& a, const int* __
                                                  const double* __restrict__ rhobar) {
     using G = ShGeom<S, K>;
     // one tile per CU (the box and source term fill the LDS), so 128 VGPRs per lane: a thread's slots
-    // are loaded in two rounds
-    constexpr int SPW = S / 16, P = G::P, IT = ShSlots<S, K>::IT, CH = (IT + 1) / 2;
+    // are loaded in one round when there are at most 12 of them (4 VGPRs each), else in two
+    constexpr int SPW = S / 16, P = G::P, IT = ShSlots<S, K>::IT, CH = IT <= 12 ? IT : (IT + 1) / 2;
     __shared__ double sphi[G::BY * P];
     __shared__ double srhs[G::BY * P];
     const ShBox b = sh_box<S, K>(a, tile);
@@ -482,8 +516,8 @@ __device__ __forceinline__ void sor_sheared_fp64(const SorArgs& a, const int* __
     for (int m = 0; m < SPW; ++m) w[m] = sh_walk<S, K>(a, b, b.s0 + wv * SPW + m, lane);
     const double idx2 = a.idx2, idy2 = a.idy2, inv_denom = a.inv_denom, omega = a.omega;
     __syncthreads();
-    sh_steps<S, K>(sphi, a, b, w, [&](int idx, double pc, double pE, double pW, double pN, double pS) {
-        const double pnew = ((pE + pW) * idx2 + (pN + pS) * idy2 - srhs[idx]) * inv_denom;
+    sh_steps<S, K>(sphi, srhs, a, b, w, [&](double r, double pc, double pE, double pW, double pN, double pS) {
+        const double pnew = ((pE + pW) * idx2 + (pN + pS) * idy2 - r) * inv_denom;
         return pc + omega * (pnew - pc);
     });
 #pragma unroll 1
@@ -508,9 +542,10 @@ __device__ __forceinline__ void sor_sheared_mixed(const SorArgs& a, const int* _
     constexpr int SPW = S / 16, P = G::P, IT = ShSlots<S, K>::IT, EO = G::P + 1;
     // separate only when the shared layout already allows just one tile per CU
     constexpr bool SEP = (size_t)G::BY * P * 12 > 80 * 1024 && (size_t)G::BY * P * 16 <= 150 * 1024;
-    // slots in flight per thread: half of them at one tile per CU (128 VGPRs per lane), else 4
+    // slots in flight per thread: at one tile per CU (128 VGPRs per lane) all of them up to 12, else
+    // half; 4 at two tiles per CU
     constexpr bool ONE = (size_t)G::BY * P * 12 > 80 * 1024;
-    constexpr int CH = ONE ? (IT + 1) / 2 : 4;
+    constexpr int CH = ONE ? (IT <= 12 ? IT : (IT + 1) / 2) : 4;
     __shared__ double sbuf[G::BY * P];
     __shared__ float sc[G::BY * P];
     __shared__ float sdsep[SEP ? G::BY * P : 1];
@@ -577,8 +612,8 @@ __device__ __forceinline__ void sor_sheared_mixed(const SorArgs& a, const int* _
     for (int m = 0; m < SPW; ++m) w[m] = sh_walk<S, K>(a, b, b.s0 + wv * SPW + m, lane);
     const float kx = a.kxf, ky = a.kyf, om = a.omegaf;
     __syncthreads();
-    sh_steps<S, K>(sd, a, b, w, [&](int idx, float dc, float dE, float dW, float dN, float dS) {
-        const float gs = sc[idx] + (dE + dW) * kx + (dN + dS) * ky;
+    sh_steps<S, K>(sd, sc, a, b, w, [&](float r, float dc, float dE, float dW, float dN, float dS) {
+        const float gs = r + (dE + dW) * kx + (dN + dS) * ky;
         return dc + om * (gs - dc);
     });
     // phi_hi + correction: from the staged phi_hi where it is still in LDS (separate correction array),
