@@ -176,6 +176,42 @@ def test_m2_default_run_matches_reference_log(tmp_path):
         assert np.array_equal(_first(path, name), fr0[name]), name
 
 
+def test_m2_default_configuration_with_guard_runs_to_t_end(tmp_path):
+    """The zero-argument M2 configuration with the HLLD guard (--hlld_guard 1e-2: faces within 1 % of the star
+    states' degeneracy take the reference's HLL flux) runs to its t_end = 0.35: status 0, the reference's log
+    pattern (the init line as the reference prints it, one t / dt / max|divB| line per stored frame with t
+    increasing to exactly t_end, dt never collapsing, max|divB| at round-off, the closing line), and every
+    frame in the file with the reference layout, the first frame the oracle's frame of the IC bit for bit."""
+    from core_collapse_amd.mhd2d import Params
+    meta, _ = D.fixture("m2")
+    ref = meta["stdout"]
+    r, wall = _run("m2", ["--hlld_guard", "1e-2"], str(tmp_path))
+    out = r.stdout.splitlines()
+    print("m2 default with the HLLD guard: %.1f s, %d lines, last %s" % (wall, len(out), out[-2]))
+    assert out[0] == ref[0] and out[-1] == "Wrote out2d_mhd.h5"
+    pat = re.compile(r"t=(\S+) dt=(\S+) max\|divB\|=(\S+)$")
+    got = [pat.match(x) for x in out[1:-1]]
+    assert all(got), [x for x in out[1:-1] if not pat.match(x)][:3]
+    t = np.array([float(g.group(1)) for g in got])
+    dt = np.array([float(g.group(2)) for g in got])
+    divb = np.array([float(g.group(3)) for g in got])
+    assert np.all(np.diff(t) > 0) and abs(t[-1] - 0.35) < 1e-12, t[-3:]
+    assert dt.min() > 1e-5, dt.min()  # the unguarded run's dt falls to 1e-9 near t = 0.23-0.30
+    assert divb.max() < 1e-10, divb.max()
+    path = os.path.join(str(tmp_path), "out2d_mhd.h5")
+    p = Params()
+    nt = len(got) + 1
+    lay = h5io.info(path)
+    want_shapes = {"/x": (192,), "/y": (192,), "/t": (nt,)}
+    want_shapes.update({f: (nt, 192, 192) for f in D.FIELDS["m2"]})
+    assert {k: v["shape"] for k, v in lay.items()} == want_shapes
+    assert all(v["f64le"] and v["layout"] == "CONTIGUOUS" for v in lay.values())
+    assert np.allclose(h5io.read(path, "/t")[1:], t, rtol=1e-6, atol=0)
+    fr0 = D.frame_m2(m2_oparams(p), m2_arrays(p, ic.mhd2d_reference_ic(p)), p.dx, p.dy)
+    for name in D.FIELDS["m2"]:
+        assert np.array_equal(_first(path, name), fr0[name]), name
+
+
 # ------------------------------------------------------------------ flags on small grids vs the oracle
 def _compare_frames(path, tag, times, frames, tol):
     t = h5io.read(path, "/t")
