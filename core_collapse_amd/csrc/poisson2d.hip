@@ -356,26 +356,6 @@ __device__ __forceinline__ LaneWalk sh_walk(const SorArgs& a, const ShBox& b, in
     return w;
 }
 
-// Lane l receives lane l + 1's value, lane 63 keeps `edge` (lane_shr1_or: lane l - 1's, lane 0 keeps
-// `edge`): wave-wide DPP shifts by one lane without bound_ctrl leave the lane without a source unwritten.
-__device__ __forceinline__ float lane_shl1_or(float edge, float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, edge), __builtin_bit_cast(int, v),
-                                                                 0x130, 0xf, 0xf, false));
-}
-__device__ __forceinline__ float lane_shr1_or(float edge, float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, edge), __builtin_bit_cast(int, v),
-                                                                 0x138, 0xf, 0xf, false));
-}
-template <int CTRL>
-__device__ __forceinline__ double lane_move_or(double edge, double v) {
-    const long long u = __builtin_bit_cast(long long, v), e = __builtin_bit_cast(long long, edge);
-    const int lo = __builtin_amdgcn_update_dpp((int)(e & 0xffffffffll), (int)(u & 0xffffffffll), CTRL, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp((int)(e >> 32), (int)(u >> 32), CTRL, 0xf, 0xf, false);
-    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
-}
-__device__ __forceinline__ double lane_shl1_or(double edge, double v) { return lane_move_or<0x130>(edge, v); }
-__device__ __forceinline__ double lane_shr1_or(double edge, double v) { return lane_move_or<0x138>(edge, v); }
-
 template <int S, int K, typename T, typename F>
 __device__ __forceinline__ void sh_steps(T* __restrict__ x, const T* __restrict__ src, const SorArgs& a,
                                          const ShBox& b, const LaneWalk (&w)[S / 16], F&& update) {
@@ -383,27 +363,23 @@ __device__ __forceinline__ void sh_steps(T* __restrict__ x, const T* __restrict_
     constexpr int SPW = S / 16, P = G::P, EO = G::P + 1;
     if (b.interior) {
         // Every lane of an active sweep works at every step (from step 0 to K - 1), so from step 1 on a
-        // lane's S neighbour is its own result of the previous step and its W neighbour lane l - 1's; its
-        // own old value is the N neighbour it read at the previous step (the cell it moves to, which only
-        // it writes in between), and its E neighbour is lane l + 1's N neighbour.  Lane 63 reads its E
-        // neighbour (written by sweep s - 1 at the previous step).  Lane 0's W neighbour lies outside the
-        // cells the tile updates in sweeps <= s, so it holds its staged value until sweep s + 1 writes it
-        // at the next step: it is read, with the source term, one step ahead.  A step waits for one round
-        // of LDS reads (the N neighbour, and lane 63's E).
-        const int lane = threadIdx.x & 63;
-        T prev[SPW], nold[SPW], rn[SPW], wn[SPW];
+        // lane's S neighbour is its own result of the previous step, and its own old value is the N
+        // neighbour it read at the previous step (the cell it moves to, which only it writes in between);
+        // the source term is read a step ahead.  A step reads its N, E and W neighbours from LDS in one
+        // round: on MI355X the plain reads cost less than DPP shifts of the neighbouring lanes' registers,
+        // the step being bound by its VALU instructions more than by LDS bandwidth.
+        T prev[SPW], nold[SPW], rn[SPW];
         bool act[SPW];  // a sweep of the tile (every lane active at every step), wave-uniform
 #pragma unroll
         for (int m = 0; m < SPW; ++m) {
             act[m] = __builtin_amdgcn_readfirstlane(w[m].lo) == 0;
-            prev[m] = nold[m] = rn[m] = wn[m] = T(0);
+            prev[m] = nold[m] = rn[m] = T(0);
             if (act[m]) {
                 const int idx = w[m].base;
                 nold[m] = x[idx + P];
                 prev[m] = update(src[idx], x[idx], x[idx + EO], x[idx - EO], nold[m], x[idx - P]);
                 x[idx] = prev[m];
                 rn[m] = src[idx + P];
-                if (lane == 0) wn[m] = x[idx + P - EO];
             }
         }
         __syncthreads();
@@ -412,17 +388,11 @@ __device__ __forceinline__ void sh_steps(T* __restrict__ x, const T* __restrict_
             for (int m = 0; m < SPW; ++m) {
                 if (act[m]) {
                     const int idx = w[m].base + d * P;
-                    const T pN = x[idx + P];
-                    T eE = T(0);
-                    if (lane == 63) eE = x[idx + EO];
-                    const T pE = lane_shl1_or(eE, pN), pW = lane_shr1_or(wn[m], prev[m]);
+                    const T pN = x[idx + P], pE = x[idx + EO], pW = x[idx - EO];
                     prev[m] = update(rn[m], nold[m], pE, pW, pN, prev[m]);
                     x[idx] = prev[m];
                     nold[m] = pN;
-                    if (d + 1 < K) {
-                        rn[m] = src[idx + P];
-                        if (lane == 0) wn[m] = x[idx + P - EO];
-                    }
+                    if (d + 1 < K) rn[m] = src[idx + P];
                 }
             }
             __syncthreads();

@@ -518,10 +518,8 @@ def sheared_run_tile(phi, rhs, tile, nsw, idx2, idy2, omega, W, K, S):
     """One sheared tile as the kernel runs it: stage the in-domain cells of the box through the staging slots,
     K steps (all reads of a step before its writes; no cell written at a step is read by another lane at that
     step), write back the cells the predicate selects.  In an interior tile the step uses the kernel's
-    registers: the old value is the N neighbour read at the previous step, the E neighbour lane l + 1's N
-    neighbour (lane W - 1 reads it), the S neighbour the lane's previous result, the W neighbour lane l - 1's
-    previous result (lane 0's read after the previous step's writes), the source term read a step ahead; each
-    is checked against the box value it stands for."""
+    registers: the old value is the N neighbour read at the previous step, the S neighbour the lane's previous
+    result, the source term read a step ahead; each is checked against the box value it stands for."""
     Ny, Nx = rhs.shape
     inv = 1.0 / (2.0 * (idx2 + idy2))
     TS, TK, TI = tile
@@ -541,7 +539,7 @@ def sheared_run_tile(phi, rhs, tile, nsw, idx2, idy2, omega, W, K, S):
     assert len(staged) == BX * BY and set(staged.values()) == {1}
     interior = ib >= 1 and ib + W + S - 2 <= Nx - 2 and \
         K * TK - (W * TI + W - 1) - (s0 + S - 1) >= 1 and K * TK + K - 1 - W * TI - s0 <= Ny - 2
-    regs = {}  # interior tiles: (s, lane) -> [prev, nold, rn, wn] after the lane's last step
+    regs = {}  # interior tiles: (s, lane) -> [prev, nold, rn] after the lane's last step
     for d in range(K):
         new, reads, writes = [], set(), set()
         old = {k: list(v) for k, v in regs.items()}  # the registers as the step starts
@@ -561,18 +559,14 @@ def sheared_run_tile(phi, rhs, tile, nsw, idx2, idy2, omega, W, K, S):
                     assert i < Nx - 1 and i > 0 and j < Ny - 1 and j > 0
                 pc, pE, pW, pN, pS, r = box[y, x], box[e], box[w], box[n], box[so], rhs[j, i]
                 if interior and d > 0:
-                    prev, nold, rn, wn = old[(s, lane)]
-                    cW = old[(s, lane - 1)][0] if lane > 0 else wn
-                    assert (nold, cW, prev, rn) == (pc, pW, pS, r), (tile, d, s, lane)
-                    if lane < W - 1:  # lane + 1's cell at this step is (i + 1, j - 1): its N neighbour is our E
-                        x1, y1 = (i + 1) - ib + 1, (i + 1) + (j - 1) - ub + 1
-                        assert (y1 + 1, x1) == e
+                    prev, nold, rn = old[(s, lane)]
+                    assert (nold, prev, rn) == (pc, pS, r), (tile, d, s, lane)
                 pnew = ((pE + pW) * idx2 + (pN + pS) * idy2 - r) * inv
                 v = pc + omega * (pnew - pc)
                 assert np.isfinite(v)
                 new.append(((y, x), v))
                 if interior:
-                    regs.setdefault((s, lane), [0.0] * 4)
+                    regs.setdefault((s, lane), [0.0] * 3)
                     regs[(s, lane)][0] = v
                     regs[(s, lane)][1] = pN
                 reads |= {e, w, n, so} - {(y, x)}  # a clamped neighbour is the cell itself
@@ -580,17 +574,11 @@ def sheared_run_tile(phi, rhs, tile, nsw, idx2, idy2, omega, W, K, S):
         assert len(writes) == len(new) and not (reads & writes), (tile, d)
         for c, v in new:
             box[c] = v
-        if interior and d + 1 < K:
-            # read a step ahead, after this step's writes: the source term and lane 0's W neighbour, which
-            # no lane writes at this step or the next
+        if interior and d + 1 < K:  # the source term of the next step, read a step ahead
             for s in range(s0, s1):
                 for lane in range(W):
                     i, j = W * TI + lane - s, K * TK + d + 1 - W * TI - lane - s
-                    x, y = i - ib + 1, i + j - ub + 1
                     regs[(s, lane)][2] = rhs[j, i]
-                    if lane == 0:
-                        assert (y - 1, x - 1) not in writes
-                        regs[(s, lane)][3] = box[y - 1, x - 1]
     for y in range(BY):
         for x in range(BX):
             i, u = ib - 1 + x, ub - 1 + y
