@@ -382,20 +382,41 @@ __device__ __forceinline__ void sh_steps(T* __restrict__ x, const T* __restrict_
                 rn[m] = src[idx + P];
             }
         }
-        __syncthreads();
-        for (int d = 1; d < K; ++d) {
+        // per-lane LDS pointers advanced by a row per step: the W neighbour at q[0], the cell at q[EO], the E
+        // neighbour at q[2 EO], the N neighbour at q[EO + P] (small non-negative offsets of one base address)
+        T* q[SPW];
+        const T* r[SPW];
 #pragma unroll
-            for (int m = 0; m < SPW; ++m) {
-                if (act[m]) {
-                    const int idx = w[m].base + d * P;
-                    const T pN = x[idx + P], pE = x[idx + EO], pW = x[idx - EO];
-                    prev[m] = update(rn[m], nold[m], pE, pW, pN, prev[m]);
-                    x[idx] = prev[m];
-                    nold[m] = pN;
-                    if (d + 1 < K) rn[m] = src[idx + P];
-                }
+        for (int m = 0; m < SPW; ++m) {
+            q[m] = x + (w[m].base - EO);
+            r[m] = src + (w[m].base + P);
+        }
+        auto step = [&](int m, int d) {
+            q[m] += P;
+            const T pN = q[m][EO + P], pE = q[m][2 * EO], pW = q[m][0];
+            prev[m] = update(rn[m], nold[m], pE, pW, pN, prev[m]);
+            q[m][EO] = prev[m];
+            nold[m] = pN;
+            r[m] += P;
+            if (d + 1 < K) rn[m] = r[m][0];
+        };
+        bool all = true;
+#pragma unroll
+        for (int m = 0; m < SPW; ++m) all = all && act[m];
+        __syncthreads();
+        if (all) {  // the usual case: the wave's loop has no per-step branch
+            for (int d = 1; d < K; ++d) {
+#pragma unroll
+                for (int m = 0; m < SPW; ++m) step(m, d);
+                __syncthreads();
             }
-            __syncthreads();
+        } else {
+            for (int d = 1; d < K; ++d) {
+#pragma unroll
+                for (int m = 0; m < SPW; ++m)
+                    if (act[m]) step(m, d);
+                __syncthreads();
+            }
         }
     } else {
         int oE[SPW], oW[SPW];
@@ -456,8 +477,11 @@ __device__ __forceinline__ void sor_sheared_fp64(const SorArgs& a, const int* __
     // one tile per CU (the box and source term fill the LDS), so 128 VGPRs per lane: a thread's slots
     // are loaded in one round when there are at most 12 of them (4 VGPRs each), else in two
     constexpr int SPW = S / 16, P = G::P, IT = ShSlots<S, K>::IT, CH = IT <= 12 ? IT : (IT + 1) / 2;
-    __shared__ double sphi[G::BY * P];
-    __shared__ double srhs[G::BY * P];
+    // one allocation with the phi box first: the steps' neighbour reads then reach it through the 16-bit
+    // offsets of the LDS instructions from one base address
+    __shared__ double sbox[2 * G::BY * P];
+    double* const sphi = sbox;
+    double* const srhs = sbox + G::BY * P;
     const ShBox b = sh_box<S, K>(a, tile);
     const double rb = rhobar[0];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
