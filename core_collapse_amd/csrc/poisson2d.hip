@@ -419,20 +419,30 @@ __device__ __forceinline__ void sh_steps(T* __restrict__ x, const T* __restrict_
             }
         }
     } else {
-        int oE[SPW], oW[SPW];
+        // edge tiles: clamped neighbours (a clamped neighbour is the cell itself), lanes active on [lo, hi);
+        // per-lane pointers advanced a row per step, the E/W offsets fixed per lane, N/S from the row
+        int oE[SPW], oW[SPW], cj[SPW];
+        T* q[SPW];
+        const T* r[SPW];
 #pragma unroll
         for (int m = 0; m < SPW; ++m) {
             oE[m] = w[m].ci < a.Nx - 1 ? EO : 0;
             oW[m] = w[m].ci > 0 ? -EO : 0;
+            cj[m] = w[m].cj0;
+            q[m] = x + w[m].base;
+            r[m] = src + w[m].base;
         }
         for (int d = 0; d < K; ++d) {
 #pragma unroll
             for (int m = 0; m < SPW; ++m) {
                 if (d >= w[m].lo && d < w[m].hi) {
-                    const int idx = w[m].base + d * P, cj = w[m].cj0 + d;
-                    const int oN = cj < a.Ny - 1 ? P : 0, oS = cj > 0 ? -P : 0;
-                    x[idx] = update(src[idx], x[idx], x[idx + oE[m]], x[idx + oW[m]], x[idx + oN], x[idx + oS]);
+                    const int oN = cj[m] < a.Ny - 1 ? P : 0, oS = cj[m] > 0 ? -P : 0;
+                    T* const c = q[m];
+                    *c = update(*r[m], *c, c[oE[m]], c[oW[m]], c[oN], c[oS]);
                 }
+                q[m] += P;
+                r[m] += P;
+                ++cj[m];
             }
             __syncthreads();
         }
