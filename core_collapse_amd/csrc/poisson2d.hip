@@ -383,7 +383,8 @@ __device__ __forceinline__ void sh_steps(T* __restrict__ x, const T* __restrict_
             }
         }
         // per-lane LDS pointers advanced by a row per step: the W neighbour at q[0], the cell at q[EO], the E
-        // neighbour at q[2 EO], the N neighbour at q[EO + P] (small non-negative offsets of one base address)
+        // neighbour at q[2 EO], the N neighbour at q[EO + P]; with the activity test outside the loop a step is
+        // two address adds and the seven FP64 (or FP32) operations of the update
         T* q[SPW];
         const T* r[SPW];
 #pragma unroll
@@ -487,8 +488,7 @@ __device__ __forceinline__ void sor_sheared_fp64(const SorArgs& a, const int* __
     // one tile per CU (the box and source term fill the LDS), so 128 VGPRs per lane: a thread's slots
     // are loaded in one round when there are at most 12 of them (4 VGPRs each), else in two
     constexpr int SPW = S / 16, P = G::P, IT = ShSlots<S, K>::IT, CH = IT <= 12 ? IT : (IT + 1) / 2;
-    // one allocation with the phi box first: the steps' neighbour reads then reach it through the 16-bit
-    // offsets of the LDS instructions from one base address
+    // the phi box and the source-term box in one allocation (147 KB at 16 x 80: one tile per CU)
     __shared__ double sbox[2 * G::BY * P];
     double* const sphi = sbox;
     double* const srhs = sbox + G::BY * P;
